@@ -1,0 +1,164 @@
+/*
+ * serac_amd.h -- C-ABI of the MI355X (gfx950) matrix-free residual / J.v path.
+ *
+ * Every entry point returns SX_OK (0) or an error code; sx_last_error() gives a
+ * thread-local message.  Vectors passed to compute entry points are DEVICE
+ * pointers (use sx_malloc / sx_memcpy_* when the caller has no HIP runtime of
+ * its own).  Work is enqueued on the form's HIP stream and is asynchronous
+ * unless stated otherwise; sx_form_synchronize() waits for it.
+ *
+ * Reference interfaces each entry point replaces are cited as
+ * path:line relative to the LLNL/serac checkout, with
+ *   fn/ = src/serac/numerics/functional/, num/ = src/serac/numerics/,
+ *   phy/ = src/serac/physics/.
+ */
+#ifndef SERAC_AMD_H
+#define SERAC_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sx_mesh  sx_mesh;
+typedef struct sx_space sx_space;
+typedef struct sx_form  sx_form;
+typedef struct sx_grad  sx_grad;
+
+/* status codes */
+enum {
+  SX_OK              = 0,
+  SX_ERR_ARG         = 1, /* invalid argument (the reference aborts via SLIC_ERROR) */
+  SX_ERR_HIP         = 2, /* HIP runtime error (message carries hipGetErrorString) */
+  SX_ERR_UNSUPPORTED = 3, /* combination not compiled (order, qfunction, precision) */
+  SX_ERR_STATE       = 4, /* call out of order (e.g. J.v before a linearization) */
+  SX_ERR_NOCONV      = 5  /* solver did not converge (results still written) */
+};
+
+/* L-vector orderings, fn/element_restriction.cpp:425-432 */
+enum { SX_ORDERING_BYNODES = 0, SX_ORDERING_BYVDIM = 1 };
+
+/* compiled quadrature-point functions (the integrand registry) */
+enum {
+  SX_QF_HEAT           = 1, /* params {k, s}: source -s, flux k grad T
+                               phy/heat_transfer.hpp:381-406,493-503 +
+                               phy/materials/thermal_material.hpp:50-55 */
+  SX_QF_MASS_DIFFUSION = 2, /* params {a, b}: {a u, b grad u}
+                               fn/tests/functional_comparisons.cpp:32-33 */
+  SX_QF_NEOHOOKEAN     = 3, /* params {density, K, G}: MaterialStressFunctor<NeoHookean>,
+                               geometric nonlinearity on, quasi-static
+                               phy/solid_mechanics.hpp:834-852,
+                               phy/materials/solid_material.hpp:112-122 */
+  SX_QF_LINEAR_ELASTIC = 4  /* params {lambda, mu}: sigma = lambda tr(eps) I + 2 mu eps */
+};
+
+#define SX_MAX_PARAMS 8
+
+/* ---- runtime helpers (no reference counterpart: device memory plumbing) ---- */
+const char* sx_last_error(void);
+int         sx_init(int device);                 /* select the device; idempotent */
+int         sx_device_name(char* buf, size_t n); /* e.g. "AMD Instinct MI355X gfx950" */
+int         sx_malloc(void** ptr, size_t bytes);
+int         sx_free(void* ptr);
+int         sx_memcpy_htod(void* dst, const void* src, size_t bytes);
+int         sx_memcpy_dtoh(void* dst, const void* src, size_t bytes);
+int         sx_memcpy_dtod(void* dst, const void* src, size_t bytes);
+int         sx_memset_zero(void* ptr, size_t bytes);
+int         sx_device_synchronize(void);
+
+/* ---- meshes ----
+ * Replaces buildCuboidMesh -> mfem::Mesh::MakeCartesian3D
+ * (src/serac/mesh/mesh_utils.cpp:149-153): N^3 hexes on [0,lx]x[0,ly]x[0,lz]
+ * with trilinear geometry, lexicographic vertex and element ids. */
+int sx_mesh_create_cuboid(int nx, int ny, int nz, double lx, double ly, double lz, sx_mesh** out);
+/* General hex mesh: coords [nverts][3]; hex_vertices [nelem][8] in mfem's
+ * hexahedron vertex order (the order of the .mesh files in data/meshes). */
+int sx_mesh_create(int nverts, const double* coords, int nelem, const int* hex_vertices, sx_mesh** out);
+/* Moves interior vertices of a cuboid mesh by h_d * U[-amp, amp] per
+ * coordinate (counter-based RNG keyed by seed, vertex id and coordinate). */
+int sx_mesh_perturb(sx_mesh* mesh, uint64_t seed, double amp);
+int sx_mesh_info(const sx_mesh* mesh, int64_t* nverts, int64_t* nelem);
+int sx_mesh_vertices(const sx_mesh* mesh, double* coords_out); /* host [nverts][3] */
+int sx_mesh_destroy(sx_mesh* mesh);
+
+/* ---- H1 spaces on hexes ----
+ * Replaces mfem::ParFiniteElementSpace + serac::ElementRestriction
+ * (fn/element_restriction.cpp:216-277,425-467): per-element node lists in
+ * lexicographic order, E layout [element][component][node]. order 1..4. */
+int sx_space_create_h1_hex(sx_mesh* mesh, int order, int components, int ordering, sx_space** out);
+int sx_space_info(const sx_space* s, int64_t* nnodes, int64_t* ndofs, int64_t* nelem, int* nodes_per_elem);
+int sx_space_element_nodes(const sx_space* s, int32_t* map_out); /* host [nelem][nodes_per_elem] */
+int sx_space_node_coords(const sx_space* s, double* xyz_out);    /* host [nnodes][3] */
+/* Essential DOFs on the boundary faces selected by face_mask
+ * (bit 0 x=0, 1 x=max, 2 y=0, 3 y=max, 4 z=0, 5 z=max) of a cuboid mesh,
+ * for the components selected by comp_mask. Writes up to cap sorted DOF ids. */
+int sx_space_boundary_dofs(const sx_space* s, int face_mask, int comp_mask, int32_t* dofs_out, int64_t cap,
+                           int64_t* count);
+/* ElementRestriction::Gather / ScatterAdd (fn/element_restriction.cpp:443-467),
+ * device pointers, bit-exact with the reference's serial loops. */
+int sx_space_gather(const sx_space* s, const double* L, double* E);
+int sx_space_scatter_add(const sx_space* s, const double* E, double* L);
+int sx_space_destroy(sx_space* s);
+
+/* ---- forms: Functional<test(trials...)> (fn/functional.hpp:189-745) ---- */
+/* constructor, fn/functional.hpp:219-273 (single rank: P = identity) */
+int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntrials, sx_form** out);
+/* AddDomainIntegral, fn/functional.hpp:286-317 (+ fn/integral.hpp:184-260).
+ * elems == NULL / nelem == 0 means the whole mesh (EntireDomain). */
+int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
+                                int ndep, const int* elems, int nelem);
+/* operator()(t, U...) / operator()(t, differentiate_wrt(U)), fn/functional.hpp:446-513.
+ * inputs: ntrials device pointers (T = L on one rank). wrt = -1: residual only;
+ * wrt = i: also stores the quadrature-point linearization w.r.t. trial i and
+ * returns its gradient operator in *grad_out (owned by the form). */
+int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out);
+int sx_form_synchronize(sx_form* f);
+int sx_form_stream(sx_form* f, void** hip_stream); /* the form's hipStream_t */
+int sx_form_destroy(sx_form* f);
+
+/* ---- gradient operator: Functional::Gradient (fn/functional.hpp:548-669) ---- */
+/* Gradient::Mult / operator() (fn/functional.hpp:556-566 -> ActionOfGradient :406-432) */
+int sx_grad_apply(sx_grad* g, const double* dx, double* df);
+/* diagonal of the assembled gradient (the values Gradient::assemble,
+ * fn/functional.hpp:569-667, would put on the diagonal), for Jacobi */
+int sx_grad_diagonal(sx_grad* g, double* diag);
+/* bytes of stored linearization (tangent) held by this gradient */
+int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes);
+/* Runs nrep applications back to back on the form's stream, bracketed by HIP
+ * events; writes the elapsed milliseconds of the whole batch and of the
+ * element kernels alone (sum over launches). Synchronous. */
+int sx_grad_apply_timed(sx_grad* g, const double* dx, double* df, int nrep, double* ms_total, double* ms_kernels,
+                        int* launches_per_apply);
+
+/* ---- solvers (num/equation_solver.cpp:22-240, 893-950, 1018-1021) ---- */
+typedef struct sx_solve_report {
+  int    newton_iterations;
+  int    linear_iterations; /* summed over Newton iterations */
+  double initial_norm;
+  double final_norm;
+  int    converged;
+} sx_solve_report;
+
+/* Jacobi-preconditioned CG on the gradient with essential rows/columns
+ * eliminated (phy/solid_mechanics.hpp:1133-1139). Solves J_bc x = b. Stopping
+ * rule of mfem::CGSolver: (r, z) <= max(rel^2 (r0, z0), abs^2). */
+int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
+                double abs_tol, int max_iter, sx_solve_report* report);
+/* Newton (num/equation_solver.cpp:81-235, no line search) with Jacobi-CG
+ * linear solves on R(x) = 0 with essential rows zeroed. x (device) holds the
+ * essential values on entry and the solution on exit. Form trial wrt. */
+int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, double* x, double rel_tol,
+                    double abs_tol, int max_iter, double lin_rel_tol, double lin_abs_tol, int lin_max_iter,
+                    sx_solve_report* report);
+
+/* ---- vector kernels used by the solvers, exposed for callers ---- */
+int sx_vec_dot(const double* a, const double* b, int64_t n, double* result); /* synchronous, deterministic */
+int sx_vec_axpy(double alpha, const double* x, double* y, int64_t n);        /* y += alpha x */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SERAC_AMD_H */

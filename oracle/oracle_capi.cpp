@@ -1,0 +1,259 @@
+// oracle_capi.cpp -- extern "C" wrapper of the CPU oracle for ctypes.
+// TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker, never by the product library.
+#include <chrono>
+
+#include "serac_oracle_form.hpp"
+
+using namespace oracle;
+
+namespace {
+thread_local std::string g_err;
+template <typename F>
+int guard(F&& f)
+{
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return 1;
+  }
+}
+struct OForm {
+  Form form;
+  OForm(const Mesh& m, const Space& s) : form(m, s) {}
+};
+}  // namespace
+
+extern "C" {
+
+const char* orc_last_error() { return g_err.c_str(); }
+
+double orc_rng_u01(uint64_t seed, uint64_t stream, uint64_t idx) { return rng_u01(seed, stream, idx); }
+
+int orc_gl_rule(int q, double* x, double* w)
+{
+  return guard([&] { gl_rule(q, x, w); });
+}
+int orc_gll_nodes(int n, double* x)
+{
+  return guard([&] { gll_nodes(n, x); });
+}
+int orc_gll_basis(int n, double x, double* B, double* G)
+{
+  return guard([&] {
+    gll_basis(n, x, B);
+    gll_dbasis(n, x, G);
+  });
+}
+
+// ---- meshes
+void* orc_mesh_cuboid(int nx, int ny, int nz, double lx, double ly, double lz)
+{
+  return new Mesh(cuboid(nx, ny, nz, lx, ly, lz));
+}
+void* orc_mesh_create(int nv, const double* X, int ne, const int* hex)
+{
+  auto* m = new Mesh();
+  m->nv   = nv;
+  m->ne   = ne;
+  m->X.assign(X, X + size_t(nv) * 3);
+  m->hex.assign(hex, hex + size_t(ne) * 8);
+  return m;
+}
+int orc_mesh_perturb(void* m, uint64_t seed, double amp)
+{
+  return guard([&] { perturb(*static_cast<Mesh*>(m), seed, amp); });
+}
+void orc_mesh_info(void* m, int64_t* out)
+{
+  auto* M = static_cast<Mesh*>(m);
+  out[0]  = M->nv;
+  out[1]  = M->ne;
+}
+void orc_mesh_vertices(void* m, double* X)
+{
+  auto* M = static_cast<Mesh*>(m);
+  std::memcpy(X, M->X.data(), M->X.size() * sizeof(double));
+}
+void orc_mesh_free(void* m) { delete static_cast<Mesh*>(m); }
+
+// ---- spaces
+void* orc_space(void* m, int p, int nc, int ordering)
+{
+  try {
+    return new Space(make_space(*static_cast<Mesh*>(m), p, nc, ordering));
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+void orc_space_info(void* s, int64_t* out)
+{
+  auto* S = static_cast<Space*>(s);
+  out[0]  = S->nnodes;
+  out[1]  = S->ndof();
+  out[2]  = S->ne;
+  out[3]  = S->npe;
+}
+void orc_space_emap(void* s, int* out)
+{
+  auto* S = static_cast<Space*>(s);
+  std::memcpy(out, S->emap.data(), S->emap.size() * sizeof(int));
+}
+void orc_space_node_coords(void* m, void* s, double* out)
+{
+  auto v = node_coords(*static_cast<Mesh*>(m), *static_cast<Space*>(s));
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+}
+void orc_space_free(void* s) { delete static_cast<Space*>(s); }
+void orc_gather(void* s, const double* L, double* E) { gather(*static_cast<Space*>(s), L, E); }
+void orc_scatter_add(void* s, const double* E, double* L) { scatter_add(*static_cast<Space*>(s), E, L); }
+
+// ---- element kernels (single element, n nodes and q points per direction)
+int orc_interpolate(int n, int q, int nc, const double* X, double* value, double* grad, int brute)
+{
+  return guard([&] {
+    Basis1D b(n, q);
+    if (brute)
+      interpolate_brute(b, nc, X, value, grad);
+    else
+      interpolate(b, nc, X, value, grad);
+  });
+}
+int orc_integrate(int n, int q, int nc, const double* source, const double* flux, double* r, int brute)
+{
+  return guard([&] {
+    Basis1D b(n, q);
+    if (brute)
+      integrate_brute(b, nc, source, flux, r);
+    else
+      integrate(b, nc, source, flux, r);
+  });
+}
+int orc_geometric_factors(void* m, int Q, double* Xq, double* Jq)
+{
+  return guard([&] {
+    std::vector<double> X, J;
+    geometric_factors(*static_cast<Mesh*>(m), Q, X, J);
+    std::memcpy(Xq, X.data(), X.size() * sizeof(double));
+    std::memcpy(Jq, J.data(), J.size() * sizeof(double));
+  });
+}
+// physical-space qfunction (no geometry): value[nc], grad[nc*3] -> source, flux
+int orc_qf_eval(int kind, const double* params, int nc, const double* value, const double* grad, double* source,
+                double* flux)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    qf_eval(P, nc, value, grad, source, flux);
+  });
+}
+
+// ---- forms
+void* orc_form(void* m, void* s)
+{
+  return new OForm(*static_cast<Mesh*>(m), *static_cast<Space*>(s));
+}
+void orc_form_free(void* f) { delete static_cast<OForm*>(f); }
+int  orc_form_add_integral(void* f, int kind, const double* params, const int* elems, int nelem)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    std::vector<int> el;
+    if (elems && nelem > 0) el.assign(elems, elems + nelem);
+    static_cast<OForm*>(f)->form.add_integral(P, el);
+  });
+}
+int orc_form_eval(void* f, const double* U, double* R, int linearize)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.eval(U, R, linearize != 0); });
+}
+int orc_form_jvp(void* f, const double* dU, double* dR)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.jvp(dU, dR); });
+}
+int orc_form_diagonal(void* f, double* diag)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.diagonal(diag); });
+}
+int orc_form_dsize(void* f, int integral) { return static_cast<OForm*>(f)->form.integrals[size_t(integral)].dsize; }
+// stored derivative of (integral, slot in domain, q): dsize doubles
+int orc_form_derivative(void* f, int integral, int64_t slot, int q, double* out)
+{
+  return guard([&] {
+    auto& F = static_cast<OForm*>(f)->form;
+    auto& I = F.integrals[size_t(integral)];
+    std::memcpy(out, &I.D[(size_t(slot) * F.nq() + q) * I.dsize], sizeof(double) * I.dsize);
+  });
+}
+int orc_form_element_matrix(void* f, int integral, int64_t slot, double* K)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.element_matrix(integral, size_t(slot), K); });
+}
+int orc_form_geometry(void* f, double* Xq, double* Jq)
+{
+  return guard([&] {
+    auto& F = static_cast<OForm*>(f)->form;
+    std::memcpy(Xq, F.Xq.data(), F.Xq.size() * sizeof(double));
+    std::memcpy(Jq, F.Jq.data(), F.Jq.size() * sizeof(double));
+  });
+}
+
+// Jacobi-PCG on the BC-eliminated stored linearization; report[4] =
+// {iterations, initial_norm, final_norm, converged}
+int orc_cg(void* f, const int* ess, int ness, const double* b, double* x, double rel, double abs_tol, int maxit,
+           double* report)
+{
+  return guard([&] {
+    auto&               F = static_cast<OForm*>(f)->form;
+    size_t              n = size_t(F.space.ndof());
+    std::vector<char>   is_ess(n, 0);
+    for (int i = 0; i < ness; i++) is_ess[size_t(ess[i])] = 1;
+    std::vector<double> bb(b, b + n), xx(n);
+    auto                rep = pcg_jacobi(F, is_ess, bb, xx, rel, abs_tol, maxit);
+    std::memcpy(x, xx.data(), n * sizeof(double));
+    report[0] = rep.iterations;
+    report[1] = rep.initial_norm;
+    report[2] = rep.final_norm;
+    report[3] = rep.converged;
+  });
+}
+
+// Newton + Jacobi-CG; x in/out holds the essential values. report[5] =
+// {newton its, linear its, initial norm, final norm, converged}
+int orc_newton(void* f, const int* ess, int ness, double* x, double rel, double abs_tol, int maxit, double lin_rel,
+               double lin_abs, int lin_max, double* report)
+{
+  return guard([&] {
+    auto&               F = static_cast<OForm*>(f)->form;
+    size_t              n = size_t(F.space.ndof());
+    std::vector<char>   is_ess(n, 0);
+    for (int i = 0; i < ness; i++) is_ess[size_t(ess[i])] = 1;
+    std::vector<double> xx(x, x + n);
+    auto                rep = newton(F, is_ess, xx, rel, abs_tol, maxit, lin_rel, lin_abs, lin_max);
+    std::memcpy(x, xx.data(), n * sizeof(double));
+    report[0] = rep.newton_iterations;
+    report[1] = rep.total_linear_iterations;
+    report[2] = rep.initial_norm;
+    report[3] = rep.final_norm;
+    report[4] = rep.converged;
+  });
+}
+
+// Wall time (seconds) of `reps` oracle J.v applications (the CPU baseline).
+double orc_time_jvp(void* f, const double* dU, double* dR, int reps)
+{
+  auto& F  = static_cast<OForm*>(f)->form;
+  auto  t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < reps; i++) F.jvp(dU, dR);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
+}  // extern "C"

@@ -1,0 +1,400 @@
+// serac_oracle_form.hpp -- CPU FP64 restatement of serac::Functional's element
+// loop (residual, linearize, J.v, element gradients) and of the Newton /
+// Jacobi-CG loop.  TEST INFRASTRUCTURE ONLY (see serac_oracle.hpp).
+#pragma once
+
+#include "serac_oracle.hpp"
+
+namespace oracle {
+
+// H1 parent <-> physical maps, fn/finite_element.hpp:246-322.
+//   J[row][col] = d x_row / d xi_col  (= jacobians(col,row,q), :259-264)
+//   grad_x u = grad_xi u . inv(J);  source *= detJ;  flux = flux . inv(J^T) detJ
+template <typename T>
+inline void parent_to_physical(int nc, const double Jm[3][3], T* grad)
+{
+  M3<double> J;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) J.a[r][c] = Jm[r][c];
+  M3<double> Ji = inv3(J);
+  for (int c = 0; c < nc; c++) {
+    T g[3] = {grad[c * 3 + 0], grad[c * 3 + 1], grad[c * 3 + 2]};
+    for (int k = 0; k < 3; k++) grad[c * 3 + k] = g[0] * Ji.a[0][k] + g[1] * Ji.a[1][k] + g[2] * Ji.a[2][k];
+  }
+}
+
+template <typename T>
+inline void physical_to_parent(int nc, const double Jm[3][3], T* source, T* flux)
+{
+  M3<double> JT;
+  for (int r = 0; r < 3; r++)
+    for (int c = 0; c < 3; c++) JT.a[r][c] = Jm[c][r];
+  double     dv  = det3(JT);
+  M3<double> JTi = inv3(JT);
+  for (int c = 0; c < nc; c++) {
+    source[c] = source[c] * dv;
+    T f[3]    = {flux[c * 3 + 0], flux[c * 3 + 1], flux[c * 3 + 2]};
+    for (int j = 0; j < 3; j++)
+      flux[c * 3 + j] = (f[0] * JTi.a[0][j] + f[1] * JTi.a[1][j] + f[2] * JTi.a[2][j]) * dv;
+  }
+}
+
+struct Integral {
+  QFParams         qf;
+  std::vector<int> elems;
+  bool             blk[2][2];
+  int              dsize = 0;  // doubles per quadrature point of stored derivative
+  std::vector<double> D;       // [elem-in-domain][q][dsize]
+};
+
+// Functional<test(trial)> with test == trial space, one trial argument.
+// Restates fn/functional.hpp:406-498 (operator(), ActionOfGradient) and
+// fn/domain_integral_kernels.hpp:155-304 (evaluation_kernel_impl,
+// action_of_gradient_kernel), :327-361 (element_gradient_kernel).
+struct Form {
+  const Mesh*           mesh = nullptr;
+  Space                 space;
+  int                   Q = 2;
+  Basis1D               basis;
+  std::vector<double>   Xq, Jq;
+  std::vector<Integral> integrals;
+
+  Form(const Mesh& m, const Space& s) : mesh(&m), space(s), Q(s.p + 1), basis(s.p + 1, s.p + 1)
+  {
+    geometric_factors(m, Q, Xq, Jq);
+  }
+
+  int nq() const { return Q * Q * Q; }
+  int nc() const { return space.nc; }
+
+  void add_integral(const QFParams& qf, const std::vector<int>& elems)
+  {
+    Integral I;
+    I.qf    = qf;
+    I.elems = elems;
+    if (I.elems.empty())
+      for (int e = 0; e < mesh->ne; e++) I.elems.push_back(e);
+    qf_blocks(qf.kind, I.blk);
+    int c    = nc();
+    I.dsize  = (I.blk[0][0] ? c * c : 0) + (I.blk[0][1] ? c * 3 * c : 0) + (I.blk[1][0] ? 3 * c * c : 0) +
+              (I.blk[1][1] ? 9 * c * c : 0);
+    integrals.push_back(std::move(I));
+  }
+
+  void jac(int e, int q, double J[3][3]) const
+  {
+    // J[row][col] = jacobians(col,row,q) with jacobians(j,i,q) = Jq[e][j][i][q]
+    for (int r = 0; r < 3; r++)
+      for (int c = 0; c < 3; c++) J[r][c] = Jq[((size_t(e) * 3 + c) * 3 + r) * nq() + q];
+  }
+
+  // One element of evaluation_kernel_impl; store_D selects the AD variant.
+  template <int N>
+  void element_eval(Integral& I, size_t slot, int e, const double* Ue, double* Re, bool store_D)
+  {
+    const int c = nc(), NQ = nq();
+    std::vector<double> val(size_t(c) * NQ), grd(size_t(c) * 3 * NQ), src(size_t(c) * NQ), flx(size_t(c) * 3 * NQ);
+    interpolate(basis, c, Ue, val.data(), grd.data());
+    for (int q = 0; q < NQ; q++) {
+      double J[3][3];
+      jac(e, q, J);
+      if (!store_D) {
+        double v[3], g[9], s[3], f[9];
+        for (int i = 0; i < c; i++) {
+          v[i] = val[size_t(i) * NQ + q];
+          for (int d = 0; d < 3; d++) g[i * 3 + d] = grd[(size_t(i) * 3 + d) * NQ + q];
+        }
+        parent_to_physical(c, J, g);
+        qf_eval(I.qf, c, v, g, s, f);
+        physical_to_parent(c, J, s, f);
+        for (int i = 0; i < c; i++) {
+          src[size_t(i) * NQ + q] = s[i];
+          for (int d = 0; d < 3; d++) flx[(size_t(i) * 3 + d) * NQ + q] = f[i * 3 + d];
+        }
+      } else {
+        // promote_each_to_dual_when (fn/tuple_tensor_dual_functions.hpp:222-272):
+        // slot i = value comp i, slot c + 3i + d = parent gradient (i, d)
+        dual<N> v[3], g[9], s[3], f[9];
+        for (int i = 0; i < c; i++) {
+          v[i]             = dual<N>(val[size_t(i) * NQ + q]);
+          v[i].gradient[i] = 1.0;
+          for (int d = 0; d < 3; d++) {
+            g[i * 3 + d]                         = dual<N>(grd[(size_t(i) * 3 + d) * NQ + q]);
+            g[i * 3 + d].gradient[c + 3 * i + d] = 1.0;
+          }
+        }
+        parent_to_physical(c, J, g);
+        qf_eval(I.qf, c, v, g, s, f);
+        physical_to_parent(c, J, s, f);
+        double* D = &I.D[(slot * NQ + q) * I.dsize];
+        int     o = 0;
+        // get_gradient blocks in the reference's tuple order (:206-213)
+        if (I.blk[0][0])
+          for (int a = 0; a < c; a++)
+            for (int b = 0; b < c; b++) D[o++] = s[a].gradient[b];
+        if (I.blk[0][1])
+          for (int a = 0; a < c; a++)
+            for (int b = 0; b < 3 * c; b++) D[o++] = s[a].gradient[c + b];
+        if (I.blk[1][0])
+          for (int a = 0; a < 3 * c; a++)
+            for (int b = 0; b < c; b++) D[o++] = f[a].gradient[b];
+        if (I.blk[1][1])
+          for (int a = 0; a < 3 * c; a++)
+            for (int b = 0; b < 3 * c; b++) D[o++] = f[a].gradient[c + b];
+        for (int i = 0; i < c; i++) {
+          src[size_t(i) * NQ + q] = s[i].value;
+          for (int d = 0; d < 3; d++) flx[(size_t(i) * 3 + d) * NQ + q] = f[i * 3 + d].value;
+        }
+      }
+    }
+    integrate(basis, c, src.data(), flx.data(), Re);
+  }
+
+  // chain_rule with the stored derivative (fn/domain_integral_kernels.hpp:223-304)
+  void element_jvp(const Integral& I, size_t slot, const double* dUe, double* dRe) const
+  {
+    const int c = nc(), NQ = nq();
+    std::vector<double> val(size_t(c) * NQ), grd(size_t(c) * 3 * NQ), src(size_t(c) * NQ, 0.0),
+        flx(size_t(c) * 3 * NQ, 0.0);
+    interpolate(basis, c, dUe, val.data(), grd.data());
+    for (int q = 0; q < NQ; q++) {
+      const double* D = &I.D[(slot * NQ + q) * I.dsize];
+      double        in_v[3], in_g[9], s[3] = {0, 0, 0}, f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int i = 0; i < c; i++) {
+        in_v[i] = val[size_t(i) * NQ + q];
+        for (int d = 0; d < 3; d++) in_g[i * 3 + d] = grd[(size_t(i) * 3 + d) * NQ + q];
+      }
+      int o = 0;
+      if (I.blk[0][0])
+        for (int a = 0; a < c; a++)
+          for (int b = 0; b < c; b++) s[a] += D[o++] * in_v[b];
+      if (I.blk[0][1])
+        for (int a = 0; a < c; a++)
+          for (int b = 0; b < 3 * c; b++) s[a] += D[o++] * in_g[b];
+      if (I.blk[1][0])
+        for (int a = 0; a < 3 * c; a++)
+          for (int b = 0; b < c; b++) f[a] += D[o++] * in_v[b];
+      if (I.blk[1][1])
+        for (int a = 0; a < 3 * c; a++)
+          for (int b = 0; b < 3 * c; b++) f[a] += D[o++] * in_g[b];
+      for (int i = 0; i < c; i++) {
+        src[size_t(i) * NQ + q] = s[i];
+        for (int d = 0; d < 3; d++) flx[(size_t(i) * 3 + d) * NQ + q] = f[i * 3 + d];
+      }
+    }
+    integrate(basis, c, src.data(), flx.data(), dRe);
+  }
+
+  void gather_elem(int e, const double* L, double* Ue) const
+  {
+    for (int i = 0; i < nc(); i++)
+      for (int j = 0; j < space.npe; j++) Ue[size_t(i) * space.npe + j] = L[space.lidx(space.emap[size_t(e) * space.npe + j], i)];
+  }
+  void scatter_elem(int e, const double* Re, double* L) const
+  {
+    for (int i = 0; i < nc(); i++)
+      for (int j = 0; j < space.npe; j++) L[space.lidx(space.emap[size_t(e) * space.npe + j], i)] += Re[size_t(i) * space.npe + j];
+  }
+
+  // residual (store_D = false) or linearize (store_D = true)
+  void eval(const double* U, double* R, bool store_D)
+  {
+    const size_t ndof = size_t(space.ndof());
+    std::fill(R, R + ndof, 0.0);
+    std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
+    for (auto& I : integrals) {
+      if (store_D) I.D.assign(I.elems.size() * size_t(nq()) * I.dsize, 0.0);
+      for (size_t k = 0; k < I.elems.size(); k++) {
+        int e = I.elems[k];
+        gather_elem(e, U, Ue.data());
+        std::fill(Re.begin(), Re.end(), 0.0);
+        if (nc() == 1)
+          element_eval<4>(I, k, e, Ue.data(), Re.data(), store_D);
+        else
+          element_eval<12>(I, k, e, Ue.data(), Re.data(), store_D);
+        scatter_elem(e, Re.data(), R);
+      }
+    }
+  }
+
+  void jvp(const double* dU, double* dR) const
+  {
+    const size_t ndof = size_t(space.ndof());
+    std::fill(dR, dR + ndof, 0.0);
+    std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
+    for (auto& I : integrals) {
+      if (I.D.empty()) throw std::runtime_error("jvp before linearize");
+      for (size_t k = 0; k < I.elems.size(); k++) {
+        int e = I.elems[k];
+        gather_elem(e, dU, Ue.data());
+        std::fill(Re.begin(), Re.end(), 0.0);
+        element_jvp(I, k, Ue.data(), Re.data());
+        scatter_elem(e, Re.data(), dR);
+      }
+    }
+  }
+
+  // Element gradient K_e[(c,j) test][(c',J) trial], row-major, by applying the
+  // stored derivative to each trial shape function (element_gradient_kernel).
+  void element_matrix(int integral, size_t slot, double* K) const
+  {
+    const Integral& I   = integrals[size_t(integral)];
+    const int       nd  = nc() * space.npe;
+    std::vector<double> Ue(nd, 0.0), Re(nd);
+    for (int col = 0; col < nd; col++) {
+      std::fill(Ue.begin(), Ue.end(), 0.0);
+      Ue[col] = 1.0;
+      std::fill(Re.begin(), Re.end(), 0.0);
+      element_jvp(I, slot, Ue.data(), Re.data());
+      for (int row = 0; row < nd; row++) K[size_t(row) * nd + col] = Re[row];
+    }
+  }
+
+  void diagonal(double* diag) const
+  {
+    const size_t ndof = size_t(space.ndof());
+    std::fill(diag, diag + ndof, 0.0);
+    const int           nd = nc() * space.npe;
+    std::vector<double> K(size_t(nd) * nd);
+    for (size_t ii = 0; ii < integrals.size(); ii++) {
+      const Integral& I = integrals[ii];
+      for (size_t k = 0; k < I.elems.size(); k++) {
+        element_matrix(int(ii), k, K.data());
+        int e = I.elems[k];
+        for (int i = 0; i < nc(); i++)
+          for (int j = 0; j < space.npe; j++) {
+            int r = i * space.npe + j;
+            diag[space.lidx(space.emap[size_t(e) * space.npe + j], i)] += K[size_t(r) * nd + r];
+          }
+      }
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Solvers.  Essential rows follow physics/solid_mechanics.hpp:1120-1139: the
+// residual has essential rows zeroed; the Jacobian is eliminated to identity
+// rows and columns.  Jacobi-PCG follows the standard preconditioned CG that
+// mfem::CGSolver implements (mfem absent: stopping rule (r, Br) <=
+// max(rel^2 (r0, Br0), abs^2); trajectories parity unpinned).
+// Newton follows num/equation_solver.cpp:81-235 without line search
+// (max_line_search_iterations = 0): norm goal = max(rel * ||r0||, abs),
+// x <- x - c, ||.|| = l2.
+// ---------------------------------------------------------------------------
+inline double dot(const std::vector<double>& a, const std::vector<double>& b)
+{
+  double s = 0;
+  for (size_t i = 0; i < a.size(); i++) s += a[i] * b[i];
+  return s;
+}
+
+struct SolveReport {
+  int    iterations = 0;
+  double initial_norm = 0, final_norm = 0;
+  bool   converged = false;
+};
+
+// J_bc v: eliminate columns (v[ess] -> 0), apply J, set rows: out[ess] = v[ess]
+inline void apply_bc_jvp(const Form& f, const std::vector<char>& is_ess, const std::vector<double>& v,
+                         std::vector<double>& out)
+{
+  std::vector<double> w = v;
+  for (size_t i = 0; i < w.size(); i++)
+    if (is_ess[i]) w[i] = 0.0;
+  f.jvp(w.data(), out.data());
+  for (size_t i = 0; i < w.size(); i++)
+    if (is_ess[i]) out[i] = v[i];
+}
+
+inline SolveReport pcg_jacobi(const Form& f, const std::vector<char>& is_ess, const std::vector<double>& b,
+                              std::vector<double>& x, double rel_tol, double abs_tol, int max_iter)
+{
+  const size_t        n = b.size();
+  std::vector<double> diag(n), r(n), z(n), d(n), Ad(n);
+  f.diagonal(diag.data());
+  for (size_t i = 0; i < n; i++)
+    if (is_ess[i]) diag[i] = 1.0;
+  std::fill(x.begin(), x.end(), 0.0);
+  r = b;
+  for (size_t i = 0; i < n; i++) z[i] = r[i] / diag[i];
+  d          = z;
+  double nom = dot(z, r);
+  SolveReport rep;
+  rep.initial_norm = std::sqrt(std::max(nom, 0.0));
+  double goal      = std::max(nom * rel_tol * rel_tol, abs_tol * abs_tol);
+  if (nom <= goal) {
+    rep.converged  = true;
+    rep.final_norm = rep.initial_norm;
+    return rep;
+  }
+  apply_bc_jvp(f, is_ess, d, Ad);
+  double den = dot(d, Ad);
+  int    it  = 0;
+  for (it = 1; it <= max_iter; it++) {
+    double alpha = nom / den;
+    for (size_t i = 0; i < n; i++) {
+      x[i] += alpha * d[i];
+      r[i] -= alpha * Ad[i];
+    }
+    for (size_t i = 0; i < n; i++) z[i] = r[i] / diag[i];
+    double betanom = dot(r, z);
+    if (betanom <= goal) {
+      rep.converged  = true;
+      rep.final_norm = std::sqrt(std::max(betanom, 0.0));
+      break;
+    }
+    double beta = betanom / nom;
+    for (size_t i = 0; i < n; i++) d[i] = z[i] + beta * d[i];
+    apply_bc_jvp(f, is_ess, d, Ad);
+    den = dot(d, Ad);
+    nom = betanom;
+    rep.final_norm = std::sqrt(std::max(betanom, 0.0));
+  }
+  rep.iterations = std::min(it, max_iter);
+  return rep;
+}
+
+struct NewtonReport {
+  int    newton_iterations = 0;
+  int    total_linear_iterations = 0;
+  double initial_norm = 0, final_norm = 0;
+  bool   converged = false;
+};
+
+// x holds the essential values on entry (and keeps them).
+inline NewtonReport newton(Form& f, const std::vector<char>& is_ess, std::vector<double>& x, double rel_tol,
+                           double abs_tol, int max_iter, double lin_rel, double lin_abs, int lin_max)
+{
+  const size_t        n = x.size();
+  std::vector<double> r(n), c(n);
+  auto                residual_norm = [&]() {
+    f.eval(x.data(), r.data(), false);
+    for (size_t i = 0; i < n; i++)
+      if (is_ess[i]) r[i] = 0.0;
+    return std::sqrt(dot(r, r));
+  };
+  NewtonReport rep;
+  double       norm = residual_norm();
+  rep.initial_norm  = norm;
+  double goal       = std::max(rel_tol * norm, abs_tol);
+  int    it         = 0;
+  for (;; it++) {
+    if (norm <= goal) {
+      rep.converged = true;
+      break;
+    }
+    if (it >= max_iter) break;
+    f.eval(x.data(), r.data(), true);  // linearize at x (also recomputes r)
+    for (size_t i = 0; i < n; i++)
+      if (is_ess[i]) r[i] = 0.0;
+    SolveReport lr = pcg_jacobi(f, is_ess, r, c, lin_rel, lin_abs, lin_max);
+    rep.total_linear_iterations += lr.iterations;
+    for (size_t i = 0; i < n; i++) x[i] -= c[i];
+    norm = residual_norm();
+  }
+  rep.newton_iterations = it;
+  rep.final_norm        = norm;
+  return rep;
+}
+
+}  // namespace oracle

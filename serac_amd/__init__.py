@@ -1,0 +1,256 @@
+"""serac_amd -- MI355X-native matrix-free residual / J.v path of serac::Functional.
+
+The compute path is the HIP library ``libserac_amd.so`` (C-ABI in
+``include/serac_amd.h``); the C++ drop-in front end is
+``include/serac_amd/functional.hpp``.  This Python layer is a thin ctypes
+mirror used by the tests and the benchmark: numpy arrays in host memory,
+``DeviceVector`` for device memory owned by the library.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+from .capi import (SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM, SX_QF_HEAT, SX_QF_LINEAR_ELASTIC,
+                   SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
+
+__all__ = [
+    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "SeracAmdError", "device_name",
+    "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC",
+    "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
+]
+
+_vp = ctypes.c_void_p
+
+
+def _dptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def device_name():
+    buf = ctypes.create_string_buffer(256)
+    capi.call("sx_device_name", buf, 256)
+    return buf.value.decode()
+
+
+class DeviceVector:
+    """FP64 vector in device memory allocated by the library."""
+
+    def __init__(self, n, data=None):
+        self.n = int(n)
+        self.ptr = _vp()
+        capi.call("sx_malloc", ctypes.byref(self.ptr), max(self.n, 1) * 8)
+        if data is None:
+            capi.call("sx_memset_zero", self.ptr, max(self.n, 1) * 8)
+        else:
+            self.upload(data)
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        return cls(a.size, a)
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        assert a.size == self.n, (a.size, self.n)
+        capi.call("sx_memcpy_htod", self.ptr, a.ctypes.data_as(_vp), self.n * 8)
+
+    def numpy(self):
+        out = np.empty(self.n)
+        capi.call("sx_memcpy_dtoh", out.ctypes.data_as(_vp), self.ptr, self.n * 8)
+        return out
+
+    def zero(self):
+        capi.call("sx_memset_zero", self.ptr, max(self.n, 1) * 8)
+
+    def free(self):
+        if self.ptr and self.ptr.value:
+            capi.call_rc("sx_free", self.ptr)
+            self.ptr = _vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Mesh:
+    def __init__(self, handle):
+        self.h = handle
+        nv, ne = ctypes.c_int64(), ctypes.c_int64()
+        capi.call("sx_mesh_info", self.h, ctypes.byref(nv), ctypes.byref(ne))
+        self.nv, self.ne = nv.value, ne.value
+
+    @classmethod
+    def cuboid(cls, nx, ny, nz, lx=1.0, ly=1.0, lz=1.0):
+        h = _vp()
+        capi.call("sx_mesh_create_cuboid", nx, ny, nz, lx, ly, lz, ctypes.byref(h))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, X, hexes):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        hexes = np.ascontiguousarray(hexes, dtype=np.int32)
+        h = _vp()
+        capi.call("sx_mesh_create", len(X), _dptr(X), len(hexes),
+                  hexes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(h))
+        return cls(h)
+
+    def perturb(self, seed, amp):
+        capi.call("sx_mesh_perturb", self.h, seed, amp)
+
+    def vertices(self):
+        X = np.zeros((self.nv, 3))
+        capi.call("sx_mesh_vertices", self.h, _dptr(X))
+        return X
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_mesh_destroy(self.h)
+            self.h = None
+
+
+class Space:
+    """H1<order, components> on a hex mesh (mfem ParFiniteElementSpace stand-in)."""
+
+    def __init__(self, mesh, order, components=1, ordering=SX_ORDERING_BYVDIM):
+        self.mesh = mesh
+        self.h = _vp()
+        capi.call("sx_space_create_h1_hex", mesh.h, order, components, ordering, ctypes.byref(self.h))
+        nn, nd, ne, npe = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
+        capi.call("sx_space_info", self.h, ctypes.byref(nn), ctypes.byref(nd), ctypes.byref(ne), ctypes.byref(npe))
+        self.nnodes, self.ndof, self.ne, self.npe = nn.value, nd.value, ne.value, npe.value
+        self.order, self.nc, self.ordering = order, components, ordering
+
+    def element_nodes(self):
+        m = np.zeros((self.ne, self.npe), dtype=np.int32)
+        capi.call("sx_space_element_nodes", self.h, _iptr(m))
+        return m
+
+    def node_coords(self):
+        x = np.zeros((self.nnodes, 3))
+        capi.call("sx_space_node_coords", self.h, _dptr(x))
+        return x
+
+    def boundary_dofs(self, face_mask, comp_mask=0x7):
+        cnt = ctypes.c_int64()
+        capi.call("sx_space_boundary_dofs", self.h, face_mask, comp_mask, None, 0, ctypes.byref(cnt))
+        out = np.zeros(cnt.value, dtype=np.int32)
+        capi.call("sx_space_boundary_dofs", self.h, face_mask, comp_mask, _iptr(out), cnt.value, ctypes.byref(cnt))
+        return out
+
+    def gather(self, L):
+        """ElementRestriction::Gather on the device: L (DeviceVector) -> E (DeviceVector)."""
+        E = DeviceVector(self.ne * self.nc * self.npe)
+        capi.call("sx_space_gather", self.h, L.ptr, E.ptr)
+        return E
+
+    def scatter_add(self, E, L):
+        capi.call("sx_space_scatter_add", self.h, E.ptr, L.ptr)
+        return L
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_space_destroy(self.h)
+            self.h = None
+
+
+class Gradient:
+    """Functional::Gradient: J.v, diagonal, CG solves."""
+
+    def __init__(self, form, handle):
+        self.form = form
+        self.h = handle
+
+    def mult(self, dx, df):
+        capi.call("sx_grad_apply", self.h, dx.ptr, df.ptr)
+        self.form.synchronize()
+
+    def __call__(self, dx):
+        df = DeviceVector(self.form.test.ndof)
+        self.mult(dx, df)
+        return df
+
+    def diagonal(self):
+        d = DeviceVector(self.form.test.ndof)
+        capi.call("sx_grad_diagonal", self.h, d.ptr)
+        self.form.synchronize()
+        return d
+
+    def tangent_bytes(self):
+        b = ctypes.c_int64()
+        capi.call("sx_grad_tangent_bytes", self.h, ctypes.byref(b))
+        return b.value
+
+    def apply_timed(self, dx, df, nrep):
+        tot, ker, per = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        capi.call("sx_grad_apply_timed", self.h, dx.ptr, df.ptr, nrep, ctypes.byref(tot), ctypes.byref(ker),
+                  ctypes.byref(per))
+        return tot.value, ker.value, per.value
+
+    def cg(self, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, raise_on_noconv=True):
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_cg_solve", self.h, _iptr(ess), len(ess), b.ptr, x.ptr, rel_tol, abs_tol, max_iter,
+                          ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
+
+class Functional:
+    """serac::Functional<test(trials...)> mirror over the C-ABI."""
+
+    def __init__(self, test, trials):
+        self.test = test
+        self.trials = list(trials)
+        arr = (_vp * len(self.trials))(*[t.h for t in self.trials])
+        self.h = _vp()
+        capi.call("sx_form_create", test.h, arr, len(self.trials), ctypes.byref(self.h))
+        self.output = DeviceVector(test.ndof)
+
+    def add_domain_integral(self, qfunc, params, depends_on=(0,), elems=None):
+        p = np.zeros(8)
+        p[: len(params)] = params
+        dep = (ctypes.c_int * len(depends_on))(*depends_on)
+        if elems is None:
+            el, nel = None, 0
+        else:
+            e = np.ascontiguousarray(elems, dtype=np.int32)
+            el, nel = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(e)
+            self._keep = e
+        capi.call("sx_form_add_domain_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on), el, nel)
+
+    def __call__(self, t, *inputs, wrt=-1):
+        """Residual into self.output (a DeviceVector reused across calls, like
+        the reference's output_T_); with wrt >= 0 also returns the Gradient."""
+        arr = (_vp * len(inputs))(*[u.ptr for u in inputs])
+        g = _vp()
+        capi.call("sx_form_eval", self.h, t, arr, self.output.ptr, wrt, ctypes.byref(g))
+        self.synchronize()
+        if wrt >= 0:
+            return self.output, Gradient(self, g)
+        return self.output
+
+    def synchronize(self):
+        capi.call("sx_form_synchronize", self.h)
+
+    def newton(self, x, ess, wrt=0, rel_tol=1e-8, abs_tol=1e-12, max_iter=20, lin_rel_tol=1e-8, lin_abs_tol=1e-12,
+               lin_max_iter=5000, raise_on_noconv=True):
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_newton_solve", self.h, wrt, _iptr(ess), len(ess), x.ptr, rel_tol, abs_tol, max_iter,
+                          lin_rel_tol, lin_abs_tol, lin_max_iter, ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_form_destroy(self.h)
+            self.h = None

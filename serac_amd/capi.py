@@ -1,0 +1,128 @@
+"""ctypes binding of the C-ABI declared in include/serac_amd.h.
+
+Loads the in-tree ``serac_amd/libserac_amd.so``.  There is no fallback: if the
+HIP library is missing or fails to load, every entry point raises.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libserac_amd.so")
+
+SX_OK = 0
+SX_ERR_ARG = 1
+SX_ERR_HIP = 2
+SX_ERR_UNSUPPORTED = 3
+SX_ERR_STATE = 4
+SX_ERR_NOCONV = 5
+
+SX_ORDERING_BYNODES = 0
+SX_ORDERING_BYVDIM = 1
+
+SX_QF_HEAT = 1
+SX_QF_MASS_DIFFUSION = 2
+SX_QF_NEOHOOKEAN = 3
+SX_QF_LINEAR_ELASTIC = 4
+
+_vp = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_int = ctypes.c_int
+
+
+class SolveReport(ctypes.Structure):
+    _fields_ = [
+        ("newton_iterations", ctypes.c_int),
+        ("linear_iterations", ctypes.c_int),
+        ("initial_norm", ctypes.c_double),
+        ("final_norm", ctypes.c_double),
+        ("converged", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SIGNATURES = {
+    "sx_last_error": (ctypes.c_char_p, []),
+    "sx_init": (_int, [_int]),
+    "sx_device_name": (_int, [ctypes.c_char_p, ctypes.c_size_t]),
+    "sx_malloc": (_int, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "sx_free": (_int, [_vp]),
+    "sx_memcpy_htod": (_int, [_vp, _vp, ctypes.c_size_t]),
+    "sx_memcpy_dtoh": (_int, [_vp, _vp, ctypes.c_size_t]),
+    "sx_memcpy_dtod": (_int, [_vp, _vp, ctypes.c_size_t]),
+    "sx_memset_zero": (_int, [_vp, ctypes.c_size_t]),
+    "sx_device_synchronize": (_int, []),
+    "sx_mesh_create_cuboid": (_int, [_int, _int, _int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.POINTER(_vp)]),
+    "sx_mesh_create": (_int, [_int, _D, _int, ctypes.POINTER(_int), ctypes.POINTER(_vp)]),
+    "sx_mesh_perturb": (_int, [_vp, ctypes.c_uint64, ctypes.c_double]),
+    "sx_mesh_info": (_int, [_vp, _I64, _I64]),
+    "sx_mesh_vertices": (_int, [_vp, _D]),
+    "sx_mesh_destroy": (_int, [_vp]),
+    "sx_space_create_h1_hex": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp)]),
+    "sx_space_info": (_int, [_vp, _I64, _I64, _I64, ctypes.POINTER(_int)]),
+    "sx_space_element_nodes": (_int, [_vp, _I32]),
+    "sx_space_node_coords": (_int, [_vp, _D]),
+    "sx_space_boundary_dofs": (_int, [_vp, _int, _int, _I32, ctypes.c_int64, _I64]),
+    "sx_space_gather": (_int, [_vp, _vp, _vp]),
+    "sx_space_scatter_add": (_int, [_vp, _vp, _vp]),
+    "sx_space_destroy": (_int, [_vp]),
+    "sx_form_create": (_int, [_vp, ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]),
+    "sx_form_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
+                                           ctypes.POINTER(_int), _int]),
+    "sx_form_eval": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _vp, _int, ctypes.POINTER(_vp)]),
+    "sx_form_synchronize": (_int, [_vp]),
+    "sx_form_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sx_form_destroy": (_int, [_vp]),
+    "sx_grad_apply": (_int, [_vp, _vp, _vp]),
+    "sx_grad_diagonal": (_int, [_vp, _vp]),
+    "sx_grad_tangent_bytes": (_int, [_vp, _I64]),
+    "sx_grad_apply_timed": (_int, [_vp, _vp, _vp, _int, _D, _D, ctypes.POINTER(_int)]),
+    "sx_cg_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
+                           ctypes.POINTER(SolveReport)]),
+    "sx_newton_solve": (_int, [_vp, _int, _I32, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double, _int,
+                               ctypes.c_double, ctypes.c_double, _int, ctypes.POINTER(SolveReport)]),
+    "sx_vec_dot": (_int, [_vp, _vp, ctypes.c_int64, _D]),
+    "sx_vec_axpy": (_int, [ctypes.c_double, _vp, _vp, ctypes.c_int64]),
+}
+
+
+class SeracAmdError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"serac_amd error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Load libserac_amd.so (raises OSError when it is missing)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; "
+                          f"g.build()'` (no CPU fallback exists)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def call(name, *args):
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != SX_OK:
+        raise SeracAmdError(rc, lib.sx_last_error().decode())
+    return rc
+
+
+def call_rc(name, *args):
+    """Like call() but returns the status code instead of raising on it."""
+    return getattr(load(), name)(*args)

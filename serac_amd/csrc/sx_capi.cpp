@@ -1,0 +1,906 @@
+// sx_capi.cpp -- the C-ABI of include/serac_amd.h: meshes, spaces, forms,
+// gradient operators and the Newton / Jacobi-CG solvers.  Host code drives
+// the HIP kernels of sx_kernels_p*.hip and sx_vector.hip on one stream per
+// form; nothing here computes on the CPU per application.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/serac_amd.h"
+#include "sx_launch.hpp"
+#include "sx_setup.hpp"
+#include "sx_vector.hpp"
+
+namespace {
+
+struct sx_failure : std::runtime_error {
+  int code;
+  sx_failure(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+thread_local std::string g_last_error;
+
+#define SX_HIP(call)                                                                                         \
+  do {                                                                                                       \
+    hipError_t sx_e_ = (call);                                                                               \
+    if (sx_e_ != hipSuccess)                                                                                 \
+      throw sx_failure(SX_ERR_HIP, std::string(#call) + " failed: " + hipGetErrorString(sx_e_));            \
+  } while (0)
+
+#define SX_REQUIRE(cond, code, msg)          \
+  do {                                       \
+    if (!(cond)) throw sx_failure(code, msg); \
+  } while (0)
+
+template <typename F>
+int api(F&& f)
+{
+  try {
+    f();
+    return SX_OK;
+  } catch (const sx_failure& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SX_ERR_ARG;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SX_ERR_ARG;
+  }
+}
+
+// owning device buffer
+struct DeviceBuffer {
+  void*  ptr   = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer&) = delete;
+  DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+  ~DeviceBuffer() { release(); }
+  void release()
+  {
+    if (ptr) (void)hipFree(ptr);
+    ptr   = nullptr;
+    bytes = 0;
+  }
+  void allocate(size_t n)
+  {
+    if (n == bytes && ptr) return;
+    release();
+    if (n == 0) return;
+    SX_HIP(hipMalloc(&ptr, n));
+    bytes = n;
+  }
+  template <typename T>
+  void upload(const std::vector<T>& v)
+  {
+    allocate(v.size() * sizeof(T));
+    if (!v.empty()) SX_HIP(hipMemcpy(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  }
+  template <typename T>
+  T* as() const
+  {
+    return static_cast<T*>(ptr);
+  }
+};
+
+struct Runtime {
+  std::once_flag                 once;
+  int                            device = 0;
+  std::unique_ptr<DeviceBuffer> tables[6];  // index N = nodes per direction (2..5)
+  std::mutex                     mu;
+};
+Runtime& rt()
+{
+  static Runtime r;
+  return r;
+}
+
+void ensure_device()
+{
+  static bool ok = false;
+  if (ok) return;
+  int n = 0;
+  SX_HIP(hipGetDeviceCount(&n));
+  SX_REQUIRE(n > 0, SX_ERR_HIP, "no HIP device visible");
+  SX_HIP(hipSetDevice(rt().device));
+  ok = true;
+}
+
+const double* element_tables(int N)
+{
+  ensure_device();
+  std::lock_guard<std::mutex> lk(rt().mu);
+  auto&                       t = rt().tables[N];
+  if (!t) {
+    t = std::make_unique<DeviceBuffer>();
+    t->upload(sxh::element_tables(N));
+  }
+  return t->as<const double>();
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handle types
+// ---------------------------------------------------------------------------
+struct sx_mesh {
+  sxh::HexMesh m;
+};
+
+struct sx_space {
+  sx_mesh*         mesh = nullptr;  // borrowed, like the reference's FE space pointers
+  int              order = 1, nc = 1, ordering = SX_ORDERING_BYVDIM, npe = 8;
+  int64_t          nnodes = 0;
+  std::vector<int> emap;    // [ne][npe]
+  std::vector<int> color;   // [ne]
+  int              ncolors = 0;
+  DeviceBuffer     d_emap;  // element order (E-vector gather)
+  DeviceBuffer     d_offs, d_ents;  // node -> (e*npe + j), increasing e
+  int64_t          ndof() const { return nnodes * nc; }
+};
+
+struct Integral {
+  int                  qf = 0;
+  double               params[SX_MAX_PARAMS] = {0};
+  std::vector<int>     depends_on;
+  int64_t              nslots = 0;
+  std::vector<int64_t> color_off;  // [ncolors + 1] slot ranges
+  DeviceBuffer         d_map;      // [slot][npe]
+  DeviceBuffer         d_coords;   // [slot][24]
+  DeviceBuffer         d_tangent;  // [slot][DS][Q^3]
+  int                  layout  = sxk::LAYOUT_GRAD_SYM;
+  bool                 has_tangent = false;
+};
+
+struct sx_grad {
+  sx_form* form = nullptr;
+  int      wrt  = 0;
+  bool     valid = false;
+};
+
+struct sx_form {
+  const sx_space*                        test = nullptr;
+  std::vector<const sx_space*>           trials;
+  std::vector<std::unique_ptr<Integral>> integrals;
+  std::vector<std::unique_ptr<sx_grad>>  grads;
+  hipStream_t                            stream = nullptr;
+  // solver scratch
+  DeviceBuffer part, scal;
+  ~sx_form()
+  {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int qf_components(int qf)
+{
+  switch (qf) {
+    case SX_QF_HEAT: return 1;
+    case SX_QF_NEOHOOKEAN:
+    case SX_QF_LINEAR_ELASTIC: return 3;
+    case SX_QF_MASS_DIFFUSION: return -1;  // scalar or vector
+  }
+  return 0;
+}
+int qf_layout(int qf) { return qf == SX_QF_MASS_DIFFUSION ? sxk::LAYOUT_FULL_SYM : sxk::LAYOUT_GRAD_SYM; }
+
+sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
+{
+  const sx_space* s = f->test;
+  sxk::ElemLaunch P{};
+  P.order    = s->order;
+  P.nc       = s->nc;
+  P.qf       = I.qf;
+  P.mode     = mode;
+  P.layout   = I.layout;
+  P.ordering = s->ordering;
+  P.map      = I.d_map.as<const int>();
+  P.coords   = I.d_coords.as<const double>();
+  P.nnodes   = s->nnodes;
+  P.D        = I.d_tangent.as<double>();
+  P.tables   = element_tables(s->order + 1);
+  for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = I.params[k];
+  P.stream = f->stream;
+  return P;
+}
+
+// one launch per colour; returns the number of launches
+int run_integral(const sx_form* f, const Integral& I, int mode, const double* u, double* r, hipEvent_t* ev = nullptr)
+{
+  sxk::ElemLaunch P = base_launch(f, I, mode);
+  P.u               = u;
+  P.r               = r;
+  int launches      = 0;
+  for (size_t c = 0; c + 1 < I.color_off.size(); c++) {
+    P.slot_begin = I.color_off[c];
+    P.slot_end   = I.color_off[c + 1];
+    if (P.slot_end <= P.slot_begin) continue;
+    if (ev) SX_HIP(hipEventRecord(ev[2 * launches], f->stream));
+    hipError_t err = hipSuccess;
+    SX_REQUIRE(sxk::launch_element(P, &err), SX_ERR_UNSUPPORTED,
+               "element kernel not compiled for order " + std::to_string(P.order) + ", qfunction " +
+                   std::to_string(P.qf) + ", " + std::to_string(P.nc) + " components");
+    SX_HIP(err);
+    if (ev) SX_HIP(hipEventRecord(ev[2 * launches + 1], f->stream));
+    launches++;
+  }
+  return launches;
+}
+
+bool depends(const Integral& I, int trial)
+{
+  return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
+}
+
+void apply_gradient(sx_grad* g, const double* dx, double* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr)
+{
+  SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
+  SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
+  SX_REQUIRE(dx && df, SX_ERR_ARG, "null vector");
+  sx_form* f = g->form;
+  SX_HIP(sxk::vec_fill(df, 0.0, f->test->ndof(), f->stream));
+  int n = 0;
+  for (auto& I : f->integrals)
+    if (depends(*I, g->wrt) && I->has_tangent)
+      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr);
+  if (nlaunch) *nlaunch = n;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// runtime helpers
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* sx_last_error(void) { return g_last_error.c_str(); }
+
+int sx_init(int device)
+{
+  return api([&] {
+    int n = 0;
+    SX_HIP(hipGetDeviceCount(&n));
+    SX_REQUIRE(device >= 0 && device < n, SX_ERR_ARG, "device index out of range");
+    rt().device = device;
+    SX_HIP(hipSetDevice(device));
+  });
+}
+
+int sx_device_name(char* buf, size_t n)
+{
+  return api([&] {
+    ensure_device();
+    hipDeviceProp_t prop;
+    SX_HIP(hipGetDeviceProperties(&prop, rt().device));
+    std::snprintf(buf, n, "%s %s", prop.name, prop.gcnArchName);
+  });
+}
+
+int sx_malloc(void** ptr, size_t bytes)
+{
+  return api([&] {
+    ensure_device();
+    SX_REQUIRE(ptr, SX_ERR_ARG, "null output pointer");
+    SX_HIP(hipMalloc(ptr, bytes));
+  });
+}
+int sx_free(void* ptr)
+{
+  return api([&] {
+    if (ptr) SX_HIP(hipFree(ptr));
+  });
+}
+int sx_memcpy_htod(void* dst, const void* src, size_t bytes)
+{
+  return api([&] { SX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int sx_memcpy_dtoh(void* dst, const void* src, size_t bytes)
+{
+  return api([&] { SX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int sx_memcpy_dtod(void* dst, const void* src, size_t bytes)
+{
+  return api([&] { SX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
+}
+int sx_memset_zero(void* ptr, size_t bytes)
+{
+  return api([&] {
+    SX_REQUIRE(bytes % sizeof(double) == 0, SX_ERR_ARG, "sx_memset_zero expects a whole number of doubles");
+    SX_HIP(sxk::vec_fill(static_cast<double*>(ptr), 0.0, int64_t(bytes / sizeof(double)), nullptr));
+    SX_HIP(hipDeviceSynchronize());
+  });
+}
+int sx_device_synchronize(void)
+{
+  return api([&] { SX_HIP(hipDeviceSynchronize()); });
+}
+
+// ---------------------------------------------------------------------------
+// meshes
+// ---------------------------------------------------------------------------
+int sx_mesh_create_cuboid(int nx, int ny, int nz, double lx, double ly, double lz, sx_mesh** out)
+{
+  return api([&] {
+    SX_REQUIRE(out, SX_ERR_ARG, "null output");
+    auto m = std::make_unique<sx_mesh>();
+    m->m   = sxh::make_cuboid(nx, ny, nz, lx, ly, lz);
+    *out   = m.release();
+  });
+}
+
+int sx_mesh_create(int nverts, const double* coords, int nelem, const int* hex_vertices, sx_mesh** out)
+{
+  return api([&] {
+    SX_REQUIRE(out && coords && hex_vertices && nverts > 0 && nelem > 0, SX_ERR_ARG, "invalid mesh arrays");
+    auto m  = std::make_unique<sx_mesh>();
+    m->m.nv = nverts;
+    m->m.ne = nelem;
+    m->m.X.assign(coords, coords + size_t(nverts) * 3);
+    m->m.hex.assign(hex_vertices, hex_vertices + size_t(nelem) * 8);
+    for (int v : m->m.hex) SX_REQUIRE(v >= 0 && v < nverts, SX_ERR_ARG, "hex vertex index out of range");
+    *out = m.release();
+  });
+}
+
+int sx_mesh_perturb(sx_mesh* mesh, uint64_t seed, double amp)
+{
+  return api([&] {
+    SX_REQUIRE(mesh, SX_ERR_ARG, "null mesh");
+    SX_REQUIRE(amp >= 0.0 && amp < 0.5, SX_ERR_ARG, "perturbation amplitude must be in [0, 0.5)");
+    sxh::perturb_interior(mesh->m, seed, amp);
+  });
+}
+
+int sx_mesh_info(const sx_mesh* mesh, int64_t* nverts, int64_t* nelem)
+{
+  return api([&] {
+    SX_REQUIRE(mesh, SX_ERR_ARG, "null mesh");
+    if (nverts) *nverts = mesh->m.nv;
+    if (nelem) *nelem = mesh->m.ne;
+  });
+}
+
+int sx_mesh_vertices(const sx_mesh* mesh, double* coords_out)
+{
+  return api([&] {
+    SX_REQUIRE(mesh && coords_out, SX_ERR_ARG, "null argument");
+    std::memcpy(coords_out, mesh->m.X.data(), mesh->m.X.size() * sizeof(double));
+  });
+}
+
+int sx_mesh_destroy(sx_mesh* mesh)
+{
+  delete mesh;
+  return SX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// spaces
+// ---------------------------------------------------------------------------
+int sx_space_create_h1_hex(sx_mesh* mesh, int order, int components, int ordering, sx_space** out)
+{
+  return api([&] {
+    SX_REQUIRE(mesh && out, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(order >= 1 && order <= 4, SX_ERR_UNSUPPORTED, "H1 hex order must be 1..4");
+    SX_REQUIRE(components == 1 || components == 3, SX_ERR_UNSUPPORTED, "components must be 1 or 3");
+    SX_REQUIRE(ordering == SX_ORDERING_BYNODES || ordering == SX_ORDERING_BYVDIM, SX_ERR_ARG, "bad ordering");
+    ensure_device();
+    auto s      = std::make_unique<sx_space>();
+    s->mesh     = mesh;
+    s->order    = order;
+    s->nc       = components;
+    s->ordering = ordering;
+    s->npe      = (order + 1) * (order + 1) * (order + 1);
+    s->nnodes   = sxh::number_nodes(mesh->m, order, s->emap);
+    SX_REQUIRE(s->ndof() < INT32_MAX, SX_ERR_UNSUPPORTED, "more than 2^31 DOFs");
+    s->ncolors = sxh::color_elements(mesh->m, order, s->emap, s->nnodes, s->color);
+    s->d_emap.upload(s->emap);
+    // node -> element entries in increasing element order
+    std::vector<int64_t> offs(size_t(s->nnodes) + 1, 0);
+    for (int v : s->emap) offs[size_t(v) + 1]++;
+    for (size_t i = 0; i < size_t(s->nnodes); i++) offs[i + 1] += offs[i];
+    std::vector<int64_t> pos(offs.begin(), offs.end() - 1);
+    std::vector<int32_t> ents(s->emap.size());
+    for (size_t k = 0; k < s->emap.size(); k++) ents[size_t(pos[size_t(s->emap[k])]++)] = int32_t(k);
+    s->d_offs.upload(offs);
+    s->d_ents.upload(ents);
+    *out = s.release();
+  });
+}
+
+int sx_space_info(const sx_space* s, int64_t* nnodes, int64_t* ndofs, int64_t* nelem, int* nodes_per_elem)
+{
+  return api([&] {
+    SX_REQUIRE(s, SX_ERR_ARG, "null space");
+    if (nnodes) *nnodes = s->nnodes;
+    if (ndofs) *ndofs = s->ndof();
+    if (nelem) *nelem = s->mesh->m.ne;
+    if (nodes_per_elem) *nodes_per_elem = s->npe;
+  });
+}
+
+int sx_space_element_nodes(const sx_space* s, int32_t* map_out)
+{
+  return api([&] {
+    SX_REQUIRE(s && map_out, SX_ERR_ARG, "null argument");
+    std::memcpy(map_out, s->emap.data(), s->emap.size() * sizeof(int32_t));
+  });
+}
+
+int sx_space_node_coords(const sx_space* s, double* xyz)
+{
+  return api([&] {
+    SX_REQUIRE(s && xyz, SX_ERR_ARG, "null argument");
+    const auto& m = s->mesh->m;
+    const int   n = s->order + 1;
+    auto        g = sxh::lobatto_points(n);
+    for (int64_t e = 0; e < m.ne; e++)
+      for (int k = 0; k < n; k++)
+        for (int j = 0; j < n; j++)
+          for (int i = 0; i < n; i++) {
+            const double w[3][2] = {{1 - g[i], g[i]}, {1 - g[j], g[j]}, {1 - g[k], g[k]}};
+            double       x[3]    = {0, 0, 0};
+            for (int c = 0; c < 8; c++) {
+              const double wt = w[0][c & 1] * w[1][(c >> 1) & 1] * w[2][(c >> 2) & 1];
+              const int    v  = m.corner(e, c);
+              for (int d = 0; d < 3; d++) x[d] += wt * m.X[3 * size_t(v) + d];
+            }
+            const int64_t node = s->emap[size_t(e) * s->npe + size_t(i + n * (j + n * k))];
+            for (int d = 0; d < 3; d++) xyz[3 * node + d] = x[d];
+          }
+  });
+}
+
+int sx_space_boundary_dofs(const sx_space* s, int face_mask, int comp_mask, int32_t* dofs_out, int64_t cap,
+                           int64_t* count)
+{
+  return api([&] {
+    SX_REQUIRE(s && count, SX_ERR_ARG, "null argument");
+    const auto& m = s->mesh->m;
+    SX_REQUIRE(m.structured, SX_ERR_UNSUPPORTED, "boundary selection by face needs a cuboid mesh");
+    const int64_t        p = s->order;
+    const int64_t        g[3] = {p * m.n[0] + 1, p * m.n[1] + 1, p * m.n[2] + 1};
+    std::vector<int32_t> dofs;
+    for (int64_t k = 0; k < g[2]; k++)
+      for (int64_t j = 0; j < g[1]; j++)
+        for (int64_t i = 0; i < g[0]; i++) {
+          const int64_t idx[3] = {i, j, k};
+          bool          on     = false;
+          for (int d = 0; d < 3; d++) {
+            if ((face_mask >> (2 * d)) & 1) on = on || idx[d] == 0;
+            if ((face_mask >> (2 * d + 1)) & 1) on = on || idx[d] == g[d] - 1;
+          }
+          if (!on) continue;
+          const int64_t node = i + g[0] * (j + g[1] * k);
+          for (int c = 0; c < s->nc; c++)
+            if ((comp_mask >> c) & 1)
+              dofs.push_back(int32_t(s->ordering ? node * s->nc + c : int64_t(c) * s->nnodes + node));
+        }
+    std::sort(dofs.begin(), dofs.end());
+    *count = int64_t(dofs.size());
+    if (dofs_out) {
+      SX_REQUIRE(cap >= int64_t(dofs.size()), SX_ERR_ARG, "boundary DOF buffer too small");
+      std::memcpy(dofs_out, dofs.data(), dofs.size() * sizeof(int32_t));
+    }
+  });
+}
+
+int sx_space_gather(const sx_space* s, const double* L, double* E)
+{
+  return api([&] {
+    SX_REQUIRE(s && L && E, SX_ERR_ARG, "null argument");
+    SX_HIP(sxk::gather_e(s->d_emap.as<const int>(), L, E, s->mesh->m.ne, s->npe, s->nc, s->nnodes, s->ordering,
+                         nullptr));
+    SX_HIP(hipDeviceSynchronize());
+  });
+}
+
+int sx_space_scatter_add(const sx_space* s, const double* E, double* L)
+{
+  return api([&] {
+    SX_REQUIRE(s && L && E, SX_ERR_ARG, "null argument");
+    SX_HIP(sxk::scatter_e(s->d_offs.as<const int64_t>(), s->d_ents.as<const int32_t>(), E, L, s->nnodes, s->npe,
+                          s->nc, s->ordering, nullptr));
+    SX_HIP(hipDeviceSynchronize());
+  });
+}
+
+int sx_space_destroy(sx_space* s)
+{
+  delete s;
+  return SX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// forms
+// ---------------------------------------------------------------------------
+int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntrials, sx_form** out)
+{
+  return api([&] {
+    SX_REQUIRE(test && out && ntrials >= 1 && trials, SX_ERR_ARG, "form needs a test space and >= 1 trial space");
+    ensure_device();
+    auto f  = std::make_unique<sx_form>();
+    f->test = test;
+    for (int i = 0; i < ntrials; i++) {
+      SX_REQUIRE(trials[i], SX_ERR_ARG, "null trial space");
+      f->trials.push_back(trials[i]);
+      auto g  = std::make_unique<sx_grad>();
+      g->form = f.get();
+      g->wrt  = i;
+      f->grads.push_back(std::move(g));
+    }
+    SX_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
+    f->part.allocate(size_t(sxk::reduction_blocks()) * sizeof(double));
+    f->scal.allocate(8 * sizeof(double));
+    *out = f.release();
+  });
+}
+
+int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
+                                int ndep, const int* elems, int nelem)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+               "qfunction parameters: at most 8 doubles");
+    const int nc_qf = qf_components(qfunc_id);
+    SX_REQUIRE(nc_qf != 0, SX_ERR_UNSUPPORTED, "unknown qfunction id " + std::to_string(qfunc_id));
+    SX_REQUIRE(ndep == 1 && depends_on, SX_ERR_UNSUPPORTED,
+               "compiled qfunctions take exactly one trial argument (DependsOn<i>)");
+    const int trial = depends_on[0];
+    SX_REQUIRE(trial >= 0 && trial < int(f->trials.size()), SX_ERR_ARG, "DependsOn index out of range");
+    const sx_space* ts = f->trials[size_t(trial)];
+    const sx_space* vs = f->test;
+    SX_REQUIRE(ts->mesh == vs->mesh, SX_ERR_UNSUPPORTED, "test and trial spaces must share a mesh");
+    SX_REQUIRE(ts->order == vs->order && ts->nc == vs->nc && ts->ordering == vs->ordering, SX_ERR_UNSUPPORTED,
+               "test and trial spaces must have the same order, components and ordering");
+    SX_REQUIRE(nc_qf < 0 || nc_qf == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
+
+    auto I = std::make_unique<Integral>();
+    I->qf  = qfunc_id;
+    for (int k = 0; k < nparams; k++) I->params[k] = params[k];
+    I->depends_on = {trial};
+    I->layout     = qf_layout(qfunc_id);
+
+    const auto& m = vs->mesh->m;
+    std::vector<int> domain;
+    if (elems && nelem > 0) {
+      domain.assign(elems, elems + nelem);
+      for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "domain element out of range");
+    } else {
+      domain.resize(size_t(m.ne));
+      for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
+    }
+    // slots: elements of the domain sorted by colour (stable in element order)
+    std::vector<int> slots;
+    slots.reserve(domain.size());
+    I->color_off.assign(size_t(vs->ncolors) + 1, 0);
+    for (int c = 0; c < vs->ncolors; c++) {
+      for (int e : domain)
+        if (vs->color[size_t(e)] == c) slots.push_back(e);
+      I->color_off[size_t(c) + 1] = int64_t(slots.size());
+    }
+    I->nslots = int64_t(slots.size());
+    std::vector<int>    map(size_t(I->nslots) * vs->npe);
+    std::vector<double> xyz(size_t(I->nslots) * 24);
+    for (int64_t k = 0; k < I->nslots; k++) {
+      const int e = slots[size_t(k)];
+      std::memcpy(&map[size_t(k) * vs->npe], &vs->emap[size_t(e) * vs->npe], sizeof(int) * vs->npe);
+      for (int c = 0; c < 8; c++) {
+        const int v = m.corner(e, c);
+        for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + c * 3 + d] = m.X[3 * size_t(v) + d];
+      }
+    }
+    I->d_map.upload(map);
+    I->d_coords.upload(xyz);
+    f->integrals.push_back(std::move(I));
+  });
+}
+
+int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out)
+{
+  return api([&] {
+    SX_REQUIRE(f && inputs && out, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(wrt >= -1 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
+    for (size_t i = 0; i < f->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
+    SX_HIP(sxk::vec_fill(out, 0.0, f->test->ndof(), f->stream));
+    const int nq = (f->test->order + 1) * (f->test->order + 1) * (f->test->order + 1);
+    for (auto& I : f->integrals) {
+      const bool lin = (wrt >= 0) && depends(*I, wrt);
+      if (lin) {
+        const size_t bytes = size_t(I->nslots) * sxk::tangent_ds(f->test->nc, I->layout) * nq * sizeof(double);
+        I->d_tangent.allocate(bytes);
+        I->has_tangent = true;
+      }
+      run_integral(f, *I, lin ? sxk::MODE_LINEARIZE : sxk::MODE_RESIDUAL, inputs[I->depends_on[0]], out);
+    }
+    (void)t;
+    if (wrt >= 0) {
+      f->grads[size_t(wrt)]->valid = true;
+      if (grad_out) *grad_out = f->grads[size_t(wrt)].get();
+    }
+  });
+}
+
+int sx_form_synchronize(sx_form* f)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_HIP(hipStreamSynchronize(f->stream));
+  });
+}
+
+int sx_form_stream(sx_form* f, void** hip_stream)
+{
+  return api([&] {
+    SX_REQUIRE(f && hip_stream, SX_ERR_ARG, "null argument");
+    *hip_stream = f->stream;
+  });
+}
+
+int sx_form_destroy(sx_form* f)
+{
+  if (f && f->stream) (void)hipStreamSynchronize(f->stream);
+  delete f;
+  return SX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gradient operator
+// ---------------------------------------------------------------------------
+int sx_grad_apply(sx_grad* g, const double* dx, double* df)
+{
+  return api([&] { apply_gradient(g, dx, df); });
+}
+
+int sx_grad_diagonal(sx_grad* g, double* diag)
+{
+  return api([&] {
+    SX_REQUIRE(g && g->form && diag, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "diagonal requested before a linearization");
+    sx_form* f = g->form;
+    SX_HIP(sxk::vec_fill(diag, 0.0, f->test->ndof(), f->stream));
+    for (auto& I : f->integrals)
+      if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
+  });
+}
+
+int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes)
+{
+  return api([&] {
+    SX_REQUIRE(g && bytes, SX_ERR_ARG, "null argument");
+    int64_t b = 0;
+    for (auto& I : g->form->integrals)
+      if (depends(*I, g->wrt) && I->has_tangent) b += int64_t(I->d_tangent.bytes);
+    *bytes = b;
+  });
+}
+
+int sx_grad_apply_timed(sx_grad* g, const double* dx, double* df, int nrep, double* ms_total, double* ms_kernels,
+                        int* launches_per_apply)
+{
+  return api([&] {
+    SX_REQUIRE(g && g->form && nrep >= 1, SX_ERR_ARG, "bad argument");
+    sx_form* f   = g->form;
+    int      per = 0;
+    apply_gradient(g, dx, df, nullptr, &per);  // count launches
+    const int               nev = 2 * per * nrep;
+    std::vector<hipEvent_t> ev(size_t(nev) + 2);
+    for (auto& e : ev) SX_HIP(hipEventCreate(&e));
+    SX_HIP(hipStreamSynchronize(f->stream));
+    SX_HIP(hipEventRecord(ev[size_t(nev)], f->stream));
+    for (int r = 0; r < nrep; r++) apply_gradient(g, dx, df, ev.data() + size_t(2 * per * r), nullptr);
+    SX_HIP(hipEventRecord(ev[size_t(nev) + 1], f->stream));
+    SX_HIP(hipEventSynchronize(ev[size_t(nev) + 1]));
+    float total = 0.0f;
+    SX_HIP(hipEventElapsedTime(&total, ev[size_t(nev)], ev[size_t(nev) + 1]));
+    double kern = 0.0;
+    for (int k = 0; k < per * nrep; k++) {
+      float t = 0.0f;
+      SX_HIP(hipEventElapsedTime(&t, ev[size_t(2 * k)], ev[size_t(2 * k + 1)]));
+      kern += t;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (ms_total) *ms_total = total;
+    if (ms_kernels) *ms_kernels = kern;
+    if (launches_per_apply) *launches_per_apply = per;
+  });
+}
+
+// ---------------------------------------------------------------------------
+// vectors
+// ---------------------------------------------------------------------------
+int sx_vec_dot(const double* a, const double* b, int64_t n, double* result)
+{
+  return api([&] {
+    SX_REQUIRE(a && b && result, SX_ERR_ARG, "null argument");
+    ensure_device();
+    DeviceBuffer part, scal;
+    part.allocate(size_t(sxk::reduction_blocks()) * sizeof(double));
+    scal.allocate(8 * sizeof(double));
+    SX_HIP(sxk::vec_dot_dev(a, b, n, part.as<double>(), scal.as<double>(), 0, 0, nullptr));
+    SX_HIP(hipMemcpy(result, scal.ptr, sizeof(double), hipMemcpyDeviceToHost));
+  });
+}
+
+int sx_vec_axpy(double alpha, const double* x, double* y, int64_t n)
+{
+  return api([&] {
+    SX_REQUIRE(x && y, SX_ERR_ARG, "null argument");
+    SX_HIP(sxk::vec_axpy(alpha, x, y, n, nullptr));
+    SX_HIP(hipDeviceSynchronize());
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// solvers
+// ---------------------------------------------------------------------------
+namespace {
+
+double read_scalar(sx_form* f, int slot)
+{
+  double v = 0.0;
+  SX_HIP(hipMemcpyAsync(&v, f->scal.as<double>() + slot, sizeof(double), hipMemcpyDeviceToHost, f->stream));
+  SX_HIP(hipStreamSynchronize(f->stream));
+  return v;
+}
+
+struct Workspace {
+  DeviceBuffer r, z, d, Ad, diag, ess;
+};
+
+// Jacobi-PCG on J_bc (rows and columns of ess eliminated) following the
+// preconditioned CG of mfem::CGSolver, as configured by
+// num/equation_solver.cpp:916-919, 1018-1021.
+void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, double* x, double rel, double abs_tol,
+        int max_iter, Workspace& w, sx_solve_report* rep)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  w.r.allocate(size_t(n) * 8);
+  w.z.allocate(size_t(n) * 8);
+  w.d.allocate(size_t(n) * 8);
+  w.Ad.allocate(size_t(n) * 8);
+  w.diag.allocate(size_t(n) * 8);
+  double *r = w.r.as<double>(), *z = w.z.as<double>(), *d = w.d.as<double>(), *Ad = w.Ad.as<double>(),
+         *diag = w.diag.as<double>();
+  double *part = f->part.as<double>(), *scal = f->scal.as<double>();
+
+  // Jacobi diagonal with identity rows for essential DOFs
+  SX_HIP(sxk::vec_fill(diag, 0.0, n, s));
+  for (auto& I : f->integrals)
+    if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
+  SX_HIP(sxk::set_rows(diag, ess_dev, ness, 1.0, s));
+
+  SX_HIP(sxk::vec_fill(x, 0.0, n, s));
+  SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  SX_HIP(sxk::jacobi_dot(r, diag, z, n, part, scal, 0, s));  // scal[0] = (r, z)
+  SX_HIP(hipMemcpyAsync(d, z, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  const double nom0 = read_scalar(f, 0);
+  rep->initial_norm = std::sqrt(std::max(nom0, 0.0));
+  const double goal = std::max(nom0 * rel * rel, abs_tol * abs_tol);
+  rep->final_norm   = rep->initial_norm;
+  rep->converged    = nom0 <= goal;
+  rep->linear_iterations = 0;
+  if (rep->converged) return;
+
+  auto apply_bc = [&](const double* in, double* outv) {
+    apply_gradient(g, in, outv);
+    SX_HIP(sxk::copy_rows(outv, in, ess_dev, ness, s));
+  };
+  apply_bc(d, Ad);
+  SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));  // scal[1] = den, scal[2] = alpha
+  int it = 0;
+  for (it = 1; it <= max_iter; it++) {
+    SX_HIP(sxk::cg_update(x, r, z, d, Ad, diag, scal, n, part, s));  // scal[3] = betanom, scal[4] = beta
+    const double betanom = read_scalar(f, 3);
+    rep->final_norm      = std::sqrt(std::max(betanom, 0.0));
+    SX_REQUIRE(std::isfinite(betanom), SX_ERR_NOCONV, "CG produced a non-finite residual");
+    if (betanom <= goal) {
+      rep->converged = 1;
+      break;
+    }
+    SX_HIP(sxk::vec_xpay_dev(z, scal + 4, d, n, s));  // d = z + beta d
+    apply_bc(d, Ad);
+    SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));
+  }
+  rep->linear_iterations = std::min(it, max_iter);
+}
+
+double residual_norm(sx_form* f, int wrt, const double* x, double* r, const int32_t* ess_dev, int64_t ness,
+                     sx_grad** gout)
+{
+  std::vector<const double*> in(f->trials.size(), x);
+  sx_grad*                   g  = nullptr;
+  int                        rc = sx_form_eval(f, 0.0, in.data(), r, gout ? wrt : -1, &g);
+  if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  if (gout) *gout = g;
+  SX_HIP(sxk::set_rows(r, ess_dev, ness, 0.0, f->stream));
+  SX_HIP(sxk::vec_dot_dev(r, r, f->test->ndof(), f->part.as<double>(), f->scal.as<double>(), 5, 0, f->stream));
+  return std::sqrt(read_scalar(f, 5));
+}
+
+}  // namespace
+
+extern "C" {
+
+int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
+                double abs_tol, int max_iter, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "CG on a gradient before a linearization");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    cg(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, double* x, double rel_tol,
+                    double abs_tol, int max_iter, double lin_rel_tol, double lin_abs_tol, int lin_max_iter,
+                    sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(f && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    const int64_t n = f->test->ndof();
+    Workspace     w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    DeviceBuffer rbuf, cbuf;
+    rbuf.allocate(size_t(n) * 8);
+    cbuf.allocate(size_t(n) * 8);
+    double*        r    = rbuf.as<double>();
+    double*        c    = cbuf.as<double>();
+    const int32_t* ess  = w.ess.as<const int32_t>();
+    double         norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+    rep.initial_norm    = norm;
+    const double goal   = std::max(rel_tol * norm, abs_tol);
+    int          it     = 0;
+    for (;; it++) {
+      SX_REQUIRE(std::isfinite(norm), SX_ERR_NOCONV, "Newton residual is not finite");
+      if (norm <= goal) {
+        rep.converged = 1;
+        break;
+      }
+      if (it >= max_iter) break;
+      sx_grad* g = nullptr;
+      residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at x
+      sx_solve_report lr{};
+      cg(g, ess, ness, r, c, lin_rel_tol, lin_abs_tol, lin_max_iter, w, &lr);
+      rep.linear_iterations += lr.linear_iterations;
+      SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
+      norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+    }
+    rep.newton_iterations = it;
+    rep.final_norm        = norm;
+    SX_HIP(hipStreamSynchronize(f->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "Newton did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// sx_vector.hip -- L/E-vector kernels: fills, AXPYs, deterministic FP64
+// reductions, E-vector gather / scatter-add, essential-row edits and the fused
+// Jacobi-PCG update kernels.  All are HBM-bound streaming kernels; grids are
+// sized for 256 CUs with grid-stride loops.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sx_vector.hpp"
+
+namespace sxk {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned grid_for(int64_t n)
+{
+  int64_t g = (n + kThreads - 1) / kThreads;
+  if (g > 8192) g = 8192;
+  if (g < 1) g = 1;
+  return unsigned(g);
+}
+
+__global__ void __launch_bounds__(kThreads) k_fill(double* x, double v, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) x[i] = v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_axpy(double a, const double* x, double* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] += a * x[i];
+}
+
+// y = x + a * y  (with a read from device memory when ap != nullptr)
+__global__ void __launch_bounds__(kThreads) k_xpay_dev(const double* x, const double* ap, double* y, int64_t n)
+{
+  const double a = *ap;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = x[i] + a * y[i];
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+  // wave64 shuffle tree, then the 4 wave partials in LDS (fixed order)
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_dot_partial(const double* a, const double* b, int64_t n, double* part)
+{
+  __shared__ double sh[4];
+  double            s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    s += a[i] * b[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Final reduction of nb partials into scal[slot]; op then updates derived
+// scalars in place:  op 1: scal[2] = scal[0] / scal[1]            (alpha)
+//                    op 2: scal[4] = scal[3] / scal[0], scal[0] = scal[3] (beta)
+__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, int nb, double* scal, int slot, int op)
+{
+  __shared__ double sh[4];
+  double            s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    scal[slot] = s;
+    if (op == 1) scal[2] = scal[0] / scal[1];
+    if (op == 2) {
+      scal[4] = scal[3] / scal[0];
+      scal[0] = scal[3];
+    }
+  }
+}
+
+// x += alpha d; r -= alpha Ad; z = r / diag; partial (r, z)
+__global__ void __launch_bounds__(kThreads) k_cg_update(double* x, double* r, double* z, const double* d,
+                                                         const double* Ad, const double* diag, const double* scal,
+                                                         int64_t n, double* part)
+{
+  __shared__ double sh[4];
+  const double      alpha = scal[2];
+  double            s     = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    x[i] += alpha * d[i];
+    const double ri = r[i] - alpha * Ad[i];
+    r[i]            = ri;
+    const double zi = ri / diag[i];
+    z[i]            = zi;
+    s += ri * zi;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// z = r / diag; partial (r, z)
+__global__ void __launch_bounds__(kThreads) k_jacobi(const double* r, const double* diag, double* z, int64_t n,
+                                                      double* part)
+{
+  __shared__ double sh[4];
+  double            s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const double zi = r[i] / diag[i];
+    z[i]            = zi;
+    s += r[i] * zi;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_set_rows(double* x, const int32_t* rows, int64_t nr, double v)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < nr; i += int64_t(gridDim.x) * kThreads)
+    x[rows[i]] = v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_copy_rows(double* dst, const double* src, const int32_t* rows,
+                                                         int64_t nr)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < nr; i += int64_t(gridDim.x) * kThreads)
+    dst[rows[i]] = src[rows[i]];
+}
+
+// E[(e*nc + c)*npe + j] = L[lidx(map[e*npe + j], c)]
+__global__ void __launch_bounds__(kThreads) k_gather_e(const int* map, const double* L, double* E, int64_t ne,
+                                                        int npe, int nc, int64_t nnodes, int ordering)
+{
+  const int64_t n = ne * nc * npe;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const int64_t j    = i % npe;
+    const int64_t ec   = i / npe;
+    const int     c    = int(ec % nc);
+    const int64_t e    = ec / nc;
+    const int64_t node = map[e * npe + j];
+    E[i]               = L[ordering ? node * nc + c : int64_t(c) * nnodes + node];
+  }
+}
+
+// node-centric scatter-add: for each (node, c) add its element entries in
+// increasing element order, the order of the reference's serial loop
+// (fn/element_restriction.cpp:456-467), so the result is bit-identical.
+__global__ void __launch_bounds__(kThreads) k_scatter_e(const int64_t* offs, const int32_t* ents, const double* E,
+                                                         double* L, int64_t nnodes, int npe, int nc, int ordering)
+{
+  const int64_t n = nnodes * nc;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const int64_t node = ordering ? i / nc : i % nnodes;
+    const int     c    = ordering ? int(i % nc) : int(i / nnodes);
+    const int64_t li   = ordering ? node * nc + c : int64_t(c) * nnodes + node;
+    double        acc  = L[li];
+    for (int64_t k = offs[node]; k < offs[node + 1]; k++) {
+      const int64_t ej = ents[k];
+      const int64_t e = ej / npe, j = ej % npe;
+      acc += E[(e * nc + c) * npe + j];
+    }
+    L[li] = acc;
+  }
+}
+
+}  // namespace
+
+hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, s, x, v, n);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpy, dim3(grid_for(n)), dim3(kThreads), 0, s, a, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_xpay_dev, dim3(grid_for(n)), dim3(kThreads), 0, s, x, a_dev, y, n);
+  return hipGetLastError();
+}
+
+int reduction_blocks() { return 1024; }
+
+hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part, double* scal, int slot, int op,
+                       hipStream_t s)
+{
+  const int nb = reduction_blocks();
+  hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kThreads), 0, s, a, b, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, op);
+  return hipGetLastError();
+}
+
+hipError_t cg_update(double* x, double* r, double* z, const double* d, const double* Ad, const double* diag,
+                     double* scal, int64_t n, double* part, hipStream_t s)
+{
+  const int nb = reduction_blocks();
+  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r, z, d, Ad, diag, scal, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, 3, 2);
+  return hipGetLastError();
+}
+
+hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n, double* part, double* scal,
+                      int slot, hipStream_t s)
+{
+  const int nb = reduction_blocks();
+  hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, r, diag, z, n, part);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0);
+  return hipGetLastError();
+}
+
+hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStream_t s)
+{
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_set_rows, dim3(grid_for(nr)), dim3(kThreads), 0, s, x, rows, nr, v);
+  return hipGetLastError();
+}
+
+hipError_t copy_rows(double* dst, const double* src, const int32_t* rows, int64_t nr, hipStream_t s)
+{
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_rows, dim3(grid_for(nr)), dim3(kThreads), 0, s, dst, src, rows, nr);
+  return hipGetLastError();
+}
+
+hipError_t gather_e(const int* map, const double* L, double* E, int64_t ne, int npe, int nc, int64_t nnodes,
+                    int ordering, hipStream_t s)
+{
+  const int64_t n = ne * nc * npe;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_e, dim3(grid_for(n)), dim3(kThreads), 0, s, map, L, E, ne, npe, nc, nnodes, ordering);
+  return hipGetLastError();
+}
+
+hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, double* L, int64_t nnodes, int npe,
+                     int nc, int ordering, hipStream_t s)
+{
+  const int64_t n = nnodes * nc;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scatter_e, dim3(grid_for(n)), dim3(kThreads), 0, s, offs, ents, E, L, nnodes, npe, nc,
+                     ordering);
+  return hipGetLastError();
+}
+
+}  // namespace sxk

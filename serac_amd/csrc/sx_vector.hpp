@@ -1,0 +1,33 @@
+// sx_vector.hpp -- host launchers of the vector kernels in sx_vector.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sxk {
+
+hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s);
+hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s);
+// y = x + (*a_dev) y
+hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s);
+
+// number of partial sums the reductions use (size of the `part` scratch)
+int reduction_blocks();
+// scal[slot] = (a, b); op 1 then sets scal[2] = scal[0]/scal[1]
+hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part, double* scal, int slot, int op,
+                       hipStream_t s);
+// x += a d, r -= a Ad, z = r/diag with a = scal[2]; scal[3] = (r, z);
+// scal[4] = scal[3]/scal[0]; scal[0] = scal[3]
+hipError_t cg_update(double* x, double* r, double* z, const double* d, const double* Ad, const double* diag,
+                     double* scal, int64_t n, double* part, hipStream_t s);
+// z = r/diag; scal[slot] = (r, z)
+hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n, double* part, double* scal,
+                      int slot, hipStream_t s);
+hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStream_t s);
+hipError_t copy_rows(double* dst, const double* src, const int32_t* rows, int64_t nr, hipStream_t s);
+hipError_t gather_e(const int* map, const double* L, double* E, int64_t ne, int npe, int nc, int64_t nnodes,
+                    int ordering, hipStream_t s);
+hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, double* L, int64_t nnodes, int npe,
+                     int nc, int ordering, hipStream_t s);
+
+}  // namespace sxk

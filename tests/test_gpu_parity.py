@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path (through the C-ABI) against the CPU oracle on the
+same seeded inputs.  Integer maps and E-vector gather/scatter are bit-exact;
+FP64 residuals, J.v and diagonals agree to 1e-12 relative (max-norm)."""
+import numpy as np
+import pytest
+
+import oracle as O
+import serac_amd as sx
+from serac_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+QF = {
+    "heat": (sx.SX_QF_HEAT, O.QF_HEAT, 1, (1.0, 1.0)),
+    "mass_diffusion": (sx.SX_QF_MASS_DIFFUSION, O.QF_MASS_DIFFUSION, 1, (1.7, 2.1)),
+    "mass_diffusion_vec": (sx.SX_QF_MASS_DIFFUSION, O.QF_MASS_DIFFUSION, 3, (1.7, 2.1)),
+    "neohookean": (sx.SX_QF_NEOHOOKEAN, O.QF_NEOHOOKEAN, 3, (1.0, 10.0, 0.25)),
+    "linear_elastic": (sx.SX_QF_LINEAR_ELASTIC, O.QF_LINEAR_ELASTIC, 3, (1.2, 0.8)),
+}
+
+
+def relerr(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+class Pair:
+    """The same problem set up on the GPU library and on the oracle."""
+
+    def __init__(self, n, p, nc, perturb=0.2, seed=42, ordering=1, mesh_arrays=None):
+        if mesh_arrays is None:
+            self.gm = sx.Mesh.cuboid(n, n, n)
+            self.om = O.Mesh.cuboid(n, n, n)
+            if perturb > 0:
+                self.gm.perturb(seed, perturb)
+                self.om.perturb(seed, perturb)
+        else:
+            X, hexes = mesh_arrays
+            self.gm = sx.Mesh.from_arrays(X, hexes)
+            self.om = O.Mesh.from_arrays(X, hexes)
+        self.gs = sx.Space(self.gm, p, nc, ordering)
+        self.os = O.Space(self.om, p, nc, ordering)
+        self.nc, self.p = nc, p
+
+    def forms(self, qf_name, elems=None):
+        gid, oid, nc, params = QF[qf_name]
+        assert nc == self.nc
+        gf = sx.Functional(self.gs, [self.gs])
+        gf.add_domain_integral(gid, params, elems=elems)
+        of = O.Form(self.os)
+        of.add_domain_integral(oid, params, elems)
+        return gf, of
+
+
+def field(space, scale, seed, stream):
+    return W.nodal_field(space.nnodes, space.nc, seed, stream, -scale, scale, space.ordering)
+
+
+def test_mesh_perturbation_bit_exact(gpu_lib):
+    gm, om = sx.Mesh.cuboid(5, 4, 3), O.Mesh.cuboid(5, 4, 3)
+    gm.perturb(7, 0.2)
+    om.perturb(7, 0.2)
+    assert np.array_equal(gm.vertices(), om.vertices())
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("nc,ordering", [(1, 1), (3, 1), (3, 0)])
+def test_gather_scatter_bit_exact(gpu_lib, p, nc, ordering):
+    pr = Pair(3, p, nc, ordering=ordering)
+    assert np.array_equal(pr.gs.element_nodes(), pr.os.emap())
+    L = field(pr.gs, 1.0, 3, 2000)
+    E_gpu = pr.gs.gather(sx.DeviceVector.from_numpy(L)).numpy()
+    E_ref = pr.os.gather(L)
+    assert np.array_equal(E_gpu, E_ref)
+    L0 = field(pr.gs, 1.0, 4, 2000)
+    Lg = sx.DeviceVector.from_numpy(L0)
+    pr.gs.scatter_add(sx.DeviceVector.from_numpy(E_ref), Lg)
+    assert np.array_equal(Lg.numpy(), pr.os.scatter_add(E_ref, L0))
+
+
+def test_unstructured_numbering_bit_exact(gpu_lib, patch_mesh):
+    X, hexes = patch_mesh
+    for p in (1, 2, 3, 4):
+        pr = Pair(0, p, 3, mesh_arrays=(X, hexes))
+        assert np.array_equal(pr.gs.element_nodes(), pr.os.emap())
+        assert np.allclose(pr.gs.node_coords(), pr.os.node_coords(), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+@pytest.mark.parametrize("qf", ["heat", "mass_diffusion", "mass_diffusion_vec", "neohookean", "linear_elastic"])
+def test_residual_jvp_diagonal_parity(gpu_lib, p, qf):
+    nc = QF[qf][2]
+    n = {1: 4, 2: 3, 3: 3, 4: 2}[p]
+    pr = Pair(n, p, nc)
+    gf, of = pr.forms(qf)
+    xyz = pr.gs.node_coords()
+    if qf == "neohookean":
+        U = W.neohookean_displacement(xyz) * 10.0
+    else:
+        U = field(pr.gs, 1.0, 11, 2000)
+    dU = field(pr.gs, 1.0, 12, 3000)
+    Ug, dUg = sx.DeviceVector.from_numpy(U), sx.DeviceVector.from_numpy(dU)
+
+    R_ref = of.residual(U)
+    assert relerr(gf(0.0, Ug).numpy(), R_ref) < RTOL
+
+    R_lin = of.linearize(U)
+    out, grad = gf(0.0, Ug, wrt=0)
+    assert relerr(out.numpy(), R_lin) < RTOL
+    assert relerr(grad(dUg).numpy(), of.jvp(dU)) < RTOL
+    assert relerr(grad.diagonal().numpy(), of.diagonal()) < RTOL
+
+
+def test_byNODES_ordering_parity(gpu_lib):
+    pr = Pair(3, 2, 3, ordering=0)
+    gf, of = pr.forms("neohookean")
+    U = field(pr.gs, 0.05, 5, 2000)
+    dU = field(pr.gs, 1.0, 6, 3000)
+    of.linearize(U)
+    out, grad = gf(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    assert relerr(out.numpy(), of.residual(U)) < RTOL
+    assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
+
+
+def test_domain_subset_parity(gpu_lib):
+    pr = Pair(4, 2, 1)
+    elems = np.array([0, 3, 5, 17, 18, 40, 63], dtype=np.int32)
+    gf, of = pr.forms("heat", elems=elems)
+    U = field(pr.gs, 1.0, 5, 2000)
+    dU = field(pr.gs, 1.0, 6, 3000)
+    of.linearize(U)
+    out, grad = gf(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    assert relerr(out.numpy(), of.residual(U)) < RTOL
+    assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
+
+
+def test_unstructured_patch_mesh_parity(gpu_lib, patch_mesh):
+    X, hexes = patch_mesh
+    pr = Pair(0, 3, 3, mesh_arrays=(X, hexes))
+    gf, of = pr.forms("neohookean")
+    U = field(pr.gs, 0.05, 5, 2000)
+    dU = field(pr.gs, 1.0, 6, 3000)
+    of.linearize(U)
+    out, grad = gf(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    assert relerr(out.numpy(), of.residual(U)) < RTOL
+    assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
+
+
+def test_run_twice_bitwise_identical(gpu_lib):
+    pr = Pair(4, 3, 3)
+    gf, _ = pr.forms("neohookean")
+    U = sx.DeviceVector.from_numpy(W.neohookean_displacement(pr.gs.node_coords()))
+    dU = sx.DeviceVector.from_numpy(field(pr.gs, 1.0, 6, 3000))
+    r1 = gf(0.0, U).numpy()
+    r2 = gf(0.0, U).numpy()
+    assert np.array_equal(r1, r2)
+    _, grad = gf(0.0, U, wrt=0)
+    a, b = grad(dU).numpy(), grad(dU).numpy()
+    assert np.array_equal(a, b)
+
+
+def test_jacobi_cg_heat_matches_oracle(gpu_lib):
+    pr = Pair(6, 2, 1, perturb=0.0)
+    gf, of = pr.forms("heat")
+    xyz = pr.gs.node_coords()
+    T = W.heat_temperature(xyz)
+    ess = pr.gs.boundary_dofs(0b000001, 0x1)  # x = 0 face
+    assert np.array_equal(ess, np.sort(np.nonzero(np.abs(xyz[:, 0]) < 1e-12)[0]).astype(np.int32))
+    R = of.linearize(T)
+    R[ess] = 0.0
+    x_ref, rep_ref = of.cg(ess, R, rel=1e-10)
+    Tg = sx.DeviceVector.from_numpy(T)
+    out, grad = gf(0.0, Tg, wrt=0)
+    b = out.numpy()
+    b[ess] = 0.0
+    x = sx.DeviceVector(pr.gs.ndof)
+    rep = grad.cg(ess, sx.DeviceVector.from_numpy(b), x, rel_tol=1e-10)
+    assert rep["converged"] and rep_ref["converged"]
+    assert abs(rep["linear_iterations"] - rep_ref["iterations"]) <= 2
+    assert relerr(x.numpy(), x_ref) < 1e-8
+
+
+def test_newton_neohookean_matches_oracle(gpu_lib):
+    pr = Pair(3, 2, 3, perturb=0.2)
+    gf, of = pr.forms("neohookean")
+    xyz = pr.gs.node_coords()
+    # clamp x = 0, compress x = 1 by 1% in x (y, z free on that face)
+    ess0 = pr.gs.boundary_dofs(0b000001, 0x7)
+    ess1 = pr.gs.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    x0 = np.zeros(pr.gs.ndof)
+    x0[ess1] = -0.01
+    x_ref, rep_ref = of.newton(ess, x0, rel=1e-10, abs_tol=1e-14, lin_rel=1e-10, lin_abs=0.0)
+    xg = sx.DeviceVector.from_numpy(x0)
+    rep = gf.newton(xg, ess, rel_tol=1e-10, abs_tol=1e-14, lin_rel_tol=1e-10, lin_abs_tol=0.0)
+    assert rep["converged"] and rep_ref["converged"]
+    assert rep["newton_iterations"] == rep_ref["newton_iterations"]
+    assert relerr(xg.numpy(), x_ref) < 1e-9
+    # the converged state satisfies the oracle's residual
+    R = of.residual(xg.numpy())
+    R[ess] = 0.0
+    assert np.linalg.norm(R) <= 1e-9 * rep_ref["initial_norm"]
