@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Headline benchmark: matrix-free J.v of the Neo-Hookean p=3 form on one MI355X.
+
+Workload (default): Functional<H1<3,3>(H1<3,3>)> with MaterialStressFunctor<NeoHookean>
+(K=10, G=0.25), 64^3 hexes of [0,1]^3 with interior vertices perturbed by
+U[-0.2h, 0.2h] (seed 42), 4^3 Gauss points, FP64.  Displacement
+u = 0.01 sin(pi x)(1, 0.5, 0.25) + U[-1e-3, 1e-3]; direction U[-1, 1].
+A step is one application of the gradient operator (J.v) to the resident
+direction vector.  The residual and the linearization are timed alongside.
+
+Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+METRIC = "GDOF/s matrix-free J·v (p=3 hex, FP64 vs FP32); Newton solve s to 1e-8"
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (n, order, perturb, params)
+    "neohookean_p3_64": (64, 3, 0.2, (1.0, 10.0, 0.25)),
+    "neohookean_p4_48": (48, 4, 0.2, (1.0, 10.0, 0.25)),
+    "neohookean_p2_128": (128, 2, 0.0, (1.0, 10.0, 0.25)),
+}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n, p, perturb, params, seconds):
+    """The oracle's J.v (the reference algorithm restated in C++, one core)
+    on an n^3 sample of the same workload."""
+    import oracle as O
+    from serac_amd import workloads as W
+
+    m = O.Mesh.cuboid(n, n, n)
+    if perturb > 0:
+        m.perturb(42, perturb)
+    s = O.Space(m, p, 3)
+    f = O.Form(s)
+    f.add_domain_integral(O.QF_NEOHOOKEAN, params)
+    U = W.neohookean_displacement(s.node_coords())
+    dU = W.direction(s.nnodes, 3)
+    f.linearize(U)
+    t1 = f.time_jvp(dU, 1)
+    reps = max(3, int(seconds / max(t1, 1e-6)))
+    t = f.time_jvp(dU, reps)
+    per = t / reps
+    return {
+        "value": s.ndof / per / 1e9,
+        "unit": "GDOF/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"oracle J.v (stored 81-entry tangent, sum factorization), {n}^3 p={p} Neo-Hookean, "
+                  f"{s.ndof} DOFs, {reps} applications in {t:.1f} s on one host core",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="neohookean_p3_64", choices=sorted(WORKLOADS))
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-sample-n", type=int, default=16)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
+    ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a known-size read (dot of two L-vectors) and write (fill) for counter scaling")
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: this build runs the hot path on a single MI355X "
+              "(multi-GPU sharding is design-only); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import serac_amd as sx
+    from serac_amd import workloads as W
+
+    n, p, perturb, params = WORKLOADS[args.workload]
+    sx.capi.call("sx_init", 0)
+    dev = sx.device_name()
+    log(f"device {dev}; workload {args.workload}")
+
+    t0 = time.time()
+    mesh = sx.Mesh.cuboid(n, n, n)
+    if perturb > 0:
+        mesh.perturb(42, perturb)
+    space = sx.Space(mesh, p, 3)
+    form = sx.Functional(space, [space])
+    form.set_option(sx.capi.SX_OPT_SCATTER,
+                    sx.capi.SX_SCATTER_ATOMIC if args.scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
+    form.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
+    form.add_domain_integral(sx.SX_QF_NEOHOOKEAN, params)
+    xyz = space.node_coords()
+    U = sx.DeviceVector.from_numpy(W.neohookean_displacement(xyz))
+    dU = sx.DeviceVector.from_numpy(W.direction(space.nnodes, 3))
+    dR = sx.DeviceVector(space.ndof)
+    del xyz
+    log(f"setup {time.time() - t0:.1f} s: {space.ndof} DOFs, {space.ne} elements")
+
+    nq1 = p + 1
+    nqp = space.ne * nq1 ** 3
+
+    # residual and linearization (timed with the host clock around a sync)
+    form(0.0, U)
+    tr = time.perf_counter()
+    for _ in range(3):
+        form(0.0, U)
+    t_res = (time.perf_counter() - tr) / 3
+    form(0.0, U, wrt=0)
+    tl = time.perf_counter()
+    _, grad = form(0.0, U, wrt=0)
+    t_lin = time.perf_counter() - tl
+    tangent_bytes = grad.tangent_bytes()
+    log(f"residual {t_res * 1e3:.2f} ms, linearize {t_lin * 1e3:.2f} ms, tangent {tangent_bytes / 1e9:.2f} GB")
+
+    # J.v: warmup, then exactly `steps` applications bracketed by device syncs
+    if args.warmup > 0:
+        grad.apply_timed(dU, dR, args.warmup)
+    sx.capi.call("sx_device_synchronize")
+    tw = time.perf_counter()
+    ms_total, ms_kern, launches = grad.apply_timed(dU, dR, args.steps)
+    sx.capi.call("sx_device_synchronize")
+    wall = time.perf_counter() - tw
+    ms_step = ms_total / args.steps
+    value = W.gdof_per_s(space.ndof, ms_step / 1e3)
+    # dominant kernel: the colour launches of the J.v element kernel
+    d_sym = 45
+    b_alg = W.jvp_algorithmic_bytes(space.ndof, nqp, space.ne, space.npe, d_sym)
+    kern_ms_per_apply = ms_kern / args.steps
+    achieved = b_alg / (kern_ms_per_apply / 1e3) / 1e9
+    log(f"J.v {ms_step:.3f} ms/step ({value:.2f} GDOF/s), element kernels {kern_ms_per_apply:.3f} ms "
+        f"over {launches} launches, {achieved:.0f} GB/s algorithmic, wall {wall:.2f} s")
+
+    if args.calibrate:
+        # reference kernels for HBM counter scaling: k_dot_partial reads 2 * 8 * ndof
+        # bytes, k_fill writes 8 * ndof bytes
+        import ctypes
+
+        res = ctypes.c_double()
+        sx.capi.call("sx_vec_dot", U.ptr, dU.ptr, space.ndof, ctypes.byref(res))
+        dR.zero()
+        log(f"calibration: dot reads {2 * 8 * space.ndof} B, fill writes {8 * space.ndof} B")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        tc = time.time()
+        cpu = cpu_baseline(args.cpu_sample_n, p, perturb, params, args.cpu_seconds)
+        log(f"cpu baseline {cpu['value']:.5f} GDOF/s ({time.time() - tc:.1f} s)")
+
+    rec = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: Neo-Hookean J.v, {n}^3 hexes, p={p}, {nq1}^3 Gauss points, "
+                        f"perturbed {perturb}h",
+            "ndof": space.ndof,
+            "elements": space.ne,
+            "qpoints": nqp,
+            "parallelism": "1 GPU",
+            "scatter": args.scatter,
+            "xcd_remap": args.xcd_remap,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "kernel": "k_element<4,3,MODE_JVP> (one launch per element colour)",
+            "algorithmic_bytes_per_apply": b_alg,
+            "kernel_ms_per_apply": round(kern_ms_per_apply, 5),
+            "launches_per_apply": launches,
+        },
+        "cpu_baseline": cpu,
+        "residual_ms": round(t_res * 1e3, 3),
+        "linearize_ms": round(t_lin * 1e3, 3),
+        "tangent_bytes": tangent_bytes,
+        "device": dev,
+    }
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

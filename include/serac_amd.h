@@ -114,6 +114,18 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
  * wrt = i: also stores the quadrature-point linearization w.r.t. trial i and
  * returns its gradient operator in *grad_out (owned by the form). */
 int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out);
+/* Form options (no reference counterpart: the reference runs serially).
+ *   SX_OPT_SCATTER: SX_SCATTER_COLORED (default) launches one kernel per
+ *     element colour and adds into the output with plain read-modify-writes:
+ *     results are bitwise reproducible.  SX_SCATTER_ATOMIC launches all
+ *     elements at once and adds with FP64 atomics: fewer L-vector passes,
+ *     last-bit rounding depends on the order the hardware applies them.
+ *     Must be set before the first AddDomainIntegral.
+ *   SX_OPT_XCD_REMAP: 1 (default) gives each of the 8 XCDs a contiguous run of
+ *     element blocks (L2 locality for shared nodes); 0 keeps hardware order. */
+enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2 };
+enum { SX_SCATTER_COLORED = 0, SX_SCATTER_ATOMIC = 1 };
+int sx_form_set_option(sx_form* f, int option, int64_t value);
 int sx_form_synchronize(sx_form* f);
 int sx_form_stream(sx_form* f, void** hip_stream); /* the form's hipStream_t */
 int sx_form_destroy(sx_form* f);

@@ -118,7 +118,7 @@ inline void gll_nodes(int n, double* x)
 // the reference's closed forms for n <= 4 typed independently in the test.
 inline void gll_basis(int n, double x, double* B)
 {
-  double nodes[8];
+  double nodes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   gll_nodes(n, nodes);
   for (int i = 0; i < n; i++) {
     double v = 1.0;
@@ -130,7 +130,7 @@ inline void gll_basis(int n, double x, double* B)
 
 inline void gll_dbasis(int n, double x, double* G)
 {
-  double nodes[8];
+  double nodes[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   gll_nodes(n, nodes);
   for (int i = 0; i < n; i++) {
     double s = 0.0;
@@ -427,7 +427,8 @@ template <typename T>
 inline void interpolate(const Basis1D& b, int nc, const T* X, T* value, T* grad)
 {
   const int n = b.n, Q = b.q;
-  std::vector<T> A10(n * n * Q), A11(n * n * Q), A20(n * Q * Q), A21(n * Q * Q), A22(n * Q * Q);
+  thread_local std::vector<T> A10, A11, A20, A21, A22;  // scratch reused across elements
+  A10.resize(n * n * Q), A11.resize(n * n * Q), A20.resize(n * Q * Q), A21.resize(n * Q * Q), A22.resize(n * Q * Q);
   const int nq = Q * Q * Q;
   for (int c = 0; c < nc; c++) {
     const T* Xc = X + size_t(c) * n * n * n;
@@ -481,7 +482,8 @@ inline void interpolate(const Basis1D& b, int nc, const T* X, T* value, T* grad)
 inline void integrate(const Basis1D& b, int nc, const double* source, const double* flux, double* r)
 {
   const int n = b.n, Q = b.q, nq = Q * Q * Q;
-  std::vector<double> A20(Q * Q * n), A21(Q * Q * n), A22(Q * Q * n), A10(Q * n * n), A11(Q * n * n);
+  thread_local std::vector<double> A20, A21, A22, A10, A11;  // scratch reused across elements
+  A20.resize(Q * Q * n), A21.resize(Q * Q * n), A22.resize(Q * Q * n), A10.resize(Q * n * n), A11.resize(Q * n * n);
   for (int c = 0; c < nc; c++) {
     auto S = [&](int q) { return source ? source[size_t(c) * nq + q] : 0.0; };
     auto F = [&](int d, int q) { return flux ? flux[(size_t(c) * 3 + d) * nq + q] : 0.0; };

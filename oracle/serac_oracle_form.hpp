@@ -154,8 +154,9 @@ struct Form {
   void element_jvp(const Integral& I, size_t slot, const double* dUe, double* dRe) const
   {
     const int c = nc(), NQ = nq();
-    std::vector<double> val(size_t(c) * NQ), grd(size_t(c) * 3 * NQ), src(size_t(c) * NQ, 0.0),
-        flx(size_t(c) * 3 * NQ, 0.0);
+    thread_local std::vector<double> val, grd, src, flx;  // scratch reused across elements
+    val.resize(size_t(c) * NQ), grd.resize(size_t(c) * 3 * NQ), src.resize(size_t(c) * NQ);
+    flx.resize(size_t(c) * 3 * NQ);
     interpolate(basis, c, dUe, val.data(), grd.data());
     for (int q = 0; q < NQ; q++) {
       const double* D = &I.D[(slot * NQ + q) * I.dsize];

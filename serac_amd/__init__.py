@@ -214,6 +214,9 @@ class Functional:
         capi.call("sx_form_create", test.h, arr, len(self.trials), ctypes.byref(self.h))
         self.output = DeviceVector(test.ndof)
 
+    def set_option(self, option, value):
+        capi.call("sx_form_set_option", self.h, option, value)
+
     def add_domain_integral(self, qfunc, params, depends_on=(0,), elems=None):
         p = np.zeros(8)
         p[: len(params)] = params

@@ -19,6 +19,11 @@ SX_ERR_NOCONV = 5
 SX_ORDERING_BYNODES = 0
 SX_ORDERING_BYVDIM = 1
 
+SX_OPT_SCATTER = 1
+SX_OPT_XCD_REMAP = 2
+SX_SCATTER_COLORED = 0
+SX_SCATTER_ATOMIC = 1
+
 SX_QF_HEAT = 1
 SX_QF_MASS_DIFFUSION = 2
 SX_QF_NEOHOOKEAN = 3
@@ -74,6 +79,7 @@ SIGNATURES = {
     "sx_form_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                            ctypes.POINTER(_int), _int]),
     "sx_form_eval": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _vp, _int, ctypes.POINTER(_vp)]),
+    "sx_form_set_option": (_int, [_vp, _int, ctypes.c_int64]),
     "sx_form_synchronize": (_int, [_vp]),
     "sx_form_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_form_destroy": (_int, [_vp]),

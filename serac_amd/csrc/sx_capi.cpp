@@ -174,6 +174,8 @@ struct sx_form {
   std::vector<std::unique_ptr<Integral>> integrals;
   std::vector<std::unique_ptr<sx_grad>>  grads;
   hipStream_t                            stream = nullptr;
+  int                                    scatter   = SX_SCATTER_COLORED;
+  int                                    xcd_remap = 1;
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -206,6 +208,8 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   P.mode     = mode;
   P.layout   = I.layout;
   P.ordering = s->ordering;
+  P.scatter   = f->scatter;
+  P.xcd_remap = f->xcd_remap;
   P.map      = I.d_map.as<const int>();
   P.coords   = I.d_coords.as<const double>();
   P.nnodes   = s->nnodes;
@@ -284,7 +288,7 @@ int sx_device_name(char* buf, size_t n)
     ensure_device();
     hipDeviceProp_t prop;
     SX_HIP(hipGetDeviceProperties(&prop, rt().device));
-    std::snprintf(buf, n, "%s %s", prop.name, prop.gcnArchName);
+    std::snprintf(buf, n, "%s %s", prop.name[0] ? prop.name : "AMD GPU", prop.gcnArchName);
   });
 }
 
@@ -583,14 +587,20 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
       domain.resize(size_t(m.ne));
       for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
     }
-    // slots: elements of the domain sorted by colour (stable in element order)
+    // slots: elements of the domain sorted by colour (stable in element
+    // order); with atomic scatter, one range in element order
     std::vector<int> slots;
     slots.reserve(domain.size());
-    I->color_off.assign(size_t(vs->ncolors) + 1, 0);
-    for (int c = 0; c < vs->ncolors; c++) {
-      for (int e : domain)
-        if (vs->color[size_t(e)] == c) slots.push_back(e);
-      I->color_off[size_t(c) + 1] = int64_t(slots.size());
+    if (f->scatter == SX_SCATTER_ATOMIC) {
+      slots = domain;
+      I->color_off = {0, int64_t(slots.size())};
+    } else {
+      I->color_off.assign(size_t(vs->ncolors) + 1, 0);
+      for (int c = 0; c < vs->ncolors; c++) {
+        for (int e : domain)
+          if (vs->color[size_t(e)] == c) slots.push_back(e);
+        I->color_off[size_t(c) + 1] = int64_t(slots.size());
+      }
     }
     I->nslots = int64_t(slots.size());
     std::vector<int>    map(size_t(I->nslots) * vs->npe);
@@ -620,7 +630,9 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
     for (auto& I : f->integrals) {
       const bool lin = (wrt >= 0) && depends(*I, wrt);
       if (lin) {
-        const size_t bytes = size_t(I->nslots) * sxk::tangent_ds(f->test->nc, I->layout) * nq * sizeof(double);
+        const size_t bytes =
+            size_t(I->nslots) * size_t(sxk::tangent_slot_stride(sxk::tangent_ds(f->test->nc, I->layout), nq)) *
+            sizeof(double);
         I->d_tangent.allocate(bytes);
         I->has_tangent = true;
       }
@@ -631,6 +643,24 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
       f->grads[size_t(wrt)]->valid = true;
       if (grad_out) *grad_out = f->grads[size_t(wrt)].get();
     }
+  });
+}
+
+int sx_form_set_option(sx_form* f, int option, int64_t value)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    switch (option) {
+      case SX_OPT_SCATTER:
+        SX_REQUIRE(value == SX_SCATTER_COLORED || value == SX_SCATTER_ATOMIC, SX_ERR_ARG, "bad scatter mode");
+        SX_REQUIRE(f->integrals.empty(), SX_ERR_STATE, "scatter mode must be set before adding integrals");
+        f->scatter = int(value);
+        return;
+      case SX_OPT_XCD_REMAP:
+        f->xcd_remap = value ? 1 : 0;
+        return;
+    }
+    throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
   });
 }
 

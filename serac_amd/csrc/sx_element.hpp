@@ -36,6 +36,37 @@ __host__ __device__ constexpr int packed_index(int r, int s)
   return r * M - (r * (r - 1)) / 2 + (s - r);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
+// With remap on, XCD x instead works through a contiguous run of logical
+// blocks, so neighbouring elements (which share nodes) hit the same L2.
+__device__ __forceinline__ int64_t logical_block(int remap)
+{
+  const int64_t b = blockIdx.x, G = gridDim.x;
+  if (!remap) return b;
+  const int64_t per = G / 8;
+  if (b >= per * 8) return b;
+  return (b % 8) * per + b / 8;
+}
+
+// Workgroup barrier that orders LDS only.  __syncthreads() also fences global
+// memory, which makes every wave drain its outstanding global loads (vmcnt 0)
+// at each barrier; the stages here exchange data through LDS alone, so the
+// stored-tangent stream may stay in flight across the barriers.
+__device__ __forceinline__ void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__device__ __forceinline__ void scatter_add(double* r, int64_t o, double v, int atomic)
+{
+  if (atomic)
+    unsafeAtomicAdd(r + o, v);
+  else
+    r[o] += v;
+}
+
 __device__ __forceinline__ int64_t lidx(int64_t node, int c, int nc, int64_t nnodes, int ordering)
 {
   return ordering ? node * nc + c : int64_t(c) * nnodes + node;
@@ -157,7 +188,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   const int     i0     = li % N;
   const int     i1     = (li / N) % N;
   const int     i2     = li / (N * N);
-  const int64_t slot   = P.slot_begin + int64_t(blockIdx.x) * EB + el;
+  const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
   const bool    active = (el < EB) && (slot < P.slot_end);
   double*       S      = smem + (el < EB ? el : 0) * PER;
 
@@ -168,28 +199,39 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   const double* tGw = stab + 3 * N * N;
   const double* txq = stab + 4 * N * N;
 
-  // stored tangent for J.v: issue every load before the gather so the HBM
-  // stream overlaps the sum-factorization stages
-  double Dr[JVP ? DS : 1];
-  if constexpr (JVP) {
-    if (active) {
-      const double* Dp = P.D + size_t(slot) * DS * NQ + li;
-#pragma unroll
-      for (int k = 0; k < DS; k++) Dr[k] = __builtin_nontemporal_load(Dp + size_t(k) * NQ);
-    }
-  }
-
-  // ---- gather
+  // ---- gather: the nodal loads are issued first ...
   int64_t node = 0;
+  double  ug[NC];
   if (active) {
     node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-    for (int c = 0; c < NC; c++) S[c * NQ + li] = P.u[lidx(node, c, NC, P.nnodes, P.ordering)];
+    for (int c = 0; c < NC; c++) ug[c] = P.u[lidx(node, c, NC, P.nnodes, P.ordering)];
+  }
+  // ... then the stored tangent for J.v.  Memory counters retire in order, so
+  // waiting for the nodal values does not wait for the tangent stream, which
+  // stays in flight through the interpolation stages.
+  double Dr[JVP ? DS : 1];
+  if constexpr (JVP) {
+    if (active) {
+      typedef double dbl2 __attribute__((ext_vector_type(2)));
+      const double* Dp = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
+#pragma unroll
+      for (int k = 0; k + 1 < DS; k += 2) {
+        const dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(Dp + tangent_offset(k, li, DS, NQ)));
+        Dr[k]        = v.x;
+        Dr[k + 1]    = v.y;
+      }
+      if constexpr (DS & 1) Dr[DS - 1] = __builtin_nontemporal_load(Dp + tangent_offset(DS - 1, li, DS, NQ));
+    }
+  }
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) S[c * NQ + li] = ug[c];
     if constexpr (!JVP) {
       for (int k = li; k < 24; k += NQ) sxv[el * 24 + k] = P.coords[size_t(slot) * 24 + k];
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- interpolate, stage x: (z, y, qx)
   if (active) {
@@ -206,7 +248,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       S[2 * R + c * NQ + li] = s1;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- stage y: (z, qy, qx)
   if (active) {
@@ -228,7 +270,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       S[5 * R + c * NQ + li] = s2;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- stage z + quadrature point: (qz, qy, qx)
   if (active) {
@@ -288,13 +330,14 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
         double Ji[9];
         inv3(J, Ji);
         const double dJ = det3(J);
-        double*      Dp = P.D + size_t(slot) * DS * NQ + li;
+        double*      Dp = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
 #pragma unroll 1
         for (int s = 0; s < M; s++) {
           double col[M];
           qpoint_tangent_column<QF, NC, FULL>(P.params, Ji, dJ, val, gp, s, col);
           // keep the upper triangle: entry (r, s) for r <= s
-          for (int r = 0; r <= s; r++) __builtin_nontemporal_store(col[r], Dp + size_t(packed_index<M>(r, s)) * NQ);
+          for (int r = 0; r <= s; r++)
+            __builtin_nontemporal_store(col[r], Dp + tangent_offset(packed_index<M>(r, s), li, DS, NQ));
         }
       }
     }
@@ -306,7 +349,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       if constexpr (HAS_SRC) S[6 * R + c * NQ + li] = src[c];
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- integrate, stage qx: (qz, qy, x)
   if (active) {
@@ -331,7 +374,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       S[5 * R + c * NQ + li] = s2;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- stage qy: (qz, y, x)
   if (active) {
@@ -352,7 +395,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       S[1 * R + c * NQ + li] = s1;
     }
   }
-  __syncthreads();
+  lds_barrier();
 
   // ---- stage qz + scatter-add: (z, y, x)
   if (active) {
@@ -366,8 +409,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
         const int idx = (qz * N + i1) * N + i0;
         r += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
       }
-      const int64_t o = lidx(node, c, NC, P.nnodes, P.ordering);
-      P.r[o] += r;
+      scatter_add(P.r, lidx(node, c, NC, P.nnodes, P.ordering), r, P.scatter);
     }
   }
 }
@@ -392,10 +434,10 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   const int     el     = tid / NQ;
   const int     li     = tid - el * NQ;
   const int     jx     = li % N, jy = (li / N) % N, jz = li / (N * N);
-  const int64_t slot   = P.slot_begin + int64_t(blockIdx.x) * EB + el;
+  const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
   const bool    active = (el < EB) && (slot < P.slot_end);
   for (int k = tid; k < TAB; k += blockDim.x) stab[k] = P.tables[k];
-  __syncthreads();
+  lds_barrier();
   if (!active) return;
   const double* tB  = stab;
   const double* tG  = stab + N * N;
@@ -405,7 +447,8 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) acc[c] = 0.0;
-  const double* Dbase = P.D + size_t(slot) * DS * NQ;
+  const double* Dbase = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
+  auto          Dat   = [&](int k, int q) { return Dbase[tangent_offset(k, q, DS, NQ)]; };
   for (int q = 0; q < NQ; q++) {
     const int    qx = q % N, qy = (q / N) % N, qz = q / (N * N);
     const double bx = tB[qx * N + jx], by = tB[qy * N + jy], bz = tB[qz * N + jz];
@@ -424,15 +467,15 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
         for (int j = 0; j < 3; j++) {
           const int r = VO + c * 3 + i, s = VO + c * 3 + j;
           const int k = r <= s ? packed_index<M>(r, s) : packed_index<M>(s, r);
-          a += tg[i] * Dbase[size_t(k) * NQ + q] * ug[j];
+          a += tg[i] * Dat(k, q) * ug[j];
         }
       if constexpr (FULL) {
-        a += tv * Dbase[size_t(packed_index<M>(c, c)) * NQ + q] * uv;
+        a += tv * Dat(packed_index<M>(c, c), q) * uv;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
           const int s = VO + c * 3 + j;
           const int k = packed_index<M>(c, s);  // c < s always
-          a += tv * Dbase[size_t(k) * NQ + q] * ug[j] + tg[j] * Dbase[size_t(k) * NQ + q] * uv;
+          a += tv * Dat(k, q) * ug[j] + tg[j] * Dat(k, q) * uv;
         }
       }
       acc[c] += a;
@@ -440,7 +483,7 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   }
   const int64_t node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-  for (int c = 0; c < NC; c++) P.r[lidx(node, c, NC, P.nnodes, P.ordering)] += acc[c];
+  for (int c = 0; c < NC; c++) scatter_add(P.r, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
 }
 
 template <int N, int NC, int MODE, class QF, int LAYOUT>

@@ -18,6 +18,8 @@ enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_I
 
 struct ElemLaunch {
   int           order, nc, qf, mode, layout, ordering;
+  int           scatter;    // 0: plain read-modify-write (colour launches), 1: FP64 atomics
+  int           xcd_remap;  // 1: give each XCD a contiguous range of element blocks
   const int*    map;     // [slot][npe] global node ids
   const double* coords;  // [slot][8 corners][3], corner = a + 2b + 4c
   int64_t       slot_begin, slot_end;
@@ -36,6 +38,17 @@ inline int tangent_ds(int nc, int layout)
 {
   int m = tangent_m(nc, layout);
   return m * (m + 1) / 2;
+}
+
+// Tangent storage per slot (element): the DS packed entries of every
+// quadrature point q, grouped in pairs so that lane q reads 16 bytes at once:
+//   entry k < 2*floor(DS/2):  [(k/2)][q][k%2]
+//   last entry when DS odd:   [floor(DS/2)][q]  (after the pairs)
+// The slot stride is DS*NQ rounded up to even (16-byte alignment).
+__host__ __device__ constexpr int64_t tangent_slot_stride(int ds, int nq) { return (int64_t(ds) * nq + 1) & ~int64_t(1); }
+__host__ __device__ constexpr int tangent_offset(int k, int q, int ds, int nq)
+{
+  return (k < (ds & ~1)) ? (k >> 1) * 2 * nq + q * 2 + (k & 1) : (ds >> 1) * 2 * nq + q;
 }
 
 // defined in sx_kernels_p{1,2,3,4}.hip; return 0 when the combination is not compiled

@@ -147,6 +147,37 @@ def test_unstructured_patch_mesh_parity(gpu_lib, patch_mesh):
     assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
 
 
+@pytest.mark.parametrize("qf,p", [("heat", 2), ("neohookean", 3), ("mass_diffusion_vec", 1)])
+@pytest.mark.parametrize("remap", [0, 1])
+def test_atomic_scatter_parity(gpu_lib, qf, p, remap):
+    nc = QF[qf][2]
+    pr = Pair(5, p, nc)
+    gid, oid, _, params = QF[qf]
+    gf = sx.Functional(pr.gs, [pr.gs])
+    gf.set_option(sx.capi.SX_OPT_SCATTER, sx.capi.SX_SCATTER_ATOMIC)
+    gf.set_option(sx.capi.SX_OPT_XCD_REMAP, remap)
+    gf.add_domain_integral(gid, params)
+    of = O.Form(pr.os)
+    of.add_domain_integral(oid, params)
+    U = field(pr.gs, 0.005, 5, 2000)  # small enough that no element inverts
+    dU = field(pr.gs, 1.0, 6, 3000)
+    R_ref = of.linearize(U)
+    assert np.isfinite(R_ref).all()
+    out, grad = gf(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    assert relerr(out.numpy(), R_ref) < RTOL
+    assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
+    assert relerr(grad.diagonal().numpy(), of.diagonal()) < RTOL
+
+
+def test_scatter_mode_must_precede_integrals(gpu_lib):
+    pr = Pair(2, 1, 1)
+    gf = sx.Functional(pr.gs, [pr.gs])
+    gf.add_domain_integral(sx.SX_QF_HEAT, (1.0, 1.0))
+    with pytest.raises(sx.SeracAmdError) as e:
+        gf.set_option(sx.capi.SX_OPT_SCATTER, sx.capi.SX_SCATTER_ATOMIC)
+    assert e.value.code == sx.capi.SX_ERR_STATE
+
+
 def test_run_twice_bitwise_identical(gpu_lib):
     pr = Pair(4, 3, 3)
     gf, _ = pr.forms("neohookean")
