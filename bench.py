@@ -4,7 +4,8 @@
 Workload (default): Functional<H1<3,3>(H1<3,3>)> with MaterialStressFunctor<NeoHookean>
 (K=10, G=0.25), 64^3 hexes of [0,1]^3 with interior vertices perturbed by
 U[-0.2h, 0.2h] (seed 42), 4^3 Gauss points, FP64.  Displacement
-u = 0.01 sin(pi x)(1, 0.5, 0.25) + U[-1e-3, 1e-3]; direction U[-1, 1].
+u = 0.01 sin(pi x)(1, 0.5, 0.25) + U[-a, a] with a = 1e-3 * 16 / n (smooth: no
+inverted elements); direction U[-1, 1].
 A step is one application of the gradient operator (J.v) to the resident
 direction vector.  The residual and the linearization are timed alongside.
 
@@ -36,6 +37,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+# HBM traffic of the J.v kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+# (scaled by plain kernels of known size); see the summary file for the commands.
+PROFILED_TRAFFIC = {
+    ("neohookean_p3_64", "atomic", 1): "profiles/jvp_neohookean_p3_64.md",
+}
+
+
+def profiled_traffic(workload, scatter, remap):
+    path = PROFILED_TRAFFIC.get((workload, scatter, remap))
+    if not path or not os.path.exists(os.path.join(ROOT, path)):
+        return None
+    for line in open(os.path.join(ROOT, path)):
+        if "traffic_bytes_per_launch:" in line:
+            return int(line.split("traffic_bytes_per_launch:")[1].split()[0])
+    return None
+
+
 def cpu_baseline(n, p, perturb, params, seconds):
     """The oracle's J.v (the reference algorithm restated in C++, one core)
     on an n^3 sample of the same workload."""
@@ -48,7 +66,7 @@ def cpu_baseline(n, p, perturb, params, seconds):
     s = O.Space(m, p, 3)
     f = O.Form(s)
     f.add_domain_integral(O.QF_NEOHOOKEAN, params)
-    U = W.neohookean_displacement(s.node_coords())
+    U = W.neohookean_displacement(s.node_coords(), noise=W.noise_for_mesh(n))
     dU = W.direction(s.nnodes, 3)
     f.linearize(U)
     t1 = f.time_jvp(dU, 1)
@@ -104,7 +122,7 @@ def main():
     form.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
     form.add_domain_integral(sx.SX_QF_NEOHOOKEAN, params)
     xyz = space.node_coords()
-    U = sx.DeviceVector.from_numpy(W.neohookean_displacement(xyz))
+    U = sx.DeviceVector.from_numpy(W.neohookean_displacement(xyz, noise=W.noise_for_mesh(n)))
     dU = sx.DeviceVector.from_numpy(W.direction(space.nnodes, 3))
     dR = sx.DeviceVector(space.ndof)
     del xyz
@@ -136,6 +154,11 @@ def main():
     wall = time.perf_counter() - tw
     ms_step = ms_total / args.steps
     value = W.gdof_per_s(space.ndof, ms_step / 1e3)
+    jv = dR.numpy()
+    if not np.isfinite(jv).all() or not np.isfinite(form(0.0, U).numpy()).all():
+        raise SystemExit("bench: non-finite residual or J.v; the measurement is invalid")
+    jv_norm = float(np.linalg.norm(jv))
+    del jv
     # dominant kernel: the colour launches of the J.v element kernel
     d_sym = 45
     b_alg = W.jvp_algorithmic_bytes(space.ndof, nqp, space.ne, space.npe, d_sym)
@@ -189,8 +212,10 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "k_element<4,3,MODE_JVP> (one launch per element colour)",
+            "traffic": profiled_traffic(args.workload, args.scatter, args.xcd_remap),
+            "traffic_source": PROFILED_TRAFFIC.get((args.workload, args.scatter, args.xcd_remap)),
+            "kernel": "k_element<4,3,MODE_JVP>" + (" (one launch per element colour)" if args.scatter == "colored"
+                                                   else " (one launch, atomic scatter)"),
             "algorithmic_bytes_per_apply": b_alg,
             "kernel_ms_per_apply": round(kern_ms_per_apply, 5),
             "launches_per_apply": launches,
@@ -199,6 +224,7 @@ def main():
         "residual_ms": round(t_res * 1e3, 3),
         "linearize_ms": round(t_lin * 1e3, 3),
         "tangent_bytes": tangent_bytes,
+        "jv_norm": jv_norm,
         "device": dev,
     }
     print(json.dumps(rec), flush=True)

@@ -61,6 +61,15 @@ CONFIGS = {
 }
 
 
+def noise_for_mesh(n):
+    """Nodal noise amplitude for an n^3 mesh: 1e-3 at n = 16, scaled with h.
+
+    A fixed +-1e-3 on the 64^3 p=3 mesh (Lobatto node spacing ~0.0043) gives
+    displacement gradients of ~0.6 and inverted elements (det(I + grad u) < 0,
+    NaN in log1p); scaling with h keeps the field a small perturbation."""
+    return 1e-3 * 16.0 / n
+
+
 def neohookean_displacement(xyz, seed=42, noise=1e-3):
     """u = 0.01 sin(pi x) (1, 0.5, 0.25) + U[-noise, noise] per component (byVDIM)."""
     nn = xyz.shape[0]

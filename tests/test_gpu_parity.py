@@ -112,6 +112,38 @@ def test_residual_jvp_diagonal_parity(gpu_lib, p, qf):
     assert relerr(grad.diagonal().numpy(), of.diagonal()) < RTOL
 
 
+GOLDEN = {
+    # name: (mesh, order, components, qfunction)
+    "heat_p2_cube3": ("cube3", 2, 1, sx.SX_QF_HEAT, (1.0, 1.0)),
+    "neohookean_p3_cube2": ("cube2", 3, 3, sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25)),
+    "neohookean_p2_patch": ("patch", 2, 3, sx.SX_QF_NEOHOOKEAN, (1.0, 1.0, 1.0)),
+    "massdiff_p4_cube2": ("cube2", 4, 1, sx.SX_QF_MASS_DIFFUSION, (1.7, 2.1)),
+}
+
+
+@pytest.mark.parametrize("name", sorted(GOLDEN))
+def test_golden_vectors(gpu_lib, name, patch_mesh):
+    import os
+
+    gold = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_vectors.npz"))
+    kind, p, nc, qf, params = GOLDEN[name]
+    if kind == "patch":
+        mesh = sx.Mesh.from_arrays(*patch_mesh)
+    else:
+        n = int(kind[-1])
+        mesh = sx.Mesh.cuboid(n, n, n)
+        mesh.perturb(42, 0.2)
+    space = sx.Space(mesh, p, nc)
+    assert np.array_equal(space.element_nodes(), gold[f"{name}/emap"])
+    f = sx.Functional(space, [space])
+    f.add_domain_integral(qf, params)
+    U = sx.DeviceVector.from_numpy(gold[f"{name}/U"])
+    assert relerr(f(0.0, U).numpy(), gold[f"{name}/R"]) < RTOL
+    _, grad = f(0.0, U, wrt=0)
+    assert relerr(grad(sx.DeviceVector.from_numpy(gold[f"{name}/dU"])).numpy(), gold[f"{name}/JdU"]) < RTOL
+    assert relerr(grad.diagonal().numpy(), gold[f"{name}/diag"]) < RTOL
+
+
 def test_byNODES_ordering_parity(gpu_lib):
     pr = Pair(3, 2, 3, ordering=0)
     gf, of = pr.forms("neohookean")
