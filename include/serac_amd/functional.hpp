@@ -1,0 +1,569 @@
+// serac_amd/functional.hpp -- C++ drop-in front end for the matrix-free path.
+//
+// Reproduces the surface of serac::Functional<test(trials...)>
+// (src/serac/numerics/functional/functional.hpp:189-745) over the C-ABI of
+// serac_amd.h: H1<p, c>, Dimension<d>, DependsOn<i...>, differentiate_wrt,
+// AddDomainIntegral, operator() returning the (reused) output vector or a
+// tuple of it and the Gradient, and Gradient::Mult / operator().  Integrands
+// are the compiled registry of the library: the reference's functor types
+// (MaterialStressFunctor<NeoHookean>, ThermalMaterialIntegrand<...>, ...)
+// are mirrored here as plain structs and mapped to qfunction ids by
+// qfunction_traits.  Vectors live in device memory (serac_amd::Vector stands
+// in for mfem::Vector with device storage); calls are synchronous, like the
+// reference's.
+//
+// Header-only; link against libserac_amd.so.
+#pragma once
+
+#include <array>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../serac_amd.h"
+
+namespace serac_amd {
+
+/// Errors from the library, with the status code of serac_amd.h.  The
+/// reference aborts through SLIC_ERROR; here the C++ caller gets an exception.
+class Error : public std::runtime_error {
+ public:
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+  int code;
+};
+
+namespace detail {
+inline void check(int rc)
+{
+  if (rc != SX_OK) throw Error(rc, sx_last_error());
+}
+}  // namespace detail
+
+// ---------------------------------------------------------------------------
+// device vector
+// ---------------------------------------------------------------------------
+class Vector {
+ public:
+  Vector() = default;
+  explicit Vector(int64_t n) { SetSize(n); }
+  explicit Vector(const std::vector<double>& host) { SetFromHost(host); }
+  Vector(const Vector& o) { *this = o; }
+  Vector(Vector&& o) noexcept : data_(o.data_), n_(o.n_) { o.data_ = nullptr, o.n_ = 0; }
+  Vector& operator=(const Vector& o)
+  {
+    if (this != &o) {
+      SetSize(o.n_);
+      if (n_) detail::check(sx_memcpy_dtod(data_, o.data_, size_t(n_) * sizeof(double)));
+    }
+    return *this;
+  }
+  Vector& operator=(Vector&& o) noexcept
+  {
+    std::swap(data_, o.data_);
+    std::swap(n_, o.n_);
+    return *this;
+  }
+  ~Vector()
+  {
+    if (data_) sx_free(data_);
+  }
+
+  /// resize to n entries, all zero
+  void SetSize(int64_t n)
+  {
+    if (n != n_) {
+      if (data_) sx_free(data_);
+      data_ = nullptr;
+      n_    = n;
+      if (n > 0) {
+        void* p = nullptr;
+        detail::check(sx_malloc(&p, size_t(n) * sizeof(double)));
+        data_ = static_cast<double*>(p);
+      }
+    }
+    if (n_ > 0) detail::check(sx_memset_zero(data_, size_t(n_) * sizeof(double)));
+  }
+  int64_t       Size() const { return n_; }
+  double*       Data() { return data_; }
+  const double* Data() const { return data_; }
+  const double* Read() const { return data_; }
+  double*       ReadWrite() { return data_; }
+
+  void SetFromHost(const std::vector<double>& h)
+  {
+    if (int64_t(h.size()) != n_) SetSize(int64_t(h.size()));
+    if (n_) detail::check(sx_memcpy_htod(data_, h.data(), h.size() * sizeof(double)));
+  }
+  std::vector<double> ToHost() const
+  {
+    std::vector<double> h(static_cast<size_t>(n_));
+    if (n_) detail::check(sx_memcpy_dtoh(h.data(), data_, h.size() * sizeof(double)));
+    return h;
+  }
+  /// y += a x (this = y)
+  void Add(double a, const Vector& x) { detail::check(sx_vec_axpy(a, x.data_, data_, n_)); }
+  double operator*(const Vector& o) const
+  {
+    double r = 0.0;
+    detail::check(sx_vec_dot(data_, o.data_, n_, &r));
+    return r;
+  }
+
+ private:
+  double* data_ = nullptr;
+  int64_t n_    = 0;
+};
+
+// ---------------------------------------------------------------------------
+// meshes and spaces (mfem::Mesh / ParFiniteElementSpace stand-ins)
+// ---------------------------------------------------------------------------
+class Mesh {
+ public:
+  /// buildCuboidMesh (src/serac/mesh/mesh_utils.cpp:149-153)
+  static Mesh Cuboid(int nx, int ny, int nz, double lx = 1.0, double ly = 1.0, double lz = 1.0)
+  {
+    Mesh m;
+    detail::check(sx_mesh_create_cuboid(nx, ny, nz, lx, ly, lz, &m.h_));
+    return m;
+  }
+  /// vertices [nv][3]; hexes [ne][8] in mfem hexahedron vertex order
+  Mesh(const std::vector<double>& vertices, const std::vector<int>& hexes)
+  {
+    detail::check(sx_mesh_create(int(vertices.size() / 3), vertices.data(), int(hexes.size() / 8), hexes.data(), &h_));
+  }
+  Mesh(Mesh&& o) noexcept : h_(o.h_) { o.h_ = nullptr; }
+  Mesh& operator=(Mesh&& o) noexcept
+  {
+    std::swap(h_, o.h_);
+    return *this;
+  }
+  Mesh(const Mesh&)            = delete;
+  Mesh& operator=(const Mesh&) = delete;
+  ~Mesh()
+  {
+    if (h_) sx_mesh_destroy(h_);
+  }
+  void    Perturb(uint64_t seed, double amp) { detail::check(sx_mesh_perturb(h_, seed, amp)); }
+  int64_t GetNE() const
+  {
+    int64_t nv = 0, ne = 0;
+    detail::check(sx_mesh_info(h_, &nv, &ne));
+    return ne;
+  }
+  sx_mesh* handle() const { return h_; }
+
+ private:
+  Mesh() = default;
+  sx_mesh* h_ = nullptr;
+};
+
+namespace Ordering {
+enum Type { byNODES = SX_ORDERING_BYNODES, byVDIM = SX_ORDERING_BYVDIM };
+}
+
+class FiniteElementSpace {
+ public:
+  FiniteElementSpace(Mesh& mesh, int order, int vdim = 1, Ordering::Type ordering = Ordering::byVDIM)
+  {
+    detail::check(sx_space_create_h1_hex(mesh.handle(), order, vdim, ordering, &h_));
+  }
+  FiniteElementSpace(const FiniteElementSpace&)            = delete;
+  FiniteElementSpace& operator=(const FiniteElementSpace&) = delete;
+  ~FiniteElementSpace()
+  {
+    if (h_) sx_space_destroy(h_);
+  }
+  int64_t GetTrueVSize() const
+  {
+    int64_t nn = 0, nd = 0, ne = 0;
+    int     npe = 0;
+    detail::check(sx_space_info(h_, &nn, &nd, &ne, &npe));
+    return nd;
+  }
+  int64_t GetVSize() const { return GetTrueVSize(); }  // one rank: L = T
+  int64_t GetNDofs() const
+  {
+    int64_t nn = 0, nd = 0, ne = 0;
+    int     npe = 0;
+    detail::check(sx_space_info(h_, &nn, &nd, &ne, &npe));
+    return nn;
+  }
+  std::vector<double> NodeCoordinates() const
+  {
+    std::vector<double> x(size_t(GetNDofs()) * 3);
+    detail::check(sx_space_node_coords(h_, x.data()));
+    return x;
+  }
+  /// essential DOFs on faces of a cuboid mesh (bit 0 x=0, 1 x=max, ...)
+  std::vector<int32_t> BoundaryDofs(int face_mask, int comp_mask = 0x7) const
+  {
+    int64_t n = 0;
+    detail::check(sx_space_boundary_dofs(h_, face_mask, comp_mask, nullptr, 0, &n));
+    std::vector<int32_t> d(static_cast<size_t>(n));
+    detail::check(sx_space_boundary_dofs(h_, face_mask, comp_mask, d.data(), n, &n));
+    return d;
+  }
+  sx_space* handle() const { return h_; }
+
+ private:
+  sx_space* h_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// function-space tags and argument wrappers (fn/finite_element.hpp:195-200,
+// fn/differentiate_wrt.hpp:12-42)
+// ---------------------------------------------------------------------------
+template <int p, int c = 1>
+struct H1 {
+  static constexpr int order      = p;
+  static constexpr int components = c;
+};
+template <int d>
+struct Dimension {
+  static constexpr int value = d;
+};
+template <int... i>
+struct DependsOn {
+};
+
+template <typename T>
+struct DifferentiateWRT {
+  const T& value;
+};
+template <typename T>
+DifferentiateWRT<T> differentiate_wrt(const T& v)
+{
+  return DifferentiateWRT<T>{v};
+}
+
+/// a subset of the mesh's elements (serac::Domain for volume elements)
+struct Domain {
+  std::vector<int> elements;  // empty: every element
+  static Domain    Elements(std::vector<int> ids) { return Domain{std::move(ids)}; }
+};
+
+// ---------------------------------------------------------------------------
+// integrands of the compiled registry
+// ---------------------------------------------------------------------------
+enum class GeometricNonlinearities { On, Off };  // phy/common.hpp:31
+
+namespace heat_transfer {
+/// phy/materials/thermal_material.hpp:20-55
+struct LinearIsotropicConductor {
+  LinearIsotropicConductor(double density = 1.0, double specific_heat_capacity = 1.0, double conductivity = 1.0)
+      : density_(density), specific_heat_capacity_(specific_heat_capacity), conductivity_(conductivity)
+  {
+    if (conductivity_ < 0.0 || density_ < 0.0 || specific_heat_capacity_ < 0.0)
+      throw Error(SX_ERR_ARG, "LinearIsotropicConductor: negative property");
+  }
+  double density_, specific_heat_capacity_, conductivity_;
+};
+/// phy/heat_transfer.hpp:381-406 (steady state: the heat-capacity term is 0)
+template <typename Material>
+struct ThermalMaterialIntegrand {
+  explicit ThermalMaterialIntegrand(Material m) : material_(m) {}
+  Material material_;
+};
+/// the integrand setSource adds (phy/heat_transfer.hpp:493-503) for a constant source
+struct ConstantSource {
+  double source;
+};
+}  // namespace heat_transfer
+
+namespace solid_mechanics {
+/// phy/materials/solid_material.hpp:98-127
+struct NeoHookean {
+  double density;
+  double K;
+  double G;
+};
+/// phy/materials/solid_material.hpp:24-50
+struct LinearIsotropic {
+  double density;
+  double K;
+  double G;
+};
+/// phy/solid_mechanics.hpp:808-853 (quasi-static: no acceleration argument)
+template <typename Material>
+struct MaterialStressFunctor {
+  MaterialStressFunctor(Material m, GeometricNonlinearities gn) : material_(m), geom_nonlin_(gn) {}
+  Material                material_;
+  GeometricNonlinearities geom_nonlin_;
+};
+}  // namespace solid_mechanics
+
+/// {a u, b grad u}, the integrand of fn/tests/functional_comparisons.cpp:84-203
+struct MassDiffusion {
+  double a, b;
+};
+
+/// Maps an integrand type to (qfunction id, parameters).  Unregistered types
+/// fail at compile time: the C-ABI cannot carry arbitrary lambdas.
+template <typename Integrand>
+struct qfunction_traits {
+  static constexpr bool registered = false;
+};
+
+template <>
+struct qfunction_traits<heat_transfer::ThermalMaterialIntegrand<heat_transfer::LinearIsotropicConductor>> {
+  static constexpr bool registered = true;
+  static int            id(const heat_transfer::ThermalMaterialIntegrand<heat_transfer::LinearIsotropicConductor>&)
+  {
+    return SX_QF_HEAT;
+  }
+  static std::vector<double> params(const heat_transfer::ThermalMaterialIntegrand<heat_transfer::LinearIsotropicConductor>& q)
+  {
+    return {q.material_.conductivity_, 0.0};
+  }
+};
+template <>
+struct qfunction_traits<heat_transfer::ConstantSource> {
+  static constexpr bool      registered = true;
+  static int                 id(const heat_transfer::ConstantSource&) { return SX_QF_HEAT; }
+  static std::vector<double> params(const heat_transfer::ConstantSource& q) { return {0.0, q.source}; }
+};
+template <>
+struct qfunction_traits<solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>> {
+  static constexpr bool registered = true;
+  static int            id(const solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>& q)
+  {
+    if (q.geom_nonlin_ != GeometricNonlinearities::On)
+      throw Error(SX_ERR_UNSUPPORTED, "NeoHookean is compiled with geometric nonlinearity on");
+    return SX_QF_NEOHOOKEAN;
+  }
+  static std::vector<double> params(const solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>& q)
+  {
+    return {q.material_.density, q.material_.K, q.material_.G};
+  }
+};
+template <>
+struct qfunction_traits<solid_mechanics::MaterialStressFunctor<solid_mechanics::LinearIsotropic>> {
+  static constexpr bool registered = true;
+  static int            id(const solid_mechanics::MaterialStressFunctor<solid_mechanics::LinearIsotropic>& q)
+  {
+    if (q.geom_nonlin_ != GeometricNonlinearities::Off)
+      throw Error(SX_ERR_UNSUPPORTED, "LinearIsotropic is compiled with geometric nonlinearity off");
+    return SX_QF_LINEAR_ELASTIC;
+  }
+  static std::vector<double> params(const solid_mechanics::MaterialStressFunctor<solid_mechanics::LinearIsotropic>& q)
+  {
+    return {q.material_.K - (2.0 / 3.0) * q.material_.G, q.material_.G};
+  }
+};
+template <>
+struct qfunction_traits<MassDiffusion> {
+  static constexpr bool      registered = true;
+  static int                 id(const MassDiffusion&) { return SX_QF_MASS_DIFFUSION; }
+  static std::vector<double> params(const MassDiffusion& q) { return {q.a, q.b}; }
+};
+
+namespace detail {
+template <typename T>
+struct is_dwrt : std::false_type {
+};
+template <typename T>
+struct is_dwrt<DifferentiateWRT<T>> : std::true_type {
+};
+template <typename T>
+const double* data_of(const T& v)
+{
+  if constexpr (is_dwrt<T>::value)
+    return v.value.Data();
+  else
+    return v.Data();
+}
+template <typename... T>
+constexpr int wrt_index()
+{
+  int  idx = -1, i = 0;
+  bool f[] = {is_dwrt<T>::value..., false};
+  for (bool b : f) {
+    if (b && idx < 0) idx = i;
+    i++;
+  }
+  return idx;
+}
+template <typename... T>
+constexpr int count_dwrt()
+{
+  return (0 + ... + int(is_dwrt<T>::value));
+}
+}  // namespace detail
+
+// ---------------------------------------------------------------------------
+// Functional<test(trials...)>
+// ---------------------------------------------------------------------------
+template <typename T>
+class Functional;
+
+template <typename test, typename... trials>
+class Functional<test(trials...)> {
+  static constexpr int num_trial_spaces = int(sizeof...(trials));
+
+ public:
+  /// Functional::Gradient (fn/functional.hpp:548-669): the action of the
+  /// stored linearization, its diagonal, and Jacobi-CG on it.
+  class Gradient {
+   public:
+    /// df = J dx  (Gradient::Mult, fn/functional.hpp:556-559)
+    void Mult(const Vector& dx, Vector& df) const
+    {
+      if (df.Size() != test_size_) df.SetSize(test_size_);
+      detail::check(sx_grad_apply(h_, dx.Data(), df.Data()));
+      detail::check(sx_form_synchronize(form_));
+    }
+    /// J dx into an internal vector (Gradient::operator(), :562-566)
+    Vector& operator()(const Vector& dx)
+    {
+      Mult(dx, df_);
+      return df_;
+    }
+    /// the diagonal the assembled matrix would have (Jacobi preconditioner)
+    Vector AssembleDiagonal() const
+    {
+      Vector d(test_size_);
+      detail::check(sx_grad_diagonal(h_, d.Data()));
+      detail::check(sx_form_synchronize(form_));
+      return d;
+    }
+    int64_t  Height() const { return test_size_; }
+    sx_grad* handle() const { return h_; }
+
+   private:
+    friend class Functional;
+    sx_grad* h_         = nullptr;
+    sx_form* form_      = nullptr;
+    int64_t  test_size_ = 0;
+    Vector   df_;
+  };
+
+  /// fn/functional.hpp:219-273
+  Functional(const FiniteElementSpace* test_fes, std::array<const FiniteElementSpace*, num_trial_spaces> trial_fes)
+  {
+    std::array<const sx_space*, num_trial_spaces> t{};
+    for (int i = 0; i < num_trial_spaces; i++) t[size_t(i)] = trial_fes[size_t(i)]->handle();
+    detail::check(sx_form_create(test_fes->handle(), t.data(), num_trial_spaces, &h_));
+    test_size_ = test_fes->GetTrueVSize();
+    output_.SetSize(test_size_);
+  }
+  Functional(const Functional&)            = delete;
+  Functional& operator=(const Functional&) = delete;
+  ~Functional()
+  {
+    if (h_) sx_form_destroy(h_);
+  }
+
+  /// AddDomainIntegral (fn/functional.hpp:286-317), whole mesh or a Domain
+  template <int dim, int... args, typename Integrand>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&)
+  {
+    AddDomainIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, Domain{});
+  }
+  template <int dim, int... args, typename Integrand>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Domain& domain)
+  {
+    static_assert(dim == 3, "the compiled kernels integrate over hexahedra (Dimension<3>)");
+    static_assert(qfunction_traits<Integrand>::registered,
+                  "integrand type is not in the compiled qfunction registry (see qfunction_traits)");
+    static_assert(((args >= 0 && args < num_trial_spaces) && ...), "DependsOn index out of range");
+    const int                 deps[] = {args...};
+    const std::vector<double> p      = qfunction_traits<Integrand>::params(integrand);
+    detail::check(sx_form_add_domain_integral(h_, qfunction_traits<Integrand>::id(integrand), p.data(), int(p.size()),
+                                              deps, int(sizeof...(args)),
+                                              domain.elements.empty() ? nullptr : domain.elements.data(),
+                                              int(domain.elements.size())));
+  }
+  /// alias of the reference (fn/functional.hpp:365-373)
+  template <int... args, typename Integrand, typename DomainT>
+  void AddVolumeIntegral(DependsOn<args...>, const Integrand& integrand, DomainT& domain)
+  {
+    AddDomainIntegral(Dimension<3>{}, DependsOn<args...>{}, integrand, domain);
+  }
+
+  /// scatter mode / XCD block order (serac_amd.h, SX_OPT_*)
+  void SetOption(int option, int64_t value) { detail::check(sx_form_set_option(h_, option, value)); }
+
+  /// operator()(t, args...) (fn/functional.hpp:501-513).  Returns the output
+  /// vector, which the next call overwrites, or a tuple of it and the
+  /// Gradient when one argument is wrapped by differentiate_wrt.
+  template <typename... T>
+  decltype(auto) operator()(double t, const T&... args)
+  {
+    static_assert(int(sizeof...(T)) == num_trial_spaces, "wrong number of arguments");
+    static_assert(detail::count_dwrt<T...>() <= 1, "only one argument may be wrapped by differentiate_wrt");
+    constexpr int                              wrt = detail::wrt_index<T...>();
+    std::array<const double*, sizeof...(T)>    in{detail::data_of(args)...};
+    sx_grad*                                   g = nullptr;
+    detail::check(sx_form_eval(h_, t, in.data(), output_.Data(), wrt, &g));
+    detail::check(sx_form_synchronize(h_));
+    if constexpr (wrt < 0) {
+      return static_cast<Vector&>(output_);
+    } else {
+      grad_.h_         = g;
+      grad_.form_      = h_;
+      grad_.test_size_ = test_size_;
+      return std::tuple<Vector&, Gradient&>(output_, grad_);
+    }
+  }
+
+  sx_form* handle() const { return h_; }
+
+ private:
+  sx_form* h_         = nullptr;
+  int64_t  test_size_ = 0;
+  Vector   output_;
+  Gradient grad_;
+};
+
+// ---------------------------------------------------------------------------
+// EquationSolver: Newton + Jacobi-CG with the reference's option names
+// (num/solver_config.hpp:342-400, num/equation_solver.cpp:81-235)
+// ---------------------------------------------------------------------------
+enum class NonlinearSolver { Newton };
+enum class LinearSolver { CG };
+enum class Preconditioner { HypreJacobi };
+
+struct LinearSolverOptions {
+  LinearSolver   linear_solver  = LinearSolver::CG;
+  Preconditioner preconditioner = Preconditioner::HypreJacobi;
+  double         relative_tol   = 1.0e-8;
+  double         absolute_tol   = 1.0e-12;
+  int            max_iterations = 300;
+};
+struct NonlinearSolverOptions {
+  NonlinearSolver nonlin_solver  = NonlinearSolver::Newton;
+  double          relative_tol   = 1.0e-8;
+  double          absolute_tol   = 1.0e-12;
+  int             max_iterations = 20;
+};
+
+class EquationSolver {
+ public:
+  EquationSolver(NonlinearSolverOptions nl = {}, LinearSolverOptions lin = {}) : nl_(nl), lin_(lin) {}
+  /// Solve residual(x) = 0 with essential rows zeroed; x holds the essential
+  /// values on entry.  Returns the report; throws Error on non-convergence.
+  template <typename Form>
+  sx_solve_report solve(Form& residual, Vector& x, const std::vector<int32_t>& essential_dofs, int wrt = 0)
+  {
+    sx_solve_report rep{};
+    int rc = sx_newton_solve(residual.handle(), wrt, essential_dofs.data(), int64_t(essential_dofs.size()), x.Data(),
+                             nl_.relative_tol, nl_.absolute_tol, nl_.max_iterations, lin_.relative_tol,
+                             lin_.absolute_tol, lin_.max_iterations, &rep);
+    last_ = rep;
+    detail::check(rc);
+    return rep;
+  }
+  const sx_solve_report& report() const { return last_; }
+
+ private:
+  NonlinearSolverOptions nl_;
+  LinearSolverOptions    lin_;
+  sx_solve_report        last_{};
+};
+
+}  // namespace serac_amd

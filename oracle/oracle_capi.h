@@ -1,0 +1,25 @@
+/* oracle_capi.h -- extern "C" entry points of the CPU oracle (liboracle.so).
+ * TEST INFRASTRUCTURE ONLY: for C/C++ test programs under tests/. */
+#ifndef SERAC_ORACLE_CAPI_H
+#define SERAC_ORACLE_CAPI_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+const char* orc_last_error(void);
+void*       orc_mesh_cuboid(int nx, int ny, int nz, double lx, double ly, double lz);
+int         orc_mesh_perturb(void* m, uint64_t seed, double amp);
+void        orc_mesh_free(void* m);
+void*       orc_space(void* m, int p, int nc, int ordering);
+void        orc_space_info(void* s, int64_t* out);
+void        orc_space_node_coords(void* m, void* s, double* out);
+void        orc_space_free(void* s);
+void*       orc_form(void* m, void* s);
+void        orc_form_free(void* f);
+int         orc_form_add_integral(void* f, int kind, const double* params, const int* elems, int nelem);
+int         orc_form_eval(void* f, const double* U, double* R, int linearize);
+int         orc_form_jvp(void* f, const double* dU, double* dR);
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -66,3 +66,25 @@ def build(jobs=None, verbose=False):
 
 if __name__ == "__main__":
     print(build(verbose=True))
+
+
+def build_cpp_tests():
+    """Compile the C++ front-end test programs under tests/cpp into tests/cpp/bin
+    (host C++ over the C-ABI; linked against the in-tree libraries)."""
+    root = os.path.dirname(HERE)
+    src_dir = os.path.join(root, "tests", "cpp")
+    out_dir = os.path.join(src_dir, "bin")
+    os.makedirs(out_dir, exist_ok=True)
+    built = []
+    for src in sorted(os.listdir(src_dir)):
+        if not src.endswith(".cpp"):
+            continue
+        exe = os.path.join(out_dir, src[:-4])
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", os.path.join(src_dir, src), "-I", os.path.join(root, "include"),
+               "-L", HERE, "-lserac_amd", "-L", os.path.join(root, "oracle"), "-loracle",
+               "-Wl,-rpath,$ORIGIN/../../../serac_amd:$ORIGIN/../../../oracle", "-o", exe]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"compiling {src} failed:\n{r.stderr}")
+        built.append(exe)
+    return built

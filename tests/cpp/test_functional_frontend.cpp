@@ -1,0 +1,230 @@
+// Front-end tests written against include/serac_amd/functional.hpp the way
+// the reference's own functional tests are written (gtest-free: each check
+// prints PASS/FAIL and the program exits non-zero on any failure).
+//   - residual and gradient of MaterialStressFunctor<NeoHookean> against the
+//     CPU oracle (1e-12), as functional_basic_h1_vector.cpp does against mfem
+//   - check_gradient.hpp:16-90: forward / central FD convergence ratios 2 / 4
+//   - solid_statics_patch.cpp:30-100: the AffineSolution interior residual
+//   - heat integrands (ThermalMaterialIntegrand + ConstantSource)
+//   - EquationSolver Newton + Jacobi-CG on a 1% compression step
+//   - error paths: unsupported configurations raise serac_amd::Error
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/serac_amd/functional.hpp"
+#include "../../oracle/oracle_capi.h"
+
+using namespace serac_amd;
+
+static int failures = 0;
+#define CHECK(cond, ...)                        \
+  do {                                          \
+    if (cond) {                                 \
+      std::printf("PASS ");                     \
+    } else {                                    \
+      std::printf("FAIL ");                     \
+      failures++;                               \
+    }                                           \
+    std::printf(__VA_ARGS__);                   \
+    std::printf("\n");                          \
+  } while (0)
+
+static double maxabs(const std::vector<double>& a)
+{
+  double m = 0;
+  for (double v : a) m = std::fmax(m, std::fabs(v));
+  return m;
+}
+static double relerr(const std::vector<double>& a, const std::vector<double>& b)
+{
+  double m = 0;
+  for (size_t i = 0; i < a.size(); i++) m = std::fmax(m, std::fabs(a[i] - b[i]));
+  return m / std::fmax(maxabs(b), 1e-300);
+}
+static double norm2(const std::vector<double>& a)
+{
+  double s = 0;
+  for (double v : a) s += v * v;
+  return std::sqrt(s);
+}
+// SplitMix64 counter RNG of the workloads (seed, stream, index)
+static uint64_t mix(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+static double u01(uint64_t seed, uint64_t stream, uint64_t i)
+{
+  return double(mix(mix(mix(seed) ^ stream) ^ i) >> 11) * 0x1.0p-53;
+}
+
+int main()
+{
+  detail::check(sx_init(0));
+  const int    n = 3, p = 2;
+  const double K = 10.0, G = 0.25;
+
+  Mesh mesh = Mesh::Cuboid(n, n, n);
+  mesh.Perturb(42, 0.2);
+  FiniteElementSpace fes(mesh, p, 3);
+  const int64_t      ndof = fes.GetTrueVSize();
+
+  using space = H1<2, 3>;
+  Functional<space(space)> residual(&fes, {&fes});
+  solid_mechanics::NeoHookean mat{1.0, K, G};
+  residual.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                             solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                 mat, GeometricNonlinearities::On),
+                             mesh);
+
+  std::vector<double> U(static_cast<size_t>(ndof)), dU(static_cast<size_t>(ndof));
+  for (int64_t i = 0; i < ndof; i++) {
+    U[size_t(i)]  = 0.02 * (u01(7, 2000, uint64_t(i)) - 0.5);
+    dU[size_t(i)] = u01(7, 3000, uint64_t(i)) - 0.5;
+  }
+  Vector u(U), du(dU);
+
+  // ---- oracle
+  void* om = orc_mesh_cuboid(n, n, n, 1, 1, 1);
+  orc_mesh_perturb(om, 42, 0.2);
+  void*        os = orc_space(om, p, 3, 1);
+  void*        of = orc_form(om, os);
+  const double prm[8] = {1.0, K, G, 0, 0, 0, 0, 0};
+  orc_form_add_integral(of, 3, prm, nullptr, 0);
+  std::vector<double> R_ref(static_cast<size_t>(ndof)), J_ref(static_cast<size_t>(ndof));
+  orc_form_eval(of, U.data(), R_ref.data(), 1);
+  orc_form_jvp(of, dU.data(), J_ref.data());
+
+  // ---- residual and gradient through the front end
+  Vector& r = residual(0.0, u);
+  CHECK(relerr(r.ToHost(), R_ref) < 1e-12, "residual vs oracle: %.2e", relerr(r.ToHost(), R_ref));
+  auto [r2, dR] = residual(0.0, differentiate_wrt(u));
+  CHECK(relerr(r2.ToHost(), R_ref) < 1e-12, "linearized residual vs oracle");
+  std::vector<double> Jv = dR(du).ToHost();
+  CHECK(relerr(Jv, J_ref) < 1e-12, "Gradient::operator() vs oracle J.v: %.2e", relerr(Jv, J_ref));
+  Vector jv2(ndof);
+  dR.Mult(du, jv2);
+  CHECK(relerr(jv2.ToHost(), Jv) == 0.0, "Gradient::Mult == Gradient::operator()");
+  std::vector<double> diag = dR.AssembleDiagonal().ToHost();
+  double              dmin = 1e300;
+  for (double v : diag) dmin = std::fmin(dmin, v);
+  CHECK(dmin > 0.0, "AssembleDiagonal is positive (min %.3e)", dmin);
+
+  // ---- check_gradient: FD convergence ratios 2 (forward) and 4 (central)
+  std::vector<double> R0 = residual(0.0, u).ToHost();
+  double              ef[2], ec[2];
+  for (int k = 0; k < 2; k++) {
+    const double        eps = k == 0 ? 1e-3 : 5e-4;
+    std::vector<double> up(U), um(U);
+    for (size_t i = 0; i < U.size(); i++) up[i] += eps * dU[i], um[i] -= eps * dU[i];
+    std::vector<double> Rp = residual(0.0, Vector(up)).ToHost();
+    std::vector<double> Rm = residual(0.0, Vector(um)).ToHost();
+    std::vector<double> df(U.size()), dc(U.size());
+    for (size_t i = 0; i < U.size(); i++) {
+      df[i] = (Rp[i] - R0[i]) / eps - Jv[i];
+      dc[i] = (Rp[i] - Rm[i]) / (2 * eps) - Jv[i];
+    }
+    ef[k] = norm2(df), ec[k] = norm2(dc);
+  }
+  CHECK(ef[0] / ef[1] > 1.8 && ef[0] / ef[1] < 2.2, "forward-difference ratio %.3f", ef[0] / ef[1]);
+  CHECK(ec[0] / ec[1] > 3.6 && ec[0] / ec[1] < 4.4, "central-difference ratio %.3f", ec[0] / ec[1]);
+
+  // ---- affine patch test (AffineSolution literals, K = G = 1)
+  {
+    const double A[3][3] = {{0.110791568544027, 0.230421268325901, 0.15167673653354},
+                            {0.198344644470483, 0.060514559793513, 0.084137393813728},
+                            {0.011544253485023, 0.060942846497753, 0.186383473579596}};
+    const double b[3]    = {0.765645367640828, 0.992487355850465, 0.162199373722092};
+    Functional<space(space)> patch(&fes, {&fes});
+    patch.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                            solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                {1.0, 1.0, 1.0}, GeometricNonlinearities::On),
+                            mesh);
+    std::vector<double> x = fes.NodeCoordinates(), Ua(static_cast<size_t>(ndof));
+    for (int64_t nd = 0; nd < fes.GetNDofs(); nd++)
+      for (int i = 0; i < 3; i++)
+        Ua[size_t(nd * 3 + i)] =
+            A[i][0] * x[size_t(nd * 3)] + A[i][1] * x[size_t(nd * 3 + 1)] + A[i][2] * x[size_t(nd * 3 + 2)] + b[i];
+    std::vector<double> Ra = patch(0.0, Vector(Ua)).ToHost();
+    double              interior = 0;
+    for (int64_t nd = 0; nd < fes.GetNDofs(); nd++) {
+      bool bnd = false;
+      for (int d = 0; d < 3; d++) bnd = bnd || x[size_t(nd * 3 + d)] < 1e-12 || x[size_t(nd * 3 + d)] > 1 - 1e-12;
+      if (!bnd)
+        for (int i = 0; i < 3; i++) interior = std::fmax(interior, std::fabs(Ra[size_t(nd * 3 + i)]));
+    }
+    CHECK(interior < 1e-13, "affine patch interior residual %.2e", interior);
+  }
+
+  // ---- heat: conduction + constant source
+  {
+    FiniteElementSpace       tfes(mesh, 2, 1);
+    Functional<H1<2>(H1<2>)> heat(&tfes, {&tfes});
+    heat.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                           heat_transfer::ThermalMaterialIntegrand<heat_transfer::LinearIsotropicConductor>(
+                               heat_transfer::LinearIsotropicConductor(1.0, 1.0, 2.0)),
+                           mesh);
+    heat.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{}, heat_transfer::ConstantSource{3.0}, mesh);
+    std::vector<double> R = heat(0.0, Vector(std::vector<double>(size_t(tfes.GetTrueVSize()), 0.0))).ToHost();
+    double              s = 0;
+    for (double v : R) s += v;
+    CHECK(std::fabs(s + 3.0) < 1e-12, "heat source integrates to -3 * volume (%.15f)", s);
+  }
+
+  // ---- EquationSolver: one 1% compression step
+  {
+    std::vector<int32_t> ess0 = fes.BoundaryDofs(0b000001, 0x7), ess1 = fes.BoundaryDofs(0b000010, 0x1);
+    std::vector<int32_t> ess(ess0);
+    ess.insert(ess.end(), ess1.begin(), ess1.end());
+    std::sort(ess.begin(), ess.end());
+    std::vector<double> x0(size_t(ndof), 0.0);
+    for (int32_t d : ess1) x0[size_t(d)] = -0.01;
+    Vector         x(x0);
+    EquationSolver solver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 20},
+                          LinearSolverOptions{LinearSolver::CG, Preconditioner::HypreJacobi, 1e-10, 0.0, 5000});
+    sx_solve_report rep = solver.solve(residual, x, ess);
+    CHECK(rep.converged && rep.newton_iterations <= 6, "Newton converged in %d iterations (%d CG)",
+          rep.newton_iterations, rep.linear_iterations);
+    std::vector<double> Rx = residual(0.0, x).ToHost();
+    for (int32_t d : ess) Rx[size_t(d)] = 0.0;
+    CHECK(norm2(Rx) <= 1e-9 * rep.initial_norm, "converged residual %.2e (initial %.2e)", norm2(Rx),
+          rep.initial_norm);
+  }
+
+  // ---- error paths
+  {
+    bool thrown = false;
+    try {
+      Functional<space(space)> bad(&fes, {&fes});
+      bad.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                            solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                mat, GeometricNonlinearities::Off),
+                            mesh);
+    } catch (const Error& e) {
+      thrown = e.code == SX_ERR_UNSUPPORTED;
+    }
+    CHECK(thrown, "unsupported integrand configuration raises Error(SX_ERR_UNSUPPORTED)");
+    thrown = false;
+    try {
+      FiniteElementSpace tfes(mesh, 2, 1);
+      Functional<H1<2>(H1<2>)> wrong(&tfes, {&tfes});
+      wrong.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                              solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                  mat, GeometricNonlinearities::On),
+                              mesh);
+    } catch (const Error& e) {
+      thrown = e.code == SX_ERR_ARG;
+    }
+    CHECK(thrown, "vector qfunction on a scalar space raises Error(SX_ERR_ARG)");
+  }
+
+  orc_form_free(of);
+  orc_space_free(os);
+  orc_mesh_free(om);
+  std::printf("%s: %d failure(s)\n", failures ? "FAILED" : "OK", failures);
+  return failures ? 1 : 0;
+}
