@@ -92,6 +92,13 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--cpu-sample-n", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="skip the FP32 / BF16 J.v measurements")
+    ap.add_argument("--solve", action="store_true", help="also time the load-stepped Newton + Jacobi-CG solve")
+    ap.add_argument("--load-steps", type=int, default=10)
+    ap.add_argument("--strain", type=float, default=0.05)
+    ap.add_argument("--newton-tol", type=float, default=1e-8)
+    ap.add_argument("--cg-tol", type=float, default=1e-6)
+    ap.add_argument("--cg-max", type=int, default=20000)
     ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--calibrate", action="store_true",
@@ -158,7 +165,6 @@ def main():
     if not np.isfinite(jv).all() or not np.isfinite(form(0.0, U).numpy()).all():
         raise SystemExit("bench: non-finite residual or J.v; the measurement is invalid")
     jv_norm = float(np.linalg.norm(jv))
-    del jv
     # dominant kernel: the colour launches of the J.v element kernel
     d_sym = 45
     b_alg = W.jvp_algorithmic_bytes(space.ndof, nqp, space.ne, space.npe, d_sym)
@@ -166,6 +172,63 @@ def main():
     achieved = b_alg / (kern_ms_per_apply / 1e3) / 1e9
     log(f"J.v {ms_step:.3f} ms/step ({value:.2f} GDOF/s), element kernels {kern_ms_per_apply:.3f} ms "
         f"over {launches} launches, {achieved:.0f} GB/s algorithmic, wall {wall:.2f} s")
+
+    # precision-split operator: FP32 vectors and arithmetic, tangent in FP32 or BF16
+    split = {}
+    if not args.no_split:
+        dU32 = sx.DeviceVector.from_numpy(W.direction(space.nnodes, 3), dtype=np.float32)
+        dR32 = sx.DeviceVector(space.ndof, dtype=np.float32)
+        for name, tprec, s_d in (("fp32", sx.capi.SX_PREC_F32, 4), ("fp32_bf16", sx.capi.SX_PREC_BF16, 2)):
+            form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
+            _, g32 = form(0.0, U, wrt=0)
+            if args.warmup > 0:
+                g32.apply_timed(dU32, dR32, args.warmup)
+            t32, k32, _ = g32.apply_timed(dU32, dR32, args.steps)
+            jv32 = dR32.numpy().astype(np.float64)
+            if not np.isfinite(jv32).all():
+                raise SystemExit(f"bench: non-finite {name} J.v; the measurement is invalid")
+            b32 = W.jvp_algorithmic_bytes(space.ndof, nqp, space.ne, space.npe, d_sym, s_v=4, s_d=s_d)
+            ach = b32 / (k32 / args.steps / 1e3) / 1e9
+            split[name] = {
+                "value": round(W.gdof_per_s(space.ndof, t32 / args.steps / 1e3), 4),
+                "ms_per_step": round(t32 / args.steps, 5),
+                "roofline_achieved_GBs": round(ach, 1),
+                "roofline_frac": round(ach / HBM_PEAK_GBS, 4),
+                "algorithmic_bytes_per_apply": b32,
+                "rel_err_vs_fp64": float(np.linalg.norm(jv32 - jv) / np.linalg.norm(jv)),
+            }
+            log(f"{name} J.v {t32 / args.steps:.3f} ms/step ({split[name]['value']:.2f} GDOF/s), "
+                f"{ach:.0f} GB/s algorithmic, rel. error vs FP64 {split[name]['rel_err_vs_fp64']:.2e}")
+        form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, sx.capi.SX_PREC_F64)
+        del dU32, dR32
+    del jv
+
+    solve = None
+    if args.solve:
+        # uniaxial compression: x = 0 clamped, x = 1 displaced by -strain in x over `load_steps`
+        ess0 = space.boundary_dofs(0b000001, 0x7)
+        ess1 = space.boundary_dofs(0b000010, 0x1)
+        ess = np.union1d(ess0, ess1).astype(np.int32)
+        final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
+        x = sx.DeviceVector(space.ndof)
+        sx.capi.call("sx_device_synchronize")
+        ts = time.perf_counter()
+        reps = sx.quasistatic_solve(form, x, ess, final, args.load_steps, warm_start=True, rel_tol=args.newton_tol,
+                                    lin_rel_tol=args.cg_tol, lin_max_iter=args.cg_max)
+        sx.capi.call("sx_device_synchronize")
+        t_solve = time.perf_counter() - ts
+        solve = {
+            "seconds": round(t_solve, 3),
+            "load_steps": args.load_steps,
+            "strain": args.strain,
+            "newton_rel_tol": args.newton_tol,
+            "cg_rel_tol": args.cg_tol,
+            "precond": "Jacobi",
+            "newton_iterations": [r["newton_iterations"] for r in reps],
+            "cg_iterations": [r["linear_iterations"] for r in reps],
+            "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0 for r in reps],
+        }
+        log(f"solve {t_solve:.2f} s: Newton {solve['newton_iterations']}, CG {solve['cg_iterations']}")
 
     if args.calibrate:
         # reference kernels for HBM counter scaling: k_dot_partial reads 2 * 8 * ndof
@@ -225,6 +288,8 @@ def main():
         "linearize_ms": round(t_lin * 1e3, 3),
         "tangent_bytes": tangent_bytes,
         "jv_norm": jv_norm,
+        "precision_split": split,
+        "newton_solve": solve,
         "device": dev,
     }
     print(json.dumps(rec), flush=True)

@@ -123,8 +123,12 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
  *     Must be set before the first AddDomainIntegral.
  *   SX_OPT_XCD_REMAP: 1 (default) gives each of the 8 XCDs a contiguous run of
  *     element blocks (L2 locality for shared nodes); 0 keeps hardware order. */
-enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2 };
+enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2, SX_OPT_TANGENT_PRECISION = 3 };
 enum { SX_SCATTER_COLORED = 0, SX_SCATTER_ATOMIC = 1 };
+/* Precisions.  SX_OPT_TANGENT_PRECISION selects how the next linearization
+ * stores the quadrature-point tangent (computed in FP64, rounded once to
+ * FP32 or BF16); J.v then streams that format. */
+enum { SX_PREC_F64 = 0, SX_PREC_F32 = 1, SX_PREC_BF16 = 2 };
 int sx_form_set_option(sx_form* f, int option, int64_t value);
 int sx_form_synchronize(sx_form* f);
 int sx_form_stream(sx_form* f, void** hip_stream); /* the form's hipStream_t */
@@ -133,6 +137,9 @@ int sx_form_destroy(sx_form* f);
 /* ---- gradient operator: Functional::Gradient (fn/functional.hpp:548-669) ---- */
 /* Gradient::Mult / operator() (fn/functional.hpp:556-566 -> ActionOfGradient :406-432) */
 int sx_grad_apply(sx_grad* g, const double* dx, double* df);
+/* The same action with FP32 vectors and FP32 arithmetic (the precision-split
+ * operator of the mixed-precision solvers; no reference counterpart). */
+int sx_grad_apply_f32(sx_grad* g, const float* dx, float* df);
 /* diagonal of the assembled gradient (the values Gradient::assemble,
  * fn/functional.hpp:569-667, would put on the diagonal), for Jacobi */
 int sx_grad_diagonal(sx_grad* g, double* diag);
@@ -140,9 +147,10 @@ int sx_grad_diagonal(sx_grad* g, double* diag);
 int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes);
 /* Runs nrep applications back to back on the form's stream, bracketed by HIP
  * events; writes the elapsed milliseconds of the whole batch and of the
- * element kernels alone (sum over launches). Synchronous. */
-int sx_grad_apply_timed(sx_grad* g, const double* dx, double* df, int nrep, double* ms_total, double* ms_kernels,
-                        int* launches_per_apply);
+ * element kernels alone (sum over launches). vec_precision: SX_PREC_F64
+ * (double vectors) or SX_PREC_F32 (float vectors). Synchronous. */
+int sx_grad_apply_timed(sx_grad* g, int vec_precision, const void* dx, void* df, int nrep, double* ms_total,
+                        double* ms_kernels, int* launches_per_apply);
 
 /* ---- solvers (num/equation_solver.cpp:22-240, 893-950, 1018-1021) ---- */
 typedef struct sx_solve_report {
@@ -164,6 +172,25 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
 int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, double* x, double rel_tol,
                     double abs_tol, int max_iter, double lin_rel_tol, double lin_abs_tol, int lin_max_iter,
                     sx_solve_report* report);
+
+/* Quasi-static load stepping (phy/solid_mechanics.hpp:1620-1630 quasiStaticSolve,
+ * :1727-1770 warmStartDisplacement): the essential values ramp linearly from
+ * their values in x (t = 0) to ess_final (host array, t = 1) over nsteps; each
+ * step optionally warm-starts with one linear solve on the previous
+ * linearization, du = du_bc + J_bc^{-1} (-R(u) - J du_bc), then runs Newton +
+ * Jacobi-CG.  x (device) is updated in place; reports[nsteps] (host) receive
+ * per-step Newton / CG iteration counts and residual norms. */
+typedef struct sx_solver_options {
+  double rel_tol;     /* Newton, relative to the step's initial residual */
+  double abs_tol;     /* Newton, absolute */
+  int    max_iter;    /* Newton */
+  double lin_rel_tol; /* Jacobi-CG */
+  double lin_abs_tol;
+  int    lin_max_iter;
+} sx_solver_options;
+int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, const double* ess_final,
+                         int nsteps, int warm_start, const sx_solver_options* opts, double* x,
+                         sx_solve_report* reports);
 
 /* ---- vector kernels used by the solvers, exposed for callers ---- */
 int sx_vec_dot(const double* a, const double* b, int64_t n, double* result); /* synchronous, deterministic */

@@ -38,34 +38,36 @@ def device_name():
 
 
 class DeviceVector:
-    """FP64 vector in device memory allocated by the library."""
+    """FP64 (or FP32) vector in device memory allocated by the library."""
 
-    def __init__(self, n, data=None):
+    def __init__(self, n, data=None, dtype=np.float64):
         self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = max(self.n, 1) * self.dtype.itemsize
         self.ptr = _vp()
-        capi.call("sx_malloc", ctypes.byref(self.ptr), max(self.n, 1) * 8)
+        capi.call("sx_malloc", ctypes.byref(self.ptr), self.nbytes + (self.nbytes % 8))
         if data is None:
-            capi.call("sx_memset_zero", self.ptr, max(self.n, 1) * 8)
+            self.zero()
         else:
             self.upload(data)
 
     @classmethod
-    def from_numpy(cls, a):
-        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
-        return cls(a.size, a)
+    def from_numpy(cls, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype).ravel()
+        return cls(a.size, a, dtype)
 
     def upload(self, a):
-        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        a = np.ascontiguousarray(a, dtype=self.dtype).ravel()
         assert a.size == self.n, (a.size, self.n)
-        capi.call("sx_memcpy_htod", self.ptr, a.ctypes.data_as(_vp), self.n * 8)
+        capi.call("sx_memcpy_htod", self.ptr, a.ctypes.data_as(_vp), self.n * self.dtype.itemsize)
 
     def numpy(self):
-        out = np.empty(self.n)
-        capi.call("sx_memcpy_dtoh", out.ctypes.data_as(_vp), self.ptr, self.n * 8)
+        out = np.empty(self.n, dtype=self.dtype)
+        capi.call("sx_memcpy_dtoh", out.ctypes.data_as(_vp), self.ptr, self.n * self.dtype.itemsize)
         return out
 
     def zero(self):
-        capi.call("sx_memset_zero", self.ptr, max(self.n, 1) * 8)
+        capi.call("sx_memset_zero", self.ptr, self.nbytes + (self.nbytes % 8))
 
     def free(self):
         if self.ptr and self.ptr.value:
@@ -168,11 +170,14 @@ class Gradient:
         self.h = handle
 
     def mult(self, dx, df):
-        capi.call("sx_grad_apply", self.h, dx.ptr, df.ptr)
+        if dx.dtype == np.float32:
+            capi.call("sx_grad_apply_f32", self.h, dx.ptr, df.ptr)
+        else:
+            capi.call("sx_grad_apply", self.h, dx.ptr, df.ptr)
         self.form.synchronize()
 
     def __call__(self, dx):
-        df = DeviceVector(self.form.test.ndof)
+        df = DeviceVector(self.form.test.ndof, dtype=dx.dtype)
         self.mult(dx, df)
         return df
 
@@ -189,7 +194,8 @@ class Gradient:
 
     def apply_timed(self, dx, df, nrep):
         tot, ker, per = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
-        capi.call("sx_grad_apply_timed", self.h, dx.ptr, df.ptr, nrep, ctypes.byref(tot), ctypes.byref(ker),
+        prec = capi.SX_PREC_F32 if dx.dtype == np.float32 else capi.SX_PREC_F64
+        capi.call("sx_grad_apply_timed", self.h, prec, dx.ptr, df.ptr, nrep, ctypes.byref(tot), ctypes.byref(ker),
                   ctypes.byref(per))
         return tot.value, ker.value, per.value
 
@@ -257,3 +263,17 @@ class Functional:
         if getattr(self, "h", None) and capi._lib is not None:
             capi._lib.sx_form_destroy(self.h)
             self.h = None
+
+
+def quasistatic_solve(form, x, ess, ess_final, nsteps, warm_start=True, rel_tol=1e-8, abs_tol=0.0, max_iter=20,
+                      lin_rel_tol=1e-6, lin_abs_tol=0.0, lin_max_iter=20000, wrt=0, raise_on_noconv=True):
+    """Load-stepped Newton + Jacobi-CG (quasiStaticSolve with warm start); x is a DeviceVector."""
+    ess = np.ascontiguousarray(ess, dtype=np.int32)
+    fin = np.ascontiguousarray(ess_final, dtype=np.float64)
+    opts = capi.SolverOptions(rel_tol, abs_tol, max_iter, lin_rel_tol, lin_abs_tol, lin_max_iter)
+    reps = (SolveReport * nsteps)()
+    rc = capi.call_rc("sx_quasistatic_solve", form.h, wrt, _iptr(ess), len(ess), _dptr(fin), nsteps,
+                      int(warm_start), ctypes.byref(opts), x.ptr, reps)
+    if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+        raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+    return [r.as_dict() for r in reps]

@@ -21,6 +21,10 @@ SX_ORDERING_BYVDIM = 1
 
 SX_OPT_SCATTER = 1
 SX_OPT_XCD_REMAP = 2
+SX_OPT_TANGENT_PRECISION = 3
+SX_PREC_F64 = 0
+SX_PREC_F32 = 1
+SX_PREC_BF16 = 2
 SX_SCATTER_COLORED = 0
 SX_SCATTER_ATOMIC = 1
 
@@ -47,6 +51,17 @@ class SolveReport(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class SolverOptions(ctypes.Structure):
+    _fields_ = [
+        ("rel_tol", ctypes.c_double),
+        ("abs_tol", ctypes.c_double),
+        ("max_iter", ctypes.c_int),
+        ("lin_rel_tol", ctypes.c_double),
+        ("lin_abs_tol", ctypes.c_double),
+        ("lin_max_iter", ctypes.c_int),
+    ]
 
 
 SIGNATURES = {
@@ -86,11 +101,14 @@ SIGNATURES = {
     "sx_grad_apply": (_int, [_vp, _vp, _vp]),
     "sx_grad_diagonal": (_int, [_vp, _vp]),
     "sx_grad_tangent_bytes": (_int, [_vp, _I64]),
-    "sx_grad_apply_timed": (_int, [_vp, _vp, _vp, _int, _D, _D, ctypes.POINTER(_int)]),
+    "sx_grad_apply_f32": (_int, [_vp, _vp, _vp]),
+    "sx_grad_apply_timed": (_int, [_vp, _int, _vp, _vp, _int, _D, _D, ctypes.POINTER(_int)]),
     "sx_cg_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                            ctypes.POINTER(SolveReport)]),
     "sx_newton_solve": (_int, [_vp, _int, _I32, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double, _int,
                                ctypes.c_double, ctypes.c_double, _int, ctypes.POINTER(SolveReport)]),
+    "sx_quasistatic_solve": (_int, [_vp, _int, _I32, ctypes.c_int64, _D, _int, _int, ctypes.POINTER(SolverOptions),
+                                    _vp, ctypes.POINTER(SolveReport)]),
     "sx_vec_dot": (_int, [_vp, _vp, ctypes.c_int64, _D]),
     "sx_vec_axpy": (_int, [ctypes.c_double, _vp, _vp, ctypes.c_int64]),
 }

@@ -157,8 +157,9 @@ struct Integral {
   std::vector<int64_t> color_off;  // [ncolors + 1] slot ranges
   DeviceBuffer         d_map;      // [slot][npe]
   DeviceBuffer         d_coords;   // [slot][24]
-  DeviceBuffer         d_tangent;  // [slot][DS][Q^3]
+  DeviceBuffer         d_tangent;  // per-slot TangentLayout (sx_element.hpp)
   int                  layout  = sxk::LAYOUT_GRAD_SYM;
+  int                  tprec   = sxk::PREC_F64;  // precision the tangent was stored in
   bool                 has_tangent = false;
 };
 
@@ -176,6 +177,7 @@ struct sx_form {
   hipStream_t                            stream = nullptr;
   int                                    scatter   = SX_SCATTER_COLORED;
   int                                    xcd_remap = 1;
+  int                                    tprec     = sxk::PREC_F64;
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -213,7 +215,9 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   P.map      = I.d_map.as<const int>();
   P.coords   = I.d_coords.as<const double>();
   P.nnodes   = s->nnodes;
-  P.D        = I.d_tangent.as<double>();
+  P.D        = I.d_tangent.ptr;
+  P.tprec    = I.tprec;
+  P.vprec    = sxk::PREC_F64;
   P.tables   = element_tables(s->order + 1);
   for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = I.params[k];
   P.stream = f->stream;
@@ -221,11 +225,13 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
 }
 
 // one launch per colour; returns the number of launches
-int run_integral(const sx_form* f, const Integral& I, int mode, const double* u, double* r, hipEvent_t* ev = nullptr)
+int run_integral(const sx_form* f, const Integral& I, int mode, const void* u, void* r, hipEvent_t* ev = nullptr,
+                 int vprec = sxk::PREC_F64)
 {
   sxk::ElemLaunch P = base_launch(f, I, mode);
   P.u               = u;
   P.r               = r;
+  P.vprec           = vprec;
   int launches      = 0;
   for (size_t c = 0; c + 1 < I.color_off.size(); c++) {
     P.slot_begin = I.color_off[c];
@@ -248,17 +254,22 @@ bool depends(const Integral& I, int trial)
   return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
 }
 
-void apply_gradient(sx_grad* g, const double* dx, double* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr)
+void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr,
+                    int vprec = sxk::PREC_F64)
 {
   SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
   SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
   SX_REQUIRE(dx && df, SX_ERR_ARG, "null vector");
+  SX_REQUIRE(vprec == sxk::PREC_F64 || vprec == sxk::PREC_F32, SX_ERR_ARG, "J.v vectors are FP64 or FP32");
   sx_form* f = g->form;
-  SX_HIP(sxk::vec_fill(df, 0.0, f->test->ndof(), f->stream));
+  if (vprec == sxk::PREC_F64)
+    SX_HIP(sxk::vec_fill(static_cast<double*>(df), 0.0, f->test->ndof(), f->stream));
+  else
+    SX_HIP(sxk::vec_fill_f32(static_cast<float*>(df), 0.0f, f->test->ndof(), f->stream));
   int n = 0;
   for (auto& I : f->integrals)
     if (depends(*I, g->wrt) && I->has_tangent)
-      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr);
+      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, vprec);
   if (nlaunch) *nlaunch = n;
 }
 
@@ -631,9 +642,9 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
       const bool lin = (wrt >= 0) && depends(*I, wrt);
       if (lin) {
         const size_t bytes =
-            size_t(I->nslots) * size_t(sxk::tangent_slot_stride(sxk::tangent_ds(f->test->nc, I->layout), nq)) *
-            sizeof(double);
+            size_t(I->nslots) * size_t(sxk::tangent_slot_bytes(sxk::tangent_ds(f->test->nc, I->layout), nq, f->tprec));
         I->d_tangent.allocate(bytes);
+        I->tprec = f->tprec;
         I->has_tangent = true;
       }
       run_integral(f, *I, lin ? sxk::MODE_LINEARIZE : sxk::MODE_RESIDUAL, inputs[I->depends_on[0]], out);
@@ -658,6 +669,11 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
         return;
       case SX_OPT_XCD_REMAP:
         f->xcd_remap = value ? 1 : 0;
+        return;
+      case SX_OPT_TANGENT_PRECISION:
+        SX_REQUIRE(value == SX_PREC_F64 || value == SX_PREC_F32 || value == SX_PREC_BF16, SX_ERR_ARG,
+                   "bad tangent precision");
+        f->tprec = int(value);
         return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
@@ -718,20 +734,25 @@ int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes)
   });
 }
 
-int sx_grad_apply_timed(sx_grad* g, const double* dx, double* df, int nrep, double* ms_total, double* ms_kernels,
-                        int* launches_per_apply)
+int sx_grad_apply_f32(sx_grad* g, const float* dx, float* df)
+{
+  return api([&] { apply_gradient(g, dx, df, nullptr, nullptr, sxk::PREC_F32); });
+}
+
+int sx_grad_apply_timed(sx_grad* g, int vec_precision, const void* dx, void* df, int nrep, double* ms_total,
+                        double* ms_kernels, int* launches_per_apply)
 {
   return api([&] {
     SX_REQUIRE(g && g->form && nrep >= 1, SX_ERR_ARG, "bad argument");
     sx_form* f   = g->form;
     int      per = 0;
-    apply_gradient(g, dx, df, nullptr, &per);  // count launches
+    apply_gradient(g, dx, df, nullptr, &per, vec_precision);  // count launches
     const int               nev = 2 * per * nrep;
     std::vector<hipEvent_t> ev(size_t(nev) + 2);
     for (auto& e : ev) SX_HIP(hipEventCreate(&e));
     SX_HIP(hipStreamSynchronize(f->stream));
     SX_HIP(hipEventRecord(ev[size_t(nev)], f->stream));
-    for (int r = 0; r < nrep; r++) apply_gradient(g, dx, df, ev.data() + size_t(2 * per * r), nullptr);
+    for (int r = 0; r < nrep; r++) apply_gradient(g, dx, df, ev.data() + size_t(2 * per * r), nullptr, vec_precision);
     SX_HIP(hipEventRecord(ev[size_t(nev) + 1], f->stream));
     SX_HIP(hipEventSynchronize(ev[size_t(nev) + 1]));
     float total = 0.0f;
@@ -887,6 +908,43 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
   return rc;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Newton (num/equation_solver.cpp:81-235, no line search): norm goal
+// max(rel ||r0||, abs); linearize, Jacobi-CG on J_bc c = r, x <- x - c.
+void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* x, const sx_solver_options& o,
+                 Workspace& w, double* r, double* c, sx_solve_report* rep)
+{
+  const int64_t n    = f->test->ndof();
+  double        norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+  rep->initial_norm  = norm;
+  const double goal  = std::max(o.rel_tol * norm, o.abs_tol);
+  int          it    = 0;
+  for (;; it++) {
+    SX_REQUIRE(std::isfinite(norm), SX_ERR_NOCONV, "Newton residual is not finite");
+    if (norm <= goal) {
+      rep->converged = 1;
+      break;
+    }
+    if (it >= o.max_iter) break;
+    sx_grad* g = nullptr;
+    residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at x
+    sx_solve_report lr{};
+    cg(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
+    rep->linear_iterations += lr.linear_iterations;
+    SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
+    norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+  }
+  rep->newton_iterations = it;
+  rep->final_norm        = norm;
+}
+
+}  // namespace
+
+extern "C" {
+
 int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, double* x, double rel_tol,
                     double abs_tol, int max_iter, double lin_rel_tol, double lin_abs_tol, int lin_max_iter,
                     sx_solve_report* report)
@@ -902,35 +960,80 @@ int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, 
     DeviceBuffer rbuf, cbuf;
     rbuf.allocate(size_t(n) * 8);
     cbuf.allocate(size_t(n) * 8);
-    double*        r    = rbuf.as<double>();
-    double*        c    = cbuf.as<double>();
-    const int32_t* ess  = w.ess.as<const int32_t>();
-    double         norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
-    rep.initial_norm    = norm;
-    const double goal   = std::max(rel_tol * norm, abs_tol);
-    int          it     = 0;
-    for (;; it++) {
-      SX_REQUIRE(std::isfinite(norm), SX_ERR_NOCONV, "Newton residual is not finite");
-      if (norm <= goal) {
-        rep.converged = 1;
-        break;
-      }
-      if (it >= max_iter) break;
-      sx_grad* g = nullptr;
-      residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at x
-      sx_solve_report lr{};
-      cg(g, ess, ness, r, c, lin_rel_tol, lin_abs_tol, lin_max_iter, w, &lr);
-      rep.linear_iterations += lr.linear_iterations;
-      SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
-      norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
-    }
-    rep.newton_iterations = it;
-    rep.final_norm        = norm;
+    const sx_solver_options o{rel_tol, abs_tol, max_iter, lin_rel_tol, lin_abs_tol, lin_max_iter};
+    newton_core(f, wrt, w.ess.as<const int32_t>(), ness, x, o, w, rbuf.as<double>(), cbuf.as<double>(), &rep);
     SX_HIP(hipStreamSynchronize(f->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "Newton did not converge");
   });
   if (report) *report = rep;
   return rc;
+}
+
+int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, const double* ess_final,
+                         int nsteps, int warm_start, const sx_solver_options* opts, double* x,
+                         sx_solve_report* reports)
+{
+  return api([&] {
+    SX_REQUIRE(f && x && opts && nsteps >= 1, SX_ERR_ARG, "bad argument");
+    SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
+    SX_REQUIRE(ness == 0 || (ess_dofs && ess_final), SX_ERR_ARG, "null essential DOF data");
+    const int64_t n = f->test->ndof();
+    hipStream_t   s = f->stream;
+    Workspace     w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    const int32_t* ess = w.ess.as<const int32_t>();
+    DeviceBuffer   rbuf, cbuf, dbc, jdbc, vals;
+    rbuf.allocate(size_t(n) * 8);
+    cbuf.allocate(size_t(n) * 8);
+    dbc.allocate(size_t(n) * 8);
+    jdbc.allocate(size_t(n) * 8);
+    vals.allocate(size_t(std::max<int64_t>(ness, 1)) * 8);
+    double *r = rbuf.as<double>(), *c = cbuf.as<double>();
+
+    // essential values at t = 0 are those x holds
+    std::vector<double> start(static_cast<size_t>(ness)), prev(static_cast<size_t>(ness)), target(static_cast<size_t>(ness)),
+        inc(static_cast<size_t>(ness));
+    {
+      std::vector<double> xh(static_cast<size_t>(n));
+      SX_HIP(hipMemcpy(xh.data(), x, size_t(n) * 8, hipMemcpyDeviceToHost));
+      for (int64_t i = 0; i < ness; i++) start[size_t(i)] = xh[size_t(ess_dofs[i])];
+    }
+    prev = start;
+    for (int k = 1; k <= nsteps; k++) {
+      const double t = double(k) / double(nsteps);
+      for (int64_t i = 0; i < ness; i++) {
+        target[size_t(i)] = start[size_t(i)] + t * (ess_final[i] - start[size_t(i)]);
+        inc[size_t(i)]    = target[size_t(i)] - prev[size_t(i)];
+      }
+      sx_solve_report rep{};
+      if (warm_start) {
+        // du_bc: the boundary increment; rhs = -(R(u) + J du_bc), free rows only
+        SX_HIP(hipMemcpyAsync(vals.ptr, inc.data(), size_t(ness) * 8, hipMemcpyHostToDevice, s));
+        SX_HIP(sxk::vec_fill(dbc.as<double>(), 0.0, n, s));
+        SX_HIP(sxk::set_rows_values(dbc.as<double>(), ess, vals.as<const double>(), ness, s));
+        sx_grad* g = nullptr;
+        residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at the converged state
+        apply_gradient(g, dbc.as<double>(), jdbc.as<double>());
+        SX_HIP(sxk::vec_axpby(-1.0, jdbc.as<double>(), -1.0, r, n, s));  // r = -(R + J du_bc)
+        SX_HIP(sxk::set_rows(r, ess, ness, 0.0, s));
+        sx_solve_report lr{};
+        cg(g, ess, ness, r, c, opts->lin_rel_tol, opts->lin_abs_tol, opts->lin_max_iter, w, &lr);
+        rep.linear_iterations += lr.linear_iterations;
+        SX_HIP(sxk::vec_axpy(1.0, c, x, n, s));
+        SX_HIP(sxk::vec_axpy(1.0, dbc.as<double>(), x, n, s));
+      } else {
+        SX_HIP(hipMemcpyAsync(vals.ptr, target.data(), size_t(ness) * 8, hipMemcpyHostToDevice, s));
+        SX_HIP(sxk::set_rows_values(x, ess, vals.as<const double>(), ness, s));
+      }
+      const int lin_ws = rep.linear_iterations;
+      newton_core(f, wrt, ess, ness, x, *opts, w, r, c, &rep);
+      rep.linear_iterations += lin_ws;
+      SX_HIP(hipStreamSynchronize(s));
+      if (reports) reports[k - 1] = rep;
+      if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "Newton did not converge in load step " + std::to_string(k));
+      prev = target;
+    }
+  });
 }
 
 }  // extern "C"

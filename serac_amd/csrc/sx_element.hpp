@@ -1,10 +1,6 @@
 // sx_element.hpp -- fused element kernels for H1 hexahedra on gfx950.
 //
-// One kernel launch processes one colour of elements (no two elements of a
-// colour share a node), so the scatter-add into the L-vector is a plain
-// read-modify-write: deterministic, no floating-point atomics.
-//
-// Per element the pipeline of fn/domain_integral_kernels.hpp:155-304 runs in
+// Per element, the pipeline of fn/domain_integral_kernels.hpp:155-304 runs in
 // one workgroup-resident pass:
 //   gather L -> LDS                           (fn/element_restriction.cpp:443-454)
 //   sum-factorized interpolate (3 stages)     (fn/detail/hexahedron_H1.inl:169-228)
@@ -18,7 +14,15 @@
 // (i2, i1, i0) = li in every stage (node (z,y,x), partial sums, or quadrature
 // point (qz,qy,qx)), which is possible because every form here uses
 // Q = p + 1 = nodes per direction (fn/functional.hpp:193).  LDS per element is
-// (6 + has_source) * nc * Q^3 doubles, reused across stages.
+// (6 + has_source) * nc * Q^3 values, reused across stages.
+//
+// The scatter-add is either a plain read-modify-write (one launch per element
+// colour: no two elements of a colour share a node) or an FP64/FP32 atomic add
+// (one launch for all elements).
+//
+// Precision: residual and linearization compute in FP64.  The stored tangent
+// is kept in S = double, float or bf16; the J.v kernel computes and exchanges
+// vectors in T = double or float.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,12 +33,56 @@
 
 namespace sxk {
 
-template <int M>
-__host__ __device__ constexpr int packed_index(int r, int s)
+// ---------------------------------------------------------------------------
+// storage types of the tangent
+// ---------------------------------------------------------------------------
+struct bf16 {
+  uint16_t bits;
+};
+
+__device__ __forceinline__ double to_real(double v, double) { return v; }
+__device__ __forceinline__ float  to_real(float v, float) { return v; }
+__device__ __forceinline__ double to_real(float v, double) { return double(v); }
+__device__ __forceinline__ float  to_real(double v, float) { return float(v); }
+__device__ __forceinline__ float  bf16_to_float(uint32_t bits16) { return __uint_as_float(bits16 << 16); }
+
+// FP64 -> storage, round to nearest even
+__device__ __forceinline__ double   to_store(double v, double*) { return v; }
+__device__ __forceinline__ float    to_store(double v, float*) { return float(v); }
+__device__ __forceinline__ uint16_t to_store(double v, bf16*)
 {
-  // upper triangle (r <= s), row-major
-  return r * M - (r * (r - 1)) / 2 + (s - r);
+  const uint32_t u = __float_as_uint(float(v));
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));  // inf / NaN
+  return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
 }
+
+template <typename S>
+struct StoreTraits {
+  using raw = S;
+};
+template <>
+struct StoreTraits<bf16> {
+  using raw = uint16_t;
+};
+
+// Per-slot tangent layout, in units of S: the DS packed entries of quadrature
+// point q are grouped VEC = 16 / sizeof(S) at a time, [g][q][VEC], so that lane
+// q reads 16 bytes per group; the last, partial group is [q][REM].
+template <typename S, int DS, int NQ>
+struct TangentLayout {
+  static constexpr int     VEC    = 16 / int(sizeof(S));
+  static constexpr int     G      = DS / VEC;
+  static constexpr int     REM    = DS - G * VEC;
+  static constexpr int64_t stride = (int64_t(DS) * NQ * int64_t(sizeof(S)) + 15) / 16 * (16 / int64_t(sizeof(S)));
+  __host__ __device__ static constexpr int offset(int k, int q)
+  {
+    return (k < G * VEC) ? (k / VEC) * VEC * NQ + q * VEC + (k % VEC) : G * VEC * NQ + q * REM + (k - G * VEC);
+  }
+};
+
+// packed index of the symmetric tangent: upper triangle, column-major, so a
+// forward-mode pass seeded at input s produces entries s(s+1)/2 .. s(s+1)/2+s
+__host__ __device__ constexpr int packed_index(int r, int s) { return r <= s ? s * (s + 1) / 2 + r : r * (r + 1) / 2 + s; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // With remap on, XCD x instead works through a contiguous run of logical
@@ -59,7 +107,8 @@ __device__ __forceinline__ void lds_barrier()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__device__ __forceinline__ void scatter_add(double* r, int64_t o, double v, int atomic)
+template <typename T>
+__device__ __forceinline__ void scatter_add(T* r, int64_t o, T v, int atomic)
 {
   if (atomic)
     unsafeAtomicAdd(r + o, v);
@@ -98,16 +147,13 @@ __device__ __forceinline__ void trilinear_jacobian(const double* X, double x0, d
 
 // Quadrature-point stage of the residual / linearization: parent-space
 // (value, grad) in, parent-space (source, flux) out, following
-// fn/finite_element.hpp:246-322 (grad_x = grad_xi inv(J);
-// source *= detJ; flux = flux inv(J)^T... * detJ).
-template <class QF, int NC, bool FULL>
-__device__ __forceinline__ void qpoint_eval(const double* prm, const double* J, const double* val,
+// fn/finite_element.hpp:246-322: grad_x = grad_xi inv(J); source *= detJ;
+// flux_parent[c][j] = detJ sum_k flux[c][k] inv(J)[j][k].
+template <class QF, int NC>
+__device__ __forceinline__ void qpoint_eval(const double* prm, const double* Ji, double dJ, const double* val,
                                             const double* gp, double* src, double* fp)
 {
-  double Ji[9];
-  inv3(J, Ji);
-  const double dJ = det3(J);
-  double       g[3 * NC], s[NC], f[3 * NC];
+  double g[3 * NC], s[NC], f[3 * NC];
 #pragma unroll
   for (int c = 0; c < NC; c++)
 #pragma unroll
@@ -163,24 +209,87 @@ __device__ __forceinline__ void qpoint_tangent_column(const double* prm, const d
           (f[c * 3 + 0].d * Ji[j * 3 + 0] + f[c * 3 + 1].d * Ji[j * 3 + 1] + f[c * 3 + 2].d * Ji[j * 3 + 2]) * dJ;
 }
 
-template <int N, int NC, int MODE, class QF, int LAYOUT>
+// Loads the DS tangent entries of quadrature point q as T.
+template <typename T, typename S, int DS, int NQ>
+__device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
+{
+  using L = TangentLayout<S, DS, NQ>;
+  typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  const S* Dp = static_cast<const S*>(base);
+#pragma unroll
+  for (int g = 0; g < L::G; g++) {
+    const u4 v = __builtin_nontemporal_load(reinterpret_cast<const u4*>(Dp + g * L::VEC * NQ + q * L::VEC));
+    if constexpr (sizeof(S) == 8) {
+      Dr[2 * g]     = T(__hiloint2double(int(v.y), int(v.x)));
+      Dr[2 * g + 1] = T(__hiloint2double(int(v.w), int(v.z)));
+    } else if constexpr (sizeof(S) == 4) {
+      Dr[4 * g]     = T(__uint_as_float(v.x));
+      Dr[4 * g + 1] = T(__uint_as_float(v.y));
+      Dr[4 * g + 2] = T(__uint_as_float(v.z));
+      Dr[4 * g + 3] = T(__uint_as_float(v.w));
+    } else {
+      const unsigned int w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        Dr[8 * g + 2 * i]     = T(__uint_as_float(w[i] << 16));
+        Dr[8 * g + 2 * i + 1] = T(__uint_as_float(w[i] & 0xffff0000u));
+      }
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < L::REM; j++) {
+    const auto raw = __builtin_nontemporal_load(
+        reinterpret_cast<const typename StoreTraits<S>::raw*>(Dp + L::G * L::VEC * NQ + q * L::REM + j));
+    if constexpr (sizeof(S) == 2)
+      Dr[L::G * L::VEC + j] = T(bf16_to_float(uint32_t(raw)));
+    else
+      Dr[L::G * L::VEC + j] = T(raw);
+  }
+}
+
+// Stores tangent entry k (FP64 value) of quadrature point q: entries arrive in
+// increasing k; whole 16-byte groups are stored at once from `pend`.
+template <typename S, int DS, int NQ>
+__device__ __forceinline__ void store_tangent_entry(void* base, int q, int k, double v,
+                                                    typename StoreTraits<S>::raw* pend)
+{
+  using L   = TangentLayout<S, DS, NQ>;
+  using raw = typename StoreTraits<S>::raw;
+  raw* Dp   = static_cast<raw*>(base);
+  if (k < L::G * L::VEC) {
+    pend[k % L::VEC] = raw(to_store(v, static_cast<S*>(nullptr)));
+    if (k % L::VEC == L::VEC - 1) {
+      typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+      u4 w;
+      __builtin_memcpy(&w, pend, 16);
+      __builtin_nontemporal_store(w, reinterpret_cast<u4*>(Dp + (k / L::VEC) * L::VEC * NQ + q * L::VEC));
+    }
+  } else {
+    __builtin_nontemporal_store(raw(to_store(v, static_cast<S*>(nullptr))),
+                                Dp + L::G * L::VEC * NQ + q * L::REM + (k - L::G * L::VEC));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// residual (MODE_RESIDUAL) and linearization (MODE_LINEARIZE, tangent in S)
+// ---------------------------------------------------------------------------
+template <int N, int NC, int MODE, class QF, int LAYOUT, typename S>
 __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
 {
-  constexpr int  NQ       = N * N * N;
-  constexpr int  EB       = 256 / NQ;
-  constexpr bool FULL     = (LAYOUT == LAYOUT_FULL_SYM);
-  constexpr bool JVP      = (MODE == MODE_JVP);
-  constexpr bool HAS_SRC  = JVP ? FULL : QF::HAS_SOURCE;
-  constexpr bool NEED_VAL = JVP ? FULL : QF::USES_VALUE;
-  constexpr int  M        = FULL ? 4 * NC : 3 * NC;
-  constexpr int  DS       = M * (M + 1) / 2;
-  constexpr int  R        = NC * NQ;  // one LDS region
-  constexpr int  PER      = (6 + (HAS_SRC ? 1 : 0)) * R;
-  constexpr int  TAB      = 4 * N * N + N;
+  constexpr int  NQ      = N * N * N;
+  constexpr int  EB      = 256 / NQ;
+  constexpr bool FULL    = (LAYOUT == LAYOUT_FULL_SYM);
+  constexpr bool HAS_SRC = QF::HAS_SOURCE;
+  constexpr int  M       = FULL ? 4 * NC : 3 * NC;
+  constexpr int  DS      = M * (M + 1) / 2;
+  constexpr int  R       = NC * NQ;  // one LDS region
+  constexpr int  PER     = (6 + (HAS_SRC ? 1 : 0)) * R;
+  constexpr int  TAB     = 4 * N * N + N;
+  using Lay              = TangentLayout<S, DS, NQ>;
 
   __shared__ double smem[EB * PER];
   __shared__ double stab[TAB];
-  __shared__ double sxv[JVP ? 1 : EB * 24];
+  __shared__ double sxv[EB * 24];
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
@@ -190,7 +299,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   const int     i2     = li / (N * N);
   const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
   const bool    active = (el < EB) && (slot < P.slot_end);
-  double*       S      = smem + (el < EB ? el : 0) * PER;
+  double*       S_     = smem + (el < EB ? el : 0) * PER;
+  const double* u      = static_cast<const double*>(P.u);
+  double*       r      = static_cast<double*>(P.r);
 
   for (int k = tid; k < TAB; k += blockDim.x) stab[k] = P.tables[k];
   const double* tB  = stab;
@@ -199,37 +310,13 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   const double* tGw = stab + 3 * N * N;
   const double* txq = stab + 4 * N * N;
 
-  // ---- gather: the nodal loads are issued first ...
+  // ---- gather
   int64_t node = 0;
-  double  ug[NC];
   if (active) {
     node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-    for (int c = 0; c < NC; c++) ug[c] = P.u[lidx(node, c, NC, P.nnodes, P.ordering)];
-  }
-  // ... then the stored tangent for J.v.  Memory counters retire in order, so
-  // waiting for the nodal values does not wait for the tangent stream, which
-  // stays in flight through the interpolation stages.
-  double Dr[JVP ? DS : 1];
-  if constexpr (JVP) {
-    if (active) {
-      typedef double dbl2 __attribute__((ext_vector_type(2)));
-      const double* Dp = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
-#pragma unroll
-      for (int k = 0; k + 1 < DS; k += 2) {
-        const dbl2 v = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(Dp + tangent_offset(k, li, DS, NQ)));
-        Dr[k]        = v.x;
-        Dr[k + 1]    = v.y;
-      }
-      if constexpr (DS & 1) Dr[DS - 1] = __builtin_nontemporal_load(Dp + tangent_offset(DS - 1, li, DS, NQ));
-    }
-  }
-  if (active) {
-#pragma unroll
-    for (int c = 0; c < NC; c++) S[c * NQ + li] = ug[c];
-    if constexpr (!JVP) {
-      for (int k = li; k < 24; k += NQ) sxv[el * 24 + k] = P.coords[size_t(slot) * 24 + k];
-    }
+    for (int c = 0; c < NC; c++) S_[c * NQ + li] = u[lidx(node, c, NC, P.nnodes, P.ordering)];
+    for (int k = li; k < 24; k += NQ) sxv[el * 24 + k] = P.coords[size_t(slot) * 24 + k];
   }
   lds_barrier();
 
@@ -237,15 +324,15 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* u  = S + c * NQ + (i2 * N + i1) * N;
+      const double* uu = S_ + c * NQ + (i2 * N + i1) * N;
       double        s0 = 0.0, s1 = 0.0;
 #pragma unroll
       for (int x = 0; x < N; x++) {
-        s0 += tB[i0 * N + x] * u[x];
-        s1 += tG[i0 * N + x] * u[x];
+        s0 += tB[i0 * N + x] * uu[x];
+        s1 += tG[i0 * N + x] * uu[x];
       }
-      S[1 * R + c * NQ + li] = s0;
-      S[2 * R + c * NQ + li] = s1;
+      S_[1 * R + c * NQ + li] = s0;
+      S_[2 * R + c * NQ + li] = s1;
     }
   }
   lds_barrier();
@@ -254,8 +341,8 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* a10 = S + 1 * R + c * NQ;
-      const double* a11 = S + 2 * R + c * NQ;
+      const double* a10 = S_ + 1 * R + c * NQ;
+      const double* a11 = S_ + 2 * R + c * NQ;
       double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
       for (int y = 0; y < N; y++) {
@@ -265,9 +352,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
         s1 += b * a11[idx];
         s2 += g * a10[idx];
       }
-      S[3 * R + c * NQ + li] = s0;
-      S[4 * R + c * NQ + li] = s1;
-      S[5 * R + c * NQ + li] = s2;
+      S_[3 * R + c * NQ + li] = s0;
+      S_[4 * R + c * NQ + li] = s1;
+      S_[5 * R + c * NQ + li] = s2;
     }
   }
   lds_barrier();
@@ -277,15 +364,15 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
     double val[NC], gp[3 * NC];
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* a20 = S + 3 * R + c * NQ;
-      const double* a21 = S + 4 * R + c * NQ;
-      const double* a22 = S + 5 * R + c * NQ;
+      const double* a20 = S_ + 3 * R + c * NQ;
+      const double* a21 = S_ + 4 * R + c * NQ;
+      const double* a22 = S_ + 5 * R + c * NQ;
       double        v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
 #pragma unroll
       for (int z = 0; z < N; z++) {
         const int    idx = (z * N + i1) * N + i0;
         const double b = tB[i2 * N + z], g = tG[i2 * N + z];
-        if constexpr (NEED_VAL) v += b * a20[idx];
+        if constexpr (QF::USES_VALUE) v += b * a20[idx];
         g0 += b * a21[idx];
         g1 += b * a22[idx];
         g2 += g * a20[idx];
@@ -296,57 +383,30 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
       gp[c * 3 + 2] = g2;
     }
 
-    double src[NC], fp[3 * NC];
-    if constexpr (JVP) {
-      // chain rule with the stored symmetric tangent (fn/tensor.hpp:1791-1869)
-      double in[M], out[M];
+    double J[9], Ji[9];
+    trilinear_jacobian(sxv + el * 24, txq[i0], txq[i1], txq[i2], J);
+    inv3(J, Ji);
+    const double dJ = det3(J);
+    double       src[NC], fp[3 * NC];
+    qpoint_eval<QF, NC>(P.params, Ji, dJ, val, gp, src, fp);
+    if constexpr (MODE == MODE_LINEARIZE) {
+      // columns s = 0..M-1 give entries (r <= s, s) in increasing packed order
+      void*                          Dp = static_cast<char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S);
+      typename StoreTraits<S>::raw pend[Lay::VEC];
 #pragma unroll
-      for (int c = 0; c < NC; c++) {
-        if constexpr (FULL) in[c] = val[c];
+      for (int s = 0; s < M; s++) {
+        double col[M];
+        qpoint_tangent_column<QF, NC, FULL>(P.params, Ji, dJ, val, gp, s, col);
 #pragma unroll
-        for (int d = 0; d < 3; d++) in[(FULL ? NC : 0) + c * 3 + d] = gp[c * 3 + d];
-      }
-#pragma unroll
-      for (int r = 0; r < M; r++) out[r] = 0.0;
-#pragma unroll
-      for (int r = 0; r < M; r++)
-#pragma unroll
-        for (int s = r; s < M; s++) {
-          const double Dk = Dr[packed_index<M>(r, s)];
-          out[r] += Dk * in[s];
-          if (s != r) out[s] += Dk * in[r];
-        }
-#pragma unroll
-      for (int c = 0; c < NC; c++) {
-        src[c] = FULL ? out[c] : 0.0;
-#pragma unroll
-        for (int d = 0; d < 3; d++) fp[c * 3 + d] = out[(FULL ? NC : 0) + c * 3 + d];
-      }
-    } else {
-      double J[9];
-      trilinear_jacobian(sxv + el * 24, txq[i0], txq[i1], txq[i2], J);
-      qpoint_eval<QF, NC, FULL>(P.params, J, val, gp, src, fp);
-      if constexpr (MODE == MODE_LINEARIZE) {
-        double Ji[9];
-        inv3(J, Ji);
-        const double dJ = det3(J);
-        double*      Dp = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
-#pragma unroll 1
-        for (int s = 0; s < M; s++) {
-          double col[M];
-          qpoint_tangent_column<QF, NC, FULL>(P.params, Ji, dJ, val, gp, s, col);
-          // keep the upper triangle: entry (r, s) for r <= s
-          for (int r = 0; r <= s; r++)
-            __builtin_nontemporal_store(col[r], Dp + tangent_offset(packed_index<M>(r, s), li, DS, NQ));
-        }
+        for (int rr = 0; rr <= s; rr++) store_tangent_entry<S, DS, NQ>(Dp, li, packed_index(rr, s), col[rr], pend);
       }
     }
     // flux -> regions 0..2 (u and stage-x partials are dead), source -> region 6
 #pragma unroll
     for (int c = 0; c < NC; c++) {
 #pragma unroll
-      for (int d = 0; d < 3; d++) S[(c * 3 + d) * NQ + li] = fp[c * 3 + d];
-      if constexpr (HAS_SRC) S[6 * R + c * NQ + li] = src[c];
+      for (int d = 0; d < 3; d++) S_[(c * 3 + d) * NQ + li] = fp[c * 3 + d];
+      if constexpr (HAS_SRC) S_[6 * R + c * NQ + li] = src[c];
     }
   }
   lds_barrier();
@@ -355,10 +415,10 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* f0 = S + (c * 3 + 0) * NQ;
-      const double* f1 = S + (c * 3 + 1) * NQ;
-      const double* f2 = S + (c * 3 + 2) * NQ;
-      const double* sr = S + 6 * R + c * NQ;
+      const double* f0 = S_ + (c * 3 + 0) * NQ;
+      const double* f1 = S_ + (c * 3 + 1) * NQ;
+      const double* f2 = S_ + (c * 3 + 2) * NQ;
+      const double* sr = S_ + 6 * R + c * NQ;
       double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
 #pragma unroll
       for (int qx = 0; qx < N; qx++) {
@@ -369,9 +429,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
         s1 += f1[q] * bw;
         s2 += f2[q] * bw;
       }
-      S[3 * R + c * NQ + li] = s0;
-      S[4 * R + c * NQ + li] = s1;
-      S[5 * R + c * NQ + li] = s2;
+      S_[3 * R + c * NQ + li] = s0;
+      S_[4 * R + c * NQ + li] = s1;
+      S_[5 * R + c * NQ + li] = s2;
     }
   }
   lds_barrier();
@@ -380,9 +440,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* a20 = S + 3 * R + c * NQ;
-      const double* a21 = S + 4 * R + c * NQ;
-      const double* a22 = S + 5 * R + c * NQ;
+      const double* a20 = S_ + 3 * R + c * NQ;
+      const double* a21 = S_ + 4 * R + c * NQ;
+      const double* a22 = S_ + 5 * R + c * NQ;
       double        s0 = 0.0, s1 = 0.0;
 #pragma unroll
       for (int qy = 0; qy < N; qy++) {
@@ -391,8 +451,8 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
         s0 += a20[idx] * bw + a21[idx] * gw;
         s1 += a22[idx] * bw;
       }
-      S[0 * R + c * NQ + li] = s0;
-      S[1 * R + c * NQ + li] = s1;
+      S_[0 * R + c * NQ + li] = s0;
+      S_[1 * R + c * NQ + li] = s1;
     }
   }
   lds_barrier();
@@ -401,24 +461,219 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      const double* a10 = S + 0 * R + c * NQ;
-      const double* a11 = S + 1 * R + c * NQ;
-      double        r   = 0.0;
+      const double* a10 = S_ + 0 * R + c * NQ;
+      const double* a11 = S_ + 1 * R + c * NQ;
+      double        acc = 0.0;
 #pragma unroll
       for (int qz = 0; qz < N; qz++) {
         const int idx = (qz * N + i1) * N + i0;
-        r += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
+        acc += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
       }
-      scatter_add(P.r, lidx(node, c, NC, P.nnodes, P.ordering), r, P.scatter);
+      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc, P.scatter);
     }
   }
 }
 
-// Diagonal of the linearized operator from the stored tangent:
+// ---------------------------------------------------------------------------
+// J.v with the stored tangent (fn/domain_integral_kernels.hpp:223-304): vectors
+// and arithmetic in T, tangent stored in S
+// ---------------------------------------------------------------------------
+template <int N, int NC, int LAYOUT, typename T, typename S>
+__global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
+{
+  constexpr int  NQ   = N * N * N;
+  constexpr int  EB   = 256 / NQ;
+  constexpr bool FULL = (LAYOUT == LAYOUT_FULL_SYM);
+  constexpr int  M    = FULL ? 4 * NC : 3 * NC;
+  constexpr int  DS   = M * (M + 1) / 2;
+  constexpr int  R    = NC * NQ;
+  constexpr int  PER  = (6 + (FULL ? 1 : 0)) * R;
+  constexpr int  TAB  = 4 * N * N;
+  using Lay           = TangentLayout<S, DS, NQ>;
+
+  __shared__ T smem[EB * PER];
+  __shared__ T stab[TAB];
+
+  const int     tid    = threadIdx.x;
+  const int     el     = tid / NQ;
+  const int     li     = tid - el * NQ;
+  const int     i0     = li % N;
+  const int     i1     = (li / N) % N;
+  const int     i2     = li / (N * N);
+  const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
+  const bool    active = (el < EB) && (slot < P.slot_end);
+  T*            S_     = smem + (el < EB ? el : 0) * PER;
+  const T*      u      = static_cast<const T*>(P.u);
+  T*            r      = static_cast<T*>(P.r);
+
+  for (int k = tid; k < TAB; k += blockDim.x) stab[k] = T(P.tables[k]);
+  const T* tB  = stab;
+  const T* tG  = stab + N * N;
+  const T* tBw = stab + 2 * N * N;
+  const T* tGw = stab + 3 * N * N;
+
+  // ---- gather loads first, then the tangent stream: memory counters retire
+  // in order, so waiting for the nodal values does not wait for the tangent,
+  // which stays in flight through the interpolation stages
+  int64_t node = 0;
+  T       ug[NC];
+  if (active) {
+    node = P.map[size_t(slot) * NQ + li];
+#pragma unroll
+    for (int c = 0; c < NC; c++) ug[c] = u[lidx(node, c, NC, P.nnodes, P.ordering)];
+  }
+  T Dr[DS];
+  if (active) load_tangent<T, S, DS, NQ>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li, Dr);
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) S_[c * NQ + li] = ug[c];
+  }
+  lds_barrier();
+
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* uu = S_ + c * NQ + (i2 * N + i1) * N;
+      T        s0 = T(0), s1 = T(0);
+#pragma unroll
+      for (int x = 0; x < N; x++) {
+        s0 += tB[i0 * N + x] * uu[x];
+        s1 += tG[i0 * N + x] * uu[x];
+      }
+      S_[1 * R + c * NQ + li] = s0;
+      S_[2 * R + c * NQ + li] = s1;
+    }
+  }
+  lds_barrier();
+
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* a10 = S_ + 1 * R + c * NQ;
+      const T* a11 = S_ + 2 * R + c * NQ;
+      T        s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+      for (int y = 0; y < N; y++) {
+        const int idx = (i2 * N + y) * N + i0;
+        const T   b = tB[i1 * N + y], g = tG[i1 * N + y];
+        s0 += b * a10[idx];
+        s1 += b * a11[idx];
+        s2 += g * a10[idx];
+      }
+      S_[3 * R + c * NQ + li] = s0;
+      S_[4 * R + c * NQ + li] = s1;
+      S_[5 * R + c * NQ + li] = s2;
+    }
+  }
+  lds_barrier();
+
+  if (active) {
+    T in[M], out[M];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* a20 = S_ + 3 * R + c * NQ;
+      const T* a21 = S_ + 4 * R + c * NQ;
+      const T* a22 = S_ + 5 * R + c * NQ;
+      T        v = T(0), g0 = T(0), g1 = T(0), g2 = T(0);
+#pragma unroll
+      for (int z = 0; z < N; z++) {
+        const int idx = (z * N + i1) * N + i0;
+        const T   b = tB[i2 * N + z], g = tG[i2 * N + z];
+        if constexpr (FULL) v += b * a20[idx];
+        g0 += b * a21[idx];
+        g1 += b * a22[idx];
+        g2 += g * a20[idx];
+      }
+      if constexpr (FULL) in[c] = v;
+      in[(FULL ? NC : 0) + c * 3 + 0] = g0;
+      in[(FULL ? NC : 0) + c * 3 + 1] = g1;
+      in[(FULL ? NC : 0) + c * 3 + 2] = g2;
+    }
+    // chain rule with the stored symmetric tangent (fn/tensor.hpp:1791-1869)
+#pragma unroll
+    for (int k = 0; k < M; k++) out[k] = T(0);
+#pragma unroll
+    for (int s = 0; s < M; s++)
+#pragma unroll
+      for (int rr = 0; rr <= s; rr++) {
+        const T Dk = Dr[packed_index(rr, s)];
+        out[rr] += Dk * in[s];
+        if (rr != s) out[s] += Dk * in[rr];
+      }
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+#pragma unroll
+      for (int d = 0; d < 3; d++) S_[(c * 3 + d) * NQ + li] = out[(FULL ? NC : 0) + c * 3 + d];
+      if constexpr (FULL) S_[6 * R + c * NQ + li] = out[c];
+    }
+  }
+  lds_barrier();
+
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* f0 = S_ + (c * 3 + 0) * NQ;
+      const T* f1 = S_ + (c * 3 + 1) * NQ;
+      const T* f2 = S_ + (c * 3 + 2) * NQ;
+      const T* sr = S_ + 6 * R + c * NQ;
+      T        s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+      for (int qx = 0; qx < N; qx++) {
+        const int q  = (i2 * N + i1) * N + qx;
+        const T   bw = tBw[qx * N + i0], gw = tGw[qx * N + i0];
+        if constexpr (FULL) s0 += sr[q] * bw;
+        s0 += f0[q] * gw;
+        s1 += f1[q] * bw;
+        s2 += f2[q] * bw;
+      }
+      S_[3 * R + c * NQ + li] = s0;
+      S_[4 * R + c * NQ + li] = s1;
+      S_[5 * R + c * NQ + li] = s2;
+    }
+  }
+  lds_barrier();
+
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* a20 = S_ + 3 * R + c * NQ;
+      const T* a21 = S_ + 4 * R + c * NQ;
+      const T* a22 = S_ + 5 * R + c * NQ;
+      T        s0 = T(0), s1 = T(0);
+#pragma unroll
+      for (int qy = 0; qy < N; qy++) {
+        const int idx = (i2 * N + qy) * N + i0;
+        const T   bw = tBw[qy * N + i1], gw = tGw[qy * N + i1];
+        s0 += a20[idx] * bw + a21[idx] * gw;
+        s1 += a22[idx] * bw;
+      }
+      S_[0 * R + c * NQ + li] = s0;
+      S_[1 * R + c * NQ + li] = s1;
+    }
+  }
+  lds_barrier();
+
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const T* a10 = S_ + 0 * R + c * NQ;
+      const T* a11 = S_ + 1 * R + c * NQ;
+      T        acc = T(0);
+#pragma unroll
+      for (int qz = 0; qz < N; qz++) {
+        const int idx = (qz * N + i1) * N + i0;
+        acc += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
+      }
+      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc, P.scatter);
+    }
+  }
+}
+
+// Diagonal of the linearized operator from the stored tangent (FP64 out):
 //   diag(c, j) = sum_q [w phi_j, w grad phi_j]_c . D(q) . [phi_j, grad phi_j]_c
 // (the diagonal entries element_gradient_kernel would assemble,
 //  fn/domain_integral_kernels.hpp:327-361).  Lane li owns node j = li.
-template <int N, int NC, int LAYOUT>
+template <int N, int NC, int LAYOUT, typename S>
 __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
 {
   constexpr int  NQ   = N * N * N;
@@ -428,6 +683,7 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   constexpr int  DS   = M * (M + 1) / 2;
   constexpr int  VO   = FULL ? NC : 0;
   constexpr int  TAB  = 4 * N * N + N;
+  using Lay           = TangentLayout<S, DS, NQ>;
   __shared__ double stab[TAB];
 
   const int     tid    = threadIdx.x;
@@ -437,18 +693,25 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
   const bool    active = (el < EB) && (slot < P.slot_end);
   for (int k = tid; k < TAB; k += blockDim.x) stab[k] = P.tables[k];
-  lds_barrier();
+  __syncthreads();
   if (!active) return;
   const double* tB  = stab;
   const double* tG  = stab + N * N;
   const double* tBw = stab + 2 * N * N;
   const double* tGw = stab + 3 * N * N;
 
+  using raw               = typename StoreTraits<S>::raw;
+  const raw* Dbase        = reinterpret_cast<const raw*>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S));
+  auto       Dat          = [&](int k, int q) -> double {
+    const raw v = Dbase[Lay::offset(k, q)];
+    if constexpr (sizeof(S) == 2)
+      return double(bf16_to_float(uint32_t(v)));
+    else
+      return double(v);
+  };
   double acc[NC];
 #pragma unroll
   for (int c = 0; c < NC; c++) acc[c] = 0.0;
-  const double* Dbase = P.D + size_t(slot) * tangent_slot_stride(DS, NQ);
-  auto          Dat   = [&](int k, int q) { return Dbase[tangent_offset(k, q, DS, NQ)]; };
   for (int q = 0; q < NQ; q++) {
     const int    qx = q % N, qy = (q / N) % N, qz = q / (N * N);
     const double bx = tB[qx * N + jx], by = tB[qy * N + jy], bz = tB[qz * N + jz];
@@ -464,86 +727,109 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) {
-          const int r = VO + c * 3 + i, s = VO + c * 3 + j;
-          const int k = r <= s ? packed_index<M>(r, s) : packed_index<M>(s, r);
-          a += tg[i] * Dat(k, q) * ug[j];
-        }
+        for (int j = 0; j < 3; j++) a += tg[i] * Dat(packed_index(VO + c * 3 + i, VO + c * 3 + j), q) * ug[j];
       if constexpr (FULL) {
-        a += tv * Dat(packed_index<M>(c, c), q) * uv;
+        a += tv * Dat(packed_index(c, c), q) * uv;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          const int s = VO + c * 3 + j;
-          const int k = packed_index<M>(c, s);  // c < s always
-          a += tv * Dat(k, q) * ug[j] + tg[j] * Dat(k, q) * uv;
+          const double d = Dat(packed_index(c, VO + c * 3 + j), q);
+          a += tv * d * ug[j] + tg[j] * d * uv;
         }
       }
       acc[c] += a;
     }
   }
   const int64_t node = P.map[size_t(slot) * NQ + li];
+  double*       out  = static_cast<double*>(P.r);
 #pragma unroll
-  for (int c = 0; c < NC; c++) scatter_add(P.r, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
+  for (int c = 0; c < NC; c++) scatter_add(out, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
 }
 
-template <int N, int NC, int MODE, class QF, int LAYOUT>
-inline hipError_t launch_k(const ElemLaunch& P)
+// ---------------------------------------------------------------------------
+// launch + dispatch
+// ---------------------------------------------------------------------------
+template <int N, typename K>
+inline hipError_t launch_grid(const ElemLaunch& P, K kernel)
 {
-  constexpr int NQ = N * N * N;
-  constexpr int EB = 256 / NQ;
+  constexpr int EB = 256 / (N * N * N);
   const int64_t n  = P.slot_end - P.slot_begin;
   if (n <= 0) return hipSuccess;
   const unsigned grid = unsigned((n + EB - 1) / EB);
-  if constexpr (MODE == MODE_DIAG)
-    hipLaunchKernelGGL((k_diagonal<N, NC, LAYOUT>), dim3(grid), dim3(256), 0, P.stream, P);
-  else
-    hipLaunchKernelGGL((k_element<N, NC, MODE, QF, LAYOUT>), dim3(grid), dim3(256), 0, P.stream, P);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, P.stream, P);
   return hipGetLastError();
 }
 
-// Dispatch on (mode, qfunction, components, tangent layout) for one order.
+template <int N, int NC, int LAYOUT, typename S>
+inline int dispatch_gradient(const ElemLaunch& P, hipError_t* err)
+{
+  if (P.mode == MODE_DIAG) {
+    *err = launch_grid<N>(P, k_diagonal<N, NC, LAYOUT, S>);
+    return 1;
+  }
+  if (P.vprec == PREC_F64) {
+    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, double, S>);
+    return 1;
+  }
+  if (P.vprec == PREC_F32) {
+    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S>);
+    return 1;
+  }
+  return 0;
+}
+
+template <int N, int NC, int LAYOUT>
+inline int dispatch_gradient_prec(const ElemLaunch& P, hipError_t* err)
+{
+  switch (P.tprec) {
+    case PREC_F64: return dispatch_gradient<N, NC, LAYOUT, double>(P, err);
+    case PREC_F32: return dispatch_gradient<N, NC, LAYOUT, float>(P, err);
+    case PREC_BF16: return dispatch_gradient<N, NC, LAYOUT, bf16>(P, err);
+  }
+  return 0;
+}
+
+template <int N, int NC, class QF, int LAYOUT>
+inline int dispatch_eval(const ElemLaunch& P, hipError_t* err)
+{
+  if (P.mode == MODE_RESIDUAL) {
+    *err = launch_grid<N>(P, k_element<N, NC, MODE_RESIDUAL, QF, LAYOUT, double>);
+    return 1;
+  }
+  switch (P.tprec) {
+    case PREC_F64: *err = launch_grid<N>(P, k_element<N, NC, MODE_LINEARIZE, QF, LAYOUT, double>); return 1;
+    case PREC_F32: *err = launch_grid<N>(P, k_element<N, NC, MODE_LINEARIZE, QF, LAYOUT, float>); return 1;
+    case PREC_BF16: *err = launch_grid<N>(P, k_element<N, NC, MODE_LINEARIZE, QF, LAYOUT, bf16>); return 1;
+  }
+  return 0;
+}
+
+// Dispatch on (mode, qfunction, components, tangent layout, precisions) for one order.
 template <int N>
 inline int dispatch_order(const ElemLaunch& P, hipError_t* err)
 {
   const bool full = (P.layout == LAYOUT_FULL_SYM);
   if (P.mode == MODE_JVP || P.mode == MODE_DIAG) {
-    const bool diag = (P.mode == MODE_DIAG);
-    if (P.nc == 1 && !full)
-      *err = diag ? launch_k<N, 1, MODE_DIAG, QFNone<1>, LAYOUT_GRAD_SYM>(P)
-                  : launch_k<N, 1, MODE_JVP, QFNone<1>, LAYOUT_GRAD_SYM>(P);
-    else if (P.nc == 1 && full)
-      *err = diag ? launch_k<N, 1, MODE_DIAG, QFNone<1>, LAYOUT_FULL_SYM>(P)
-                  : launch_k<N, 1, MODE_JVP, QFNone<1>, LAYOUT_FULL_SYM>(P);
-    else if (P.nc == 3 && !full)
-      *err = diag ? launch_k<N, 3, MODE_DIAG, QFNone<3>, LAYOUT_GRAD_SYM>(P)
-                  : launch_k<N, 3, MODE_JVP, QFNone<3>, LAYOUT_GRAD_SYM>(P);
-    else if (P.nc == 3 && full)
-      *err = diag ? launch_k<N, 3, MODE_DIAG, QFNone<3>, LAYOUT_FULL_SYM>(P)
-                  : launch_k<N, 3, MODE_JVP, QFNone<3>, LAYOUT_FULL_SYM>(P);
-    else
-      return 0;
-    return 1;
+    if (P.nc == 1) return full ? dispatch_gradient_prec<N, 1, LAYOUT_FULL_SYM>(P, err)
+                               : dispatch_gradient_prec<N, 1, LAYOUT_GRAD_SYM>(P, err);
+    if (P.nc == 3) return full ? dispatch_gradient_prec<N, 3, LAYOUT_FULL_SYM>(P, err)
+                               : dispatch_gradient_prec<N, 3, LAYOUT_GRAD_SYM>(P, err);
+    return 0;
   }
-  const bool lin = (P.mode == MODE_LINEARIZE);
-#define SXK_QF(QFT, NCV, LAY)                                                                     \
-  *err = lin ? launch_k<N, NCV, MODE_LINEARIZE, QFT, LAY>(P) : launch_k<N, NCV, MODE_RESIDUAL, QFT, LAY>(P); \
-  return 1;
   switch (P.qf) {
     case QF_ID_HEAT:
-      if (P.nc == 1) { SXK_QF(QFHeat, 1, LAYOUT_GRAD_SYM) }
+      if (P.nc == 1) return dispatch_eval<N, 1, QFHeat, LAYOUT_GRAD_SYM>(P, err);
       break;
     case QF_ID_MASS_DIFFUSION:
-      if (P.nc == 1) { SXK_QF(QFMassDiffusion<1>, 1, LAYOUT_FULL_SYM) }
-      if (P.nc == 3) { SXK_QF(QFMassDiffusion<3>, 3, LAYOUT_FULL_SYM) }
+      if (P.nc == 1) return dispatch_eval<N, 1, QFMassDiffusion<1>, LAYOUT_FULL_SYM>(P, err);
+      if (P.nc == 3) return dispatch_eval<N, 3, QFMassDiffusion<3>, LAYOUT_FULL_SYM>(P, err);
       break;
     case QF_ID_NEOHOOKEAN:
-      if (P.nc == 3) { SXK_QF(QFNeoHookean, 3, LAYOUT_GRAD_SYM) }
+      if (P.nc == 3) return dispatch_eval<N, 3, QFNeoHookean, LAYOUT_GRAD_SYM>(P, err);
       break;
     case QF_ID_LINEAR_ELASTIC:
-      if (P.nc == 3) { SXK_QF(QFLinearElastic, 3, LAYOUT_GRAD_SYM) }
+      if (P.nc == 3) return dispatch_eval<N, 3, QFLinearElastic, LAYOUT_GRAD_SYM>(P, err);
       break;
   }
-#undef SXK_QF
   return 0;
 }
 

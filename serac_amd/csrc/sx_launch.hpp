@@ -14,19 +14,24 @@ enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3 }
 //   FULL_SYM: d[source, parent flux]/d[value, parent grad], M = 4 nc
 enum Layout { LAYOUT_GRAD_SYM = 0, LAYOUT_FULL_SYM = 1 };
 
+// precisions of the stored tangent (tprec) and of J.v vectors (vprec)
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
+
 enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_ID_LINEAR_ELASTIC = 4 };
 
 struct ElemLaunch {
   int           order, nc, qf, mode, layout, ordering;
-  int           scatter;    // 0: plain read-modify-write (colour launches), 1: FP64 atomics
+  int           tprec;      // Precision of the stored tangent
+  int           vprec;      // Precision of the J.v vectors (residual/linearize: FP64)
+  int           scatter;    // 0: plain read-modify-write (colour launches), 1: atomics
   int           xcd_remap;  // 1: give each XCD a contiguous range of element blocks
-  const int*    map;     // [slot][npe] global node ids
-  const double* coords;  // [slot][8 corners][3], corner = a + 2b + 4c
+  const int*    map;        // [slot][npe] global node ids
+  const double* coords;     // [slot][8 corners][3], corner = a + 2b + 4c
   int64_t       slot_begin, slot_end;
   int64_t       nnodes;
-  const double* u;       // input L-vector (trial)
-  double*       r;       // output L-vector (test), accumulated into
-  double*       D;       // stored tangent [slot][DS][Q^3]
+  const void*   u;       // input L-vector (trial), FP64 or (J.v) vprec
+  void*         r;       // output L-vector (test), accumulated into
+  void*         D;       // stored tangent, per-slot layout of sx_element.hpp TangentLayout
   const double* tables;  // B, G, Bw, Gw [Q][N] and GL points [Q]
   double        params[8];
   double        t;
@@ -39,16 +44,11 @@ inline int tangent_ds(int nc, int layout)
   int m = tangent_m(nc, layout);
   return m * (m + 1) / 2;
 }
-
-// Tangent storage per slot (element): the DS packed entries of every
-// quadrature point q, grouped in pairs so that lane q reads 16 bytes at once:
-//   entry k < 2*floor(DS/2):  [(k/2)][q][k%2]
-//   last entry when DS odd:   [floor(DS/2)][q]  (after the pairs)
-// The slot stride is DS*NQ rounded up to even (16-byte alignment).
-__host__ __device__ constexpr int64_t tangent_slot_stride(int ds, int nq) { return (int64_t(ds) * nq + 1) & ~int64_t(1); }
-__host__ __device__ constexpr int tangent_offset(int k, int q, int ds, int nq)
+inline int precision_bytes(int prec) { return prec == PREC_F64 ? 8 : prec == PREC_F32 ? 4 : 2; }
+// bytes of tangent per slot: DS * NQ entries rounded up to 16 bytes
+inline int64_t tangent_slot_bytes(int ds, int nq, int prec)
 {
-  return (k < (ds & ~1)) ? (k >> 1) * 2 * nq + q * 2 + (k & 1) : (ds >> 1) * 2 * nq + q;
+  return (int64_t(ds) * nq * precision_bytes(prec) + 15) / 16 * 16;
 }
 
 // defined in sx_kernels_p{1,2,3,4}.hip; return 0 when the combination is not compiled

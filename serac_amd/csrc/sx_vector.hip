@@ -26,6 +26,11 @@ __global__ void __launch_bounds__(kThreads) k_fill(double* x, double v, int64_t 
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) x[i] = v;
 }
 
+__global__ void __launch_bounds__(kThreads) k_fill_f32(float* x, float v, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) x[i] = v;
+}
+
 __global__ void __launch_bounds__(kThreads) k_axpy(double a, const double* x, double* y, int64_t n)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
@@ -122,6 +127,19 @@ __global__ void __launch_bounds__(kThreads) k_set_rows(double* x, const int32_t*
     x[rows[i]] = v;
 }
 
+__global__ void __launch_bounds__(kThreads) k_set_rows_values(double* x, const int32_t* rows, const double* vals,
+                                                               int64_t nr)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < nr; i += int64_t(gridDim.x) * kThreads)
+    x[rows[i]] = vals[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_axpby(double a, const double* x, double b, double* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = a * x[i] + b * y[i];
+}
+
 __global__ void __launch_bounds__(kThreads) k_copy_rows(double* dst, const double* src, const int32_t* rows,
                                                          int64_t nr)
 {
@@ -174,6 +192,13 @@ hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t vec_fill_f32(float* x, float v, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n)), dim3(kThreads), 0, s, x, v, n);
+  return hipGetLastError();
+}
+
 hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s)
 {
   if (n <= 0) return hipSuccess;
@@ -221,6 +246,20 @@ hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStr
 {
   if (nr <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_set_rows, dim3(grid_for(nr)), dim3(kThreads), 0, s, x, rows, nr, v);
+  return hipGetLastError();
+}
+
+hipError_t set_rows_values(double* x, const int32_t* rows, const double* vals, int64_t nr, hipStream_t s)
+{
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_set_rows_values, dim3(grid_for(nr)), dim3(kThreads), 0, s, x, rows, vals, nr);
+  return hipGetLastError();
+}
+
+hipError_t vec_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(n)), dim3(kThreads), 0, s, a, x, b, y, n);
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 namespace sxk {
 
 hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s);
+hipError_t vec_fill_f32(float* x, float v, int64_t n, hipStream_t s);
 hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s);
 // y = x + (*a_dev) y
 hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s);
@@ -24,6 +25,10 @@ hipError_t cg_update(double* x, double* r, double* z, const double* d, const dou
 hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n, double* part, double* scal,
                       int slot, hipStream_t s);
 hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStream_t s);
+// x[rows[i]] = vals[i]
+hipError_t set_rows_values(double* x, const int32_t* rows, const double* vals, int64_t nr, hipStream_t s);
+// y = a x + b y  (elementwise, FP64)
+hipError_t vec_axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t s);
 hipError_t copy_rows(double* dst, const double* src, const int32_t* rows, int64_t nr, hipStream_t s);
 hipError_t gather_e(const int* map, const double* L, double* E, int64_t ne, int npe, int nc, int64_t nnodes,
                     int ordering, hipStream_t s);

@@ -201,6 +201,43 @@ def test_atomic_scatter_parity(gpu_lib, qf, p, remap):
     assert relerr(grad.diagonal().numpy(), of.diagonal()) < RTOL
 
 
+def rel2(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+# (tangent precision, vector precision): tolerance on ||Jv - Jv_ref|| / ||Jv_ref||
+PRECISIONS = {
+    ("f64", "f64"): 1e-13,
+    ("f32", "f64"): 1e-6,
+    ("f32", "f32"): 1e-5,
+    ("bf16", "f32"): 1.5e-2,
+}
+_PREC = {"f64": 0, "f32": 1, "bf16": 2}
+
+
+@pytest.mark.parametrize("tp,vp", sorted(PRECISIONS))
+@pytest.mark.parametrize("qf,p", [("neohookean", 3), ("heat", 2), ("mass_diffusion_vec", 4), ("neohookean", 4)])
+def test_precision_split_jvp(gpu_lib, tp, vp, qf, p):
+    nc = QF[qf][2]
+    pr = Pair(3 if p < 4 else 2, p, nc)
+    gid, oid, _, params = QF[qf]
+    gf = sx.Functional(pr.gs, [pr.gs])
+    gf.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, _PREC[tp])
+    gf.add_domain_integral(gid, params)
+    of = O.Form(pr.os)
+    of.add_domain_integral(oid, params)
+    U = field(pr.gs, 0.005, 5, 2000)
+    dU = field(pr.gs, 1.0, 6, 3000)
+    R_ref = of.linearize(U)
+    out, grad = gf(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    assert relerr(out.numpy(), R_ref) < RTOL  # the residual stays FP64
+    dtype = np.float32 if vp == "f32" else np.float64
+    jv = grad(sx.DeviceVector.from_numpy(dU, dtype=dtype)).numpy().astype(np.float64)
+    assert rel2(jv, of.jvp(dU)) < PRECISIONS[(tp, vp)]
+    diag_tol = {"f64": 1e-12, "f32": 1e-6, "bf16": 1e-2}[tp]
+    assert rel2(grad.diagonal().numpy(), of.diagonal()) < diag_tol
+
+
 def test_scatter_mode_must_precede_integrals(gpu_lib):
     pr = Pair(2, 1, 1)
     gf = sx.Functional(pr.gs, [pr.gs])
@@ -242,6 +279,30 @@ def test_jacobi_cg_heat_matches_oracle(gpu_lib):
     assert rep["converged"] and rep_ref["converged"]
     assert abs(rep["linear_iterations"] - rep_ref["iterations"]) <= 2
     assert relerr(x.numpy(), x_ref) < 1e-8
+
+
+@pytest.mark.parametrize("warm_start", [0, 1])
+def test_quasistatic_load_steps_match_oracle(gpu_lib, warm_start):
+    pr = Pair(3, 2, 3, perturb=0.2)
+    gf, of = pr.forms("neohookean")
+    ess0 = pr.gs.boundary_dofs(0b000001, 0x7)
+    ess1 = pr.gs.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(ess, ess1), -0.02, 0.0)
+    nsteps = 4
+    xg = sx.DeviceVector(pr.gs.ndof)
+    reps = sx.quasistatic_solve(gf, xg, ess, final, nsteps, warm_start=bool(warm_start), rel_tol=1e-10,
+                                lin_rel_tol=1e-10)
+    assert all(r["converged"] for r in reps)
+    x = np.zeros(pr.gs.ndof)
+    for k in range(1, nsteps + 1):
+        x[ess] = final * k / nsteps
+        x, rep = of.newton(ess, x, rel=1e-10, abs_tol=0.0, lin_rel=1e-10, lin_abs=0.0)
+        assert rep["converged"]
+    assert relerr(xg.numpy(), x) < 1e-8
+    R = of.residual(xg.numpy())
+    R[ess] = 0.0
+    assert np.linalg.norm(R) < 1e-8 * np.linalg.norm(of.residual(np.zeros(pr.gs.ndof)) + 1e-30) + 1e-12
 
 
 def test_newton_neohookean_matches_oracle(gpu_lib):
