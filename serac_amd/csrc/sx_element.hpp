@@ -214,20 +214,28 @@ template <typename T, typename S, int DS, int NQ>
 __device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
 {
   using L = TangentLayout<S, DS, NQ>;
+  // native vector types per storage type: a 16-byte integer vector
+  // reassembled into doubles gets split by the back end into a 16-byte load
+  // plus a second, 8-byte load of the same address
   typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+  typedef double       d2 __attribute__((ext_vector_type(2)));
+  typedef float        f4 __attribute__((ext_vector_type(4)));
   const S* Dp = static_cast<const S*>(base);
 #pragma unroll
   for (int g = 0; g < L::G; g++) {
-    const u4 v = __builtin_nontemporal_load(reinterpret_cast<const u4*>(Dp + g * L::VEC * NQ + q * L::VEC));
+    const S* gp = Dp + g * L::VEC * NQ + q * L::VEC;
     if constexpr (sizeof(S) == 8) {
-      Dr[2 * g]     = T(__hiloint2double(int(v.y), int(v.x)));
-      Dr[2 * g + 1] = T(__hiloint2double(int(v.w), int(v.z)));
+      const d2 v    = __builtin_nontemporal_load(reinterpret_cast<const d2*>(gp));
+      Dr[2 * g]     = T(v.x);
+      Dr[2 * g + 1] = T(v.y);
     } else if constexpr (sizeof(S) == 4) {
-      Dr[4 * g]     = T(__uint_as_float(v.x));
-      Dr[4 * g + 1] = T(__uint_as_float(v.y));
-      Dr[4 * g + 2] = T(__uint_as_float(v.z));
-      Dr[4 * g + 3] = T(__uint_as_float(v.w));
+      const f4 v    = __builtin_nontemporal_load(reinterpret_cast<const f4*>(gp));
+      Dr[4 * g]     = T(v.x);
+      Dr[4 * g + 1] = T(v.y);
+      Dr[4 * g + 2] = T(v.z);
+      Dr[4 * g + 3] = T(v.w);
     } else {
+      const u4 v = __builtin_nontemporal_load(reinterpret_cast<const u4*>(gp));
       const unsigned int w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
       for (int i = 0; i < 4; i++) {
