@@ -26,6 +26,7 @@ typedef struct sx_mesh  sx_mesh;
 typedef struct sx_space sx_space;
 typedef struct sx_form  sx_form;
 typedef struct sx_grad  sx_grad;
+typedef struct sx_qdata sx_qdata;
 
 /* status codes */
 enum {
@@ -51,7 +52,13 @@ enum {
                                geometric nonlinearity on, quasi-static
                                phy/solid_mechanics.hpp:834-852,
                                phy/materials/solid_material.hpp:112-122 */
-  SX_QF_LINEAR_ELASTIC = 4  /* params {lambda, mu}: sigma = lambda tr(eps) I + 2 mu eps */
+  SX_QF_LINEAR_ELASTIC = 4, /* params {lambda, mu}: sigma = lambda tr(eps) I + 2 mu eps */
+  SX_QF_J2_LINEAR      = 5  /* params {E, nu, sigma_y, Hi, Hk, density, geom_nonlin (0/1)}:
+                               MaterialStressFunctor<J2SmallStrain<LinearHardening>>,
+                               quasi-static; state of 10 doubles per quadrature point
+                               {plastic strain [3][3] row-major, accumulated plastic strain}
+                               phy/materials/solid_material.hpp:132-148,199-258,
+                               phy/solid_mechanics.hpp:834-852 */
 };
 
 #define SX_MAX_PARAMS 8
@@ -102,6 +109,16 @@ int sx_space_gather(const sx_space* s, const double* L, double* E);
 int sx_space_scatter_add(const sx_space* s, const double* E, double* L);
 int sx_space_destroy(sx_space* s);
 
+/* ---- quadrature-point state: QuadratureData<State> (fn/quadrature_data.hpp:111-167) ---- */
+/* nelem x nq records of `size` doubles, each set to init[0..size) */
+int sx_qdata_create(int64_t nelem, int nq, int size, const double* init, sx_qdata** out);
+/* out[3] = {nelem, nq, size} */
+int sx_qdata_info(const sx_qdata* q, int64_t* out);
+/* host copies in the reference's order [elem][q][size]; synchronize the device */
+int sx_qdata_get(const sx_qdata* q, double* host);
+int sx_qdata_set(sx_qdata* q, const double* host);
+int sx_qdata_destroy(sx_qdata* q);
+
 /* ---- forms: Functional<test(trials...)> (fn/functional.hpp:189-745) ---- */
 /* constructor, fn/functional.hpp:219-273 (single rank: P = identity) */
 int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntrials, sx_form** out);
@@ -109,6 +126,15 @@ int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntri
  * elems == NULL / nelem == 0 means the whole mesh (EntireDomain). */
 int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
                                 int ndep, const int* elems, int nelem);
+/* AddDomainIntegral with a QuadratureData argument (fn/functional.hpp:286-317,
+ * fn/quadrature_data.hpp:111-167).  qdata must hold one record per element of
+ * the domain (in domain order) and quadrature point; it is borrowed and must
+ * outlive the form.  Required for qfunctions with state, refused otherwise. */
+int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                      const int* depends_on, int ndep, const int* elems, int nelem, sx_qdata* qdata);
+/* updateQdata(bool), fn/functional.hpp:523: while set, every evaluation writes
+ * the qfunction's updated state back to its quadrature data. */
+int sx_form_update_qdata(sx_form* f, int flag);
 /* operator()(t, U...) / operator()(t, differentiate_wrt(U)), fn/functional.hpp:446-513.
  * inputs: ntrials device pointers (T = L on one rank). wrt = -1: residual only;
  * wrt = i: also stores the quadrature-point linearization w.r.t. trial i and
@@ -178,8 +204,11 @@ int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, 
  * their values in x (t = 0) to ess_final (host array, t = 1) over nsteps; each
  * step optionally warm-starts with one linear solve on the previous
  * linearization, du = du_bc + J_bc^{-1} (-R(u) - J du_bc), then runs Newton +
- * Jacobi-CG.  x (device) is updated in place; reports[nsteps] (host) receive
- * per-step Newton / CG iteration counts and residual norms. */
+ * Jacobi-CG.  After each converged step, integrals with quadrature data get
+ * one more residual evaluation with the state committed (as the reference's
+ * time step does, phy/solid_mechanics.hpp:1278-1287).  x (device) is updated in
+ * place; reports[nsteps] (host) receive per-step Newton / CG iteration counts
+ * and residual norms. */
 typedef struct sx_solver_options {
   double rel_tol;     /* Newton, relative to the step's initial residual */
   double abs_tol;     /* Newton, absolute */

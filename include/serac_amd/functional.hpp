@@ -18,6 +18,7 @@
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -288,6 +289,25 @@ struct LinearIsotropic {
   double K;
   double G;
 };
+/// phy/materials/solid_material.hpp:132-148
+struct LinearHardening {
+  double sigma_y;  ///< yield strength
+  double Hi;       ///< isotropic hardening modulus
+};
+/// phy/materials/solid_material.hpp:199-258 (compiled for LinearHardening)
+template <typename HardeningType>
+struct J2SmallStrain {
+  double        E;          ///< Young's modulus
+  double        nu;         ///< Poisson's ratio
+  HardeningType hardening;  ///< flow stress hardening model
+  double        Hk;         ///< kinematic hardening modulus
+  double        density;    ///< mass density
+  /// the hysteresis variables of one quadrature point (10 doubles, in this order)
+  struct State {
+    double plastic_strain[3][3];
+    double accumulated_plastic_strain;
+  };
+};
 /// phy/solid_mechanics.hpp:808-853 (quasi-static: no acceleration argument)
 template <typename Material>
 struct MaterialStressFunctor {
@@ -360,6 +380,65 @@ struct qfunction_traits<MassDiffusion> {
   static constexpr bool      registered = true;
   static int                 id(const MassDiffusion&) { return SX_QF_MASS_DIFFUSION; }
   static std::vector<double> params(const MassDiffusion& q) { return {q.a, q.b}; }
+};
+template <>
+struct qfunction_traits<
+    solid_mechanics::MaterialStressFunctor<solid_mechanics::J2SmallStrain<solid_mechanics::LinearHardening>>> {
+  using material_type  = solid_mechanics::J2SmallStrain<solid_mechanics::LinearHardening>;
+  using integrand_type = solid_mechanics::MaterialStressFunctor<material_type>;
+  using state_type     = material_type::State;
+  static constexpr bool registered = true;
+  static int            id(const integrand_type&) { return SX_QF_J2_LINEAR; }
+  static std::vector<double> params(const integrand_type& q)
+  {
+    const auto& m = q.material_;
+    return {m.E,  m.nu, m.hardening.sigma_y, m.hardening.Hi, m.Hk, m.density,
+            q.geom_nonlin_ == GeometricNonlinearities::On ? 1.0 : 0.0};
+  }
+};
+
+/// QuadratureData<State> (fn/quadrature_data.hpp:111-167): one State per
+/// quadrature point of every element of a domain, kept on the device.  State
+/// must be a struct of doubles.
+template <typename State>
+class QuadratureData {
+ public:
+  static_assert(sizeof(State) % sizeof(double) == 0, "State must consist of doubles");
+  static constexpr int size = int(sizeof(State) / sizeof(double));
+  QuadratureData(int64_t num_elements, int qpts_per_element, const State& init = State{})
+      : nelem_(num_elements), nq_(qpts_per_element)
+  {
+    double v[size];
+    std::memcpy(v, &init, sizeof(State));
+    int rc = sx_qdata_create(num_elements, qpts_per_element, size, v, &h_);
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+  }
+  QuadratureData(const QuadratureData&)            = delete;
+  QuadratureData& operator=(const QuadratureData&) = delete;
+  ~QuadratureData()
+  {
+    if (h_) sx_qdata_destroy(h_);
+  }
+  /// host copy, [element][qpoint]
+  std::vector<State> Get() const
+  {
+    std::vector<State> out(size_t(nelem_) * size_t(nq_));
+    int                rc = sx_qdata_get(h_, reinterpret_cast<double*>(out.data()));
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+    return out;
+  }
+  void Set(const std::vector<State>& st)
+  {
+    if (st.size() != size_t(nelem_) * size_t(nq_)) throw Error(SX_ERR_ARG, "QuadratureData::Set: wrong size");
+    int rc = sx_qdata_set(h_, reinterpret_cast<const double*>(st.data()));
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+  }
+  sx_qdata* handle() const { return h_; }
+
+ private:
+  sx_qdata* h_ = nullptr;
+  int64_t   nelem_;
+  int       nq_;
 };
 
 namespace detail {
@@ -478,6 +557,33 @@ class Functional<test(trials...)> {
                                               domain.elements.empty() ? nullptr : domain.elements.data(),
                                               int(domain.elements.size())));
   }
+  /// AddDomainIntegral with quadrature-point state (the reference's trailing
+  /// shared_ptr<QuadratureData<S>> argument); the form keeps a reference
+  template <int dim, int... args, typename Integrand, typename State>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Domain& domain,
+                         std::shared_ptr<QuadratureData<State>> qdata)
+  {
+    static_assert(dim == 3, "the compiled kernels integrate over hexahedra (Dimension<3>)");
+    static_assert(qfunction_traits<Integrand>::registered,
+                  "integrand type is not in the compiled qfunction registry (see qfunction_traits)");
+    static_assert(((args >= 0 && args < num_trial_spaces) && ...), "DependsOn index out of range");
+    const int                 deps[] = {args...};
+    const std::vector<double> p      = qfunction_traits<Integrand>::params(integrand);
+    detail::check(sx_form_add_domain_integral_qdata(
+        h_, qfunction_traits<Integrand>::id(integrand), p.data(), int(p.size()), deps, int(sizeof...(args)),
+        domain.elements.empty() ? nullptr : domain.elements.data(), int(domain.elements.size()),
+        qdata ? qdata->handle() : nullptr));
+    if (qdata) qdata_keep_.push_back(std::static_pointer_cast<void>(qdata));
+  }
+  template <int dim, int... args, typename Integrand, typename State>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&,
+                         std::shared_ptr<QuadratureData<State>> qdata)
+  {
+    AddDomainIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, Domain{}, std::move(qdata));
+  }
+  /// updateQdata (fn/functional.hpp:523)
+  void updateQdata(bool update_flag) { detail::check(sx_form_update_qdata(h_, update_flag ? 1 : 0)); }
+
   /// alias of the reference (fn/functional.hpp:365-373)
   template <int... args, typename Integrand, typename DomainT>
   void AddVolumeIntegral(DependsOn<args...>, const Integrand& integrand, DomainT& domain)
@@ -514,10 +620,11 @@ class Functional<test(trials...)> {
   sx_form* handle() const { return h_; }
 
  private:
-  sx_form* h_         = nullptr;
-  int64_t  test_size_ = 0;
-  Vector   output_;
-  Gradient grad_;
+  sx_form*                           h_         = nullptr;
+  int64_t                            test_size_ = 0;
+  Vector                             output_;
+  Gradient                           grad_;
+  std::vector<std::shared_ptr<void>> qdata_keep_;  // the form borrows the state buffers
 };
 
 // ---------------------------------------------------------------------------

@@ -20,6 +20,7 @@ QF_HEAT = 1
 QF_MASS_DIFFUSION = 2
 QF_NEOHOOKEAN = 3
 QF_LINEAR_ELASTIC = 4
+QF_J2_LINEAR = 5
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -59,6 +60,13 @@ def _load():
         "orc_integrate": (ctypes.c_int, [ctypes.c_int] * 3 + [_D, _D, _D, ctypes.c_int]),
         "orc_geometric_factors": (ctypes.c_int, [_vp, ctypes.c_int, _D, _D]),
         "orc_qf_eval": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D]),
+        "orc_qf_eval_state": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D, _D]),
+        "orc_qf_tangent": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D]),
+        "orc_form_add_integral_state": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int, _D]),
+        "orc_form_update_qdata": (None, [_vp, ctypes.c_int]),
+        "orc_form_ssize": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "orc_form_get_qdata": (ctypes.c_int, [_vp, ctypes.c_int, _D]),
+        "orc_form_set_qdata": (ctypes.c_int, [_vp, ctypes.c_int, _D]),
         "orc_form": (_vp, [_vp, _vp]),
         "orc_form_free": (None, [_vp]),
         "orc_form_add_integral": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int]),
@@ -157,6 +165,26 @@ def qf_eval(kind, params, nc, value, grad):
     return s, f.reshape(nc, 3)
 
 
+def qf_eval_state(kind, params, nc, value, grad, state):
+    """qfunction with quadrature-point state; returns (source, flux, updated state)."""
+    v = np.ascontiguousarray(value, dtype=np.float64)
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    st = np.array(state, dtype=np.float64)
+    s, f = np.zeros(nc), np.zeros(3 * nc)
+    _check(lib().orc_qf_eval_state(kind, _dp(_params(params)), nc, _dp(v), _dp(g), _dp(s), _dp(f), _dp(st)))
+    return s, f.reshape(nc, 3), st
+
+
+def qf_tangent(kind, params, nc, value, grad, state=None):
+    """Forward-mode d flux / d grad at one point, (3 nc) x (3 nc)."""
+    v = np.ascontiguousarray(value, dtype=np.float64)
+    g = np.ascontiguousarray(grad, dtype=np.float64)
+    st = np.ascontiguousarray(state if state is not None else np.zeros(16), dtype=np.float64)
+    D = np.zeros((3 * nc, 3 * nc))
+    _check(lib().orc_qf_tangent(kind, _dp(_params(params)), nc, _dp(v), _dp(g), _dp(st), _dp(D)))
+    return D
+
+
 class Mesh:
     def __init__(self, handle):
         self.h = handle
@@ -245,6 +273,29 @@ class Form:
         el = np.ascontiguousarray(elems if elems is not None else [], dtype=np.int32)
         _check(lib().orc_form_add_integral(self.h, kind, _dp(_params(params)), _ip(el), len(el)))
         self.nintegrals += 1
+
+    def add_domain_integral_state(self, kind, params, state_init, elems=None):
+        """AddDomainIntegral with QuadratureData starting at state_init at every point."""
+        el = np.ascontiguousarray(elems if elems is not None else [], dtype=np.int32)
+        st = np.ascontiguousarray(state_init, dtype=np.float64)
+        _check(lib().orc_form_add_integral_state(self.h, kind, _dp(_params(params)), _ip(el), len(el), _dp(st)))
+        self.nintegrals += 1
+
+    def update_qdata(self, flag):
+        lib().orc_form_update_qdata(self.h, int(bool(flag)))
+
+    def qdata(self, integral=0, nelem=None):
+        """State of an integral as [elem-in-domain][q][size]."""
+        ss = lib().orc_form_ssize(self.h, integral)
+        nq = (self.space.p + 1) ** 3
+        ne = self.space.ne if nelem is None else nelem
+        out = np.zeros((ne, nq, ss))
+        _check(lib().orc_form_get_qdata(self.h, integral, _dp(out)))
+        return out
+
+    def set_qdata(self, state, integral=0):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        _check(lib().orc_form_set_qdata(self.h, integral, _dp(st)))
 
     def residual(self, U):
         U = np.ascontiguousarray(U, dtype=np.float64)

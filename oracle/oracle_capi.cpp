@@ -153,6 +153,41 @@ int orc_qf_eval(int kind, const double* params, int nc, const double* value, con
   });
 }
 
+// as orc_qf_eval for a qfunction with quadrature-point state (in/out)
+int orc_qf_eval_state(int kind, const double* params, int nc, const double* value, const double* grad,
+                      double* source, double* flux, double* state)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    qf_eval(P, nc, value, grad, source, flux, state);
+  });
+}
+// forward-mode tangent at one point: dflux[a][b] = d flux_a / d grad_b
+// ((3 nc) x (3 nc), row-major); the state is read, not updated
+int orc_qf_tangent(int kind, const double* params, int nc, const double* value, const double* grad,
+                   const double* state, double* dflux)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    dual<9> v[3], g[9], s[3], f[9];
+    for (int i = 0; i < nc; i++) v[i] = dual<9>(value[i]);
+    for (int k = 0; k < 3 * nc; k++) {
+      g[k]             = dual<9>(grad[k]);
+      g[k].gradient[k] = 1.0;
+    }
+    double st[16] = {0};
+    const int ss  = qf_state_size(kind);
+    if (ss) std::copy_n(state, ss, st);
+    qf_eval(P, nc, v, g, s, f, ss ? st : nullptr);
+    for (int a = 0; a < 3 * nc; a++)
+      for (int b = 0; b < 3 * nc; b++) dflux[a * 3 * nc + b] = f[a].gradient[b];
+  });
+}
+
 // ---- forms
 void* orc_form(void* m, void* s)
 {
@@ -168,6 +203,36 @@ int  orc_form_add_integral(void* f, int kind, const double* params, const int* e
     std::vector<int> el;
     if (elems && nelem > 0) el.assign(elems, elems + nelem);
     static_cast<OForm*>(f)->form.add_integral(P, el);
+  });
+}
+// AddDomainIntegral with QuadratureData: every point's state starts at state_init
+int orc_form_add_integral_state(void* f, int kind, const double* params, const int* elems, int nelem,
+                                const double* state_init)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    std::vector<int> el;
+    if (elems && nelem > 0) el.assign(elems, elems + nelem);
+    static_cast<OForm*>(f)->form.add_integral(P, el, state_init);
+  });
+}
+void orc_form_update_qdata(void* f, int flag) { static_cast<OForm*>(f)->form.update_qdata = flag != 0; }
+int  orc_form_ssize(void* f, int integral) { return static_cast<OForm*>(f)->form.integrals[size_t(integral)].ssize; }
+// state of an integral, [elem-in-domain][q][ssize]
+int orc_form_get_qdata(void* f, int integral, double* out)
+{
+  return guard([&] {
+    auto& I = static_cast<OForm*>(f)->form.integrals[size_t(integral)];
+    std::memcpy(out, I.state.data(), I.state.size() * sizeof(double));
+  });
+}
+int orc_form_set_qdata(void* f, int integral, const double* in)
+{
+  return guard([&] {
+    auto& I = static_cast<OForm*>(f)->form.integrals[size_t(integral)];
+    std::memcpy(I.state.data(), in, I.state.size() * sizeof(double));
   });
 }
 int orc_form_eval(void* f, const double* U, double* R, int linearize)

@@ -19,6 +19,10 @@ void        orc_form_free(void* f);
 int         orc_form_add_integral(void* f, int kind, const double* params, const int* elems, int nelem);
 int         orc_form_eval(void* f, const double* U, double* R, int linearize);
 int         orc_form_jvp(void* f, const double* dU, double* dR);
+int         orc_form_add_integral_state(void* f, int kind, const double* params, const int* elems, int nelem,
+                                        const double* state_init);
+void        orc_form_update_qdata(void* f, int flag);
+int         orc_form_get_qdata(void* f, int integral, double* out);
 #ifdef __cplusplus
 }
 #endif

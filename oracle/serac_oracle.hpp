@@ -719,6 +719,14 @@ inline dual<N> sqrt(const dual<N>& a)
   return r;
 }
 inline double value_of(double a) { return a; }
+// r.gradient = -q.gradient / df_dx: the derivative of the root x(q) of
+// f(x, q) = 0 when df/dq = 1 (fn/tuple_tensor_dual_functions.hpp:661-666)
+inline void set_root_dual(double&, double, double) {}
+template <int N>
+inline void set_root_dual(dual<N>& r, const dual<N>& q, double df_dx)
+{
+  for (int i = 0; i < N; i++) r.gradient[i] = -q.gradient[i] / df_dx;
+}
 template <int N>
 inline double value_of(const dual<N>& a)
 {
@@ -768,7 +776,7 @@ inline M3<T> inv3(const M3<T>& A)
 // grad[c][3]) and returns (source[c], flux[c][3]) in physical space, like the
 // integrands the reference calls at fn/domain_integral_kernels.hpp:106-150.
 // ---------------------------------------------------------------------------
-enum QFKind { QF_HEAT = 1, QF_MASS_DIFFUSION = 2, QF_NEOHOOKEAN = 3, QF_LINEAR_ELASTIC = 4 };
+enum QFKind { QF_HEAT = 1, QF_MASS_DIFFUSION = 2, QF_NEOHOOKEAN = 3, QF_LINEAR_ELASTIC = 4, QF_J2_LINEAR = 5 };
 
 struct QFParams {
   int    kind = QF_HEAT;
@@ -787,8 +795,150 @@ struct QFParams {
 //   flux = sigma . inv(I+H)^T . det(I+H), source = density * 0 (quasi-static).
 //                                                       p = {density, K, G}
 // LinearElastic: sigma = lambda tr(eps) I + 2 mu eps, flux = sigma.  p = {lambda, mu}
+// state size in doubles per quadrature point (0: stateless)
+inline int qf_state_size(int kind) { return kind == QF_J2_LINEAR ? 10 : 0; }
+
+// flux = stress . inv(F)^T . det(F), F = I + H with geometric nonlinearity,
+// else F = I (MaterialStressFunctor, phy/solid_mechanics.hpp:843-849)
 template <typename T>
-inline void qf_eval(const QFParams& P, int nc, const T* value, const T* grad, T* source, T* flux)
+inline void stress_to_flux(const M3<T>& sigma, const T* H, bool geom_nonlin, T* flux)
+{
+  if (!geom_nonlin) {
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) flux[i * 3 + j] = sigma.a[i][j];
+    return;
+  }
+  M3<T> F;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) F.a[i][j] = H[i * 3 + j];
+  for (int i = 0; i < 3; i++) F.a[i][i] = F.a[i][i] + 1.0;
+  M3<T> Fi = inv3(F);
+  T     dF = det3(F);
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) {
+      // (sigma . Fi^T)_ij = sum_k sigma_ik Fi_jk
+      T s = sigma.a[i][0] * Fi.a[j][0] + sigma.a[i][1] * Fi.a[j][1] + sigma.a[i][2] * Fi.a[j][2];
+      flux[i * 3 + j] = s * dF;
+    }
+}
+
+// solve_scalar_equation (fn/tuple_tensor_dual_functions.hpp:573-672) on the
+// J2 consistency residual f(x) = q - a x - (sigma_y + Hi (eqps_old + x)),
+// a = 3G + Hk: Newton safeguarded by bisection inside [lo, hi], on values.
+// df_dx is the gradient f(make_dual(x)) carries, -a - Hi.
+struct ScalarSolve {
+  double x = 0, df_dx = 0;
+  bool   converged  = false;
+  int    iterations = 0;
+};
+inline ScalarSolve j2_scalar_solve(double q, double a, double sy, double Hi, double e0, double x0, double lo,
+                                   double hi, double xtol, double rtol, int max_iter)
+{
+  auto         f    = [&](double x) { return q - a * x - (sy + Hi * (e0 + x)); };
+  const double dfdx = -a - Hi;
+  double       fl = f(lo), fh = f(hi);
+  if (fl * fh > 0) throw std::runtime_error("solve_scalar_equation: root not bracketed by input bounds");
+  ScalarSolve r;
+  double      x = 0.0;
+  if (fl == 0) {
+    x           = lo;
+    r.converged = true;
+  } else if (fh == 0) {
+    x           = hi;
+    r.converged = true;
+  }
+  double df = dfdx;
+  if (!r.converged) {
+    double xl = lo, xh = hi;
+    if (fl > 0) {
+      xl = hi;
+      xh = lo;
+    }
+    if (x0 < lo || x0 > hi) x0 = 0.5 * (lo + hi);
+    x              = x0;
+    double dx_old  = std::abs(hi - lo);
+    double dx      = dx_old;
+    double fval    = f(x);
+    while (!r.converged) {
+      if (r.iterations == max_iter) break;
+      if ((x - xh) * df - fval > 0 || (x - xl) * df - fval < 0 || std::abs(2. * fval) > std::abs(dx_old * df)) {
+        dx_old      = dx;
+        dx          = 0.5 * (xh - xl);
+        x           = xl + dx;
+        r.converged = (x == xl);
+      } else {
+        dx_old        = dx;
+        dx            = fval / df;
+        const double t = x;
+        x -= dx;
+        r.converged = (x == t);
+      }
+      fval        = f(x);
+      df          = dfdx;
+      r.converged = r.converged || (std::abs(dx) < xtol) || (std::abs(fval) < rtol);
+      if (fval < 0)
+        xl = x;
+      else
+        xh = x;
+      ++r.iterations;
+    }
+  }
+  r.x     = x;
+  r.df_dx = df;
+  return r;
+}
+
+// J2SmallStrain<LinearHardening> (phy/materials/solid_material.hpp:132-148,
+// 199-258).  H = du/dX (3x3 row-major); state = {plastic strain [3][3]
+// row-major, accumulated plastic strain}, updated in place when the return
+// map runs.  p = {E, nu, sigma_y, Hi, Hk, ...}.
+template <typename T>
+inline M3<T> j2_small_strain(const double* p, const T* H, double* state)
+{
+  const double   E = p[0], nu = p[1], sy = p[2], Hi = p[3], Hk = p[4];
+  constexpr double tol = 1e-10;
+  const double   K = E / (3.0 * (1.0 - 2.0 * nu));
+  const double   G = 0.5 * E / (1.0 + nu);
+  T              el[9];
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) el[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]) - state[i * 3 + j];
+  T tr_el = T(0.0);
+  for (int i = 0; i < 3; i++) tr_el = tr_el + el[i * 4];
+  const T pr = K * tr_el;
+  T       s[9];
+  for (int k = 0; k < 9; k++) s[k] = el[k];
+  for (int i = 0; i < 3; i++) s[i * 4] = s[i * 4] - tr_el / 3.0;
+  for (int k = 0; k < 9; k++) s[k] = (2.0 * G) * s[k];
+  T eta[9];
+  for (int k = 0; k < 9; k++) eta[k] = s[k] - ((2.0 / 3.0) * Hk) * state[k];
+  T sq = T(0.0);
+  for (int k = 0; k < 9; k++) sq = sq + eta[k] * eta[k];
+  using std::sqrt;
+  const T      q   = std::sqrt(1.5) * sqrt(sq);
+  const double e0  = state[9];
+  const double a   = 3.0 * G + Hk;
+  const double qv  = value_of(q);
+  if (qv - a * 0.0 - (sy + Hi * (e0 + 0.0)) > tol * sy) {
+    const double upper = (qv - (sy + Hi * e0)) / a;
+    ScalarSolve  r     = j2_scalar_solve(qv, a, sy, Hi, e0, 0.0, 0.0, upper, 0.0, tol * sy, 25);
+    // dual root: x' = -df / df_dx with df = q' (f(x, q) carries q's gradient)
+    T delta = T(r.x);
+    set_root_dual(delta, q, r.df_dx);
+    T Np[9];
+    for (int k = 0; k < 9; k++) Np[k] = (1.5 * eta[k]) / q;
+    for (int k = 0; k < 9; k++) s[k] = s[k] - ((2.0 * G) * delta) * Np[k];
+    state[9] += r.x;
+    for (int k = 0; k < 9; k++) state[k] += r.x * value_of(Np[k]);
+  }
+  M3<T> sigma;
+  for (int i = 0; i < 3; i++)
+    for (int j = 0; j < 3; j++) sigma.a[i][j] = (i == j) ? s[i * 3 + j] + pr : s[i * 3 + j];
+  return sigma;
+}
+
+template <typename T>
+inline void qf_eval(const QFParams& P, int nc, const T* value, const T* grad, T* source, T* flux,
+                    double* state = nullptr)
 {
   switch (P.kind) {
     case QF_HEAT: {
@@ -834,6 +984,15 @@ inline void qf_eval(const QFParams& P, int nc, const T* value, const T* grad, T*
       for (int c = 0; c < 3; c++) source[c] = T(0.0);
       return;
     }
+    case QF_J2_LINEAR: {
+      // MaterialStressFunctor<J2SmallStrain<LinearHardening>>, quasi-static;
+      // p = {E, nu, sigma_y, Hi, Hk, density, geom_nonlin}
+      if (!state) throw std::runtime_error("qf_eval: J2 needs quadrature-point state");
+      M3<T> sigma = j2_small_strain(P.p, grad, state);
+      stress_to_flux(sigma, grad, P.p[6] != 0.0, flux);
+      for (int c = 0; c < 3; c++) source[c] = T(0.0);
+      return;
+    }
     case QF_LINEAR_ELASTIC: {
       const double lam = P.p[0], mu = P.p[1];
       T            tr  = grad[0] + grad[4] + grad[8];
@@ -860,6 +1019,7 @@ inline void qf_blocks(int kind, bool blk[2][2])
     case QF_MASS_DIFFUSION: blk[0][0] = blk[1][1] = true; break;
     case QF_NEOHOOKEAN: blk[1][1] = true; break;
     case QF_LINEAR_ELASTIC: blk[1][1] = true; break;
+    case QF_J2_LINEAR: blk[1][1] = true; break;
   }
 }
 inline bool qf_has_source(int kind) { return kind == QF_HEAT || kind == QF_MASS_DIFFUSION; }

@@ -45,6 +45,10 @@ struct Integral {
   bool             blk[2][2];
   int              dsize = 0;  // doubles per quadrature point of stored derivative
   std::vector<double> D;       // [elem-in-domain][q][dsize]
+  // QuadratureData<State> (fn/quadrature_data.hpp:111-167): ssize doubles per
+  // quadrature point, [elem-in-domain][q][ssize]; empty when stateless
+  int                 ssize = 0;
+  std::vector<double> state;
 };
 
 // Functional<test(trial)> with test == trial space, one trial argument.
@@ -58,6 +62,7 @@ struct Form {
   Basis1D               basis;
   std::vector<double>   Xq, Jq;
   std::vector<Integral> integrals;
+  bool                  update_qdata = false;  // updateQdata(bool), fn/functional.hpp:523
 
   Form(const Mesh& m, const Space& s) : mesh(&m), space(s), Q(s.p + 1), basis(s.p + 1, s.p + 1)
   {
@@ -67,7 +72,8 @@ struct Form {
   int nq() const { return Q * Q * Q; }
   int nc() const { return space.nc; }
 
-  void add_integral(const QFParams& qf, const std::vector<int>& elems)
+  // state_init: qf_state_size(kind) doubles every quadrature point starts from
+  void add_integral(const QFParams& qf, const std::vector<int>& elems, const double* state_init = nullptr)
   {
     Integral I;
     I.qf    = qf;
@@ -78,6 +84,12 @@ struct Form {
     int c    = nc();
     I.dsize  = (I.blk[0][0] ? c * c : 0) + (I.blk[0][1] ? c * 3 * c : 0) + (I.blk[1][0] ? 3 * c * c : 0) +
               (I.blk[1][1] ? 9 * c * c : 0);
+    I.ssize = qf_state_size(qf.kind);
+    if (I.ssize) {
+      I.state.assign(I.elems.size() * size_t(nq()) * I.ssize, 0.0);
+      if (state_init)
+        for (size_t k = 0; k < I.state.size(); k++) I.state[k] = state_init[k % size_t(I.ssize)];
+    }
     integrals.push_back(std::move(I));
   }
 
@@ -98,6 +110,11 @@ struct Form {
     for (int q = 0; q < NQ; q++) {
       double J[3][3];
       jac(e, q, J);
+      // batch_apply_qf (fn/domain_integral_kernels.hpp:125-150): the qfunction
+      // works on a copy of the point's state, written back when update_qdata
+      double  st[16];
+      double* stp = I.ssize ? st : nullptr;
+      if (I.ssize) std::copy_n(&I.state[(slot * NQ + q) * I.ssize], I.ssize, st);
       if (!store_D) {
         double v[3], g[9], s[3], f[9];
         for (int i = 0; i < c; i++) {
@@ -105,7 +122,7 @@ struct Form {
           for (int d = 0; d < 3; d++) g[i * 3 + d] = grd[(size_t(i) * 3 + d) * NQ + q];
         }
         parent_to_physical(c, J, g);
-        qf_eval(I.qf, c, v, g, s, f);
+        qf_eval(I.qf, c, v, g, s, f, stp);
         physical_to_parent(c, J, s, f);
         for (int i = 0; i < c; i++) {
           src[size_t(i) * NQ + q] = s[i];
@@ -124,7 +141,7 @@ struct Form {
           }
         }
         parent_to_physical(c, J, g);
-        qf_eval(I.qf, c, v, g, s, f);
+        qf_eval(I.qf, c, v, g, s, f, stp);
         physical_to_parent(c, J, s, f);
         double* D = &I.D[(slot * NQ + q) * I.dsize];
         int     o = 0;
@@ -146,6 +163,7 @@ struct Form {
           for (int d = 0; d < 3; d++) flx[(size_t(i) * 3 + d) * NQ + q] = f[i * 3 + d].value;
         }
       }
+      if (I.ssize && update_qdata) std::copy_n(st, I.ssize, &I.state[(slot * NQ + q) * I.ssize]);
     }
     integrate(basis, c, src.data(), flx.data(), Re);
   }

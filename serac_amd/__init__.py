@@ -11,12 +11,12 @@ import ctypes
 import numpy as np
 
 from . import capi
-from .capi import (SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM, SX_QF_HEAT, SX_QF_LINEAR_ELASTIC,
+from .capi import (SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM, SX_QF_HEAT, SX_QF_J2_LINEAR, SX_QF_LINEAR_ELASTIC,
                    SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
 
 __all__ = [
-    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "SeracAmdError", "device_name",
-    "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC",
+    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "SeracAmdError", "device_name",
+    "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC", "SX_QF_J2_LINEAR",
     "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
 ]
 
@@ -209,6 +209,33 @@ class Gradient:
         return rep.as_dict()
 
 
+class QuadratureData:
+    """QuadratureData<State>: nelem x nq records of `size` doubles on the device."""
+
+    def __init__(self, nelem, nq, init):
+        init = np.ascontiguousarray(init, dtype=np.float64)
+        h = _vp()
+        capi.call("sx_qdata_create", nelem, nq, len(init), _dptr(init), ctypes.byref(h))
+        self.h = h
+        self.nelem, self.nq, self.size = nelem, nq, len(init)
+
+    def numpy(self):
+        """The state as [elem][q][size]."""
+        out = np.zeros((self.nelem, self.nq, self.size))
+        capi.call("sx_qdata_get", self.h, _dptr(out))
+        return out
+
+    def set(self, state):
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        assert st.size == self.nelem * self.nq * self.size
+        capi.call("sx_qdata_set", self.h, _dptr(st))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            capi.load().sx_qdata_destroy(self.h)
+            self.h = None
+
+
 class Functional:
     """serac::Functional<test(trials...)> mirror over the C-ABI."""
 
@@ -223,7 +250,8 @@ class Functional:
     def set_option(self, option, value):
         capi.call("sx_form_set_option", self.h, option, value)
 
-    def add_domain_integral(self, qfunc, params, depends_on=(0,), elems=None):
+    def add_domain_integral(self, qfunc, params, depends_on=(0,), elems=None, qdata=None):
+        """AddDomainIntegral; qdata (a QuadratureData) for qfunctions with state."""
         p = np.zeros(8)
         p[: len(params)] = params
         dep = (ctypes.c_int * len(depends_on))(*depends_on)
@@ -233,7 +261,14 @@ class Functional:
             e = np.ascontiguousarray(elems, dtype=np.int32)
             el, nel = e.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), len(e)
             self._keep = e
-        capi.call("sx_form_add_domain_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on), el, nel)
+        capi.call("sx_form_add_domain_integral_qdata", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on),
+                  el, nel, qdata.h if qdata is not None else None)
+        if qdata is not None:
+            self._qdata = getattr(self, "_qdata", []) + [qdata]  # the form borrows it
+
+    def update_qdata(self, flag):
+        """updateQdata(bool): while set, evaluations commit the quadrature-point state."""
+        capi.call("sx_form_update_qdata", self.h, int(bool(flag)))
 
     def __call__(self, t, *inputs, wrt=-1):
         """Residual into self.output (a DeviceVector reused across calls, like

@@ -32,6 +32,7 @@ SX_QF_HEAT = 1
 SX_QF_MASS_DIFFUSION = 2
 SX_QF_NEOHOOKEAN = 3
 SX_QF_LINEAR_ELASTIC = 4
+SX_QF_J2_LINEAR = 5
 
 _vp = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -93,6 +94,14 @@ SIGNATURES = {
     "sx_form_create": (_int, [_vp, ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]),
     "sx_form_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                            ctypes.POINTER(_int), _int]),
+    "sx_form_add_domain_integral_qdata": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
+                                                  ctypes.POINTER(_int), _int, _vp]),
+    "sx_form_update_qdata": (_int, [_vp, _int]),
+    "sx_qdata_create": (_int, [ctypes.c_int64, _int, _int, _D, ctypes.POINTER(_vp)]),
+    "sx_qdata_info": (_int, [_vp, _I64]),
+    "sx_qdata_get": (_int, [_vp, _D]),
+    "sx_qdata_set": (_int, [_vp, _D]),
+    "sx_qdata_destroy": (_int, [_vp]),
     "sx_form_eval": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _vp, _int, ctypes.POINTER(_vp)]),
     "sx_form_set_option": (_int, [_vp, _int, ctypes.c_int64]),
     "sx_form_synchronize": (_int, [_vp]),

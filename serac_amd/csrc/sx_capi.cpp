@@ -149,6 +149,15 @@ struct sx_space {
   int64_t          ndof() const { return nnodes * nc; }
 };
 
+// QuadratureData<State> (fn/quadrature_data.hpp:111-167): `size` doubles per
+// quadrature point of `nelem` elements; on the device as [elem][k][q] so that
+// the lanes of an element (one per point) read consecutive doubles.
+struct sx_qdata {
+  int64_t      nelem = 0;
+  int          nq = 0, size = 0;
+  DeviceBuffer d;
+};
+
 struct Integral {
   int                  qf = 0;
   double               params[SX_MAX_PARAMS] = {0};
@@ -161,6 +170,8 @@ struct Integral {
   int                  layout  = sxk::LAYOUT_GRAD_SYM;
   int                  tprec   = sxk::PREC_F64;  // precision the tangent was stored in
   bool                 has_tangent = false;
+  sx_qdata*            qdata = nullptr;  // borrowed; must outlive the form
+  DeviceBuffer         d_qrow;           // [slot] -> element position in the domain
 };
 
 struct sx_grad {
@@ -178,6 +189,7 @@ struct sx_form {
   int                                    scatter   = SX_SCATTER_COLORED;
   int                                    xcd_remap = 1;
   int                                    tprec     = sxk::PREC_F64;
+  int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -193,12 +205,20 @@ int qf_components(int qf)
   switch (qf) {
     case SX_QF_HEAT: return 1;
     case SX_QF_NEOHOOKEAN:
-    case SX_QF_LINEAR_ELASTIC: return 3;
+    case SX_QF_LINEAR_ELASTIC:
+    case SX_QF_J2_LINEAR: return 3;
     case SX_QF_MASS_DIFFUSION: return -1;  // scalar or vector
   }
   return 0;
 }
-int qf_layout(int qf) { return qf == SX_QF_MASS_DIFFUSION ? sxk::LAYOUT_FULL_SYM : sxk::LAYOUT_GRAD_SYM; }
+int qf_layout(int qf)
+{
+  if (qf == SX_QF_MASS_DIFFUSION) return sxk::LAYOUT_FULL_SYM;
+  if (qf == SX_QF_J2_LINEAR) return sxk::LAYOUT_GRAD_FULL;
+  return sxk::LAYOUT_GRAD_SYM;
+}
+// doubles of quadrature-point state the qfunction carries
+int qf_state_size(int qf) { return qf == SX_QF_J2_LINEAR ? 10 : 0; }
 
 sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
 {
@@ -220,6 +240,11 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   P.vprec    = sxk::PREC_F64;
   P.tables   = element_tables(s->order + 1);
   for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = I.params[k];
+  if (I.qdata) {
+    P.qdata        = I.qdata->d.as<double>();
+    P.qrow         = I.d_qrow.as<const int>();
+    P.update_state = f->update_qdata;
+  }
   P.stream = f->stream;
   return P;
 }
@@ -566,6 +591,12 @@ int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntri
 int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
                                 int ndep, const int* elems, int nelem)
 {
+  return sx_form_add_domain_integral_qdata(f, qfunc_id, params, nparams, depends_on, ndep, elems, nelem, nullptr);
+}
+
+int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                      const int* depends_on, int ndep, const int* elems, int nelem, sx_qdata* qdata)
+{
   return api([&] {
     SX_REQUIRE(f, SX_ERR_ARG, "null form");
     SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
@@ -582,6 +613,10 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
     SX_REQUIRE(ts->order == vs->order && ts->nc == vs->nc && ts->ordering == vs->ordering, SX_ERR_UNSUPPORTED,
                "test and trial spaces must have the same order, components and ordering");
     SX_REQUIRE(nc_qf < 0 || nc_qf == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
+    const int ssize = qf_state_size(qfunc_id);
+    SX_REQUIRE(ssize == 0 || qdata, SX_ERR_ARG, "this qfunction needs quadrature-point state (sx_qdata)");
+    SX_REQUIRE(ssize > 0 || !qdata, SX_ERR_ARG, "this qfunction carries no quadrature-point state");
+    const int nq = (vs->order + 1) * (vs->order + 1) * (vs->order + 1);
 
     auto I = std::make_unique<Integral>();
     I->qf  = qfunc_id;
@@ -598,22 +633,35 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
       domain.resize(size_t(m.ne));
       for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
     }
-    // slots: elements of the domain sorted by colour (stable in element
-    // order); with atomic scatter, one range in element order
-    std::vector<int> slots;
-    slots.reserve(domain.size());
+    if (qdata) {
+      SX_REQUIRE(qdata->nelem == int64_t(domain.size()) && qdata->nq == nq && qdata->size == ssize, SX_ERR_ARG,
+                 "quadrature data shape (" + std::to_string(qdata->nelem) + " x " + std::to_string(qdata->nq) +
+                     " x " + std::to_string(qdata->size) + ") does not match the domain (" +
+                     std::to_string(domain.size()) + " x " + std::to_string(nq) + " x " + std::to_string(ssize) +
+                     ")");
+    }
+    // slots: positions in the domain sorted by element colour (stable); with
+    // atomic scatter, one range in domain order
+    std::vector<int> pos;
+    pos.reserve(domain.size());
     if (f->scatter == SX_SCATTER_ATOMIC) {
-      slots = domain;
-      I->color_off = {0, int64_t(slots.size())};
+      for (size_t i = 0; i < domain.size(); i++) pos.push_back(int(i));
+      I->color_off = {0, int64_t(pos.size())};
     } else {
       I->color_off.assign(size_t(vs->ncolors) + 1, 0);
       for (int c = 0; c < vs->ncolors; c++) {
-        for (int e : domain)
-          if (vs->color[size_t(e)] == c) slots.push_back(e);
-        I->color_off[size_t(c) + 1] = int64_t(slots.size());
+        for (size_t i = 0; i < domain.size(); i++)
+          if (vs->color[size_t(domain[i])] == c) pos.push_back(int(i));
+        I->color_off[size_t(c) + 1] = int64_t(pos.size());
       }
     }
+    std::vector<int> slots(pos.size());
+    for (size_t k = 0; k < pos.size(); k++) slots[k] = domain[size_t(pos[k])];
     I->nslots = int64_t(slots.size());
+    if (qdata) {
+      I->qdata = qdata;
+      I->d_qrow.upload(pos);
+    }
     std::vector<int>    map(size_t(I->nslots) * vs->npe);
     std::vector<double> xyz(size_t(I->nslots) * 24);
     for (int64_t k = 0; k < I->nslots; k++) {
@@ -628,6 +676,80 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
     I->d_coords.upload(xyz);
     f->integrals.push_back(std::move(I));
   });
+}
+
+int sx_form_update_qdata(sx_form* f, int flag)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    f->update_qdata = flag ? 1 : 0;
+  });
+}
+
+int sx_qdata_create(int64_t nelem, int nq, int size, const double* init, sx_qdata** out)
+{
+  return api([&] {
+    SX_REQUIRE(out && nelem >= 0 && nq > 0 && size > 0, SX_ERR_ARG, "bad quadrature data shape");
+    SX_REQUIRE(init, SX_ERR_ARG, "null initial state");
+    auto q   = std::make_unique<sx_qdata>();
+    q->nelem = nelem;
+    q->nq    = nq;
+    q->size  = size;
+    std::vector<double> h(size_t(nelem) * size_t(size) * size_t(nq));
+    for (int64_t e = 0; e < nelem; e++)
+      for (int k = 0; k < size; k++)
+        std::fill_n(&h[(size_t(e) * size_t(size) + size_t(k)) * size_t(nq)], nq, init[k]);
+    q->d.upload(h);
+    *out = q.release();
+  });
+}
+
+int sx_qdata_info(const sx_qdata* q, int64_t* out)
+{
+  return api([&] {
+    SX_REQUIRE(q && out, SX_ERR_ARG, "null argument");
+    out[0] = q->nelem;
+    out[1] = q->nq;
+    out[2] = q->size;
+  });
+}
+
+int sx_qdata_get(const sx_qdata* q, double* host)
+{
+  return api([&] {
+    SX_REQUIRE(q && host, SX_ERR_ARG, "null argument");
+    const size_t        n = size_t(q->nelem) * size_t(q->size) * size_t(q->nq);
+    std::vector<double> h(n);
+    SX_HIP(hipDeviceSynchronize());
+    SX_HIP(hipMemcpy(h.data(), q->d.ptr, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (int64_t e = 0; e < q->nelem; e++)
+      for (int k = 0; k < q->size; k++)
+        for (int p = 0; p < q->nq; p++)
+          host[(size_t(e) * size_t(q->nq) + size_t(p)) * size_t(q->size) + size_t(k)] =
+              h[(size_t(e) * size_t(q->size) + size_t(k)) * size_t(q->nq) + size_t(p)];
+  });
+}
+
+int sx_qdata_set(sx_qdata* q, const double* host)
+{
+  return api([&] {
+    SX_REQUIRE(q && host, SX_ERR_ARG, "null argument");
+    const size_t        n = size_t(q->nelem) * size_t(q->size) * size_t(q->nq);
+    std::vector<double> h(n);
+    for (int64_t e = 0; e < q->nelem; e++)
+      for (int k = 0; k < q->size; k++)
+        for (int p = 0; p < q->nq; p++)
+          h[(size_t(e) * size_t(q->size) + size_t(k)) * size_t(q->nq) + size_t(p)] =
+              host[(size_t(e) * size_t(q->nq) + size_t(p)) * size_t(q->size) + size_t(k)];
+    SX_HIP(hipDeviceSynchronize());
+    SX_HIP(hipMemcpy(q->d.ptr, h.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  });
+}
+
+int sx_qdata_destroy(sx_qdata* q)
+{
+  delete q;
+  return SX_OK;
 }
 
 int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out)
@@ -999,6 +1121,8 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
       for (int64_t i = 0; i < ness; i++) start[size_t(i)] = xh[size_t(ess_dofs[i])];
     }
     prev = start;
+    bool has_qdata = false;
+    for (auto& I : f->integrals) has_qdata = has_qdata || I->qdata;
     for (int k = 1; k <= nsteps; k++) {
       const double t = double(k) / double(nsteps);
       for (int64_t i = 0; i < ness; i++) {
@@ -1031,6 +1155,14 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
       SX_HIP(hipStreamSynchronize(s));
       if (reports) reports[k - 1] = rep;
       if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "Newton did not converge in load step " + std::to_string(k));
+      // after equilibrium, evaluate the residual once more with the state
+      // buffers committed (phy/solid_mechanics.hpp:1278-1287)
+      if (has_qdata) {
+        const int saved = f->update_qdata;
+        f->update_qdata = 1;
+        residual_norm(f, wrt, x, r, ess, ness, nullptr);
+        f->update_qdata = saved;
+      }
       prev = target;
     }
   });

@@ -84,6 +84,22 @@ struct TangentLayout {
 // forward-mode pass seeded at input s produces entries s(s+1)/2 .. s(s+1)/2+s
 __host__ __device__ constexpr int packed_index(int r, int s) { return r <= s ? s * (s + 1) / 2 + r : r * (r + 1) / 2 + s; }
 
+// Shape of the stored tangent for a layout: M inputs/outputs per point
+// (VAL: the field value is an input and the source an output), DS stored
+// entries, and where entry D(r, s) = d out_r / d in_s lives.  Column s of a
+// forward-mode pass holds entries idx(r, s) for r < rows(s), in increasing
+// storage order.
+template <int LAYOUT, int NC>
+struct TanShape {
+  static constexpr bool VAL = (LAYOUT == LAYOUT_FULL_SYM);
+  static constexpr bool SYM = (LAYOUT != LAYOUT_GRAD_FULL);
+  static constexpr int  M   = VAL ? 4 * NC : 3 * NC;
+  static constexpr int  DS  = SYM ? M * (M + 1) / 2 : M * M;
+  static constexpr int  VO  = VAL ? NC : 0;  // offset of the gradient inputs
+  __host__ __device__ static constexpr int idx(int r, int s) { return SYM ? packed_index(r, s) : s * M + r; }
+  __host__ __device__ static constexpr int rows(int s) { return SYM ? s + 1 : M; }
+};
+
 // Workgroups are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).
 // With remap on, XCD x instead works through a contiguous run of logical
 // blocks, so neighbouring elements (which share nodes) hit the same L2.
@@ -151,7 +167,7 @@ __device__ __forceinline__ void trilinear_jacobian(const double* X, double x0, d
 // flux_parent[c][j] = detJ sum_k flux[c][k] inv(J)[j][k].
 template <class QF, int NC>
 __device__ __forceinline__ void qpoint_eval(const double* prm, const double* Ji, double dJ, const double* val,
-                                            const double* gp, double* src, double* fp)
+                                            const double* gp, double* src, double* fp, double* st)
 {
   double g[3 * NC], s[NC], f[3 * NC];
 #pragma unroll
@@ -161,7 +177,10 @@ __device__ __forceinline__ void qpoint_eval(const double* prm, const double* Ji,
       g[c * 3 + k] = gp[c * 3 + 0] * Ji[0 * 3 + k] + gp[c * 3 + 1] * Ji[1 * 3 + k] + gp[c * 3 + 2] * Ji[2 * 3 + k];
 #pragma unroll
   for (int c = 0; c < NC; c++) s[c] = 0.0;
-  QF::eval(prm, val, g, s, f);
+  if constexpr (QF::STATE > 0)
+    QF::eval(prm, val, g, s, f, st);
+  else
+    QF::eval(prm, val, g, s, f);
 #pragma unroll
   for (int c = 0; c < NC; c++) {
     src[c] = s[c] * dJ;
@@ -176,7 +195,8 @@ __device__ __forceinline__ void qpoint_eval(const double* prm, const double* Ji,
 // derivative of [source(NC)] [parent flux (NC x 3)] (or only the flux).
 template <class QF, int NC, bool FULL>
 __device__ __forceinline__ void qpoint_tangent_column(const double* prm, const double* Ji, double dJ,
-                                                      const double* val, const double* gp, int s, double* col)
+                                                      const double* val, const double* gp, const double* st0,
+                                                      int s, double* col)
 {
   constexpr int VOFF = FULL ? NC : 0;
   d1            v[NC], g[3 * NC], sr[NC], f[3 * NC];
@@ -196,7 +216,15 @@ __device__ __forceinline__ void qpoint_tangent_column(const double* prm, const d
     }
 #pragma unroll
   for (int c = 0; c < NC; c++) sr[c] = d1(0.0);
-  QF::eval(prm, v, g, sr, f);
+  if constexpr (QF::STATE > 0) {
+    // every pass starts from the point's state before this evaluation
+    double st[QF::STATE];
+#pragma unroll
+    for (int k = 0; k < QF::STATE; k++) st[k] = st0[k];
+    QF::eval(prm, v, g, sr, f, st);
+  } else {
+    QF::eval(prm, v, g, sr, f);
+  }
   if constexpr (FULL) {
 #pragma unroll
     for (int c = 0; c < NC; c++) col[c] = sr[c].d * dJ;
@@ -286,10 +314,12 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
 {
   constexpr int  NQ      = N * N * N;
   constexpr int  EB      = 256 / NQ;
-  constexpr bool FULL    = (LAYOUT == LAYOUT_FULL_SYM);
+  using Shape            = TanShape<LAYOUT, NC>;
+  constexpr bool FULL    = Shape::VAL;
   constexpr bool HAS_SRC = QF::HAS_SOURCE;
-  constexpr int  M       = FULL ? 4 * NC : 3 * NC;
-  constexpr int  DS      = M * (M + 1) / 2;
+  constexpr int  M       = Shape::M;
+  constexpr int  DS      = Shape::DS;
+  constexpr int  NST     = QF::STATE;
   constexpr int  R       = NC * NQ;  // one LDS region
   constexpr int  PER     = (6 + (HAS_SRC ? 1 : 0)) * R;
   constexpr int  TAB     = 4 * N * N + N;
@@ -317,6 +347,17 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   const double* tBw = stab + 2 * N * N;
   const double* tGw = stab + 3 * N * N;
   const double* txq = stab + 4 * N * N;
+
+  // ---- quadrature-point state (issued first, consumed at the qfunction)
+  double  st[NST > 0 ? NST : 1];
+  double* stp = nullptr;
+  if constexpr (NST > 0) {
+    if (active) {
+      stp = P.qdata + size_t(P.qrow[slot]) * NST * NQ + li;
+#pragma unroll
+      for (int k = 0; k < NST; k++) st[k] = stp[k * NQ];
+    }
+  }
 
   // ---- gather
   int64_t node = 0;
@@ -396,17 +437,27 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
     inv3(J, Ji);
     const double dJ = det3(J);
     double       src[NC], fp[3 * NC];
-    qpoint_eval<QF, NC>(P.params, Ji, dJ, val, gp, src, fp);
     if constexpr (MODE == MODE_LINEARIZE) {
-      // columns s = 0..M-1 give entries (r <= s, s) in increasing packed order
+      // columns s = 0..M-1 give entries idx(r, s), r < rows(s), in increasing
+      // storage order; every column starts from the state before this call
       void*                          Dp = static_cast<char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S);
       typename StoreTraits<S>::raw pend[Lay::VEC];
 #pragma unroll
       for (int s = 0; s < M; s++) {
         double col[M];
-        qpoint_tangent_column<QF, NC, FULL>(P.params, Ji, dJ, val, gp, s, col);
+        qpoint_tangent_column<QF, NC, FULL>(P.params, Ji, dJ, val, gp, st, s, col);
 #pragma unroll
-        for (int rr = 0; rr <= s; rr++) store_tangent_entry<S, DS, NQ>(Dp, li, packed_index(rr, s), col[rr], pend);
+        for (int rr = 0; rr < Shape::rows(s); rr++)
+          store_tangent_entry<S, DS, NQ>(Dp, li, Shape::idx(rr, s), col[rr], pend);
+      }
+    }
+    qpoint_eval<QF, NC>(P.params, Ji, dJ, val, gp, src, fp, st);
+    if constexpr (NST > 0) {
+      // batch_apply_qf (fn/domain_integral_kernels.hpp:125-150): commit the
+      // point's updated state only when asked to
+      if (P.update_state) {
+#pragma unroll
+        for (int k = 0; k < NST; k++) stp[k * NQ] = st[k];
       }
     }
     // flux -> regions 0..2 (u and stage-x partials are dead), source -> region 6
@@ -491,9 +542,10 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
 {
   constexpr int  NQ   = N * N * N;
   constexpr int  EB   = 256 / NQ;
-  constexpr bool FULL = (LAYOUT == LAYOUT_FULL_SYM);
-  constexpr int  M    = FULL ? 4 * NC : 3 * NC;
-  constexpr int  DS   = M * (M + 1) / 2;
+  using Shape         = TanShape<LAYOUT, NC>;
+  constexpr bool FULL = Shape::VAL;
+  constexpr int  M    = Shape::M;
+  constexpr int  DS   = Shape::DS;
   constexpr int  R    = NC * NQ;
   constexpr int  PER  = (6 + (FULL ? 1 : 0)) * R;
   constexpr int  TAB  = 4 * N * N;
@@ -603,10 +655,10 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
 #pragma unroll
     for (int s = 0; s < M; s++)
 #pragma unroll
-      for (int rr = 0; rr <= s; rr++) {
-        const T Dk = Dr[packed_index(rr, s)];
+      for (int rr = 0; rr < Shape::rows(s); rr++) {
+        const T Dk = Dr[Shape::idx(rr, s)];
         out[rr] += Dk * in[s];
-        if (rr != s) out[s] += Dk * in[rr];
+        if (Shape::SYM && rr != s) out[s] += Dk * in[rr];
       }
 #pragma unroll
     for (int c = 0; c < NC; c++) {
@@ -686,10 +738,10 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
 {
   constexpr int  NQ   = N * N * N;
   constexpr int  EB   = 256 / NQ;
-  constexpr bool FULL = (LAYOUT == LAYOUT_FULL_SYM);
-  constexpr int  M    = FULL ? 4 * NC : 3 * NC;
-  constexpr int  DS   = M * (M + 1) / 2;
-  constexpr int  VO   = FULL ? NC : 0;
+  using Shape         = TanShape<LAYOUT, NC>;
+  constexpr bool FULL = Shape::VAL;
+  constexpr int  DS   = Shape::DS;
+  constexpr int  VO   = Shape::VO;
   constexpr int  TAB  = 4 * N * N + N;
   using Lay           = TangentLayout<S, DS, NQ>;
   __shared__ double stab[TAB];
@@ -735,12 +787,12 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
 #pragma unroll
       for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int j = 0; j < 3; j++) a += tg[i] * Dat(packed_index(VO + c * 3 + i, VO + c * 3 + j), q) * ug[j];
+        for (int j = 0; j < 3; j++) a += tg[i] * Dat(Shape::idx(VO + c * 3 + i, VO + c * 3 + j), q) * ug[j];
       if constexpr (FULL) {
-        a += tv * Dat(packed_index(c, c), q) * uv;
+        a += tv * Dat(Shape::idx(c, c), q) * uv;
 #pragma unroll
         for (int j = 0; j < 3; j++) {
-          const double d = Dat(packed_index(c, VO + c * 3 + j), q);
+          const double d = Dat(Shape::idx(c, VO + c * 3 + j), q);
           a += tv * d * ug[j] + tg[j] * d * uv;
         }
       }
@@ -815,12 +867,16 @@ inline int dispatch_eval(const ElemLaunch& P, hipError_t* err)
 template <int N>
 inline int dispatch_order(const ElemLaunch& P, hipError_t* err)
 {
-  const bool full = (P.layout == LAYOUT_FULL_SYM);
   if (P.mode == MODE_JVP || P.mode == MODE_DIAG) {
-    if (P.nc == 1) return full ? dispatch_gradient_prec<N, 1, LAYOUT_FULL_SYM>(P, err)
-                               : dispatch_gradient_prec<N, 1, LAYOUT_GRAD_SYM>(P, err);
-    if (P.nc == 3) return full ? dispatch_gradient_prec<N, 3, LAYOUT_FULL_SYM>(P, err)
-                               : dispatch_gradient_prec<N, 3, LAYOUT_GRAD_SYM>(P, err);
+    if (P.nc == 1) {
+      if (P.layout == LAYOUT_GRAD_SYM) return dispatch_gradient_prec<N, 1, LAYOUT_GRAD_SYM>(P, err);
+      if (P.layout == LAYOUT_FULL_SYM) return dispatch_gradient_prec<N, 1, LAYOUT_FULL_SYM>(P, err);
+    }
+    if (P.nc == 3) {
+      if (P.layout == LAYOUT_GRAD_SYM) return dispatch_gradient_prec<N, 3, LAYOUT_GRAD_SYM>(P, err);
+      if (P.layout == LAYOUT_FULL_SYM) return dispatch_gradient_prec<N, 3, LAYOUT_FULL_SYM>(P, err);
+      if (P.layout == LAYOUT_GRAD_FULL) return dispatch_gradient_prec<N, 3, LAYOUT_GRAD_FULL>(P, err);
+    }
     return 0;
   }
   switch (P.qf) {
@@ -836,6 +892,9 @@ inline int dispatch_order(const ElemLaunch& P, hipError_t* err)
       break;
     case QF_ID_LINEAR_ELASTIC:
       if (P.nc == 3) return dispatch_eval<N, 3, QFLinearElastic, LAYOUT_GRAD_SYM>(P, err);
+      break;
+    case QF_ID_J2:
+      if (P.nc == 3) return dispatch_eval<N, 3, QFJ2, LAYOUT_GRAD_FULL>(P, err);
       break;
   }
   return 0;

@@ -9,15 +9,18 @@ namespace sxk {
 
 enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3 };
 
-// Stored-tangent layouts (symmetric, upper triangle packed):
-//   GRAD_SYM: d(parent flux)/d(parent grad), M = 3 nc   (heat: 6, Neo-Hookean: 45)
-//   FULL_SYM: d[source, parent flux]/d[value, parent grad], M = 4 nc
-enum Layout { LAYOUT_GRAD_SYM = 0, LAYOUT_FULL_SYM = 1 };
+// Stored-tangent layouts:
+//   GRAD_SYM:  d(parent flux)/d(parent grad), M = 3 nc, symmetric, upper
+//              triangle packed (heat: 6 entries, Neo-Hookean: 45)
+//   FULL_SYM:  d[source, parent flux]/d[value, parent grad], M = 4 nc, symmetric
+//   GRAD_FULL: d(parent flux)/d(parent grad), M = 3 nc, all M^2 entries
+//              (J2 plasticity: 81)
+enum Layout { LAYOUT_GRAD_SYM = 0, LAYOUT_FULL_SYM = 1, LAYOUT_GRAD_FULL = 2 };
 
 // precisions of the stored tangent (tprec) and of J.v vectors (vprec)
 enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
 
-enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_ID_LINEAR_ELASTIC = 4 };
+enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_ID_LINEAR_ELASTIC = 4, QF_ID_J2 = 5 };
 
 struct ElemLaunch {
   int           order, nc, qf, mode, layout, ordering;
@@ -33,6 +36,9 @@ struct ElemLaunch {
   void*         r;       // output L-vector (test), accumulated into
   void*         D;       // stored tangent, per-slot layout of sx_element.hpp TangentLayout
   const double* tables;  // B, G, Bw, Gw [Q][N] and GL points [Q]
+  double*       qdata;         // quadrature-point state [row][k][q] (residual/linearize)
+  const int*    qrow;          // [slot] -> state row (the element's position in its domain)
+  int           update_state;  // write the updated state back
   double        params[8];
   double        t;
   hipStream_t   stream;
@@ -42,7 +48,7 @@ inline int tangent_m(int nc, int layout) { return layout == LAYOUT_FULL_SYM ? 4 
 inline int tangent_ds(int nc, int layout)
 {
   int m = tangent_m(nc, layout);
-  return m * (m + 1) / 2;
+  return layout == LAYOUT_GRAD_FULL ? m * m : m * (m + 1) / 2;
 }
 inline int precision_bytes(int prec) { return prec == PREC_F64 ? 8 : prec == PREC_F32 ? 4 : 2; }
 // bytes of tangent per slot: DS * NQ entries rounded up to 16 bytes

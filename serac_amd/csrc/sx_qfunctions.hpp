@@ -37,6 +37,11 @@ __device__ __forceinline__ d1 operator/(d1 a, d1 b)
 __device__ __forceinline__ d1 operator/(d1 a, double b) { return d1(a.v / b, a.d / b); }
 __device__ __forceinline__ d1 operator/(double a, d1 b) { return d1(a / b.v, -(a / (b.v * b.v)) * b.d); }
 __device__ __forceinline__ d1 dlog1p(d1 a) { return d1(::log1p(a.v), a.d / (1.0 + a.v)); }
+__device__ __forceinline__ d1 dsqrt(d1 a) { return d1(::sqrt(a.v), a.d / (2.0 * ::sqrt(a.v))); }
+__device__ __forceinline__ double dsqrt(double a) { return ::sqrt(a); }
+// the derivative of a root x(q) of f(x, q) = 0 with df/dq = 1: x' = -q' / df_dx
+__device__ __forceinline__ double root_dual(double x, double, double) { return x; }
+__device__ __forceinline__ d1 root_dual(double x, d1 q, double df_dx) { return d1(x, -q.d / df_dx); }
 __device__ __forceinline__ double dlog1p(double a) { return ::log1p(a); }
 __device__ __forceinline__ double vpart(double a) { return a; }
 __device__ __forceinline__ double vpart(d1 a) { return a.v; }
@@ -91,6 +96,7 @@ struct QFHeat {
   static constexpr int  NC         = 1;
   static constexpr bool HAS_SOURCE = true;
   static constexpr bool USES_VALUE = false;
+  static constexpr int  STATE      = 0;
   template <typename T>
   __device__ __forceinline__ static void eval(const double* p, const T*, const T* g, T* src, T* flux)
   {
@@ -107,6 +113,7 @@ struct QFMassDiffusion {
   static constexpr int  NC         = C;
   static constexpr bool HAS_SOURCE = true;
   static constexpr bool USES_VALUE = true;
+  static constexpr int  STATE      = 0;
   template <typename T>
   __device__ __forceinline__ static void eval(const double* p, const T* v, const T* g, T* src, T* flux)
   {
@@ -128,6 +135,7 @@ struct QFNeoHookean {
   static constexpr int  NC         = 3;
   static constexpr bool HAS_SOURCE = false;
   static constexpr bool USES_VALUE = false;
+  static constexpr int  STATE      = 0;
   template <typename T>
   __device__ __forceinline__ static void eval(const double* p, const T*, const T* H, T*, T* flux)
   {
@@ -170,6 +178,7 @@ struct QFLinearElastic {
   static constexpr int  NC         = 3;
   static constexpr bool HAS_SOURCE = false;
   static constexpr bool USES_VALUE = false;
+  static constexpr int  STATE      = 0;
   template <typename T>
   __device__ __forceinline__ static void eval(const double* p, const T*, const T* g, T*, T* flux)
   {
@@ -186,12 +195,148 @@ struct QFLinearElastic {
   }
 };
 
+// flux = sigma . inv(F)^T . det(F), F = I + H (MaterialStressFunctor with
+// geometric nonlinearity, phy/solid_mechanics.hpp:843-849)
+template <typename T>
+__device__ __forceinline__ void stress_to_flux_large(const T* sig, const T* H, T* flux)
+{
+  T F[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) F[k] = H[k];
+  F[0] = F[0] + 1.0;
+  F[4] = F[4] + 1.0;
+  F[8] = F[8] + 1.0;
+  T Fi[9];
+  inv3(F, Fi);
+  T dF = det3(F);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      flux[i * 3 + j] =
+          (sig[i * 3 + 0] * Fi[j * 3 + 0] + sig[i * 3 + 1] * Fi[j * 3 + 1] + sig[i * 3 + 2] * Fi[j * 3 + 2]) * dF;
+}
+
+// MaterialStressFunctor<J2SmallStrain<LinearHardening>>, quasi-static
+// (phy/materials/solid_material.hpp:132-148, 199-258; phy/solid_mechanics.hpp:834-852).
+// p = {E, nu, sigma_y, Hi, Hk, density, geom_nonlin}.  st = {plastic strain
+// [3][3] row-major, accumulated plastic strain}: the caller's copy of the
+// point's state, updated in place when the return map runs.
+//
+// The return map solves the consistency residual
+//   f(x) = q - (3G + Hk) x - (sigma_y + Hi (eqps_old + x))
+// with the bracketed Newton/bisection of solve_scalar_equation
+// (fn/tuple_tensor_dual_functions.hpp:573-672) on values; the dual part of
+// the root is -q' / df_dx.  A root that is not bracketed (the reference
+// aborts there) returns NaN stresses.
+struct QFJ2 {
+  static constexpr int    NC         = 3;
+  static constexpr bool   HAS_SOURCE = false;
+  static constexpr bool   USES_VALUE = false;
+  static constexpr int    STATE      = 10;
+  static constexpr double TOL        = 1e-10;
+
+  __device__ __forceinline__ static double solve(double q, double a, double sy, double Hi, double e0, double hi,
+                                                 double* df_out)
+  {
+    const double dfdx = -a - Hi;
+    const double lo   = 0.0;
+    const double rtol = TOL * sy;
+    auto         f    = [&](double x) { return q - a * x - (sy + Hi * (e0 + x)); };
+    const double fl = f(lo), fh = f(hi);
+    *df_out         = dfdx;
+    if (fl * fh > 0) return __builtin_nan("");
+    if (fl == 0) return lo;
+    if (fh == 0) return hi;
+    double xl = lo, xh = hi;
+    if (fl > 0) {
+      xl = hi;
+      xh = lo;
+    }
+    double x = 0.0;  // initial guess, inside [lo, hi]
+    double dx_old = fabs(hi - lo), dx = dx_old, fval = f(x);
+    bool   conv = false;
+    for (int it = 0; it < 25 && !conv; it++) {
+      if ((x - xh) * dfdx - fval > 0 || (x - xl) * dfdx - fval < 0 || fabs(2. * fval) > fabs(dx_old * dfdx)) {
+        dx_old = dx;
+        dx     = 0.5 * (xh - xl);
+        x      = xl + dx;
+        conv   = (x == xl);
+      } else {
+        dx_old         = dx;
+        dx             = fval / dfdx;
+        const double t = x;
+        x -= dx;
+        conv = (x == t);
+      }
+      fval = f(x);
+      conv = conv || (fabs(fval) < rtol);
+      if (fval < 0)
+        xl = x;
+      else
+        xh = x;
+    }
+    return x;
+  }
+
+  template <typename T>
+  __device__ __forceinline__ static void eval(const double* p, const T*, const T* H, T*, T* flux, double* st)
+  {
+    const double E = p[0], nu = p[1], sy = p[2], Hi = p[3], Hk = p[4];
+    const double K = E / (3.0 * (1.0 - 2.0 * nu));
+    const double G = 0.5 * E / (1.0 + nu);
+    T            s[9];
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int j = 0; j < 3; j++) s[i * 3 + j] = 0.5 * (H[i * 3 + j] + H[j * 3 + i]) - st[i * 3 + j];
+    const T tr = s[0] + s[4] + s[8];
+    const T pr = K * tr;
+#pragma unroll
+    for (int i = 0; i < 3; i++) s[i * 4] = s[i * 4] - tr / 3.0;
+#pragma unroll
+    for (int k = 0; k < 9; k++) s[k] = (2.0 * G) * s[k];
+    T eta[9];
+    T sq = T(0.0);
+#pragma unroll
+    for (int k = 0; k < 9; k++) {
+      eta[k] = s[k] - ((2.0 / 3.0) * Hk) * st[k];
+      sq     = sq + eta[k] * eta[k];
+    }
+    const T      q  = 1.2247448713915890491 * dsqrt(sq);  // sqrt(1.5)
+    const double qv = vpart(q), e0 = st[9], a = 3.0 * G + Hk;
+    if (qv - a * 0.0 - (sy + Hi * (e0 + 0.0)) > TOL * sy) {
+      double       df_dx;
+      const double x     = solve(qv, a, sy, Hi, e0, (qv - (sy + Hi * e0)) / a, &df_dx);
+      const T      delta = root_dual(x, q, df_dx);
+      const T      c     = (2.0 * G) * delta;
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const T Np = (1.5 * eta[k]) / q;
+        s[k]       = s[k] - c * Np;
+        st[k] += x * vpart(Np);
+      }
+      st[9] += x;
+    }
+    s[0] = s[0] + pr;
+    s[4] = s[4] + pr;
+    s[8] = s[8] + pr;
+    if (p[6] != 0.0) {
+      stress_to_flux_large(s, H, flux);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; k++) flux[k] = s[k];
+    }
+  }
+};
+
 // placeholder for kernels that do not call a qfunction (J.v, diagonal)
 template <int C>
 struct QFNone {
   static constexpr int  NC         = C;
   static constexpr bool HAS_SOURCE = false;
   static constexpr bool USES_VALUE = false;
+  static constexpr int  STATE      = 0;
   template <typename T>
   __device__ __forceinline__ static void eval(const double*, const T*, const T*, T*, T*)
   {

@@ -7,10 +7,14 @@
 //   - solid_statics_patch.cpp:30-100: the AffineSolution interior residual
 //   - heat integrands (ThermalMaterialIntegrand + ConstantSource)
 //   - EquationSolver Newton + Jacobi-CG on a 1% compression step
+//   - J2 plasticity with QuadratureData: residual, J.v and the state
+//     updateQdata commits, against the oracle (1e-12)
 //   - error paths: unsupported configurations raise serac_amd::Error
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstring>
+#include <memory>
 #include <vector>
 
 #include "../../include/serac_amd/functional.hpp"
@@ -193,6 +197,49 @@ int main()
     for (int32_t d : ess) Rx[size_t(d)] = 0.0;
     CHECK(norm2(Rx) <= 1e-9 * rep.initial_norm, "converged residual %.2e (initial %.2e)", norm2(Rx),
           rep.initial_norm);
+  }
+
+  // ---- J2 plasticity with QuadratureData and updateQdata
+  {
+    using Mat   = solid_mechanics::J2SmallStrain<solid_mechanics::LinearHardening>;
+    using State = Mat::State;
+    Mat  j2{200.0, 0.3, {0.5, 2.0}, 0.0, 1.0};
+    auto qdata  = std::make_shared<QuadratureData<State>>(mesh.GetNE(), (p + 1) * (p + 1) * (p + 1));
+    Functional<space(space)> plastic(&fes, {&fes});
+    plastic.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                              solid_mechanics::MaterialStressFunctor<Mat>(j2, GeometricNonlinearities::On), mesh,
+                              qdata);
+    const std::vector<double> xyz = fes.NodeCoordinates();
+    std::vector<double>       Us(static_cast<size_t>(ndof));
+    for (int64_t i = 0; i < ndof / 3; i++) {
+      Us[size_t(3 * i)]     = 0.01 * xyz[size_t(3 * i + 1)];  // shear beyond yield
+      Us[size_t(3 * i + 1)] = 0.002 * xyz[size_t(3 * i)] * xyz[size_t(3 * i + 2)];
+    }
+    void*        oj     = orc_form(om, os);
+    const double jp[8]  = {200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0, 0};
+    const double st0[10] = {0};
+    orc_form_add_integral_state(oj, 5, jp, nullptr, 0, st0);
+    std::vector<double> Rj(static_cast<size_t>(ndof)), Jj(static_cast<size_t>(ndof));
+    orc_form_eval(oj, Us.data(), Rj.data(), 1);
+    orc_form_jvp(oj, dU.data(), Jj.data());
+    Vector us(Us);
+    auto [rj, dRj] = plastic(0.0, differentiate_wrt(us));
+    CHECK(relerr(rj.ToHost(), Rj) < 1e-12, "J2 residual vs oracle: %.2e", relerr(rj.ToHost(), Rj));
+    std::vector<double> jvj = dRj(du).ToHost();
+    CHECK(relerr(jvj, Jj) < 1e-12, "J2 J.v vs oracle: %.2e", relerr(jvj, Jj));
+    CHECK(qdata->Get()[0].accumulated_plastic_strain == 0.0, "state untouched without updateQdata");
+    plastic.updateQdata(true);
+    plastic(0.0, us);
+    plastic.updateQdata(false);
+    orc_form_update_qdata(oj, 1);
+    orc_form_eval(oj, Us.data(), Rj.data(), 0);
+    std::vector<State>  st = qdata->Get();
+    std::vector<double> st_ref(st.size() * 10), st_gpu(st.size() * 10);
+    orc_form_get_qdata(oj, 0, st_ref.data());
+    std::memcpy(st_gpu.data(), st.data(), st_gpu.size() * sizeof(double));
+    CHECK(maxabs(st_ref) > 0 && relerr(st_gpu, st_ref) < 1e-12, "committed state vs oracle: %.2e",
+          relerr(st_gpu, st_ref));
+    orc_form_free(oj);
   }
 
   // ---- error paths
