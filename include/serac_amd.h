@@ -27,6 +27,7 @@ typedef struct sx_space sx_space;
 typedef struct sx_form  sx_form;
 typedef struct sx_grad  sx_grad;
 typedef struct sx_qdata sx_qdata;
+typedef struct sx_csr   sx_csr;
 
 /* status codes */
 enum {
@@ -171,6 +172,22 @@ int sx_grad_apply_f32(sx_grad* g, const float* dx, float* df);
 int sx_grad_diagonal(sx_grad* g, double* diag);
 /* bytes of stored linearization (tangent) held by this gradient */
 int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes);
+/* Gradient::assemble (fn/functional.hpp:569-667 with element_gradient_kernel,
+ * fn/domain_integral_kernels.hpp:327-361): the linearization as a sparse
+ * matrix, rows = test DOFs, columns = trial DOFs, in CSR form on the device
+ * (int64 row offsets, int32 columns sorted within each row, FP64 values).
+ * The pattern couples every pair of DOFs that share an element.  With
+ * coloured scatter the values are bitwise reproducible.  The caller owns *out
+ * (sx_csr_destroy).  Synchronous. */
+int sx_grad_assemble(sx_grad* g, sx_csr** out);
+int sx_csr_info(const sx_csr* A, int64_t* nrows, int64_t* ncols, int64_t* nnz);
+/* device pointers into the matrix (valid until sx_csr_destroy) */
+int sx_csr_arrays(const sx_csr* A, const int64_t** row_ptr, const int32_t** col, const double** val);
+/* host copies: row_ptr[nrows + 1], col[nnz], val[nnz]; any may be NULL */
+int sx_csr_download(const sx_csr* A, int64_t* row_ptr, int32_t* col, double* val);
+/* y = A x (device vectors; one wavefront per row, reproducible). Synchronous. */
+int sx_csr_spmv(const sx_csr* A, const double* x, double* y);
+int sx_csr_destroy(sx_csr* A);
 /* Runs nrep applications back to back on the form's stream, bracketed by HIP
  * events; writes the elapsed milliseconds of the whole batch and of the
  * element kernels alone (sum over launches). vec_precision: SX_PREC_F64

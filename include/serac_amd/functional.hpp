@@ -397,6 +397,47 @@ struct qfunction_traits<
   }
 };
 
+/// Assembled sparse matrix (device CSR: int64 row offsets, int32 columns
+/// sorted per row, FP64 values), as Gradient::assemble returns it.
+class SparseMatrix {
+ public:
+  explicit SparseMatrix(sx_csr* h) : h_(h)
+  {
+    int rc = sx_csr_info(h_, &nrows_, &ncols_, &nnz_);
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+  }
+  SparseMatrix(const SparseMatrix&)            = delete;
+  SparseMatrix& operator=(const SparseMatrix&) = delete;
+  ~SparseMatrix()
+  {
+    if (h_) sx_csr_destroy(h_);
+  }
+  int64_t Height() const { return nrows_; }
+  int64_t Width() const { return ncols_; }
+  int64_t NumNonZeroElems() const { return nnz_; }
+  /// y = A x on the device
+  void Mult(const Vector& x, Vector& y) const
+  {
+    if (y.Size() != nrows_) y.SetSize(nrows_);
+    int rc = sx_csr_spmv(h_, x.Data(), y.Data());
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+  }
+  /// host copies of the CSR arrays
+  void ToHost(std::vector<int64_t>& row_ptr, std::vector<int32_t>& col, std::vector<double>& val) const
+  {
+    row_ptr.resize(size_t(nrows_) + 1);
+    col.resize(size_t(nnz_));
+    val.resize(size_t(nnz_));
+    int rc = sx_csr_download(h_, row_ptr.data(), col.data(), val.data());
+    if (rc != SX_OK) throw Error(rc, sx_last_error());
+  }
+  sx_csr* handle() const { return h_; }
+
+ private:
+  sx_csr* h_ = nullptr;
+  int64_t nrows_ = 0, ncols_ = 0, nnz_ = 0;
+};
+
 /// QuadratureData<State> (fn/quadrature_data.hpp:111-167): one State per
 /// quadrature point of every element of a domain, kept on the device.  State
 /// must be a struct of doubles.
@@ -509,6 +550,14 @@ class Functional<test(trials...)> {
       detail::check(sx_grad_diagonal(h_, d.Data()));
       detail::check(sx_form_synchronize(form_));
       return d;
+    }
+    /// Gradient::assemble (fn/functional.hpp:569-667): the linearization as
+    /// a device CSR matrix (the HypreParMatrix of the reference on one rank)
+    std::unique_ptr<SparseMatrix> assemble() const
+    {
+      sx_csr* A = nullptr;
+      detail::check(sx_grad_assemble(h_, &A));
+      return std::unique_ptr<SparseMatrix>(new SparseMatrix(A));
     }
     int64_t  Height() const { return test_size_; }
     sx_grad* handle() const { return h_; }

@@ -15,7 +15,7 @@ from .capi import (SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM, SX_QF_HEAT, SX_QF_J2
                    SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
 
 __all__ = [
-    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "SeracAmdError", "device_name",
+    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "SeracAmdError", "device_name",
     "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC", "SX_QF_J2_LINEAR",
     "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
 ]
@@ -187,6 +187,12 @@ class Gradient:
         self.form.synchronize()
         return d
 
+    def assemble(self):
+        """Gradient::assemble: the linearization as a CSRMatrix on the device."""
+        h = _vp()
+        capi.call("sx_grad_assemble", self.h, ctypes.byref(h))
+        return CSRMatrix(h)
+
     def tangent_bytes(self):
         b = ctypes.c_int64()
         capi.call("sx_grad_tangent_bytes", self.h, ctypes.byref(b))
@@ -207,6 +213,40 @@ class Gradient:
         if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
             raise SeracAmdError(rc, capi.load().sx_last_error().decode())
         return rep.as_dict()
+
+
+class CSRMatrix:
+    """Assembled sparse linearization (device CSR, int64 rows, int32 columns)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        nr, nc, nnz = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        capi.call("sx_csr_info", self.h, ctypes.byref(nr), ctypes.byref(nc), ctypes.byref(nnz))
+        self.shape = (nr.value, nc.value)
+        self.nnz = nnz.value
+
+    def arrays(self):
+        """(row_ptr, col, val) as host numpy arrays."""
+        rp = np.zeros(self.shape[0] + 1, dtype=np.int64)
+        col = np.zeros(self.nnz, dtype=np.int32)
+        val = np.zeros(self.nnz)
+        capi.call("sx_csr_download", self.h, rp.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                  col.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _dptr(val))
+        return rp, col, val
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+
+        rp, col, val = self.arrays()
+        return sp.csr_matrix((val, col, rp), shape=self.shape)
+
+    def mult(self, x, y):
+        capi.call("sx_csr_spmv", self.h, x.ptr, y.ptr)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            capi.load().sx_csr_destroy(self.h)
+            self.h = None
 
 
 class QuadratureData:

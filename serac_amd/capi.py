@@ -97,6 +97,12 @@ SIGNATURES = {
     "sx_form_add_domain_integral_qdata": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                                   ctypes.POINTER(_int), _int, _vp]),
     "sx_form_update_qdata": (_int, [_vp, _int]),
+    "sx_grad_assemble": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sx_csr_info": (_int, [_vp, _I64, _I64, _I64]),
+    "sx_csr_arrays": (_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "sx_csr_download": (_int, [_vp, _I64, _I32, _D]),
+    "sx_csr_spmv": (_int, [_vp, _vp, _vp]),
+    "sx_csr_destroy": (_int, [_vp]),
     "sx_qdata_create": (_int, [ctypes.c_int64, _int, _int, _D, ctypes.POINTER(_vp)]),
     "sx_qdata_info": (_int, [_vp, _I64]),
     "sx_qdata_get": (_int, [_vp, _D]),

@@ -146,7 +146,18 @@ struct sx_space {
   int              ncolors = 0;
   DeviceBuffer     d_emap;  // element order (E-vector gather)
   DeviceBuffer     d_offs, d_ents;  // node -> (e*npe + j), increasing e
+  // DOF-level sparsity of forms on this space (built on first assembly)
+  std::vector<int64_t> csr_row;
+  std::vector<int32_t> csr_col;
   int64_t          ndof() const { return nnodes * nc; }
+};
+
+// Assembled linearization (Gradient::assemble, fn/functional.hpp:569-667): CSR
+// on the device, int64 row offsets, sorted int32 columns per row.
+struct sx_csr {
+  int64_t      nrows = 0, ncols = 0, nnz = 0;
+  DeviceBuffer row, col, val;
+  hipStream_t  stream = nullptr;
 };
 
 // QuadratureData<State> (fn/quadrature_data.hpp:111-167): `size` doubles per
@@ -272,6 +283,66 @@ int run_integral(const sx_form* f, const Integral& I, int mode, const void* u, v
     launches++;
   }
   return launches;
+}
+
+// Sparsity of the space's forms: DOF (node a, component c) couples with
+// (node b, c') for every node b sharing an element with a; rows and columns
+// follow the space's ordering, columns sorted within a row.
+void build_csr_pattern(sx_space* s)
+{
+  if (!s->csr_row.empty()) return;
+  const int64_t nn = s->nnodes, npe = s->npe, ne = int64_t(s->emap.size()) / npe;
+  const int     nc = s->nc;
+  std::vector<int64_t> eptr(size_t(nn) + 1, 0);
+  for (int v : s->emap) eptr[size_t(v) + 1]++;
+  for (int64_t a = 0; a < nn; a++) eptr[size_t(a) + 1] += eptr[size_t(a)];
+  std::vector<int32_t> elist(static_cast<size_t>(eptr[size_t(nn)]));
+  {
+    std::vector<int64_t> fill(eptr.begin(), eptr.end() - 1);
+    for (int64_t e = 0; e < ne; e++)
+      for (int64_t j = 0; j < npe; j++) elist[size_t(fill[size_t(s->emap[size_t(e * npe + j)])]++)] = int32_t(e);
+  }
+  std::vector<int64_t> nptr(size_t(nn) + 1, 0);
+  std::vector<int32_t> adj, buf;
+  std::vector<int64_t> mark(size_t(nn), -1);
+  for (int64_t a = 0; a < nn; a++) {
+    buf.clear();
+    for (int64_t k = eptr[size_t(a)]; k < eptr[size_t(a) + 1]; k++) {
+      const int64_t e = elist[size_t(k)];
+      for (int64_t j = 0; j < npe; j++) {
+        const int b = s->emap[size_t(e * npe + j)];
+        if (mark[size_t(b)] != a) {
+          mark[size_t(b)] = a;
+          buf.push_back(b);
+        }
+      }
+    }
+    std::sort(buf.begin(), buf.end());
+    adj.insert(adj.end(), buf.begin(), buf.end());
+    nptr[size_t(a) + 1] = int64_t(adj.size());
+  }
+  auto& rp = s->csr_row;
+  auto& cl = s->csr_col;
+  rp.assign(size_t(nn * nc) + 1, 0);
+  cl.reserve(size_t(nptr[size_t(nn)]) * size_t(nc) * size_t(nc));
+  int64_t row = 0;
+  auto    emit_row = [&](int64_t a) {
+    if (s->ordering == SX_ORDERING_BYVDIM) {
+      for (int64_t k = nptr[size_t(a)]; k < nptr[size_t(a) + 1]; k++)
+        for (int c2 = 0; c2 < nc; c2++) cl.push_back(int32_t(adj[size_t(k)] * nc + c2));
+    } else {
+      for (int c2 = 0; c2 < nc; c2++)
+        for (int64_t k = nptr[size_t(a)]; k < nptr[size_t(a) + 1]; k++) cl.push_back(int32_t(c2 * nn + adj[size_t(k)]));
+    }
+    rp[size_t(++row)] = int64_t(cl.size());
+  };
+  if (s->ordering == SX_ORDERING_BYVDIM) {
+    for (int64_t a = 0; a < nn; a++)
+      for (int c = 0; c < nc; c++) emit_row(a);
+  } else {
+    for (int c = 0; c < nc; c++)
+      for (int64_t a = 0; a < nn; a++) emit_row(a);
+  }
 }
 
 bool depends(const Integral& I, int trial)
@@ -843,6 +914,91 @@ int sx_grad_diagonal(sx_grad* g, double* diag)
     for (auto& I : f->integrals)
       if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
   });
+}
+
+int sx_grad_assemble(sx_grad* g, sx_csr** out)
+{
+  return api([&] {
+    SX_REQUIRE(g && g->form && out, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "assembly requested before a linearization");
+    sx_form*  f  = g->form;
+    sx_space* ts = const_cast<sx_space*>(f->test);
+    SX_REQUIRE(ts->ndof() < (int64_t(1) << 31), SX_ERR_UNSUPPORTED, "CSR columns are int32");
+    build_csr_pattern(ts);
+    auto A    = std::make_unique<sx_csr>();
+    A->nrows  = ts->ndof();
+    A->ncols  = f->trials[size_t(g->wrt)]->ndof();
+    A->nnz    = int64_t(ts->csr_col.size());
+    A->stream = f->stream;
+    A->row.upload(ts->csr_row);
+    A->col.upload(ts->csr_col);
+    A->val.allocate(size_t(std::max<int64_t>(A->nnz, 1)) * sizeof(double));
+    SX_HIP(sxk::vec_fill(A->val.as<double>(), 0.0, A->nnz, f->stream));
+    for (auto& I : f->integrals) {
+      if (!(depends(*I, g->wrt) && I->has_tangent)) continue;
+      sxk::ElemLaunch P = base_launch(f, *I, sxk::MODE_ASSEMBLE);
+      P.csr_row         = A->row.as<const int64_t>();
+      P.csr_col         = A->col.as<const int32_t>();
+      P.csr_val         = A->val.as<double>();
+      for (size_t c = 0; c + 1 < I->color_off.size(); c++) {
+        P.slot_begin = I->color_off[c];
+        P.slot_end   = I->color_off[c + 1];
+        if (P.slot_end <= P.slot_begin) continue;
+        hipError_t err = hipSuccess;
+        SX_REQUIRE(sxk::launch_element(P, &err), SX_ERR_UNSUPPORTED, "assembly kernel not compiled for this form");
+        SX_HIP(err);
+      }
+    }
+    SX_HIP(hipStreamSynchronize(f->stream));
+    *out = A.release();
+  });
+}
+
+int sx_csr_info(const sx_csr* A, int64_t* nrows, int64_t* ncols, int64_t* nnz)
+{
+  return api([&] {
+    SX_REQUIRE(A, SX_ERR_ARG, "null matrix");
+    if (nrows) *nrows = A->nrows;
+    if (ncols) *ncols = A->ncols;
+    if (nnz) *nnz = A->nnz;
+  });
+}
+
+int sx_csr_arrays(const sx_csr* A, const int64_t** row_ptr, const int32_t** col, const double** val)
+{
+  return api([&] {
+    SX_REQUIRE(A, SX_ERR_ARG, "null matrix");
+    if (row_ptr) *row_ptr = A->row.as<const int64_t>();
+    if (col) *col = A->col.as<const int32_t>();
+    if (val) *val = A->val.as<const double>();
+  });
+}
+
+int sx_csr_download(const sx_csr* A, int64_t* row_ptr, int32_t* col, double* val)
+{
+  return api([&] {
+    SX_REQUIRE(A, SX_ERR_ARG, "null matrix");
+    SX_HIP(hipStreamSynchronize(A->stream));
+    if (row_ptr) SX_HIP(hipMemcpy(row_ptr, A->row.ptr, size_t(A->nrows + 1) * 8, hipMemcpyDeviceToHost));
+    if (col && A->nnz) SX_HIP(hipMemcpy(col, A->col.ptr, size_t(A->nnz) * 4, hipMemcpyDeviceToHost));
+    if (val && A->nnz) SX_HIP(hipMemcpy(val, A->val.ptr, size_t(A->nnz) * 8, hipMemcpyDeviceToHost));
+  });
+}
+
+int sx_csr_spmv(const sx_csr* A, const double* x, double* y)
+{
+  return api([&] {
+    SX_REQUIRE(A && x && y, SX_ERR_ARG, "null argument");
+    SX_HIP(sxk::csr_spmv(A->row.as<const int64_t>(), A->col.as<const int32_t>(), A->val.as<const double>(), x, y,
+                         A->nrows, A->stream));
+    SX_HIP(hipStreamSynchronize(A->stream));
+  });
+}
+
+int sx_csr_destroy(sx_csr* A)
+{
+  delete A;
+  return SX_OK;
 }
 
 int sx_grad_tangent_bytes(const sx_grad* g, int64_t* bytes)

@@ -805,6 +805,114 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   for (int c = 0; c < NC; c++) scatter_add(out, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
 }
 
+// Element gradient assembled into CSR values (element_gradient_kernel,
+// fn/domain_integral_kernels.hpp:327-361, and Gradient::assemble,
+// fn/functional.hpp:569-667, which scatters K_e through lookup tables):
+//   K_e[(c,i)][(c',j)] = sum_q [w phi_i, w grad phi_i]_c . D(q) . [phi_j, grad phi_j]_c'
+// One workgroup per element: the element's tangent is staged in LDS as FP64
+// ([q][DS]), lane i owns test node i and loops over trial nodes j and
+// quadrature points; each entry is added at its CSR position, found by binary
+// search in the row's sorted columns.  Plain read-modify-write when the launch
+// covers one element colour (deterministic), atomics otherwise.
+template <int N, int NC, int LAYOUT, typename S, bool STAGE>
+__global__ void __launch_bounds__(128) k_assemble(const ElemLaunch P)
+{
+  constexpr int NQ = N * N * N;
+  using Shape      = TanShape<LAYOUT, NC>;
+  constexpr bool VAL = Shape::VAL;
+  constexpr int  DS  = Shape::DS;
+  constexpr int  VO  = Shape::VO;
+  constexpr int  TAB = 4 * N * N + N;
+  using Lay          = TangentLayout<S, DS, NQ>;
+  __shared__ double stab[TAB];
+  extern __shared__ double sD[];  // [q][DS]
+
+  const int     li   = threadIdx.x;
+  const int64_t slot = P.slot_begin + blockIdx.x;
+  for (int k = li; k < TAB; k += blockDim.x) stab[k] = P.tables[k];
+  using raw       = typename StoreTraits<S>::raw;
+  const raw* Dsrc = reinterpret_cast<const raw*>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S));
+  auto       Dat  = [&](int k, int q) -> double {
+    const raw v = Dsrc[Lay::offset(k, q)];
+    if constexpr (sizeof(S) == 2)
+      return double(bf16_to_float(uint32_t(v)));
+    else
+      return double(v);
+  };
+  if constexpr (STAGE)
+    for (int t = li; t < NQ * DS; t += blockDim.x) sD[t] = Dat(t % DS, t / DS);
+  __syncthreads();
+  if (li >= NQ) return;
+  const double* tB  = stab;
+  const double* tG  = stab + N * N;
+  const double* tBw = stab + 2 * N * N;
+  const double* tGw = stab + 3 * N * N;
+  const int     ix = li % N, iy = (li / N) % N, iz = li / (N * N);
+  const int64_t node_i = P.map[size_t(slot) * NQ + li];
+
+  for (int jn = 0; jn < NQ; jn++) {
+    const int jx = jn % N, jy = (jn / N) % N, jz = jn / (N * N);
+    double    acc[NC][NC];
+#pragma unroll
+    for (int c = 0; c < NC; c++)
+#pragma unroll
+      for (int c2 = 0; c2 < NC; c2++) acc[c][c2] = 0.0;
+    for (int q = 0; q < NQ; q++) {
+      const int qx = q % N, qy = (q / N) % N, qz = q / (N * N);
+      // tangent entries of point q: staged in LDS, or read through the caches
+      auto Dq = [&](int k) -> double {
+        if constexpr (STAGE)
+          return sD[q * DS + k];
+        else
+          return Dat(k, q);
+      };
+      const double  ug[3] = {tG[qx * N + jx] * tB[qy * N + jy] * tB[qz * N + jz],
+                             tB[qx * N + jx] * tG[qy * N + jy] * tB[qz * N + jz],
+                             tB[qx * N + jx] * tB[qy * N + jy] * tG[qz * N + jz]};
+      const double  tg[3] = {tGw[qx * N + ix] * tBw[qy * N + iy] * tBw[qz * N + iz],
+                             tBw[qx * N + ix] * tGw[qy * N + iy] * tBw[qz * N + iz],
+                             tBw[qx * N + ix] * tBw[qy * N + iy] * tGw[qz * N + iz]};
+      const double  uv = tB[qx * N + jx] * tB[qy * N + jy] * tB[qz * N + jz];
+      const double  tv = tBw[qx * N + ix] * tBw[qy * N + iy] * tBw[qz * N + iz];
+#pragma unroll
+      for (int c = 0; c < NC; c++)
+#pragma unroll
+        for (int c2 = 0; c2 < NC; c2++) {
+          double a = 0.0;
+#pragma unroll
+          for (int x = 0; x < 3; x++)
+#pragma unroll
+            for (int y = 0; y < 3; y++) a += tg[x] * Dq(Shape::idx(VO + c * 3 + x, VO + c2 * 3 + y)) * ug[y];
+          if constexpr (VAL) {
+            a += tv * Dq(Shape::idx(c, c2)) * uv;
+#pragma unroll
+            for (int y = 0; y < 3; y++)
+              a += tv * Dq(Shape::idx(c, VO + c2 * 3 + y)) * ug[y] + tg[y] * Dq(Shape::idx(VO + c * 3 + y, c2)) * uv;
+          }
+          acc[c][c2] += a;
+        }
+    }
+    const int64_t node_j = P.map[size_t(slot) * NQ + jn];
+#pragma unroll
+    for (int c = 0; c < NC; c++) {
+      const int64_t row = lidx(node_i, c, NC, P.nnodes, P.ordering);
+#pragma unroll
+      for (int c2 = 0; c2 < NC; c2++) {
+        const int32_t colv = int32_t(lidx(node_j, c2, NC, P.nnodes, P.ordering));
+        int64_t       lo = P.csr_row[row], hi = P.csr_row[row + 1];  // the column is in [lo, hi)
+        while (hi - lo > 1) {
+          const int64_t mid = (lo + hi) >> 1;
+          if (P.csr_col[mid] <= colv)
+            lo = mid;
+          else
+            hi = mid;
+        }
+        scatter_add(P.csr_val, lo, acc[c][c2], P.scatter);
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launch + dispatch
 // ---------------------------------------------------------------------------
@@ -822,6 +930,19 @@ inline hipError_t launch_grid(const ElemLaunch& P, K kernel)
 template <int N, int NC, int LAYOUT, typename S>
 inline int dispatch_gradient(const ElemLaunch& P, hipError_t* err)
 {
+  if (P.mode == MODE_ASSEMBLE) {
+    const int64_t n = P.slot_end - P.slot_begin;
+    if (n <= 0) return 1;
+    // the element's tangent is staged in LDS when it fits in 48 KB
+    constexpr int    NQ    = N * N * N;
+    constexpr size_t bytes = size_t(NQ) * TanShape<LAYOUT, NC>::DS * sizeof(double);
+    constexpr bool   stage = bytes <= 48 * 1024;
+    const unsigned   threads = unsigned((NQ + 63) / 64 * 64);
+    hipLaunchKernelGGL((k_assemble<N, NC, LAYOUT, S, stage>), dim3(unsigned(n)), dim3(threads), stage ? bytes : 0,
+                       P.stream, P);
+    *err = hipGetLastError();
+    return 1;
+  }
   if (P.mode == MODE_DIAG) {
     *err = launch_grid<N>(P, k_diagonal<N, NC, LAYOUT, S>);
     return 1;
@@ -867,7 +988,7 @@ inline int dispatch_eval(const ElemLaunch& P, hipError_t* err)
 template <int N>
 inline int dispatch_order(const ElemLaunch& P, hipError_t* err)
 {
-  if (P.mode == MODE_JVP || P.mode == MODE_DIAG) {
+  if (P.mode == MODE_JVP || P.mode == MODE_DIAG || P.mode == MODE_ASSEMBLE) {
     if (P.nc == 1) {
       if (P.layout == LAYOUT_GRAD_SYM) return dispatch_gradient_prec<N, 1, LAYOUT_GRAD_SYM>(P, err);
       if (P.layout == LAYOUT_FULL_SYM) return dispatch_gradient_prec<N, 1, LAYOUT_FULL_SYM>(P, err);

@@ -7,7 +7,7 @@
 
 namespace sxk {
 
-enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3 };
+enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3, MODE_ASSEMBLE = 4 };
 
 // Stored-tangent layouts:
 //   GRAD_SYM:  d(parent flux)/d(parent grad), M = 3 nc, symmetric, upper
@@ -39,6 +39,9 @@ struct ElemLaunch {
   double*       qdata;         // quadrature-point state [row][k][q] (residual/linearize)
   const int*    qrow;          // [slot] -> state row (the element's position in its domain)
   int           update_state;  // write the updated state back
+  const int64_t* csr_row;      // assembly: CSR row offsets (test DOFs)
+  const int32_t* csr_col;      // assembly: sorted column ids per row (trial DOFs)
+  double*        csr_val;      // assembly: values, accumulated into
   double        params[8];
   double        t;
   hipStream_t   stream;

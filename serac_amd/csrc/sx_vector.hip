@@ -183,6 +183,23 @@ __global__ void __launch_bounds__(kThreads) k_scatter_e(const int64_t* offs, con
   }
 }
 
+// y = A x for CSR A: one wavefront per row; lanes stride over the row and
+// their partial sums are combined in a fixed tree, so results are
+// reproducible run to run.
+__global__ void __launch_bounds__(kThreads) k_csr_spmv(const int64_t* rp, const int32_t* col, const double* val,
+                                                       const double* x, double* y, int64_t nrows)
+{
+  const int     lane  = threadIdx.x & 63;
+  const int64_t wpb   = kThreads / 64;
+  for (int64_t row = int64_t(blockIdx.x) * wpb + (threadIdx.x >> 6); row < nrows; row += int64_t(gridDim.x) * wpb) {
+    double s = 0.0;
+    for (int64_t k = rp[row] + lane; k < rp[row + 1]; k += 64) s += val[k] * x[col[k]];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+    if (lane == 0) y[row] = s;
+  }
+}
+
 }  // namespace
 
 hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s)
@@ -286,6 +303,14 @@ hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, 
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_scatter_e, dim3(grid_for(n)), dim3(kThreads), 0, s, offs, ents, E, L, nnodes, npe, nc,
                      ordering);
+  return hipGetLastError();
+}
+
+hipError_t csr_spmv(const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                    int64_t nrows, hipStream_t s)
+{
+  if (nrows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_spmv, dim3(grid_for(nrows * 64)), dim3(kThreads), 0, s, rp, col, val, x, y, nrows);
   return hipGetLastError();
 }
 

@@ -35,4 +35,8 @@ hipError_t gather_e(const int* map, const double* L, double* E, int64_t ne, int 
 hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, double* L, int64_t nnodes, int npe,
                      int nc, int ordering, hipStream_t s);
 
+// y = A x, A in CSR (int64 row offsets, int32 columns)
+hipError_t csr_spmv(const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
+                    int64_t nrows, hipStream_t s);
+
 }  // namespace sxk

@@ -7,6 +7,7 @@
 //   - solid_statics_patch.cpp:30-100: the AffineSolution interior residual
 //   - heat integrands (ThermalMaterialIntegrand + ConstantSource)
 //   - EquationSolver Newton + Jacobi-CG on a 1% compression step
+//   - Gradient::assemble: CSR SpMV against the matrix-free action
 //   - J2 plasticity with QuadratureData: residual, J.v and the state
 //     updateQdata commits, against the oracle (1e-12)
 //   - error paths: unsupported configurations raise serac_amd::Error
@@ -113,6 +114,14 @@ int main()
   Vector jv2(ndof);
   dR.Mult(du, jv2);
   CHECK(relerr(jv2.ToHost(), Jv) == 0.0, "Gradient::Mult == Gradient::operator()");
+  {
+    // Gradient::assemble: CSR SpMV equals the matrix-free action
+    auto   A = dR.assemble();
+    Vector y(ndof);
+    A->Mult(du, y);
+    CHECK(A->Height() == ndof && A->NumNonZeroElems() > 0 && relerr(y.ToHost(), Jv) < 1e-12,
+          "assemble(): CSR SpMV vs J.v %.2e (nnz %lld)", relerr(y.ToHost(), Jv), (long long)A->NumNonZeroElems());
+  }
   std::vector<double> diag = dR.AssembleDiagonal().ToHost();
   double              dmin = 1e300;
   for (double v : diag) dmin = std::fmin(dmin, v);
