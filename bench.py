@@ -26,11 +26,14 @@ METRIC = "GDOF/s matrix-free J·v (p=3 hex, FP64 vs FP32); Newton solve s to 1e-
 HBM_PEAK_GBS = 8000.0
 
 WORKLOADS = {
-    # name: (n, order, perturb, params)
-    "neohookean_p3_64": (64, 3, 0.2, (1.0, 10.0, 0.25)),
-    "neohookean_p4_48": (48, 4, 0.2, (1.0, 10.0, 0.25)),
-    "neohookean_p2_128": (128, 2, 0.0, (1.0, 10.0, 0.25)),
+    # name: (n, order, perturb, params, qfunction); d = minimal stored tangent per point
+    "neohookean_p3_64": (64, 3, 0.2, (1.0, 10.0, 0.25), "neohookean"),
+    "neohookean_p4_48": (48, 4, 0.2, (1.0, 10.0, 0.25), "neohookean"),
+    "neohookean_p2_128": (128, 2, 0.0, (1.0, 10.0, 0.25), "neohookean"),
+    # J2 with linear hardening: E, nu, sigma_y, Hi, Hk, density, geometric nonlinearity
+    "j2_p2_96": (96, 2, 0.0, (200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0), "j2"),
 }
+TANGENT_ENTRIES = {"neohookean": 45, "j2": 81}
 
 
 def log(msg):
@@ -54,7 +57,7 @@ def profiled_traffic(workload, scatter, remap):
     return None
 
 
-def cpu_baseline(n, p, perturb, params, seconds):
+def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
     """The oracle's J.v (the reference algorithm restated in C++, one core)
     on an n^3 sample of the same workload."""
     import oracle as O
@@ -65,8 +68,13 @@ def cpu_baseline(n, p, perturb, params, seconds):
         m.perturb(42, perturb)
     s = O.Space(m, p, 3)
     f = O.Form(s)
-    f.add_domain_integral(O.QF_NEOHOOKEAN, params)
-    U = W.neohookean_displacement(s.node_coords(), noise=W.noise_for_mesh(n))
+    if qf == "j2":
+        f.add_domain_integral_state(O.QF_J2_LINEAR, params, np.zeros(10))
+        U = np.zeros(s.ndof)
+        U[0::3] = 0.03 * s.node_coords()[:, 2]
+    else:
+        f.add_domain_integral(O.QF_NEOHOOKEAN, params)
+        U = W.neohookean_displacement(s.node_coords(), noise=W.noise_for_mesh(n))
     dU = W.direction(s.nnodes, 3)
     f.linearize(U)
     t1 = f.time_jvp(dU, 1)
@@ -78,7 +86,7 @@ def cpu_baseline(n, p, perturb, params, seconds):
         "unit": "GDOF/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle J.v (stored 81-entry tangent, sum factorization), {n}^3 p={p} Neo-Hookean, "
+        "sample": f"oracle J.v (stored 81-entry tangent, sum factorization), {n}^3 p={p} {qf}, "
                   f"{s.ndof} DOFs, {reps} applications in {t:.1f} s on one host core",
     }
 
@@ -99,7 +107,12 @@ def main():
     ap.add_argument("--newton-tol", type=float, default=1e-8)
     ap.add_argument("--cg-tol", type=float, default=1e-6)
     ap.add_argument("--cg-max", type=int, default=20000)
+    ap.add_argument("--solve-precision", default="f64",
+                    help="comma-separated solve modes: f64 (plain FP64 Jacobi-CG), mixed_f32 / mixed_bf16 "
+                         "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov)")
     ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
+    ap.add_argument("--split-scatter", default="colored", choices=["colored", "atomic"],
+                    help="scatter mode of the FP32 / BF16 J.v measurements")
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a known-size read (dot of two L-vectors) and write (fill) for counter scaling")
@@ -113,7 +126,7 @@ def main():
     import serac_amd as sx
     from serac_amd import workloads as W
 
-    n, p, perturb, params = WORKLOADS[args.workload]
+    n, p, perturb, params, qf = WORKLOADS[args.workload]
     sx.capi.call("sx_init", 0)
     dev = sx.device_name()
     log(f"device {dev}; workload {args.workload}")
@@ -123,19 +136,37 @@ def main():
     if perturb > 0:
         mesh.perturb(42, perturb)
     space = sx.Space(mesh, p, 3)
-    form = sx.Functional(space, [space])
-    form.set_option(sx.capi.SX_OPT_SCATTER,
-                    sx.capi.SX_SCATTER_ATOMIC if args.scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
-    form.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
-    form.add_domain_integral(sx.SX_QF_NEOHOOKEAN, params)
+    nq1 = p + 1
+    qdatas = []
+
+    def make_form(scatter):
+        f = sx.Functional(space, [space])
+        f.set_option(sx.capi.SX_OPT_SCATTER,
+                     sx.capi.SX_SCATTER_ATOMIC if scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
+        f.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
+        if qf == "j2":
+            qd = sx.QuadratureData(space.ne, nq1 ** 3, np.zeros(10))
+            qdatas.append(qd)
+            f.add_domain_integral(sx.SX_QF_J2_LINEAR, params, qdata=qd)
+        else:
+            f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, params)
+        return f
+
+    form = make_form(args.scatter)
     xyz = space.node_coords()
-    U = sx.DeviceVector.from_numpy(W.neohookean_displacement(xyz, noise=W.noise_for_mesh(n)))
+    if qf == "j2":
+        # simple shear to 3 % (the end of the cyclic path): every point is plastic
+        Uh = np.zeros(space.ndof)
+        Uh[0::3] = 0.03 * xyz[:, 2]
+        U = sx.DeviceVector.from_numpy(Uh)
+        del Uh
+    else:
+        U = sx.DeviceVector.from_numpy(W.neohookean_displacement(xyz, noise=W.noise_for_mesh(n)))
     dU = sx.DeviceVector.from_numpy(W.direction(space.nnodes, 3))
     dR = sx.DeviceVector(space.ndof)
     del xyz
     log(f"setup {time.time() - t0:.1f} s: {space.ndof} DOFs, {space.ne} elements")
 
-    nq1 = p + 1
     nqp = space.ne * nq1 ** 3
 
     # residual and linearization (timed with the host clock around a sync)
@@ -166,7 +197,7 @@ def main():
         raise SystemExit("bench: non-finite residual or J.v; the measurement is invalid")
     jv_norm = float(np.linalg.norm(jv))
     # dominant kernel: the colour launches of the J.v element kernel
-    d_sym = 45
+    d_sym = TANGENT_ENTRIES[qf]
     b_alg = W.jvp_algorithmic_bytes(space.ndof, nqp, space.ne, space.npe, d_sym)
     kern_ms_per_apply = ms_kern / args.steps
     achieved = b_alg / (kern_ms_per_apply / 1e3) / 1e9
@@ -176,11 +207,16 @@ def main():
     # precision-split operator: FP32 vectors and arithmetic, tangent in FP32 or BF16
     split = {}
     if not args.no_split:
+        # FP32 scatter-adds go faster as coloured read-modify-writes than as
+        # atomics, so the split operator runs on a coloured form
+        grad = None
+        del form
+        sform = make_form(args.split_scatter)
         dU32 = sx.DeviceVector.from_numpy(W.direction(space.nnodes, 3), dtype=np.float32)
         dR32 = sx.DeviceVector(space.ndof, dtype=np.float32)
         for name, tprec, s_d in (("fp32", sx.capi.SX_PREC_F32, 4), ("fp32_bf16", sx.capi.SX_PREC_BF16, 2)):
-            form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
-            _, g32 = form(0.0, U, wrt=0)
+            sform.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
+            _, g32 = sform(0.0, U, wrt=0)
             if args.warmup > 0:
                 g32.apply_timed(dU32, dR32, args.warmup)
             t32, k32, _ = g32.apply_timed(dU32, dR32, args.steps)
@@ -199,43 +235,70 @@ def main():
             }
             log(f"{name} J.v {t32 / args.steps:.3f} ms/step ({split[name]['value']:.2f} GDOF/s), "
                 f"{ach:.0f} GB/s algorithmic, rel. error vs FP64 {split[name]['rel_err_vs_fp64']:.2e}")
-        form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, sx.capi.SX_PREC_F64)
-        del dU32, dR32
+            split[name]["scatter"] = args.split_scatter
+            del g32
+        del dU32, dR32, sform
+        form = make_form(args.scatter)
     del jv
 
     solve = None
+    if args.solve and qf != "neohookean":
+        raise SystemExit("bench: --solve runs the uniaxial-compression load path of the Neo-Hookean workloads")
     if args.solve:
         # uniaxial compression: x = 0 clamped, x = 1 displaced by -strain in x over `load_steps`
         ess0 = space.boundary_dofs(0b000001, 0x7)
         ess1 = space.boundary_dofs(0b000010, 0x1)
         ess = np.union1d(ess0, ess1).astype(np.int32)
         final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
-        x = sx.DeviceVector(space.ndof)
-        sx.capi.call("sx_device_synchronize")
-        # one load step per call (the same arithmetic as one call over all
-        # steps: each call starts from the essential values x holds) so that
-        # every step reports as it finishes
-        reps, t_solve = [], 0.0
-        for k in range(1, args.load_steps + 1):
-            ts = time.perf_counter()
-            reps += sx.quasistatic_solve(form, x, ess, final * (k / args.load_steps), 1, warm_start=True,
-                                         rel_tol=args.newton_tol, lin_rel_tol=args.cg_tol, lin_max_iter=args.cg_max)
+        modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
+                 "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
+                 "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32)}
+        solve = {}
+        del form
+        for mode in args.solve_precision.split(","):
+            tprec, arith = modes[mode]
+            # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
+            scatter = args.scatter if mode == "f64" else args.split_scatter
+            form = make_form(scatter)
+            form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
+            form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
+            x = sx.DeviceVector(space.ndof)
             sx.capi.call("sx_device_synchronize")
-            t_solve += time.perf_counter() - ts
-            log(f"load step {k}/{args.load_steps}: Newton {reps[-1]['newton_iterations']}, "
-                f"CG {reps[-1]['linear_iterations']}, {t_solve:.2f} s so far")
-        solve = {
-            "seconds": round(t_solve, 3),
-            "load_steps": args.load_steps,
-            "strain": args.strain,
-            "newton_rel_tol": args.newton_tol,
-            "cg_rel_tol": args.cg_tol,
-            "precond": "Jacobi",
-            "newton_iterations": [r["newton_iterations"] for r in reps],
-            "cg_iterations": [r["linear_iterations"] for r in reps],
-            "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0 for r in reps],
-        }
-        log(f"solve {t_solve:.2f} s: Newton {solve['newton_iterations']}, CG {solve['cg_iterations']}")
+            # one load step per call (the same arithmetic as one call over all
+            # steps: each call starts from the essential values x holds) so that
+            # every step reports as it finishes
+            reps, t_solve = [], 0.0
+            for k in range(1, args.load_steps + 1):
+                ts = time.perf_counter()
+                reps += sx.quasistatic_solve(form, x, ess, final * (k / args.load_steps), 1, warm_start=True,
+                                             rel_tol=args.newton_tol, lin_rel_tol=args.cg_tol,
+                                             lin_max_iter=args.cg_max)
+                sx.capi.call("sx_device_synchronize")
+                t_solve += time.perf_counter() - ts
+                log(f"{mode} load step {k}/{args.load_steps}: Newton {reps[-1]['newton_iterations']}, "
+                    f"CG {reps[-1]['linear_iterations']}, {t_solve:.2f} s so far")
+            solve[mode] = {
+                "seconds": round(t_solve, 3),
+                "load_steps": args.load_steps,
+                "strain": args.strain,
+                "newton_rel_tol": args.newton_tol,
+                "cg_rel_tol": args.cg_tol,
+                "precond": "Jacobi",
+                "tangent": {0: "f64", 1: "f32", 2: "bf16"}[tprec],
+                "jvp_arithmetic": {0: "f64", 1: "f32"}[arith],
+                "newton_iterations": [r["newton_iterations"] for r in reps],
+                "cg_iterations": [r["linear_iterations"] for r in reps],
+                "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0
+                                       for r in reps],
+            }
+            solve[mode]["scatter"] = scatter
+            log(f"{mode} solve {t_solve:.2f} s: Newton {solve[mode]['newton_iterations']}, "
+                f"CG {solve[mode]['cg_iterations']}")
+            del x, form
+        form = make_form(args.scatter)
+        if "f64" in solve:
+            for mode in solve:
+                solve[mode]["speedup_vs_f64"] = round(solve["f64"]["seconds"] / solve[mode]["seconds"], 3)
 
     if args.calibrate:
         # reference kernels for HBM counter scaling: k_dot_partial reads 2 * 8 * ndof
@@ -250,7 +313,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         tc = time.time()
-        cpu = cpu_baseline(args.cpu_sample_n, p, perturb, params, args.cpu_seconds)
+        cpu = cpu_baseline(args.cpu_sample_n, p, perturb, params, args.cpu_seconds, qf)
         log(f"cpu baseline {cpu['value']:.5f} GDOF/s ({time.time() - tc:.1f} s)")
 
     rec = {
@@ -267,8 +330,8 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: Neo-Hookean J.v, {n}^3 hexes, p={p}, {nq1}^3 Gauss points, "
-                        f"perturbed {perturb}h",
+            "workload": f"{args.workload}: {'J2 plasticity' if qf == 'j2' else 'Neo-Hookean'} J.v, {n}^3 hexes, "
+                        f"p={p}, {nq1}^3 Gauss points, perturbed {perturb}h",
             "ndof": space.ndof,
             "elements": space.ne,
             "qpoints": nqp,
@@ -284,8 +347,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic(args.workload, args.scatter, args.xcd_remap),
             "traffic_source": PROFILED_TRAFFIC.get((args.workload, args.scatter, args.xcd_remap)),
-            "kernel": "k_element<4,3,MODE_JVP>" + (" (one launch per element colour)" if args.scatter == "colored"
-                                                   else " (one launch, atomic scatter)"),
+            "kernel": f"k_jvp<{nq1},3,{'GRAD_FULL' if qf == 'j2' else 'GRAD_SYM'},double,double,double>"
+                      + (" (one launch per element colour)" if args.scatter == "colored"
+                         else " (one launch, atomic scatter)"),
             "algorithmic_bytes_per_apply": b_alg,
             "kernel_ms_per_apply": round(kern_ms_per_apply, 5),
             "launches_per_apply": launches,

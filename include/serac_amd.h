@@ -62,7 +62,22 @@ enum {
                                phy/solid_mechanics.hpp:834-852 */
 };
 
-#define SX_MAX_PARAMS 8
+/* compiled boundary (face) qfunctions; the source they return is multiplied
+ * by |dX/dxi1 x dX/dxi2| and integrated against the test function values
+ * (fn/boundary_integral_kernels.hpp:106-205) */
+enum {
+  SX_BQF_TRACTION = 101, /* params {A (3x3 row-major), b (3)}: setTraction with
+                            traction(X, n) = A n + b, source = -(A n_hat + b)
+                            phy/solid_mechanics.hpp:1023-1037 */
+  SX_BQF_PRESSURE = 102, /* params {p, geom_nonlin (0/1)}: setPressure, source =
+                            p (dx/dxi1 x dx/dxi2) / |dX/dxi1 x dX/dxi2| with x = X + u
+                            when geom_nonlin, else x = X  phy/solid_mechanics.hpp:1071-1101 */
+  SX_BQF_FLUX     = 103  /* params {q0, h, T_inf}, scalar: setFluxBCs with
+                            flux = q0 + h (T - T_inf) (constant flux or Robin)
+                            phy/heat_transfer.hpp:536-552 */
+};
+
+#define SX_MAX_PARAMS 16
 
 /* ---- runtime helpers (no reference counterpart: device memory plumbing) ---- */
 const char* sx_last_error(void);
@@ -127,6 +142,13 @@ int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntri
  * elems == NULL / nelem == 0 means the whole mesh (EntireDomain). */
 int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
                                 int ndep, const int* elems, int nelem);
+/* AddBoundaryIntegral, fn/functional.hpp:328-354 (fn/integral.hpp:275-347).
+ * Integrates over the faces that belong to one element, selected by
+ * face_mask among the planes of the mesh's bounding box (bit 0 x = min,
+ * 1 x = max, 2 y = min, 3 y = max, 4 z = min, 5 z = max); face_mask = 0 is the
+ * whole boundary (EntireBoundary).  qfunc_id is one of SX_BQF_*. */
+int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                  const int* depends_on, int ndep, int face_mask);
 /* AddDomainIntegral with a QuadratureData argument (fn/functional.hpp:286-317,
  * fn/quadrature_data.hpp:111-167).  qdata must hold one record per element of
  * the domain (in domain order) and quadrature point; it is borrowed and must
@@ -150,11 +172,16 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
  *     Must be set before the first AddDomainIntegral.
  *   SX_OPT_XCD_REMAP: 1 (default) gives each of the 8 XCDs a contiguous run of
  *     element blocks (L2 locality for shared nodes); 0 keeps hardware order. */
-enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2, SX_OPT_TANGENT_PRECISION = 3 };
+enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2, SX_OPT_TANGENT_PRECISION = 3, SX_OPT_JVP_ARITHMETIC = 4 };
 enum { SX_SCATTER_COLORED = 0, SX_SCATTER_ATOMIC = 1 };
 /* Precisions.  SX_OPT_TANGENT_PRECISION selects how the next linearization
  * stores the quadrature-point tangent (computed in FP64, rounded once to
- * FP32 or BF16); J.v then streams that format. */
+ * FP32 or BF16); J.v then streams that format.  SX_OPT_JVP_ARITHMETIC
+ * (SX_PREC_F64 default, or SX_PREC_F32) sets the element arithmetic of J.v on
+ * FP64 vectors (sx_grad_apply and the solvers): with SX_PREC_F32 the element
+ * work runs in FP32 while the vectors, the assembly into them and every
+ * Krylov dot product and norm stay FP64 (the precision-split operator of the
+ * mixed-precision Newton). */
 enum { SX_PREC_F64 = 0, SX_PREC_F32 = 1, SX_PREC_BF16 = 2 };
 int sx_form_set_option(sx_form* f, int option, int64_t value);
 int sx_form_synchronize(sx_form* f);

@@ -248,6 +248,14 @@ struct Domain {
   static Domain    Elements(std::vector<int> ids) { return Domain{std::move(ids)}; }
 };
 
+/// boundary faces on planes of the mesh's bounding box (serac::Domain for
+/// boundary elements, EntireBoundary when face_mask = 0)
+struct BoundaryDomain {
+  int                   face_mask = 0;  // bit 0 x = min, 1 x = max, 2 y = min, 3 y = max, 4 z = min, 5 z = max
+  static BoundaryDomain EntireBoundary() { return BoundaryDomain{0}; }
+  static BoundaryDomain Planes(int mask) { return BoundaryDomain{mask}; }
+};
+
 // ---------------------------------------------------------------------------
 // integrands of the compiled registry
 // ---------------------------------------------------------------------------
@@ -320,6 +328,61 @@ struct MaterialStressFunctor {
 /// {a u, b grad u}, the integrand of fn/tests/functional_comparisons.cpp:84-203
 struct MassDiffusion {
   double a, b;
+};
+
+namespace solid_mechanics {
+/// the boundary integrand setTraction adds (phy/solid_mechanics.hpp:1023-1037)
+/// for traction(X, n) = A n + b
+struct LinearTraction {
+  double A[3][3];
+  double b[3];
+};
+/// the boundary integrand setPressure adds (phy/solid_mechanics.hpp:1071-1101)
+struct Pressure {
+  double                  pressure;
+  GeometricNonlinearities geom_nonlin;
+};
+}  // namespace solid_mechanics
+namespace heat_transfer {
+/// the boundary integrand setFluxBCs adds (phy/heat_transfer.hpp:536-552) for
+/// flux = q0 + h (T - T_inf)
+struct LinearFlux {
+  double q0, h, T_inf;
+};
+}  // namespace heat_transfer
+
+/// boundary integrand registry (SX_BQF_*)
+template <typename Integrand>
+struct bqfunction_traits {
+  static constexpr bool registered = false;
+};
+template <>
+struct bqfunction_traits<solid_mechanics::LinearTraction> {
+  static constexpr bool      registered = true;
+  static int                 id(const solid_mechanics::LinearTraction&) { return SX_BQF_TRACTION; }
+  static std::vector<double> params(const solid_mechanics::LinearTraction& q)
+  {
+    std::vector<double> p;
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) p.push_back(q.A[i][j]);
+    for (int i = 0; i < 3; i++) p.push_back(q.b[i]);
+    return p;
+  }
+};
+template <>
+struct bqfunction_traits<solid_mechanics::Pressure> {
+  static constexpr bool      registered = true;
+  static int                 id(const solid_mechanics::Pressure&) { return SX_BQF_PRESSURE; }
+  static std::vector<double> params(const solid_mechanics::Pressure& q)
+  {
+    return {q.pressure, q.geom_nonlin == GeometricNonlinearities::On ? 1.0 : 0.0};
+  }
+};
+template <>
+struct bqfunction_traits<heat_transfer::LinearFlux> {
+  static constexpr bool      registered = true;
+  static int                 id(const heat_transfer::LinearFlux&) { return SX_BQF_FLUX; }
+  static std::vector<double> params(const heat_transfer::LinearFlux& q) { return {q.q0, q.h, q.T_inf}; }
 };
 
 /// Maps an integrand type to (qfunction id, parameters).  Unregistered types
@@ -630,6 +693,33 @@ class Functional<test(trials...)> {
   {
     AddDomainIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, Domain{}, std::move(qdata));
   }
+  /// AddBoundaryIntegral (fn/functional.hpp:328-354) over the whole boundary or
+  /// the faces on chosen planes
+  template <int dim, int... args, typename Integrand>
+  void AddBoundaryIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand,
+                           const BoundaryDomain& domain)
+  {
+    static_assert(dim == 2, "boundary integrals on hexahedral meshes are over quadrilateral faces (Dimension<2>)");
+    static_assert(bqfunction_traits<Integrand>::registered,
+                  "integrand type is not in the compiled boundary qfunction registry (see bqfunction_traits)");
+    static_assert(((args >= 0 && args < num_trial_spaces) && ...), "DependsOn index out of range");
+    const int                 deps[] = {args...};
+    const std::vector<double> p      = bqfunction_traits<Integrand>::params(integrand);
+    detail::check(sx_form_add_boundary_integral(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
+                                                int(p.size()), deps, int(sizeof...(args)), domain.face_mask));
+  }
+  template <int dim, int... args, typename Integrand>
+  void AddBoundaryIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&)
+  {
+    AddBoundaryIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, BoundaryDomain::EntireBoundary());
+  }
+  /// alias of the reference (fn/functional.hpp:385-393)
+  template <int... args, typename Integrand, typename DomainT>
+  void AddSurfaceIntegral(DependsOn<args...>, const Integrand& integrand, const DomainT& domain)
+  {
+    AddBoundaryIntegral(Dimension<2>{}, DependsOn<args...>{}, integrand, domain);
+  }
+
   /// updateQdata (fn/functional.hpp:523)
   void updateQdata(bool update_flag) { detail::check(sx_form_update_qdata(h_, update_flag ? 1 : 0)); }
 

@@ -21,6 +21,9 @@ QF_MASS_DIFFUSION = 2
 QF_NEOHOOKEAN = 3
 QF_LINEAR_ELASTIC = 4
 QF_J2_LINEAR = 5
+BQF_TRACTION = 101
+BQF_PRESSURE = 102
+BQF_FLUX = 103
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -64,6 +67,10 @@ def _load():
         "orc_qf_tangent": (ctypes.c_int, [ctypes.c_int, _D, ctypes.c_int, _D, _D, _D, _D]),
         "orc_form_add_integral_state": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int, _D]),
         "orc_form_update_qdata": (None, [_vp, ctypes.c_int]),
+        "orc_form_add_boundary_integral": (ctypes.c_int, [_vp, ctypes.c_int, _D, ctypes.c_int]),
+        "orc_form_nfaces": (ctypes.c_int, [_vp, ctypes.c_int]),
+        "orc_form_face": (None, [_vp, ctypes.c_int, ctypes.c_int, _I]),
+        "orc_form_face_matrix": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _D]),
         "orc_form_ssize": (ctypes.c_int, [_vp, ctypes.c_int]),
         "orc_form_get_qdata": (ctypes.c_int, [_vp, ctypes.c_int, _D]),
         "orc_form_set_qdata": (ctypes.c_int, [_vp, ctypes.c_int, _D]),
@@ -114,7 +121,7 @@ def _check(rc):
 
 
 def _params(p):
-    a = np.zeros(8)
+    a = np.zeros(16)
     a[: len(p)] = p
     return a
 
@@ -283,6 +290,25 @@ class Form:
 
     def update_qdata(self, flag):
         lib().orc_form_update_qdata(self.h, int(bool(flag)))
+
+    def add_boundary_integral(self, kind, params, face_mask=0):
+        """AddBoundaryIntegral over the bounding-box planes of face_mask (0: whole boundary)."""
+        _check(lib().orc_form_add_boundary_integral(self.h, kind, _dp(_params(params)), face_mask))
+        self.nbintegrals = getattr(self, "nbintegrals", 0) + 1
+
+    def faces(self, bi=0):
+        """(element, local face) of each face of boundary integral bi."""
+        n = lib().orc_form_nfaces(self.h, bi)
+        out = np.zeros((n, 2), dtype=np.int32)
+        for k in range(n):
+            lib().orc_form_face(self.h, bi, k, _ip(out[k]))
+        return out
+
+    def face_matrix(self, bi, k):
+        nd = self.space.nc * self.space.npe
+        K = np.zeros((nd, nd))
+        _check(lib().orc_form_face_matrix(self.h, bi, k, _dp(K)))
+        return K
 
     def qdata(self, integral=0, nelem=None):
         """State of an integral as [elem-in-domain][q][size]."""

@@ -148,7 +148,7 @@ int orc_qf_eval(int kind, const double* params, int nc, const double* value, con
   return guard([&] {
     QFParams P;
     P.kind = kind;
-    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
     qf_eval(P, nc, value, grad, source, flux);
   });
 }
@@ -160,7 +160,7 @@ int orc_qf_eval_state(int kind, const double* params, int nc, const double* valu
   return guard([&] {
     QFParams P;
     P.kind = kind;
-    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
     qf_eval(P, nc, value, grad, source, flux, state);
   });
 }
@@ -172,7 +172,7 @@ int orc_qf_tangent(int kind, const double* params, int nc, const double* value, 
   return guard([&] {
     QFParams P;
     P.kind = kind;
-    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
     dual<9> v[3], g[9], s[3], f[9];
     for (int i = 0; i < nc; i++) v[i] = dual<9>(value[i]);
     for (int k = 0; k < 3 * nc; k++) {
@@ -199,7 +199,7 @@ int  orc_form_add_integral(void* f, int kind, const double* params, const int* e
   return guard([&] {
     QFParams P;
     P.kind = kind;
-    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
     std::vector<int> el;
     if (elems && nelem > 0) el.assign(elems, elems + nelem);
     static_cast<OForm*>(f)->form.add_integral(P, el);
@@ -212,11 +212,34 @@ int orc_form_add_integral_state(void* f, int kind, const double* params, const i
   return guard([&] {
     QFParams P;
     P.kind = kind;
-    for (int i = 0; i < 8; i++) P.p[i] = params[i];
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
     std::vector<int> el;
     if (elems && nelem > 0) el.assign(elems, elems + nelem);
     static_cast<OForm*>(f)->form.add_integral(P, el, state_init);
   });
+}
+// AddBoundaryIntegral on the bounding-box planes of face_mask (0: whole boundary)
+int orc_form_add_boundary_integral(void* f, int kind, const double* params, int face_mask)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
+    static_cast<OForm*>(f)->form.add_boundary_integral(P, face_mask);
+  });
+}
+int orc_form_nfaces(void* f, int bi) { return int(static_cast<OForm*>(f)->form.bintegrals[size_t(bi)].faces.size()); }
+// face k of boundary integral bi: out = {element, local face}
+void orc_form_face(void* f, int bi, int k, int* out)
+{
+  const auto& F = static_cast<OForm*>(f)->form.bintegrals[size_t(bi)].faces[size_t(k)];
+  out[0]        = F[0];
+  out[1]        = F[1];
+}
+// boundary face matrix over its element's DOFs (after a linearization)
+int orc_form_face_matrix(void* f, int bi, int k, double* K)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.face_matrix(bi, size_t(k), K); });
 }
 void orc_form_update_qdata(void* f, int flag) { static_cast<OForm*>(f)->form.update_qdata = flag != 0; }
 int  orc_form_ssize(void* f, int integral) { return static_cast<OForm*>(f)->form.integrals[size_t(integral)].ssize; }

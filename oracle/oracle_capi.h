@@ -19,6 +19,8 @@ void        orc_form_free(void* f);
 int         orc_form_add_integral(void* f, int kind, const double* params, const int* elems, int nelem);
 int         orc_form_eval(void* f, const double* U, double* R, int linearize);
 int         orc_form_jvp(void* f, const double* dU, double* dR);
+int         orc_qf_eval(int kind, const double* params, int nc, const double* value, const double* grad, double* source,
+                        double* flux);
 int         orc_form_add_integral_state(void* f, int kind, const double* params, const int* elems, int nelem,
                                         const double* state_init);
 void        orc_form_update_qdata(void* f, int flag);

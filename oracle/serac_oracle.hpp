@@ -780,7 +780,7 @@ enum QFKind { QF_HEAT = 1, QF_MASS_DIFFUSION = 2, QF_NEOHOOKEAN = 3, QF_LINEAR_E
 
 struct QFParams {
   int    kind = QF_HEAT;
-  double p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double p[16] = {0};
 };
 
 // Heat: ThermalMaterialIntegrand(LinearIsotropicConductor) + setSource
@@ -1007,6 +1007,58 @@ inline void qf_eval(const QFParams& P, int nc, const T* value, const T* grad, T*
     }
   }
   throw std::runtime_error("qf_eval: unknown qfunction");
+}
+
+// ---------------------------------------------------------------------------
+// Boundary qfunctions (the integrands AddBoundaryIntegral receives,
+// fn/boundary_integral_kernels.hpp:106-130): reference position X, dX/dxi
+// (3 x 2, dX[i][k] = d X_i / d xi_k), field value u[nc] and face-parent
+// derivative du[nc][2] -> source[nc] (the caller multiplies by
+// |dX/dxi1 x dX/dxi2|).
+//   TRACTION: setTraction (phy/solid_mechanics.hpp:1023-1037), traction(X, n) =
+//     A n + b, source = -1 * (A n_hat + b).              p = {A (9), b (3)}
+//   PRESSURE: setPressure (:1071-1101), source = p n / |dX/dxi1 x dX/dxi2| with
+//     n = dx/dxi1 x dx/dxi2, x = X + u if geom_nonlin else X. p = {p, geom}
+//   FLUX: setFluxBCs (phy/heat_transfer.hpp:536-552), flux = q0 + h (T - T_inf).
+//                                                        p = {q0, h, T_inf}
+// ---------------------------------------------------------------------------
+enum BQFKind { BQF_TRACTION = 101, BQF_PRESSURE = 102, BQF_FLUX = 103 };
+inline int bqf_components(int kind) { return kind == BQF_FLUX ? 1 : 3; }
+
+template <typename T>
+inline void bqf_eval(const QFParams& P, const double* X, const double dX[3][2], const T* u, const T* du, T* src)
+{
+  (void)X;
+  switch (P.kind) {
+    case BQF_TRACTION: {
+      double n[3] = {dX[1][0] * dX[2][1] - dX[2][0] * dX[1][1], dX[2][0] * dX[0][1] - dX[0][0] * dX[2][1],
+                     dX[0][0] * dX[1][1] - dX[1][0] * dX[0][1]};
+      double nn   = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int i = 0; i < 3; i++) n[i] = n[i] / nn;
+      for (int i = 0; i < 3; i++)
+        src[i] = T(-1.0 * (P.p[i * 3 + 0] * n[0] + P.p[i * 3 + 1] * n[1] + P.p[i * 3 + 2] * n[2] + P.p[9 + i]));
+      return;
+    }
+    case BQF_PRESSURE: {
+      const bool g = P.p[1] != 0.0;
+      T          a[3], b[3];
+      for (int i = 0; i < 3; i++) {
+        a[i] = g ? T(dX[i][0]) + du[i * 2 + 0] : T(dX[i][0]);
+        b[i] = g ? T(dX[i][1]) + du[i * 2 + 1] : T(dX[i][1]);
+      }
+      T      n[3] = {a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]};
+      double r[3] = {dX[1][0] * dX[2][1] - dX[2][0] * dX[1][1], dX[2][0] * dX[0][1] - dX[0][0] * dX[2][1],
+                     dX[0][0] * dX[1][1] - dX[1][0] * dX[0][1]};
+      double nr   = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      for (int i = 0; i < 3; i++) src[i] = P.p[0] * (n[i] / nr);
+      return;
+    }
+    case BQF_FLUX: {
+      src[0] = P.p[0] + P.p[1] * (u[0] - P.p[2]);
+      return;
+    }
+  }
+  throw std::runtime_error("bqf_eval: unknown boundary qfunction");
 }
 
 // Which derivative blocks the reference's compile-time `zero` sparsity keeps

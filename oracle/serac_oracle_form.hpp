@@ -3,6 +3,9 @@
 // Jacobi-CG loop.  TEST INFRASTRUCTURE ONLY (see serac_oracle.hpp).
 #pragma once
 
+#include <array>
+#include <map>
+
 #include "serac_oracle.hpp"
 
 namespace oracle {
@@ -51,6 +54,15 @@ struct Integral {
   std::vector<double> state;
 };
 
+// AddBoundaryIntegral (fn/functional.hpp:328-354): faces (element, local face
+// 2 * axis + side) that belong to one element.  D holds the forward-mode
+// tangent d(source * scale)/d[u, du/dxi] per face quadrature point.
+struct BIntegral {
+  QFParams                        qf;
+  std::vector<std::array<int, 2>> faces;
+  std::vector<double>             D;  // [face][q][nc][3 nc]
+};
+
 // Functional<test(trial)> with test == trial space, one trial argument.
 // Restates fn/functional.hpp:406-498 (operator(), ActionOfGradient) and
 // fn/domain_integral_kernels.hpp:155-304 (evaluation_kernel_impl,
@@ -62,6 +74,7 @@ struct Form {
   Basis1D               basis;
   std::vector<double>   Xq, Jq;
   std::vector<Integral> integrals;
+  std::vector<BIntegral> bintegrals;
   bool                  update_qdata = false;  // updateQdata(bool), fn/functional.hpp:523
 
   Form(const Mesh& m, const Space& s) : mesh(&m), space(s), Q(s.p + 1), basis(s.p + 1, s.p + 1)
@@ -91,6 +104,223 @@ struct Form {
         for (size_t k = 0; k < I.state.size(); k++) I.state[k] = state_init[k % size_t(I.ssize)];
     }
     integrals.push_back(std::move(I));
+  }
+
+  // boundary faces on the bounding-box planes chosen by face_mask (bit 2d:
+  // axis d at its minimum, bit 2d+1: at its maximum; 0: the whole boundary)
+  void add_boundary_integral(const QFParams& qf, int face_mask)
+  {
+    if (bqf_components(qf.kind) != nc()) throw std::runtime_error("boundary qfunction / space components");
+    std::map<std::array<int, 4>, int> count;
+    auto face_verts = [&](int e, int lf) {
+      std::array<int, 4> v;
+      int                k = 0;
+      for (int c = 0; c < 8; c++)
+        if (((c >> (lf / 2)) & 1) == lf % 2) v[size_t(k++)] = mesh->hex[size_t(e) * 8 + size_t(LEX2V[c])];
+      std::sort(v.begin(), v.end());
+      return v;
+    };
+    for (int e = 0; e < mesh->ne; e++)
+      for (int lf = 0; lf < 6; lf++) count[face_verts(e, lf)]++;
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int v = 0; v < mesh->nv; v++)
+      for (int d = 0; d < 3; d++) {
+        lo[d] = std::min(lo[d], mesh->X[size_t(3 * v + d)]);
+        hi[d] = std::max(hi[d], mesh->X[size_t(3 * v + d)]);
+      }
+    BIntegral B;
+    B.qf = qf;
+    for (int e = 0; e < mesh->ne; e++)
+      for (int lf = 0; lf < 6; lf++) {
+        auto v = face_verts(e, lf);
+        if (count[v] != 1) continue;
+        bool take = face_mask == 0;
+        for (int bit = 0; bit < 6 && !take; bit++) {
+          if (!((face_mask >> bit) & 1)) continue;
+          int    d     = bit / 2;
+          double plane = (bit & 1) ? hi[d] : lo[d], tol = 1e-12 * std::max(hi[d] - lo[d], 1.0);
+          bool   on    = true;
+          for (int x : v) on = on && std::fabs(mesh->X[size_t(3 * x + d)] - plane) <= tol;
+          take = on;
+        }
+        if (take) B.faces.push_back({e, lf});
+      }
+    bintegrals.push_back(std::move(B));
+  }
+
+  // Evaluates boundary face (e, lf) on element DOFs Ue -> Re (added); with D,
+  // also the tangent, or (jvp) the tangent applied to Ue.
+  //   mode 0: residual, 1: residual + tangent into D, 2: J.v with D
+  // The face is parametrized by its two tangential parent axes in increasing
+  // order, swapped when dX/dxi1 x dX/dxi2 does not point away from the
+  // element's centroid.
+  void face_eval(const QFParams& qf, int e, int lf, const double* Ue, double* Re, double* D, int mode) const
+  {
+    const int c = nc(), n = space.p + 1, Qn = Q;
+    const int d = lf / 2, side = lf % 2;
+    int       ax[2] = {d == 0 ? 1 : 0, d == 2 ? 1 : 2};
+    double    Xc[8][3];
+    for (int k = 0; k < 8; k++) {
+      const int v = mesh->hex[size_t(e) * 8 + size_t(LEX2V[k])];
+      for (int i = 0; i < 3; i++) Xc[k][i] = mesh->X[size_t(3 * v + i)];
+    }
+    // trilinear map and its parent derivatives at xi
+    auto geo = [&](const double xi[3], double X[3], double dXdxi[3][3]) {
+      for (int i = 0; i < 3; i++) {
+        X[i] = 0;
+        for (int j = 0; j < 3; j++) dXdxi[i][j] = 0;
+      }
+      for (int k = 0; k < 8; k++) {
+        double N[3], dN[3];
+        for (int j = 0; j < 3; j++) {
+          const int b = (k >> j) & 1;
+          N[j]        = b ? xi[j] : 1.0 - xi[j];
+          dN[j]       = b ? 1.0 : -1.0;
+        }
+        for (int i = 0; i < 3; i++) {
+          X[i] += Xc[k][i] * N[0] * N[1] * N[2];
+          dXdxi[i][0] += Xc[k][i] * dN[0] * N[1] * N[2];
+          dXdxi[i][1] += Xc[k][i] * N[0] * dN[1] * N[2];
+          dXdxi[i][2] += Xc[k][i] * N[0] * N[1] * dN[2];
+        }
+      }
+    };
+    {
+      double xi[3] = {0.5, 0.5, 0.5}, X0[3], Xf[3], J0[3][3], Jf[3][3];
+      geo(xi, X0, J0);
+      xi[d] = side;
+      geo(xi, Xf, Jf);
+      const double* a = nullptr;
+      (void)a;
+      double t1[3] = {Jf[0][ax[0]], Jf[1][ax[0]], Jf[2][ax[0]]}, t2[3] = {Jf[0][ax[1]], Jf[1][ax[1]], Jf[2][ax[1]]};
+      double nrm[3] = {t1[1] * t2[2] - t1[2] * t2[1], t1[2] * t2[0] - t1[0] * t2[2], t1[0] * t2[1] - t1[1] * t2[0]};
+      double out    = nrm[0] * (Xf[0] - X0[0]) + nrm[1] * (Xf[1] - X0[1]) + nrm[2] * (Xf[2] - X0[2]);
+      if (out < 0) std::swap(ax[0], ax[1]);
+    }
+    const int M = 3 * c;
+    for (int q2 = 0; q2 < Qn; q2++)
+      for (int q1 = 0; q1 < Qn; q1++) {
+        double xi[3];
+        xi[d]     = side;
+        xi[ax[0]] = basis.xq[size_t(q1)];
+        xi[ax[1]] = basis.xq[size_t(q2)];
+        double X[3], J[3][3];
+        geo(xi, X, J);
+        double dX[3][2];
+        for (int i = 0; i < 3; i++) dX[i][0] = J[i][ax[0]], dX[i][1] = J[i][ax[1]];
+        double cr[3] = {dX[1][0] * dX[2][1] - dX[2][0] * dX[1][1], dX[2][0] * dX[0][1] - dX[0][0] * dX[2][1],
+                        dX[0][0] * dX[1][1] - dX[1][0] * dX[0][1]};
+        const double scale = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+        const double w     = basis.wq[size_t(q1)] * basis.wq[size_t(q2)];
+        // element basis and its derivatives along the two face axes
+        std::vector<double> Bx(static_cast<size_t>(3 * n)), Gx(static_cast<size_t>(3 * n));
+        for (int j = 0; j < 3; j++) {
+          gll_basis(n, xi[j], &Bx[size_t(j * n)]);
+          gll_dbasis(n, xi[j], &Gx[size_t(j * n)]);
+        }
+        const int           npe = n * n * n;
+        std::vector<double> Nv(static_cast<size_t>(npe)), N1(static_cast<size_t>(npe)), N2(static_cast<size_t>(npe));
+        for (int iz = 0; iz < n; iz++)
+          for (int iy = 0; iy < n; iy++)
+            for (int ix = 0; ix < n; ix++) {
+              const int    i = (iz * n + iy) * n + ix;
+              const int    id[3] = {ix, iy, iz};
+              double       v = 1, g1 = 1, g2 = 1;
+              for (int j = 0; j < 3; j++) {
+                const double b = Bx[size_t(j * n + id[j])], g = Gx[size_t(j * n + id[j])];
+                v *= b;
+                g1 *= (j == ax[0]) ? g : b;
+                g2 *= (j == ax[1]) ? g : b;
+              }
+              Nv[size_t(i)] = v, N1[size_t(i)] = g1, N2[size_t(i)] = g2;
+            }
+        double in[9];
+        for (int cc = 0; cc < c; cc++) {
+          double v = 0, g1 = 0, g2 = 0;
+          for (int i = 0; i < npe; i++) {
+            const double U = Ue[size_t(cc) * npe + size_t(i)];
+            v += Nv[size_t(i)] * U;
+            g1 += N1[size_t(i)] * U;
+            g2 += N2[size_t(i)] * U;
+          }
+          in[cc] = v, in[c + 2 * cc] = g1, in[c + 2 * cc + 1] = g2;
+        }
+        double     out[3];
+        const int  q  = q2 * Qn + q1;
+        double*    Dq = D ? D + size_t(q) * size_t(c * M) : nullptr;
+        if (mode == 2) {
+          for (int r = 0; r < c; r++) {
+            out[r] = 0;
+            for (int s2 = 0; s2 < M; s2++) out[r] += Dq[r * M + s2] * in[s2];
+          }
+        } else {
+          double src[3];
+          bqf_eval(qf, X, dX, in, in + c, src);
+          for (int r = 0; r < c; r++) out[r] = src[r] * scale;
+          if (mode == 1) {
+            dual<9> din[9], ds[3];
+            for (int s2 = 0; s2 < M; s2++) {
+              din[s2]              = dual<9>(in[s2]);
+              din[s2].gradient[s2] = 1.0;
+            }
+            bqf_eval(qf, X, dX, din, din + c, ds);
+            for (int r = 0; r < c; r++)
+              for (int s2 = 0; s2 < M; s2++) Dq[r * M + s2] = (ds[r] * scale).gradient[size_t(s2)];
+          }
+        }
+        for (int cc = 0; cc < c; cc++)
+          for (int i = 0; i < npe; i++) Re[size_t(cc) * npe + size_t(i)] += w * Nv[size_t(i)] * out[cc];
+      }
+  }
+
+  void boundary_eval(const double* U, double* R, bool store_D)
+  {
+    const int           c = nc(), npe = space.npe, nq2 = Q * Q;
+    std::vector<double> Ue(static_cast<size_t>(c * npe)), Re(static_cast<size_t>(c * npe));
+    for (auto& B : bintegrals) {
+      if (store_D) B.D.assign(B.faces.size() * size_t(nq2) * size_t(c * 3 * c), 0.0);
+      for (size_t k = 0; k < B.faces.size(); k++) {
+        const int e = B.faces[k][0];
+        gather_elem(e, U, Ue.data());
+        std::fill(Re.begin(), Re.end(), 0.0);
+        face_eval(B.qf, e, B.faces[k][1], Ue.data(), Re.data(),
+                  store_D ? &B.D[k * size_t(nq2) * size_t(c * 3 * c)] : nullptr, store_D ? 1 : 0);
+        scatter_elem(e, Re.data(), R);
+      }
+    }
+  }
+
+  void boundary_jvp(const double* dU, double* dR) const
+  {
+    const int           c = nc(), npe = space.npe, nq2 = Q * Q;
+    std::vector<double> Ue(static_cast<size_t>(c * npe)), Re(static_cast<size_t>(c * npe));
+    for (auto& B : bintegrals) {
+      if (B.D.empty()) throw std::runtime_error("boundary jvp before linearize");
+      for (size_t k = 0; k < B.faces.size(); k++) {
+        const int e = B.faces[k][0];
+        gather_elem(e, dU, Ue.data());
+        std::fill(Re.begin(), Re.end(), 0.0);
+        face_eval(B.qf, e, B.faces[k][1], Ue.data(), Re.data(),
+                  const_cast<double*>(&B.D[k * size_t(nq2) * size_t(c * 3 * c)]), 2);
+        scatter_elem(e, Re.data(), dR);
+      }
+    }
+  }
+
+  // the boundary face's matrix over its element's DOFs [(c,i)][(c',j)]
+  void face_matrix(int bintegral, size_t k, double* K) const
+  {
+    const BIntegral& B  = bintegrals[size_t(bintegral)];
+    const int        c = nc(), npe = space.npe, nd = c * npe, nq2 = Q * Q;
+    std::vector<double> Ue(static_cast<size_t>(nd)), Re(static_cast<size_t>(nd));
+    for (int col = 0; col < nd; col++) {
+      std::fill(Ue.begin(), Ue.end(), 0.0);
+      Ue[size_t(col)] = 1.0;
+      std::fill(Re.begin(), Re.end(), 0.0);
+      face_eval(B.qf, B.faces[k][0], B.faces[k][1], Ue.data(), Re.data(),
+                const_cast<double*>(&B.D[k * size_t(nq2) * size_t(c * 3 * c)]), 2);
+      for (int row = 0; row < nd; row++) K[size_t(row) * nd + size_t(col)] = Re[size_t(row)];
+    }
   }
 
   void jac(int e, int q, double J[3][3]) const
@@ -234,6 +464,7 @@ struct Form {
         scatter_elem(e, Re.data(), R);
       }
     }
+    boundary_eval(U, R, store_D);
   }
 
   void jvp(const double* dU, double* dR) const
@@ -251,6 +482,7 @@ struct Form {
         scatter_elem(e, Re.data(), dR);
       }
     }
+    boundary_jvp(dU, dR);
   }
 
   // Element gradient K_e[(c,j) test][(c',J) trial], row-major, by applying the
@@ -280,6 +512,18 @@ struct Form {
       for (size_t k = 0; k < I.elems.size(); k++) {
         element_matrix(int(ii), k, K.data());
         int e = I.elems[k];
+        for (int i = 0; i < nc(); i++)
+          for (int j = 0; j < space.npe; j++) {
+            int r = i * space.npe + j;
+            diag[space.lidx(space.emap[size_t(e) * space.npe + j], i)] += K[size_t(r) * nd + r];
+          }
+      }
+    }
+    for (size_t bi = 0; bi < bintegrals.size(); bi++) {
+      const BIntegral& B = bintegrals[bi];
+      for (size_t k = 0; k < B.faces.size(); k++) {
+        face_matrix(int(bi), k, K.data());
+        int e = B.faces[k][0];
         for (int i = 0; i < nc(); i++)
           for (int j = 0; j < space.npe; j++) {
             int r = i * space.npe + j;

@@ -11,12 +11,14 @@ import ctypes
 import numpy as np
 
 from . import capi
-from .capi import (SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM, SX_QF_HEAT, SX_QF_J2_LINEAR, SX_QF_LINEAR_ELASTIC,
+from .capi import (SX_BQF_FLUX, SX_BQF_PRESSURE, SX_BQF_TRACTION, SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM,
+                   SX_QF_HEAT, SX_QF_J2_LINEAR, SX_QF_LINEAR_ELASTIC,
                    SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
 
 __all__ = [
     "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "SeracAmdError", "device_name",
     "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC", "SX_QF_J2_LINEAR",
+    "SX_BQF_TRACTION", "SX_BQF_PRESSURE", "SX_BQF_FLUX",
     "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
 ]
 
@@ -292,7 +294,7 @@ class Functional:
 
     def add_domain_integral(self, qfunc, params, depends_on=(0,), elems=None, qdata=None):
         """AddDomainIntegral; qdata (a QuadratureData) for qfunctions with state."""
-        p = np.zeros(8)
+        p = np.zeros(16)
         p[: len(params)] = params
         dep = (ctypes.c_int * len(depends_on))(*depends_on)
         if elems is None:
@@ -305,6 +307,15 @@ class Functional:
                   el, nel, qdata.h if qdata is not None else None)
         if qdata is not None:
             self._qdata = getattr(self, "_qdata", []) + [qdata]  # the form borrows it
+
+    def add_boundary_integral(self, qfunc, params, depends_on=(0,), face_mask=0):
+        """AddBoundaryIntegral over the bounding-box planes of face_mask
+        (bit 0 x = min, 1 x = max, 2 y = min, ...; 0: the whole boundary)."""
+        p = np.zeros(16)
+        p[: len(params)] = params
+        dep = (ctypes.c_int * len(depends_on))(*depends_on)
+        capi.call("sx_form_add_boundary_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on),
+                  face_mask)
 
     def update_qdata(self, flag):
         """updateQdata(bool): while set, evaluations commit the quadrature-point state."""

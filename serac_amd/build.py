@@ -23,6 +23,7 @@ SOURCES = [
     "sx_kernels_p3.hip",
     "sx_kernels_p4.hip",
     "sx_vector.hip",
+    "sx_boundary.hip",
     "sx_capi.cpp",
 ]
 

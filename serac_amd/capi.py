@@ -22,6 +22,7 @@ SX_ORDERING_BYVDIM = 1
 SX_OPT_SCATTER = 1
 SX_OPT_XCD_REMAP = 2
 SX_OPT_TANGENT_PRECISION = 3
+SX_OPT_JVP_ARITHMETIC = 4
 SX_PREC_F64 = 0
 SX_PREC_F32 = 1
 SX_PREC_BF16 = 2
@@ -33,6 +34,9 @@ SX_QF_MASS_DIFFUSION = 2
 SX_QF_NEOHOOKEAN = 3
 SX_QF_LINEAR_ELASTIC = 4
 SX_QF_J2_LINEAR = 5
+SX_BQF_TRACTION = 101
+SX_BQF_PRESSURE = 102
+SX_BQF_FLUX = 103
 
 _vp = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -97,6 +101,7 @@ SIGNATURES = {
     "sx_form_add_domain_integral_qdata": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                                   ctypes.POINTER(_int), _int, _vp]),
     "sx_form_update_qdata": (_int, [_vp, _int]),
+    "sx_form_add_boundary_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, _int]),
     "sx_grad_assemble": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_csr_info": (_int, [_vp, _I64, _I64, _I64]),
     "sx_csr_arrays": (_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),

@@ -5,16 +5,19 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
 
 #include "../../include/serac_amd.h"
+#include "sx_boundary.hpp"
 #include "sx_launch.hpp"
 #include "sx_setup.hpp"
 #include "sx_vector.hpp"
@@ -185,6 +188,20 @@ struct Integral {
   DeviceBuffer         d_qrow;           // [slot] -> element position in the domain
 };
 
+// AddBoundaryIntegral (fn/functional.hpp:328-354): faces of the selected
+// boundary, each as its element's node trace in outward (xi1, xi2) order.
+struct BoundaryIntegral {
+  int                  qf = 0, nc = 1;
+  double               params[SX_MAX_PARAMS] = {0};
+  std::vector<int>     depends_on;
+  int64_t              nslots = 0;
+  std::vector<int64_t> color_off;  // [ncolors + 1] slot ranges
+  DeviceBuffer         d_fmap;     // [slot][Q*Q]
+  DeviceBuffer         d_corners;  // [slot][4][3]
+  DeviceBuffer         d_tangent;  // [slot][nc * 3nc][Q*Q], FP64
+  bool                 has_tangent = false;
+};
+
 struct sx_grad {
   sx_form* form = nullptr;
   int      wrt  = 0;
@@ -195,12 +212,14 @@ struct sx_form {
   const sx_space*                        test = nullptr;
   std::vector<const sx_space*>           trials;
   std::vector<std::unique_ptr<Integral>> integrals;
+  std::vector<std::unique_ptr<BoundaryIntegral>> bintegrals;
   std::vector<std::unique_ptr<sx_grad>>  grads;
   hipStream_t                            stream = nullptr;
   int                                    scatter   = SX_SCATTER_COLORED;
   int                                    xcd_remap = 1;
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
+  int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -345,6 +364,144 @@ void build_csr_pattern(sx_space* s)
   }
 }
 
+// ---- boundary faces ----
+// Local face lf of a hex (parent axes x, y, z): the fixed axis and the
+// (xi1, xi2) parametrization whose dX/dxi1 x dX/dxi2 points out of a
+// positively oriented element.  Node (a, b) of the face, a along xi1, is the
+// element's lexicographic node (i0, i1, i2) below (n = nodes per direction).
+inline void face_node(int lf, int a, int b, int n, int& i0, int& i1, int& i2)
+{
+  const int m = n - 1;
+  switch (lf) {
+    case 0: i0 = 0, i1 = b, i2 = a; break;  // x = 0: (z, y)
+    case 1: i0 = m, i1 = a, i2 = b; break;  // x = 1: (y, z)
+    case 2: i0 = a, i1 = 0, i2 = b; break;  // y = 0: (x, z)
+    case 3: i0 = b, i1 = m, i2 = a; break;  // y = 1: (z, x)
+    case 4: i0 = b, i1 = a, i2 = 0; break;  // z = 0: (y, x)
+    default: i0 = a, i1 = b, i2 = m; break; // z = 1: (x, y)
+  }
+}
+
+struct BoundaryFace {
+  int e, lf;
+};
+
+// faces that belong to exactly one element, in (element, local face) order,
+// restricted by face_mask to the bounding-box planes (bit 0 x = min, 1 x = max,
+// 2 y = min, ...); face_mask 0 selects the whole boundary
+std::vector<BoundaryFace> boundary_faces(const sxh::HexMesh& m, int face_mask)
+{
+  struct Key {
+    std::array<int, 4> v;
+    int                e, lf;
+  };
+  std::vector<Key> keys;
+  keys.reserve(size_t(m.ne) * 6);
+  for (int64_t e = 0; e < m.ne; e++)
+    for (int lf = 0; lf < 6; lf++) {
+      Key k;
+      for (int c = 0; c < 4; c++) {
+        int i0, i1, i2;
+        face_node(lf, c & 1, c >> 1, 2, i0, i1, i2);
+        k.v[size_t(c)] = m.corner(e, i0 + 2 * i1 + 4 * i2);
+      }
+      std::sort(k.v.begin(), k.v.end());
+      k.e  = int(e);
+      k.lf = lf;
+      keys.push_back(k);
+    }
+  std::vector<size_t> order(keys.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return keys[a].v < keys[b].v; });
+  std::vector<char> single(keys.size(), 0);
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    while (j < order.size() && keys[order[j]].v == keys[order[i]].v) j++;
+    if (j - i == 1) single[order[i]] = 1;
+    i = j;
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t v = 0; v < m.nv; v++)
+    for (int d = 0; d < 3; d++) {
+      lo[d] = std::min(lo[d], m.X[size_t(3 * v + d)]);
+      hi[d] = std::max(hi[d], m.X[size_t(3 * v + d)]);
+    }
+  std::vector<BoundaryFace> out;
+  for (size_t i = 0; i < keys.size(); i++) {
+    if (!single[i]) continue;
+    bool take = (face_mask == 0);
+    for (int bit = 0; bit < 6 && !take; bit++) {
+      if (!(face_mask >> bit & 1)) continue;
+      const int    d = bit / 2;
+      const double plane = (bit & 1) ? hi[d] : lo[d], tol = 1e-12 * std::max(hi[d] - lo[d], 1.0);
+      bool         on = true;
+      for (int c = 0; c < 4; c++) on = on && std::fabs(m.X[size_t(3 * keys[i].v[size_t(c)] + d)] - plane) <= tol;
+      take = on;
+    }
+    if (take) out.push_back({keys[i].e, keys[i].lf});
+  }
+  return out;
+}
+
+bool is_boundary_qf(int qf) { return qf == SX_BQF_TRACTION || qf == SX_BQF_PRESSURE || qf == SX_BQF_FLUX; }
+int  bqf_components(int qf) { return qf == SX_BQF_FLUX ? 1 : 3; }
+
+sxk::FaceLaunch face_launch(const sx_form* f, const BoundaryIntegral& B, int mode)
+{
+  const sx_space* s = f->test;
+  sxk::FaceLaunch P{};
+  P.order    = s->order;
+  P.nc       = s->nc;
+  P.qf       = B.qf;
+  P.mode     = mode;
+  P.ordering = s->ordering;
+  P.fmap     = B.d_fmap.as<const int>();
+  P.corners  = B.d_corners.as<const double>();
+  P.nnodes   = s->nnodes;
+  P.D        = B.d_tangent.as<double>();
+  P.tables   = element_tables(s->order + 1);
+  for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = B.params[k];
+  P.stream = f->stream;
+  return P;
+}
+
+// one launch per face colour; returns the number of launches
+int run_boundary(const sx_form* f, const BoundaryIntegral& B, int mode, const void* u, void* r, hipEvent_t* ev,
+                 int vprec, const sx_csr* A = nullptr)
+{
+  sxk::FaceLaunch P = face_launch(f, B, mode);
+  P.u               = u;
+  P.r               = r;
+  P.vprec           = vprec == sxk::PREC_F32 ? 1 : 0;
+  if (A) {
+    P.csr_row = A->row.as<const int64_t>();
+    P.csr_col = A->col.as<const int32_t>();
+    P.csr_val = A->val.as<double>();
+  }
+  int launches = 0;
+  for (size_t c = 0; c + 1 < B.color_off.size(); c++) {
+    P.slot_begin = B.color_off[c];
+    P.slot_end   = B.color_off[c + 1];
+    if (P.slot_end <= P.slot_begin) continue;
+    if (ev) SX_HIP(hipEventRecord(ev[2 * launches], f->stream));
+    hipError_t err = hipSuccess;
+    SX_REQUIRE(sxk::launch_face(P, &err), SX_ERR_UNSUPPORTED,
+               "boundary kernel not compiled for order " + std::to_string(P.order) + ", qfunction " +
+                   std::to_string(P.qf));
+    SX_HIP(err);
+    if (ev) SX_HIP(hipEventRecord(ev[2 * launches + 1], f->stream));
+    launches++;
+  }
+  return launches;
+}
+
+bool depends(const BoundaryIntegral& B, int trial)
+{
+  for (int d : B.depends_on)
+    if (d == trial) return true;
+  return false;
+}
+
 bool depends(const Integral& I, int trial)
 {
   return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
@@ -362,10 +519,15 @@ void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullp
     SX_HIP(sxk::vec_fill(static_cast<double*>(df), 0.0, f->test->ndof(), f->stream));
   else
     SX_HIP(sxk::vec_fill_f32(static_cast<float*>(df), 0.0f, f->test->ndof(), f->stream));
-  int n = 0;
+  // FP64 vectors with FP32 element arithmetic when the form asks for it
+  const int kprec = (vprec == sxk::PREC_F64 && f->jvp_arith == sxk::PREC_F32) ? sxk::PREC_MIXED : vprec;
+  int       n     = 0;
   for (auto& I : f->integrals)
     if (depends(*I, g->wrt) && I->has_tangent)
-      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, vprec);
+      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, kprec);
+  for (auto& B : f->bintegrals)
+    if (depends(*B, g->wrt) && B->has_tangent)
+      n += run_boundary(f, *B, sxk::FMODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, vprec);
   if (nlaunch) *nlaunch = n;
 }
 
@@ -671,7 +833,7 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
   return api([&] {
     SX_REQUIRE(f, SX_ERR_ARG, "null form");
     SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
-               "qfunction parameters: at most 8 doubles");
+               "qfunction parameters: at most 16 doubles");
     const int nc_qf = qf_components(qfunc_id);
     SX_REQUIRE(nc_qf != 0, SX_ERR_UNSUPPORTED, "unknown qfunction id " + std::to_string(qfunc_id));
     SX_REQUIRE(ndep == 1 && depends_on, SX_ERR_UNSUPPORTED,
@@ -746,6 +908,82 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
     I->d_map.upload(map);
     I->d_coords.upload(xyz);
     f->integrals.push_back(std::move(I));
+  });
+}
+
+int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                  const int* depends_on, int ndep, int face_mask)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+               "qfunction parameters: at most 16 doubles");
+    SX_REQUIRE(is_boundary_qf(qfunc_id), SX_ERR_UNSUPPORTED, "unknown boundary qfunction id " + std::to_string(qfunc_id));
+    SX_REQUIRE(ndep == 1 && depends_on, SX_ERR_UNSUPPORTED,
+               "compiled boundary qfunctions take exactly one trial argument (DependsOn<i>)");
+    SX_REQUIRE(face_mask >= 0 && face_mask < 64, SX_ERR_ARG, "face_mask selects among 6 planes");
+    const int trial = depends_on[0];
+    SX_REQUIRE(trial >= 0 && trial < int(f->trials.size()), SX_ERR_ARG, "DependsOn index out of range");
+    const sx_space* vs = f->test;
+    SX_REQUIRE(f->trials[size_t(trial)] == vs || (f->trials[size_t(trial)]->mesh == vs->mesh &&
+                                                  f->trials[size_t(trial)]->order == vs->order &&
+                                                  f->trials[size_t(trial)]->nc == vs->nc &&
+                                                  f->trials[size_t(trial)]->ordering == vs->ordering),
+               SX_ERR_UNSUPPORTED, "test and trial spaces must match");
+    SX_REQUIRE(bqf_components(qfunc_id) == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
+    auto B = std::make_unique<BoundaryIntegral>();
+    B->qf  = qfunc_id;
+    B->nc  = vs->nc;
+    for (int k = 0; k < nparams; k++) B->params[k] = params[k];
+    B->depends_on = {trial};
+    const auto& m = vs->mesh->m;
+    const int   n = vs->order + 1, nf = n * n;
+    std::vector<BoundaryFace> faces = boundary_faces(m, face_mask);
+    // greedy colouring: faces of one colour share no node
+    std::vector<int> fnodes(faces.size() * size_t(nf));
+    for (size_t i = 0; i < faces.size(); i++)
+      for (int b = 0; b < n; b++)
+        for (int a = 0; a < n; a++) {
+          int i0, i1, i2;
+          face_node(faces[i].lf, a, b, n, i0, i1, i2);
+          fnodes[i * size_t(nf) + size_t(b * n + a)] =
+              vs->emap[size_t(faces[i].e) * size_t(vs->npe) + size_t((i2 * n + i1) * n + i0)];
+        }
+    std::vector<int>              color(faces.size(), -1);
+    std::vector<std::vector<int>> node_colors(size_t(vs->nnodes));
+    int                           ncolors = 0;
+    for (size_t i = 0; i < faces.size(); i++) {
+      int c = 0;
+      for (;; c++) {
+        bool used = false;
+        for (int k = 0; k < nf && !used; k++)
+          for (int x : node_colors[size_t(fnodes[i * size_t(nf) + size_t(k)])]) used = used || x == c;
+        if (!used) break;
+      }
+      color[i] = c;
+      ncolors  = std::max(ncolors, c + 1);
+      for (int k = 0; k < nf; k++) node_colors[size_t(fnodes[i * size_t(nf) + size_t(k)])].push_back(c);
+    }
+    std::vector<int>    map;
+    std::vector<double> corners;
+    B->color_off.assign(size_t(ncolors) + 1, 0);
+    for (int c = 0; c < ncolors; c++) {
+      for (size_t i = 0; i < faces.size(); i++) {
+        if (color[i] != c) continue;
+        map.insert(map.end(), fnodes.begin() + long(i * size_t(nf)), fnodes.begin() + long((i + 1) * size_t(nf)));
+        for (int k = 0; k < 4; k++) {
+          int i0, i1, i2;
+          face_node(faces[i].lf, k & 1, k >> 1, 2, i0, i1, i2);
+          const int v = m.corner(faces[i].e, i0 + 2 * i1 + 4 * i2);
+          for (int d = 0; d < 3; d++) corners.push_back(m.X[size_t(3 * v + d)]);
+        }
+      }
+      B->color_off[size_t(c) + 1] = int64_t(map.size() / size_t(nf));
+    }
+    B->nslots = int64_t(faces.size());
+    B->d_fmap.upload(map);
+    B->d_corners.upload(corners);
+    f->bintegrals.push_back(std::move(B));
   });
 }
 
@@ -842,6 +1080,16 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
       }
       run_integral(f, *I, lin ? sxk::MODE_LINEARIZE : sxk::MODE_RESIDUAL, inputs[I->depends_on[0]], out);
     }
+    for (auto& B : f->bintegrals) {
+      const bool lin = (wrt >= 0) && depends(*B, wrt);
+      if (lin) {
+        const int q2 = (f->test->order + 1) * (f->test->order + 1);
+        B->d_tangent.allocate(size_t(B->nslots) * size_t(B->nc) * size_t(3 * B->nc) * size_t(q2) * sizeof(double));
+        B->has_tangent = true;
+      }
+      run_boundary(f, *B, lin ? sxk::FMODE_LINEARIZE : sxk::FMODE_RESIDUAL, inputs[B->depends_on[0]], out, nullptr,
+                   sxk::PREC_F64);
+    }
     (void)t;
     if (wrt >= 0) {
       f->grads[size_t(wrt)]->valid = true;
@@ -867,6 +1115,10 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
         SX_REQUIRE(value == SX_PREC_F64 || value == SX_PREC_F32 || value == SX_PREC_BF16, SX_ERR_ARG,
                    "bad tangent precision");
         f->tprec = int(value);
+        return;
+      case SX_OPT_JVP_ARITHMETIC:
+        SX_REQUIRE(value == SX_PREC_F64 || value == SX_PREC_F32, SX_ERR_ARG, "J.v arithmetic is FP64 or FP32");
+        f->jvp_arith = int(value);
         return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
@@ -913,6 +1165,9 @@ int sx_grad_diagonal(sx_grad* g, double* diag)
     SX_HIP(sxk::vec_fill(diag, 0.0, f->test->ndof(), f->stream));
     for (auto& I : f->integrals)
       if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
+    for (auto& B : f->bintegrals)
+      if (depends(*B, g->wrt) && B->has_tangent)
+        run_boundary(f, *B, sxk::FMODE_DIAG, nullptr, diag, nullptr, sxk::PREC_F64);
   });
 }
 
@@ -949,6 +1204,9 @@ int sx_grad_assemble(sx_grad* g, sx_csr** out)
         SX_HIP(err);
       }
     }
+    for (auto& B : f->bintegrals)
+      if (depends(*B, g->wrt) && B->has_tangent)
+        run_boundary(f, *B, sxk::FMODE_ASSEMBLE, nullptr, nullptr, nullptr, sxk::PREC_F64, A.get());
     SX_HIP(hipStreamSynchronize(f->stream));
     *out = A.release();
   });
@@ -1114,6 +1372,9 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   SX_HIP(sxk::vec_fill(diag, 0.0, n, s));
   for (auto& I : f->integrals)
     if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
+  for (auto& B : f->bintegrals)
+    if (depends(*B, g->wrt) && B->has_tangent)
+      run_boundary(f, *B, sxk::FMODE_DIAG, nullptr, diag, nullptr, sxk::PREC_F64);
   SX_HIP(sxk::set_rows(diag, ess_dev, ness, 1.0, s));
 
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));

@@ -21,8 +21,9 @@
 // (one launch for all elements).
 //
 // Precision: residual and linearization compute in FP64.  The stored tangent
-// is kept in S = double, float or bf16; the J.v kernel computes and exchanges
-// vectors in T = double or float.
+// is kept in S = double, float or bf16; the J.v kernel computes in T = double
+// or float and exchanges vectors in V = double or float (float arithmetic on
+// double vectors is the operator of the mixed-precision solvers).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -537,7 +538,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
 // J.v with the stored tangent (fn/domain_integral_kernels.hpp:223-304): vectors
 // and arithmetic in T, tangent stored in S
 // ---------------------------------------------------------------------------
-template <int N, int NC, int LAYOUT, typename T, typename S>
+template <int N, int NC, int LAYOUT, typename T, typename S, typename V>
 __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
 {
   constexpr int  NQ   = N * N * N;
@@ -563,8 +564,8 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
   const bool    active = (el < EB) && (slot < P.slot_end);
   T*            S_     = smem + (el < EB ? el : 0) * PER;
-  const T*      u      = static_cast<const T*>(P.u);
-  T*            r      = static_cast<T*>(P.r);
+  const V*      u      = static_cast<const V*>(P.u);
+  V*            r      = static_cast<V*>(P.r);
 
   for (int k = tid; k < TAB; k += blockDim.x) stab[k] = T(P.tables[k]);
   const T* tB  = stab;
@@ -580,7 +581,7 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   if (active) {
     node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-    for (int c = 0; c < NC; c++) ug[c] = u[lidx(node, c, NC, P.nnodes, P.ordering)];
+    for (int c = 0; c < NC; c++) ug[c] = T(u[lidx(node, c, NC, P.nnodes, P.ordering)]);
   }
   T Dr[DS];
   if (active) load_tangent<T, S, DS, NQ>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li, Dr);
@@ -724,7 +725,7 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
         const int idx = (qz * N + i1) * N + i0;
         acc += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
       }
-      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc, P.scatter);
+      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), V(acc), P.scatter);
     }
   }
 }
@@ -948,11 +949,15 @@ inline int dispatch_gradient(const ElemLaunch& P, hipError_t* err)
     return 1;
   }
   if (P.vprec == PREC_F64) {
-    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, double, S>);
+    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, double, S, double>);
     return 1;
   }
   if (P.vprec == PREC_F32) {
-    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S>);
+    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, float>);
+    return 1;
+  }
+  if (P.vprec == PREC_MIXED) {
+    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, double>);
     return 1;
   }
   return 0;

@@ -17,8 +17,9 @@ enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3, 
 //              (J2 plasticity: 81)
 enum Layout { LAYOUT_GRAD_SYM = 0, LAYOUT_FULL_SYM = 1, LAYOUT_GRAD_FULL = 2 };
 
-// precisions of the stored tangent (tprec) and of J.v vectors (vprec)
-enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2 };
+// precisions of the stored tangent (tprec) and of J.v vectors (vprec);
+// PREC_MIXED (J.v only): FP64 vectors, FP32 element arithmetic
+enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_MIXED = 3 };
 
 enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_ID_LINEAR_ELASTIC = 4, QF_ID_J2 = 5 };
 
@@ -42,7 +43,7 @@ struct ElemLaunch {
   const int64_t* csr_row;      // assembly: CSR row offsets (test DOFs)
   const int32_t* csr_col;      // assembly: sorted column ids per row (trial DOFs)
   double*        csr_val;      // assembly: values, accumulated into
-  double        params[8];
+  double        params[16];
   double        t;
   hipStream_t   stream;
 };

@@ -8,6 +8,7 @@
 //   - heat integrands (ThermalMaterialIntegrand + ConstantSource)
 //   - EquationSolver Newton + Jacobi-CG on a 1% compression step
 //   - Gradient::assemble: CSR SpMV against the matrix-free action
+//   - AddSurfaceIntegral: the traction patch test (P(A) . N on the max faces)
 //   - J2 plasticity with QuadratureData: residual, J.v and the state
 //     updateQdata commits, against the oracle (1e-12)
 //   - error paths: unsupported configurations raise serac_amd::Error
@@ -98,7 +99,7 @@ int main()
   orc_mesh_perturb(om, 42, 0.2);
   void*        os = orc_space(om, p, 3, 1);
   void*        of = orc_form(om, os);
-  const double prm[8] = {1.0, K, G, 0, 0, 0, 0, 0};
+  const double prm[16] = {1.0, K, G};
   orc_form_add_integral(of, 3, prm, nullptr, 0);
   std::vector<double> R_ref(static_cast<size_t>(ndof)), J_ref(static_cast<size_t>(ndof));
   orc_form_eval(of, U.data(), R_ref.data(), 1);
@@ -173,6 +174,41 @@ int main()
     CHECK(interior < 1e-13, "affine patch interior residual %.2e", interior);
   }
 
+  // ---- traction patch test (solid_statics_patch.cpp): the affine solution
+  //      with traction P(A) . N on the max faces zeroes every non-Dirichlet row
+  {
+    const double Ap[3][3] = {{0.110791568544027, 0.230421268325901, 0.15167673653354},
+                             {0.198344644470483, 0.060514559793513, 0.084137393813728},
+                             {0.011544253485023, 0.060942846497753, 0.186383473579596}};
+    const double bp[3]    = {0.765645367640828, 0.992487355850465, 0.162199373722092};
+    double       val[3] = {0, 0, 0}, flux[9];
+    const double nh[16] = {1.0, 1.0, 1.0};
+    orc_qf_eval(3, nh, 3, val, &Ap[0][0], val, flux);
+    solid_mechanics::LinearTraction trac{};
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) trac.A[i][j] = flux[i * 3 + j];
+    Functional<space(space)> patch(&fes, {&fes});
+    patch.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                            solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                {1.0, 1.0, 1.0}, GeometricNonlinearities::On),
+                            mesh);
+    patch.AddSurfaceIntegral(DependsOn<0>{}, trac, BoundaryDomain::Planes(0b101010));
+    std::vector<double> x = fes.NodeCoordinates(), Ua(static_cast<size_t>(ndof));
+    for (int64_t nd = 0; nd < fes.GetNDofs(); nd++)
+      for (int i = 0; i < 3; i++)
+        Ua[size_t(nd * 3 + i)] = Ap[i][0] * x[size_t(nd * 3)] + Ap[i][1] * x[size_t(nd * 3 + 1)] +
+                                 Ap[i][2] * x[size_t(nd * 3 + 2)] + bp[i];
+    std::vector<double> Ra = patch(0.0, Vector(Ua)).ToHost();
+    double              worst = 0;
+    for (int64_t nd = 0; nd < fes.GetNDofs(); nd++) {
+      bool dirichlet = false;
+      for (int d = 0; d < 3; d++) dirichlet = dirichlet || x[size_t(nd * 3 + d)] < 1e-12;
+      if (!dirichlet)
+        for (int i = 0; i < 3; i++) worst = std::fmax(worst, std::fabs(Ra[size_t(nd * 3 + i)]));
+    }
+    CHECK(worst < 1e-13, "traction patch (AddSurfaceIntegral) non-Dirichlet residual %.2e", worst);
+  }
+
   // ---- heat: conduction + constant source
   {
     FiniteElementSpace       tfes(mesh, 2, 1);
@@ -225,7 +261,7 @@ int main()
       Us[size_t(3 * i + 1)] = 0.002 * xyz[size_t(3 * i)] * xyz[size_t(3 * i + 2)];
     }
     void*        oj     = orc_form(om, os);
-    const double jp[8]  = {200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0, 0};
+    const double jp[16] = {200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0};
     const double st0[10] = {0};
     orc_form_add_integral_state(oj, 5, jp, nullptr, 0, st0);
     std::vector<double> Rj(static_cast<size_t>(ndof)), Jj(static_cast<size_t>(ndof));
