@@ -265,6 +265,26 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
                          int nsteps, int warm_start, const sx_solver_options* opts, double* x,
                          sx_solve_report* reports);
 
+/* ---- learned coarse-grid preconditioner: the U-Net (no reference counterpart) ----
+ * A 4-level 3D convolutional U-Net (16 -> 32 -> 64 -> 64 channels, 3x3x3
+ * kernels, BF16 weights and activations, FP32 accumulation) on a vertex grid
+ * of nx x ny x nz nodes (each 2^3 m + 1), mapping 3 channels to 3 channels,
+ * channels-last [z][y][x][3].  Parameters: per layer [27][cin][cout] weights
+ * then [cout] biases, FP32 masters. */
+typedef struct sx_unet sx_unet;
+int sx_unet_create(int nx, int ny, int nz, uint64_t seed, sx_unet** out);
+int sx_unet_info(const sx_unet* net, int64_t* nparam, int64_t* nvox);
+int sx_unet_get_params(const sx_unet* net, float* host);
+int sx_unet_set_params(sx_unet* net, const float* host);
+int sx_unet_get_grads(const sx_unet* net, float* host);
+/* y = N(x); device FP32 [nvox][3] */
+int sx_unet_forward(sx_unet* net, const float* x, float* y);
+/* forward + backward of loss = |N(x) - t|^2 / |t|^2; gradients accumulate */
+int sx_unet_backward(sx_unet* net, const float* x, const float* t, double* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the accumulated gradients, which it clears */
+int sx_unet_adam(sx_unet* net, double lr);
+int sx_unet_destroy(sx_unet* net);
+
 /* ---- vector kernels used by the solvers, exposed for callers ---- */
 int sx_vec_dot(const double* a, const double* b, int64_t n, double* result); /* synchronous, deterministic */
 int sx_vec_axpy(double alpha, const double* x, double* y, int64_t n);        /* y += alpha x */

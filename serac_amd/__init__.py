@@ -16,7 +16,7 @@ from .capi import (SX_BQF_FLUX, SX_BQF_PRESSURE, SX_BQF_TRACTION, SX_ORDERING_BY
                    SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
 
 __all__ = [
-    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "SeracAmdError", "device_name",
+    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "UNet", "SeracAmdError", "device_name",
     "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC", "SX_QF_J2_LINEAR",
     "SX_BQF_TRACTION", "SX_BQF_PRESSURE", "SX_BQF_FLUX",
     "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
@@ -246,8 +246,63 @@ class CSRMatrix:
         capi.call("sx_csr_spmv", self.h, x.ptr, y.ptr)
 
     def __del__(self):
-        if getattr(self, "h", None):
-            capi.load().sx_csr_destroy(self.h)
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_csr_destroy(self.h)
+            self.h = None
+
+
+class UNet:
+    """The learned preconditioner's U-Net on an nx x ny x nz vertex grid (3 -> 3 channels)."""
+
+    LAYERS = [(3, 0, 16), (16, 0, 16), (16, 0, 32), (32, 0, 32), (32, 0, 64), (64, 0, 64), (64, 0, 64),
+              (64, 0, 64), (64, 64, 64), (64, 32, 32), (32, 16, 16), (16, 0, 3)]
+
+    def __init__(self, nx, ny, nz, seed=0):
+        h = _vp()
+        capi.call("sx_unet_create", nx, ny, nz, seed, ctypes.byref(h))
+        self.h = h
+        self.dims = (nx, ny, nz)
+        npar, nvox = ctypes.c_int64(), ctypes.c_int64()
+        capi.call("sx_unet_info", self.h, ctypes.byref(npar), ctypes.byref(nvox))
+        self.nparam, self.nvox = npar.value, nvox.value
+
+    def _fptr(self, a):
+        return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+    def params(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        capi.call("sx_unet_get_params", self.h, self._fptr(out))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        capi.call("sx_unet_set_params", self.h, self._fptr(p))
+
+    def grads(self):
+        out = np.zeros(self.nparam, dtype=np.float32)
+        capi.call("sx_unet_get_grads", self.h, self._fptr(out))
+        return out
+
+    def forward(self, x):
+        """x: [nvox, 3] (numpy or float32 DeviceVector) -> numpy [nvox, 3]"""
+        xd = x if isinstance(x, DeviceVector) else DeviceVector.from_numpy(np.asarray(x).ravel(), dtype=np.float32)
+        yd = DeviceVector(self.nvox * 3, dtype=np.float32)
+        capi.call("sx_unet_forward", self.h, xd.ptr, yd.ptr)
+        return yd.numpy().reshape(self.nvox, 3)
+
+    def backward(self, x, t):
+        xd = x if isinstance(x, DeviceVector) else DeviceVector.from_numpy(np.asarray(x).ravel(), dtype=np.float32)
+        td = t if isinstance(t, DeviceVector) else DeviceVector.from_numpy(np.asarray(t).ravel(), dtype=np.float32)
+        loss = ctypes.c_double()
+        capi.call("sx_unet_backward", self.h, xd.ptr, td.ptr, ctypes.byref(loss))
+        return loss.value
+
+    def adam(self, lr):
+        capi.call("sx_unet_adam", self.h, lr)
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_unet_destroy(self.h)
             self.h = None
 
 
@@ -273,8 +328,8 @@ class QuadratureData:
         capi.call("sx_qdata_set", self.h, _dptr(st))
 
     def __del__(self):
-        if getattr(self, "h", None):
-            capi.load().sx_qdata_destroy(self.h)
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_qdata_destroy(self.h)
             self.h = None
 
 

@@ -101,6 +101,15 @@ SIGNATURES = {
     "sx_form_add_domain_integral_qdata": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                                   ctypes.POINTER(_int), _int, _vp]),
     "sx_form_update_qdata": (_int, [_vp, _int]),
+    "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "sx_unet_info": (_int, [_vp, _I64, _I64]),
+    "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "sx_unet_set_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "sx_unet_get_grads": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),
+    "sx_unet_forward": (_int, [_vp, _vp, _vp]),
+    "sx_unet_backward": (_int, [_vp, _vp, _vp, _D]),
+    "sx_unet_adam": (_int, [_vp, ctypes.c_double]),
+    "sx_unet_destroy": (_int, [_vp]),
     "sx_form_add_boundary_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, _int]),
     "sx_grad_assemble": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_csr_info": (_int, [_vp, _I64, _I64, _I64]),

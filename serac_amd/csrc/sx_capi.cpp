@@ -31,6 +31,15 @@ struct sx_failure : std::runtime_error {
 
 thread_local std::string g_last_error;
 
+}  // namespace
+
+// the message sx_last_error() returns, set by other translation units of the library
+namespace sxh {
+void set_last_error(const std::string& m) { g_last_error = m; }
+}  // namespace sxh
+
+namespace {
+
 #define SX_HIP(call)                                                                                         \
   do {                                                                                                       \
     hipError_t sx_e_ = (call);                                                                               \

@@ -1,0 +1,414 @@
+// sx_unet.hip -- the learned coarse-grid preconditioner: a 4-level 3D
+// convolutional U-Net (16 -> 32 -> 64 -> 64 channels, 3x3x3 kernels, BF16
+// weights and activations, FP32 accumulation) on the vertex grid of a
+// structured hex mesh, with its training kernels (backward, Adam) and the
+// coupling to the order-p nodal residual.
+//
+// Tensors are channels-last, [z][y][x][c], on vertex-centred grids: level
+// l + 1 has (n_l - 1) / 2 + 1 nodes per direction and its node i sits on node
+// 2i of level l.  Every convolution is 3x3x3 with zero padding; the
+// encoder's down convolutions have stride 2, the decoder prolongs (trilinear,
+// vertex-centred) and convolves the concatenation [prolonged, skip].
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sx_unet.hpp"
+
+namespace sxu {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
+__device__ __forceinline__ uint16_t f2bf(float f)
+{
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return uint16_t((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  return uint16_t((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float ld(const uint16_t* p, int64_t i) { return bf2f(p[i]); }
+__device__ __forceinline__ float ld(const float* p, int64_t i) { return p[i]; }
+__device__ __forceinline__ void st(uint16_t* p, int64_t i, float v) { p[i] = f2bf(v); }
+__device__ __forceinline__ void st(float* p, int64_t i, float v) { p[i] = v; }
+
+inline unsigned blocks_for(int64_t n) { return unsigned((n + kThreads - 1) / kThreads); }
+
+// source node of output node o along one direction for tap offset d in {-1,0,1}
+//   mode 0: o + d; mode 1: 2o + d; mode 2: (o + d) / 2 when o + d is even
+__device__ __forceinline__ int src_index(int o, int d, int mode, int n_in)
+{
+  int s;
+  if (mode == 0)
+    s = o + d;
+  else if (mode == 1)
+    s = 2 * o + d;
+  else {
+    const int t = o + d;
+    if (t & 1) return -1;
+    s = t / 2;
+  }
+  return (s >= 0 && s < n_in) ? s : -1;
+}
+
+// y[o][co] = bias[co] + sum_{tap, ci} w[tap][ci][co] x[src(o, tap)][ci],
+// x = [xa (ca channels), xb (cb channels)]; optional ReLU.  One thread per
+// output node; the tap's weights are staged in LDS.
+template <int COUT, typename TI, typename TO>
+__global__ void __launch_bounds__(kThreads) k_conv(ConvArgs A)
+{
+  extern __shared__ uint16_t wl[];  // [ca + cb][COUT] of the current tap
+  const int     cin = A.ca + A.cb;
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int64_t o    = int64_t(blockIdx.x) * kThreads + threadIdx.x;
+  const bool    act  = o < nout;
+  int           ox = 0, oy = 0, oz = 0;
+  if (act) {
+    ox = int(o % A.dout[0]);
+    oy = int((o / A.dout[0]) % A.dout[1]);
+    oz = int(o / (int64_t(A.dout[0]) * A.dout[1]));
+  }
+  float acc[COUT];
+#pragma unroll
+  for (int c = 0; c < COUT; c++) acc[c] = 0.0f;
+  const uint16_t* W  = static_cast<const uint16_t*>(A.w);
+  const TI*       xa = static_cast<const TI*>(A.xa);
+  const TI*       xb = static_cast<const TI*>(A.xb);
+  for (int tap = 0; tap < 27; tap++) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < cin * COUT; t += kThreads) wl[t] = W[int64_t(tap) * cin * COUT + t];
+    __syncthreads();
+    if (!act) continue;
+    const int sx = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
+    const int sy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
+    const int sz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
+    if (sx < 0 || sy < 0 || sz < 0) continue;
+    const int64_t s = (int64_t(sz) * A.din[1] + sy) * A.din[0] + sx;
+    for (int ci = 0; ci < A.ca; ci++) {
+      const float     v  = ld(xa, s * A.ca + ci);
+      const uint16_t* wr = wl + ci * COUT;
+#pragma unroll
+      for (int c = 0; c < COUT; c++) acc[c] += v * bf2f(wr[c]);
+    }
+    for (int ci = 0; ci < A.cb; ci++) {
+      const float     v  = ld(xb, s * A.cb + ci);
+      const uint16_t* wr = wl + (A.ca + ci) * COUT;
+#pragma unroll
+      for (int c = 0; c < COUT; c++) acc[c] += v * bf2f(wr[c]);
+    }
+  }
+  if (!act) return;
+  TO* y = static_cast<TO*>(A.y);
+#pragma unroll
+  for (int c = 0; c < COUT; c++) {
+    float v = acc[c] + (A.bias ? A.bias[c] : 0.0f);
+    if (A.relu) v = v > 0.0f ? v : 0.0f;
+    st(y, o * COUT + c, v);
+  }
+}
+
+// dw[tap][ci][co] += sum_o x[src(o, tap)][ci] g[o][co] (x = [xa, xb]) and, for
+// tap 0, db[co] += sum_o g[o][co].  grid = (ceil(nout / 64), 27).
+template <int COUT, typename TI>
+__global__ void __launch_bounds__(kThreads) k_conv_wgrad(WGradArgs A)
+{
+  constexpr int VC = 64;
+  __shared__ float sg[VC * COUT];
+  __shared__ float sx[VC * 128];
+  const int     cin  = A.ca + A.cb;
+  const int     tap  = blockIdx.y;
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int64_t o0   = int64_t(blockIdx.x) * VC;
+  const TI*     xa   = static_cast<const TI*>(A.xa);
+  const TI*     xb   = static_cast<const TI*>(A.xb);
+  for (int t = threadIdx.x; t < VC * COUT; t += kThreads) {
+    const int64_t o = o0 + t / COUT;
+    sg[t]           = o < nout ? A.g[o * COUT + t % COUT] : 0.0f;
+  }
+  for (int j = threadIdx.x; j < VC; j += kThreads) {
+    const int64_t o  = o0 + j;
+    int64_t       s  = -1;
+    if (o < nout) {
+      const int ox = int(o % A.dout[0]), oy = int((o / A.dout[0]) % A.dout[1]);
+      const int oz = int(o / (int64_t(A.dout[0]) * A.dout[1]));
+      const int ix = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
+      const int iy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
+      const int iz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
+      if (ix >= 0 && iy >= 0 && iz >= 0) s = (int64_t(iz) * A.din[1] + iy) * A.din[0] + ix;
+    }
+    for (int ci = 0; ci < A.ca; ci++) sx[j * 128 + ci] = s >= 0 ? ld(xa, s * A.ca + ci) : 0.0f;
+    for (int ci = 0; ci < A.cb; ci++) sx[j * 128 + A.ca + ci] = s >= 0 ? ld(xb, s * A.cb + ci) : 0.0f;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < cin * COUT; e += kThreads) {
+    const int ci = e / COUT, co = e % COUT;
+    float     s  = 0.0f;
+    for (int j = 0; j < VC; j++) s += sx[j * 128 + ci] * sg[j * COUT + co];
+    atomicAdd(&A.dw[(int64_t(tap) * cin + ci) * COUT + co], s);
+  }
+  if (tap == 0 && A.db)
+    for (int co = threadIdx.x; co < COUT; co += kThreads) {
+      float s = 0.0f;
+      for (int j = 0; j < VC; j++) s += sg[j * COUT + co];
+      atomicAdd(&A.db[co], s);
+    }
+}
+
+// wt[tap'][co][ci] = w[26 - tap'][c0 + ci][co] for ci < cpart: the weights of
+// the data-gradient convolution of input part [c0, c0 + cpart)
+__global__ void k_flip_weights(const uint16_t* w, int cin, int cout, int c0, int cpart, uint16_t* wt)
+{
+  const int64_t n = int64_t(27) * cout * cpart;
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    const int ci = int(t % cpart), co = int((t / cpart) % cout), tp = int(t / (int64_t(cpart) * cout));
+    wt[t]        = w[(int64_t(26 - tp) * cin + c0 + ci) * cout + co];
+  }
+}
+
+// g[i] *= (y[i] > 0): the ReLU derivative from the activation's output
+__global__ void k_relu_bwd(float* g, const uint16_t* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    if (!(bf2f(y[i]) > 0.0f)) g[i] = 0.0f;
+}
+
+// vertex-centred trilinear prolongation, coarse nc^3 -> fine nf^3, C channels:
+// fine node f takes coarse f/2 when f is even and the mean of (f-1)/2 and
+// (f+1)/2 when odd, per direction
+template <typename TI, typename TO>
+__global__ void k_prolong(const TI* xc, Dims3 dc, TO* yf, Dims3 df, int C, int accumulate)
+{
+  const int64_t n = int64_t(df.n[0]) * df.n[1] * df.n[2] * C;
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % C);
+    const int64_t f = t / C;
+    const int     fx = int(f % df.n[0]), fy = int((f / df.n[0]) % df.n[1]), fz = int(f / (int64_t(df.n[0]) * df.n[1]));
+    int           lo[3], hi[3];
+    const int     fv[3] = {fx, fy, fz};
+    float         w[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d] = fv[d] / 2;
+      hi[d] = (fv[d] & 1) ? lo[d] + 1 : lo[d];
+      w[d]  = (fv[d] & 1) ? 0.5f : 1.0f;
+    }
+    float s = 0.0f;
+    for (int k = 0; k < 8; k++) {
+      const int bx = k & 1, by = (k >> 1) & 1, bz = k >> 2;
+      if ((bx && hi[0] == lo[0]) || (by && hi[1] == lo[1]) || (bz && hi[2] == lo[2])) continue;
+      const int   cx = bx ? hi[0] : lo[0], cy = by ? hi[1] : lo[1], cz = bz ? hi[2] : lo[2];
+      const float ww = w[0] * w[1] * w[2];
+      s += ww * ld(xc, ((int64_t(cz) * dc.n[1] + cy) * dc.n[0] + cx) * C + c);
+    }
+    if (accumulate)
+      st(yf, t, ld(yf, t) + s);
+    else
+      st(yf, t, s);
+  }
+}
+
+// transpose of k_prolong: coarse node o gathers fine nodes 2o + d, d in
+// {-1, 0, 1}, with weights {0.5, 1, 0.5} per direction
+__global__ void k_restrict(const float* gf, Dims3 df, float* gc, Dims3 dc, int C)
+{
+  const int64_t n = int64_t(dc.n[0]) * dc.n[1] * dc.n[2] * C;
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % C);
+    const int64_t o = t / C;
+    const int     ov[3] = {int(o % dc.n[0]), int((o / dc.n[0]) % dc.n[1]), int(o / (int64_t(dc.n[0]) * dc.n[1]))};
+    float         s     = 0.0f;
+    for (int dz = -1; dz <= 1; dz++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dx = -1; dx <= 1; dx++) {
+          const int fx = 2 * ov[0] + dx, fy = 2 * ov[1] + dy, fz = 2 * ov[2] + dz;
+          if (fx < 0 || fy < 0 || fz < 0 || fx >= df.n[0] || fy >= df.n[1] || fz >= df.n[2]) continue;
+          const float w = (dx ? 0.5f : 1.0f) * (dy ? 0.5f : 1.0f) * (dz ? 0.5f : 1.0f);
+          s += w * gf[((int64_t(fz) * df.n[1] + fy) * df.n[0] + fx) * C + c];
+        }
+    gc[t] = s;
+  }
+}
+
+__global__ void k_fill_f(float* x, float v, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) x[i] = v;
+}
+__global__ void k_add_f(float* y, const float* x, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) y[i] += x[i];
+}
+
+// relative squared error: sums[0] += (y - t)^2, sums[1] += t^2 (FP64)
+__global__ void k_loss_sums(const float* y, const float* t, int64_t n, double* sums)
+{
+  __shared__ double s0[kThreads], s1[kThreads];
+  double a = 0.0, b = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const double d = double(y[i]) - double(t[i]);
+    a += d * d;
+    b += double(t[i]) * double(t[i]);
+  }
+  s0[threadIdx.x] = a;
+  s1[threadIdx.x] = b;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) {
+      s0[threadIdx.x] += s0[threadIdx.x + w];
+      s1[threadIdx.x] += s1[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    atomicAdd(&sums[0], s0[0]);
+    atomicAdd(&sums[1], s1[0]);
+  }
+}
+// dy = 2 (y - t) / sum t^2
+__global__ void k_loss_grad(const float* y, const float* t, int64_t n, const double* sums, float* dy)
+{
+  const double inv = sums[1] > 0 ? 2.0 / sums[1] : 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    dy[i] = float((double(y[i]) - double(t[i])) * inv);
+}
+
+// Adam on FP32 master weights; refreshes the BF16 copy and clears the gradient
+__global__ void k_adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.0f - b1) * gi;
+    const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i]           = mi;
+    v[i]           = vi;
+    const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    w[i]           = wi;
+    if (wbf) wbf[i] = f2bf(wi);
+    g[i] = 0.0f;
+  }
+}
+
+__global__ void k_to_bf16(const float* x, uint16_t* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = f2bf(x[i]);
+}
+
+template <int COUT>
+hipError_t launch_conv_t(const ConvArgs& A, bool in_f32, bool out_f32, hipStream_t s)
+{
+  const int64_t nout  = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const size_t  shmem = size_t(A.ca + A.cb) * COUT * sizeof(uint16_t);
+  const dim3    grid{blocks_for(nout)}, block{unsigned(kThreads)};
+  if (in_f32 && out_f32)
+    hipLaunchKernelGGL((k_conv<COUT, float, float>), grid, block, shmem, s, A);
+  else if (in_f32)
+    hipLaunchKernelGGL((k_conv<COUT, float, uint16_t>), grid, block, shmem, s, A);
+  else if (out_f32)
+    hipLaunchKernelGGL((k_conv<COUT, uint16_t, float>), grid, block, shmem, s, A);
+  else
+    hipLaunchKernelGGL((k_conv<COUT, uint16_t, uint16_t>), grid, block, shmem, s, A);
+  return hipGetLastError();
+}
+
+template <int COUT>
+hipError_t launch_wgrad_t(const WGradArgs& A, bool in_f32, hipStream_t s)
+{
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const dim3    grid{unsigned((nout + 63) / 64), 27u}, block{unsigned(kThreads)};
+  if (in_f32)
+    hipLaunchKernelGGL((k_conv_wgrad<COUT, float>), grid, block, 0, s, A);
+  else
+    hipLaunchKernelGGL((k_conv_wgrad<COUT, uint16_t>), grid, block, 0, s, A);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t conv(const ConvArgs& A, int cout, bool in_f32, bool out_f32, hipStream_t s)
+{
+  if (A.ca + A.cb > 128) return hipErrorInvalidValue;
+  switch (cout) {
+    case 3: return launch_conv_t<3>(A, in_f32, out_f32, s);
+    case 16: return launch_conv_t<16>(A, in_f32, out_f32, s);
+    case 32: return launch_conv_t<32>(A, in_f32, out_f32, s);
+    case 64: return launch_conv_t<64>(A, in_f32, out_f32, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t conv_wgrad(const WGradArgs& A, int cout, bool in_f32, hipStream_t s)
+{
+  if (A.ca + A.cb > 128) return hipErrorInvalidValue;
+  switch (cout) {
+    case 3: return launch_wgrad_t<3>(A, in_f32, s);
+    case 16: return launch_wgrad_t<16>(A, in_f32, s);
+    case 32: return launch_wgrad_t<32>(A, in_f32, s);
+    case 64: return launch_wgrad_t<64>(A, in_f32, s);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t flip_weights(const uint16_t* w, int cin, int cout, int c0, int cpart, uint16_t* wt, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_flip_weights, dim3(256), dim3(kThreads), 0, s, w, cin, cout, c0, cpart, wt);
+  return hipGetLastError();
+}
+hipError_t relu_bwd(float* g, const uint16_t* y, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_relu_bwd, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(kThreads), 0, s, g, y, n);
+  return hipGetLastError();
+}
+hipError_t prolong_bf16(const uint16_t* xc, Dims3 dc, uint16_t* yf, Dims3 df, int C, hipStream_t s)
+{
+  const int64_t n = int64_t(df.n[0]) * df.n[1] * df.n[2] * C;
+  hipLaunchKernelGGL((k_prolong<uint16_t, uint16_t>), dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads),
+                     0, s, xc, dc, yf, df, C, 0);
+  return hipGetLastError();
+}
+hipError_t restrict_f32(const float* gf, Dims3 df, float* gc, Dims3 dc, int C, hipStream_t s)
+{
+  const int64_t n = int64_t(dc.n[0]) * dc.n[1] * dc.n[2] * C;
+  hipLaunchKernelGGL(k_restrict, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, gf, df, gc, dc,
+                     C);
+  return hipGetLastError();
+}
+hipError_t fill_f32(float* x, float v, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_fill_f, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, x, v, n);
+  return hipGetLastError();
+}
+hipError_t add_f32(float* y, const float* x, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_add_f, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, y, x, n);
+  return hipGetLastError();
+}
+hipError_t loss(const float* y, const float* t, int64_t n, double* sums, float* dy, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_loss_sums, dim3(std::min<unsigned>(blocks_for(n), 1024)), dim3(kThreads), 0, s, y, t, n, sums);
+  hipLaunchKernelGGL(k_loss_grad, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, y, t, n, sums,
+                     dy);
+  return hipGetLastError();
+}
+hipError_t adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n, float lr, float b1, float b2,
+                float eps, int step, hipStream_t s)
+{
+  const float c1 = 1.0f - std::pow(b1, float(step)), c2 = 1.0f - std::pow(b2, float(step));
+  hipLaunchKernelGGL(k_adam, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(kThreads), 0, s, w, m, v, g, wbf, n,
+                     lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_to_bf16, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, x, y, n);
+  return hipGetLastError();
+}
+
+}  // namespace sxu

@@ -1,0 +1,53 @@
+// sx_unet.hpp -- kernels of the learned coarse-grid preconditioner
+// (sx_unet.hip) and the network that drives them (sx_learn.cpp).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sxu {
+
+struct Dims3 {
+  int n[3];
+};
+
+// one 3x3x3 convolution; x = concat(xa, xb) along channels, channels-last
+struct ConvArgs {
+  const void*     xa;
+  int             ca;
+  const void*     xb;  // may be null (cb = 0)
+  int             cb;
+  int             din[3], dout[3];
+  int             mode;  // 0: stride 1; 1: stride 2; 2: transpose of stride 2 (weights flipped)
+  const uint16_t* w;     // BF16 [27][ca + cb][cout]
+  const float*    bias;  // [cout] or null
+  int             relu;
+  void*           y;  // [nout][cout]
+};
+
+struct WGradArgs {
+  const void*  xa;
+  int          ca;
+  const void*  xb;
+  int          cb;
+  int          din[3], dout[3];
+  int          mode;  // of the forward convolution (0 or 1)
+  const float* g;     // dL/d(pre-activation) [nout][cout]
+  float*       dw;    // [27][ca + cb][cout], accumulated into
+  float*       db;    // [cout], accumulated into (may be null)
+};
+
+hipError_t conv(const ConvArgs& A, int cout, bool in_f32, bool out_f32, hipStream_t s);
+hipError_t conv_wgrad(const WGradArgs& A, int cout, bool in_f32, hipStream_t s);
+hipError_t flip_weights(const uint16_t* w, int cin, int cout, int c0, int cpart, uint16_t* wt, hipStream_t s);
+hipError_t relu_bwd(float* g, const uint16_t* y, int64_t n, hipStream_t s);
+hipError_t prolong_bf16(const uint16_t* xc, Dims3 dc, uint16_t* yf, Dims3 df, int C, hipStream_t s);
+hipError_t restrict_f32(const float* gf, Dims3 df, float* gc, Dims3 dc, int C, hipStream_t s);
+hipError_t fill_f32(float* x, float v, int64_t n, hipStream_t s);
+hipError_t add_f32(float* y, const float* x, int64_t n, hipStream_t s);
+hipError_t loss(const float* y, const float* t, int64_t n, double* sums, float* dy, hipStream_t s);
+hipError_t adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n, float lr, float b1, float b2,
+                float eps, int step, hipStream_t s);
+hipError_t to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t s);
+
+}  // namespace sxu
