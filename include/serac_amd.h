@@ -285,6 +285,34 @@ int sx_unet_backward(sx_unet* net, const float* x, const float* t, double* loss)
 int sx_unet_adam(sx_unet* net, double lr);
 int sx_unet_destroy(sx_unet* net);
 
+/* ---- learned coarse-grid preconditioner (no reference counterpart) ----
+ * For a 3-component byVDIM space on a structured (cuboid) mesh of order p:
+ *   M(r) = D^-1 r + s P N(P^T r / s),   s = rms(P^T r),
+ * D the Jacobi diagonal, P the interpolation from the mesh's vertex (Q1) grid
+ * to the order-p nodes and N the U-Net above on the vertex grid (each
+ * dimension 8 m + 1 vertices).  Used inside flexible CG; Jacobi-CG finishes a
+ * solve whose (r, D^-1 r) stalls for `stall_window` iterations.  N trains on
+ * (P^T r_k, x* - x_k at the vertices) pairs recorded from converged Jacobi-CG
+ * solves every `stride` iterations. */
+typedef struct sx_learned sx_learned;
+enum { SX_PRECOND_JACOBI = 0, SX_PRECOND_RECORD = 1, SX_PRECOND_LEARNED = 2 };
+int sx_learned_create(const sx_space* s, uint64_t seed, sx_learned** out);
+int sx_learned_configure(sx_learned* L, int stride, int max_pairs, int stall_window);
+/* pairs in the bank, fallbacks to Jacobi-CG so far, the network (owned by L) */
+int sx_learned_info(const sx_learned* L, int* npairs, int64_t* fallbacks, sx_unet** net);
+/* `steps` Adam steps on random recorded pairs; mean loss of the last tenth */
+int sx_learned_train(sx_learned* L, int steps, double lr, uint64_t seed, double* mean_loss);
+/* z = M(r) for the current linearization (device vectors) */
+int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* r, double* z);
+int sx_learned_destroy(sx_learned* L);
+/* the linear solves of sx_newton_solve / sx_quasistatic_solve on this form:
+ * SX_PRECOND_JACOBI (Jacobi-CG), SX_PRECOND_RECORD (Jacobi-CG recording
+ * pairs into L), SX_PRECOND_LEARNED (flexible CG with L).  L is borrowed. */
+int sx_form_set_preconditioner(sx_form* f, sx_learned* L, int mode);
+/* flexible CG with the learned preconditioner on J_bc x = b (x0 = 0) */
+int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                 double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
+
 /* ---- vector kernels used by the solvers, exposed for callers ---- */
 int sx_vec_dot(const double* a, const double* b, int64_t n, double* result); /* synchronous, deterministic */
 int sx_vec_axpy(double alpha, const double* x, double* y, int64_t n);        /* y += alpha x */

@@ -16,7 +16,7 @@ from .capi import (SX_BQF_FLUX, SX_BQF_PRESSURE, SX_BQF_TRACTION, SX_ORDERING_BY
                    SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
 
 __all__ = [
-    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "UNet", "SeracAmdError", "device_name",
+    "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "UNet", "Learned", "SeracAmdError", "device_name",
     "SX_QF_HEAT", "SX_QF_MASS_DIFFUSION", "SX_QF_NEOHOOKEAN", "SX_QF_LINEAR_ELASTIC", "SX_QF_J2_LINEAR",
     "SX_BQF_TRACTION", "SX_BQF_PRESSURE", "SX_BQF_FLUX",
     "SX_ORDERING_BYNODES", "SX_ORDERING_BYVDIM",
@@ -195,6 +195,16 @@ class Gradient:
         capi.call("sx_grad_assemble", self.h, ctypes.byref(h))
         return CSRMatrix(h)
 
+    def fcg(self, learned, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, raise_on_noconv=True):
+        """Flexible CG with the learned preconditioner on J_bc x = b."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_fcg_solve", self.h, learned.h, _iptr(ess), len(ess), b.ptr, x.ptr, rel_tol, abs_tol,
+                          max_iter, ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
     def tangent_bytes(self):
         b = ctypes.c_int64()
         capi.call("sx_grad_tangent_bytes", self.h, ctypes.byref(b))
@@ -301,8 +311,49 @@ class UNet:
         capi.call("sx_unet_adam", self.h, lr)
 
     def __del__(self):
-        if getattr(self, "h", None) and capi._lib is not None:
+        if getattr(self, "h", None) and getattr(self, "owned", True) and capi._lib is not None:
             capi._lib.sx_unet_destroy(self.h)
+            self.h = None
+
+
+class Learned:
+    """The learned coarse-grid preconditioner of a 3-component structured space."""
+
+    def __init__(self, space, seed=0):
+        h = _vp()
+        capi.call("sx_learned_create", space.h, seed, ctypes.byref(h))
+        self.h = h
+        self.space = space
+        self.net = UNet.__new__(UNet)  # a view of the network the preconditioner owns
+        npairs, fb, net = ctypes.c_int(), ctypes.c_int64(), _vp()
+        capi.call("sx_learned_info", self.h, ctypes.byref(npairs), ctypes.byref(fb), ctypes.byref(net))
+        self.net.h, self.net.owned = net, False
+        npar, nvox = ctypes.c_int64(), ctypes.c_int64()
+        capi.call("sx_unet_info", net, ctypes.byref(npar), ctypes.byref(nvox))
+        self.net.nparam, self.net.nvox = npar.value, nvox.value
+
+    def configure(self, stride=50, max_pairs=256, stall_window=20):
+        capi.call("sx_learned_configure", self.h, stride, max_pairs, stall_window)
+
+    def info(self):
+        npairs, fb, net = ctypes.c_int(), ctypes.c_int64(), _vp()
+        capi.call("sx_learned_info", self.h, ctypes.byref(npairs), ctypes.byref(fb), ctypes.byref(net))
+        return {"pairs": npairs.value, "fallbacks": fb.value}
+
+    def train(self, steps, lr=1e-3, seed=0):
+        loss = ctypes.c_double()
+        capi.call("sx_learned_train", self.h, steps, lr, seed, ctypes.byref(loss))
+        return loss.value
+
+    def apply(self, grad, ess, r):
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        z = DeviceVector(self.space.ndof)
+        capi.call("sx_learned_apply", self.h, grad.h, _iptr(ess), len(ess), r.ptr, z.ptr)
+        return z
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_learned_destroy(self.h)
             self.h = None
 
 
@@ -371,6 +422,11 @@ class Functional:
         dep = (ctypes.c_int * len(depends_on))(*depends_on)
         capi.call("sx_form_add_boundary_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on),
                   face_mask)
+
+    def set_preconditioner(self, learned, mode):
+        """Linear solves of newton / quasistatic_solve: SX_PRECOND_JACOBI, _RECORD or _LEARNED."""
+        capi.call("sx_form_set_preconditioner", self.h, learned.h if learned is not None else None, mode)
+        self._learned = learned  # the form borrows it
 
     def update_qdata(self, flag):
         """updateQdata(bool): while set, evaluations commit the quadrature-point state."""

@@ -37,6 +37,9 @@ SX_QF_J2_LINEAR = 5
 SX_BQF_TRACTION = 101
 SX_BQF_PRESSURE = 102
 SX_BQF_FLUX = 103
+SX_PRECOND_JACOBI = 0
+SX_PRECOND_RECORD = 1
+SX_PRECOND_LEARNED = 2
 
 _vp = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -101,6 +104,15 @@ SIGNATURES = {
     "sx_form_add_domain_integral_qdata": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int,
                                                   ctypes.POINTER(_int), _int, _vp]),
     "sx_form_update_qdata": (_int, [_vp, _int]),
+    "sx_learned_create": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
+    "sx_learned_configure": (_int, [_vp, _int, _int, _int]),
+    "sx_learned_info": (_int, [_vp, ctypes.POINTER(_int), _I64, ctypes.POINTER(_vp)]),
+    "sx_learned_train": (_int, [_vp, _int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "sx_learned_apply": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp]),
+    "sx_learned_destroy": (_int, [_vp]),
+    "sx_form_set_preconditioner": (_int, [_vp, _vp, _int]),
+    "sx_fcg_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
+                            ctypes.POINTER(SolveReport)]),
     "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "sx_unet_info": (_int, [_vp, _I64, _I64]),
     "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

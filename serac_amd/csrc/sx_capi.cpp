@@ -9,9 +9,11 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <map>
 #include <mutex>
+#include <random>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -20,6 +22,7 @@
 #include "sx_boundary.hpp"
 #include "sx_launch.hpp"
 #include "sx_setup.hpp"
+#include "sx_unet.hpp"
 #include "sx_vector.hpp"
 
 namespace {
@@ -211,6 +214,27 @@ struct BoundaryIntegral {
   bool                 has_tangent = false;
 };
 
+// The learned coarse-grid preconditioner of a structured vector space:
+//   M(r) = D^-1 r + s P N(P^T r / s),  s = rms(P^T r)
+// with P the vertex (Q1) -> order-p interpolation and N the U-Net; plus the
+// bank of (coarse residual, coarse converged correction) pairs it trains on.
+struct sx_learned {
+  const sx_space*                            space = nullptr;
+  sx_unet*                                   net   = nullptr;
+  sxu::FEGrid                                G{};
+  int64_t                                    nv = 0;  // vertex nodes
+  DeviceBuffer                               rc, xin, y, ss, tmp;
+  std::vector<std::unique_ptr<DeviceBuffer>> pr, pt;  // pairs: P^T r_k and the correction at the vertices
+  std::vector<char>                          ok;      // the pair's correction is final (its solve converged)
+  int                                        max_pairs = 256, stride = 50, stall = 20;
+  int                                        next = 0;  // ring position once the bank is full
+  int64_t                                    fallbacks = 0;
+  ~sx_learned()
+  {
+    if (net) sx_unet_destroy(net);
+  }
+};
+
 struct sx_grad {
   sx_form* form = nullptr;
   int      wrt  = 0;
@@ -229,6 +253,8 @@ struct sx_form {
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
+  sx_learned*                            learned      = nullptr;        // borrowed
+  int                                    learn_mode   = 0;              // SX_PRECOND_*
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -1356,14 +1382,29 @@ double read_scalar(sx_form* f, int slot)
 }
 
 struct Workspace {
-  DeviceBuffer r, z, d, Ad, diag, ess;
+  DeviceBuffer r, z, d, Ad, diag, ess, t;
 };
 
 // Jacobi-PCG on J_bc (rows and columns of ess eliminated) following the
 // preconditioned CG of mfem::CGSolver, as configured by
 // num/equation_solver.cpp:916-919, 1018-1021.
+using CgHook = std::function<void(int it, const double* x, const double* r)>;
+
+void compute_diag(sx_grad* g, const int32_t* ess_dev, int64_t ness, double* diag)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  SX_HIP(sxk::vec_fill(diag, 0.0, n, f->stream));
+  for (auto& I : f->integrals)
+    if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
+  for (auto& B : f->bintegrals)
+    if (depends(*B, g->wrt) && B->has_tangent)
+      run_boundary(f, *B, sxk::FMODE_DIAG, nullptr, diag, nullptr, sxk::PREC_F64);
+  SX_HIP(sxk::set_rows(diag, ess_dev, ness, 1.0, f->stream));
+}
+
 void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, double* x, double rel, double abs_tol,
-        int max_iter, Workspace& w, sx_solve_report* rep)
+        int max_iter, Workspace& w, sx_solve_report* rep, const CgHook* hook = nullptr)
 {
   sx_form*      f = g->form;
   const int64_t n = f->test->ndof();
@@ -1378,13 +1419,7 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   double *part = f->part.as<double>(), *scal = f->scal.as<double>();
 
   // Jacobi diagonal with identity rows for essential DOFs
-  SX_HIP(sxk::vec_fill(diag, 0.0, n, s));
-  for (auto& I : f->integrals)
-    if (depends(*I, g->wrt) && I->has_tangent) run_integral(f, *I, sxk::MODE_DIAG, nullptr, diag);
-  for (auto& B : f->bintegrals)
-    if (depends(*B, g->wrt) && B->has_tangent)
-      run_boundary(f, *B, sxk::FMODE_DIAG, nullptr, diag, nullptr, sxk::PREC_F64);
-  SX_HIP(sxk::set_rows(diag, ess_dev, ness, 1.0, s));
+  compute_diag(g, ess_dev, ness, diag);
 
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));
   SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
@@ -1396,6 +1431,7 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   rep->final_norm   = rep->initial_norm;
   rep->converged    = nom0 <= goal;
   rep->linear_iterations = 0;
+  if (hook) (*hook)(0, x, r);
   if (rep->converged) return;
 
   auto apply_bc = [&](const double* in, double* outv) {
@@ -1407,6 +1443,7 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   int it = 0;
   for (it = 1; it <= max_iter; it++) {
     SX_HIP(sxk::cg_update(x, r, z, d, Ad, diag, scal, n, part, s));  // scal[3] = betanom, scal[4] = beta
+    if (hook) (*hook)(it, x, r);
     const double betanom = read_scalar(f, 3);
     rep->final_norm      = std::sqrt(std::max(betanom, 0.0));
     SX_REQUIRE(std::isfinite(betanom), SX_ERR_NOCONV, "CG produced a non-finite residual");
@@ -1419,6 +1456,173 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
     SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));
   }
   rep->linear_iterations = std::min(it, max_iter);
+}
+
+// ---- learned preconditioner ----
+void learned_apply(sx_learned* L, sx_form* f, const double* diag, const int32_t* ess, int64_t ness, const double* r,
+                   double* z)
+{
+  hipStream_t   s  = f->stream;
+  const int64_t n  = f->test->ndof();
+  const int64_t nc = L->nv * 3;
+  SX_HIP(sxk::jacobi_dot(r, diag, z, n, f->part.as<double>(), f->scal.as<double>(), 6, s));  // z = D^-1 r
+  SX_HIP(sxu::fe_restrict(L->G, r, L->rc.as<float>(), s));
+  SX_HIP(hipMemsetAsync(L->ss.ptr, 0, sizeof(double), s));
+  SX_HIP(sxu::sumsq(L->rc.as<float>(), nc, L->ss.as<double>(), s));
+  SX_HIP(sxu::normalize(L->rc.as<float>(), L->ss.as<const double>(), nc, nc, L->xin.as<float>(), s));
+  SX_HIP(hipStreamSynchronize(s));
+  const int rc = sx_unet_forward(L->net, L->xin.as<const float>(), L->y.as<float>());
+  if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  SX_HIP(sxu::fe_prolong_add(L->G, L->y.as<const float>(), L->ss.as<const double>(), nc, z, s));
+  SX_HIP(sxk::set_rows(z, ess, ness, 0.0, s));
+}
+
+// records (P^T r_k, x_k at the vertices) every `stride` iterations of a CG
+// solve; finish() turns the x_k into corrections x* - x_k
+struct PairRecorder {
+  sx_learned*      L;
+  hipStream_t      s;
+  std::vector<int> slots;
+  void             operator()(int it, const double* x, const double* r)
+  {
+    if (it % L->stride) return;
+    int k;
+    if (int(L->pr.size()) < L->max_pairs) {
+      k = int(L->pr.size());
+      L->pr.push_back(std::make_unique<DeviceBuffer>());
+      L->pt.push_back(std::make_unique<DeviceBuffer>());
+      L->ok.push_back(0);
+      L->pr[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
+      L->pt[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
+    } else {
+      k       = L->next;
+      L->next = (L->next + 1) % L->max_pairs;
+    }
+    L->ok[size_t(k)] = 0;
+    SX_HIP(sxu::fe_restrict(L->G, r, L->pr[size_t(k)]->as<float>(), s));
+    SX_HIP(sxu::fe_inject(L->G, x, L->pt[size_t(k)]->as<float>(), s));
+    slots.push_back(k);
+  }
+  void finish(const double* xstar)
+  {
+    SX_HIP(sxu::fe_inject(L->G, xstar, L->tmp.as<float>(), s));
+    for (int k : slots) {
+      SX_HIP(sxu::sub_f32(L->tmp.as<const float>(), L->pt[size_t(k)]->as<const float>(), L->pt[size_t(k)]->as<float>(),
+                          L->nv * 3, s));
+      L->ok[size_t(k)] = 1;
+    }
+    SX_HIP(hipStreamSynchronize(s));
+  }
+};
+
+// Flexible CG (FCG(1): beta = (z_k, q_{k-1}) / (d_{k-1}, q_{k-1})) with the
+// learned preconditioner.  Stops on the Jacobi-preconditioned residual
+// measure (r, D^-1 r) <= max(rel^2 (r0, D^-1 r0), abs^2), the rule of the
+// Jacobi-CG it replaces; when that measure has not improved for `stall`
+// iterations, Jacobi-CG finishes the solve from the current iterate.
+void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
+         double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  w.r.allocate(size_t(n) * 8);
+  w.z.allocate(size_t(n) * 8);
+  w.d.allocate(size_t(n) * 8);
+  w.Ad.allocate(size_t(n) * 8);
+  w.diag.allocate(size_t(n) * 8);
+  w.t.allocate(size_t(n) * 8);
+  double *r = w.r.as<double>(), *z = w.z.as<double>(), *d = w.d.as<double>(), *q = w.Ad.as<double>(),
+         *diag = w.diag.as<double>(), *t = w.t.as<double>();
+  double *part = f->part.as<double>(), *scal = f->scal.as<double>();
+  auto    dot  = [&](const double* a, const double* c) {
+    SX_HIP(sxk::vec_dot_dev(a, c, n, part, scal, 5, 0, s));
+    return read_scalar(f, 5);
+  };
+  auto measure = [&]() {
+    SX_HIP(sxk::jacobi_dot(r, diag, t, n, part, scal, 7, s));
+    return read_scalar(f, 7);
+  };
+  auto apply_bc = [&](const double* in, double* outv) {
+    apply_gradient(g, in, outv);
+    SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
+  };
+  compute_diag(g, ess, ness, diag);
+  SX_HIP(sxk::vec_fill(x, 0.0, n, s));
+  SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  const double mu0  = measure();
+  const double goal = std::max(mu0 * rel * rel, abs_tol * abs_tol);
+  rep->initial_norm = std::sqrt(std::max(mu0, 0.0));
+  rep->final_norm   = rep->initial_norm;
+  rep->converged    = mu0 <= goal;
+  rep->linear_iterations = 0;
+  if (rep->converged) return;
+  learned_apply(L, f, diag, ess, ness, r, z);
+  SX_HIP(hipMemcpyAsync(d, z, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  apply_bc(d, q);
+  double dq = dot(d, q), alpha = dot(d, r) / dq;
+  double best = mu0, mu = mu0;
+  int    since = 0, it = 0;
+  bool   stalled = false;
+  for (it = 1; it <= max_iter; it++) {
+    SX_HIP(sxk::vec_axpy(alpha, d, x, n, s));
+    SX_HIP(sxk::vec_axpy(-alpha, q, r, n, s));
+    mu              = measure();
+    rep->final_norm = std::sqrt(std::max(mu, 0.0));
+    SX_REQUIRE(std::isfinite(mu), SX_ERR_NOCONV, "flexible CG produced a non-finite residual");
+    if (mu <= goal) {
+      rep->converged = 1;
+      break;
+    }
+    if (mu < best) {
+      best  = mu;
+      since = 0;
+    } else if (++since >= L->stall) {
+      stalled = true;
+      break;
+    }
+    learned_apply(L, f, diag, ess, ness, r, z);
+    const double beta = dot(z, q) / dq;
+    SX_HIP(sxk::vec_axpby(1.0, z, -beta, d, n, s));  // d = z - beta d
+    apply_bc(d, q);
+    dq    = dot(d, q);
+    alpha = dot(d, r) / dq;
+  }
+  rep->linear_iterations = std::min(it, max_iter);
+  if (stalled && it < max_iter) {
+    // Jacobi-CG from the current iterate: solve A delta = r to the same goal
+    L->fallbacks++;
+    Workspace       w2;
+    DeviceBuffer    rb, delta;
+    sx_solve_report lr{};
+    rb.allocate(size_t(n) * 8);
+    delta.allocate(size_t(n) * 8);
+    SX_HIP(hipMemcpyAsync(rb.ptr, r, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    cg(g, ess, ness, rb.as<double>(), delta.as<double>(), std::sqrt(goal / mu), abs_tol, max_iter - it, w2, &lr);
+    SX_HIP(sxk::vec_axpy(1.0, delta.as<double>(), x, n, s));
+    rep->linear_iterations += lr.linear_iterations;
+    rep->converged  = lr.converged;
+    rep->final_norm = lr.final_norm;
+  }
+}
+
+// the linear solve of the Newton loops: Jacobi-CG, Jacobi-CG recording
+// training pairs, or flexible CG with the learned preconditioner
+void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
+                  double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
+{
+  sx_form* f = g->form;
+  if (f->learned && f->learn_mode == SX_PRECOND_LEARNED) {
+    fcg(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
+  } else if (f->learned && f->learn_mode == SX_PRECOND_RECORD) {
+    PairRecorder rec{f->learned, f->stream, {}};
+    CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
+    cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, &hook);
+    // an unconverged solve gives no targets: its slots stay marked unusable
+    if (rep->converged) rec.finish(x);
+  } else {
+    cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
+  }
 }
 
 double residual_norm(sx_form* f, int wrt, const double* x, double* r, const int32_t* ess_dev, int64_t ness,
@@ -1480,7 +1684,7 @@ void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* 
     sx_grad* g = nullptr;
     residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at x
     sx_solve_report lr{};
-    cg(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
+    linear_solve(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
     rep->linear_iterations += lr.linear_iterations;
     SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
     norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
@@ -1567,7 +1771,7 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
         SX_HIP(sxk::vec_axpby(-1.0, jdbc.as<double>(), -1.0, r, n, s));  // r = -(R + J du_bc)
         SX_HIP(sxk::set_rows(r, ess, ness, 0.0, s));
         sx_solve_report lr{};
-        cg(g, ess, ness, r, c, opts->lin_rel_tol, opts->lin_abs_tol, opts->lin_max_iter, w, &lr);
+        linear_solve(g, ess, ness, r, c, opts->lin_rel_tol, opts->lin_abs_tol, opts->lin_max_iter, w, &lr);
         rep.linear_iterations += lr.linear_iterations;
         SX_HIP(sxk::vec_axpy(1.0, c, x, n, s));
         SX_HIP(sxk::vec_axpy(1.0, dbc.as<double>(), x, n, s));
@@ -1592,6 +1796,148 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
       prev = target;
     }
   });
+}
+
+// ---------------------------------------------------------------------------
+// learned coarse-grid preconditioner
+// ---------------------------------------------------------------------------
+int sx_learned_create(const sx_space* sp, uint64_t seed, sx_learned** out)
+{
+  return api([&] {
+    SX_REQUIRE(sp && out, SX_ERR_ARG, "null argument");
+    const auto& m = sp->mesh->m;
+    SX_REQUIRE(m.structured, SX_ERR_UNSUPPORTED, "the learned preconditioner needs a structured (cuboid) mesh");
+    SX_REQUIRE(sp->nc == 3 && sp->ordering == SX_ORDERING_BYVDIM, SX_ERR_UNSUPPORTED,
+               "the learned preconditioner acts on 3-component byVDIM spaces");
+    auto L   = std::make_unique<sx_learned>();
+    L->space = sp;
+    L->G.p   = sp->order;
+    for (int d = 0; d < 3; d++) {
+      L->G.nv[d] = m.n[d] + 1;
+      L->G.nn[d] = sp->order * m.n[d] + 1;
+    }
+    const auto xi = sxh::lobatto_points(sp->order + 1);
+    for (int k = 0; k <= sp->order; k++) L->G.xi[k] = xi[size_t(k)];
+    L->nv = int64_t(L->G.nv[0]) * L->G.nv[1] * L->G.nv[2];
+    const int rc = sx_unet_create(L->G.nv[0], L->G.nv[1], L->G.nv[2], seed, &L->net);
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    L->rc.allocate(size_t(L->nv) * 3 * 4);
+    L->xin.allocate(size_t(L->nv) * 3 * 4);
+    L->y.allocate(size_t(L->nv) * 3 * 4);
+    L->tmp.allocate(size_t(L->nv) * 3 * 4);
+    L->ss.allocate(sizeof(double));
+    *out = L.release();
+  });
+}
+
+int sx_learned_configure(sx_learned* L, int stride, int max_pairs, int stall_window)
+{
+  return api([&] {
+    SX_REQUIRE(L && stride >= 1 && max_pairs >= 1 && stall_window >= 1, SX_ERR_ARG, "bad argument");
+    SX_REQUIRE(int(L->pr.size()) <= max_pairs, SX_ERR_STATE, "the bank already holds more pairs");
+    L->stride    = stride;
+    L->max_pairs = max_pairs;
+    L->stall     = stall_window;
+  });
+}
+
+int sx_learned_info(const sx_learned* L, int* npairs, int64_t* fallbacks, sx_unet** net)
+{
+  return api([&] {
+    SX_REQUIRE(L, SX_ERR_ARG, "null argument");
+    if (npairs) *npairs = int(std::count(L->ok.begin(), L->ok.end(), char(1)));
+    if (fallbacks) *fallbacks = L->fallbacks;
+    if (net) *net = L->net;
+  });
+}
+
+int sx_learned_train(sx_learned* L, int steps, double lr, uint64_t seed, double* mean_loss)
+{
+  return api([&] {
+    SX_REQUIRE(L && steps >= 0, SX_ERR_ARG, "bad argument");
+    std::vector<size_t> usable;
+    for (size_t j = 0; j < L->ok.size(); j++)
+      if (L->ok[j]) usable.push_back(j);
+    SX_REQUIRE(!usable.empty(), SX_ERR_STATE, "no training pairs recorded");
+    std::mt19937_64 rng(seed);
+    const int64_t   nc = L->nv * 3;
+    DeviceBuffer    tin, tt;
+    tin.allocate(size_t(nc) * 4);
+    tt.allocate(size_t(nc) * 4);
+    double sum = 0.0;
+    int    cnt = 0;
+    for (int k = 0; k < steps; k++) {
+      const size_t j = usable[size_t(rng() % usable.size())];
+      // both the input and the target are scaled by the input's RMS
+      SX_HIP(hipMemset(L->ss.ptr, 0, sizeof(double)));
+      SX_HIP(sxu::sumsq(L->pr[j]->as<const float>(), nc, L->ss.as<double>(), nullptr));
+      SX_HIP(sxu::normalize(L->pr[j]->as<const float>(), L->ss.as<const double>(), nc, nc, tin.as<float>(), nullptr));
+      SX_HIP(sxu::normalize(L->pt[j]->as<const float>(), L->ss.as<const double>(), nc, nc, tt.as<float>(), nullptr));
+      SX_HIP(hipDeviceSynchronize());
+      double loss = 0.0;
+      int    rc   = sx_unet_backward(L->net, tin.as<const float>(), tt.as<const float>(), &loss);
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+      rc = sx_unet_adam(L->net, lr);
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+      if (k >= steps - std::max(1, steps / 10)) {
+        sum += loss;
+        cnt++;
+      }
+    }
+    if (mean_loss) *mean_loss = cnt ? sum / cnt : 0.0;
+  });
+}
+
+int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* r, double* z)
+{
+  return api([&] {
+    SX_REQUIRE(L && g && g->form && r && z, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "preconditioner applied before a linearization");
+    SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    w.diag.allocate(size_t(g->form->test->ndof()) * 8);
+    compute_diag(g, w.ess.as<const int32_t>(), ness, w.diag.as<double>());
+    learned_apply(L, g->form, w.diag.as<const double>(), w.ess.as<const int32_t>(), ness, r, z);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+  });
+}
+
+int sx_learned_destroy(sx_learned* L)
+{
+  delete L;
+  return SX_OK;
+}
+
+int sx_form_set_preconditioner(sx_form* f, sx_learned* L, int mode)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(mode == SX_PRECOND_JACOBI || mode == SX_PRECOND_RECORD || mode == SX_PRECOND_LEARNED, SX_ERR_ARG,
+               "bad preconditioner mode");
+    SX_REQUIRE(mode == SX_PRECOND_JACOBI || L, SX_ERR_ARG, "this mode needs a learned preconditioner");
+    SX_REQUIRE(!L || L->space == f->test, SX_ERR_ARG, "the preconditioner belongs to another space");
+    f->learned    = mode == SX_PRECOND_JACOBI ? nullptr : L;
+    f->learn_mode = mode;
+  });
+}
+
+int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                 double rel_tol, double abs_tol, int max_iter, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && L && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    fcg(g, L, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    SX_REQUIRE(rep.converged, SX_ERR_NOCONV, "flexible CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
 }
 
 }  // extern "C"

@@ -301,6 +301,115 @@ __global__ void k_to_bf16(const float* x, uint16_t* y, int64_t n)
     y[i] = f2bf(x[i]);
 }
 
+// ---- coupling to the order-p nodal space of a structured mesh ----
+// P interpolates vertex (Q1) values to the order-p nodes; in 1D, node j and
+// vertex i couple through w(j - p i), w(d) = 1 - xi_d for 0 <= d <= p and
+// xi_{p+d} for -p <= d < 0 (xi: the GLL points on [0, 1]); P is the tensor
+// product.  Fine vectors are byVDIM FP64 with 3 components.
+__device__ __forceinline__ double fe_w(const FEGrid& G, int d)
+{
+  return d >= 0 ? 1.0 - G.xi[d] : G.xi[G.p + d];
+}
+
+// rc = P^T r (FP32 out)
+__global__ void k_fe_restrict(FEGrid G, const double* r, float* rc)
+{
+  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % 3);
+    const int64_t i = t / 3;
+    const int     iv[3] = {int(i % G.nv[0]), int((i / G.nv[0]) % G.nv[1]), int(i / (int64_t(G.nv[0]) * G.nv[1]))};
+    double        s     = 0.0;
+    for (int dz = -G.p; dz <= G.p; dz++) {
+      const int jz = G.p * iv[2] + dz;
+      if (jz < 0 || jz >= G.nn[2]) continue;
+      const double wz = fe_w(G, dz);
+      for (int dy = -G.p; dy <= G.p; dy++) {
+        const int jy = G.p * iv[1] + dy;
+        if (jy < 0 || jy >= G.nn[1]) continue;
+        const double wzy = wz * fe_w(G, dy);
+        for (int dx = -G.p; dx <= G.p; dx++) {
+          const int jx = G.p * iv[0] + dx;
+          if (jx < 0 || jx >= G.nn[0]) continue;
+          s += wzy * fe_w(G, dx) * r[((int64_t(jz) * G.nn[1] + jy) * G.nn[0] + jx) * 3 + c];
+        }
+      }
+    }
+    rc[t] = float(s);
+  }
+}
+
+// z += sqrt(ss / n) * P ec  (ss: the sum of squares the input was normalized by)
+__global__ void k_fe_prolong_add(FEGrid G, const float* ec, const double* ss, int64_t nss, double* z)
+{
+  const double  scale = ss[0] > 0 ? sqrt(ss[0] / double(nss)) : 0.0;
+  const int64_t nn    = int64_t(G.nn[0]) * G.nn[1] * G.nn[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nn * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % 3);
+    const int64_t j = t / 3;
+    const int     jv[3] = {int(j % G.nn[0]), int((j / G.nn[0]) % G.nn[1]), int(j / (int64_t(G.nn[0]) * G.nn[1]))};
+    int           e[3], k[3];
+    for (int d = 0; d < 3; d++) {
+      e[d] = jv[d] / G.p;
+      k[d] = jv[d] - G.p * e[d];
+    }
+    double s = 0.0;
+    for (int b = 0; b < 8; b++) {
+      const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+      if ((bx && k[0] == 0) || (by && k[1] == 0) || (bz && k[2] == 0)) continue;
+      const double w = (bx ? G.xi[k[0]] : 1.0 - G.xi[k[0]]) * (by ? G.xi[k[1]] : 1.0 - G.xi[k[1]]) *
+                       (bz ? G.xi[k[2]] : 1.0 - G.xi[k[2]]);
+      const int64_t v = (int64_t(e[2] + bz) * G.nv[1] + (e[1] + by)) * G.nv[0] + (e[0] + bx);
+      s += w * double(ec[v * 3 + c]);
+    }
+    z[t] += scale * s;
+  }
+}
+
+// xc[i][c] = x at vertex node p i
+__global__ void k_fe_inject(FEGrid G, const double* x, float* xc)
+{
+  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % 3);
+    const int64_t i = t / 3;
+    const int     ix = int(i % G.nv[0]), iy = int((i / G.nv[0]) % G.nv[1]), iz = int(i / (int64_t(G.nv[0]) * G.nv[1]));
+    const int64_t j  = (int64_t(G.p * iz) * G.nn[1] + G.p * iy) * G.nn[0] + G.p * ix;
+    xc[t]            = float(x[j * 3 + c]);
+  }
+}
+
+// ss[0] += sum x^2 (FP64)
+__global__ void k_sumsq(const float* x, int64_t n, double* ss)
+{
+  __shared__ double sh[kThreads];
+  double a = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    a += double(x[i]) * double(x[i]);
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = kThreads / 2; w > 0; w >>= 1) {
+    if (int(threadIdx.x) < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(ss, sh[0]);
+}
+
+// y = x / sqrt(ss / nss)  (0 when ss = 0)
+__global__ void k_normalize(const float* x, const double* ss, int64_t nss, int64_t n, float* y)
+{
+  const double inv = ss[0] > 0 ? 1.0 / sqrt(ss[0] / double(nss)) : 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = float(double(x[i]) * inv);
+}
+
+// y = a - b (FP32)
+__global__ void k_sub_f(const float* a, const float* b, float* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = a[i] - b[i];
+}
+
 template <int COUT>
 hipError_t launch_conv_t(const ConvArgs& A, bool in_f32, bool out_f32, hipStream_t s)
 {
@@ -403,6 +512,41 @@ hipError_t adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n
   const float c1 = 1.0f - std::pow(b1, float(step)), c2 = 1.0f - std::pow(b2, float(step));
   hipLaunchKernelGGL(k_adam, dim3(std::min<unsigned>(blocks_for(n), 4096)), dim3(kThreads), 0, s, w, m, v, g, wbf, n,
                      lr, b1, b2, eps, c1, c2);
+  return hipGetLastError();
+}
+hipError_t fe_restrict(const FEGrid& G, const double* r, float* rc, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
+  hipLaunchKernelGGL(k_fe_restrict, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, r, rc);
+  return hipGetLastError();
+}
+hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, int64_t nss, double* z, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nn[0]) * G.nn[1] * G.nn[2] * 3;
+  hipLaunchKernelGGL(k_fe_prolong_add, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, ec, ss,
+                     nss, z);
+  return hipGetLastError();
+}
+hipError_t fe_inject(const FEGrid& G, const double* x, float* xc, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
+  hipLaunchKernelGGL(k_fe_inject, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, x, xc);
+  return hipGetLastError();
+}
+hipError_t sumsq(const float* x, int64_t n, double* ss, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(blocks_for(n), 1024)), dim3(kThreads), 0, s, x, n, ss);
+  return hipGetLastError();
+}
+hipError_t normalize(const float* x, const double* ss, int64_t nss, int64_t n, float* y, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_normalize, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, x, ss, nss, n,
+                     y);
+  return hipGetLastError();
+}
+hipError_t sub_f32(const float* a, const float* b, float* y, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub_f, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, a, b, y, n);
   return hipGetLastError();
 }
 hipError_t to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t s)

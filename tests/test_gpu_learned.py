@@ -1,0 +1,119 @@
+"""The learned coarse-grid preconditioner: M(r) = D^-1 r + s P N(P^T r / s)
+against a numpy restatement (P built from the GLL points, N through the
+network's own forward), flexible CG with it against Jacobi-CG, and the
+record -> train -> solve loop."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+import serac_amd as sx
+from serac_amd import workloads as W
+
+from test_gpu_parity import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def interp_1d(n, p):
+    """Q1 -> Qp interpolation on one direction: (p n + 1) x (n + 1)."""
+    xi = O.gll_nodes(p + 1)
+    P = np.zeros((p * n + 1, n + 1))
+    for j in range(p * n + 1):
+        for i in range(n + 1):
+            d = j - p * i
+            if 0 <= d <= p:
+                P[j, i] = 1.0 - xi[d]
+            elif -p <= d < 0:
+                P[j, i] = xi[p + d]
+    return P
+
+
+def setup(n=8, p=2, perturb=0.2):
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, perturb)
+    s = sx.Space(m, p, 3, 1)
+    f = sx.Functional(s, [s])
+    f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+    ess = s.boundary_dofs(0b000001, 0x7)
+    return m, s, f, ess
+
+
+def test_learned_apply_matches_numpy(gpu_lib):
+    n, p = 8, 2
+    m, s, f, ess = setup(n, p)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    L = sx.Learned(s, seed=4)
+    r = W.nodal_field(s.nnodes, 3, 7, 3000, -1.0, 1.0, 1)
+    r[ess] = 0.0
+    z = L.apply(grad, ess, sx.DeviceVector.from_numpy(r)).numpy()
+    P1 = sp.csr_matrix(interp_1d(n, p))
+    P = sp.kron(P1, sp.kron(P1, P1)).tocsr()
+    rc = (P.T @ r.reshape(-1, 3)).astype(np.float32)
+    scale = np.sqrt(np.mean(rc.astype(np.float64) ** 2))
+    y = L.net.forward((rc / scale).astype(np.float32)).astype(np.float64)
+    diag = grad.diagonal().numpy()
+    diag[ess] = 1.0
+    z_ref = r / diag + scale * (P @ y).ravel()
+    z_ref[ess] = 0.0
+    assert relerr(z, z_ref) < 1e-4
+    # P^T and P are transposes of each other: <P^T r, c> = <r, P c>
+    c = np.random.default_rng(1).standard_normal(rc.shape)
+    assert abs(np.sum(rc * c) - np.sum(r.reshape(-1, 3) * (P @ c))) < 1e-4 * np.abs(np.sum(rc * c))
+
+
+def test_fcg_with_untrained_network_reaches_cg_solution(gpu_lib):
+    # a small displacement keeps the nearly incompressible tangent definite
+    # (at 0.005 per node its geometric stiffness makes it indefinite)
+    m, s, f, ess = setup(8, 2)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    b = W.nodal_field(s.nnodes, 3, 8, 3000, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    bd = sx.DeviceVector.from_numpy(b)
+    x_cg = sx.DeviceVector(s.ndof)
+    rep_cg = grad.cg(ess, bd, x_cg, rel_tol=1e-10)
+    L = sx.Learned(s, seed=4)
+    x_f = sx.DeviceVector(s.ndof)
+    rep_f = grad.fcg(L, ess, bd, x_f, rel_tol=1e-10)
+    assert rep_cg["converged"] and rep_f["converged"]
+    assert relerr(x_f.numpy(), x_cg.numpy()) < 1e-7
+
+
+def test_record_train_and_solve(gpu_lib):
+    # record pairs from the Jacobi-CG solves of a Newton step, train, then the
+    # next Newton step's linear solves run flexible CG with the network
+    m, s, f, ess0 = setup(8, 2)
+    ess1 = s.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    L = sx.Learned(s, seed=5)
+    L.configure(stride=10, max_pairs=64, stall_window=20)
+    f.set_preconditioner(L, sx.capi.SX_PRECOND_RECORD)
+    x = sx.DeviceVector(s.ndof)
+    final = np.where(np.isin(ess, ess1), -0.01, 0.0)
+    reps = sx.quasistatic_solve(f, x, ess, final, 1, rel_tol=1e-10, lin_rel_tol=1e-8)
+    assert reps[0]["converged"]
+    npairs = L.info()["pairs"]
+    assert npairs >= 5
+    l0 = L.train(1, lr=1e-3, seed=1)
+    l1 = L.train(300, lr=1e-3, seed=2)
+    assert l1 < l0
+    f.set_preconditioner(L, sx.capi.SX_PRECOND_LEARNED)
+    x_l = sx.DeviceVector.from_numpy(x.numpy())
+    reps_l = sx.quasistatic_solve(f, x_l, ess, 2 * final, 1, rel_tol=1e-10, lin_rel_tol=1e-8)
+    f.set_preconditioner(None, sx.capi.SX_PRECOND_JACOBI)
+    x_j = sx.DeviceVector.from_numpy(x.numpy())
+    reps_j = sx.quasistatic_solve(f, x_j, ess, 2 * final, 1, rel_tol=1e-10, lin_rel_tol=1e-8)
+    assert reps_l[0]["converged"] and reps_j[0]["converged"]
+    assert relerr(x_l.numpy(), x_j.numpy()) < 1e-6
+
+
+def test_learned_requires_structured_vector_space(gpu_lib, patch_mesh):
+    X, hexes = patch_mesh
+    s = sx.Space(sx.Mesh.from_arrays(X, hexes), 1, 3, 1)
+    with pytest.raises(sx.SeracAmdError):
+        sx.Learned(s)
+    s1 = sx.Space(sx.Mesh.cuboid(8, 8, 8), 1, 1, 1)
+    with pytest.raises(sx.SeracAmdError):
+        sx.Learned(s1)
