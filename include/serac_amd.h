@@ -292,14 +292,21 @@ int sx_unet_destroy(sx_unet* net);
  * to the order-p nodes and N the U-Net above on the vertex grid (each
  * dimension 8 m + 1 vertices).  Used inside flexible CG; Jacobi-CG finishes a
  * solve whose (r, D^-1 r) stalls for `stall_window` iterations.  N trains on
- * (P^T r_k, x* - x_k at the vertices) pairs recorded from converged Jacobi-CG
- * solves every `stride` iterations. */
+ * pairs recorded from Jacobi-CG solves every `stride` iterations: the input
+ * P^T r_k and the target avg(x_{k+window}) - avg(x_k), the correction of the
+ * next `window` iterations as a hat-weighted vertex mean (P^T x) / (P^T 1). */
 typedef struct sx_learned sx_learned;
 enum { SX_PRECOND_JACOBI = 0, SX_PRECOND_RECORD = 1, SX_PRECOND_LEARNED = 2 };
 int sx_learned_create(const sx_space* s, uint64_t seed, sx_learned** out);
-int sx_learned_configure(sx_learned* L, int stride, int max_pairs, int stall_window);
+int sx_learned_configure(sx_learned* L, int stride, int window, int max_pairs, int stall_window);
 /* pairs in the bank, fallbacks to Jacobi-CG so far, the network (owned by L) */
 int sx_learned_info(const sx_learned* L, int* npairs, int64_t* fallbacks, sx_unet** net);
+/* Jacobi-CG on J_bc x = b (as sx_cg_solve) recording pairs into L */
+int sx_learned_record_cg(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                         double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
+/* copy usable pair k (0 <= k < npairs) to host FP32 arrays [nvertex][3]:
+ * the restricted residual P^T r_k and the vertex-mean correction */
+int sx_learned_get_pair(const sx_learned* L, int k, float* r_host, float* t_host);
 /* `steps` Adam steps on random recorded pairs; mean loss of the last tenth */
 int sx_learned_train(sx_learned* L, int steps, double lr, uint64_t seed, double* mean_loss);
 /* z = M(r) for the current linearization (device vectors) */

@@ -332,18 +332,34 @@ class Learned:
         capi.call("sx_unet_info", net, ctypes.byref(npar), ctypes.byref(nvox))
         self.net.nparam, self.net.nvox = npar.value, nvox.value
 
-    def configure(self, stride=50, max_pairs=256, stall_window=20):
-        capi.call("sx_learned_configure", self.h, stride, max_pairs, stall_window)
+    def configure(self, stride=50, window=50, max_pairs=256, stall_window=20):
+        capi.call("sx_learned_configure", self.h, stride, window, max_pairs, stall_window)
 
     def info(self):
         npairs, fb, net = ctypes.c_int(), ctypes.c_int64(), _vp()
         capi.call("sx_learned_info", self.h, ctypes.byref(npairs), ctypes.byref(fb), ctypes.byref(net))
         return {"pairs": npairs.value, "fallbacks": fb.value}
 
+    def pair(self, k):
+        """Usable pair k as (P^T r_k, vertex-mean correction of the next `window` CG iterations), [nvertex][3] FP32."""
+        nv = self.net.nvox
+        r = np.zeros((nv, 3), dtype=np.float32)
+        t = np.zeros((nv, 3), dtype=np.float32)
+        capi.call("sx_learned_get_pair", self.h, k, r.ctypes.data, t.ctypes.data)
+        return r, t
+
     def train(self, steps, lr=1e-3, seed=0):
         loss = ctypes.c_double()
         capi.call("sx_learned_train", self.h, steps, lr, seed, ctypes.byref(loss))
         return loss.value
+
+    def record_cg(self, grad, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000):
+        """Jacobi-CG on J_bc x = b recording training pairs."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        capi.call("sx_learned_record_cg", self.h, grad.h, _iptr(ess), len(ess), b.ptr, x.ptr, rel_tol, abs_tol,
+                  max_iter, ctypes.byref(rep))
+        return rep.as_dict()
 
     def apply(self, grad, ess, r):
         ess = np.ascontiguousarray(ess, dtype=np.int32)

@@ -105,14 +105,17 @@ SIGNATURES = {
                                                   ctypes.POINTER(_int), _int, _vp]),
     "sx_form_update_qdata": (_int, [_vp, _int]),
     "sx_learned_create": (_int, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
-    "sx_learned_configure": (_int, [_vp, _int, _int, _int]),
+    "sx_learned_configure": (_int, [_vp, _int, _int, _int, _int]),
     "sx_learned_info": (_int, [_vp, ctypes.POINTER(_int), _I64, ctypes.POINTER(_vp)]),
+    "sx_learned_get_pair": (_int, [_vp, _int, ctypes.c_void_p, ctypes.c_void_p]),
     "sx_learned_train": (_int, [_vp, _int, ctypes.c_double, ctypes.c_uint64, _D]),
     "sx_learned_apply": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp]),
     "sx_learned_destroy": (_int, [_vp]),
     "sx_form_set_preconditioner": (_int, [_vp, _vp, _int]),
     "sx_fcg_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                             ctypes.POINTER(SolveReport)]),
+    "sx_learned_record_cg": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                    _int, ctypes.POINTER(SolveReport)]),
     "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "sx_unet_info": (_int, [_vp, _I64, _I64]),
     "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

@@ -224,9 +224,9 @@ struct sx_learned {
   sxu::FEGrid                                G{};
   int64_t                                    nv = 0;  // vertex nodes
   DeviceBuffer                               rc, xin, y, ss, tmp;
-  std::vector<std::unique_ptr<DeviceBuffer>> pr, pt;  // pairs: P^T r_k and the correction at the vertices
-  std::vector<char>                          ok;      // the pair's correction is final (its solve converged)
-  int                                        max_pairs = 256, stride = 50, stall = 20;
+  std::vector<std::unique_ptr<DeviceBuffer>> pr, pt;  // pairs: P^T r_k and the vertex-mean correction
+  std::vector<char>                          ok;      // the pair's target is complete
+  int                                        max_pairs = 256, stride = 50, window = 50, stall = 20;
   int                                        next = 0;  // ring position once the bank is full
   int64_t                                    fallbacks = 0;
   ~sx_learned()
@@ -1477,15 +1477,52 @@ void learned_apply(sx_learned* L, sx_form* f, const double* diag, const int32_t*
   SX_HIP(sxk::set_rows(z, ess, ness, 0.0, s));
 }
 
-// records (P^T r_k, x_k at the vertices) every `stride` iterations of a CG
-// solve; finish() turns the x_k into corrections x* - x_k
+// records a pair every `stride` iterations of a Jacobi-CG solve: the input
+// P^T r_k and, as the target, the correction the next `window` iterations make,
+// avg(x_{k+window}) - avg(x_k), with avg the hat-weighted vertex mean
+// (P^T x) / (P^T 1).  A solve that converges sooner closes its open pairs with
+// its final iterate; one that stops unconverged leaves them unusable.
 struct PairRecorder {
-  sx_learned*      L;
-  hipStream_t      s;
-  std::vector<int> slots;
-  void             operator()(int it, const double* x, const double* r)
+  struct Open {
+    int           slot, start;
+    DeviceBuffer* x0;  // avg(x_start), FP64
+  };
+  sx_learned*                                L;
+  hipStream_t                                s;
+  std::vector<Open>                          open;
+  std::vector<std::unique_ptr<DeviceBuffer>> pool;  // FP64 vertex vectors
+  std::vector<DeviceBuffer*>                 spare;
+  DeviceBuffer                               now;   // avg(x_it), FP64
+  PairRecorder(sx_learned* learned, hipStream_t stream) : L(learned), s(stream)
   {
-    if (it % L->stride) return;
+    now.allocate(size_t(L->nv) * 3 * 8);
+  }
+  DeviceBuffer* take()
+  {
+    if (spare.empty()) {
+      pool.push_back(std::make_unique<DeviceBuffer>());
+      pool.back()->allocate(size_t(L->nv) * 3 * 8);
+      return pool.back().get();
+    }
+    DeviceBuffer* b = spare.back();
+    spare.pop_back();
+    return b;
+  }
+  void operator()(int it, const double* x, const double* r)
+  {
+    const bool closes = std::any_of(open.begin(), open.end(), [&](auto& o) { return it - o.start >= L->window; });
+    const bool starts = it % L->stride == 0;
+    if (!closes && !starts) return;
+    SX_HIP(sxu::fe_average(L->G, x, now.as<double>(), s));
+    for (auto o = open.begin(); o != open.end();) {
+      if (it - o->start >= L->window) {
+        close(*o);
+        o = open.erase(o);
+      } else {
+        ++o;
+      }
+    }
+    if (!starts) return;
     int k;
     if (int(L->pr.size()) < L->max_pairs) {
       k = int(L->pr.size());
@@ -1497,20 +1534,36 @@ struct PairRecorder {
     } else {
       k       = L->next;
       L->next = (L->next + 1) % L->max_pairs;
+      for (auto o = open.begin(); o != open.end();) {
+        if (o->slot == k) {
+          spare.push_back(o->x0);
+          o = open.erase(o);
+        } else {
+          ++o;
+        }
+      }
     }
     L->ok[size_t(k)] = 0;
     SX_HIP(sxu::fe_restrict(L->G, r, L->pr[size_t(k)]->as<float>(), s));
-    SX_HIP(sxu::fe_inject(L->G, x, L->pt[size_t(k)]->as<float>(), s));
-    slots.push_back(k);
+    DeviceBuffer* x0 = take();
+    SX_HIP(hipMemcpyAsync(x0->ptr, now.ptr, size_t(L->nv) * 3 * 8, hipMemcpyDeviceToDevice, s));
+    open.push_back({k, it, x0});
   }
-  void finish(const double* xstar)
+  // pt[slot] = avg(x_now) - avg(x_start), formed in FP64
+  void close(const Open& o)
   {
-    SX_HIP(sxu::fe_inject(L->G, xstar, L->tmp.as<float>(), s));
-    for (int k : slots) {
-      SX_HIP(sxu::sub_f32(L->tmp.as<const float>(), L->pt[size_t(k)]->as<const float>(), L->pt[size_t(k)]->as<float>(),
-                          L->nv * 3, s));
-      L->ok[size_t(k)] = 1;
+    SX_HIP(sxu::sub_f64(now.as<const double>(), o.x0->as<const double>(), L->pt[size_t(o.slot)]->as<float>(),
+                        L->nv * 3, s));
+    L->ok[size_t(o.slot)] = 1;
+    spare.push_back(o.x0);
+  }
+  void finish(const double* x, bool converged)
+  {
+    if (converged && !open.empty()) {
+      SX_HIP(sxu::fe_average(L->G, x, now.as<double>(), s));
+      for (auto& o : open) close(o);
     }
+    open.clear();
     SX_HIP(hipStreamSynchronize(s));
   }
 };
@@ -1615,11 +1668,10 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
   if (f->learned && f->learn_mode == SX_PRECOND_LEARNED) {
     fcg(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   } else if (f->learned && f->learn_mode == SX_PRECOND_RECORD) {
-    PairRecorder rec{f->learned, f->stream, {}};
+    PairRecorder rec(f->learned, f->stream);
     CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, &hook);
-    // an unconverged solve gives no targets: its slots stay marked unusable
-    if (rep->converged) rec.finish(x);
+    rec.finish(x, rep->converged);
   } else {
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   }
@@ -1830,12 +1882,13 @@ int sx_learned_create(const sx_space* sp, uint64_t seed, sx_learned** out)
   });
 }
 
-int sx_learned_configure(sx_learned* L, int stride, int max_pairs, int stall_window)
+int sx_learned_configure(sx_learned* L, int stride, int window, int max_pairs, int stall_window)
 {
   return api([&] {
-    SX_REQUIRE(L && stride >= 1 && max_pairs >= 1 && stall_window >= 1, SX_ERR_ARG, "bad argument");
+    SX_REQUIRE(L && stride >= 1 && window >= 1 && max_pairs >= 1 && stall_window >= 1, SX_ERR_ARG, "bad argument");
     SX_REQUIRE(int(L->pr.size()) <= max_pairs, SX_ERR_STATE, "the bank already holds more pairs");
     L->stride    = stride;
+    L->window    = window;
     L->max_pairs = max_pairs;
     L->stall     = stall_window;
   });
@@ -1848,6 +1901,43 @@ int sx_learned_info(const sx_learned* L, int* npairs, int64_t* fallbacks, sx_une
     if (npairs) *npairs = int(std::count(L->ok.begin(), L->ok.end(), char(1)));
     if (fallbacks) *fallbacks = L->fallbacks;
     if (net) *net = L->net;
+  });
+}
+
+int sx_learned_record_cg(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                         double rel_tol, double abs_tol, int max_iter, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(L && g && g->form && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    PairRecorder rec(L, g->form->stream);
+    CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
+    cg(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep, &hook);
+    rec.finish(x, rep.converged);
+    SX_REQUIRE(rep.converged, SX_ERR_NOCONV, "CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+int sx_learned_get_pair(const sx_learned* L, int k, float* r_host, float* t_host)
+{
+  return api([&] {
+    SX_REQUIRE(L && r_host && t_host && k >= 0, SX_ERR_ARG, "bad argument");
+    int j = -1;
+    for (size_t i = 0, c = 0; i < L->ok.size(); i++)
+      if (L->ok[i] && int(c++) == k) {
+        j = int(i);
+        break;
+      }
+    SX_REQUIRE(j >= 0, SX_ERR_ARG, "no such pair");
+    const size_t bytes = size_t(L->nv) * 3 * 4;
+    SX_HIP(hipMemcpy(r_host, L->pr[size_t(j)]->ptr, bytes, hipMemcpyDeviceToHost));
+    SX_HIP(hipMemcpy(t_host, L->pt[size_t(j)]->ptr, bytes, hipMemcpyDeviceToHost));
   });
 }
 

@@ -311,15 +311,17 @@ __device__ __forceinline__ double fe_w(const FEGrid& G, int d)
   return d >= 0 ? 1.0 - G.xi[d] : G.xi[G.p + d];
 }
 
-// rc = P^T r (FP32 out)
-__global__ void k_fe_restrict(FEGrid G, const double* r, float* rc)
+// rc = P^T r (FP32 out); with `average`, each row is divided by its weight
+// sum (P^T 1), the hat-weighted mean of the nodal values around the vertex
+template <typename TO>
+__global__ void k_fe_restrict(FEGrid G, const double* r, TO* rc, int average)
 {
   const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
   for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
     const int     c = int(t % 3);
     const int64_t i = t / 3;
     const int     iv[3] = {int(i % G.nv[0]), int((i / G.nv[0]) % G.nv[1]), int(i / (int64_t(G.nv[0]) * G.nv[1]))};
-    double        s     = 0.0;
+    double        s = 0.0, ws = 0.0;
     for (int dz = -G.p; dz <= G.p; dz++) {
       const int jz = G.p * iv[2] + dz;
       if (jz < 0 || jz >= G.nn[2]) continue;
@@ -331,11 +333,13 @@ __global__ void k_fe_restrict(FEGrid G, const double* r, float* rc)
         for (int dx = -G.p; dx <= G.p; dx++) {
           const int jx = G.p * iv[0] + dx;
           if (jx < 0 || jx >= G.nn[0]) continue;
-          s += wzy * fe_w(G, dx) * r[((int64_t(jz) * G.nn[1] + jy) * G.nn[0] + jx) * 3 + c];
+          const double w = wzy * fe_w(G, dx);
+          s += w * r[((int64_t(jz) * G.nn[1] + jy) * G.nn[0] + jx) * 3 + c];
+          ws += w;
         }
       }
     }
-    rc[t] = float(s);
+    rc[t] = TO(average ? s / ws : s);
   }
 }
 
@@ -403,11 +407,12 @@ __global__ void k_normalize(const float* x, const double* ss, int64_t nss, int64
     y[i] = float(double(x[i]) * inv);
 }
 
-// y = a - b (FP32)
-__global__ void k_sub_f(const float* a, const float* b, float* y, int64_t n)
+// y = a - b, formed in the inputs' precision and stored as FP32
+template <typename TI>
+__global__ void k_sub_f(const TI* a, const TI* b, float* y, int64_t n)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-    y[i] = a[i] - b[i];
+    y[i] = float(a[i] - b[i]);
 }
 
 template <int COUT>
@@ -514,10 +519,18 @@ hipError_t adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n
                      lr, b1, b2, eps, c1, c2);
   return hipGetLastError();
 }
-hipError_t fe_restrict(const FEGrid& G, const double* r, float* rc, hipStream_t s)
+hipError_t fe_restrict(const FEGrid& G, const double* r, float* rc, hipStream_t s, bool average)
 {
   const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
-  hipLaunchKernelGGL(k_fe_restrict, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, r, rc);
+  hipLaunchKernelGGL(k_fe_restrict<float>, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, r,
+                     rc, int(average));
+  return hipGetLastError();
+}
+hipError_t fe_average(const FEGrid& G, const double* x, double* xc, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
+  hipLaunchKernelGGL(k_fe_restrict<double>, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, x,
+                     xc, 1);
   return hipGetLastError();
 }
 hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, int64_t nss, double* z, hipStream_t s)
@@ -546,7 +559,12 @@ hipError_t normalize(const float* x, const double* ss, int64_t nss, int64_t n, f
 }
 hipError_t sub_f32(const float* a, const float* b, float* y, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_sub_f, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, a, b, y, n);
+  hipLaunchKernelGGL(k_sub_f<float>, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, a, b, y, n);
+  return hipGetLastError();
+}
+hipError_t sub_f64(const double* a, const double* b, float* y, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub_f<double>, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, a, b, y, n);
   return hipGetLastError();
 }
 hipError_t to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t s)

@@ -1,7 +1,8 @@
 """The learned coarse-grid preconditioner: M(r) = D^-1 r + s P N(P^T r / s)
 against a numpy restatement (P built from the GLL points, N through the
 network's own forward), flexible CG with it against Jacobi-CG, and the
-record -> train -> solve loop."""
+record -> train -> solve loop, and the recorded training pairs against
+reproduced CG iterates."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -81,6 +82,51 @@ def test_fcg_with_untrained_network_reaches_cg_solution(gpu_lib):
     assert relerr(x_f.numpy(), x_cg.numpy()) < 1e-7
 
 
+def test_recorded_pairs_match_reproduced_cg_iterates(gpu_lib):
+    # each pair is (P^T r_k, avg(x_{k+W}) - avg(x_k)); Jacobi-CG is
+    # deterministic, so x_k is reproduced by a solve capped at k iterations
+    n, p = 8, 2
+    m, s, f, ess = setup(n, p)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    b = W.nodal_field(s.nnodes, 3, 8, 3000, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    bd = sx.DeviceVector.from_numpy(b)
+    L = sx.Learned(s, seed=1)
+    stride, window = 20, 30
+    L.configure(stride=stride, window=window, max_pairs=1000)
+    x = sx.DeviceVector(s.ndof)
+    rep = L.record_cg(grad, ess, bd, x, rel_tol=1e-8)
+    iters = rep["linear_iterations"]
+    npairs = L.info()["pairs"]
+    assert npairs == iters // stride + 1
+    P1 = sp.csr_matrix(interp_1d(n, p))
+    P = sp.kron(P1, sp.kron(P1, P1)).tocsr()
+    wsum = np.asarray(P.sum(axis=0)).ravel()[:, None]
+
+    def iterate(k):
+        xk = sx.DeviceVector(s.ndof)
+        if k > 0:
+            grad.cg(ess, bd, xk, rel_tol=1e-8, max_iter=k, raise_on_noconv=False)
+        return xk.numpy()
+
+    for j in (0, 1, npairs - 1):
+        k0, k1 = j * stride, min(j * stride + window, iters)
+        x0, x1 = iterate(k0), iterate(k1)
+        if k1 == iters:
+            assert np.array_equal(x1, x.numpy())
+        rin, tgt = L.pair(j)
+        jx = grad(sx.DeviceVector.from_numpy(x0)).numpy()
+        jx[ess] = x0[ess]
+        r0 = b - jx
+        rc = P.T @ r0.reshape(-1, 3)
+        t_ref = (P.T @ x1.reshape(-1, 3) - P.T @ x0.reshape(-1, 3)) / wsum
+        # CG updates its residual recursively; by the last pair |r_k| is 1e-8
+        # of |b| and the recursion has drifted from b - J x_k by ~1e-5 of |r_k|
+        assert relerr(rin, rc) < (1e-5 if j < 2 else 1e-3), j
+        assert relerr(tgt, t_ref) < 1e-5, j
+
+
 def test_record_train_and_solve(gpu_lib):
     # record pairs from the Jacobi-CG solves of a Newton step, train, then the
     # next Newton step's linear solves run flexible CG with the network
@@ -88,7 +134,7 @@ def test_record_train_and_solve(gpu_lib):
     ess1 = s.boundary_dofs(0b000010, 0x1)
     ess = np.union1d(ess0, ess1).astype(np.int32)
     L = sx.Learned(s, seed=5)
-    L.configure(stride=10, max_pairs=64, stall_window=20)
+    L.configure(stride=10, window=50, max_pairs=64, stall_window=20)
     f.set_preconditioner(L, sx.capi.SX_PRECOND_RECORD)
     x = sx.DeviceVector(s.ndof)
     final = np.where(np.isin(ess, ess1), -0.01, 0.0)
