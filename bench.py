@@ -28,12 +28,74 @@ HBM_PEAK_GBS = 8000.0
 WORKLOADS = {
     # name: (n, order, perturb, params, qfunction); d = minimal stored tangent per point
     "neohookean_p3_64": (64, 3, 0.2, (1.0, 10.0, 0.25), "neohookean"),
+    "neohookean_p3_24": (24, 3, 0.2, (1.0, 10.0, 0.25), "neohookean"),  # quick runs of the same path
     "neohookean_p4_48": (48, 4, 0.2, (1.0, 10.0, 0.25), "neohookean"),
     "neohookean_p2_128": (128, 2, 0.0, (1.0, 10.0, 0.25), "neohookean"),
     # J2 with linear hardening: E, nu, sigma_y, Hi, Hk, density, geometric nonlinearity
     "j2_p2_96": (96, 2, 0.0, (200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0), "j2"),
 }
 TANGENT_ENTRIES = {"neohookean": 45, "j2": 81}
+
+
+def unet_flops(nv):
+    """Multiply-adds x 2 of one U-Net forward pass on an nv^3 vertex grid (3x3x3 convolutions only)."""
+    import serac_amd as sx
+
+    levels = [nv]
+    for _ in range(3):
+        levels.append((levels[-1] - 1) // 2 + 1)
+    out_level = [0, 0, 1, 1, 2, 2, 3, 3, 2, 1, 0, 0]
+    return sum(2 * 27 * (ca + cb) * co * levels[lv] ** 3 for (ca, cb, co), lv in zip(sx.UNet.LAYERS, out_level))
+
+
+def unet_bench(sx, n, steps, warmup):
+    """U-Net inference and training step (forward, backward, Adam) timed with device syncs on the host clock."""
+    net = sx.UNet(n + 1, n + 1, n + 1, seed=1)
+    rng = np.random.default_rng(0)
+    x = sx.DeviceVector.from_numpy(rng.standard_normal(net.nvox * 3), dtype=np.float32)
+    t = sx.DeviceVector.from_numpy(rng.standard_normal(net.nvox * 3), dtype=np.float32)
+    y = sx.DeviceVector(net.nvox * 3, dtype=np.float32)
+    reps = max(1, min(steps, 20))
+    loss = ctypes_double()
+    for _ in range(max(1, warmup)):
+        sx.capi.call("sx_unet_forward", net.h, x.ptr, y.ptr)
+    sx.capi.call("sx_device_synchronize")
+    tf = time.perf_counter()
+    for _ in range(reps):
+        sx.capi.call("sx_unet_forward", net.h, x.ptr, y.ptr)
+    sx.capi.call("sx_device_synchronize")
+    t_fwd = (time.perf_counter() - tf) / reps
+    for _ in range(max(1, warmup)):
+        sx.capi.call("sx_unet_backward", net.h, x.ptr, t.ptr, loss)
+        sx.capi.call("sx_unet_adam", net.h, 1e-4)
+    sx.capi.call("sx_device_synchronize")
+    tb = time.perf_counter()
+    for _ in range(reps):
+        sx.capi.call("sx_unet_backward", net.h, x.ptr, t.ptr, loss)
+        sx.capi.call("sx_unet_adam", net.h, 1e-4)
+    sx.capi.call("sx_device_synchronize")
+    t_train = (time.perf_counter() - tb) / reps
+    fl = unet_flops(n + 1)
+    return {
+        "grid": [n + 1] * 3,
+        "params": net.nparam,
+        "forward_ms": round(t_fwd * 1e3, 4),
+        "train_step_ms": round(t_train * 1e3, 4),
+        "forward_gflop": round(fl / 1e9, 3),
+        "train_step_gflop": round(3 * fl / 1e9, 3),
+        "forward_tflops": round(fl / t_fwd / 1e12, 2),
+        "train_tflops": round(3 * fl / t_train / 1e12, 2),
+        "flops_note": "2 x multiply-adds of the 3x3x3 convolutions; a training step counts 3x the forward",
+        "kernels": "direct convolution on FP32 FMA units (BF16 weights and activations, FP32 accumulation)",
+        "peak_tflops_fp32_vector": 157.3,
+        "peak_tflops_bf16_mfma_dense": 2500.0,
+    }
+
+
+def ctypes_double():
+    import ctypes
+
+    return ctypes.byref(ctypes.c_double())
 
 
 def log(msg):
@@ -109,7 +171,17 @@ def main():
     ap.add_argument("--cg-max", type=int, default=20000)
     ap.add_argument("--solve-precision", default="f64",
                     help="comma-separated solve modes: f64 (plain FP64 Jacobi-CG), mixed_f32 / mixed_bf16 "
-                         "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov)")
+                         "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov), "
+                         "mixed_f32_learned (mixed_f32 whose first load step records training pairs; the U-Net "
+                         "then trains and flexible CG with it solves the remaining steps)")
+    ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps of the learned preconditioner")
+    ap.add_argument("--learn-lr", type=float, default=1e-3)
+    ap.add_argument("--learn-stride", type=int, default=50, help="CG iterations between recorded pairs")
+    ap.add_argument("--learn-window", type=int, default=50, help="CG iterations a target correction spans")
+    ap.add_argument("--learn-pairs", type=int, default=256, help="capacity of the pair bank")
+    ap.add_argument("--unet", action="store_true",
+                    help="also time the learned preconditioner's U-Net (inference and one training step) on the "
+                         "workload's vertex grid")
     ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
     ap.add_argument("--split-scatter", default="colored", choices=["colored", "atomic"],
                     help="scatter mode of the FP32 / BF16 J.v measurements")
@@ -241,6 +313,12 @@ def main():
         form = make_form(args.scatter)
     del jv
 
+    unet = None
+    if args.unet:
+        unet = unet_bench(sx, n, args.steps, args.warmup)
+        log(f"U-Net on {n + 1}^3 vertices: forward {unet['forward_ms']:.3f} ms ({unet['forward_tflops']:.1f} TFLOP/s), "
+            f"training step {unet['train_step_ms']:.3f} ms ({unet['train_tflops']:.1f} TFLOP/s)")
+
     solve = None
     if args.solve and qf != "neohookean":
         raise SystemExit("bench: --solve runs the uniaxial-compression load path of the Neo-Hookean workloads")
@@ -252,7 +330,8 @@ def main():
         final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
         modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
                  "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-                 "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32)}
+                 "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
+                 "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
         solve = {}
         del form
         for mode in args.solve_precision.split(","):
@@ -262,6 +341,12 @@ def main():
             form = make_form(scatter)
             form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
             form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
+            learned, t_train, train_loss = None, 0.0, None
+            if mode.endswith("_learned"):
+                learned = sx.Learned(space, seed=11)
+                learned.configure(stride=args.learn_stride, window=args.learn_window, max_pairs=args.learn_pairs,
+                                  stall_window=20)
+                form.set_preconditioner(learned, sx.capi.SX_PRECOND_RECORD)
             x = sx.DeviceVector(space.ndof)
             sx.capi.call("sx_device_synchronize")
             # one load step per call (the same arithmetic as one call over all
@@ -277,6 +362,15 @@ def main():
                 t_solve += time.perf_counter() - ts
                 log(f"{mode} load step {k}/{args.load_steps}: Newton {reps[-1]['newton_iterations']}, "
                     f"CG {reps[-1]['linear_iterations']}, {t_solve:.2f} s so far")
+                if learned is not None and k == 1:
+                    # train on the first step's pairs (timed), then precondition the rest
+                    tt = time.perf_counter()
+                    train_loss = learned.train(args.learn_steps, lr=args.learn_lr, seed=3)
+                    sx.capi.call("sx_device_synchronize")
+                    t_train = time.perf_counter() - tt
+                    log(f"{mode}: trained {args.learn_steps} Adam steps on {learned.info()['pairs']} pairs in "
+                        f"{t_train:.2f} s, loss of the last tenth {train_loss:.4f}")
+                    form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
             solve[mode] = {
                 "seconds": round(t_solve, 3),
                 "load_steps": args.load_steps,
@@ -292,6 +386,22 @@ def main():
                                        for r in reps],
             }
             solve[mode]["scatter"] = scatter
+            if learned is not None:
+                # the wall time above includes the training
+                solve[mode]["seconds"] = round(t_solve + t_train, 3)
+                solve[mode]["precond"] = "Jacobi (step 1, recording) then learned U-Net in flexible CG"
+                solve[mode]["learned"] = {
+                    "train_seconds": round(t_train, 3),
+                    "adam_steps": args.learn_steps,
+                    "lr": args.learn_lr,
+                    "pairs": learned.info()["pairs"],
+                    "stride": args.learn_stride,
+                    "window": args.learn_window,
+                    "train_loss_last_tenth": train_loss,
+                    "fallbacks_to_jacobi_cg": learned.info()["fallbacks"],
+                }
+                form.set_preconditioner(None, sx.capi.SX_PRECOND_JACOBI)
+                del learned
             log(f"{mode} solve {t_solve:.2f} s: Newton {solve[mode]['newton_iterations']}, "
                 f"CG {solve[mode]['cg_iterations']}")
             del x, form
@@ -361,6 +471,7 @@ def main():
         "jv_norm": jv_norm,
         "precision_split": split,
         "newton_solve": solve,
+        "learned_unet": unet,
         "device": dev,
     }
     print(json.dumps(rec), flush=True)

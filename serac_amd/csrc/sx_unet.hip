@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -115,51 +116,119 @@ __global__ void __launch_bounds__(kThreads) k_conv(ConvArgs A)
   }
 }
 
-// dw[tap][ci][co] += sum_o x[src(o, tap)][ci] g[o][co] (x = [xa, xb]) and, for
-// tap 0, db[co] += sum_o g[o][co].  grid = (ceil(nout / 64), 27).
-template <int COUT, typename TI>
-__global__ void __launch_bounds__(kThreads) k_conv_wgrad(WGradArgs A)
+// The same convolution with one lane per (output node, output channel):
+// 256 / COUT nodes per workgroup, the tap's weights and the nodes' input
+// values staged in LDS.  Used on the coarse levels, where one thread per node
+// would leave most of the GPU idle.
+template <int COUT, typename TI, typename TO>
+__global__ void __launch_bounds__(kThreads) k_conv_cp(ConvArgs A)
 {
-  constexpr int VC = 64;
+  constexpr int VPB = kThreads / COUT;
+  extern __shared__ float cp_smem[];  // xs: [VPB][cin] FP32, then wl: [cin][COUT] BF16
+  const int      cin  = A.ca + A.cb;
+  float*         xs   = cp_smem;
+  uint16_t*      wl   = reinterpret_cast<uint16_t*>(cp_smem + VPB * cin);
+  const int64_t  nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int      c    = int(threadIdx.x) % COUT;
+  const int      vl   = int(threadIdx.x) / COUT;
+  const int64_t  o    = int64_t(blockIdx.x) * VPB + vl;
+  const uint16_t* W   = static_cast<const uint16_t*>(A.w);
+  const TI*      xa   = static_cast<const TI*>(A.xa);
+  const TI*      xb   = static_cast<const TI*>(A.xb);
+  float          acc  = 0.0f;
+  for (int tap = 0; tap < 27; tap++) {
+    __syncthreads();
+    for (int t = threadIdx.x; t < cin * COUT; t += kThreads) wl[t] = W[int64_t(tap) * cin * COUT + t];
+    for (int t = threadIdx.x; t < VPB * cin; t += kThreads) {
+      const int     v  = t / cin, ci = t - v * cin;
+      const int64_t oo = int64_t(blockIdx.x) * VPB + v;
+      float         xv = 0.0f;
+      if (oo < nout) {
+        const int ox = int(oo % A.dout[0]);
+        const int oy = int((oo / A.dout[0]) % A.dout[1]);
+        const int oz = int(oo / (int64_t(A.dout[0]) * A.dout[1]));
+        const int sx = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
+        const int sy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
+        const int sz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
+        if (sx >= 0 && sy >= 0 && sz >= 0) {
+          const int64_t sidx = (int64_t(sz) * A.din[1] + sy) * A.din[0] + sx;
+          xv = ci < A.ca ? ld(xa, sidx * A.ca + ci) : ld(xb, sidx * A.cb + (ci - A.ca));
+        }
+      }
+      xs[t] = xv;
+    }
+    __syncthreads();
+    const float* xv = xs + vl * cin;
+    for (int ci = 0; ci < cin; ci++) acc += xv[ci] * bf2f(wl[ci * COUT + c]);
+  }
+  if (o >= nout) return;
+  float v = acc + (A.bias ? A.bias[c] : 0.0f);
+  if (A.relu) v = v > 0.0f ? v : 0.0f;
+  st(static_cast<TO*>(A.y), o * COUT + c, v);
+}
+
+// dw[tap][ci][co] += sum_o x[src(o, tap)][ci] g[o][co] (x = [xa, xb]) and, for
+// tap 0, db[co] += sum_o g[o][co].  grid = (ceil(nout / (64 chunks)), 27): each
+// workgroup sums `chunks` runs of 64 output nodes in registers before one
+// atomic add per weight.
+template <int COUT, typename TI>
+__global__ void __launch_bounds__(kThreads) k_conv_wgrad(WGradArgs A, int chunks)
+{
+  constexpr int VC   = 64;
+  constexpr int EMAX = (128 * COUT + kThreads - 1) / kThreads;  // weights per thread at cin <= 128
   __shared__ float sg[VC * COUT];
   __shared__ float sx[VC * 128];
   const int     cin  = A.ca + A.cb;
   const int     tap  = blockIdx.y;
   const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
-  const int64_t o0   = int64_t(blockIdx.x) * VC;
   const TI*     xa   = static_cast<const TI*>(A.xa);
   const TI*     xb   = static_cast<const TI*>(A.xb);
-  for (int t = threadIdx.x; t < VC * COUT; t += kThreads) {
-    const int64_t o = o0 + t / COUT;
-    sg[t]           = o < nout ? A.g[o * COUT + t % COUT] : 0.0f;
-  }
-  for (int j = threadIdx.x; j < VC; j += kThreads) {
-    const int64_t o  = o0 + j;
-    int64_t       s  = -1;
-    if (o < nout) {
-      const int ox = int(o % A.dout[0]), oy = int((o / A.dout[0]) % A.dout[1]);
-      const int oz = int(o / (int64_t(A.dout[0]) * A.dout[1]));
-      const int ix = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
-      const int iy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
-      const int iz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
-      if (ix >= 0 && iy >= 0 && iz >= 0) s = (int64_t(iz) * A.din[1] + iy) * A.din[0] + ix;
+  float         part[EMAX];
+#pragma unroll
+  for (int k = 0; k < EMAX; k++) part[k] = 0.0f;
+  float bpart = 0.0f;
+  for (int ch = 0; ch < chunks; ch++) {
+    const int64_t o0 = (int64_t(blockIdx.x) * chunks + ch) * VC;
+    if (o0 >= nout) break;
+    __syncthreads();
+    for (int t = threadIdx.x; t < VC * COUT; t += kThreads) {
+      const int64_t o = o0 + t / COUT;
+      sg[t]           = o < nout ? A.g[o * COUT + t % COUT] : 0.0f;
     }
-    for (int ci = 0; ci < A.ca; ci++) sx[j * 128 + ci] = s >= 0 ? ld(xa, s * A.ca + ci) : 0.0f;
-    for (int ci = 0; ci < A.cb; ci++) sx[j * 128 + A.ca + ci] = s >= 0 ? ld(xb, s * A.cb + ci) : 0.0f;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < cin * COUT; e += kThreads) {
-    const int ci = e / COUT, co = e % COUT;
-    float     s  = 0.0f;
-    for (int j = 0; j < VC; j++) s += sx[j * 128 + ci] * sg[j * COUT + co];
-    atomicAdd(&A.dw[(int64_t(tap) * cin + ci) * COUT + co], s);
-  }
-  if (tap == 0 && A.db)
-    for (int co = threadIdx.x; co < COUT; co += kThreads) {
-      float s = 0.0f;
-      for (int j = 0; j < VC; j++) s += sg[j * COUT + co];
-      atomicAdd(&A.db[co], s);
+    for (int j = threadIdx.x; j < VC; j += kThreads) {
+      const int64_t o = o0 + j;
+      int64_t       s = -1;
+      if (o < nout) {
+        const int ox = int(o % A.dout[0]), oy = int((o / A.dout[0]) % A.dout[1]);
+        const int oz = int(o / (int64_t(A.dout[0]) * A.dout[1]));
+        const int ix = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
+        const int iy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
+        const int iz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
+        if (ix >= 0 && iy >= 0 && iz >= 0) s = (int64_t(iz) * A.din[1] + iy) * A.din[0] + ix;
+      }
+      for (int ci = 0; ci < A.ca; ci++) sx[j * 128 + ci] = s >= 0 ? ld(xa, s * A.ca + ci) : 0.0f;
+      for (int ci = 0; ci < A.cb; ci++) sx[j * 128 + A.ca + ci] = s >= 0 ? ld(xb, s * A.cb + ci) : 0.0f;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < EMAX; k++) {
+      const int e = int(threadIdx.x) + k * kThreads;
+      if (e < cin * COUT) {
+        const int ci = e / COUT, co = e % COUT;
+        float     s  = 0.0f;
+        for (int j = 0; j < VC; j++) s += sx[j * 128 + ci] * sg[j * COUT + co];
+        part[k] += s;
+      }
+    }
+    if (tap == 0 && int(threadIdx.x) < COUT)
+      for (int j = 0; j < VC; j++) bpart += sg[j * COUT + threadIdx.x];
+  }
+#pragma unroll
+  for (int k = 0; k < EMAX; k++) {
+    const int e = int(threadIdx.x) + k * kThreads;
+    if (e < cin * COUT) atomicAdd(&A.dw[int64_t(tap) * cin * COUT + e], part[k]);
+  }
+  if (tap == 0 && A.db && int(threadIdx.x) < COUT) atomicAdd(&A.db[threadIdx.x], bpart);
 }
 
 // wt[tap'][co][ci] = w[26 - tap'][c0 + ci][co] for ci < cpart: the weights of
@@ -415,10 +484,35 @@ __global__ void k_sub_f(const TI* a, const TI* b, float* y, int64_t n)
     y[i] = float(a[i] - b[i]);
 }
 
+// output nodes below which the channel-parallel kernel runs (one thread per
+// node would fill fewer than 128 workgroups)
+constexpr int64_t kChannelParallelBelow = 32768;
+
+template <int COUT>
+hipError_t launch_conv_cp(const ConvArgs& A, bool in_f32, bool out_f32, hipStream_t s)
+{
+  constexpr int VPB   = kThreads / COUT;
+  const int64_t nout  = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int     cin   = A.ca + A.cb;
+  const size_t  shmem = size_t(VPB) * cin * sizeof(float) + size_t(cin) * COUT * sizeof(uint16_t);
+  const dim3    grid{unsigned((nout + VPB - 1) / VPB)}, block{unsigned(kThreads)};
+  if (in_f32 && out_f32)
+    hipLaunchKernelGGL((k_conv_cp<COUT, float, float>), grid, block, shmem, s, A);
+  else if (in_f32)
+    hipLaunchKernelGGL((k_conv_cp<COUT, float, uint16_t>), grid, block, shmem, s, A);
+  else if (out_f32)
+    hipLaunchKernelGGL((k_conv_cp<COUT, uint16_t, float>), grid, block, shmem, s, A);
+  else
+    hipLaunchKernelGGL((k_conv_cp<COUT, uint16_t, uint16_t>), grid, block, shmem, s, A);
+  return hipGetLastError();
+}
+
 template <int COUT>
 hipError_t launch_conv_t(const ConvArgs& A, bool in_f32, bool out_f32, hipStream_t s)
 {
   const int64_t nout  = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  if constexpr (COUT >= 16)
+    if (nout < kChannelParallelBelow) return launch_conv_cp<COUT>(A, in_f32, out_f32, s);
   const size_t  shmem = size_t(A.ca + A.cb) * COUT * sizeof(uint16_t);
   const dim3    grid{blocks_for(nout)}, block{unsigned(kThreads)};
   if (in_f32 && out_f32)
@@ -436,11 +530,14 @@ template <int COUT>
 hipError_t launch_wgrad_t(const WGradArgs& A, bool in_f32, hipStream_t s)
 {
   const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
-  const dim3    grid{unsigned((nout + 63) / 64), 27u}, block{unsigned(kThreads)};
+  const int64_t runs = (nout + 63) / 64;
+  // about 3072 workgroups over the 27 taps, at most 32 runs of 64 nodes each
+  const int  chunks = int(std::max<int64_t>(1, std::min<int64_t>(32, runs * 27 / 3072)));
+  const dim3 grid{unsigned((runs + chunks - 1) / chunks), 27u}, block{unsigned(kThreads)};
   if (in_f32)
-    hipLaunchKernelGGL((k_conv_wgrad<COUT, float>), grid, block, 0, s, A);
+    hipLaunchKernelGGL((k_conv_wgrad<COUT, float>), grid, block, 0, s, A, chunks);
   else
-    hipLaunchKernelGGL((k_conv_wgrad<COUT, uint16_t>), grid, block, 0, s, A);
+    hipLaunchKernelGGL((k_conv_wgrad<COUT, uint16_t>), grid, block, 0, s, A, chunks);
   return hipGetLastError();
 }
 

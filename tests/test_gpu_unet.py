@@ -76,7 +76,10 @@ def reference(params, x, dims):
     return conv(c0, 11, relu=False)
 
 
-@pytest.mark.parametrize("dims", [(17, 17, 17), (17, 9, 33)])
+# below 32768 output nodes a convolution runs one lane per (node, channel);
+# the 41 x 33 x 33 grid also exercises the one-thread-per-node kernel on its
+# finest level
+@pytest.mark.parametrize("dims", [(17, 17, 17), (17, 9, 33), (41, 33, 33)])
 def test_unet_forward_backward_match_torch(gpu_lib, dims):
     net = sx.UNet(*dims, seed=3)
     rng = np.random.default_rng(5)
