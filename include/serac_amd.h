@@ -102,6 +102,19 @@ int sx_mesh_create(int nverts, const double* coords, int nelem, const int* hex_v
 /* Moves interior vertices of a cuboid mesh by h_d * U[-amp, amp] per
  * coordinate (counter-based RNG keyed by seed, vertex id and coordinate). */
 int sx_mesh_perturb(sx_mesh* mesh, uint64_t seed, double amp);
+/* Replaces buildMeshFromFile -> mfem::Mesh(std::istream&)
+ * (src/serac/mesh/mesh_utils.cpp:19-43) for MFEM mesh v1.0 text (the format of
+ * data/meshes/*.mesh): 3D hexahedra (geometry 5), quadrilateral boundary
+ * elements (geometry 3), straight (vertex) geometry.  Element and boundary
+ * attributes are kept.  Curved meshes (`nodes`) and other element types give
+ * SX_ERR_UNSUPPORTED; malformed text gives SX_ERR_ARG.  Cuboid meshes carry
+ * mfem's Cartesian boundary attributes: 1 z=0, 2 y=0, 3 x=max, 4 y=max,
+ * 5 x=0, 6 z=max. */
+int sx_mesh_read_mfem(const char* text, int64_t nbytes, sx_mesh** out);
+int sx_mesh_read_mfem_file(const char* path, sx_mesh** out);
+int sx_mesh_attributes(const sx_mesh* mesh, int32_t* attrs_out);                    /* host [nelem] */
+int sx_mesh_boundary_info(const sx_mesh* mesh, int64_t* nbdr);
+int sx_mesh_boundary(const sx_mesh* mesh, int32_t* quads_out, int32_t* attrs_out); /* host [nbdr][4], [nbdr] */
 int sx_mesh_info(const sx_mesh* mesh, int64_t* nverts, int64_t* nelem);
 int sx_mesh_vertices(const sx_mesh* mesh, double* coords_out); /* host [nverts][3] */
 int sx_mesh_destroy(sx_mesh* mesh);
@@ -119,6 +132,12 @@ int sx_space_node_coords(const sx_space* s, double* xyz_out);    /* host [nnodes
  * for the components selected by comp_mask. Writes up to cap sorted DOF ids. */
 int sx_space_boundary_dofs(const sx_space* s, int face_mask, int comp_mask, int32_t* dofs_out, int64_t cap,
                            int64_t* count);
+/* Essential DOFs on the boundary elements whose attribute a has bit a-1 set
+ * in attr_mask (the essential boundary attribute sets of serac's
+ * setDisplacementBCs / setTemperatureBCs, phy/solid_mechanics.hpp), for the
+ * components in comp_mask; works on any hex mesh with boundary elements. */
+int sx_space_boundary_dofs_attr(const sx_space* s, uint64_t attr_mask, int comp_mask, int32_t* dofs_out, int64_t cap,
+                                int64_t* count);
 /* ElementRestriction::Gather / ScatterAdd (fn/element_restriction.cpp:443-467),
  * device pointers, bit-exact with the reference's serial loops. */
 int sx_space_gather(const sx_space* s, const double* L, double* E);
@@ -149,6 +168,11 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
  * whole boundary (EntireBoundary).  qfunc_id is one of SX_BQF_*. */
 int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams,
                                   const int* depends_on, int ndep, int face_mask);
+/* The same on the boundary elements whose attribute a has bit a-1 set in
+ * attr_mask (Domain::ofBoundaryElements by attribute).  Each selected boundary
+ * element must be an exterior face of the mesh. */
+int sx_form_add_boundary_integral_attr(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                       const int* depends_on, int ndep, uint64_t attr_mask);
 /* AddDomainIntegral with a QuadratureData argument (fn/functional.hpp:286-317,
  * fn/quadrature_data.hpp:111-167).  qdata must hold one record per element of
  * the domain (in domain order) and quadrature point; it is borrowed and must

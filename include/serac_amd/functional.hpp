@@ -19,6 +19,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <set>
 #include <stdexcept>
 #include <string>
 #include <tuple>
@@ -42,6 +43,16 @@ namespace detail {
 inline void check(int rc)
 {
   if (rc != SX_OK) throw Error(rc, sx_last_error());
+}
+/// bit a - 1 for each attribute a in 1..64
+inline uint64_t attribute_mask(const std::set<int>& attributes)
+{
+  uint64_t m = 0;
+  for (int a : attributes) {
+    if (a < 1 || a > 64) throw Error(SX_ERR_ARG, "attribute " + std::to_string(a) + " outside 1..64");
+    m |= uint64_t(1) << (a - 1);
+  }
+  return m;
 }
 }  // namespace detail
 
@@ -149,6 +160,13 @@ class Mesh {
   {
     if (h_) sx_mesh_destroy(h_);
   }
+  /// buildMeshFromFile (src/serac/mesh/mesh_utils.cpp:19-43) for MFEM v1.0 hex meshes
+  static Mesh FromFile(const std::string& path)
+  {
+    Mesh m;
+    detail::check(sx_mesh_read_mfem_file(path.c_str(), &m.h_));
+    return m;
+  }
   void    Perturb(uint64_t seed, double amp) { detail::check(sx_mesh_perturb(h_, seed, amp)); }
   int64_t GetNE() const
   {
@@ -209,6 +227,17 @@ class FiniteElementSpace {
     detail::check(sx_space_boundary_dofs(h_, face_mask, comp_mask, d.data(), n, &n));
     return d;
   }
+  /// essential DOFs on the boundary elements with the given attributes, the
+  /// attribute sets of setDisplacementBCs / setTemperatureBCs
+  std::vector<int32_t> BoundaryDofs(const std::set<int>& attributes, int comp_mask = 0x7) const
+  {
+    const uint64_t mask = detail::attribute_mask(attributes);
+    int64_t        n    = 0;
+    detail::check(sx_space_boundary_dofs_attr(h_, mask, comp_mask, nullptr, 0, &n));
+    std::vector<int32_t> d(static_cast<size_t>(n));
+    detail::check(sx_space_boundary_dofs_attr(h_, mask, comp_mask, d.data(), n, &n));
+    return d;
+  }
   sx_space* handle() const { return h_; }
 
  private:
@@ -248,12 +277,18 @@ struct Domain {
   static Domain    Elements(std::vector<int> ids) { return Domain{std::move(ids)}; }
 };
 
-/// boundary faces on planes of the mesh's bounding box (serac::Domain for
-/// boundary elements, EntireBoundary when face_mask = 0)
+/// boundary faces (serac::Domain for boundary elements): on planes of the
+/// mesh's bounding box (EntireBoundary when face_mask = 0), or the boundary
+/// elements with chosen attributes (Domain::ofBoundaryElements by attribute)
 struct BoundaryDomain {
   int                   face_mask = 0;  // bit 0 x = min, 1 x = max, 2 y = min, 3 y = max, 4 z = min, 5 z = max
-  static BoundaryDomain EntireBoundary() { return BoundaryDomain{0}; }
-  static BoundaryDomain Planes(int mask) { return BoundaryDomain{mask}; }
+  uint64_t              attr_mask = 0;  // bit a - 1 for attribute a; when nonzero it selects the faces
+  static BoundaryDomain EntireBoundary() { return BoundaryDomain{0, 0}; }
+  static BoundaryDomain Planes(int mask) { return BoundaryDomain{mask, 0}; }
+  static BoundaryDomain Attributes(const std::set<int>& attributes)
+  {
+    return BoundaryDomain{0, detail::attribute_mask(attributes)};
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -705,8 +740,12 @@ class Functional<test(trials...)> {
     static_assert(((args >= 0 && args < num_trial_spaces) && ...), "DependsOn index out of range");
     const int                 deps[] = {args...};
     const std::vector<double> p      = bqfunction_traits<Integrand>::params(integrand);
-    detail::check(sx_form_add_boundary_integral(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
-                                                int(p.size()), deps, int(sizeof...(args)), domain.face_mask));
+    if (domain.attr_mask)
+      detail::check(sx_form_add_boundary_integral_attr(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
+                                                       int(p.size()), deps, int(sizeof...(args)), domain.attr_mask));
+    else
+      detail::check(sx_form_add_boundary_integral(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
+                                                  int(p.size()), deps, int(sizeof...(args)), domain.face_mask));
   }
   template <int dim, int... args, typename Integrand>
   void AddBoundaryIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&)

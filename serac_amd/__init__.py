@@ -7,6 +7,7 @@ mirror used by the tests and the benchmark: numpy arrays in host memory,
 ``DeviceVector`` for device memory owned by the library.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -83,6 +84,16 @@ class DeviceVector:
             pass
 
 
+def attr_mask(attrs):
+    """Bit a - 1 for each attribute a in 1..64."""
+    m = 0
+    for a in attrs:
+        if not 1 <= int(a) <= 64:
+            raise ValueError(f"attribute {a} outside 1..64")
+        m |= 1 << (int(a) - 1)
+    return m
+
+
 class Mesh:
     def __init__(self, handle):
         self.h = handle
@@ -104,6 +115,32 @@ class Mesh:
         capi.call("sx_mesh_create", len(X), _dptr(X), len(hexes),
                   hexes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), ctypes.byref(h))
         return cls(h)
+
+    @classmethod
+    def read_mfem(cls, path=None, text=None):
+        """buildMeshFromFile for MFEM mesh v1.0 hex meshes: from a file path or from the text itself."""
+        h = _vp()
+        if text is not None:
+            raw = text.encode() if isinstance(text, str) else bytes(text)
+            capi.call("sx_mesh_read_mfem", raw, len(raw), ctypes.byref(h))
+        else:
+            capi.call("sx_mesh_read_mfem_file", os.fsencode(path), ctypes.byref(h))
+        return cls(h)
+
+    def attributes(self):
+        """Element attributes [ne]."""
+        out = np.zeros(self.ne, dtype=np.int32)
+        capi.call("sx_mesh_attributes", self.h, _iptr(out))
+        return out
+
+    def boundary(self):
+        """Boundary quadrilaterals [nbdr][4] (vertex ids) and their attributes [nbdr]."""
+        nb = ctypes.c_int64()
+        capi.call("sx_mesh_boundary_info", self.h, ctypes.byref(nb))
+        quads = np.zeros((nb.value, 4), dtype=np.int32)
+        attrs = np.zeros(nb.value, dtype=np.int32)
+        capi.call("sx_mesh_boundary", self.h, _iptr(quads), _iptr(attrs))
+        return quads, attrs
 
     def perturb(self, seed, amp):
         capi.call("sx_mesh_perturb", self.h, seed, amp)
@@ -146,6 +183,15 @@ class Space:
         capi.call("sx_space_boundary_dofs", self.h, face_mask, comp_mask, None, 0, ctypes.byref(cnt))
         out = np.zeros(cnt.value, dtype=np.int32)
         capi.call("sx_space_boundary_dofs", self.h, face_mask, comp_mask, _iptr(out), cnt.value, ctypes.byref(cnt))
+        return out
+
+    def boundary_dofs_attr(self, attrs, comp_mask=0x7):
+        """Essential DOFs on the boundary elements with an attribute in `attrs` (1..64)."""
+        mask = attr_mask(attrs)
+        cnt = ctypes.c_int64()
+        capi.call("sx_space_boundary_dofs_attr", self.h, mask, comp_mask, None, 0, ctypes.byref(cnt))
+        out = np.zeros(cnt.value, dtype=np.int32)
+        capi.call("sx_space_boundary_dofs_attr", self.h, mask, comp_mask, _iptr(out), cnt.value, ctypes.byref(cnt))
         return out
 
     def gather(self, L):
@@ -430,14 +476,19 @@ class Functional:
         if qdata is not None:
             self._qdata = getattr(self, "_qdata", []) + [qdata]  # the form borrows it
 
-    def add_boundary_integral(self, qfunc, params, depends_on=(0,), face_mask=0):
+    def add_boundary_integral(self, qfunc, params, depends_on=(0,), face_mask=0, attrs=None):
         """AddBoundaryIntegral over the bounding-box planes of face_mask
-        (bit 0 x = min, 1 x = max, 2 y = min, ...; 0: the whole boundary)."""
+        (bit 0 x = min, 1 x = max, 2 y = min, ...; 0: the whole boundary), or,
+        with `attrs`, over the boundary elements with those attributes."""
         p = np.zeros(16)
         p[: len(params)] = params
         dep = (ctypes.c_int * len(depends_on))(*depends_on)
-        capi.call("sx_form_add_boundary_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on),
-                  face_mask)
+        if attrs is not None:
+            capi.call("sx_form_add_boundary_integral_attr", self.h, qfunc, _dptr(p), len(params), dep,
+                      len(depends_on), attr_mask(attrs))
+        else:
+            capi.call("sx_form_add_boundary_integral", self.h, qfunc, _dptr(p), len(params), dep, len(depends_on),
+                      face_mask)
 
     def set_preconditioner(self, learned, mode):
         """Linear solves of newton / quasistatic_solve: SX_PRECOND_JACOBI, _RECORD or _LEARNED."""

@@ -87,6 +87,13 @@ SIGNATURES = {
                                      ctypes.POINTER(_vp)]),
     "sx_mesh_create": (_int, [_int, _D, _int, ctypes.POINTER(_int), ctypes.POINTER(_vp)]),
     "sx_mesh_perturb": (_int, [_vp, ctypes.c_uint64, ctypes.c_double]),
+    "sx_mesh_read_mfem": (_int, [ctypes.c_char_p, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "sx_mesh_read_mfem_file": (_int, [ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "sx_mesh_attributes": (_int, [_vp, _I32]),
+    "sx_mesh_boundary_info": (_int, [_vp, _I64]),
+    "sx_mesh_boundary": (_int, [_vp, _I32, _I32]),
+    "sx_space_boundary_dofs_attr": (_int, [_vp, ctypes.c_uint64, _int, _I32, ctypes.c_int64, _I64]),
+    "sx_form_add_boundary_integral_attr": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, ctypes.c_uint64]),
     "sx_mesh_info": (_int, [_vp, _I64, _I64]),
     "sx_mesh_vertices": (_int, [_vp, _D]),
     "sx_mesh_destroy": (_int, [_vp]),

@@ -421,20 +421,19 @@ struct BoundaryFace {
   int e, lf;
 };
 
-// faces that belong to exactly one element, in (element, local face) order,
-// restricted by face_mask to the bounding-box planes (bit 0 x = min, 1 x = max,
-// 2 y = min, ...); face_mask 0 selects the whole boundary
-std::vector<BoundaryFace> boundary_faces(const sxh::HexMesh& m, int face_mask)
+struct FaceKey {
+  std::array<int, 4> v;  // sorted corner vertices
+  int                e, lf;
+};
+
+// every (element, local face) keyed by its sorted corner vertices
+std::vector<FaceKey> element_face_keys(const sxh::HexMesh& m)
 {
-  struct Key {
-    std::array<int, 4> v;
-    int                e, lf;
-  };
-  std::vector<Key> keys;
+  std::vector<FaceKey> keys;
   keys.reserve(size_t(m.ne) * 6);
   for (int64_t e = 0; e < m.ne; e++)
     for (int lf = 0; lf < 6; lf++) {
-      Key k;
+      FaceKey k;
       for (int c = 0; c < 4; c++) {
         int i0, i1, i2;
         face_node(lf, c & 1, c >> 1, 2, i0, i1, i2);
@@ -445,7 +444,46 @@ std::vector<BoundaryFace> boundary_faces(const sxh::HexMesh& m, int face_mask)
       k.lf = lf;
       keys.push_back(k);
     }
-  std::vector<size_t> order(keys.size());
+  return keys;
+}
+
+// the faces of the boundary elements whose attribute a has bit a - 1 set in
+// attr_mask, in boundary-element order (the selection of
+// Domain::ofBoundaryElements by attribute and of the essential boundary
+// attribute sets).  A boundary element must be a face of the mesh; with
+// exterior_only it must belong to exactly one element, whose face it is.
+std::vector<BoundaryFace> attribute_faces(const sxh::HexMesh& m, uint64_t attr_mask, bool exterior_only)
+{
+  std::vector<FaceKey> keys = element_face_keys(m);
+  std::sort(keys.begin(), keys.end(), [](const FaceKey& a, const FaceKey& b) {
+    return a.v != b.v ? a.v < b.v : (a.e != b.e ? a.e < b.e : a.lf < b.lf);
+  });
+  std::vector<BoundaryFace> out;
+  const size_t              nb = m.bdr_attr.size();
+  for (size_t q = 0; q < nb; q++) {
+    const int a = m.bdr_attr[q];
+    if (a < 1 || a > 64 || !((attr_mask >> (a - 1)) & 1)) continue;
+    FaceKey k{};
+    for (int c = 0; c < 4; c++) k.v[size_t(c)] = m.bdr[4 * q + size_t(c)];
+    std::sort(k.v.begin(), k.v.end());
+    auto lo = std::lower_bound(keys.begin(), keys.end(), k, [](const FaceKey& x, const FaceKey& y) { return x.v < y.v; });
+    SX_REQUIRE(lo != keys.end() && lo->v == k.v, SX_ERR_ARG,
+               "boundary element " + std::to_string(q) + " is not a face of the mesh");
+    const bool shared = (lo + 1) != keys.end() && (lo + 1)->v == k.v;
+    SX_REQUIRE(!(exterior_only && shared), SX_ERR_UNSUPPORTED,
+               "boundary element " + std::to_string(q) + " is an interior face; boundary integrals need exterior faces");
+    out.push_back({lo->e, lo->lf});
+  }
+  return out;
+}
+
+// faces that belong to exactly one element, in (element, local face) order,
+// restricted by face_mask to the bounding-box planes (bit 0 x = min, 1 x = max,
+// 2 y = min, ...); face_mask 0 selects the whole boundary
+std::vector<BoundaryFace> boundary_faces(const sxh::HexMesh& m, int face_mask)
+{
+  const std::vector<FaceKey> keys = element_face_keys(m);
+  std::vector<size_t>        order(keys.size());
   for (size_t i = 0; i < order.size(); i++) order[i] = i;
   std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return keys[a].v < keys[b].v; });
   std::vector<char> single(keys.size(), 0);
@@ -657,8 +695,64 @@ int sx_mesh_create(int nverts, const double* coords, int nelem, const int* hex_v
     m->m.ne = nelem;
     m->m.X.assign(coords, coords + size_t(nverts) * 3);
     m->m.hex.assign(hex_vertices, hex_vertices + size_t(nelem) * 8);
+    m->m.attr.assign(size_t(nelem), 1);
     for (int v : m->m.hex) SX_REQUIRE(v >= 0 && v < nverts, SX_ERR_ARG, "hex vertex index out of range");
     *out = m.release();
+  });
+}
+
+int sx_mesh_read_mfem(const char* text, int64_t nbytes, sx_mesh** out)
+{
+  return api([&] {
+    SX_REQUIRE(text && out && nbytes >= 0, SX_ERR_ARG, "null argument");
+    auto m = std::make_unique<sx_mesh>();
+    try {
+      m->m = sxh::parse_mfem_mesh(std::string(text, size_t(nbytes)));
+    } catch (const sxh::mesh_unsupported& e) {
+      throw sx_failure(SX_ERR_UNSUPPORTED, e.what());
+    }
+    *out = m.release();
+  });
+}
+
+int sx_mesh_read_mfem_file(const char* path, sx_mesh** out)
+{
+  return api([&] {
+    SX_REQUIRE(path && out, SX_ERR_ARG, "null argument");
+    std::FILE* fp = std::fopen(path, "rb");
+    SX_REQUIRE(fp, SX_ERR_ARG, std::string("cannot open mesh file '") + path + "'");
+    std::string text;
+    char        buf[65536];
+    size_t      got;
+    while ((got = std::fread(buf, 1, sizeof(buf), fp)) > 0) text.append(buf, got);
+    std::fclose(fp);
+    const int rc = sx_mesh_read_mfem(text.data(), int64_t(text.size()), out);
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  });
+}
+
+int sx_mesh_attributes(const sx_mesh* mesh, int32_t* attrs)
+{
+  return api([&] {
+    SX_REQUIRE(mesh && attrs, SX_ERR_ARG, "null argument");
+    for (int64_t e = 0; e < mesh->m.ne; e++) attrs[e] = mesh->m.attr.empty() ? 1 : mesh->m.attr[size_t(e)];
+  });
+}
+
+int sx_mesh_boundary_info(const sx_mesh* mesh, int64_t* nbdr)
+{
+  return api([&] {
+    SX_REQUIRE(mesh && nbdr, SX_ERR_ARG, "null argument");
+    *nbdr = int64_t(mesh->m.bdr_attr.size());
+  });
+}
+
+int sx_mesh_boundary(const sx_mesh* mesh, int32_t* quads, int32_t* attrs)
+{
+  return api([&] {
+    SX_REQUIRE(mesh, SX_ERR_ARG, "null mesh");
+    if (quads) std::copy(mesh->m.bdr.begin(), mesh->m.bdr.end(), quads);
+    if (attrs) std::copy(mesh->m.bdr_attr.begin(), mesh->m.bdr_attr.end(), attrs);
   });
 }
 
@@ -797,6 +891,34 @@ int sx_space_boundary_dofs(const sx_space* s, int face_mask, int comp_mask, int3
               dofs.push_back(int32_t(s->ordering ? node * s->nc + c : int64_t(c) * s->nnodes + node));
         }
     std::sort(dofs.begin(), dofs.end());
+    *count = int64_t(dofs.size());
+    if (dofs_out) {
+      SX_REQUIRE(cap >= int64_t(dofs.size()), SX_ERR_ARG, "boundary DOF buffer too small");
+      std::memcpy(dofs_out, dofs.data(), dofs.size() * sizeof(int32_t));
+    }
+  });
+}
+
+int sx_space_boundary_dofs_attr(const sx_space* s, uint64_t attr_mask, int comp_mask, int32_t* dofs_out, int64_t cap,
+                                int64_t* count)
+{
+  return api([&] {
+    SX_REQUIRE(s && count, SX_ERR_ARG, "null argument");
+    const int                 n     = s->order + 1;
+    std::vector<BoundaryFace> faces = attribute_faces(s->mesh->m, attr_mask, false);
+    std::vector<int32_t>      dofs;
+    for (const auto& fc : faces)
+      for (int b = 0; b < n; b++)
+        for (int a = 0; a < n; a++) {
+          int i0, i1, i2;
+          face_node(fc.lf, a, b, n, i0, i1, i2);
+          const int64_t node = s->emap[size_t(fc.e) * size_t(s->npe) + size_t((i2 * n + i1) * n + i0)];
+          for (int c = 0; c < s->nc; c++)
+            if ((comp_mask >> c) & 1)
+              dofs.push_back(int32_t(s->ordering ? node * s->nc + c : int64_t(c) * s->nnodes + node));
+        }
+    std::sort(dofs.begin(), dofs.end());
+    dofs.erase(std::unique(dofs.begin(), dofs.end()), dofs.end());
     *count = int64_t(dofs.size());
     if (dofs_out) {
       SX_REQUIRE(cap >= int64_t(dofs.size()), SX_ERR_ARG, "boundary DOF buffer too small");
@@ -946,8 +1068,13 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
   });
 }
 
-int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams,
-                                  const int* depends_on, int ndep, int face_mask)
+namespace {
+
+// selects the faces of a boundary integral on the form's test-space mesh
+using FaceSelector = std::function<std::vector<BoundaryFace>(const sxh::HexMesh&)>;
+
+int add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
+                          int ndep, const FaceSelector& select)
 {
   return api([&] {
     SX_REQUIRE(f, SX_ERR_ARG, "null form");
@@ -956,7 +1083,6 @@ int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params
     SX_REQUIRE(is_boundary_qf(qfunc_id), SX_ERR_UNSUPPORTED, "unknown boundary qfunction id " + std::to_string(qfunc_id));
     SX_REQUIRE(ndep == 1 && depends_on, SX_ERR_UNSUPPORTED,
                "compiled boundary qfunctions take exactly one trial argument (DependsOn<i>)");
-    SX_REQUIRE(face_mask >= 0 && face_mask < 64, SX_ERR_ARG, "face_mask selects among 6 planes");
     const int trial = depends_on[0];
     SX_REQUIRE(trial >= 0 && trial < int(f->trials.size()), SX_ERR_ARG, "DependsOn index out of range");
     const sx_space* vs = f->test;
@@ -973,7 +1099,7 @@ int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params
     B->depends_on = {trial};
     const auto& m = vs->mesh->m;
     const int   n = vs->order + 1, nf = n * n;
-    std::vector<BoundaryFace> faces = boundary_faces(m, face_mask);
+    std::vector<BoundaryFace> faces = select(m);
     // greedy colouring: faces of one colour share no node
     std::vector<int> fnodes(faces.size() * size_t(nf));
     for (size_t i = 0; i < faces.size(); i++)
@@ -1019,6 +1145,27 @@ int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params
     B->d_fmap.upload(map);
     B->d_corners.upload(corners);
     f->bintegrals.push_back(std::move(B));
+  });
+}
+
+}  // namespace
+
+int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                  const int* depends_on, int ndep, int face_mask)
+{
+  if (!(face_mask >= 0 && face_mask < 64)) {
+    g_last_error = "face_mask selects among 6 planes";
+    return SX_ERR_ARG;
+  }
+  return add_boundary_integral(f, qfunc_id, params, nparams, depends_on, ndep,
+                               [face_mask](const sxh::HexMesh& m) { return boundary_faces(m, face_mask); });
+}
+
+int sx_form_add_boundary_integral_attr(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                       const int* depends_on, int ndep, uint64_t attr_mask)
+{
+  return add_boundary_integral(f, qfunc_id, params, nparams, depends_on, ndep, [attr_mask](const sxh::HexMesh& m) {
+    return attribute_faces(m, attr_mask, true);
   });
 }
 

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <cstdint>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
@@ -116,8 +117,11 @@ constexpr int kCornerToVertex[8] = {0, 1, 3, 2, 4, 5, 7, 6};
 
 struct HexMesh {
   int64_t             nv = 0, ne = 0;
-  std::vector<double> X;    // [nv][3]
-  std::vector<int>    hex;  // [ne][8], mfem order
+  std::vector<double> X;         // [nv][3]
+  std::vector<int>    hex;       // [ne][8], mfem order
+  std::vector<int>    attr;      // [ne] element attributes (1 when not given)
+  std::vector<int>    bdr;       // [nb][4] boundary quadrilaterals (vertex ids)
+  std::vector<int>    bdr_attr;  // [nb] boundary attributes
   bool                structured = false;
   int                 n[3]       = {0, 0, 0};
   double              len[3]     = {1, 1, 1};
@@ -152,6 +156,137 @@ inline HexMesh make_cuboid(int nx, int ny, int nz, double lx, double ly, double 
         for (int c = 0; c < 8; c++)
           m.hex[size_t(e) * 8 + kCornerToVertex[c]] =
               int(m.vertex_id(ex + (c & 1), ey + ((c >> 1) & 1), ez + ((c >> 2) & 1)));
+  m.attr.assign(size_t(m.ne), 1);
+  // boundary quadrilaterals with mfem's Cartesian attributes: 1 z = 0, 2 y = 0,
+  // 3 x = max, 4 y = max, 5 x = 0, 6 z = max (vertices counter-clockwise seen
+  // from outside)
+  auto quad = [&](int attr, int64_t a, int64_t b, int64_t c, int64_t d) {
+    m.bdr.insert(m.bdr.end(), {int(a), int(b), int(c), int(d)});
+    m.bdr_attr.push_back(attr);
+  };
+  for (int j = 0; j < ny; j++)
+    for (int i = 0; i < nx; i++) {
+      quad(1, m.vertex_id(i, j, 0), m.vertex_id(i, j + 1, 0), m.vertex_id(i + 1, j + 1, 0), m.vertex_id(i + 1, j, 0));
+      quad(6, m.vertex_id(i, j, nz), m.vertex_id(i + 1, j, nz), m.vertex_id(i + 1, j + 1, nz),
+           m.vertex_id(i, j + 1, nz));
+    }
+  for (int k = 0; k < nz; k++)
+    for (int i = 0; i < nx; i++) {
+      quad(2, m.vertex_id(i, 0, k), m.vertex_id(i + 1, 0, k), m.vertex_id(i + 1, 0, k + 1), m.vertex_id(i, 0, k + 1));
+      quad(4, m.vertex_id(i, ny, k), m.vertex_id(i, ny, k + 1), m.vertex_id(i + 1, ny, k + 1),
+           m.vertex_id(i + 1, ny, k));
+    }
+  for (int k = 0; k < nz; k++)
+    for (int j = 0; j < ny; j++) {
+      quad(5, m.vertex_id(0, j, k), m.vertex_id(0, j, k + 1), m.vertex_id(0, j + 1, k + 1), m.vertex_id(0, j + 1, k));
+      quad(3, m.vertex_id(nx, j, k), m.vertex_id(nx, j + 1, k), m.vertex_id(nx, j + 1, k + 1),
+           m.vertex_id(nx, j, k + 1));
+    }
+  return m;
+}
+
+// Unsupported content of an otherwise well-formed mesh file.
+struct mesh_unsupported : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// MFEM mesh v1.0 text, the format of the reference's data/meshes/*.mesh and of
+// mfem::Mesh(std::istream&) behind buildMeshFromFile
+// (src/serac/mesh/mesh_utils.cpp:19-43): sections `dimension`, `elements`
+// (`attr geom v...`), `boundary` and `vertices` (count, space dimension, one
+// coordinate row per vertex); `#` starts a comment.  Only 3D meshes of
+// hexahedra (geometry 5) with quadrilateral boundary elements (geometry 3)
+// and straight vertex geometry are accepted; curved meshes (a `nodes`
+// section) and other element types raise mesh_unsupported.
+inline HexMesh parse_mfem_mesh(const std::string& text)
+{
+  std::vector<std::string> lines;
+  {
+    std::istringstream in(text);
+    std::string        line;
+    while (std::getline(in, line)) {
+      const size_t h = line.find('#');
+      if (h != std::string::npos) line.resize(h);
+      const size_t b = line.find_first_not_of(" \t\r");
+      if (b == std::string::npos) continue;
+      const size_t e = line.find_last_not_of(" \t\r");
+      lines.push_back(line.substr(b, e - b + 1));
+    }
+  }
+  auto bad = [](const std::string& what) { return std::invalid_argument("MFEM mesh: " + what); };
+  if (lines.empty() || lines[0].rfind("MFEM mesh v1.0", 0) != 0) throw bad("expected the header 'MFEM mesh v1.0'");
+  auto ints = [&](size_t i, size_t want) {
+    if (i >= lines.size()) throw bad("truncated file");
+    std::istringstream  in(lines[i]);
+    std::vector<double> v;
+    double              x;
+    while (in >> x) v.push_back(x);
+    if (v.size() != want) throw bad("line '" + lines[i] + "' has " + std::to_string(v.size()) + " numbers, expected " +
+                                    std::to_string(want));
+    return v;
+  };
+  auto count = [&](size_t i) {
+    if (i >= lines.size()) throw bad("truncated file");
+    const double v = ints(i, 1)[0];
+    if (v < 0 || v != std::floor(v)) throw bad("bad count '" + lines[i] + "'");
+    return int64_t(v);
+  };
+  HexMesh m;
+  bool    have_dim = false, have_elems = false, have_verts = false;
+  size_t  i        = 1;
+  while (i < lines.size()) {
+    const std::string& key = lines[i];
+    if (key == "dimension") {
+      if (count(i + 1) != 3) throw mesh_unsupported("MFEM mesh: only 3D meshes are supported");
+      have_dim = true;
+      i += 2;
+    } else if (key == "elements" || key == "boundary") {
+      const bool    elem = key == "elements";
+      const int64_t n    = count(i + 1);
+      for (int64_t k = 0; k < n; k++) {
+        const size_t li = i + 2 + size_t(k);
+        if (li >= lines.size()) throw bad("truncated " + key + " section");
+        std::istringstream in(lines[li]);
+        int                attr = 0, geom = -1;
+        in >> attr >> geom;
+        if (elem && geom != 5) throw mesh_unsupported("MFEM mesh: only hexahedra (geometry 5) are supported");
+        if (!elem && geom != 3)
+          throw mesh_unsupported("MFEM mesh: boundary elements of a hex mesh must be quadrilaterals (geometry 3)");
+        const int nvtx = elem ? 8 : 4;
+        const auto v   = ints(li, size_t(2 + nvtx));
+        for (int c = 0; c < nvtx; c++) (elem ? m.hex : m.bdr).push_back(int(v[size_t(2 + c)]));
+        (elem ? m.attr : m.bdr_attr).push_back(attr);
+      }
+      if (elem) {
+        m.ne       = n;
+        have_elems = true;
+      }
+      i += 2 + size_t(n);
+    } else if (key == "vertices") {
+      m.nv = count(i + 1);
+      if (i + 2 < lines.size() && lines[i + 2] == "nodes")
+        throw mesh_unsupported("MFEM mesh: curved meshes (a nodes section) are not supported");
+      if (count(i + 2) != 3) throw mesh_unsupported("MFEM mesh: vertices must have 3 coordinates");
+      for (int64_t k = 0; k < m.nv; k++) {
+        const auto x = ints(i + 3 + size_t(k), 3);
+        m.X.insert(m.X.end(), x.begin(), x.end());
+      }
+      have_verts = true;
+      i += 3 + size_t(m.nv);
+    } else if (key == "nodes") {
+      throw mesh_unsupported("MFEM mesh: curved meshes (a nodes section) are not supported");
+    } else if (key == "mfem_serial_mesh_end" || key == "mfem_mesh_end") {
+      break;
+    } else {
+      throw bad("unknown section '" + key + "'");
+    }
+  }
+  if (!have_dim || !have_elems || !have_verts) throw bad("missing dimension, elements or vertices section");
+  if (m.ne == 0 || m.nv == 0) throw bad("empty mesh");
+  for (int v : m.hex)
+    if (v < 0 || v >= m.nv) throw bad("element vertex index out of range");
+  for (int v : m.bdr)
+    if (v < 0 || v >= m.nv) throw bad("boundary vertex index out of range");
   return m;
 }
 

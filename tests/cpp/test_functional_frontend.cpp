@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <set>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -312,6 +314,41 @@ int main()
       thrown = e.code == SX_ERR_ARG;
     }
     CHECK(thrown, "vector qfunction on a scalar space raises Error(SX_ERR_ARG)");
+  }
+
+  // buildMeshFromFile + boundary attribute sets on the reference's beam
+  // (data/meshes/beam-hex.mesh: attribute 1 at x = 0, 2 at x = 8)
+  {
+    const char* dir  = std::getenv("SERAC_AMD_MESH_DIR");
+    std::string path = std::string(dir ? dir : "tests/golden/meshes") + "/beam-hex.mesh";
+    Mesh        beam = Mesh::FromFile(path);
+    CHECK(beam.GetNE() == 8, "beam-hex.mesh has 8 hexes (%lld)", (long long)beam.GetNE());
+    FiniteElementSpace bfes(beam, 2, 3);
+    const auto         sup = bfes.BoundaryDofs(std::set<int>{1}), both = bfes.BoundaryDofs(std::set<int>{1, 2});
+    CHECK(sup.size() == 27 && both.size() == 54, "attribute DOFs: %zu on {1}, %zu on {1, 2}", sup.size(),
+          both.size());
+    const std::vector<double> bx = bfes.NodeCoordinates();
+    bool                      on = true;
+    for (int32_t d : sup) on = on && std::fabs(bx[size_t(d / 3) * 3]) < 1e-12;
+    CHECK(on, "attribute 1 DOFs lie on x = 0");
+    solid_mechanics::LinearTraction trac{};
+    trac.b[2] = -0.5;
+    Vector                 ub(std::vector<double>(static_cast<size_t>(bfes.GetTrueVSize()), 0.0));
+    std::vector<double>    res[2];
+    for (int k = 0; k < 2; k++) {
+      Functional<H1<2, 3>(H1<2, 3>)> bf(&bfes, {&bfes});
+      bf.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                           solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                               {1.0, 10.0, 1.0}, GeometricNonlinearities::On),
+                           beam);
+      bf.AddSurfaceIntegral(DependsOn<0>{}, trac,
+                            k == 0 ? BoundaryDomain::Attributes({2}) : BoundaryDomain::Planes(0b000010));
+      res[k] = bf(0.0, ub).ToHost();
+    }
+    double total = 0.0;
+    for (size_t i = 2; i < res[0].size(); i += 3) total += res[0][i];
+    CHECK(relerr(res[0], res[1]) < 1e-14 && std::fabs(total - 0.5) < 1e-12,
+          "tip traction on attribute 2 == on the x = max plane (%.2e), total %.15f", relerr(res[0], res[1]), total);
   }
 
   orc_form_free(of);
