@@ -196,7 +196,16 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
  *     Must be set before the first AddDomainIntegral.
  *   SX_OPT_XCD_REMAP: 1 (default) gives each of the 8 XCDs a contiguous run of
  *     element blocks (L2 locality for shared nodes); 0 keeps hardware order. */
-enum { SX_OPT_SCATTER = 1, SX_OPT_XCD_REMAP = 2, SX_OPT_TANGENT_PRECISION = 3, SX_OPT_JVP_ARITHMETIC = 4 };
+enum {
+  SX_OPT_SCATTER           = 1,
+  SX_OPT_XCD_REMAP         = 2,
+  SX_OPT_TANGENT_PRECISION = 3,
+  SX_OPT_JVP_ARITHMETIC    = 4,
+  SX_OPT_LINEAR_SOLVER     = 5, /* SX_LINSOLVER_CG (default) or _GMRES: the linear solves of the Newton loops */
+  SX_OPT_GMRES_RESTART     = 6  /* GMRES restart length (default 50, mfem's GMRESSolver default) */
+};
+/* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
+enum { SX_LINSOLVER_CG = 0, SX_LINSOLVER_GMRES = 1 };
 enum { SX_SCATTER_COLORED = 0, SX_SCATTER_ATOMIC = 1 };
 /* Precisions.  SX_OPT_TANGENT_PRECISION selects how the next linearization
  * stores the quadrature-point tangent (computed in FP64, rounded once to
@@ -260,6 +269,13 @@ typedef struct sx_solve_report {
  * rule of mfem::CGSolver: (r, z) <= max(rel^2 (r0, z0), abs^2). */
 int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
                 double abs_tol, int max_iter, sx_solve_report* report);
+/* Restarted GMRES(restart) with left Jacobi preconditioning on J_bc x = b
+ * (x0 = 0), mfem::GMRESSolver + HypreSmoother Jacobi as built by
+ * buildLinearSolverAndPreconditioner (num/equation_solver.cpp:916-919,
+ * 1018-1021): stops when ||D^-1 (b - J_bc x)|| <= max(rel ||D^-1 b||, abs).
+ * For non-symmetric tangents (J2 under geometric nonlinearity). */
+int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
+                   double abs_tol, int max_iter, int restart, sx_solve_report* report);
 /* Newton (num/equation_solver.cpp:81-235, no line search) with Jacobi-CG
  * linear solves on R(x) = 0 with essential rows zeroed. x (device) holds the
  * essential values on entry and the solution on exit. Form trial wrt. */

@@ -806,11 +806,11 @@ class Functional<test(trials...)> {
 };
 
 // ---------------------------------------------------------------------------
-// EquationSolver: Newton + Jacobi-CG with the reference's option names
+// EquationSolver: Newton + Jacobi-CG / Jacobi-GMRES with the reference's option names
 // (num/solver_config.hpp:342-400, num/equation_solver.cpp:81-235)
 // ---------------------------------------------------------------------------
 enum class NonlinearSolver { Newton };
-enum class LinearSolver { CG };
+enum class LinearSolver { CG, GMRES };
 enum class Preconditioner { HypreJacobi };
 
 struct LinearSolverOptions {
@@ -819,6 +819,7 @@ struct LinearSolverOptions {
   double         relative_tol   = 1.0e-8;
   double         absolute_tol   = 1.0e-12;
   int            max_iterations = 300;
+  int            gmres_restart  = 50;  // mfem::GMRESSolver's default Krylov dimension
 };
 struct NonlinearSolverOptions {
   NonlinearSolver nonlin_solver  = NonlinearSolver::Newton;
@@ -836,6 +837,9 @@ class EquationSolver {
   sx_solve_report solve(Form& residual, Vector& x, const std::vector<int32_t>& essential_dofs, int wrt = 0)
   {
     sx_solve_report rep{};
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINEAR_SOLVER,
+                                     lin_.linear_solver == LinearSolver::GMRES ? SX_LINSOLVER_GMRES : SX_LINSOLVER_CG));
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_GMRES_RESTART, lin_.gmres_restart));
     int rc = sx_newton_solve(residual.handle(), wrt, essential_dofs.data(), int64_t(essential_dofs.size()), x.Data(),
                              nl_.relative_tol, nl_.absolute_tol, nl_.max_iterations, lin_.relative_tol,
                              lin_.absolute_tol, lin_.max_iterations, &rep);

@@ -23,6 +23,10 @@ SX_OPT_SCATTER = 1
 SX_OPT_XCD_REMAP = 2
 SX_OPT_TANGENT_PRECISION = 3
 SX_OPT_JVP_ARITHMETIC = 4
+SX_OPT_LINEAR_SOLVER = 5
+SX_OPT_GMRES_RESTART = 6
+SX_LINSOLVER_CG = 0
+SX_LINSOLVER_GMRES = 1
 SX_PREC_F64 = 0
 SX_PREC_F32 = 1
 SX_PREC_BF16 = 2
@@ -156,6 +160,8 @@ SIGNATURES = {
     "sx_grad_apply_timed": (_int, [_vp, _int, _vp, _vp, _int, _D, _D, ctypes.POINTER(_int)]),
     "sx_cg_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                            ctypes.POINTER(SolveReport)]),
+    "sx_gmres_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int, _int,
+                              ctypes.POINTER(SolveReport)]),
     "sx_newton_solve": (_int, [_vp, _int, _I32, ctypes.c_int64, _vp, ctypes.c_double, ctypes.c_double, _int,
                                ctypes.c_double, ctypes.c_double, _int, ctypes.POINTER(SolveReport)]),
     "sx_quasistatic_solve": (_int, [_vp, _int, _I32, ctypes.c_int64, _D, _int, _int, ctypes.POINTER(SolverOptions),

@@ -253,6 +253,8 @@ struct sx_form {
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
+  int                                    linsolver    = SX_LINSOLVER_CG;  // linear solver of the Newton loops
+  int                                    gmres_restart = 50;
   sx_learned*                            learned      = nullptr;        // borrowed
   int                                    learn_mode   = 0;              // SX_PRECOND_*
   // solver scratch
@@ -1302,6 +1304,14 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
         SX_REQUIRE(value == SX_PREC_F64 || value == SX_PREC_F32, SX_ERR_ARG, "J.v arithmetic is FP64 or FP32");
         f->jvp_arith = int(value);
         return;
+      case SX_OPT_LINEAR_SOLVER:
+        SX_REQUIRE(value == SX_LINSOLVER_CG || value == SX_LINSOLVER_GMRES, SX_ERR_ARG, "linear solver is CG or GMRES");
+        f->linsolver = int(value);
+        return;
+      case SX_OPT_GMRES_RESTART:
+        SX_REQUIRE(value >= 1 && value <= 1000, SX_ERR_ARG, "GMRES restart length must be in 1..1000");
+        f->gmres_restart = int(value);
+        return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
   });
@@ -1530,6 +1540,7 @@ double read_scalar(sx_form* f, int slot)
 
 struct Workspace {
   DeviceBuffer r, z, d, Ad, diag, ess, t;
+  DeviceBuffer V, hb, gpart;  // GMRES basis, coefficients, partial dots
 };
 
 // Jacobi-PCG on J_bc (rows and columns of ess eliminated) following the
@@ -1603,6 +1614,120 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
     SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));
   }
   rep->linear_iterations = std::min(it, max_iter);
+}
+
+// Restarted GMRES(m) with left Jacobi preconditioning on J_bc x = b, x0 = 0,
+// the mfem::GMRESSolver configured by num/equation_solver.cpp:916-919 (with
+// HypreSmoother Jacobi, :1018-1021): Arnoldi on D^-1 J_bc, the residual norm
+// ||D^-1 (b - J_bc x)|| goes down to max(rel ||D^-1 b||, abs).  Orthogonalises
+// by classical Gram-Schmidt applied twice (two batched dot launches, one host
+// read per iteration); Givens rotations and the small triangular solve run on
+// the host.  Each restart recomputes the true preconditioned residual, which
+// also decides convergence.
+void gmres(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double* x, double rel, double abs_tol,
+           int max_iter, int m, Workspace& w, sx_solve_report* rep)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  SX_REQUIRE(m >= 1 && m <= 1000, SX_ERR_ARG, "GMRES restart length must be in 1..1000");
+  w.r.allocate(size_t(n) * 8);
+  w.t.allocate(size_t(n) * 8);
+  w.diag.allocate(size_t(n) * 8);
+  w.V.allocate(size_t(m + 1) * size_t(n) * 8);
+  const int hn = 2 * (m + 1) + 1;  // h (CGS pass 1), h (pass 2), |w|^2
+  w.hb.allocate(size_t(std::max(hn, m + 1)) * 8);
+  w.gpart.allocate(size_t(m + 1) * size_t(sxk::multi_dot_blocks()) * 8);
+  double *r = w.r.as<double>(), *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
+  double *hb = w.hb.as<double>(), *part = w.gpart.as<double>();
+  auto    vec = [&](int j) { return V + int64_t(j) * n; };
+  auto    apply_bc = [&](const double* in, double* outv) {
+    apply_gradient(g, in, outv);
+    SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
+  };
+  compute_diag(g, ess, ness, diag);
+  SX_HIP(sxk::vec_fill(x, 0.0, n, s));
+  std::vector<double> H(size_t(m + 1) * size_t(m), 0.0);
+  std::vector<double> cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), gv(static_cast<size_t>(m + 1));
+  std::vector<double> hh(static_cast<size_t>(hn));
+  auto   Hij  = [&](int i, int j) -> double& { return H[size_t(j) * size_t(m + 1) + size_t(i)]; };
+  double goal = 0.0;
+  int    it   = 0;
+  bool   first = true;
+  rep->converged = 0;
+  for (;;) {
+    // z = D^-1 (b - J_bc x)
+    apply_bc(x, t);
+    SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    SX_HIP(sxk::vec_axpy(-1.0, t, r, n, s));
+    SX_HIP(sxk::jacobi_apply(r, diag, vec(0), n, s));
+    SX_HIP(sxk::multi_dot(vec(0), n, 1, vec(0), n, part, hb, false, s));
+    double beta2 = 0.0;
+    SX_HIP(hipMemcpyAsync(&beta2, hb, 8, hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    const double beta = std::sqrt(std::max(beta2, 0.0));
+    SX_REQUIRE(std::isfinite(beta), SX_ERR_NOCONV, "GMRES produced a non-finite residual");
+    if (first) {
+      rep->initial_norm = beta;
+      goal              = std::max(rel * beta, abs_tol);
+      first             = false;
+    }
+    rep->final_norm = beta;
+    if (beta <= goal) {
+      rep->converged = 1;
+      break;
+    }
+    if (it >= max_iter) break;
+    SX_HIP(sxk::scale_copy(1.0 / beta, vec(0), vec(0), n, s));
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0]    = beta;
+    int jend = 0;
+    for (int j = 0; j < m && it < max_iter; j++) {
+      double* wv = vec(j + 1);
+      apply_bc(vec(j), t);
+      SX_HIP(sxk::jacobi_apply(t, diag, wv, n, s));
+      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
+      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
+      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
+      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
+      SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
+      SX_HIP(hipMemcpyAsync(hh.data(), hb, size_t(hn) * 8, hipMemcpyDeviceToHost, s));
+      SX_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i <= j; i++) Hij(i, j) = hh[size_t(i)] + hh[size_t(m + 1 + i)];
+      const double hnext = std::sqrt(std::max(hh[size_t(2 * (m + 1))], 0.0));
+      Hij(j + 1, j)      = hnext;
+      // previous rotations, then the one that zeroes H(j+1, j)
+      for (int i = 0; i < j; i++) {
+        const double a = Hij(i, j), c = Hij(i + 1, j);
+        Hij(i, j)     = cs[size_t(i)] * a + sn[size_t(i)] * c;
+        Hij(i + 1, j) = -sn[size_t(i)] * a + cs[size_t(i)] * c;
+      }
+      const double a = Hij(j, j), c = Hij(j + 1, j), den = std::hypot(a, c);
+      cs[size_t(j)]    = den > 0 ? a / den : 1.0;
+      sn[size_t(j)]    = den > 0 ? c / den : 0.0;
+      Hij(j, j)        = den;
+      Hij(j + 1, j)    = 0.0;
+      gv[size_t(j + 1)] = -sn[size_t(j)] * gv[size_t(j)];
+      gv[size_t(j)]     = cs[size_t(j)] * gv[size_t(j)];
+      it++;
+      jend            = j + 1;
+      rep->final_norm = std::fabs(gv[size_t(j + 1)]);
+      SX_REQUIRE(std::isfinite(rep->final_norm), SX_ERR_NOCONV, "GMRES produced a non-finite residual");
+      if (hnext > 0.0) SX_HIP(sxk::scale_copy(1.0 / hnext, wv, wv, n, s));
+      if (rep->final_norm <= goal || hnext == 0.0) break;
+    }
+    // y = H^-1 g (upper triangular), x += V y
+    std::vector<double> y(size_t(jend), 0.0);
+    for (int i = jend - 1; i >= 0; i--) {
+      double acc = gv[size_t(i)];
+      for (int k = i + 1; k < jend; k++) acc -= Hij(i, k) * y[size_t(k)];
+      y[size_t(i)] = Hij(i, i) != 0.0 ? acc / Hij(i, i) : 0.0;
+    }
+    SX_HIP(hipMemcpyAsync(hb, y.data(), size_t(jend) * 8, hipMemcpyHostToDevice, s));
+    SX_HIP(sxk::multi_axpy(V, n, jend, hb, 1.0, x, n, s));
+    SX_HIP(hipStreamSynchronize(s));  // y lives on the host stack
+  }
+  rep->linear_iterations = it;
 }
 
 // ---- learned preconditioner ----
@@ -1819,6 +1944,8 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
     CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, &hook);
     rec.finish(x, rep->converged);
+  } else if (f->linsolver == SX_LINSOLVER_GMRES) {
+    gmres(g, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
   } else {
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   }
@@ -1860,6 +1987,24 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
 }
 
 }  // extern "C"
+
+extern "C" int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                              double rel_tol, double abs_tol, int max_iter, int restart, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "GMRES on a gradient before a linearization");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    gmres(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "GMRES did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
 
 namespace {
 

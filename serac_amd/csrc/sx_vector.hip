@@ -121,6 +121,56 @@ __global__ void __launch_bounds__(kThreads) k_jacobi(const double* r, const doub
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
+// part[j * gridDim.x + bx] = partial (V_j, w) over block bx's grid-stride
+// range; grid = (nb, k), V = k vectors of length n at stride ld
+__global__ void __launch_bounds__(kThreads) k_multi_dot_partial(const double* V, int64_t ld, const double* w, int64_t n,
+                                                                double* part)
+{
+  __shared__ double sh[4];
+  const double*     v = V + int64_t(blockIdx.y) * ld;
+  double            s = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    s += v[i] * w[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[int64_t(blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// out[j] (+)= sum of row j of part[k][nb], one block per row, fixed order
+__global__ void __launch_bounds__(kThreads) k_reduce_rows(const double* part, int nb, double* out, int accumulate)
+{
+  __shared__ double sh[4];
+  const double*     row = part + int64_t(blockIdx.x) * nb;
+  double            s   = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s += row[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = accumulate ? out[blockIdx.x] + s : s;
+}
+
+// y += sign * sum_j c[j] V_j, coefficients read from device memory
+__global__ void __launch_bounds__(kThreads) k_multi_axpy(const double* V, int64_t ld, int k, const double* c,
+                                                         double sign, double* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    double s = 0.0;
+    for (int j = 0; j < k; j++) s += c[j] * V[int64_t(j) * ld + i];
+    y[i] += sign * s;
+  }
+}
+
+// y = a x
+__global__ void __launch_bounds__(kThreads) k_scale_copy(double a, const double* x, double* y, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = a * x[i];
+}
+
+// z = x / diag (Jacobi application without a reduction)
+__global__ void __launch_bounds__(kThreads) k_div(const double* x, const double* diag, double* z, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    z[i] = x[i] / diag[i];
+}
+
 __global__ void __launch_bounds__(kThreads) k_set_rows(double* x, const int32_t* rows, int64_t nr, double v)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < nr; i += int64_t(gridDim.x) * kThreads)
@@ -256,6 +306,40 @@ hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n,
   const int nb = reduction_blocks();
   hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, r, diag, z, n, part);
   hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0);
+  return hipGetLastError();
+}
+
+int multi_dot_blocks() { return 256; }
+
+hipError_t multi_dot(const double* V, int64_t ld, int k, const double* w, int64_t n, double* part, double* out,
+                     bool accumulate, hipStream_t s)
+{
+  if (k <= 0) return hipSuccess;
+  const int nb = multi_dot_blocks();
+  hipLaunchKernelGGL(k_multi_dot_partial, dim3(unsigned(nb), unsigned(k)), dim3(kThreads), 0, s, V, ld, w, n, part);
+  hipLaunchKernelGGL(k_reduce_rows, dim3(unsigned(k)), dim3(kThreads), 0, s, part, nb, out, int(accumulate));
+  return hipGetLastError();
+}
+
+hipError_t multi_axpy(const double* V, int64_t ld, int k, const double* c_dev, double sign, double* y, int64_t n,
+                      hipStream_t s)
+{
+  if (k <= 0 || n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_multi_axpy, dim3(grid_for(n)), dim3(kThreads), 0, s, V, ld, k, c_dev, sign, y, n);
+  return hipGetLastError();
+}
+
+hipError_t scale_copy(double a, const double* x, double* y, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kThreads), 0, s, a, x, y, n);
+  return hipGetLastError();
+}
+
+hipError_t jacobi_apply(const double* x, const double* diag, double* z, int64_t n, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_div, dim3(grid_for(n)), dim3(kThreads), 0, s, x, diag, z, n);
   return hipGetLastError();
 }
 

@@ -24,6 +24,17 @@ hipError_t cg_update(double* x, double* r, double* z, const double* d, const dou
 // z = r/diag; scal[slot] = (r, z)
 hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n, double* part, double* scal,
                       int slot, hipStream_t s);
+// GMRES building blocks.  multi_dot: out[j] (+)= (V_j, w) for j < k, V_j at
+// V + j ld; part needs k * multi_dot_blocks() doubles.  multi_axpy:
+// y += sign * sum_j c[j] V_j (c in device memory).  scale_copy: y = a x.
+// jacobi_apply: z = x / diag.
+int        multi_dot_blocks();
+hipError_t multi_dot(const double* V, int64_t ld, int k, const double* w, int64_t n, double* part, double* out,
+                     bool accumulate, hipStream_t s);
+hipError_t multi_axpy(const double* V, int64_t ld, int k, const double* c_dev, double sign, double* y, int64_t n,
+                      hipStream_t s);
+hipError_t scale_copy(double a, const double* x, double* y, int64_t n, hipStream_t s);
+hipError_t jacobi_apply(const double* x, const double* diag, double* z, int64_t n, hipStream_t s);
 hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStream_t s);
 // x[rows[i]] = vals[i]
 hipError_t set_rows_values(double* x, const int32_t* rows, const double* vals, int64_t nr, hipStream_t s);

@@ -244,6 +244,15 @@ int main()
     for (int32_t d : ess) Rx[size_t(d)] = 0.0;
     CHECK(norm2(Rx) <= 1e-9 * rep.initial_norm, "converged residual %.2e (initial %.2e)", norm2(Rx),
           rep.initial_norm);
+    // the same step with LinearSolver::GMRES reaches the same displacement
+    Vector         xg(x0);
+    EquationSolver gsolver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 20},
+                           LinearSolverOptions{LinearSolver::GMRES, Preconditioner::HypreJacobi, 1e-10, 0.0, 5000});
+    sx_solve_report grep_ = gsolver.solve(residual, xg, ess);
+    CHECK(grep_.converged && relerr(xg.ToHost(), x.ToHost()) < 1e-8, "GMRES Newton matches CG Newton (%.2e, %d GMRES)",
+          relerr(xg.ToHost(), x.ToHost()), grep_.linear_iterations);
+    // later solves on this form go back to CG
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINEAR_SOLVER, SX_LINSOLVER_CG));
   }
 
   // ---- J2 plasticity with QuadratureData and updateQdata
