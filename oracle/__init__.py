@@ -89,6 +89,7 @@ def _load():
         "orc_newton": (ctypes.c_int, [_vp, _I, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]),
         "orc_time_jvp": (ctypes.c_double, [_vp, _D, _D, ctypes.c_int]),
+        "orc_time_jvp_threads": (ctypes.c_double, [_vp, _D, _D, ctypes.c_int, ctypes.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -387,6 +388,18 @@ class Form:
         dU = np.ascontiguousarray(dU, dtype=np.float64)
         dR = np.zeros(self.space.ndof)
         return lib().orc_time_jvp(self.h, _dp(dU), _dp(dR), reps)
+
+    def jvp_threads(self, dU, nthreads):
+        """J.v over nthreads host threads (per-thread scatter, then a sum)."""
+        dU = np.ascontiguousarray(dU, dtype=np.float64)
+        dR = np.zeros(self.space.ndof)
+        lib().orc_time_jvp_threads(self.h, _dp(dU), _dp(dR), 1, nthreads)
+        return dR
+
+    def time_jvp_threads(self, dU, reps, nthreads):
+        dU = np.ascontiguousarray(dU, dtype=np.float64)
+        dR = np.zeros(self.space.ndof)
+        return lib().orc_time_jvp_threads(self.h, _dp(dU), _dp(dR), reps, nthreads)
 
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:

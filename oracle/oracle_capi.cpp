@@ -344,4 +344,14 @@ double orc_time_jvp(void* f, const double* dU, double* dR, int reps)
   return std::chrono::duration<double>(t1 - t0).count();
 }
 
+// the same with the elements split over nthreads host threads (OpenMP)
+double orc_time_jvp_threads(void* f, const double* dU, double* dR, int reps, int nthreads)
+{
+  auto& F  = static_cast<OForm*>(f)->form;
+  auto  t0 = std::chrono::steady_clock::now();
+  for (int i = 0; i < reps; i++) F.jvp_threads(dU, dR, nthreads);
+  auto t1 = std::chrono::steady_clock::now();
+  return std::chrono::duration<double>(t1 - t0).count();
+}
+
 }  // extern "C"

@@ -3,6 +3,9 @@
 // Jacobi-CG loop.  TEST INFRASTRUCTURE ONLY (see serac_oracle.hpp).
 #pragma once
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <array>
 #include <map>
 
@@ -482,6 +485,47 @@ struct Form {
         scatter_elem(e, Re.data(), dR);
       }
     }
+    boundary_jvp(dU, dR);
+  }
+
+  // J.v on `nthreads` host threads for the CPU baseline: each thread takes a
+  // contiguous range of elements and scatters into its own L-vector; the
+  // vectors are then summed.  jvp() above keeps the reference's serial order.
+  void jvp_threads(const double* dU, double* dR, int nthreads) const
+  {
+    const size_t ndof = size_t(space.ndof());
+    nthreads          = std::max(1, nthreads);
+    std::vector<double> acc(size_t(nthreads) * ndof, 0.0);
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+      const int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+      const int tid = 0, nt = 1;
+#endif
+      double*             out = acc.data() + size_t(tid) * ndof;
+      std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
+      for (auto& I : integrals) {
+        if (I.D.empty()) continue;
+        const size_t ne = I.elems.size(), lo = ne * size_t(tid) / size_t(nt), hi = ne * size_t(tid + 1) / size_t(nt);
+        for (size_t k = lo; k < hi; k++) {
+          const int e = I.elems[k];
+          gather_elem(e, dU, Ue.data());
+          std::fill(Re.begin(), Re.end(), 0.0);
+          element_jvp(I, k, Ue.data(), Re.data());
+          scatter_elem(e, Re.data(), out);
+        }
+      }
+#pragma omp barrier
+      const size_t lo = ndof * size_t(tid) / size_t(nt), hi = ndof * size_t(tid + 1) / size_t(nt);
+      for (size_t i = lo; i < hi; i++) {
+        double v = 0.0;
+        for (int t = 0; t < nt; t++) v += acc[size_t(t) * ndof + i];
+        dR[i] = v;
+      }
+    }
+    for (auto& I : integrals)
+      if (I.D.empty()) throw std::runtime_error("jvp before linearize");
     boundary_jvp(dU, dR);
   }
 
