@@ -208,8 +208,6 @@ def main():
     ap.add_argument("--split-scatter", default="colored", choices=["colored", "atomic"],
                     help="scatter mode of the FP32 / BF16 J.v measurements")
     ap.add_argument("--xcd-remap", type=int, default=1)
-    ap.add_argument("--element-order", default="domain", choices=["domain", "bricks"],
-                    help="launch order of the atomic-scatter element kernels")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a known-size read (dot of two L-vectors) and write (fill) for counter scaling")
     args = ap.parse_args()
@@ -240,8 +238,6 @@ def main():
         f.set_option(sx.capi.SX_OPT_SCATTER,
                      sx.capi.SX_SCATTER_ATOMIC if scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
         f.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
-        f.set_option(sx.capi.SX_OPT_ELEMENT_ORDER,
-                     sx.capi.SX_ORDER_BRICKS if args.element_order == "bricks" else sx.capi.SX_ORDER_DOMAIN)
         if qf == "j2":
             qd = sx.QuadratureData(space.ne, nq1 ** 3, np.zeros(10))
             qdatas.append(qd)
@@ -486,7 +482,6 @@ def main():
             "parallelism": "1 GPU",
             "scatter": args.scatter,
             "xcd_remap": args.xcd_remap,
-            "element_order": args.element_order,
         },
         "roofline": {
             "bound": "hbm",

@@ -202,12 +202,8 @@ enum {
   SX_OPT_TANGENT_PRECISION = 3,
   SX_OPT_JVP_ARITHMETIC    = 4,
   SX_OPT_LINEAR_SOLVER     = 5, /* SX_LINSOLVER_CG (default) or _GMRES: the linear solves of the Newton loops */
-  SX_OPT_GMRES_RESTART     = 6, /* GMRES restart length (default 50, mfem's GMRESSolver default) */
-  SX_OPT_ELEMENT_ORDER     = 7  /* launch order of atomic-scatter integrals on cuboid meshes (set before adding
-                                   integrals): SX_ORDER_DOMAIN (default) or SX_ORDER_BRICKS (8x8x8 element
-                                   bricks; slower, as its concurrent atomics collide) */
+  SX_OPT_GMRES_RESTART     = 6  /* GMRES restart length (default 50, mfem's GMRESSolver default) */
 };
-enum { SX_ORDER_DOMAIN = 0, SX_ORDER_BRICKS = 1 };
 /* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
 enum { SX_LINSOLVER_CG = 0, SX_LINSOLVER_GMRES = 1 };
 enum { SX_SCATTER_COLORED = 0, SX_SCATTER_ATOMIC = 1 };

@@ -27,9 +27,6 @@ SX_OPT_LINEAR_SOLVER = 5
 SX_OPT_GMRES_RESTART = 6
 SX_LINSOLVER_CG = 0
 SX_LINSOLVER_GMRES = 1
-SX_OPT_ELEMENT_ORDER = 7
-SX_ORDER_DOMAIN = 0
-SX_ORDER_BRICKS = 1
 SX_PREC_F64 = 0
 SX_PREC_F32 = 1
 SX_PREC_BF16 = 2

@@ -254,7 +254,6 @@ struct sx_form {
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
   int                                    linsolver    = SX_LINSOLVER_CG;  // linear solver of the Newton loops
-  int                                    elem_order   = SX_ORDER_DOMAIN;  // slot order of atomic-scatter integrals
   int                                    gmres_restart = 50;
   sx_learned*                            learned      = nullptr;        // borrowed
   int                                    learn_mode   = 0;              // SX_PRECOND_*
@@ -1039,20 +1038,6 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
     pos.reserve(domain.size());
     if (f->scatter == SX_SCATTER_ATOMIC) {
       for (size_t i = 0; i < domain.size(); i++) pos.push_back(int(i));
-      if (f->elem_order == SX_ORDER_BRICKS && m.structured) {
-        // 8 x 8 x 8 element bricks, bricks x-fastest, elements x-fastest inside:
-        // the elements that share a node run close together in time.  Measured
-        // slower than domain order (1.64 vs 1.37 ms per FP64 J.v at 64^3, p=3):
-        // concurrent elements then add to the same nodes at once, and the FP64
-        // atomics on those lines serialize
-        constexpr int B   = 8;
-        auto          key = [&](int e) {
-          const int ex = e % m.n[0], ey = (e / m.n[0]) % m.n[1], ez = e / (m.n[0] * m.n[1]);
-          return std::array<int, 6>{ez / B, ey / B, ex / B, ez % B, ey % B, ex % B};
-        };
-        std::stable_sort(pos.begin(), pos.end(),
-                         [&](int a, int b) { return key(domain[size_t(a)]) < key(domain[size_t(b)]); });
-      }
       I->color_off = {0, int64_t(pos.size())};
     } else {
       I->color_off.assign(size_t(vs->ncolors) + 1, 0);
@@ -1318,11 +1303,6 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
       case SX_OPT_JVP_ARITHMETIC:
         SX_REQUIRE(value == SX_PREC_F64 || value == SX_PREC_F32, SX_ERR_ARG, "J.v arithmetic is FP64 or FP32");
         f->jvp_arith = int(value);
-        return;
-      case SX_OPT_ELEMENT_ORDER:
-        SX_REQUIRE(value == SX_ORDER_DOMAIN || value == SX_ORDER_BRICKS, SX_ERR_ARG, "bad element order");
-        SX_REQUIRE(f->integrals.empty(), SX_ERR_STATE, "element order must be set before adding integrals");
-        f->elem_order = int(value);
         return;
       case SX_OPT_LINEAR_SOLVER:
         SX_REQUIRE(value == SX_LINSOLVER_CG || value == SX_LINSOLVER_GMRES, SX_ERR_ARG, "linear solver is CG or GMRES");
