@@ -356,6 +356,12 @@ int sx_learned_destroy(sx_learned* L);
  * SX_PRECOND_JACOBI (Jacobi-CG), SX_PRECOND_RECORD (Jacobi-CG recording
  * pairs into L), SX_PRECOND_LEARNED (flexible CG with L).  L is borrowed. */
 int sx_form_set_preconditioner(sx_form* f, sx_learned* L, int mode);
+/* flexible GMRES(restart) (right preconditioning) on J_bc x = b (x0 = 0) with
+ * the learned preconditioner, or with Jacobi when L is NULL; minimises
+ * ||b - J_bc x|| down to max(rel ||b||, abs).  The Newton loops use it when
+ * the form has SX_PRECOND_LEARNED and SX_OPT_LINEAR_SOLVER = GMRES. */
+int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                    double rel_tol, double abs_tol, int max_iter, int restart, sx_solve_report* report);
 /* flexible CG with the learned preconditioner on J_bc x = b (x0 = 0) */
 int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
                  double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);

@@ -251,6 +251,17 @@ class Gradient:
             raise SeracAmdError(rc, capi.load().sx_last_error().decode())
         return rep.as_dict()
 
+    def fgmres(self, learned, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, restart=50,
+               raise_on_noconv=True):
+        """Flexible GMRES with the learned preconditioner (Jacobi when learned is None) on J_bc x = b."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_fgmres_solve", self.h, learned.h if learned is not None else None, _iptr(ess),
+                          len(ess), b.ptr, x.ptr, rel_tol, abs_tol, max_iter, restart, ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
     def tangent_bytes(self):
         b = ctypes.c_int64()
         capi.call("sx_grad_tangent_bytes", self.h, ctypes.byref(b))

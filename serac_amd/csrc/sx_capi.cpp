@@ -1541,6 +1541,7 @@ double read_scalar(sx_form* f, int slot)
 struct Workspace {
   DeviceBuffer r, z, d, Ad, diag, ess, t;
   DeviceBuffer V, hb, gpart;  // GMRES basis, coefficients, partial dots
+  DeviceBuffer Zb;            // flexible GMRES: the preconditioned directions
 };
 
 // Jacobi-PCG on J_bc (rows and columns of ess eliminated) following the
@@ -1931,13 +1932,131 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
   }
 }
 
+// Flexible GMRES(m) (Saad's FGMRES: right preconditioning with a
+// preconditioner that may change every iteration) on J_bc x = b, x0 = 0, with
+// the learned M(r) (Jacobi alone when L is null).  Minimises ||b - J_bc x||;
+// stops at max(rel ||b||, abs), checked on the true residual at each restart.
+// Stores the m + 1 Arnoldi vectors and the m preconditioned directions.
+void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
+            double abs_tol, int max_iter, int m, Workspace& w, sx_solve_report* rep)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  SX_REQUIRE(m >= 1 && m <= 1000, SX_ERR_ARG, "GMRES restart length must be in 1..1000");
+  w.r.allocate(size_t(n) * 8);
+  w.t.allocate(size_t(n) * 8);
+  w.diag.allocate(size_t(n) * 8);
+  w.V.allocate(size_t(m + 1) * size_t(n) * 8);
+  w.Zb.allocate(size_t(m) * size_t(n) * 8);
+  const int hn = 2 * (m + 1) + 1;
+  w.hb.allocate(size_t(std::max(hn, m + 1)) * 8);
+  w.gpart.allocate(size_t(m + 1) * size_t(sxk::multi_dot_blocks()) * 8);
+  double *r = w.r.as<double>(), *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
+  double *Z = w.Zb.as<double>(), *hb = w.hb.as<double>(), *part = w.gpart.as<double>();
+  auto    vec  = [&](int j) { return V + int64_t(j) * n; };
+  auto    zvec = [&](int j) { return Z + int64_t(j) * n; };
+  auto    apply_bc = [&](const double* in, double* outv) {
+    apply_gradient(g, in, outv);
+    SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
+  };
+  auto precondition = [&](const double* in, double* outv) {
+    if (L) {
+      learned_apply(L, f, diag, ess, ness, in, outv);
+    } else {
+      SX_HIP(sxk::jacobi_apply(in, diag, outv, n, s));
+    }
+  };
+  compute_diag(g, ess, ness, diag);
+  SX_HIP(sxk::vec_fill(x, 0.0, n, s));
+  std::vector<double> H(size_t(m + 1) * size_t(m), 0.0);
+  std::vector<double> cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), gv(static_cast<size_t>(m + 1));
+  std::vector<double> hh(static_cast<size_t>(hn));
+  auto   Hij   = [&](int i, int j) -> double& { return H[size_t(j) * size_t(m + 1) + size_t(i)]; };
+  double goal  = 0.0;
+  int    it    = 0;
+  bool   first = true;
+  rep->converged = 0;
+  for (;;) {
+    apply_bc(x, t);
+    SX_HIP(hipMemcpyAsync(vec(0), b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+    SX_HIP(sxk::vec_axpy(-1.0, t, vec(0), n, s));
+    SX_HIP(sxk::multi_dot(vec(0), n, 1, vec(0), n, part, hb, false, s));
+    double beta2 = 0.0;
+    SX_HIP(hipMemcpyAsync(&beta2, hb, 8, hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    const double beta = std::sqrt(std::max(beta2, 0.0));
+    SX_REQUIRE(std::isfinite(beta), SX_ERR_NOCONV, "flexible GMRES produced a non-finite residual");
+    if (first) {
+      rep->initial_norm = beta;
+      goal              = std::max(rel * beta, abs_tol);
+      first             = false;
+    }
+    rep->final_norm = beta;
+    if (beta <= goal) {
+      rep->converged = 1;
+      break;
+    }
+    if (it >= max_iter) break;
+    SX_HIP(sxk::scale_copy(1.0 / beta, vec(0), vec(0), n, s));
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0]    = beta;
+    int jend = 0;
+    for (int j = 0; j < m && it < max_iter; j++) {
+      double* wv = vec(j + 1);
+      precondition(vec(j), zvec(j));
+      apply_bc(zvec(j), wv);
+      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
+      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
+      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
+      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
+      SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
+      SX_HIP(hipMemcpyAsync(hh.data(), hb, size_t(hn) * 8, hipMemcpyDeviceToHost, s));
+      SX_HIP(hipStreamSynchronize(s));
+      for (int i = 0; i <= j; i++) Hij(i, j) = hh[size_t(i)] + hh[size_t(m + 1 + i)];
+      const double hnext = std::sqrt(std::max(hh[size_t(2 * (m + 1))], 0.0));
+      Hij(j + 1, j)      = hnext;
+      for (int i = 0; i < j; i++) {
+        const double a = Hij(i, j), c = Hij(i + 1, j);
+        Hij(i, j)     = cs[size_t(i)] * a + sn[size_t(i)] * c;
+        Hij(i + 1, j) = -sn[size_t(i)] * a + cs[size_t(i)] * c;
+      }
+      const double a = Hij(j, j), c = Hij(j + 1, j), den = std::hypot(a, c);
+      cs[size_t(j)]     = den > 0 ? a / den : 1.0;
+      sn[size_t(j)]     = den > 0 ? c / den : 0.0;
+      Hij(j, j)         = den;
+      Hij(j + 1, j)     = 0.0;
+      gv[size_t(j + 1)] = -sn[size_t(j)] * gv[size_t(j)];
+      gv[size_t(j)]     = cs[size_t(j)] * gv[size_t(j)];
+      it++;
+      jend            = j + 1;
+      rep->final_norm = std::fabs(gv[size_t(j + 1)]);
+      SX_REQUIRE(std::isfinite(rep->final_norm), SX_ERR_NOCONV, "flexible GMRES produced a non-finite residual");
+      if (hnext > 0.0) SX_HIP(sxk::scale_copy(1.0 / hnext, wv, wv, n, s));
+      if (rep->final_norm <= goal || hnext == 0.0) break;
+    }
+    std::vector<double> y(size_t(jend), 0.0);
+    for (int i = jend - 1; i >= 0; i--) {
+      double acc = gv[size_t(i)];
+      for (int k = i + 1; k < jend; k++) acc -= Hij(i, k) * y[size_t(k)];
+      y[size_t(i)] = Hij(i, i) != 0.0 ? acc / Hij(i, i) : 0.0;
+    }
+    SX_HIP(hipMemcpyAsync(hb, y.data(), size_t(jend) * 8, hipMemcpyHostToDevice, s));
+    SX_HIP(sxk::multi_axpy(Z, n, jend, hb, 1.0, x, n, s));
+    SX_HIP(hipStreamSynchronize(s));
+  }
+  rep->linear_iterations = it;
+}
+
 // the linear solve of the Newton loops: Jacobi-CG, Jacobi-CG recording
 // training pairs, or flexible CG with the learned preconditioner
 void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
                   double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
 {
   sx_form* f = g->form;
-  if (f->learned && f->learn_mode == SX_PRECOND_LEARNED) {
+  if (f->learned && f->learn_mode == SX_PRECOND_LEARNED && f->linsolver == SX_LINSOLVER_GMRES) {
+    fgmres(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
+  } else if (f->learned && f->learn_mode == SX_PRECOND_LEARNED) {
     fcg(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   } else if (f->learned && f->learn_mode == SX_PRECOND_RECORD) {
     PairRecorder rec(f->learned, f->stream);
@@ -2302,6 +2421,25 @@ int sx_form_set_preconditioner(sx_form* f, sx_learned* L, int mode)
     f->learned    = mode == SX_PRECOND_JACOBI ? nullptr : L;
     f->learn_mode = mode;
   });
+}
+
+int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                    double rel_tol, double abs_tol, int max_iter, int restart, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    SX_REQUIRE(!L || g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    fgmres(g, L, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    SX_REQUIRE(rep.converged, SX_ERR_NOCONV, "flexible GMRES did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
 }
 
 int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,

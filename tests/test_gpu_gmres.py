@@ -79,6 +79,26 @@ def test_gmres_heat_matches_direct_solve_and_reference_iterations(gpu_lib, resta
     assert abs(rep["linear_iterations"] - it_ref) <= 2, (rep["linear_iterations"], it_ref)
 
 
+def test_fgmres_jacobi_heat_matches_direct_solve(gpu_lib):
+    # flexible GMRES (right preconditioning) with plain Jacobi as its preconditioner
+    m = sx.Mesh.cuboid(6, 6, 6)
+    m.perturb(42, 0.2)
+    s = sx.Space(m, 2, 1, 1)
+    f = sx.Functional(s, [s])
+    f.add_domain_integral(sx.SX_QF_HEAT, (1.0, 1.0))
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(W.nodal_field(s.nnodes, 1, 3, 2000, -1.0, 1.0, 1)), wrt=0)
+    ess = s.boundary_dofs(0b000001, 0x1)
+    b = W.nodal_field(s.nnodes, 1, 4, 2001, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    A = eliminated(grad.assemble().to_scipy(), ess)
+    for restart in (50, 8):
+        x = sx.DeviceVector(s.ndof)
+        rep = grad.fgmres(None, ess, sx.DeviceVector.from_numpy(b), x, rel_tol=1e-12, restart=restart)
+        assert rep["converged"]
+        assert np.linalg.norm(b - A @ x.numpy()) <= 1.01e-12 * np.linalg.norm(b)
+        assert relerr(x.numpy(), spla.spsolve(A.tocsc(), b)) < 1e-9
+
+
 def test_gmres_solves_the_nonsymmetric_j2_tangent(gpu_lib):
     pr = J2Pair(3, 2, J2_PARAMS["c4"], state_seed=4)
     U = pr.shear_field(0.02, 1e-3, 15)

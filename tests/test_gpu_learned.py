@@ -163,3 +163,41 @@ def test_learned_requires_structured_vector_space(gpu_lib, patch_mesh):
     s1 = sx.Space(sx.Mesh.cuboid(8, 8, 8), 1, 1, 1)
     with pytest.raises(sx.SeracAmdError):
         sx.Learned(s1)
+
+
+def test_fgmres_with_learned_preconditioner_meets_the_true_residual_goal(gpu_lib):
+    from test_gpu_gmres import eliminated
+
+    # restarted GMRES stagnates on the larger ill-conditioned systems with any
+    # of these preconditioners; on 8^3 p=1 (2187 DOFs) a 500-vector restart
+    # keeps the whole Krylov space
+    m, s, f, ess = setup(8, 1)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    b = W.nodal_field(s.nnodes, 3, 8, 3000, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    A = eliminated(grad.assemble().to_scipy(), ess)
+    L = sx.Learned(s, seed=4)
+    x = sx.DeviceVector(s.ndof)
+    rep = grad.fgmres(L, ess, sx.DeviceVector.from_numpy(b), x, rel_tol=1e-8, max_iter=20000, restart=500)
+    assert rep["converged"]
+    assert np.linalg.norm(b - A @ x.numpy()) <= 1.01e-8 * np.linalg.norm(b)
+
+
+def test_newton_with_learned_fgmres(gpu_lib):
+    # inexact Newton: FGMRES to 1e-4 per step, Newton to 1e-10
+    m, s, f, ess0 = setup(8, 1)
+    ess1 = s.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(ess, ess1), -0.01, 0.0)
+    x_j = sx.DeviceVector(s.ndof)
+    sx.quasistatic_solve(f, x_j, ess, final, 1, rel_tol=1e-10, lin_rel_tol=1e-8, lin_max_iter=50000)
+    L = sx.Learned(s, seed=5)
+    f.set_option(sx.capi.SX_OPT_LINEAR_SOLVER, sx.capi.SX_LINSOLVER_GMRES)
+    f.set_option(sx.capi.SX_OPT_GMRES_RESTART, 500)
+    f.set_preconditioner(L, sx.capi.SX_PRECOND_LEARNED)
+    x_l = sx.DeviceVector(s.ndof)
+    reps = sx.quasistatic_solve(f, x_l, ess, final, 1, rel_tol=1e-10, lin_rel_tol=1e-4, lin_max_iter=50000,
+                                max_iter=40)
+    assert reps[0]["converged"]
+    assert relerr(x_l.numpy(), x_j.numpy()) < 1e-6
