@@ -323,11 +323,17 @@ int sx_unet_forward(sx_unet* net, const float* x, float* y);
 int sx_unet_backward(sx_unet* net, const float* x, const float* t, double* loss);
 /* one Adam step (beta 0.9 / 0.999, eps 1e-8) with the accumulated gradients, which it clears */
 int sx_unet_adam(sx_unet* net, double lr);
+/* activation of the hidden layers: SX_UNET_RELU (default) or SX_UNET_IDENTITY
+ * (a linear network: the map it learns, residual -> correction, is linear);
+ * re-initialises the parameters for it (variance gain/fan_in, gain 2 for
+ * ReLU, 1 for identity with the output layer scaled by 1e-2) and clears Adam */
+enum { SX_UNET_RELU = 0, SX_UNET_IDENTITY = 1 };
+int sx_unet_set_activation(sx_unet* net, int activation, uint64_t seed);
 int sx_unet_destroy(sx_unet* net);
 
 /* ---- learned coarse-grid preconditioner (no reference counterpart) ----
  * For a 3-component byVDIM space on a structured (cuboid) mesh of order p:
- *   M(r) = D^-1 r + s P N(P^T r / s),   s = rms(P^T r),
+ *   M(r) = D^-1 r + gain s P N(P^T r / s),   s = rms(P^T r),
  * D the Jacobi diagonal, P the interpolation from the mesh's vertex (Q1) grid
  * to the order-p nodes and N the U-Net above on the vertex grid (each
  * dimension 8 m + 1 vertices).  Used inside flexible CG; Jacobi-CG finishes a
@@ -347,8 +353,18 @@ int sx_learned_record_cg(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int
 /* copy usable pair k (0 <= k < npairs) to host FP32 arrays [nvertex][3]:
  * the restricted residual P^T r_k and the vertex-mean correction */
 int sx_learned_get_pair(const sx_learned* L, int k, float* r_host, float* t_host);
-/* `steps` Adam steps on random recorded pairs; mean loss of the last tenth */
-int sx_learned_train(sx_learned* L, int steps, double lr, uint64_t seed, double* mean_loss);
+/* `npairs` pairs from the linearization g itself: e = P e_c for random smooth
+ * vertex fields e_c (a multiscale sum of trilinearly interpolated uniform
+ * lattices; zero on the essential DOFs) with r = J_bc e.  e is then the
+ * converged correction of residual r, so the pair is (P^T r, avg(e)). */
+int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, int npairs,
+                              uint64_t seed);
+/* `steps` Adam steps, each on `batch` random pairs (gradients summed); mean
+ * loss of the last tenth.  Sets the target scale `gain` first, the geometric
+ * mean over the bank of rms(target) / rms(input): N learns t / (s gain) from
+ * P^T r / s, and M(r) = D^-1 r + gain s P N(P^T r / s). */
+int sx_learned_train(sx_learned* L, int steps, int batch, double lr, uint64_t seed, double* mean_loss);
+int sx_learned_gain(const sx_learned* L, double* gain);
 /* z = M(r) for the current linearization (device vectors) */
 int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* r, double* z);
 int sx_learned_destroy(sx_learned* L);

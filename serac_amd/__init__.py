@@ -378,6 +378,10 @@ class UNet:
     def adam(self, lr):
         capi.call("sx_unet_adam", self.h, lr)
 
+    def set_activation(self, activation, seed=0):
+        """SX_UNET_RELU or SX_UNET_IDENTITY; re-initialises the parameters."""
+        capi.call("sx_unet_set_activation", self.h, activation, seed)
+
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "owned", True) and capi._lib is not None:
             capi._lib.sx_unet_destroy(self.h)
@@ -416,10 +420,24 @@ class Learned:
         capi.call("sx_learned_get_pair", self.h, k, r.ctypes.data, t.ctypes.data)
         return r, t
 
-    def train(self, steps, lr=1e-3, seed=0):
+    def train(self, steps, lr=1e-3, seed=0, batch=1):
         loss = ctypes.c_double()
-        capi.call("sx_learned_train", self.h, steps, lr, seed, ctypes.byref(loss))
+        capi.call("sx_learned_train", self.h, steps, batch, lr, seed, ctypes.byref(loss))
         return loss.value
+
+    def generate_pairs(self, grad, ess, npairs, seed=0):
+        """Pairs (P^T J_bc e, avg e) for random smooth corrections e, from the linearization grad."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        capi.call("sx_learned_generate_pairs", self.h, grad.h, _iptr(ess), len(ess), npairs, seed)
+
+    def gain(self):
+        g = ctypes.c_double()
+        capi.call("sx_learned_gain", self.h, ctypes.byref(g))
+        return g.value
+
+    def set_activation(self, activation, seed=0):
+        """SX_UNET_RELU or SX_UNET_IDENTITY; re-initialises the network."""
+        capi.call("sx_unet_set_activation", self.net.h, activation, seed)
 
     def record_cg(self, grad, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000):
         """Jacobi-CG on J_bc x = b recording training pairs."""

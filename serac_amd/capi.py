@@ -44,6 +44,8 @@ SX_BQF_FLUX = 103
 SX_PRECOND_JACOBI = 0
 SX_PRECOND_RECORD = 1
 SX_PRECOND_LEARNED = 2
+SX_UNET_RELU = 0
+SX_UNET_IDENTITY = 1
 
 _vp = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
@@ -119,7 +121,9 @@ SIGNATURES = {
     "sx_learned_configure": (_int, [_vp, _int, _int, _int, _int]),
     "sx_learned_info": (_int, [_vp, ctypes.POINTER(_int), _I64, ctypes.POINTER(_vp)]),
     "sx_learned_get_pair": (_int, [_vp, _int, ctypes.c_void_p, ctypes.c_void_p]),
-    "sx_learned_train": (_int, [_vp, _int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "sx_learned_train": (_int, [_vp, _int, _int, ctypes.c_double, ctypes.c_uint64, _D]),
+    "sx_learned_generate_pairs": (_int, [_vp, _vp, _I32, ctypes.c_int64, _int, ctypes.c_uint64]),
+    "sx_learned_gain": (_int, [_vp, _D]),
     "sx_learned_apply": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp]),
     "sx_learned_destroy": (_int, [_vp]),
     "sx_form_set_preconditioner": (_int, [_vp, _vp, _int]),
@@ -127,6 +131,7 @@ SIGNATURES = {
                             ctypes.POINTER(SolveReport)]),
     "sx_learned_record_cg": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                     _int, ctypes.POINTER(SolveReport)]),
+    "sx_unet_set_activation": (_int, [_vp, _int, ctypes.c_uint64]),
     "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "sx_unet_info": (_int, [_vp, _I64, _I64]),
     "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

@@ -215,7 +215,7 @@ struct BoundaryIntegral {
 };
 
 // The learned coarse-grid preconditioner of a structured vector space:
-//   M(r) = D^-1 r + s P N(P^T r / s),  s = rms(P^T r)
+//   M(r) = D^-1 r + gain s P N(P^T r / s),  s = rms(P^T r)
 // with P the vertex (Q1) -> order-p interpolation and N the U-Net; plus the
 // bank of (coarse residual, coarse converged correction) pairs it trains on.
 struct sx_learned {
@@ -229,6 +229,7 @@ struct sx_learned {
   int                                        max_pairs = 256, stride = 50, window = 50, stall = 20;
   int                                        next = 0;  // ring position once the bank is full
   int64_t                                    fallbacks = 0;
+  double                                     gain      = 1.0;  // target scale: N learns t / (s gain)
   ~sx_learned()
   {
     if (net) sx_unet_destroy(net);
@@ -1746,8 +1747,69 @@ void learned_apply(sx_learned* L, sx_form* f, const double* diag, const int32_t*
   SX_HIP(hipStreamSynchronize(s));
   const int rc = sx_unet_forward(L->net, L->xin.as<const float>(), L->y.as<float>());
   if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-  SX_HIP(sxu::fe_prolong_add(L->G, L->y.as<const float>(), L->ss.as<const double>(), nc, z, s));
+  SX_HIP(sxu::fe_prolong_add(L->G, L->y.as<const float>(), L->ss.as<const double>(), nc, z, s, L->gain));
   SX_HIP(sxk::set_rows(z, ess, ness, 0.0, s));
+}
+
+// a slot of the pair bank for a new pair: appended while the bank has room,
+// then the ring position
+int take_pair_slot(sx_learned* L)
+{
+  int k;
+  if (int(L->pr.size()) < L->max_pairs) {
+    k = int(L->pr.size());
+    L->pr.push_back(std::make_unique<DeviceBuffer>());
+    L->pt.push_back(std::make_unique<DeviceBuffer>());
+    L->ok.push_back(0);
+    L->pr[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
+    L->pt[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
+  } else {
+    k       = L->next;
+    L->next = (L->next + 1) % L->max_pairs;
+  }
+  L->ok[size_t(k)] = 0;
+  return k;
+}
+
+// A random smooth field on the vertex grid: sum over levels m = 2, 3, 5, 9,
+// ... (m - 1 doubling up to the grid) of uniform[-1, 1] values on an m^3
+// lattice, weighted 1/m and interpolated trilinearly (vertex-aligned).
+std::vector<float> multiscale_field(const sxu::FEGrid& G, std::mt19937_64& rng)
+{
+  const int64_t      nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
+  std::vector<float> e(size_t(nv) * 3, 0.0f);
+  std::uniform_real_distribution<double> U(-1.0, 1.0);
+  const int nmax = std::max({G.nv[0], G.nv[1], G.nv[2]});
+  for (int m = 2; m <= nmax; m = 2 * m - 1) {
+    int md[3];
+    for (int d = 0; d < 3; d++) md[d] = std::min(m, G.nv[d]);
+    std::vector<double> c(size_t(md[0]) * md[1] * md[2] * 3);
+    for (auto& v : c) v = U(rng) / m;
+    for (int64_t k = 0; k < G.nv[2]; k++)
+      for (int64_t j = 0; j < G.nv[1]; j++)
+        for (int64_t i = 0; i < G.nv[0]; i++) {
+          const int64_t idx[3] = {i, j, k};
+          int           lo[3];
+          double        fr[3];
+          for (int d = 0; d < 3; d++) {
+            const double x = G.nv[d] > 1 ? double(idx[d]) * (md[d] - 1) / double(G.nv[d] - 1) : 0.0;
+            lo[d]          = std::min(int(x), std::max(md[d] - 2, 0));
+            fr[d]          = md[d] > 1 ? x - lo[d] : 0.0;
+          }
+          for (int comp = 0; comp < 3; comp++) {
+            double acc = 0.0;
+            for (int b = 0; b < 8; b++) {
+              const int    bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+              const double w  = (bx ? fr[0] : 1 - fr[0]) * (by ? fr[1] : 1 - fr[1]) * (bz ? fr[2] : 1 - fr[2]);
+              if (w == 0.0) continue;
+              const int64_t ci = (int64_t(lo[2] + bz) * md[1] + (lo[1] + by)) * md[0] + (lo[0] + bx);
+              acc += w * c[size_t(ci * 3 + comp)];
+            }
+            e[size_t(((k * G.nv[1] + j) * G.nv[0] + i) * 3 + comp)] += float(acc);
+          }
+        }
+  }
+  return e;
 }
 
 // records a pair every `stride` iterations of a Jacobi-CG solve: the input
@@ -1796,27 +1858,15 @@ struct PairRecorder {
       }
     }
     if (!starts) return;
-    int k;
-    if (int(L->pr.size()) < L->max_pairs) {
-      k = int(L->pr.size());
-      L->pr.push_back(std::make_unique<DeviceBuffer>());
-      L->pt.push_back(std::make_unique<DeviceBuffer>());
-      L->ok.push_back(0);
-      L->pr[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
-      L->pt[size_t(k)]->allocate(size_t(L->nv) * 3 * 4);
-    } else {
-      k       = L->next;
-      L->next = (L->next + 1) % L->max_pairs;
-      for (auto o = open.begin(); o != open.end();) {
-        if (o->slot == k) {
-          spare.push_back(o->x0);
-          o = open.erase(o);
-        } else {
-          ++o;
-        }
+    const int k = take_pair_slot(L);
+    for (auto o = open.begin(); o != open.end();) {
+      if (o->slot == k) {
+        spare.push_back(o->x0);
+        o = open.erase(o);
+      } else {
+        ++o;
       }
     }
-    L->ok[size_t(k)] = 0;
     SX_HIP(sxu::fe_restrict(L->G, r, L->pr[size_t(k)]->as<float>(), s));
     DeviceBuffer* x0 = take();
     SX_HIP(hipMemcpyAsync(x0->ptr, now.ptr, size_t(L->nv) * 3 * 8, hipMemcpyDeviceToDevice, s));
@@ -2352,40 +2402,115 @@ int sx_learned_get_pair(const sx_learned* L, int k, float* r_host, float* t_host
   });
 }
 
-int sx_learned_train(sx_learned* L, int steps, double lr, uint64_t seed, double* mean_loss)
+int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, int npairs,
+                              uint64_t seed)
 {
   return api([&] {
-    SX_REQUIRE(L && steps >= 0, SX_ERR_ARG, "bad argument");
+    SX_REQUIRE(L && g && g->form && npairs >= 0, SX_ERR_ARG, "bad argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "pairs from a gradient before a linearization");
+    SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    sx_form*      f  = g->form;
+    hipStream_t   s  = f->stream;
+    const int64_t n  = f->test->ndof();
+    const int64_t nc = L->nv * 3;
+    DeviceBuffer  ess, e, r, ec, one;
+    ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    e.allocate(size_t(n) * 8);
+    r.allocate(size_t(n) * 8);
+    ec.allocate(size_t(nc) * 4);
+    one.upload(std::vector<double>{double(nc)});  // prolongation scale sqrt(nc / nc) = 1
+    std::mt19937_64 rng(seed);
+    for (int p = 0; p < npairs; p++) {
+      // e = P e_c (zero on essential DOFs), r = J_bc e: e is the converged
+      // correction of residual r
+      const std::vector<float> h = multiscale_field(L->G, rng);
+      SX_HIP(hipMemcpyAsync(ec.ptr, h.data(), h.size() * 4, hipMemcpyHostToDevice, s));
+      SX_HIP(sxk::vec_fill(e.as<double>(), 0.0, n, s));
+      SX_HIP(sxu::fe_prolong_add(L->G, ec.as<const float>(), one.as<const double>(), nc, e.as<double>(), s));
+      SX_HIP(sxk::set_rows(e.as<double>(), ess.as<const int32_t>(), ness, 0.0, s));
+      apply_gradient(g, e.as<const double>(), r.as<double>());
+      SX_HIP(sxk::copy_rows(r.as<double>(), e.as<const double>(), ess.as<const int32_t>(), ness, s));
+      const int k = take_pair_slot(L);
+      SX_HIP(sxu::fe_restrict(L->G, r.as<const double>(), L->pr[size_t(k)]->as<float>(), s));
+      SX_HIP(sxu::fe_restrict(L->G, e.as<const double>(), L->pt[size_t(k)]->as<float>(), s, true));
+      SX_HIP(hipStreamSynchronize(s));  // h is reused by the next pair
+      L->ok[size_t(k)] = 1;
+    }
+  });
+}
+
+int sx_learned_train(sx_learned* L, int steps, int batch, double lr, uint64_t seed, double* mean_loss)
+{
+  return api([&] {
+    SX_REQUIRE(L && steps >= 0 && batch >= 1, SX_ERR_ARG, "bad argument");
     std::vector<size_t> usable;
     for (size_t j = 0; j < L->ok.size(); j++)
       if (L->ok[j]) usable.push_back(j);
     SX_REQUIRE(!usable.empty(), SX_ERR_STATE, "no training pairs recorded");
+    const int64_t nc = L->nv * 3;
+    // the target scale: geometric mean over the bank of rms(target) / rms(input)
+    {
+      DeviceBuffer sq;
+      sq.allocate(2 * usable.size() * sizeof(double));
+      SX_HIP(hipMemset(sq.ptr, 0, 2 * usable.size() * sizeof(double)));
+      for (size_t i = 0; i < usable.size(); i++) {
+        SX_HIP(sxu::sumsq(L->pr[usable[i]]->as<const float>(), nc, sq.as<double>() + 2 * i, nullptr));
+        SX_HIP(sxu::sumsq(L->pt[usable[i]]->as<const float>(), nc, sq.as<double>() + 2 * i + 1, nullptr));
+      }
+      std::vector<double> h(2 * usable.size());
+      SX_HIP(hipMemcpy(h.data(), sq.ptr, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+      double lsum = 0.0;
+      int    cnt  = 0;
+      for (size_t i = 0; i < usable.size(); i++)
+        if (h[2 * i] > 0 && h[2 * i + 1] > 0) {
+          lsum += 0.5 * std::log(h[2 * i + 1] / h[2 * i]);
+          cnt++;
+        }
+      L->gain = cnt ? std::exp(lsum / cnt) : 1.0;
+    }
     std::mt19937_64 rng(seed);
-    const int64_t   nc = L->nv * 3;
-    DeviceBuffer    tin, tt;
+    DeviceBuffer    tin, tt, ss2;
     tin.allocate(size_t(nc) * 4);
     tt.allocate(size_t(nc) * 4);
+    ss2.allocate(sizeof(double));
     double sum = 0.0;
     int    cnt = 0;
     for (int k = 0; k < steps; k++) {
-      const size_t j = usable[size_t(rng() % usable.size())];
-      // both the input and the target are scaled by the input's RMS
-      SX_HIP(hipMemset(L->ss.ptr, 0, sizeof(double)));
-      SX_HIP(sxu::sumsq(L->pr[j]->as<const float>(), nc, L->ss.as<double>(), nullptr));
-      SX_HIP(sxu::normalize(L->pr[j]->as<const float>(), L->ss.as<const double>(), nc, nc, tin.as<float>(), nullptr));
-      SX_HIP(sxu::normalize(L->pt[j]->as<const float>(), L->ss.as<const double>(), nc, nc, tt.as<float>(), nullptr));
-      SX_HIP(hipDeviceSynchronize());
-      double loss = 0.0;
-      int    rc   = sx_unet_backward(L->net, tin.as<const float>(), tt.as<const float>(), &loss);
-      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-      rc = sx_unet_adam(L->net, lr);
+      double step_loss = 0.0;
+      for (int b = 0; b < batch; b++) {
+        const size_t j = usable[size_t(rng() % usable.size())];
+        // input scaled by its RMS s, target by s * gain
+        SX_HIP(hipMemset(L->ss.ptr, 0, sizeof(double)));
+        SX_HIP(sxu::sumsq(L->pr[j]->as<const float>(), nc, L->ss.as<double>(), nullptr));
+        double ssh = 0.0;
+        SX_HIP(hipMemcpy(&ssh, L->ss.ptr, sizeof(double), hipMemcpyDeviceToHost));
+        const double ssg = ssh * L->gain * L->gain;
+        SX_HIP(hipMemcpy(ss2.ptr, &ssg, sizeof(double), hipMemcpyHostToDevice));
+        SX_HIP(sxu::normalize(L->pr[j]->as<const float>(), L->ss.as<const double>(), nc, nc, tin.as<float>(), nullptr));
+        SX_HIP(sxu::normalize(L->pt[j]->as<const float>(), ss2.as<const double>(), nc, nc, tt.as<float>(), nullptr));
+        SX_HIP(hipDeviceSynchronize());
+        double loss = 0.0;
+        const int rc = sx_unet_backward(L->net, tin.as<const float>(), tt.as<const float>(), &loss);
+        if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+        step_loss += loss / batch;
+      }
+      const int rc = sx_unet_adam(L->net, lr);
       if (rc != SX_OK) throw sx_failure(rc, g_last_error);
       if (k >= steps - std::max(1, steps / 10)) {
-        sum += loss;
+        sum += step_loss;
         cnt++;
       }
     }
     if (mean_loss) *mean_loss = cnt ? sum / cnt : 0.0;
+  });
+}
+
+int sx_learned_gain(const sx_learned* L, double* gain)
+{
+  return api([&] {
+    SX_REQUIRE(L && gain, SX_ERR_ARG, "null argument");
+    *gain = L->gain;
   });
 }
 

@@ -86,7 +86,8 @@ struct sx_unet {
   // parameters: per layer weights [27][cin][cout] then bias [cout], FP32 master
   int64_t woff[12], boff[12], nparam = 0;
   Buf     P, G, M, V, Wbf;  // FP32 master, gradient, Adam moments; BF16 weights (same offsets)
-  int     step = 0;
+  int     step     = 0;
+  int     identity = 0;  // 1: no ReLU (a linear network)
   // activations (BF16) and their gradients (FP32)
   Buf act[12];   // output of layer k
   Buf up[3];     // prolonged inputs of layers 8, 9, 10
@@ -131,7 +132,7 @@ sxu::ConvArgs conv_args(const sx_unet* N, int k, const void* xa, const void* xb)
   A.mode = L.mode;
   A.w    = N->Wbf.as<uint16_t>() + N->woff[k];
   A.bias = N->P.as<float>() + N->boff[k];
-  A.relu = L.relu;
+  A.relu = L.relu && !N->identity;
   return A;
 }
 
@@ -214,7 +215,7 @@ void backward(sx_unet* N)
   for (int j = 2; j >= 0; j--) {
     const int k = 8 + j;
     g           = N->gact[k].as<float>();
-    UN_HIP(sxu::relu_bwd(g, N->act[k].as<uint16_t>(), nel(k), s));
+    if (!N->identity) UN_HIP(sxu::relu_bwd(g, N->act[k].as<uint16_t>(), nel(k), s));
     weight_grad(N, k, N->up[j].p, N->act[skip[j]].p, g);
     data_grad(N, k, g, 0, kLayers[k].cin_a, N->gup[j].as<float>());
     // the skip's gradient from the decoder starts the skip's accumulation
@@ -227,7 +228,7 @@ void backward(sx_unet* N)
   // encoder 7 .. 0
   for (int k = 7; k >= 0; k--) {
     g = N->gact[k].as<float>();
-    UN_HIP(sxu::relu_bwd(g, N->act[k].as<uint16_t>(), nel(k), s));
+    if (!N->identity) UN_HIP(sxu::relu_bwd(g, N->act[k].as<uint16_t>(), nel(k), s));
     const void* xin = k == 0 ? N->xin.p : N->act[k - 1].p;
     weight_grad(N, k, xin, nullptr, g);
     if (k == 0) break;
@@ -243,6 +244,29 @@ void backward(sx_unet* N)
       data_grad(N, k, g, 0, kLayers[k].cin_a, gin);
     }
   }
+}
+
+// Normal weights with variance gain / fan_in (fan_in = 27 cin), zero biases,
+// cleared Adam state.  ReLU: gain 2 (He).  Identity: gain 1, and the output
+// layer scaled by 1e-2 so that a new network starts near N = 0.
+void init_params(sx_unet* N, uint64_t seed)
+{
+  const int64_t      off = N->nparam;
+  std::vector<float> h(static_cast<size_t>(off), 0.0f);
+  std::mt19937_64    rng(seed);
+  for (int k = 0; k < 12; k++) {
+    const float gain = N->identity ? 1.0f : 2.0f;
+    const float sd   = std::sqrt(gain / float(27 * N->cin(k))) * (N->identity && k == 11 ? 0.01f : 1.0f);
+    std::normal_distribution<float> nd(0.0f, sd);
+    for (int64_t i = 0; i < 27 * N->cin(k) * kLayers[k].cout; i++) h[size_t(N->woff[k] + i)] = nd(rng);
+  }
+  UN_HIP(hipMemcpy(N->P.p, h.data(), size_t(off) * 4, hipMemcpyHostToDevice));
+  UN_HIP(hipMemset(N->G.p, 0, size_t(off) * 4));
+  UN_HIP(hipMemset(N->M.p, 0, size_t(off) * 4));
+  UN_HIP(hipMemset(N->V.p, 0, size_t(off) * 4));
+  UN_HIP(sxu::to_bf16(N->P.as<float>(), N->Wbf.as<uint16_t>(), off, nullptr));
+  UN_HIP(hipDeviceSynchronize());
+  N->step = 0;
 }
 
 void init(sx_unet* N, int nx, int ny, int nz, uint64_t seed)
@@ -263,23 +287,12 @@ void init(sx_unet* N, int nx, int ny, int nz, uint64_t seed)
     off += kLayers[k].cout;
   }
   N->nparam = off;
-  // He initialisation of the weights, zero biases
-  std::vector<float>  h(static_cast<size_t>(off), 0.0f);
-  std::mt19937_64     rng(seed);
-  for (int k = 0; k < 12; k++) {
-    std::normal_distribution<float> nd(0.0f, std::sqrt(2.0f / float(27 * N->cin(k))));
-    for (int64_t i = 0; i < 27 * N->cin(k) * kLayers[k].cout; i++) h[size_t(N->woff[k] + i)] = nd(rng);
-  }
   N->P.alloc(size_t(off) * 4);
   N->G.alloc(size_t(off) * 4);
   N->M.alloc(size_t(off) * 4);
   N->V.alloc(size_t(off) * 4);
   N->Wbf.alloc(size_t(off) * 2);
-  UN_HIP(hipMemcpy(N->P.p, h.data(), size_t(off) * 4, hipMemcpyHostToDevice));
-  UN_HIP(hipMemset(N->G.p, 0, size_t(off) * 4));
-  UN_HIP(hipMemset(N->M.p, 0, size_t(off) * 4));
-  UN_HIP(hipMemset(N->V.p, 0, size_t(off) * 4));
-  UN_HIP(sxu::to_bf16(N->P.as<float>(), N->Wbf.as<uint16_t>(), off, nullptr));
+  init_params(N, seed);
   for (int k = 0; k < 12; k++) {
     const int64_t n = N->nvox[kLayers[k].lout] * kLayers[k].cout;
     if (k < 11) N->act[k].alloc(size_t(n) * 2);
@@ -326,6 +339,17 @@ int sx_unet_create(int nx, int ny, int nz, uint64_t seed, sx_unet** out)
     auto N = std::make_unique<sx_unet>();
     init(N.get(), nx, ny, nz, seed);
     *out = N.release();
+  });
+}
+
+int sx_unet_set_activation(sx_unet* N, int activation, uint64_t seed)
+{
+  return unet_api([&] {
+    if (!N) throw unet_failure(SX_ERR_ARG, "null network");
+    if (activation != SX_UNET_RELU && activation != SX_UNET_IDENTITY)
+      throw unet_failure(SX_ERR_ARG, "activation is SX_UNET_RELU or SX_UNET_IDENTITY");
+    N->identity = activation == SX_UNET_IDENTITY ? 1 : 0;
+    init_params(N, seed);
   });
 }
 

@@ -413,9 +413,9 @@ __global__ void k_fe_restrict(FEGrid G, const double* r, TO* rc, int average)
 }
 
 // z += sqrt(ss / n) * P ec  (ss: the sum of squares the input was normalized by)
-__global__ void k_fe_prolong_add(FEGrid G, const float* ec, const double* ss, int64_t nss, double* z)
+__global__ void k_fe_prolong_add(FEGrid G, const float* ec, const double* ss, int64_t nss, double gain, double* z)
 {
-  const double  scale = ss[0] > 0 ? sqrt(ss[0] / double(nss)) : 0.0;
+  const double  scale = ss[0] > 0 ? gain * sqrt(ss[0] / double(nss)) : 0.0;
   const int64_t nn    = int64_t(G.nn[0]) * G.nn[1] * G.nn[2];
   for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nn * 3; t += int64_t(gridDim.x) * kThreads) {
     const int     c = int(t % 3);
@@ -630,11 +630,12 @@ hipError_t fe_average(const FEGrid& G, const double* x, double* xc, hipStream_t 
                      xc, 1);
   return hipGetLastError();
 }
-hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, int64_t nss, double* z, hipStream_t s)
+hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, int64_t nss, double* z, hipStream_t s,
+                          double gain)
 {
   const int64_t n = int64_t(G.nn[0]) * G.nn[1] * G.nn[2] * 3;
   hipLaunchKernelGGL(k_fe_prolong_add, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, ec, ss,
-                     nss, z);
+                     nss, gain, z);
   return hipGetLastError();
 }
 hipError_t fe_inject(const FEGrid& G, const double* x, float* xc, hipStream_t s)

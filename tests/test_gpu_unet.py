@@ -52,11 +52,13 @@ def grad_flat(params):
     return np.concatenate(g)
 
 
-def reference(params, x, dims):
+def reference(params, x, dims, hidden_relu=True):
     def conv(x, k, stride=1, relu=True):
         W, B = params[k]
         y = F.conv3d(x, R(W), B, stride=stride, padding=1)
-        return R(torch.relu(y)) if relu else y
+        if not relu:
+            return y
+        return R(torch.relu(y)) if hidden_relu else R(y)
 
     def up(x, like):
         return R(F.interpolate(x, size=like.shape[2:], mode="trilinear", align_corners=True))
@@ -79,9 +81,12 @@ def reference(params, x, dims):
 # below 32768 output nodes a convolution runs one lane per (node, channel);
 # the 41 x 33 x 33 grid also exercises the one-thread-per-node kernel on its
 # finest level
-@pytest.mark.parametrize("dims", [(17, 17, 17), (17, 9, 33), (41, 33, 33)])
-def test_unet_forward_backward_match_torch(gpu_lib, dims):
+@pytest.mark.parametrize("dims,act", [((17, 17, 17), "relu"), ((17, 9, 33), "relu"), ((41, 33, 33), "relu"),
+                                      ((17, 17, 17), "identity")])
+def test_unet_forward_backward_match_torch(gpu_lib, dims, act):
     net = sx.UNet(*dims, seed=3)
+    if act == "identity":
+        net.set_activation(sx.capi.SX_UNET_IDENTITY, seed=3)
     rng = np.random.default_rng(5)
     # give the biases some value so that they are exercised
     p = net.params()
@@ -95,7 +100,7 @@ def test_unet_forward_backward_match_torch(gpu_lib, dims):
     t = rng.standard_normal((net.nvox, 3)).astype(np.float32)
     y = net.forward(x)
     params = torch_params(net, net.params())
-    yt = reference(params, to_torch(x, dims, 3), dims)
+    yt = reference(params, to_torch(x, dims, 3), dims, hidden_relu=(act == "relu"))
     y_ref = yt.detach().numpy()[0].transpose(1, 2, 3, 0).reshape(-1, 3)
     assert np.linalg.norm(y - y_ref) / np.linalg.norm(y_ref) < 1e-2
     assert np.abs(y - y_ref).max() < 5e-2 * np.abs(y_ref).max()
@@ -116,6 +121,21 @@ def test_unet_forward_backward_match_torch(gpu_lib, dims):
         a, b = g[off:off + n], g_ref[off:off + n]
         assert np.linalg.norm(a - b) <= 8e-2 * np.linalg.norm(b), f"layer {k}"
         off += n
+
+
+def test_identity_initialisation_scales(gpu_lib):
+    # identity activation: weights ~ N(0, 1 / fan_in), output layer 1e-2 of that
+    net = sx.UNet(17, 17, 17, seed=4)
+    net.set_activation(sx.capi.SX_UNET_IDENTITY, seed=4)
+    p = net.params()
+    off = 0
+    for k, (ca, cb, co) in enumerate(sx.UNet.LAYERS):
+        nw = 27 * (ca + cb) * co
+        w = p[off:off + nw]
+        expect = np.sqrt(1.0 / (27 * (ca + cb))) * (1e-2 if k == 11 else 1.0)
+        assert abs(w.std() / expect - 1.0) < 0.15, k
+        assert np.all(p[off + nw:off + nw + co] == 0.0)
+        off += nw + co
 
 
 def test_unet_adam_reduces_loss_on_a_fixed_pair(gpu_lib):
