@@ -328,6 +328,9 @@ int sx_unet_adam(sx_unet* net, double lr);
  * re-initialises the parameters for it (variance gain/fan_in, gain 2 for
  * ReLU, 1 for identity with the output layer scaled by 1e-2) and clears Adam */
 enum { SX_UNET_RELU = 0, SX_UNET_IDENTITY = 1 };
+/* forward convolutions with BF16 input and output on the matrix cores
+ * (implicit GEMM, mfma_f32_16x16x32_bf16; default on) or on the vector units */
+int sx_unet_set_mfma(sx_unet* net, int on);
 int sx_unet_set_activation(sx_unet* net, int activation, uint64_t seed);
 int sx_unet_destroy(sx_unet* net);
 

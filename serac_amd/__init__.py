@@ -382,6 +382,10 @@ class UNet:
         """SX_UNET_RELU or SX_UNET_IDENTITY; re-initialises the parameters."""
         capi.call("sx_unet_set_activation", self.h, activation, seed)
 
+    def set_mfma(self, on):
+        """Forward convolutions on the matrix cores (default) or the vector units."""
+        capi.call("sx_unet_set_mfma", self.h, int(bool(on)))
+
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "owned", True) and capi._lib is not None:
             capi._lib.sx_unet_destroy(self.h)

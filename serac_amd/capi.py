@@ -132,6 +132,7 @@ SIGNATURES = {
     "sx_learned_record_cg": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                     _int, ctypes.POINTER(SolveReport)]),
     "sx_unet_set_activation": (_int, [_vp, _int, ctypes.c_uint64]),
+    "sx_unet_set_mfma": (_int, [_vp, _int]),
     "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "sx_unet_info": (_int, [_vp, _I64, _I64]),
     "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

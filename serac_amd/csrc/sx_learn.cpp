@@ -88,6 +88,12 @@ struct sx_unet {
   Buf     P, G, M, V, Wbf;  // FP32 master, gradient, Adam moments; BF16 weights (same offsets)
   int     step     = 0;
   int     identity = 0;  // 1: no ReLU (a linear network)
+  // BF16 weights k-contiguous per output channel for the MFMA convolutions,
+  // rebuilt from Wbf on the first forward pass after the weights change
+  Buf     Wt;
+  int64_t wtoff[12] = {};
+  bool    wt_dirty  = true;
+  int     use_mfma  = 1;
   // activations (BF16) and their gradients (FP32)
   Buf act[12];   // output of layer k
   Buf up[3];     // prolonged inputs of layers 8, 9, 10
@@ -139,10 +145,20 @@ sxu::ConvArgs conv_args(const sx_unet* N, int k, const void* xa, const void* xb)
 void forward(sx_unet* N)
 {
   hipStream_t s = N->stream;
+  if (N->use_mfma && N->wt_dirty) {
+    for (int k = 0; k < 12; k++)
+      if (N->wtoff[k] >= 0)
+        UN_HIP(sxu::weights_kmajor(N->Wbf.as<uint16_t>() + N->woff[k], int(N->cin(k)), kLayers[k].cout,
+                                   N->Wt.as<uint16_t>() + N->wtoff[k], s));
+    N->wt_dirty = false;
+  }
   auto run = [&](int k, const void* xa, const void* xb, void* y, bool out_f32) {
     sxu::ConvArgs A = conv_args(N, k, xa, xb);
     A.y             = y;
-    UN_HIP(sxu::conv(A, kLayers[k].cout, false, out_f32, s));
+    if (N->use_mfma && N->wtoff[k] >= 0 && sxu::conv_mfma_supported(A, kLayers[k].cout, false, out_f32))
+      UN_HIP(sxu::conv_mfma(A, kLayers[k].cout, N->Wt.as<uint16_t>() + N->wtoff[k], s));
+    else
+      UN_HIP(sxu::conv(A, kLayers[k].cout, false, out_f32, s));
   };
   run(0, N->xin.p, nullptr, N->act[0].p, false);
   for (int k = 1; k < 8; k++) run(k, N->act[k - 1].p, nullptr, N->act[k].p, false);
@@ -266,7 +282,8 @@ void init_params(sx_unet* N, uint64_t seed)
   UN_HIP(hipMemset(N->V.p, 0, size_t(off) * 4));
   UN_HIP(sxu::to_bf16(N->P.as<float>(), N->Wbf.as<uint16_t>(), off, nullptr));
   UN_HIP(hipDeviceSynchronize());
-  N->step = 0;
+  N->step     = 0;
+  N->wt_dirty = true;
 }
 
 void init(sx_unet* N, int nx, int ny, int nz, uint64_t seed)
@@ -287,6 +304,19 @@ void init(sx_unet* N, int nx, int ny, int nz, uint64_t seed)
     off += kLayers[k].cout;
   }
   N->nparam = off;
+  // k-major copies for the layers the MFMA convolution takes (cin % 8 == 0,
+  // cout a multiple of 16)
+  int64_t wto = 0;
+  for (int k = 0; k < 12; k++) {
+    const int cin = int(N->cin(k)), cout = kLayers[k].cout;
+    if (cin % 8 == 0 && kLayers[k].cin_a % 8 == 0 && cout % 16 == 0) {
+      N->wtoff[k] = wto;
+      wto += int64_t(cout) * sxu::mfma_kpad(cin);
+    } else {
+      N->wtoff[k] = -1;
+    }
+  }
+  N->Wt.alloc(size_t(wto) * 2);
   N->P.alloc(size_t(off) * 4);
   N->G.alloc(size_t(off) * 4);
   N->M.alloc(size_t(off) * 4);
@@ -384,6 +414,7 @@ int sx_unet_set_params(sx_unet* N, const float* host)
     UN_HIP(hipMemcpy(N->P.p, host, size_t(N->nparam) * 4, hipMemcpyHostToDevice));
     UN_HIP(sxu::to_bf16(N->P.as<float>(), N->Wbf.as<uint16_t>(), N->nparam, N->stream));
     UN_HIP(hipStreamSynchronize(N->stream));
+    N->wt_dirty = true;
   });
 }
 
@@ -431,6 +462,16 @@ int sx_unet_adam(sx_unet* N, double lr)
     UN_HIP(sxu::adam(N->P.as<float>(), N->M.as<float>(), N->V.as<float>(), N->G.as<float>(), N->Wbf.as<uint16_t>(),
                      N->nparam, float(lr), 0.9f, 0.999f, 1e-8f, N->step, N->stream));
     UN_HIP(hipStreamSynchronize(N->stream));
+    N->wt_dirty = true;
+  });
+}
+
+int sx_unet_set_mfma(sx_unet* N, int on)
+{
+  return unet_api([&] {
+    if (!N) throw unet_failure(SX_ERR_ARG, "null network");
+    N->use_mfma = on ? 1 : 0;
+    N->wt_dirty = true;
   });
 }
 

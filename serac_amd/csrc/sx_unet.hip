@@ -167,6 +167,108 @@ __global__ void __launch_bounds__(kThreads) k_conv_cp(ConvArgs A)
   st(static_cast<TO*>(A.y), o * COUT + c, v);
 }
 
+// The same convolution as an implicit GEMM on the matrix cores: rows are
+// output nodes, columns output channels, K = (tap, input channel) in 27 * cin,
+// padded to a multiple of 32.  mfma_f32_16x16x32_bf16 tiles: lane l holds
+// A(row l % 16, k 8 (l / 16) .. + 7) -- eight consecutive channels of one tap of
+// one node, a single 16-byte load from the channels-last input -- and
+// B(k 8 (l / 16) .. + 7, column l % 16) from the weights stored k-contiguous
+// per output channel (wt[co][Kpad]); D(row 4 (l / 16) + r, column l % 16).
+// Each wave computes RT row tiles (16 RT nodes) x COUT / 16 column tiles; FP32
+// accumulation, bias and ReLU in the epilogue, BF16 output.  Requires BF16
+// input and output, stride 1 or 2 (modes 0, 1) and cin % 8 == 0.
+#ifndef MFMA_RT16
+#define MFMA_RT16 1
+#endif
+#ifndef MFMA_RT32
+#define MFMA_RT32 1
+#endif
+#ifndef MFMA_RT64
+#define MFMA_RT64 1
+#endif
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float  f32x4_t __attribute__((ext_vector_type(4)));
+
+template <int COUT, int RT>
+__global__ void __launch_bounds__(kThreads) k_conv_mfma(ConvArgs A, const uint16_t* wt, int kpad)
+{
+  constexpr int NT = COUT / 16;
+  const int     cin  = A.ca + A.cb;
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int     lane = int(threadIdx.x) & 63, wave = int(threadIdx.x) >> 6;
+  const int     m = lane & 15, kg = lane >> 4;
+  const int64_t base = (int64_t(blockIdx.x) * (kThreads / 64) + wave) * (RT * 16);
+  int           ox[RT], oy[RT], oz[RT];
+  bool          ok[RT];
+#pragma unroll
+  for (int rt = 0; rt < RT; rt++) {
+    const int64_t o = base + rt * 16 + m;
+    ok[rt]          = o < nout;
+    const int64_t oc = ok[rt] ? o : 0;
+    ox[rt]           = int(oc % A.dout[0]);
+    oy[rt]           = int((oc / A.dout[0]) % A.dout[1]);
+    oz[rt]           = int(oc / (int64_t(A.dout[0]) * A.dout[1]));
+  }
+  f32x4_t acc[RT][NT];
+#pragma unroll
+  for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) acc[rt][ct] = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint16_t* xa = static_cast<const uint16_t*>(A.xa);
+  const uint16_t* xb = static_cast<const uint16_t*>(A.xb);
+  const int       K  = 27 * cin;
+  for (int k0 = 8 * kg; k0 < kpad; k0 += 32) {
+    bf16x8_t av[RT];
+    const int tap = k0 / cin, ci = k0 - tap * cin;
+#pragma unroll
+    for (int rt = 0; rt < RT; rt++) {
+      av[rt] = bf16x8_t{};
+      if (k0 < K && ok[rt]) {
+        const int sx = src_index(ox[rt], tap % 3 - 1, A.mode, A.din[0]);
+        const int sy = src_index(oy[rt], (tap / 3) % 3 - 1, A.mode, A.din[1]);
+        const int sz = src_index(oz[rt], tap / 9 - 1, A.mode, A.din[2]);
+        if (sx >= 0 && sy >= 0 && sz >= 0) {
+          const int64_t   sidx = (int64_t(sz) * A.din[1] + sy) * A.din[0] + sx;
+          const uint16_t* src  = ci < A.ca ? xa + sidx * A.ca + ci : xb + sidx * A.cb + (ci - A.ca);
+          av[rt]               = *reinterpret_cast<const bf16x8_t*>(src);
+        }
+      }
+    }
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) {
+      const bf16x8_t bv = *reinterpret_cast<const bf16x8_t*>(wt + int64_t(ct * 16 + m) * kpad + k0);
+#pragma unroll
+      for (int rt = 0; rt < RT; rt++) acc[rt][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[rt], bv, acc[rt][ct], 0, 0, 0);
+    }
+  }
+  uint16_t* y = static_cast<uint16_t*>(A.y);
+#pragma unroll
+  for (int rt = 0; rt < RT; rt++)
+#pragma unroll
+    for (int ct = 0; ct < NT; ct++) {
+      const int   co = ct * 16 + m;
+      const float b  = A.bias ? A.bias[co] : 0.0f;
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        const int64_t o = base + rt * 16 + 4 * kg + r;
+        if (o >= nout) continue;
+        float v = acc[rt][ct][r] + b;
+        if (A.relu) v = v > 0.0f ? v : 0.0f;
+        y[o * COUT + co] = f2bf(v);
+      }
+    }
+}
+
+// wt[co][k] = w[k / cin][k % cin][co] for k < 27 cin, 0 up to kpad
+__global__ void k_weights_kmajor(const uint16_t* w, int cin, int cout, int kpad, uint16_t* wt)
+{
+  const int64_t n = int64_t(cout) * kpad;
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    const int co = int(t / kpad), k = int(t % kpad);
+    wt[t]        = k < 27 * cin ? w[(int64_t(k / cin) * cin + k % cin) * cout + co] : uint16_t(0);
+  }
+}
+
 // dw[tap][ci][co] += sum_o x[src(o, tap)][ci] g[o][co] (x = [xa, xb]) and, for
 // tap 0, db[co] += sum_o g[o][co].  grid = (ceil(nout / (64 chunks)), 27): each
 // workgroup sums `chunks` runs of 64 output nodes in registers before one
@@ -553,6 +655,40 @@ hipError_t conv(const ConvArgs& A, int cout, bool in_f32, bool out_f32, hipStrea
     case 64: return launch_conv_t<64>(A, in_f32, out_f32, s);
   }
   return hipErrorInvalidValue;
+}
+
+int mfma_kpad(int cin) { return (27 * cin + 31) / 32 * 32; }
+
+bool conv_mfma_supported(const ConvArgs& A, int cout, bool in_f32, bool out_f32)
+{
+  const int cin = A.ca + A.cb;
+  return !in_f32 && !out_f32 && (A.mode == 0 || A.mode == 1) && cin % 8 == 0 && A.ca % 8 == 0 &&
+         (cout == 16 || cout == 32 || cout == 64);
+}
+
+hipError_t weights_kmajor(const uint16_t* w, int cin, int cout, uint16_t* wt, hipStream_t s)
+{
+  const int     kpad = mfma_kpad(cin);
+  const int64_t n    = int64_t(cout) * kpad;
+  hipLaunchKernelGGL(k_weights_kmajor, dim3(std::min<unsigned>(blocks_for(n), 2048)), dim3(kThreads), 0, s, w, cin,
+                     cout, kpad, wt);
+  return hipGetLastError();
+}
+
+hipError_t conv_mfma(const ConvArgs& A, int cout, const uint16_t* wt, hipStream_t s)
+{
+  if (!conv_mfma_supported(A, cout, false, false)) return hipErrorInvalidValue;
+  const int     kpad = mfma_kpad(A.ca + A.cb);
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  // nodes per workgroup: 4 waves x RT row tiles of 16
+  auto grid = [&](int rt) { return dim3{unsigned((nout + 64 * rt - 1) / (64 * rt))}; };
+  const dim3 block{unsigned(kThreads)};
+  switch (cout) {
+    case 16: hipLaunchKernelGGL((k_conv_mfma<16, MFMA_RT16>), grid(MFMA_RT16), block, 0, s, A, wt, kpad); break;
+    case 32: hipLaunchKernelGGL((k_conv_mfma<32, MFMA_RT32>), grid(MFMA_RT32), block, 0, s, A, wt, kpad); break;
+    default: hipLaunchKernelGGL((k_conv_mfma<64, MFMA_RT64>), grid(MFMA_RT64), block, 0, s, A, wt, kpad); break;
+  }
+  return hipGetLastError();
 }
 
 hipError_t conv_wgrad(const WGradArgs& A, int cout, bool in_f32, hipStream_t s)

@@ -46,6 +46,13 @@ struct WGradArgs {
 };
 
 hipError_t conv(const ConvArgs& A, int cout, bool in_f32, bool out_f32, hipStream_t s);
+// the matrix-core (MFMA) implicit-GEMM form of conv for BF16 input and output,
+// modes 0 and 1, cin % 8 == 0 and cout in {16, 32, 64}; wt holds the weights
+// k-contiguous per output channel ([cout][mfma_kpad(cin)], weights_kmajor)
+int        mfma_kpad(int cin);
+bool       conv_mfma_supported(const ConvArgs& A, int cout, bool in_f32, bool out_f32);
+hipError_t weights_kmajor(const uint16_t* w, int cin, int cout, uint16_t* wt, hipStream_t s);
+hipError_t conv_mfma(const ConvArgs& A, int cout, const uint16_t* wt, hipStream_t s);
 hipError_t conv_wgrad(const WGradArgs& A, int cout, bool in_f32, hipStream_t s);
 hipError_t flip_weights(const uint16_t* w, int cin, int cout, int c0, int cpart, uint16_t* wt, hipStream_t s);
 hipError_t relu_bwd(float* g, const uint16_t* y, int64_t n, hipStream_t s);

@@ -123,6 +123,27 @@ def test_unet_forward_backward_match_torch(gpu_lib, dims, act):
         off += n
 
 
+@pytest.mark.parametrize("dims", [(17, 17, 17), (33, 17, 9), (65, 65, 65)])
+def test_mfma_forward_matches_vector_forward(gpu_lib, dims):
+    # the matrix-core convolutions against the vector-unit ones: the same BF16
+    # operands, FP32 sums in another order
+    net = sx.UNet(*dims, seed=6)
+    rng = np.random.default_rng(8)
+    p = net.params()
+    off = 0
+    for ca, cb, co in sx.UNet.LAYERS:
+        off += 27 * (ca + cb) * co
+        p[off:off + co] = rng.uniform(-0.05, 0.05, co)
+        off += co
+    net.set_params(p)
+    x = rng.standard_normal((net.nvox, 3)).astype(np.float32)
+    net.set_mfma(False)
+    y_vec = net.forward(x)
+    net.set_mfma(True)
+    y_mfma = net.forward(x)
+    assert np.linalg.norm(y_mfma - y_vec) <= 1e-2 * np.linalg.norm(y_vec)
+
+
 def test_identity_initialisation_scales(gpu_lib):
     # identity activation: weights ~ N(0, 1 / fan_in), output layer 1e-2 of that
     net = sx.UNet(17, 17, 17, seed=4)
