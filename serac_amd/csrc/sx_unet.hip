@@ -259,6 +259,60 @@ __global__ void __launch_bounds__(kThreads) k_conv_mfma(ConvArgs A, const uint16
     }
 }
 
+// The same with one 16 x 16 output tile (16 nodes, 16 channels) per wave, for
+// levels with few nodes: waves = row tiles x COUT / 16.
+template <int COUT>
+__global__ void __launch_bounds__(kThreads) k_conv_mfma_tile(ConvArgs A, const uint16_t* wt, int kpad)
+{
+  constexpr int NT   = COUT / 16;
+  const int     cin  = A.ca + A.cb;
+  const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const int     lane = int(threadIdx.x) & 63;
+  const int     m = lane & 15, kg = lane >> 4;
+  const int64_t w    = int64_t(blockIdx.x) * (kThreads / 64) + (int(threadIdx.x) >> 6);
+  const int64_t nrow = (nout + 15) / 16;
+  if (w >= nrow * NT) return;  // whole wave: no barrier follows
+  const int64_t base = (w / NT) * 16;
+  const int     ct   = int(w % NT);
+  const int64_t o0   = base + m;
+  const bool    ok   = o0 < nout;
+  const int64_t oc   = ok ? o0 : 0;
+  const int     ox = int(oc % A.dout[0]), oy = int((oc / A.dout[0]) % A.dout[1]);
+  const int     oz = int(oc / (int64_t(A.dout[0]) * A.dout[1]));
+  f32x4_t       acc = f32x4_t{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint16_t* xa = static_cast<const uint16_t*>(A.xa);
+  const uint16_t* xb = static_cast<const uint16_t*>(A.xb);
+  const uint16_t* wr = wt + int64_t(ct * 16 + m) * kpad;
+  const int       K  = 27 * cin;
+  for (int k0 = 8 * kg; k0 < kpad; k0 += 32) {
+    const int tap = k0 / cin, ci = k0 - tap * cin;
+    bf16x8_t  av  = bf16x8_t{};
+    if (k0 < K && ok) {
+      const int sx = src_index(ox, tap % 3 - 1, A.mode, A.din[0]);
+      const int sy = src_index(oy, (tap / 3) % 3 - 1, A.mode, A.din[1]);
+      const int sz = src_index(oz, tap / 9 - 1, A.mode, A.din[2]);
+      if (sx >= 0 && sy >= 0 && sz >= 0) {
+        const int64_t   sidx = (int64_t(sz) * A.din[1] + sy) * A.din[0] + sx;
+        const uint16_t* src  = ci < A.ca ? xa + sidx * A.ca + ci : xb + sidx * A.cb + (ci - A.ca);
+        av                   = *reinterpret_cast<const bf16x8_t*>(src);
+      }
+    }
+    const bf16x8_t bv = *reinterpret_cast<const bf16x8_t*>(wr + k0);
+    acc               = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+  }
+  uint16_t*   y  = static_cast<uint16_t*>(A.y);
+  const int   co = ct * 16 + m;
+  const float bb = A.bias ? A.bias[co] : 0.0f;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const int64_t o = base + 4 * kg + r;
+    if (o >= nout) continue;
+    float v = acc[r] + bb;
+    if (A.relu) v = v > 0.0f ? v : 0.0f;
+    y[o * COUT + co] = f2bf(v);
+  }
+}
+
 // wt[co][k] = w[k / cin][k % cin][co] for k < 27 cin, 0 up to kpad
 __global__ void k_weights_kmajor(const uint16_t* w, int cin, int cout, int kpad, uint16_t* wt)
 {
@@ -680,9 +734,20 @@ hipError_t conv_mfma(const ConvArgs& A, int cout, const uint16_t* wt, hipStream_
   if (!conv_mfma_supported(A, cout, false, false)) return hipErrorInvalidValue;
   const int     kpad = mfma_kpad(A.ca + A.cb);
   const int64_t nout = int64_t(A.dout[0]) * A.dout[1] * A.dout[2];
+  const dim3 block{unsigned(kThreads)};
+  if (nout < kChannelParallelBelow) {
+    // few nodes: one 16 x 16 tile per wave
+    const int64_t waves = (nout + 15) / 16 * (cout / 16);
+    const dim3    g{unsigned((waves + 3) / 4)};
+    switch (cout) {
+      case 16: hipLaunchKernelGGL(k_conv_mfma_tile<16>, g, block, 0, s, A, wt, kpad); break;
+      case 32: hipLaunchKernelGGL(k_conv_mfma_tile<32>, g, block, 0, s, A, wt, kpad); break;
+      default: hipLaunchKernelGGL(k_conv_mfma_tile<64>, g, block, 0, s, A, wt, kpad); break;
+    }
+    return hipGetLastError();
+  }
   // nodes per workgroup: 4 waves x RT row tiles of 16
   auto grid = [&](int rt) { return dim3{unsigned((nout + 64 * rt - 1) / (64 * rt))}; };
-  const dim3 block{unsigned(kThreads)};
   switch (cout) {
     case 16: hipLaunchKernelGGL((k_conv_mfma<16, MFMA_RT16>), grid(MFMA_RT16), block, 0, s, A, wt, kpad); break;
     case 32: hipLaunchKernelGGL((k_conv_mfma<32, MFMA_RT32>), grid(MFMA_RT32), block, 0, s, A, wt, kpad); break;
