@@ -385,6 +385,29 @@ int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t 
 int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
                  double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
 
+/* ---- quantities of interest: Functional<double(trial)> (fn/functional_qoi.inl:59-477) ----
+ * q(U) = sum over domain integrals of  int q(X, u, grad u) dV  on one trial
+ * space, with the compiled integrands below; the value is summed element by
+ * element in domain order (reproducible).  sx_qoi_gradient writes dq/dU (the
+ * HypreParVector of Gradient::assemble, fn/functional_qoi.inl:422); the
+ * directional derivative Gradient::operator()(dU) is its dot product with dU. */
+typedef struct sx_qoi sx_qoi;
+enum {
+  SX_QOI_MEASURE           = 1, /* int 1 dV (TrivialIntegrator of fn/tests/functional_qoi.cpp) */
+  SX_QOI_POSITION          = 2, /* int X_d dV, params {d}: the first moment (ZeroIndexIntegrator) */
+  SX_QOI_VALUE             = 3, /* int u_c dV, params {c} (GetZeroIntegrator) */
+  SX_QOI_NEOHOOKEAN_ENERGY = 4  /* int psi(I + grad u) dV, params {density, K, G}: the strain energy whose
+                                   first Piola stress is the flux of SX_QF_NEOHOOKEAN */
+};
+int sx_qoi_create(const sx_space* trial, sx_qoi** out);
+/* elems == NULL / nelem == 0: the whole mesh */
+int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* elems, int nelem);
+/* Functional<double(trial)>::operator()(t, U); u is a device L-vector */
+int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value);
+/* dq/dU at U into grad (device L-vector) */
+int sx_qoi_gradient(sx_qoi* q, double t, const double* u, double* grad);
+int sx_qoi_destroy(sx_qoi* q);
+
 /* ---- vector kernels used by the solvers, exposed for callers ---- */
 int sx_vec_dot(const double* a, const double* b, int64_t n, double* result); /* synchronous, deterministic */
 int sx_vec_axpy(double alpha, const double* x, double* y, int64_t n);        /* y += alpha x */

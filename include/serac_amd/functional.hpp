@@ -806,6 +806,131 @@ class Functional<test(trials...)> {
 };
 
 // ---------------------------------------------------------------------------
+// quantities of interest: Functional<double(trial)> (fn/functional_qoi.inl:59-477)
+// ---------------------------------------------------------------------------
+namespace qoi {
+struct Measure {  ///< int 1 dV (TrivialIntegrator of fn/tests/functional_qoi.cpp)
+};
+struct Position {  ///< int X_d dV, the first moment (ZeroIndexIntegrator for d = 0)
+  int direction;
+};
+struct Value {  ///< int u_c dV (GetZeroIntegrator for c = 0)
+  int component;
+};
+}  // namespace qoi
+namespace solid_mechanics {
+/// int psi(I + grad u) dV, the strain energy of a hyperelastic material
+template <typename Material>
+struct StrainEnergy {
+  Material material;
+};
+}  // namespace solid_mechanics
+
+/// maps QoI integrand types to the compiled registry (SX_QOI_*)
+template <typename Q>
+struct qoi_traits {
+  static constexpr bool registered = false;
+};
+template <>
+struct qoi_traits<qoi::Measure> {
+  static constexpr bool      registered = true;
+  static int                 id(const qoi::Measure&) { return SX_QOI_MEASURE; }
+  static std::vector<double> params(const qoi::Measure&) { return {}; }
+};
+template <>
+struct qoi_traits<qoi::Position> {
+  static constexpr bool      registered = true;
+  static int                 id(const qoi::Position&) { return SX_QOI_POSITION; }
+  static std::vector<double> params(const qoi::Position& q) { return {double(q.direction)}; }
+};
+template <>
+struct qoi_traits<qoi::Value> {
+  static constexpr bool      registered = true;
+  static int                 id(const qoi::Value&) { return SX_QOI_VALUE; }
+  static std::vector<double> params(const qoi::Value& q) { return {double(q.component)}; }
+};
+template <>
+struct qoi_traits<solid_mechanics::StrainEnergy<solid_mechanics::NeoHookean>> {
+  static constexpr bool registered = true;
+  static int id(const solid_mechanics::StrainEnergy<solid_mechanics::NeoHookean>&) { return SX_QOI_NEOHOOKEAN_ENERGY; }
+  static std::vector<double> params(const solid_mechanics::StrainEnergy<solid_mechanics::NeoHookean>& q)
+  {
+    return {q.material.density, q.material.K, q.material.G};
+  }
+};
+
+template <typename trial>
+class Functional<double(trial)> {
+ public:
+  /// Functional<double(...)>::Gradient (fn/functional_qoi.inl:407-477)
+  class Gradient {
+   public:
+    /// the directional derivative dq/dU . dx
+    double operator()(const Vector& dx) const
+    {
+      double v = 0.0;
+      detail::check(sx_vec_dot(g_.Data(), dx.Data(), g_.Size(), &v));
+      return v;
+    }
+    /// dq/dU (the HypreParVector of assemble(), fn/functional_qoi.inl:422)
+    Vector assemble() const { return g_; }
+
+   private:
+    friend class Functional;
+    Vector g_;
+  };
+
+  explicit Functional(std::array<const FiniteElementSpace*, 1> trial_space)
+  {
+    detail::check(sx_qoi_create(trial_space[0]->handle(), &h_));
+  }
+  Functional(const Functional&)            = delete;
+  Functional& operator=(const Functional&) = delete;
+  ~Functional()
+  {
+    if (h_) sx_qoi_destroy(h_);
+  }
+
+  /// AddDomainIntegral (fn/functional_qoi.inl:160-190)
+  template <int dim, int... args, typename Integrand>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Domain& domain)
+  {
+    static_assert(dim == 3, "this build integrates over hexahedra (Dimension<3>)");
+    static_assert(qoi_traits<Integrand>::registered,
+                  "QoI integrand type is not in the compiled registry (see qoi_traits)");
+    const std::vector<double> p = qoi_traits<Integrand>::params(integrand);
+    detail::check(sx_qoi_add_domain_integral(h_, qoi_traits<Integrand>::id(integrand), p.data(), int(p.size()),
+                                             domain.elements.empty() ? nullptr : domain.elements.data(),
+                                             int(domain.elements.size())));
+  }
+  template <int dim, int... args, typename Integrand>
+  void AddDomainIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&)
+  {
+    AddDomainIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, Domain{});
+  }
+
+  /// operator()(t, U) -> q(U) (fn/functional_qoi.inl:380-400)
+  double operator()(double t, const Vector& U)
+  {
+    double v = 0.0;
+    detail::check(sx_qoi_eval(h_, t, U.Data(), &v));
+    return v;
+  }
+  /// operator()(t, differentiate_wrt(U)) -> {q(U), Gradient&} (fn/functional_qoi.inl:323-378)
+  std::tuple<double, Gradient&> operator()(double t, DifferentiateWRT<Vector> U)
+  {
+    const double v = (*this)(t, U.value);
+    if (grad_.g_.Size() != U.value.Size()) grad_.g_.SetSize(U.value.Size());
+    detail::check(sx_qoi_gradient(h_, t, U.value.Data(), grad_.g_.Data()));
+    return {v, grad_};
+  }
+
+ private:
+  sx_qoi*  h_ = nullptr;
+  Gradient grad_;
+};
+
+// ---------------------------------------------------------------------------
 // EquationSolver: Newton + Jacobi-CG / Jacobi-GMRES with the reference's option names
 // (num/solver_config.hpp:342-400, num/equation_solver.cpp:81-235)
 // ---------------------------------------------------------------------------

@@ -463,6 +463,41 @@ class Learned:
             self.h = None
 
 
+class QoI:
+    """Functional<double(trial)>: a scalar quantity of interest on one trial space."""
+
+    def __init__(self, space):
+        h = _vp()
+        capi.call("sx_qoi_create", space.h, ctypes.byref(h))
+        self.h = h
+        self.space = space
+
+    def add_domain_integral(self, qoi, params=(), elems=None):
+        p = np.zeros(16)
+        p[: len(params)] = params
+        if elems is None:
+            capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), None, 0)
+        else:
+            el = (ctypes.c_int * len(elems))(*elems)
+            capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), el, len(elems))
+
+    def __call__(self, t, U):
+        v = ctypes.c_double()
+        capi.call("sx_qoi_eval", self.h, t, U.ptr, ctypes.byref(v))
+        return v.value
+
+    def gradient(self, t, U):
+        """dq/dU as a DeviceVector."""
+        g = DeviceVector(self.space.ndof)
+        capi.call("sx_qoi_gradient", self.h, t, U.ptr, g.ptr)
+        return g
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_qoi_destroy(self.h)
+            self.h = None
+
+
 class QuadratureData:
     """QuadratureData<State>: nelem x nq records of `size` doubles on the device."""
 

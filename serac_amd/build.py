@@ -24,6 +24,7 @@ SOURCES = [
     "sx_kernels_p4.hip",
     "sx_vector.hip",
     "sx_boundary.hip",
+    "sx_qoi.hip",
     "sx_unet.hip",
     "sx_learn.cpp",
     "sx_capi.cpp",

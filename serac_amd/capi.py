@@ -45,6 +45,10 @@ SX_PRECOND_JACOBI = 0
 SX_PRECOND_RECORD = 1
 SX_PRECOND_LEARNED = 2
 SX_UNET_RELU = 0
+SX_QOI_MEASURE = 1
+SX_QOI_POSITION = 2
+SX_QOI_VALUE = 3
+SX_QOI_NEOHOOKEAN_ENERGY = 4
 SX_UNET_IDENTITY = 1
 
 _vp = ctypes.c_void_p
@@ -133,6 +137,11 @@ SIGNATURES = {
                                     _int, ctypes.POINTER(SolveReport)]),
     "sx_unet_set_activation": (_int, [_vp, _int, ctypes.c_uint64]),
     "sx_unet_set_mfma": (_int, [_vp, _int]),
+    "sx_qoi_create": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sx_qoi_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int]),
+    "sx_qoi_eval": (_int, [_vp, ctypes.c_double, _vp, _D]),
+    "sx_qoi_gradient": (_int, [_vp, ctypes.c_double, _vp, _vp]),
+    "sx_qoi_destroy": (_int, [_vp]),
     "sx_unet_create": (_int, [_int, _int, _int, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "sx_unet_info": (_int, [_vp, _I64, _I64]),
     "sx_unet_get_params": (_int, [_vp, ctypes.POINTER(ctypes.c_float)]),

@@ -22,6 +22,7 @@
 #include "sx_boundary.hpp"
 #include "sx_launch.hpp"
 #include "sx_setup.hpp"
+#include "sx_qoi.hpp"
 #include "sx_unet.hpp"
 #include "sx_vector.hpp"
 
@@ -268,8 +269,13 @@ struct sx_form {
 
 namespace {
 
+// set while the library adds an integrand of its own (the QoI gradients),
+// which the public entry points do not accept
+thread_local bool g_internal_qf = false;
+
 int qf_components(int qf)
 {
+  if (qf == sxk::QF_ID_QOI_VALUE) return g_internal_qf ? -1 : 0;
   switch (qf) {
     case SX_QF_HEAT: return 1;
     case SX_QF_NEOHOOKEAN:
@@ -2583,6 +2589,165 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
   });
   if (report) *report = rep;
   return rc;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// quantities of interest: Functional<double(trial)> (fn/functional_qoi.inl:59-477)
+// ---------------------------------------------------------------------------
+struct sx_qoi {
+  struct Term {
+    int          id = 0;
+    double       params[SX_MAX_PARAMS] = {};
+    int64_t      nslots = 0;
+    DeviceBuffer d_map, d_coords, d_part;
+  };
+  const sx_space*   space = nullptr;
+  std::vector<std::unique_ptr<Term>> terms;
+  sx_form*          grad  = nullptr;  // the gradients as residuals of derived integrands
+  ~sx_qoi()
+  {
+    if (grad) sx_form_destroy(grad);
+  }
+};
+
+extern "C" {
+
+int sx_qoi_create(const sx_space* trial, sx_qoi** out)
+{
+  return api([&] {
+    SX_REQUIRE(trial && out, SX_ERR_ARG, "null argument");
+    auto q   = std::make_unique<sx_qoi>();
+    q->space = trial;
+    const sx_space* sp[1] = {trial};
+    const int       rc    = sx_form_create(trial, sp, 1, &q->grad);
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    *out = q.release();
+  });
+}
+
+int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* elems,
+                               int nelem)
+{
+  return api([&] {
+    SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
+    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+               "QoI parameters: at most 16 doubles");
+    SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE ||
+                   qoi_id == SX_QOI_NEOHOOKEAN_ENERGY,
+               SX_ERR_UNSUPPORTED, "unknown QoI integrand id " + std::to_string(qoi_id));
+    const sx_space* s = q->space;
+    SX_REQUIRE(qoi_id != SX_QOI_NEOHOOKEAN_ENERGY || s->nc == 3, SX_ERR_ARG,
+               "the Neo-Hookean strain energy needs a 3-component space");
+    if (qoi_id == SX_QOI_POSITION)
+      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
+                 "position QoI: params {d}, d in 0..2");
+    if (qoi_id == SX_QOI_VALUE)
+      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < s->nc && params[0] == std::floor(params[0]),
+                 SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
+    const auto&      m = s->mesh->m;
+    std::vector<int> domain;
+    if (elems && nelem > 0) {
+      domain.assign(elems, elems + nelem);
+      for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "element id out of range");
+    } else {
+      domain.resize(size_t(m.ne));
+      for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
+    }
+    auto  Tp = std::make_unique<sx_qoi::Term>();
+    auto& T  = *Tp;
+    T.id     = qoi_id;
+    for (int k = 0; k < nparams; k++) T.params[k] = params[k];
+    T.nslots = int64_t(domain.size());
+    std::vector<int>    map(size_t(T.nslots) * size_t(s->npe));
+    std::vector<double> xyz(size_t(T.nslots) * 24);
+    for (int64_t k = 0; k < T.nslots; k++) {
+      const int e = domain[size_t(k)];
+      std::copy(s->emap.begin() + long(size_t(e) * size_t(s->npe)), s->emap.begin() + long(size_t(e + 1) * size_t(s->npe)),
+                map.begin() + long(size_t(k) * size_t(s->npe)));
+      for (int c = 0; c < 8; c++) {
+        const int v = m.corner(e, c);
+        for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + size_t(c * 3 + d)] = m.X[size_t(3 * v + d)];
+      }
+    }
+    T.d_map.upload(map);
+    T.d_coords.upload(xyz);
+    T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
+    // the gradient: int u_c dV -> source e_c; the strain energy -> the
+    // Neo-Hookean residual (its flux P is d psi / dF); measure and position
+    // do not depend on U
+    const int dep[1] = {0};
+    if (qoi_id == SX_QOI_VALUE) {
+      g_internal_qf = true;
+      const int rc  = sx_form_add_domain_integral(q->grad, sxk::QF_ID_QOI_VALUE, T.params, 1, dep, 1,
+                                                  elems && nelem > 0 ? elems : nullptr, nelem);
+      g_internal_qf = false;
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    } else if (qoi_id == SX_QOI_NEOHOOKEAN_ENERGY) {
+      const int rc = sx_form_add_domain_integral(q->grad, SX_QF_NEOHOOKEAN, T.params, 3, dep, 1,
+                                                 elems && nelem > 0 ? elems : nullptr, nelem);
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    }
+    q->terms.push_back(std::move(Tp));
+  });
+}
+
+int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value)
+{
+  return api([&] {
+    SX_REQUIRE(q && u && value, SX_ERR_ARG, "null argument");
+    const sx_space* s      = q->space;
+    hipStream_t     stream = q->grad->stream;
+    double          total  = 0.0;
+    for (auto& Tp : q->terms) {
+      const sx_qoi::Term& T = *Tp;
+      sxk::QoiLaunch      P{};
+      P.order    = s->order;
+      P.nc       = s->nc;
+      P.qoi      = T.id;
+      P.ordering = s->ordering;
+      P.map      = T.d_map.as<const int>();
+      P.coords   = T.d_coords.as<const double>();
+      P.nslots   = T.nslots;
+      P.nnodes   = s->nnodes;
+      P.u        = u;
+      P.tables   = element_tables(s->order + 1);
+      for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = T.params[k];
+      P.partial = Tp->d_part.as<double>();
+      P.stream  = stream;
+      SX_HIP(sxk::launch_qoi(P));
+      std::vector<double> part(size_t(T.nslots));
+      SX_HIP(hipMemcpyAsync(part.data(), T.d_part.ptr, part.size() * 8, hipMemcpyDeviceToHost, stream));
+      SX_HIP(hipStreamSynchronize(stream));
+      for (double v : part) total += v;  // element order: reproducible
+    }
+    (void)t;
+    *value = total;
+  });
+}
+
+int sx_qoi_gradient(sx_qoi* q, double t, const double* u, double* grad)
+{
+  return api([&] {
+    SX_REQUIRE(q && u && grad, SX_ERR_ARG, "null argument");
+    if (q->grad->integrals.empty()) {
+      SX_HIP(sxk::vec_fill(grad, 0.0, q->space->ndof(), q->grad->stream));
+      SX_HIP(hipStreamSynchronize(q->grad->stream));
+      return;
+    }
+    const double* in[1] = {u};
+    sx_grad*      g     = nullptr;
+    const int     rc    = sx_form_eval(q->grad, t, in, grad, -1, &g);
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    SX_HIP(hipStreamSynchronize(q->grad->stream));
+  });
+}
+
+int sx_qoi_destroy(sx_qoi* q)
+{
+  delete q;
+  return SX_OK;
 }
 
 }  // extern "C"

@@ -1022,6 +1022,18 @@ inline int dispatch_order(const ElemLaunch& P, hipError_t* err)
     case QF_ID_J2:
       if (P.nc == 3) return dispatch_eval<N, 3, QFJ2, LAYOUT_GRAD_FULL>(P, err);
       break;
+    case QF_ID_QOI_VALUE:
+      // residual only: the QoI gradient is never linearized
+      if (P.mode != MODE_RESIDUAL) return 0;
+      if (P.nc == 1) {
+        *err = launch_grid<N>(P, k_element<N, 1, MODE_RESIDUAL, QFQoiValue<1>, LAYOUT_GRAD_SYM, double>);
+        return 1;
+      }
+      if (P.nc == 3) {
+        *err = launch_grid<N>(P, k_element<N, 3, MODE_RESIDUAL, QFQoiValue<3>, LAYOUT_GRAD_SYM, double>);
+        return 1;
+      }
+      break;
   }
   return 0;
 }

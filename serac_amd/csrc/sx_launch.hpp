@@ -21,7 +21,14 @@ enum Layout { LAYOUT_GRAD_SYM = 0, LAYOUT_FULL_SYM = 1, LAYOUT_GRAD_FULL = 2 };
 // PREC_MIXED (J.v only): FP64 vectors, FP32 element arithmetic
 enum Precision { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16 = 2, PREC_MIXED = 3 };
 
-enum QFId { QF_ID_HEAT = 1, QF_ID_MASS_DIFFUSION = 2, QF_ID_NEOHOOKEAN = 3, QF_ID_LINEAR_ELASTIC = 4, QF_ID_J2 = 5 };
+enum QFId {
+  QF_ID_HEAT           = 1,
+  QF_ID_MASS_DIFFUSION = 2,
+  QF_ID_NEOHOOKEAN     = 3,
+  QF_ID_LINEAR_ELASTIC = 4,
+  QF_ID_J2             = 5,
+  QF_ID_QOI_VALUE      = 6  // internal: the gradient of the QoI int u_c dV
+};
 
 struct ElemLaunch {
   int           order, nc, qf, mode, layout, ordering;

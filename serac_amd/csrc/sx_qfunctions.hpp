@@ -173,6 +173,27 @@ struct QFNeoHookean {
   }
 };
 
+// gradient of the quantity of interest  int u_c dV  (Functional<double(trial)>,
+// fn/functional_qoi.inl): source e_c, no flux.  p = {c}
+template <int C>
+struct QFQoiValue {
+  static constexpr int  NC         = C;
+  static constexpr bool HAS_SOURCE = true;
+  static constexpr bool USES_VALUE = false;
+  static constexpr int  STATE      = 0;
+  template <typename T>
+  __device__ __forceinline__ static void eval(const double* p, const T*, const T*, T* src, T* flux)
+  {
+    const int c0 = int(p[0]);
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      src[c] = T(c == c0 ? 1.0 : 0.0);
+#pragma unroll
+      for (int d = 0; d < 3; d++) flux[c * 3 + d] = T(0.0);
+    }
+  }
+};
+
 // small-strain linear elasticity: sigma = lambda tr(eps) I + 2 mu eps.  p = {lambda, mu}
 struct QFLinearElastic {
   static constexpr int  NC         = 3;

@@ -325,6 +325,26 @@ int main()
     CHECK(thrown, "vector qfunction on a scalar space raises Error(SX_ERR_ARG)");
   }
 
+  // quantities of interest: Functional<double(trial)>
+  {
+    Functional<double(space)> volume({&fes});
+    volume.AddDomainIntegral(Dimension<3>{}, DependsOn<>{}, qoi::Measure{}, mesh);
+    volume.AddDomainIntegral(Dimension<3>{}, DependsOn<>{}, qoi::Position{0}, mesh);
+    const double vq = volume(0.0, u);
+    CHECK(std::fabs(vq - 1.5) < 1e-13, "QoI measure + x-moment of the perturbed cube: %.15f", vq);
+    Functional<double(space)> energy({&fes});
+    energy.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                             solid_mechanics::StrainEnergy<solid_mechanics::NeoHookean>{mat}, mesh);
+    auto [e, dE] = energy(0.0, differentiate_wrt(u));
+    const std::vector<double> g = dE.assemble().ToHost(), R = residual(0.0, u).ToHost();
+    CHECK(e > 0.0 && relerr(g, R) < 1e-13, "strain-energy gradient == Neo-Hookean residual (%.2e), energy %.3e",
+          relerr(g, R), e);
+    const double dd = dE(du);
+    double       gd = 0.0;
+    for (size_t i = 0; i < g.size(); i++) gd += g[i] * dU[i];
+    CHECK(std::fabs(dd - gd) <= 1e-12 * std::fabs(gd), "QoI Gradient::operator()(dU) == dq/dU . dU");
+  }
+
   // buildMeshFromFile + boundary attribute sets on the reference's beam
   // (data/meshes/beam-hex.mesh: attribute 1 at x = 0, 2 at x = 8)
   {
