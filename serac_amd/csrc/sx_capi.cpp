@@ -224,7 +224,7 @@ struct sx_learned {
   sx_unet*                                   net   = nullptr;
   sxu::FEGrid                                G{};
   int64_t                                    nv = 0;  // vertex nodes
-  DeviceBuffer                               rc, xin, y, ss, tmp;
+  DeviceBuffer                               rc, xin, y, ss;
   std::vector<std::unique_ptr<DeviceBuffer>> pr, pt;  // pairs: P^T r_k and the vertex-mean correction
   std::vector<char>                          ok;      // the pair's target is complete
   int                                        max_pairs = 256, stride = 50, window = 50, stall = 20;
@@ -2343,7 +2343,6 @@ int sx_learned_create(const sx_space* sp, uint64_t seed, sx_learned** out)
     L->rc.allocate(size_t(L->nv) * 3 * 4);
     L->xin.allocate(size_t(L->nv) * 3 * 4);
     L->y.allocate(size_t(L->nv) * 3 * 4);
-    L->tmp.allocate(size_t(L->nv) * 3 * 4);
     L->ss.allocate(sizeof(double));
     *out = L.release();
   });

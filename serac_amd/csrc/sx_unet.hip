@@ -595,18 +595,6 @@ __global__ void k_fe_prolong_add(FEGrid G, const float* ec, const double* ss, in
   }
 }
 
-// xc[i][c] = x at vertex node p i
-__global__ void k_fe_inject(FEGrid G, const double* x, float* xc)
-{
-  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
-  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
-    const int     c = int(t % 3);
-    const int64_t i = t / 3;
-    const int     ix = int(i % G.nv[0]), iy = int((i / G.nv[0]) % G.nv[1]), iz = int(i / (int64_t(G.nv[0]) * G.nv[1]));
-    const int64_t j  = (int64_t(G.p * iz) * G.nn[1] + G.p * iy) * G.nn[0] + G.p * ix;
-    xc[t]            = float(x[j * 3 + c]);
-  }
-}
 
 // ss[0] += sum x^2 (FP64)
 __global__ void k_sumsq(const float* x, int64_t n, double* ss)
@@ -839,12 +827,6 @@ hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, in
                      nss, gain, z);
   return hipGetLastError();
 }
-hipError_t fe_inject(const FEGrid& G, const double* x, float* xc, hipStream_t s)
-{
-  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
-  hipLaunchKernelGGL(k_fe_inject, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, G, x, xc);
-  return hipGetLastError();
-}
 hipError_t sumsq(const float* x, int64_t n, double* ss, hipStream_t s)
 {
   hipLaunchKernelGGL(k_sumsq, dim3(std::min<unsigned>(blocks_for(n), 1024)), dim3(kThreads), 0, s, x, n, ss);
@@ -854,11 +836,6 @@ hipError_t normalize(const float* x, const double* ss, int64_t nss, int64_t n, f
 {
   hipLaunchKernelGGL(k_normalize, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, x, ss, nss, n,
                      y);
-  return hipGetLastError();
-}
-hipError_t sub_f32(const float* a, const float* b, float* y, int64_t n, hipStream_t s)
-{
-  hipLaunchKernelGGL(k_sub_f<float>, dim3(std::min<unsigned>(blocks_for(n), 8192)), dim3(kThreads), 0, s, a, b, y, n);
   return hipGetLastError();
 }
 hipError_t sub_f64(const double* a, const double* b, float* y, int64_t n, hipStream_t s)

@@ -65,15 +65,13 @@ hipError_t adam(float* w, float* m, float* v, float* g, uint16_t* wbf, int64_t n
                 float eps, int step, hipStream_t s);
 hipError_t to_bf16(const float* x, uint16_t* y, int64_t n, hipStream_t s);
 // rc = P^T r (or, with `average`, (P^T r) / (P^T 1) row by row);
-// z += gain sqrt(ss / nss) P ec; xc = x at the vertex nodes
+// z += gain sqrt(ss / nss) P ec
 hipError_t fe_restrict(const FEGrid& G, const double* r, float* rc, hipStream_t s, bool average = false);
 hipError_t fe_prolong_add(const FEGrid& G, const float* ec, const double* ss, int64_t nss, double* z, hipStream_t s,
                           double gain = 1.0);
-hipError_t fe_inject(const FEGrid& G, const double* x, float* xc, hipStream_t s);
-// ss[0] += sum x^2;  y = x / sqrt(ss / nss);  y = a - b
+// ss[0] += sum x^2;  y = x / sqrt(ss / nss)
 hipError_t sumsq(const float* x, int64_t n, double* ss, hipStream_t s);
 hipError_t normalize(const float* x, const double* ss, int64_t nss, int64_t n, float* y, hipStream_t s);
-hipError_t sub_f32(const float* a, const float* b, float* y, int64_t n, hipStream_t s);
 // y = a - b formed in FP64, stored as FP32
 hipError_t sub_f64(const double* a, const double* b, float* y, int64_t n, hipStream_t s);
 // xc = (P^T x) / (P^T 1) in FP64
