@@ -402,6 +402,12 @@ enum {
 int sx_qoi_create(const sx_space* trial, sx_qoi** out);
 /* elems == NULL / nelem == 0: the whole mesh */
 int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* elems, int nelem);
+/* AddBoundaryIntegral (fn/functional_qoi.inl:204-230) of SX_QOI_MEASURE,
+ * _POSITION or _VALUE over the boundary faces on the bounding-box planes of
+ * face_mask (0: the whole boundary) or, when attr_mask != 0, over the
+ * boundary elements with those attributes; dA = |dX/dxi1 x dX/dxi2| */
+int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, int face_mask,
+                                 uint64_t attr_mask);
 /* Functional<double(trial)>::operator()(t, U); u is a device L-vector */
 int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value);
 /* dq/dU at U into grad (device L-vector) */

@@ -909,6 +909,24 @@ class Functional<double(trial)> {
     AddDomainIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, Domain{});
   }
 
+  /// AddBoundaryIntegral (fn/functional_qoi.inl:204-230): measure, position or value over boundary faces
+  template <int dim, int... args, typename Integrand>
+  void AddBoundaryIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand,
+                           const BoundaryDomain& domain)
+  {
+    static_assert(dim == 2, "boundary integrals on hexahedral meshes are over quadrilateral faces (Dimension<2>)");
+    static_assert(qoi_traits<Integrand>::registered,
+                  "QoI integrand type is not in the compiled registry (see qoi_traits)");
+    const std::vector<double> p = qoi_traits<Integrand>::params(integrand);
+    detail::check(sx_qoi_add_boundary_integral(h_, qoi_traits<Integrand>::id(integrand), p.data(), int(p.size()),
+                                               domain.face_mask, domain.attr_mask));
+  }
+  template <int dim, int... args, typename Integrand>
+  void AddBoundaryIntegral(Dimension<dim>, DependsOn<args...>, const Integrand& integrand, const Mesh&)
+  {
+    AddBoundaryIntegral(Dimension<dim>{}, DependsOn<args...>{}, integrand, BoundaryDomain::EntireBoundary());
+  }
+
   /// operator()(t, U) -> q(U) (fn/functional_qoi.inl:380-400)
   double operator()(double t, const Vector& U)
   {

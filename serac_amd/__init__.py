@@ -481,6 +481,13 @@ class QoI:
             el = (ctypes.c_int * len(elems))(*elems)
             capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), el, len(elems))
 
+    def add_boundary_integral(self, qoi, params=(), face_mask=0, attrs=None):
+        """Boundary QoI over the planes of face_mask (0: the whole boundary) or the attributes `attrs`."""
+        p = np.zeros(16)
+        p[: len(params)] = params
+        capi.call("sx_qoi_add_boundary_integral", self.h, qoi, _dptr(p), len(params), face_mask,
+                  attr_mask(attrs) if attrs is not None else 0)
+
     def __call__(self, t, U):
         v = ctypes.c_double()
         capi.call("sx_qoi_eval", self.h, t, U.ptr, ctypes.byref(v))

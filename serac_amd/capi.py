@@ -139,6 +139,7 @@ SIGNATURES = {
     "sx_unet_set_mfma": (_int, [_vp, _int]),
     "sx_qoi_create": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_qoi_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int]),
+    "sx_qoi_add_boundary_integral": (_int, [_vp, _int, _D, _int, _int, ctypes.c_uint64]),
     "sx_qoi_eval": (_int, [_vp, ctypes.c_double, _vp, _D]),
     "sx_qoi_gradient": (_int, [_vp, ctypes.c_double, _vp, _vp]),
     "sx_qoi_destroy": (_int, [_vp]),

@@ -2597,7 +2597,8 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
 // ---------------------------------------------------------------------------
 struct sx_qoi {
   struct Term {
-    int          id = 0;
+    int          id   = 0;
+    bool         face = false;  // a boundary integral: slots are faces
     double       params[SX_MAX_PARAMS] = {};
     int64_t      nslots = 0;
     DeviceBuffer d_map, d_coords, d_part;
@@ -2692,6 +2693,74 @@ int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int 
   });
 }
 
+int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, int face_mask,
+                                 uint64_t attr_mask)
+{
+  return api([&] {
+    SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
+    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+               "QoI parameters: at most 16 doubles");
+    SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE, SX_ERR_UNSUPPORTED,
+               "boundary QoIs: SX_QOI_MEASURE, SX_QOI_POSITION or SX_QOI_VALUE");
+    SX_REQUIRE(face_mask >= 0 && face_mask < 64, SX_ERR_ARG, "face_mask selects among 6 planes");
+    const sx_space* vs = q->space;
+    if (qoi_id == SX_QOI_POSITION)
+      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
+                 "position QoI: params {d}, d in 0..2");
+    if (qoi_id == SX_QOI_VALUE)
+      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < vs->nc && params[0] == std::floor(params[0]),
+                 SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
+    const auto&                     m     = vs->mesh->m;
+    const std::vector<BoundaryFace> faces = attr_mask ? attribute_faces(m, attr_mask, true) : boundary_faces(m, face_mask);
+    const int                       n = vs->order + 1, nf = n * n;
+    auto                            Tp = std::make_unique<sx_qoi::Term>();
+    auto&                           T  = *Tp;
+    T.id                               = qoi_id;
+    T.face                             = true;
+    for (int k = 0; k < nparams; k++) T.params[k] = params[k];
+    T.nslots = int64_t(faces.size());
+    std::vector<int>    fmap(faces.size() * size_t(nf));
+    std::vector<double> corners(faces.size() * 12);
+    for (size_t i = 0; i < faces.size(); i++) {
+      for (int b = 0; b < n; b++)
+        for (int a = 0; a < n; a++) {
+          int i0, i1, i2;
+          face_node(faces[i].lf, a, b, n, i0, i1, i2);
+          fmap[i * size_t(nf) + size_t(b * n + a)] =
+              vs->emap[size_t(faces[i].e) * size_t(vs->npe) + size_t((i2 * n + i1) * n + i0)];
+        }
+      for (int k = 0; k < 4; k++) {
+        int i0, i1, i2;
+        face_node(faces[i].lf, k & 1, k >> 1, 2, i0, i1, i2);
+        const int v = m.corner(faces[i].e, i0 + 2 * i1 + 4 * i2);
+        for (int d = 0; d < 3; d++) corners[i * 12 + size_t(k * 3 + d)] = m.X[size_t(3 * v + d)];
+      }
+    }
+    T.d_map.upload(fmap);
+    T.d_coords.upload(corners);
+    T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
+    if (qoi_id == SX_QOI_VALUE) {
+      // the gradient int phi_j dA on component c: the traction integrand with
+      // A = 0, b = -e_c (residual -(A n + b)), or the flux integrand q0 = 1
+      const int dep[1] = {0};
+      const int c      = int(params[0]);
+      int       rc;
+      if (vs->nc == 3) {
+        double tp[12] = {0};
+        tp[9 + c]     = -1.0;
+        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grad, SX_BQF_TRACTION, tp, 12, dep, 1, attr_mask)
+                       : sx_form_add_boundary_integral(q->grad, SX_BQF_TRACTION, tp, 12, dep, 1, face_mask);
+      } else {
+        const double fp[3] = {1.0, 0.0, 0.0};
+        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grad, SX_BQF_FLUX, fp, 3, dep, 1, attr_mask)
+                       : sx_form_add_boundary_integral(q->grad, SX_BQF_FLUX, fp, 3, dep, 1, face_mask);
+      }
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    }
+    q->terms.push_back(std::move(Tp));
+  });
+}
+
 int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value)
 {
   return api([&] {
@@ -2702,6 +2771,7 @@ int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value)
     for (auto& Tp : q->terms) {
       const sx_qoi::Term& T = *Tp;
       sxk::QoiLaunch      P{};
+      const bool          face = T.face;
       P.order    = s->order;
       P.nc       = s->nc;
       P.qoi      = T.id;
@@ -2715,7 +2785,7 @@ int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value)
       for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = T.params[k];
       P.partial = Tp->d_part.as<double>();
       P.stream  = stream;
-      SX_HIP(sxk::launch_qoi(P));
+      SX_HIP(face ? sxk::launch_qoi_face(P) : sxk::launch_qoi(P));
       std::vector<double> part(size_t(T.nslots));
       SX_HIP(hipMemcpyAsync(part.data(), T.d_part.ptr, part.size() * 8, hipMemcpyDeviceToHost, stream));
       SX_HIP(hipStreamSynchronize(stream));
@@ -2730,7 +2800,7 @@ int sx_qoi_gradient(sx_qoi* q, double t, const double* u, double* grad)
 {
   return api([&] {
     SX_REQUIRE(q && u && grad, SX_ERR_ARG, "null argument");
-    if (q->grad->integrals.empty()) {
+    if (q->grad->integrals.empty() && q->grad->bintegrals.empty()) {
       SX_HIP(sxk::vec_fill(grad, 0.0, q->space->ndof(), q->grad->stream));
       SX_HIP(hipStreamSynchronize(q->grad->stream));
       return;

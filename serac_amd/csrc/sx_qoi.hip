@@ -187,6 +187,97 @@ __global__ void __launch_bounds__(256) k_qoi(const QoiLaunch P)
   }
 }
 
+// Boundary faces: one thread per face sums q dA w over its Q x Q points.  The
+// face's nodes are fmap[slot][b n + a] (a along xi1), its bilinear geometry the
+// corners [slot][a + 2 b][3]; dA = |dX/dxi1 x dX/dxi2|.
+template <int N, int NC, int QOI>
+__global__ void __launch_bounds__(256) k_qoi_face(const QoiLaunch P)
+{
+  const int64_t slot = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (slot >= P.nslots) return;
+  const double* tB  = P.tables;
+  const double* tBw = P.tables + 2 * N * N;
+  const double* txq = P.tables + 4 * N * N;
+  const double* C   = P.coords + size_t(slot) * 12;
+  const int*    fm  = P.map + size_t(slot) * N * N;
+  double        ue[NC][N * N];
+#pragma unroll
+  for (int k = 0; k < N * N; k++)
+#pragma unroll
+    for (int c = 0; c < NC; c++) ue[c][k] = P.u[node_dof(fm[k], c, NC, P.nnodes, P.ordering)];
+  double sum = 0.0;
+  for (int q2 = 0; q2 < N; q2++)
+    for (int q1 = 0; q1 < N; q1++) {
+      const double x1 = txq[q1], x2 = txq[q2];
+      double       X[3], t1[3], t2[3];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double c00 = C[0 * 3 + i], c10 = C[1 * 3 + i], c01 = C[2 * 3 + i], c11 = C[3 * 3 + i];
+        X[i]  = (1 - x1) * (1 - x2) * c00 + x1 * (1 - x2) * c10 + (1 - x1) * x2 * c01 + x1 * x2 * c11;
+        t1[i] = (1 - x2) * (c10 - c00) + x2 * (c11 - c01);
+        t2[i] = (1 - x1) * (c01 - c00) + x1 * (c11 - c10);
+      }
+      const double n0 = t1[1] * t2[2] - t1[2] * t2[1], n1 = t1[2] * t2[0] - t1[0] * t2[2];
+      const double n2 = t1[0] * t2[1] - t1[1] * t2[0];
+      const double dA = ::sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+      double       w1 = 0.0, w2 = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        w1 += tBw[q1 * N + k];
+        w2 += tBw[q2 * N + k];
+      }
+      double v[NC];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < N; b++)
+#pragma unroll
+          for (int a = 0; a < N; a++) acc += tB[q1 * N + a] * tB[q2 * N + b] * ue[c][b * N + a];
+        v[c] = acc;
+      }
+      double q;
+      if constexpr (QOI == QOI_ID_MEASURE) {
+        q = 1.0;
+      } else if constexpr (QOI == QOI_ID_POSITION) {
+        const int d = int(P.params[0]);
+        q           = X[d < 0 ? 0 : (d > 2 ? 2 : d)];
+      } else {
+        const int c = int(P.params[0]);
+        q           = v[c < 0 ? 0 : (c >= NC ? NC - 1 : c)];
+      }
+      sum += q * dA * (w1 * w2);
+    }
+  P.partial[slot] = sum;
+}
+
+template <int N, int NC, int QOI>
+hipError_t launch_face3(const QoiLaunch& P)
+{
+  if (P.nslots <= 0) return hipSuccess;
+  hipLaunchKernelGGL((k_qoi_face<N, NC, QOI>), dim3(unsigned((P.nslots + 255) / 256)), dim3(256), 0, P.stream, P);
+  return hipGetLastError();
+}
+
+template <int N, int NC>
+hipError_t launch_face2(const QoiLaunch& P)
+{
+  switch (P.qoi) {
+    case QOI_ID_MEASURE: return launch_face3<N, NC, QOI_ID_MEASURE>(P);
+    case QOI_ID_POSITION: return launch_face3<N, NC, QOI_ID_POSITION>(P);
+    case QOI_ID_VALUE: return launch_face3<N, NC, QOI_ID_VALUE>(P);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int N>
+hipError_t launch_face1(const QoiLaunch& P)
+{
+  if (P.nc == 1) return launch_face2<N, 1>(P);
+  if (P.nc == 3) return launch_face2<N, 3>(P);
+  return hipErrorInvalidValue;
+}
+
 template <int N, int NC, int QOI>
 hipError_t launch3(const QoiLaunch& P)
 {
@@ -220,6 +311,17 @@ hipError_t launch1(const QoiLaunch& P)
 }
 
 }  // namespace
+
+hipError_t launch_qoi_face(const QoiLaunch& P)
+{
+  switch (P.order) {
+    case 1: return launch_face1<2>(P);
+    case 2: return launch_face1<3>(P);
+    case 3: return launch_face1<4>(P);
+    case 4: return launch_face1<5>(P);
+  }
+  return hipErrorInvalidValue;
+}
 
 hipError_t launch_qoi(const QoiLaunch& P)
 {

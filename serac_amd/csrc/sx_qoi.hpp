@@ -11,8 +11,8 @@ enum QoiId { QOI_ID_MEASURE = 1, QOI_ID_POSITION = 2, QOI_ID_VALUE = 3, QOI_ID_N
 
 struct QoiLaunch {
   int           order, nc, qoi, ordering;
-  const int*    map;      // [slot][npe] global node ids
-  const double* coords;   // [slot][8 corners][3]
+  const int*    map;      // [slot][npe] global node ids (faces: [slot][Q*Q])
+  const double* coords;   // [slot][8 corners][3] (faces: [slot][4][3])
   int64_t       nslots;
   int64_t       nnodes;
   const double* u;        // FP64 L-vector of the trial space
@@ -23,5 +23,8 @@ struct QoiLaunch {
 };
 
 hipError_t launch_qoi(const QoiLaunch& P);
+// boundary faces: map = [slot][Q*Q] face nodes (b Q + a), coords = [slot][4][3]
+// corners (a + 2 b); measure, position and value QoIs
+hipError_t launch_qoi_face(const QoiLaunch& P);
 
 }  // namespace sxk

@@ -332,6 +332,11 @@ int main()
     volume.AddDomainIntegral(Dimension<3>{}, DependsOn<>{}, qoi::Position{0}, mesh);
     const double vq = volume(0.0, u);
     CHECK(std::fabs(vq - 1.5) < 1e-13, "QoI measure + x-moment of the perturbed cube: %.15f", vq);
+    Functional<double(space)> sum_of_measures({&fes});
+    sum_of_measures.AddDomainIntegral(Dimension<3>{}, DependsOn<>{}, qoi::Measure{}, mesh);
+    sum_of_measures.AddBoundaryIntegral(Dimension<2>{}, DependsOn<>{}, qoi::Measure{}, mesh);
+    const double sq = sum_of_measures(0.0, u);
+    CHECK(std::fabs(sq - 7.0) < 1e-13, "QoI sum of measures (volume + surface) of the cube: %.15f", sq);
     Functional<double(space)> energy({&fes});
     energy.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
                              solid_mechanics::StrainEnergy<solid_mechanics::NeoHookean>{mat}, mesh);

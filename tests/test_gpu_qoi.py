@@ -134,3 +134,45 @@ def test_qoi_arguments_are_checked(gpu_lib):
     f = sx.Functional(s1, [s1])
     with pytest.raises(sx.SeracAmdError):
         f.add_domain_integral(6, (0.0,))
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("nc", [1, 3])
+def test_boundary_measures_moments_and_values(gpu_lib, p, nc):
+    # the reference's SumOfMeasures: int 1 dV + int 1 dA.  The perturbation
+    # moves interior vertices only, so the boundary is the flat unit-cube
+    # surface (area 6; int x dA = 0 + 1 + 4 * 0.5)
+    m, s = cube(3, 0.2, p, nc)
+    xyz = s.node_coords()
+    Uh = np.zeros(s.ndof)
+    c = nc - 1
+    Uh[c::nc] = 0.4 + 1.2 * xyz[:, 0]
+    U = sx.DeviceVector.from_numpy(Uh)
+    q = sx.QoI(s)
+    q.add_domain_integral(sx.capi.SX_QOI_MEASURE)
+    q.add_boundary_integral(sx.capi.SX_QOI_MEASURE)
+    assert abs(q(0.0, U) - 7.0) < 1e-13
+    qx = sx.QoI(s)
+    qx.add_boundary_integral(sx.capi.SX_QOI_POSITION, (0,))
+    assert abs(qx(0.0, U) - 3.0) < 1e-13
+    qa = sx.QoI(s)
+    qa.add_boundary_integral(sx.capi.SX_QOI_MEASURE, attrs=[3, 5])  # the x = max and x = 0 faces
+    assert abs(qa(0.0, U) - 2.0) < 1e-13
+    qv = sx.QoI(s)
+    qv.add_boundary_integral(sx.capi.SX_QOI_VALUE, (c,))
+    assert abs(qv(0.0, U) - (0.4 * 6 + 1.2 * 3)) < 1e-12
+    g = qv.gradient(0.0, U).numpy()
+    assert abs(g[c::nc].sum() - 6.0) < 1e-12
+    V = W.nodal_field(s.nnodes, nc, 5, 2000, -1.0, 1.0, 1)
+    dq = qv(0.0, sx.DeviceVector.from_numpy(Uh + V)) - qv(0.0, U)
+    assert abs(dq - g @ V) < 1e-12 * max(1.0, abs(dq))
+
+
+def test_beam_hex_boundary_by_attribute(gpu_lib):
+    m = sx.Mesh.read_mfem(os.path.join(ROOT, "tests", "golden", "meshes", "beam-hex.mesh"))
+    s = sx.Space(m, 2, 3, 1)
+    U = sx.DeviceVector(s.ndof)
+    for attrs, area in (([1], 1.0), ([2], 1.0), ([3], 32.0), ([1, 2, 3], 34.0)):
+        q = sx.QoI(s)
+        q.add_boundary_integral(sx.capi.SX_QOI_MEASURE, attrs=attrs)
+        assert abs(q(0.0, U) - area) < 1e-12, attrs
