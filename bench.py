@@ -175,6 +175,109 @@ def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
     }
 
 
+def run_solve(sx, args, space, make_form, precisions, mesh_label):
+    """The load-stepped uniaxial compression (x = 0 clamped, x = 1 displaced by -strain in x over
+    `load_steps`, warm start, Newton + Krylov per step) in each solve mode; wall time per mode."""
+    ess0 = space.boundary_dofs(0b000001, 0x7)
+    ess1 = space.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
+    modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
+             "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
+             "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
+             "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
+    solve, finals = {}, {}
+    for mode in precisions.split(","):
+        tprec, arith = modes[mode]
+        # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
+        scatter = args.scatter if mode == "f64" else args.split_scatter
+        form = make_form(scatter)
+        form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
+        form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
+        learned, t_train, train_loss = None, 0.0, None
+        if mode.endswith("_learned"):
+            # a linear U-Net trained (timed) on pairs from the linearization at
+            # the undeformed state, then flexible CG in every load step
+            tt = time.perf_counter()
+            learned = sx.Learned(space, seed=11)
+            learned.set_activation(sx.capi.SX_UNET_IDENTITY, seed=11)
+            learned.configure(max_pairs=args.learn_pairs, stall_window=20)
+            _, g0 = form(0.0, sx.DeviceVector(space.ndof), wrt=0)
+            learned.generate_pairs(g0, ess, args.learn_pairs, seed=5)
+            del g0
+            train_loss = learned.train(args.learn_steps, lr=args.learn_lr, seed=3, batch=args.learn_batch)
+            sx.capi.call("sx_device_synchronize")
+            t_train = time.perf_counter() - tt
+            log(f"{mode}: trained {args.learn_steps} Adam steps x {args.learn_batch} pairs of "
+                f"{learned.info()['pairs']} in {t_train:.2f} s (pair generation included), loss of the last "
+                f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
+            form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
+        x = sx.DeviceVector(space.ndof)
+        sx.capi.call("sx_device_synchronize")
+        # one load step per call (the same arithmetic as one call over all
+        # steps: each call starts from the essential values x holds) so that
+        # every step reports as it finishes
+        reps, t_solve = [], 0.0
+        for k in range(1, args.load_steps + 1):
+            ts = time.perf_counter()
+            reps += sx.quasistatic_solve(form, x, ess, final * (k / args.load_steps), 1, warm_start=True,
+                                         rel_tol=args.newton_tol, lin_rel_tol=args.cg_tol,
+                                         lin_max_iter=args.cg_max)
+            sx.capi.call("sx_device_synchronize")
+            t_solve += time.perf_counter() - ts
+            log(f"{mode} load step {k}/{args.load_steps}: Newton {reps[-1]['newton_iterations']}, "
+                f"CG {reps[-1]['linear_iterations']}, {t_solve:.2f} s so far")
+        solve[mode] = {
+            "seconds": round(t_solve, 3),
+            "load_steps": args.load_steps,
+            "strain": args.strain,
+            "newton_rel_tol": args.newton_tol,
+            "cg_rel_tol": args.cg_tol,
+            "precond": "Jacobi",
+            "tangent": {0: "f64", 1: "f32", 2: "bf16"}[tprec],
+            "jvp_arithmetic": {0: "f64", 1: "f32"}[arith],
+            "newton_iterations": [r["newton_iterations"] for r in reps],
+            "cg_iterations": [r["linear_iterations"] for r in reps],
+            "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0
+                                   for r in reps],
+        }
+        solve[mode]["scatter"] = scatter
+        solve[mode]["mesh"] = mesh_label
+        if learned is not None:
+            # the wall time above includes the training
+            solve[mode]["seconds"] = round(t_solve + t_train, 3)
+            solve[mode]["precond"] = "learned linear U-Net + Jacobi in flexible CG (Jacobi-CG fallback)"
+            solve[mode]["learned"] = {
+                "train_seconds": round(t_train, 3),
+                "adam_steps": args.learn_steps,
+                "batch": args.learn_batch,
+                "lr": args.learn_lr,
+                "pairs": learned.info()["pairs"],
+                "pairs_from": "the linearization at the undeformed state: r = J_bc e for random smooth e",
+                "activation": "identity",
+                "gain": learned.gain(),
+                "train_loss_last_tenth": train_loss,
+                "fallbacks_to_jacobi_cg": learned.info()["fallbacks"],
+            }
+            form.set_preconditioner(None, sx.capi.SX_PRECOND_JACOBI)
+            del learned
+        log(f"{mode} solve {t_solve:.2f} s: Newton {solve[mode]['newton_iterations']}, "
+            f"CG {solve[mode]['cg_iterations']}")
+        finals[mode] = x.numpy()
+        del x, form
+    if "f64" in finals:
+        ref = finals["f64"]
+        for mode, xm in finals.items():
+            # final displacement against the FP64 Jacobi-CG solve
+            solve[mode]["x_max_err_rel"] = float(np.abs(xm - ref).max() / np.abs(ref).max())
+            solve[mode]["x_rms_err_rel"] = float(np.sqrt(np.mean((xm - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
+    del finals
+    if "f64" in solve:
+        for mode in solve:
+            solve[mode]["speedup_vs_f64"] = round(solve["f64"]["seconds"] / solve[mode]["seconds"], 3)
+    return solve
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,7 +288,10 @@ def main():
     ap.add_argument("--cpu-sample-n", type=int, default=16)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the FP32 / BF16 J.v measurements")
-    ap.add_argument("--solve", action="store_true", help="also time the load-stepped Newton + Jacobi-CG solve")
+    ap.add_argument("--solve", action="store_true",
+                    help="time the load-stepped Newton + Krylov solve on the workload's mesh (without it, a 24^3 "
+                         "solve of the same load path runs in FP64 and mixed precision)")
+    ap.add_argument("--no-default-solve", action="store_true", help="skip the 24^3 solve")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--strain", type=float, default=0.05)
     ap.add_argument("--newton-tol", type=float, default=1e-8)
@@ -233,13 +339,14 @@ def main():
     nq1 = p + 1
     qdatas = []
 
-    def make_form(scatter):
-        f = sx.Functional(space, [space])
+    def make_form(scatter, sp=None):
+        sp = space if sp is None else sp
+        f = sx.Functional(sp, [sp])
         f.set_option(sx.capi.SX_OPT_SCATTER,
                      sx.capi.SX_SCATTER_ATOMIC if scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
         f.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
         if qf == "j2":
-            qd = sx.QuadratureData(space.ne, nq1 ** 3, np.zeros(10))
+            qd = sx.QuadratureData(sp.ne, nq1 ** 3, np.zeros(10))
             qdatas.append(qd)
             f.add_domain_integral(sx.SX_QF_J2_LINEAR, params, qdata=qd)
         else:
@@ -349,105 +456,22 @@ def main():
     if args.solve and qf != "neohookean":
         raise SystemExit("bench: --solve runs the uniaxial-compression load path of the Neo-Hookean workloads")
     if args.solve:
-        # uniaxial compression: x = 0 clamped, x = 1 displaced by -strain in x over `load_steps`
-        ess0 = space.boundary_dofs(0b000001, 0x7)
-        ess1 = space.boundary_dofs(0b000010, 0x1)
-        ess = np.union1d(ess0, ess1).astype(np.int32)
-        final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
-        modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
-                 "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-                 "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
-                 "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
-        solve, finals = {}, {}
         del form
-        for mode in args.solve_precision.split(","):
-            tprec, arith = modes[mode]
-            # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
-            scatter = args.scatter if mode == "f64" else args.split_scatter
-            form = make_form(scatter)
-            form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
-            form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
-            learned, t_train, train_loss = None, 0.0, None
-            if mode.endswith("_learned"):
-                # a linear U-Net trained (timed) on pairs from the linearization at
-                # the undeformed state, then flexible CG in every load step
-                tt = time.perf_counter()
-                learned = sx.Learned(space, seed=11)
-                learned.set_activation(sx.capi.SX_UNET_IDENTITY, seed=11)
-                learned.configure(max_pairs=args.learn_pairs, stall_window=20)
-                _, g0 = form(0.0, sx.DeviceVector(space.ndof), wrt=0)
-                learned.generate_pairs(g0, ess, args.learn_pairs, seed=5)
-                del g0
-                train_loss = learned.train(args.learn_steps, lr=args.learn_lr, seed=3, batch=args.learn_batch)
-                sx.capi.call("sx_device_synchronize")
-                t_train = time.perf_counter() - tt
-                log(f"{mode}: trained {args.learn_steps} Adam steps x {args.learn_batch} pairs of "
-                    f"{learned.info()['pairs']} in {t_train:.2f} s (pair generation included), loss of the last "
-                    f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
-                form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
-            x = sx.DeviceVector(space.ndof)
-            sx.capi.call("sx_device_synchronize")
-            # one load step per call (the same arithmetic as one call over all
-            # steps: each call starts from the essential values x holds) so that
-            # every step reports as it finishes
-            reps, t_solve = [], 0.0
-            for k in range(1, args.load_steps + 1):
-                ts = time.perf_counter()
-                reps += sx.quasistatic_solve(form, x, ess, final * (k / args.load_steps), 1, warm_start=True,
-                                             rel_tol=args.newton_tol, lin_rel_tol=args.cg_tol,
-                                             lin_max_iter=args.cg_max)
-                sx.capi.call("sx_device_synchronize")
-                t_solve += time.perf_counter() - ts
-                log(f"{mode} load step {k}/{args.load_steps}: Newton {reps[-1]['newton_iterations']}, "
-                    f"CG {reps[-1]['linear_iterations']}, {t_solve:.2f} s so far")
-            solve[mode] = {
-                "seconds": round(t_solve, 3),
-                "load_steps": args.load_steps,
-                "strain": args.strain,
-                "newton_rel_tol": args.newton_tol,
-                "cg_rel_tol": args.cg_tol,
-                "precond": "Jacobi",
-                "tangent": {0: "f64", 1: "f32", 2: "bf16"}[tprec],
-                "jvp_arithmetic": {0: "f64", 1: "f32"}[arith],
-                "newton_iterations": [r["newton_iterations"] for r in reps],
-                "cg_iterations": [r["linear_iterations"] for r in reps],
-                "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0
-                                       for r in reps],
-            }
-            solve[mode]["scatter"] = scatter
-            if learned is not None:
-                # the wall time above includes the training
-                solve[mode]["seconds"] = round(t_solve + t_train, 3)
-                solve[mode]["precond"] = "learned linear U-Net + Jacobi in flexible CG (Jacobi-CG fallback)"
-                solve[mode]["learned"] = {
-                    "train_seconds": round(t_train, 3),
-                    "adam_steps": args.learn_steps,
-                    "batch": args.learn_batch,
-                    "lr": args.learn_lr,
-                    "pairs": learned.info()["pairs"],
-                    "pairs_from": "the linearization at the undeformed state: r = J_bc e for random smooth e",
-                    "activation": "identity",
-                    "gain": learned.gain(),
-                    "train_loss_last_tenth": train_loss,
-                    "fallbacks_to_jacobi_cg": learned.info()["fallbacks"],
-                }
-                form.set_preconditioner(None, sx.capi.SX_PRECOND_JACOBI)
-                del learned
-            log(f"{mode} solve {t_solve:.2f} s: Newton {solve[mode]['newton_iterations']}, "
-                f"CG {solve[mode]['cg_iterations']}")
-            finals[mode] = x.numpy()
-            del x, form
+        solve = run_solve(sx, args, space, make_form, args.solve_precision,
+                          f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
         form = make_form(args.scatter)
-        if "f64" in finals:
-            ref = finals["f64"]
-            for mode, xm in finals.items():
-                # final displacement against the FP64 Jacobi-CG solve
-                solve[mode]["x_max_err_rel"] = float(np.abs(xm - ref).max() / np.abs(ref).max())
-                solve[mode]["x_rms_err_rel"] = float(np.sqrt(np.mean((xm - ref) ** 2)) / np.sqrt(np.mean(ref ** 2)))
-        del finals
-        if "f64" in solve:
-            for mode in solve:
-                solve[mode]["speedup_vs_f64"] = round(solve["f64"]["seconds"] / solve[mode]["seconds"], 3)
+    elif qf == "neohookean" and not args.no_default_solve:
+        # the same load path on a 24^3 mesh, FP64 against mixed precision: the
+        # solve on the full mesh takes minutes (run it with --solve)
+        ns = 24
+        smesh = sx.Mesh.cuboid(ns, ns, ns)
+        if perturb > 0:
+            smesh.perturb(42, perturb)
+        sspace = sx.Space(smesh, p, 3)
+        solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32",
+                          f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs; the full-size solve "
+                          f"is bench.py --solve)")
+        del sspace, smesh
 
     if args.calibrate:
         # reference kernels for HBM counter scaling: k_dot_partial reads 2 * 8 * ndof

@@ -589,15 +589,17 @@ bool depends(const Integral& I, int trial)
   return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
 }
 
+// df = J dx; out_zeroed: df already holds zeros (the fill is skipped)
 void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr,
-                    int vprec = sxk::PREC_F64)
+                    int vprec = sxk::PREC_F64, bool out_zeroed = false)
 {
   SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
   SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
   SX_REQUIRE(dx && df, SX_ERR_ARG, "null vector");
   SX_REQUIRE(vprec == sxk::PREC_F64 || vprec == sxk::PREC_F32, SX_ERR_ARG, "J.v vectors are FP64 or FP32");
   sx_form* f = g->form;
-  if (vprec == sxk::PREC_F64)
+  if (out_zeroed) {
+  } else if (vprec == sxk::PREC_F64)
     SX_HIP(sxk::vec_fill(static_cast<double*>(df), 0.0, f->test->ndof(), f->stream));
   else
     SX_HIP(sxk::vec_fill_f32(static_cast<float*>(df), 0.0f, f->test->ndof(), f->stream));
@@ -982,7 +984,7 @@ int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntri
     }
     SX_HIP(hipStreamCreateWithFlags(&f->stream, hipStreamNonBlocking));
     f->part.allocate(size_t(sxk::reduction_blocks()) * sizeof(double));
-    f->scal.allocate(8 * sizeof(double));
+    f->scal.allocate(16 * sizeof(double));
     *out = f.release();
   });
 }
@@ -1546,7 +1548,7 @@ double read_scalar(sx_form* f, int slot)
 }
 
 struct Workspace {
-  DeviceBuffer r, z, d, Ad, diag, ess, t;
+  DeviceBuffer r, z, d, Ad, diag, ess, t, part2;
   DeviceBuffer V, hb, gpart;  // GMRES basis, coefficients, partial dots
   DeviceBuffer Zb;            // flexible GMRES: the preconditioned directions
 };
@@ -1604,24 +1606,45 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
     apply_gradient(g, in, outv);
     SX_HIP(sxk::copy_rows(outv, in, ess_dev, ness, s));
   };
+  // One iteration is five launches: the J.v (into the Ad that cg_step_x
+  // zeroed), the essential rows, the partials of (d, Ad), then cg_step_x and
+  // cg_step_d, which reduce the partials themselves.  The convergence test
+  // runs on the device (scal[8] = stop flag, scal[9] = the iteration that set
+  // it): once (r, z) <= goal the step kernels leave x, r, z, d and the scalars
+  // alone, so the host issues iterations in chunks and reads the state once
+  // per chunk.  x, the iteration count and the final norm are those of a test
+  // after every iteration; the J.v launches queued behind the converged
+  // iteration (fewer than a chunk) are wasted.  A hook sees every iteration,
+  // so with one the chunks are single iterations.
+  w.part2.allocate(size_t(sxk::reduction_blocks()) * 8);
+  double* part_rz = w.part2.as<double>();
+  double* stop    = scal + 8;
+  SX_HIP(sxk::vec_fill(stop, 0.0, 2, s));
   apply_bc(d, Ad);
-  SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));  // scal[1] = den, scal[2] = alpha
-  int it = 0;
-  for (it = 1; it <= max_iter; it++) {
-    SX_HIP(sxk::cg_update(x, r, z, d, Ad, diag, scal, n, part, s));  // scal[3] = betanom, scal[4] = beta
-    if (hook) (*hook)(it, x, r);
-    const double betanom = read_scalar(f, 3);
-    rep->final_norm      = std::sqrt(std::max(betanom, 0.0));
-    SX_REQUIRE(std::isfinite(betanom), SX_ERR_NOCONV, "CG produced a non-finite residual");
-    if (betanom <= goal) {
-      rep->converged = 1;
-      break;
+  SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
+  int it = 1;
+  while (it <= max_iter) {
+    const int chunk = hook ? 1 : std::min(std::max(it / 4, 1), std::min(32, max_iter - it + 1));
+    for (int j = 0; j < chunk; j++, it++) {
+      SX_HIP(sxk::cg_step_x(x, r, z, Ad, d, diag, scal, part, part_rz, n, stop, s));
+      if (hook) (*hook)(it, x, r);
+      SX_HIP(sxk::cg_step_d(d, z, scal, part_rz, n, goal, it, stop, s));
+      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true);
+      SX_HIP(sxk::copy_rows(Ad, d, ess_dev, ness, s));
+      SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
     }
-    SX_HIP(sxk::vec_xpay_dev(z, scal + 4, d, n, s));  // d = z + beta d
-    apply_bc(d, Ad);
-    SX_HIP(sxk::vec_dot_dev(d, Ad, n, part, scal, 1, 1, s));
+    double st[7];  // scal[3..9]
+    SX_HIP(hipMemcpyAsync(st, scal + 3, sizeof(st), hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    rep->final_norm = std::sqrt(std::max(st[0], 0.0));
+    SX_REQUIRE(std::isfinite(st[0]), SX_ERR_NOCONV, "CG produced a non-finite residual");
+    if (st[5] != 0.0) {
+      rep->converged         = 1;
+      rep->linear_iterations = int(st[6]);
+      return;
+    }
   }
-  rep->linear_iterations = std::min(it, max_iter);
+  rep->linear_iterations = max_iter;
 }
 
 // Restarted GMRES(m) with left Jacobi preconditioning on J_bc x = b, x0 = 0,

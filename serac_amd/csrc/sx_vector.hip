@@ -37,9 +37,11 @@ __global__ void __launch_bounds__(kThreads) k_axpy(double a, const double* x, do
     y[i] += a * x[i];
 }
 
-// y = x + a * y  (with a read from device memory when ap != nullptr)
-__global__ void __launch_bounds__(kThreads) k_xpay_dev(const double* x, const double* ap, double* y, int64_t n)
+// y = x + a * y with a read from device memory; nothing once stop[0] is set
+__global__ void __launch_bounds__(kThreads) k_xpay_dev(const double* x, const double* ap, double* y, int64_t n,
+                                                        const double* stop)
 {
+  if (stop && stop[0] != 0.0) return;
   const double a = *ap;
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
     y[i] = x[i] + a * y[i];
@@ -70,18 +72,26 @@ __global__ void __launch_bounds__(kThreads) k_dot_partial(const double* a, const
 // Final reduction of nb partials into scal[slot]; op then updates derived
 // scalars in place:  op 1: scal[2] = scal[0] / scal[1]            (alpha)
 //                    op 2: scal[4] = scal[3] / scal[0], scal[0] = scal[3] (beta)
-__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, int nb, double* scal, int slot, int op)
+// With stop != nullptr nothing is written once stop[0] is set, and op 2 sets
+// stop = {1, it} when the new (r, z) is at most goal or not a number.
+__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, int nb, double* scal, int slot, int op,
+                                                            double goal, int it, double* stop)
 {
   __shared__ double sh[4];
   double            s = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
   s = block_sum(s, sh);
   if (threadIdx.x == 0) {
+    if (stop && stop[0] != 0.0) return;
     scal[slot] = s;
     if (op == 1) scal[2] = scal[0] / scal[1];
     if (op == 2) {
       scal[4] = scal[3] / scal[0];
       scal[0] = scal[3];
+      if (stop && !(s > goal)) {
+        stop[0] = 1.0;
+        stop[1] = double(it);
+      }
     }
   }
 }
@@ -89,9 +99,10 @@ __global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, i
 // x += alpha d; r -= alpha Ad; z = r / diag; partial (r, z)
 __global__ void __launch_bounds__(kThreads) k_cg_update(double* x, double* r, double* z, const double* d,
                                                          const double* Ad, const double* diag, const double* scal,
-                                                         int64_t n, double* part)
+                                                         int64_t n, double* part, const double* stop)
 {
   __shared__ double sh[4];
+  if (stop && stop[0] != 0.0) return;  // the whole block leaves before any barrier
   const double      alpha = scal[2];
   double            s     = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
@@ -104,6 +115,79 @@ __global__ void __launch_bounds__(kThreads) k_cg_update(double* x, double* r, do
   }
   s = block_sum(s, sh);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// Sum of the nb reduction partials, the same fixed order as k_reduce_final
+// (every block of a kernel gets the same bits)
+__device__ __forceinline__ double sum_partials(const double* part, int nb, double* sh, double* bc)
+{
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *bc = s;
+  __syncthreads();
+  return *bc;
+}
+
+// First half of a Jacobi-PCG iteration: den = sum of part_den (the partials
+// of (d, Ad)), alpha = scal[0] / den; x += alpha d; r -= alpha Ad; z = r /
+// diag; Ad = 0 for the next J.v; partials of (r, z) into part_rz.  Block 0
+// records scal[1] = den, scal[2] = alpha and scal[5] = scal[0].
+__global__ void __launch_bounds__(kThreads) k_cg_step_x(double* x, double* r, double* z, double* Ad, const double* d,
+                                                         const double* diag, double* scal, const double* part_den,
+                                                         double* part_rz, int nb, int64_t n, const double* stop)
+{
+  __shared__ double sh[4];
+  __shared__ double bc;
+  if (stop[0] != 0.0) return;  // the whole block leaves before any barrier
+  const double den   = sum_partials(part_den, nb, sh, &bc);
+  const double nom   = scal[0];
+  const double alpha = nom / den;
+  double       s     = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    x[i] += alpha * d[i];
+    const double ri = r[i] - alpha * Ad[i];
+    Ad[i]           = 0.0;
+    r[i]            = ri;
+    const double zi = ri / diag[i];
+    z[i]            = zi;
+    s += ri * zi;
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) {
+    part_rz[blockIdx.x] = s;
+    if (blockIdx.x == 0) {
+      scal[1] = den;
+      scal[2] = alpha;
+      scal[5] = nom;
+    }
+  }
+}
+
+// Second half: betanom = sum of part_rz, beta = betanom / scal[5]; d = z +
+// beta d.  Block 0 records scal[0] = scal[3] = betanom, scal[4] = beta and,
+// when betanom is at most goal or not a number, stop = {1, it}.  A block that
+// starts after that write leaves d alone, which no later step reads.
+__global__ void __launch_bounds__(kThreads) k_cg_step_d(double* d, const double* z, double* scal,
+                                                         const double* part_rz, int nb, int64_t n, double goal, int it,
+                                                         double* stop)
+{
+  __shared__ double sh[4];
+  __shared__ double bc;
+  if (stop[0] != 0.0) return;
+  const double betanom = sum_partials(part_rz, nb, sh, &bc);
+  const double beta    = betanom / scal[5];
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    d[i] = z[i] + beta * d[i];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[0] = betanom;
+    scal[3] = betanom;
+    scal[4] = beta;
+    if (!(betanom > goal)) {
+      stop[1] = double(it);
+      stop[0] = 1.0;
+    }
+  }
 }
 
 // z = r / diag; partial (r, z)
@@ -273,30 +357,54 @@ hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s)
+hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s,
+                        const double* stop)
 {
   if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_xpay_dev, dim3(grid_for(n)), dim3(kThreads), 0, s, x, a_dev, y, n);
+  hipLaunchKernelGGL(k_xpay_dev, dim3(grid_for(n)), dim3(kThreads), 0, s, x, a_dev, y, n, stop);
   return hipGetLastError();
 }
 
 int reduction_blocks() { return 1024; }
 
 hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part, double* scal, int slot, int op,
-                       hipStream_t s)
+                       hipStream_t s, double* stop)
 {
   const int nb = reduction_blocks();
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kThreads), 0, s, a, b, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, op);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, op, 0.0, 0, stop);
   return hipGetLastError();
 }
 
 hipError_t cg_update(double* x, double* r, double* z, const double* d, const double* Ad, const double* diag,
-                     double* scal, int64_t n, double* part, hipStream_t s)
+                     double* scal, int64_t n, double* part, hipStream_t s, double goal, int it, double* stop)
 {
   const int nb = reduction_blocks();
-  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r, z, d, Ad, diag, scal, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, 3, 2);
+  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r, z, d, Ad, diag, scal, n, part, stop);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, 3, 2, goal, it, stop);
+  return hipGetLastError();
+}
+
+hipError_t dot_partial(const double* a, const double* b, int64_t n, double* part, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_dot_partial, dim3(reduction_blocks()), dim3(kThreads), 0, s, a, b, n, part);
+  return hipGetLastError();
+}
+
+hipError_t cg_step_x(double* x, double* r, double* z, double* Ad, const double* d, const double* diag, double* scal,
+                     const double* part_den, double* part_rz, int64_t n, const double* stop, hipStream_t s)
+{
+  const int nb = reduction_blocks();
+  hipLaunchKernelGGL(k_cg_step_x, dim3(nb), dim3(kThreads), 0, s, x, r, z, Ad, d, diag, scal, part_den, part_rz, nb,
+                     n, stop);
+  return hipGetLastError();
+}
+
+hipError_t cg_step_d(double* d, const double* z, double* scal, const double* part_rz, int64_t n, double goal, int it,
+                     double* stop, hipStream_t s)
+{
+  const int nb = reduction_blocks();
+  hipLaunchKernelGGL(k_cg_step_d, dim3(nb), dim3(kThreads), 0, s, d, z, scal, part_rz, nb, n, goal, it, stop);
   return hipGetLastError();
 }
 
@@ -305,7 +413,7 @@ hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n,
 {
   const int nb = reduction_blocks();
   hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, r, diag, z, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0, 0.0, 0, nullptr);
   return hipGetLastError();
 }
 
