@@ -185,12 +185,14 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
     modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
              "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
              "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
-             "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
+             "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
+             "f64_mg": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
+             "mixed_f32_mg": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
     solve, finals = {}, {}
     for mode in precisions.split(","):
         tprec, arith = modes[mode]
         # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
-        scatter = args.scatter if mode == "f64" else args.split_scatter
+        scatter = args.scatter if mode.startswith("f64") else args.split_scatter
         form = make_form(scatter)
         form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
         form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
@@ -212,6 +214,12 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
                 f"{learned.info()['pairs']} in {t_train:.2f} s (pair generation included), loss of the last "
                 f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
             form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
+        mg = None
+        if mode.endswith("_mg"):
+            # the multigrid V-cycle (set up once per linearization, inside the timed solve)
+            mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
+                              smoothing_range=args.mg_range)
+            form.set_multigrid(mg)
         x = sx.DeviceVector(space.ndof)
         sx.capi.call("sx_device_synchronize")
         # one load step per call (the same arithmetic as one call over all
@@ -243,6 +251,16 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
         }
         solve[mode]["scatter"] = scatter
         solve[mode]["mesh"] = mesh_label
+        if mg is not None:
+            info = mg.info()
+            solve[mode]["precond"] = "geometric multigrid V-cycle (Chebyshev-Jacobi, Galerkin vertex-grid levels)"
+            solve[mode]["multigrid"] = {
+                "fine_degree": args.mg_fine_degree, "coarse_degree": args.mg_coarse_degree,
+                "smoothing_range": args.mg_range, "setups": info["setups"],
+                "levels": [lv["dims"] for lv in info["levels"]],
+            }
+            form.set_multigrid(None)
+            del mg
         if learned is not None:
             # the wall time above includes the training
             solve[mode]["seconds"] = round(t_solve + t_train, 3)
@@ -302,6 +320,9 @@ def main():
                          "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov), "
                          "mixed_f32_learned (mixed_f32 with flexible CG and the learned U-Net preconditioner, "
                          "trained first on pairs from the undeformed linearization; training time included)")
+    ap.add_argument("--mg-fine-degree", type=int, default=2, help="Chebyshev degree on the order-p level")
+    ap.add_argument("--mg-coarse-degree", type=int, default=2, help="Chebyshev degree on the vertex levels")
+    ap.add_argument("--mg-range", type=float, default=12.0, help="Chebyshev smoothing range")
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps of the learned preconditioner")
     ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--learn-batch", type=int, default=8, help="pairs per Adam step")

@@ -385,6 +385,47 @@ int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t 
 int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
                  double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
 
+/* ---- geometric multigrid preconditioner ----
+ * Stands where the reference's linear solver options offer hypre's AMG
+ * (num/equation_solver.cpp:1022-1051, Preconditioner::HypreAMG): for a
+ * 3-component byVDIM space on a structured (cuboid) mesh of order p, a V-cycle
+ * over the order-p nodal grid (level 0, the form's matrix-free J_bc) and the
+ * mesh's vertex grids (level 1; each further level halves the intervals while
+ * every direction has an even count and the level holds more than 200 DOFs).
+ * Coarse operators are Galerkin P^T A P (27-point 3x3-block stencils, found
+ * by probing: 81 applications of the finer operator per level), smoothing is
+ * Chebyshev-Jacobi of degree `fine_degree` on level 0 and `coarse_degree`
+ * above, on [upper / smoothing_range, upper] with upper = 1.15 x a
+ * `power_iterations`-step power-iteration estimate of the largest eigenvalue
+ * of D^-1 A, and the coarsest level (at most 1536 DOFs) is solved with a
+ * dense inverse.  Essential DOFs are eliminated on every level (a vertex
+ * component is constrained when its own node's is) and M is the identity on
+ * them.  Defaults: degrees 2 and 2, range 12, 12 power iterations. */
+typedef struct sx_multigrid sx_multigrid;
+int sx_mg_create(const sx_space* s, sx_multigrid** out);
+int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double smoothing_range, int power_iterations);
+/* number of levels and of setups so far */
+int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups);
+/* level l: DOFs, grid points per direction (dims[3]) and the eigenvalue estimate of the last setup */
+int sx_mg_level(const sx_multigrid* M, int level, int64_t* ndof, int* dims, double* lmax);
+/* builds the hierarchy for the current linearization of g (host DOF list) */
+int sx_mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess_dofs, int64_t ness);
+/* z = M r with the hierarchy of the last sx_mg_setup (device vectors) */
+int sx_mg_apply(sx_multigrid* M, const double* r, double* z);
+/* y = A_l x on level l of the last setup: level 0 is J with the essential
+ * rows zeroed (x must be zero there), level l >= 1 the Galerkin stencil */
+int sx_mg_level_apply(sx_multigrid* M, int level, const double* x, double* y);
+/* CG on J_bc x = b (x0 = 0) preconditioned by M: the recurrences and the
+ * stopping rule (r, M r) <= max(rel^2 (r0, M r0), abs^2) of mfem::CGSolver;
+ * sets the hierarchy up first */
+int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                    double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
+/* the CG solves of sx_newton_solve / sx_quasistatic_solve on this form use M
+ * (set up once per linearization); NULL returns to Jacobi.  M is borrowed:
+ * detach it before sx_mg_destroy. */
+int sx_form_set_multigrid(sx_form* f, sx_multigrid* M);
+int sx_mg_destroy(sx_multigrid* M);
+
 /* ---- quantities of interest: Functional<double(trial)> (fn/functional_qoi.inl:59-477) ----
  * q(U) = sum over domain integrals of  int q(X, u, grad u) dV  on one trial
  * space, with the compiled integrands below; the value is summed element by

@@ -284,6 +284,16 @@ class Gradient:
         return rep.as_dict()
 
 
+    def mg_cg(self, mg, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, raise_on_noconv=True):
+        """CG on J_bc x = b preconditioned by the multigrid V-cycle (set up for this linearization first)."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_mg_pcg_solve", self.h, mg.h, _iptr(ess), len(ess), b.ptr, x.ptr, rel_tol, abs_tol,
+                          max_iter, ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
     def gmres(self, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, restart=50, raise_on_noconv=True):
         """Restarted GMRES with left Jacobi preconditioning on J_bc x = b."""
         ess = np.ascontiguousarray(ess, dtype=np.int32)
@@ -389,6 +399,48 @@ class UNet:
     def __del__(self):
         if getattr(self, "h", None) and getattr(self, "owned", True) and capi._lib is not None:
             capi._lib.sx_unet_destroy(self.h)
+            self.h = None
+
+
+class Multigrid:
+    """Geometric multigrid V-cycle of a 3-component structured space: the order-p nodal grid, then the
+    mesh's vertex grids with Galerkin stencils, Chebyshev-Jacobi smoothing and a dense coarsest solve."""
+
+    def __init__(self, space, fine_degree=2, coarse_degree=2, smoothing_range=12.0, power_iterations=12):
+        h = _vp()
+        capi.call("sx_mg_create", space.h, ctypes.byref(h))
+        self.h = h
+        self.space = space
+        self.configure(fine_degree, coarse_degree, smoothing_range, power_iterations)
+
+    def configure(self, fine_degree=2, coarse_degree=2, smoothing_range=12.0, power_iterations=12):
+        capi.call("sx_mg_configure", self.h, fine_degree, coarse_degree, smoothing_range, power_iterations)
+
+    def info(self):
+        nl, ns = ctypes.c_int(), ctypes.c_int64()
+        capi.call("sx_mg_info", self.h, ctypes.byref(nl), ctypes.byref(ns))
+        levels = []
+        for lvl in range(nl.value):
+            nd, dims, lm = ctypes.c_int64(), (ctypes.c_int * 3)(), ctypes.c_double()
+            capi.call("sx_mg_level", self.h, lvl, ctypes.byref(nd), dims, ctypes.byref(lm))
+            levels.append({"ndof": nd.value, "dims": tuple(dims), "lmax": lm.value})
+        return {"levels": levels, "setups": ns.value}
+
+    def setup(self, grad, ess):
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        capi.call("sx_mg_setup", self.h, grad.h, _iptr(ess), len(ess))
+
+    def apply(self, r, z):
+        """z = M r with the hierarchy of the last setup."""
+        capi.call("sx_mg_apply", self.h, r.ptr, z.ptr)
+
+    def level_apply(self, level, x, y):
+        """y = A_level x (level 0: J with the essential rows zeroed; above: the Galerkin stencil)."""
+        capi.call("sx_mg_level_apply", self.h, level, x.ptr, y.ptr)
+
+    def __del__(self):
+        if getattr(self, "h", None) and capi._lib is not None:
+            capi._lib.sx_mg_destroy(self.h)
             self.h = None
 
 
@@ -580,6 +632,11 @@ class Functional:
         """Linear solves of newton / quasistatic_solve: SX_PRECOND_JACOBI, _RECORD or _LEARNED."""
         capi.call("sx_form_set_preconditioner", self.h, learned.h if learned is not None else None, mode)
         self._learned = learned  # the form borrows it
+
+    def set_multigrid(self, mg):
+        """CG solves of newton / quasistatic_solve preconditioned by the multigrid V-cycle (None: Jacobi)."""
+        capi.call("sx_form_set_multigrid", self.h, mg.h if mg is not None else None)
+        self._mg = mg  # the form borrows it
 
     def update_qdata(self, flag):
         """updateQdata(bool): while set, evaluations commit the quadrature-point state."""

@@ -25,6 +25,7 @@ SOURCES = [
     "sx_vector.hip",
     "sx_boundary.hip",
     "sx_qoi.hip",
+    "sx_mg.hip",
     "sx_unet.hip",
     "sx_learn.cpp",
     "sx_capi.cpp",

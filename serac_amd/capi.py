@@ -130,6 +130,18 @@ SIGNATURES = {
     "sx_learned_gain": (_int, [_vp, _D]),
     "sx_learned_apply": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp]),
     "sx_learned_destroy": (_int, [_vp]),
+    "sx_mg_create": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sx_mg_configure": (_int, [_vp, _int, _int, ctypes.c_double, _int]),
+    "sx_mg_info": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_int64)]),
+    "sx_mg_level": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_int),
+                           ctypes.POINTER(ctypes.c_double)]),
+    "sx_mg_setup": (_int, [_vp, _vp, _I32, ctypes.c_int64]),
+    "sx_mg_apply": (_int, [_vp, _vp, _vp]),
+    "sx_mg_level_apply": (_int, [_vp, _int, _vp, _vp]),
+    "sx_mg_pcg_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
+                               _vp]),
+    "sx_form_set_multigrid": (_int, [_vp, _vp]),
+    "sx_mg_destroy": (_int, [_vp]),
     "sx_form_set_preconditioner": (_int, [_vp, _vp, _int]),
     "sx_fcg_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                             ctypes.POINTER(SolveReport)]),

@@ -23,6 +23,7 @@
 #include "sx_launch.hpp"
 #include "sx_setup.hpp"
 #include "sx_qoi.hpp"
+#include "sx_mg.hpp"
 #include "sx_unet.hpp"
 #include "sx_vector.hpp"
 
@@ -243,6 +244,33 @@ struct sx_grad {
   bool     valid = false;
 };
 
+// Geometric multigrid preconditioner of a structured 3-component space: a
+// V-cycle over the order-p nodal grid (level 0: the form's matrix-free J_bc)
+// and the mesh's vertex grids (level 1, then halving the intervals), with
+// Galerkin coarse operators P^T A P found by probing, Chebyshev-Jacobi
+// smoothing on every level but the coarsest, and a dense solve there.
+struct MgLevel {
+  sxm::Grid3          g{};
+  int64_t             n = 0;  // vector entries
+  DeviceBuffer        S, dinv, mask, x, b, r, d, t;
+  double              lmax = 0.0, theta = 1.0;
+  std::vector<double> c1, c2;  // Chebyshev steps 2..degree
+};
+
+struct sx_multigrid {
+  const sx_space*      space = nullptr;
+  sxu::FEGrid          G{};
+  std::vector<MgLevel> lv;
+  DeviceBuffer         coarse_inv, part, scal, diag;
+  DeviceBuffer         ess;             // essential DOFs of the last sx_mg_setup
+  int64_t              ness   = 0;
+  sx_grad*             set_g  = nullptr;  // the gradient of the last sx_mg_setup
+  int                  ncoarse = 0;
+  int                  fine_degree = 2, coarse_degree = 2, power_iters = 12;
+  double               range = 12.0, safety = 1.15;
+  int64_t              setups = 0;
+};
+
 struct sx_form {
   const sx_space*                        test = nullptr;
   std::vector<const sx_space*>           trials;
@@ -258,6 +286,7 @@ struct sx_form {
   int                                    linsolver    = SX_LINSOLVER_CG;  // linear solver of the Newton loops
   int                                    gmres_restart = 50;
   sx_learned*                            learned      = nullptr;        // borrowed
+  sx_multigrid*                          mg           = nullptr;        // borrowed: multigrid-PCG in the Newton loops
   int                                    learn_mode   = 0;              // SX_PRECOND_*
   // solver scratch
   DeviceBuffer part, scal;
@@ -1647,6 +1676,303 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   rep->linear_iterations = max_iter;
 }
 
+// ---------------------------------------------------------------------------
+// geometric multigrid
+// ---------------------------------------------------------------------------
+// y = A_l x: level 0 is J with the essential rows zeroed (x is zero there),
+// the vertex levels apply their stencils
+void mg_level_apply(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, int l, const double* x,
+                    double* y)
+{
+  hipStream_t s = g->form->stream;
+  if (l == 0) {
+    apply_gradient(g, x, y);
+    SX_HIP(sxk::set_rows(y, ess, ness, 0.0, s));
+  } else {
+    SX_HIP(sxm::stencil_apply(M->lv[size_t(l)].S.as<double>(), M->lv[size_t(l)].g, x, y, s));
+  }
+}
+
+double mg_read_dot(sx_multigrid* M, const double* a, const double* b, int64_t n, hipStream_t s)
+{
+  SX_HIP(sxk::vec_dot_dev(a, b, n, M->part.as<double>(), M->scal.as<double>(), 0, 0, s));
+  double v = 0.0;
+  SX_HIP(hipMemcpyAsync(&v, M->scal.as<double>(), sizeof(double), hipMemcpyDeviceToHost, s));
+  SX_HIP(hipStreamSynchronize(s));
+  return v;
+}
+
+// dense inverse of the coarsest operator: symmetrised, constrained rows set to
+// identity, Cholesky, then L^-T L^-1
+void mg_coarse_inverse(sx_multigrid* M, hipStream_t s)
+{
+  MgLevel&      c  = M->lv.back();
+  const int64_t nv = int64_t(c.g.n[0]) * c.g.n[1] * c.g.n[2];
+  const int     n  = M->ncoarse;
+  std::vector<double>  S(size_t(nv) * 243);
+  std::vector<uint8_t> mask(static_cast<size_t>(n));
+  SX_HIP(hipMemcpyAsync(S.data(), c.S.ptr, S.size() * 8, hipMemcpyDeviceToHost, s));
+  SX_HIP(hipMemcpyAsync(mask.data(), c.mask.ptr, mask.size(), hipMemcpyDeviceToHost, s));
+  SX_HIP(hipStreamSynchronize(s));
+  std::vector<double> A(size_t(n) * size_t(n), 0.0);
+  for (int64_t v = 0; v < nv; v++) {
+    const int ix = int(v % c.g.n[0]), iy = int((v / c.g.n[0]) % c.g.n[1]), iz = int(v / (int64_t(c.g.n[0]) * c.g.n[1]));
+    for (int o = 0; o < 27; o++) {
+      const int jx = ix + o % 3 - 1, jy = iy + (o / 3) % 3 - 1, jz = iz + o / 9 - 1;
+      if (jx < 0 || jx >= c.g.n[0] || jy < 0 || jy >= c.g.n[1] || jz < 0 || jz >= c.g.n[2]) continue;
+      const int64_t j = (int64_t(jz) * c.g.n[1] + jy) * c.g.n[0] + jx;
+      for (int co = 0; co < 3; co++)
+        for (int ci = 0; ci < 3; ci++)
+          A[size_t(v * 3 + co) * size_t(n) + size_t(j * 3 + ci)] = S[size_t(o * 9 + co * 3 + ci) * size_t(nv) + size_t(v)];
+    }
+  }
+  for (int i = 0; i < n; i++) {
+    if (mask[size_t(i)]) A[size_t(i) * n + i] = 1.0;
+    for (int j = 0; j < i; j++) {
+      const double a = 0.5 * (A[size_t(i) * n + j] + A[size_t(j) * n + i]);
+      A[size_t(i) * n + j] = a;
+      A[size_t(j) * n + i] = a;
+    }
+  }
+  // Cholesky in place (lower triangle)
+  for (int j = 0; j < n; j++) {
+    double d = A[size_t(j) * n + j];
+    for (int k = 0; k < j; k++) d -= A[size_t(j) * n + k] * A[size_t(j) * n + k];
+    SX_REQUIRE(d > 0.0 && std::isfinite(d), SX_ERR_NOCONV,
+               "multigrid: the coarsest operator is not positive definite");
+    const double ljj = std::sqrt(d);
+    A[size_t(j) * n + j] = ljj;
+    for (int i = j + 1; i < n; i++) {
+      double a = A[size_t(i) * n + j];
+      for (int k = 0; k < j; k++) a -= A[size_t(i) * n + k] * A[size_t(j) * n + k];
+      A[size_t(i) * n + j] = a / ljj;
+    }
+  }
+  // X = L^-T L^-1, one column of the identity at a time
+  std::vector<double> X(static_cast<size_t>(n) * static_cast<size_t>(n)), y(static_cast<size_t>(n));
+  for (int col = 0; col < n; col++) {
+    for (int i = 0; i < n; i++) {
+      double a = i == col ? 1.0 : 0.0;
+      for (int k = 0; k < i; k++) a -= A[size_t(i) * n + k] * y[size_t(k)];
+      y[size_t(i)] = a / A[size_t(i) * n + i];
+    }
+    for (int i = n - 1; i >= 0; i--) {
+      double a = y[size_t(i)];
+      for (int k = i + 1; k < n; k++) a -= A[size_t(k) * n + i] * X[size_t(k) * n + col];
+      X[size_t(i) * n + col] = a / A[size_t(i) * n + i];
+    }
+  }
+  M->coarse_inv.upload(X);
+}
+
+// Builds the hierarchy for the current linearization of g with the
+// essential DOFs ess (device list): masks, the fine Jacobi diagonal, the
+// Galerkin operators by probing (27 vertex colours x 3 components per level;
+// a colour's vertices are 3 apart, so each vertex sees exactly one of them in
+// its 27-point neighbourhood), the Chebyshev intervals from power iteration
+// on D^-1 A, and the coarsest inverse.
+void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
+{
+  sx_form*    f = g->form;
+  hipStream_t s = f->stream;
+  SX_REQUIRE(f->test == M->space, SX_ERR_ARG, "the multigrid belongs to another space");
+  const int L = int(M->lv.size());
+  MgLevel&  l0 = M->lv[0];
+  SX_HIP(hipMemsetAsync(l0.mask.ptr, 0, size_t(l0.n), s));
+  SX_HIP(sxm::rows_mask(ess, ness, l0.mask.as<uint8_t>(), s));
+  SX_HIP(sxm::vertex_mask(M->G, l0.mask.as<uint8_t>(), M->lv[1].mask.as<uint8_t>(), s));
+  for (int l = 1; l + 1 < L; l++)
+    SX_HIP(sxm::coarsen_mask(M->lv[size_t(l)].mask.as<uint8_t>(), M->lv[size_t(l)].g,
+                             M->lv[size_t(l + 1)].mask.as<uint8_t>(), M->lv[size_t(l + 1)].g, s));
+  compute_diag(g, ess, ness, M->diag.as<double>());
+  SX_HIP(sxm::diag_inverse(M->diag.as<double>(), l0.n, ess, ness, l0.dinv.as<double>(), s));
+  for (int l = 0; l + 1 < L; l++) {
+    MgLevel& F = M->lv[size_t(l)];
+    MgLevel& C = M->lv[size_t(l + 1)];
+    SX_HIP(sxk::vec_fill(C.S.as<double>(), 0.0, C.n * 81, s));
+    for (int cz = 0; cz < 3; cz++)
+      for (int cy = 0; cy < 3; cy++)
+        for (int cx = 0; cx < 3; cx++)
+          for (int ci = 0; ci < 3; ci++) {
+            SX_HIP(sxm::probe_fill(C.x.as<double>(), C.g, cx, cy, cz, ci, C.mask.as<uint8_t>(), s));
+            SX_HIP(sxk::vec_fill(F.d.as<double>(), 0.0, F.n, s));
+            if (l == 0) {
+              SX_HIP(sxm::fe_prolong_add(M->G, C.x.as<double>(), F.d.as<double>(), s));
+              SX_HIP(sxk::set_rows(F.d.as<double>(), ess, ness, 0.0, s));
+            } else {
+              SX_HIP(sxm::grid_prolong_add(C.x.as<double>(), C.g, F.d.as<double>(), F.g, F.mask.as<uint8_t>(), s));
+            }
+            mg_level_apply(M, g, ess, ness, l, F.d.as<double>(), F.t.as<double>());
+            if (l == 0)
+              SX_HIP(sxm::fe_restrict(M->G, F.t.as<double>(), C.r.as<double>(), C.mask.as<uint8_t>(), s));
+            else
+              SX_HIP(sxm::grid_restrict(F.t.as<double>(), F.g, C.r.as<double>(), C.g, C.mask.as<uint8_t>(), s));
+            SX_HIP(sxm::probe_extract(C.r.as<double>(), C.g, cx, cy, cz, ci, C.mask.as<uint8_t>(), C.S.as<double>(),
+                                      s));
+          }
+    SX_HIP(sxm::stencil_dinv(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), C.dinv.as<double>(), s));
+  }
+  // Chebyshev intervals [upper / range, upper], upper = safety * the power
+  // iteration's estimate of the largest eigenvalue of D^-1 A
+  for (int l = 0; l + 1 < L; l++) {
+    MgLevel& V = M->lv[size_t(l)];
+    double*  v = V.d.as<double>();
+    SX_HIP(sxm::seed_field(v, V.n, uint64_t(l + 1), V.mask.as<uint8_t>(), s));
+    double nrm = std::sqrt(mg_read_dot(M, v, v, V.n, s));
+    SX_REQUIRE(nrm > 0.0, SX_ERR_STATE, "multigrid: every DOF of a level is constrained");
+    SX_HIP(sxm::scale(v, 1.0 / nrm, V.n, s));
+    double lam = 0.0;
+    for (int it = 0; it < M->power_iters; it++) {
+      mg_level_apply(M, g, ess, ness, l, v, V.t.as<double>());
+      SX_HIP(sxm::mul_dinv(V.dinv.as<double>(), V.t.as<double>(), v, V.n, s));
+      lam = std::sqrt(mg_read_dot(M, v, v, V.n, s));
+      SX_REQUIRE(lam > 0.0 && std::isfinite(lam), SX_ERR_NOCONV, "multigrid: power iteration broke down");
+      SX_HIP(sxm::scale(v, 1.0 / lam, V.n, s));
+    }
+    V.lmax            = lam;
+    const double up   = M->safety * lam, lo = up / M->range;
+    const double th   = 0.5 * (up + lo), de = 0.5 * (up - lo), sigma = th / de;
+    const int    deg  = l == 0 ? M->fine_degree : M->coarse_degree;
+    double       rho  = 1.0 / sigma;
+    V.theta           = th;
+    V.c1.clear();
+    V.c2.clear();
+    for (int k = 2; k <= deg; k++) {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      V.c1.push_back(rn * rho);
+      V.c2.push_back(2.0 * rn / de);
+      rho = rn;
+    }
+  }
+  mg_coarse_inverse(M, s);
+  M->setups++;
+}
+
+// Chebyshev smoothing of A_l x = b: from x = 0 (pre; leaves r = b - A x) or
+// from the current x (post)
+void mg_smooth(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, int l, const double* b, double* x,
+               bool pre)
+{
+  MgLevel&    V = M->lv[size_t(l)];
+  hipStream_t s = g->form->stream;
+  double *    r = V.r.as<double>(), *d = V.d.as<double>(), *t = V.t.as<double>();
+  if (pre) {
+    SX_HIP(sxm::cheb_init(b, V.dinv.as<double>(), V.theta, x, d, r, 0, 1, V.n, s));
+  } else {
+    mg_level_apply(M, g, ess, ness, l, x, t);
+    SX_HIP(sxm::residual(b, t, r, V.n, s));
+    SX_HIP(sxm::cheb_init(r, V.dinv.as<double>(), V.theta, x, d, r, 1, 0, V.n, s));
+  }
+  for (size_t k = 0; k < V.c1.size(); k++) {
+    mg_level_apply(M, g, ess, ness, l, d, t);
+    SX_HIP(sxm::cheb_step(r, t, d, V.dinv.as<double>(), V.c1[k], V.c2[k], x, V.n, s));
+  }
+  if (pre) {
+    mg_level_apply(M, g, ess, ness, l, d, t);
+    SX_HIP(sxm::sub_inplace(r, t, V.n, s));
+  }
+}
+
+// x = V-cycle(b) on level l (x is overwritten)
+void mg_vcycle(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, int l, const double* b, double* x)
+{
+  hipStream_t s = g->form->stream;
+  const int   L = int(M->lv.size());
+  MgLevel&    V = M->lv[size_t(l)];
+  if (l == L - 1) {
+    SX_HIP(sxm::dense_mv(M->coarse_inv.as<double>(), M->ncoarse, b, x, s));
+    return;
+  }
+  MgLevel& C = M->lv[size_t(l + 1)];
+  mg_smooth(M, g, ess, ness, l, b, x, true);
+  if (l == 0)
+    SX_HIP(sxm::fe_restrict(M->G, V.r.as<double>(), C.b.as<double>(), C.mask.as<uint8_t>(), s));
+  else
+    SX_HIP(sxm::grid_restrict(V.r.as<double>(), V.g, C.b.as<double>(), C.g, C.mask.as<uint8_t>(), s));
+  mg_vcycle(M, g, ess, ness, l + 1, C.b.as<double>(), C.x.as<double>());
+  if (l == 0) {
+    SX_HIP(sxm::fe_prolong_add(M->G, C.x.as<double>(), x, s));
+    SX_HIP(sxk::set_rows(x, ess, ness, 0.0, s));
+  } else {
+    SX_HIP(sxm::grid_prolong_add(C.x.as<double>(), C.g, x, V.g, V.mask.as<uint8_t>(), s));
+  }
+  mg_smooth(M, g, ess, ness, l, b, x, false);
+}
+
+// z = M r: the V-cycle on the free DOFs, the identity on the essential ones
+void mg_apply(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, const double* r, double* z)
+{
+  hipStream_t s  = g->form->stream;
+  MgLevel&    l0 = M->lv[0];
+  SX_HIP(hipMemcpyAsync(l0.b.ptr, r, size_t(l0.n) * 8, hipMemcpyDeviceToDevice, s));
+  SX_HIP(sxk::set_rows(l0.b.as<double>(), ess, ness, 0.0, s));
+  mg_vcycle(M, g, ess, ness, 0, l0.b.as<double>(), z);
+  SX_HIP(sxk::copy_rows(z, r, ess, ness, s));
+}
+
+// CG on J_bc x = b preconditioned by the V-cycle (mfem::CGSolver's
+// recurrences and stopping rule (r, M r) <= max(rel^2 (r0, M r0), abs^2),
+// num/equation_solver.cpp:916-919, with multigrid in place of the Jacobi
+// smoother), the convergence test on the device as in cg
+void mg_pcg(sx_grad* g, sx_multigrid* M, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
+            double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  mg_setup(M, g, ess, ness);
+  w.r.allocate(size_t(n) * 8);
+  w.z.allocate(size_t(n) * 8);
+  w.d.allocate(size_t(n) * 8);
+  w.Ad.allocate(size_t(n) * 8);
+  w.part2.allocate(size_t(sxk::reduction_blocks()) * 8);
+  double *r = w.r.as<double>(), *z = w.z.as<double>(), *d = w.d.as<double>(), *Ad = w.Ad.as<double>();
+  double *part = f->part.as<double>(), *scal = f->scal.as<double>(), *part_rz = w.part2.as<double>();
+  double* stop = scal + 8;
+  SX_HIP(sxk::vec_fill(x, 0.0, n, s));
+  SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  mg_apply(M, g, ess, ness, r, z);
+  SX_HIP(sxk::vec_dot_dev(r, z, n, part, scal, 0, 0, s));  // scal[0] = (r, z)
+  SX_HIP(hipMemcpyAsync(d, z, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
+  const double nom0 = read_scalar(f, 0);
+  SX_REQUIRE(std::isfinite(nom0) && nom0 >= 0.0, SX_ERR_NOCONV, "multigrid-CG: the preconditioner is not positive");
+  rep->initial_norm      = std::sqrt(nom0);
+  const double goal      = std::max(nom0 * rel * rel, abs_tol * abs_tol);
+  rep->final_norm        = rep->initial_norm;
+  rep->converged         = nom0 <= goal;
+  rep->linear_iterations = 0;
+  if (rep->converged) return;
+  SX_HIP(sxk::vec_fill(stop, 0.0, 2, s));
+  apply_gradient(g, d, Ad);
+  SX_HIP(sxk::copy_rows(Ad, d, ess, ness, s));
+  SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
+  int it = 1;
+  while (it <= max_iter) {
+    // a V-cycle costs several J.v, so the host runs at most 4 iterations ahead
+    const int chunk = std::min(std::max(it / 8, 1), std::min(4, max_iter - it + 1));
+    for (int j = 0; j < chunk; j++, it++) {
+      SX_HIP(sxm::pcg_step_xr(x, r, Ad, d, scal, part, sxk::reduction_blocks(), n, stop, s));
+      mg_apply(M, g, ess, ness, r, z);
+      SX_HIP(sxk::dot_partial(r, z, n, part_rz, s));
+      SX_HIP(sxk::cg_step_d(d, z, scal, part_rz, n, goal, it, stop, s));
+      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true);
+      SX_HIP(sxk::copy_rows(Ad, d, ess, ness, s));
+      SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
+    }
+    double st[7];  // scal[3..9]
+    SX_HIP(hipMemcpyAsync(st, scal + 3, sizeof(st), hipMemcpyDeviceToHost, s));
+    SX_HIP(hipStreamSynchronize(s));
+    rep->final_norm = std::sqrt(std::max(st[0], 0.0));
+    SX_REQUIRE(std::isfinite(st[0]), SX_ERR_NOCONV, "multigrid-CG produced a non-finite residual");
+    if (st[5] != 0.0) {
+      rep->converged         = 1;
+      rep->linear_iterations = int(st[6]);
+      return;
+    }
+  }
+  rep->linear_iterations = max_iter;
+}
+
 // Restarted GMRES(m) with left Jacobi preconditioning on J_bc x = b, x0 = 0,
 // the mfem::GMRESSolver configured by num/equation_solver.cpp:916-919 (with
 // HypreSmoother Jacobi, :1018-1021): Arnoldi on D^-1 J_bc, the residual norm
@@ -2144,6 +2470,8 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
     rec.finish(x, rep->converged);
   } else if (f->linsolver == SX_LINSOLVER_GMRES) {
     gmres(g, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
+  } else if (f->mg) {
+    mg_pcg(g, f->mg, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   } else {
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   }
@@ -2338,6 +2666,178 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
       prev = target;
     }
   });
+}
+
+// ---------------------------------------------------------------------------
+// geometric multigrid preconditioner
+// ---------------------------------------------------------------------------
+int sx_mg_create(const sx_space* sp, sx_multigrid** out)
+{
+  return api([&] {
+    SX_REQUIRE(sp && out, SX_ERR_ARG, "null argument");
+    const auto& m = sp->mesh->m;
+    SX_REQUIRE(m.structured, SX_ERR_UNSUPPORTED, "multigrid needs a structured (cuboid) mesh");
+    SX_REQUIRE(sp->nc == 3 && sp->ordering == SX_ORDERING_BYVDIM, SX_ERR_UNSUPPORTED,
+               "multigrid acts on 3-component byVDIM spaces");
+    auto M   = std::make_unique<sx_multigrid>();
+    M->space = sp;
+    M->G.p   = sp->order;
+    for (int d = 0; d < 3; d++) {
+      M->G.nv[d] = m.n[d] + 1;
+      M->G.nn[d] = sp->order * m.n[d] + 1;
+    }
+    const auto xi = sxh::lobatto_points(sp->order + 1);
+    for (int k = 0; k <= sp->order; k++) M->G.xi[k] = xi[size_t(k)];
+    // level 0: the nodal grid; level 1: the vertex grid; then halve the
+    // intervals while every direction has an even count and the level holds
+    // more than 200 DOFs
+    std::vector<sxm::Grid3> grids(2);
+    for (int d = 0; d < 3; d++) grids[1].n[d] = M->G.nv[d];
+    for (;;) {
+      const sxm::Grid3 c  = grids.back();
+      const int64_t    nd = int64_t(c.n[0]) * c.n[1] * c.n[2] * 3;
+      bool             ok = nd > 200;
+      for (int d = 0; d < 3; d++) ok = ok && (c.n[d] - 1) % 2 == 0 && c.n[d] - 1 >= 2;
+      if (!ok) break;
+      sxm::Grid3 nc{};
+      for (int d = 0; d < 3; d++) nc.n[d] = (c.n[d] - 1) / 2 + 1;
+      grids.push_back(nc);
+    }
+    M->lv      = std::vector<MgLevel>(grids.size());
+    M->lv[0].n = sp->ndof();
+    for (size_t l = 1; l < M->lv.size(); l++) {
+      M->lv[l].g = grids[l];
+      M->lv[l].n = int64_t(grids[l].n[0]) * grids[l].n[1] * grids[l].n[2] * 3;
+    }
+    M->ncoarse = int(M->lv.back().n);
+    SX_REQUIRE(M->ncoarse <= 1536, SX_ERR_UNSUPPORTED,
+               "multigrid: the coarsest vertex grid is too large for its dense solve (mesh sizes need factors of 2)");
+    for (size_t l = 0; l < M->lv.size(); l++) {
+      MgLevel& V = M->lv[l];
+      V.dinv.allocate(size_t(V.n) * 8);
+      V.mask.allocate(size_t(V.n));
+      V.r.allocate(size_t(V.n) * 8);
+      V.d.allocate(size_t(V.n) * 8);
+      V.t.allocate(size_t(V.n) * 8);
+      V.b.allocate(size_t(V.n) * 8);
+      if (l > 0) {
+        V.x.allocate(size_t(V.n) * 8);
+        V.S.allocate(size_t(V.n) * 81 * 8);
+      }
+    }
+    M->diag.allocate(size_t(M->lv[0].n) * 8);
+    M->part.allocate(size_t(sxk::reduction_blocks()) * 8);
+    M->scal.allocate(16 * 8);
+    *out = M.release();
+  });
+}
+
+int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double smoothing_range, int power_iterations)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(fine_degree >= 1 && fine_degree <= 16 && coarse_degree >= 1 && coarse_degree <= 16, SX_ERR_ARG,
+               "Chebyshev degrees must be in 1..16");
+    SX_REQUIRE(smoothing_range > 1.0, SX_ERR_ARG, "the smoothing range must exceed 1");
+    SX_REQUIRE(power_iterations >= 1 && power_iterations <= 1000, SX_ERR_ARG, "power iterations must be in 1..1000");
+    M->fine_degree   = fine_degree;
+    M->coarse_degree = coarse_degree;
+    M->range         = smoothing_range;
+    M->power_iters   = power_iterations;
+  });
+}
+
+int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    if (nlevels) *nlevels = int(M->lv.size());
+    if (setups) *setups = M->setups;
+  });
+}
+
+int sx_mg_level(const sx_multigrid* M, int level, int64_t* ndof, int* dims, double* lmax)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(level >= 0 && level < int(M->lv.size()), SX_ERR_ARG, "level out of range");
+    const MgLevel& V = M->lv[size_t(level)];
+    if (ndof) *ndof = V.n;
+    if (dims)
+      for (int d = 0; d < 3; d++) dims[d] = level == 0 ? M->G.nn[d] : V.g.n[d];
+    if (lmax) *lmax = V.lmax;
+  });
+}
+
+int sx_mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess_dofs, int64_t ness)
+{
+  return api([&] {
+    SX_REQUIRE(M && g && g->form, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid setup on a gradient before a linearization");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    for (int64_t k = 0; k < ness; k++)
+      SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
+    M->ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    M->ness  = ness;
+    M->set_g = g;
+    mg_setup(M, g, M->ess.as<const int32_t>(), ness);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+  });
+}
+
+int sx_mg_apply(sx_multigrid* M, const double* r, double* z)
+{
+  return api([&] {
+    SX_REQUIRE(M && r && z, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(M->set_g && M->set_g->valid, SX_ERR_STATE, "multigrid applied before sx_mg_setup");
+    mg_apply(M, M->set_g, M->ess.as<const int32_t>(), M->ness, r, z);
+    SX_HIP(hipStreamSynchronize(M->set_g->form->stream));
+  });
+}
+
+int sx_mg_level_apply(sx_multigrid* M, int level, const double* x, double* y)
+{
+  return api([&] {
+    SX_REQUIRE(M && x && y, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(M->set_g && M->set_g->valid, SX_ERR_STATE, "multigrid applied before sx_mg_setup");
+    SX_REQUIRE(level >= 0 && level < int(M->lv.size()), SX_ERR_ARG, "level out of range");
+    mg_level_apply(M, M->set_g, M->ess.as<const int32_t>(), M->ness, level, x, y);
+    SX_HIP(hipStreamSynchronize(M->set_g->form->stream));
+  });
+}
+
+int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                    double rel_tol, double abs_tol, int max_iter, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && M && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid-CG on a gradient before a linearization");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    for (int64_t k = 0; k < ness; k++)
+      SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    mg_pcg(g, M, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "multigrid-CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+int sx_form_set_multigrid(sx_form* f, sx_multigrid* M)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(!M || M->space == f->test, SX_ERR_ARG, "the multigrid belongs to another space");
+    f->mg = M;
+  });
+}
+
+int sx_mg_destroy(sx_multigrid* M)
+{
+  return api([&] { delete M; });
 }
 
 // ---------------------------------------------------------------------------

@@ -1,0 +1,541 @@
+// sx_mg.hip -- kernels of the geometric multigrid V-cycle on the vertex grids
+// of a structured mesh: grid transfers, the 27-point block-stencil operator of
+// the coarse levels, Galerkin probing, Chebyshev smoothing updates and the
+// dense coarsest solve.  All FP64.
+#include <hip/hip_runtime.h>
+
+#include "sx_mg.hpp"
+
+namespace sxm {
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned blocks_for(int64_t n, unsigned cap = 8192)
+{
+  const int64_t b = (n + kThreads - 1) / kThreads;
+  return unsigned(b < 1 ? 1 : (b > cap ? cap : b));
+}
+
+__device__ __forceinline__ int64_t nverts(const Grid3& g) { return int64_t(g.n[0]) * g.n[1] * g.n[2]; }
+
+__device__ __forceinline__ void coords(const Grid3& g, int64_t v, int& ix, int& iy, int& iz)
+{
+  ix = int(v % g.n[0]);
+  iy = int((v / g.n[0]) % g.n[1]);
+  iz = int(v / (int64_t(g.n[0]) * g.n[1]));
+}
+
+// weight of the vertex at offset 0 for a node d = j - p i away from it
+__device__ __forceinline__ double fe_weight(const sxu::FEGrid& G, int d)
+{
+  return d >= 0 ? 1.0 - G.xi[d] : G.xi[G.p + d];
+}
+
+__global__ void __launch_bounds__(kThreads) k_fe_restrict(sxu::FEGrid G, const double* r, double* rc,
+                                                          const uint8_t* mask)
+{
+  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    if (mask[t]) {
+      rc[t] = 0.0;
+      continue;
+    }
+    const int     c = int(t % 3);
+    const int64_t i = t / 3;
+    const int     iv[3] = {int(i % G.nv[0]), int((i / G.nv[0]) % G.nv[1]), int(i / (int64_t(G.nv[0]) * G.nv[1]))};
+    double        s = 0.0;
+    for (int dz = -G.p; dz <= G.p; dz++) {
+      const int jz = G.p * iv[2] + dz;
+      if (jz < 0 || jz >= G.nn[2]) continue;
+      const double wz = fe_weight(G, dz);
+      for (int dy = -G.p; dy <= G.p; dy++) {
+        const int jy = G.p * iv[1] + dy;
+        if (jy < 0 || jy >= G.nn[1]) continue;
+        const double wzy = wz * fe_weight(G, dy);
+        for (int dx = -G.p; dx <= G.p; dx++) {
+          const int jx = G.p * iv[0] + dx;
+          if (jx < 0 || jx >= G.nn[0]) continue;
+          s += wzy * fe_weight(G, dx) * r[((int64_t(jz) * G.nn[1] + jy) * G.nn[0] + jx) * 3 + c];
+        }
+      }
+    }
+    rc[t] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_fe_prolong_add(sxu::FEGrid G, const double* ec, double* z)
+{
+  const int64_t nn = int64_t(G.nn[0]) * G.nn[1] * G.nn[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nn * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int     c = int(t % 3);
+    const int64_t j = t / 3;
+    const int     jv[3] = {int(j % G.nn[0]), int((j / G.nn[0]) % G.nn[1]), int(j / (int64_t(G.nn[0]) * G.nn[1]))};
+    int           e[3], k[3];
+    for (int d = 0; d < 3; d++) {
+      e[d] = jv[d] / G.p;
+      k[d] = jv[d] - G.p * e[d];
+    }
+    double s = 0.0;
+    for (int b = 0; b < 8; b++) {
+      const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+      if ((bx && k[0] == 0) || (by && k[1] == 0) || (bz && k[2] == 0)) continue;
+      const double w = (bx ? G.xi[k[0]] : 1.0 - G.xi[k[0]]) * (by ? G.xi[k[1]] : 1.0 - G.xi[k[1]]) *
+                       (bz ? G.xi[k[2]] : 1.0 - G.xi[k[2]]);
+      const int64_t v = (int64_t(e[2] + bz) * G.nv[1] + (e[1] + by)) * G.nv[0] + (e[0] + bx);
+      s += w * ec[v * 3 + c];
+    }
+    z[t] += s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_grid_restrict(const double* rf, Grid3 f, double* rc, Grid3 c,
+                                                            const uint8_t* maskc)
+{
+  const int64_t nc = nverts(c);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nc * 3; t += int64_t(gridDim.x) * kThreads) {
+    if (maskc[t]) {
+      rc[t] = 0.0;
+      continue;
+    }
+    const int comp = int(t % 3);
+    int       ix, iy, iz;
+    coords(c, t / 3, ix, iy, iz);
+    double s = 0.0;
+    for (int oz = -1; oz <= 1; oz++) {
+      const int jz = 2 * iz + oz;
+      if (jz < 0 || jz >= f.n[2]) continue;
+      const double wz = oz ? 0.5 : 1.0;
+      for (int oy = -1; oy <= 1; oy++) {
+        const int jy = 2 * iy + oy;
+        if (jy < 0 || jy >= f.n[1]) continue;
+        const double wzy = wz * (oy ? 0.5 : 1.0);
+        for (int ox = -1; ox <= 1; ox++) {
+          const int jx = 2 * ix + ox;
+          if (jx < 0 || jx >= f.n[0]) continue;
+          s += wzy * (ox ? 0.5 : 1.0) * rf[((int64_t(jz) * f.n[1] + jy) * f.n[0] + jx) * 3 + comp];
+        }
+      }
+    }
+    rc[t] = s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_grid_prolong_add(const double* ec, Grid3 c, double* zf, Grid3 f,
+                                                               const uint8_t* maskf)
+{
+  const int64_t nf = nverts(f);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nf * 3; t += int64_t(gridDim.x) * kThreads) {
+    if (maskf[t]) continue;
+    const int comp = int(t % 3);
+    int       iv[3];
+    coords(f, t / 3, iv[0], iv[1], iv[2]);
+    // per direction: the coarse vertices (one or two) and their weights
+    int    lo[3], cnt[3];
+    double w[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d]  = iv[d] >> 1;
+      cnt[d] = (iv[d] & 1) ? 2 : 1;
+      w[d]   = (iv[d] & 1) ? 0.5 : 1.0;
+    }
+    double s = 0.0;
+    for (int bz = 0; bz < cnt[2]; bz++)
+      for (int by = 0; by < cnt[1]; by++)
+        for (int bx = 0; bx < cnt[0]; bx++) {
+          const int64_t v = (int64_t(lo[2] + bz) * c.n[1] + (lo[1] + by)) * c.n[0] + (lo[0] + bx);
+          s += w[0] * w[1] * w[2] * ec[v * 3 + comp];
+        }
+    zf[t] += s;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_stencil_apply(const double* S, Grid3 g, const double* x, double* y)
+{
+  const int64_t nv = nverts(g);
+  for (int64_t v = int64_t(blockIdx.x) * kThreads + threadIdx.x; v < nv; v += int64_t(gridDim.x) * kThreads) {
+    int ix, iy, iz;
+    coords(g, v, ix, iy, iz);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int oz = -1; oz <= 1; oz++) {
+      const int jz = iz + oz;
+      if (jz < 0 || jz >= g.n[2]) continue;
+      for (int oy = -1; oy <= 1; oy++) {
+        const int jy = iy + oy;
+        if (jy < 0 || jy >= g.n[1]) continue;
+        for (int ox = -1; ox <= 1; ox++) {
+          const int jx = ix + ox;
+          if (jx < 0 || jx >= g.n[0]) continue;
+          const int64_t j  = (int64_t(jz) * g.n[1] + jy) * g.n[0] + jx;
+          const double  x0 = x[j * 3], x1 = x[j * 3 + 1], x2 = x[j * 3 + 2];
+          const double* s  = S + int64_t((oz + 1) * 9 + (oy + 1) * 3 + (ox + 1)) * 9 * nv + v;
+          a0 += s[0] * x0 + s[nv] * x1 + s[2 * nv] * x2;
+          a1 += s[3 * nv] * x0 + s[4 * nv] * x1 + s[5 * nv] * x2;
+          a2 += s[6 * nv] * x0 + s[7 * nv] * x1 + s[8 * nv] * x2;
+        }
+      }
+    }
+    y[v * 3]     = a0;
+    y[v * 3 + 1] = a1;
+    y[v * 3 + 2] = a2;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_probe_fill(double* e, Grid3 g, int cx, int cy, int cz, int ci,
+                                                         const uint8_t* mask)
+{
+  const int64_t nv = nverts(g);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    int ix, iy, iz;
+    coords(g, t / 3, ix, iy, iz);
+    const bool on = int(t % 3) == ci && ix % 3 == cx && iy % 3 == cy && iz % 3 == cz && !mask[t];
+    e[t]          = on ? 1.0 : 0.0;
+  }
+}
+
+// the offset o in {-1, 0, 1} with (i + o) = c (mod 3)
+__device__ __forceinline__ int colour_offset(int i, int c)
+{
+  const int o = ((c - i) % 3 + 3) % 3;
+  return o == 2 ? -1 : o;
+}
+
+__global__ void __launch_bounds__(kThreads) k_probe_extract(const double* y, Grid3 g, int cx, int cy, int cz, int ci,
+                                                            const uint8_t* mask, double* S)
+{
+  const int64_t nv = nverts(g);
+  for (int64_t v = int64_t(blockIdx.x) * kThreads + threadIdx.x; v < nv; v += int64_t(gridDim.x) * kThreads) {
+    int ix, iy, iz;
+    coords(g, v, ix, iy, iz);
+    const int ox = colour_offset(ix, cx), oy = colour_offset(iy, cy), oz = colour_offset(iz, cz);
+    const int jx = ix + ox, jy = iy + oy, jz = iz + oz;
+    if (jx < 0 || jx >= g.n[0] || jy < 0 || jy >= g.n[1] || jz < 0 || jz >= g.n[2]) continue;
+    const int64_t j = (int64_t(jz) * g.n[1] + jy) * g.n[0] + jx;
+    if (mask[j * 3 + ci]) continue;
+    const int o = (oz + 1) * 9 + (oy + 1) * 3 + (ox + 1);
+    for (int co = 0; co < 3; co++)
+      if (!mask[v * 3 + co]) S[(int64_t(o) * 9 + co * 3 + ci) * nv + v] = y[v * 3 + co];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_stencil_dinv(const double* S, Grid3 g, const uint8_t* mask,
+                                                           double* dinv)
+{
+  const int64_t nv = nverts(g);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int64_t v = t / 3;
+    const int     c = int(t % 3);
+    dinv[t]         = mask[t] ? 0.0 : 1.0 / S[(int64_t(13) * 9 + c * 3 + c) * nv + v];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_diag_inverse(const double* diag, int64_t n, double* dinv)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    dinv[i] = 1.0 / diag[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_set_rows0(double* x, const int32_t* rows, int64_t nr)
+{
+  for (int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x; k < nr; k += int64_t(gridDim.x) * kThreads)
+    x[rows[k]] = 0.0;
+}
+
+__global__ void __launch_bounds__(kThreads) k_rows_mask(const int32_t* rows, int64_t nr, uint8_t* mask)
+{
+  for (int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x; k < nr; k += int64_t(gridDim.x) * kThreads)
+    mask[rows[k]] = 1;
+}
+
+__global__ void __launch_bounds__(kThreads) k_vertex_mask(sxu::FEGrid G, const uint8_t* node_mask, uint8_t* vmask)
+{
+  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
+    const int64_t i    = t / 3;
+    const int     ix   = int(i % G.nv[0]), iy = int((i / G.nv[0]) % G.nv[1]), iz = int(i / (int64_t(G.nv[0]) * G.nv[1]));
+    const int64_t node = (int64_t(G.p) * iz * G.nn[1] + int64_t(G.p) * iy) * G.nn[0] + int64_t(G.p) * ix;
+    vmask[t]           = node_mask[node * 3 + t % 3];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_coarsen_mask(const uint8_t* mf, Grid3 f, uint8_t* mc, Grid3 c)
+{
+  const int64_t nc = nverts(c);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nc * 3; t += int64_t(gridDim.x) * kThreads) {
+    int ix, iy, iz;
+    coords(c, t / 3, ix, iy, iz);
+    const int64_t j = (int64_t(2 * iz) * f.n[1] + 2 * iy) * f.n[0] + 2 * ix;
+    mc[t]           = mf[j * 3 + t % 3];
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_cheb_init(const double* b, const double* dinv, double inv_theta,
+                                                        double* x, double* d, double* r, int x_add, int copy_r,
+                                                        int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const double bi = b[i];
+    const double di = dinv[i] * bi * inv_theta;
+    d[i]            = di;
+    x[i]            = x_add ? x[i] + di : di;
+    if (copy_r) r[i] = bi;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_cheb_step(double* r, const double* t, double* d, const double* dinv,
+                                                        double c1, double c2, double* x, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    const double ri = r[i] - t[i];
+    r[i]            = ri;
+    const double di = c1 * d[i] + c2 * dinv[i] * ri;
+    d[i]            = di;
+    x[i] += di;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_residual(const double* b, const double* t, double* r, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    r[i] = b[i] - t[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_sub_inplace(double* r, const double* t, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    r[i] -= t[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_apply_mask(double* x, const uint8_t* mask, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    if (mask[i]) x[i] = 0.0;
+}
+
+// one wave per row
+__global__ void __launch_bounds__(kThreads) k_dense_mv(const double* A, int n, const double* b, double* x)
+{
+  const int row  = int(blockIdx.x) * (kThreads / 64) + int(threadIdx.x >> 6);
+  const int lane = int(threadIdx.x & 63);
+  if (row >= n) return;  // whole waves leave; no barrier follows
+  double s = 0.0;
+  for (int k = lane; k < n; k += 64) s += A[int64_t(row) * n + k] * b[k];
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (lane == 0) x[row] = s;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) s = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  return s;
+}
+
+__global__ void __launch_bounds__(kThreads) k_pcg_step_xr(double* x, double* r, double* Ad, const double* d,
+                                                          double* scal, const double* part_den, int nb, int64_t n,
+                                                          const double* stop)
+{
+  __shared__ double sh[4];
+  __shared__ double bc;
+  if (stop[0] != 0.0) return;  // the whole block leaves before any barrier
+  double s = 0.0;
+  for (int i = threadIdx.x; i < nb; i += kThreads) s += part_den[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) bc = s;
+  __syncthreads();
+  const double den = bc, nom = scal[0], alpha = nom / den;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    x[i] += alpha * d[i];
+    r[i] -= alpha * Ad[i];
+    Ad[i] = 0.0;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[1] = den;
+    scal[2] = alpha;
+    scal[5] = nom;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_mul_dinv(const double* dinv, const double* t, double* v, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    v[i] = dinv[i] * t[i];
+}
+
+__global__ void __launch_bounds__(kThreads) k_scale(double* v, double a, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    v[i] *= a;
+}
+
+// a fixed pseudo-random field in [-1, 1) (splitmix64 of the index and seed)
+__global__ void __launch_bounds__(kThreads) k_seed_field(double* v, int64_t n, uint64_t seed, const uint8_t* mask)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    uint64_t z = uint64_t(i) + seed * 0x9E3779B97F4A7C15ull + 0x9E3779B97F4A7C15ull;
+    z          = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z          = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z          = z ^ (z >> 31);
+    const double u = double(z >> 11) * (1.0 / 9007199254740992.0);
+    v[i]           = (mask && mask[i]) ? 0.0 : 2.0 * u - 1.0;
+  }
+}
+
+}  // namespace
+
+hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const uint8_t* mask, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
+  hipLaunchKernelGGL(k_fe_restrict, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, r, rc, mask);
+  return hipGetLastError();
+}
+
+hipError_t fe_prolong_add(const sxu::FEGrid& G, const double* ec, double* z, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nn[0]) * G.nn[1] * G.nn[2] * 3;
+  hipLaunchKernelGGL(k_fe_prolong_add, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, ec, z);
+  return hipGetLastError();
+}
+
+hipError_t grid_restrict(const double* rf, Grid3 f, double* rc, Grid3 c, const uint8_t* maskc, hipStream_t s)
+{
+  const int64_t n = int64_t(c.n[0]) * c.n[1] * c.n[2] * 3;
+  hipLaunchKernelGGL(k_grid_restrict, dim3(blocks_for(n)), dim3(kThreads), 0, s, rf, f, rc, c, maskc);
+  return hipGetLastError();
+}
+
+hipError_t grid_prolong_add(const double* ec, Grid3 c, double* zf, Grid3 f, const uint8_t* maskf, hipStream_t s)
+{
+  const int64_t n = int64_t(f.n[0]) * f.n[1] * f.n[2] * 3;
+  hipLaunchKernelGGL(k_grid_prolong_add, dim3(blocks_for(n)), dim3(kThreads), 0, s, ec, c, zf, f, maskf);
+  return hipGetLastError();
+}
+
+hipError_t stencil_apply(const double* S, Grid3 g, const double* x, double* y, hipStream_t s)
+{
+  const int64_t n = int64_t(g.n[0]) * g.n[1] * g.n[2];
+  hipLaunchKernelGGL(k_stencil_apply, dim3(blocks_for(n)), dim3(kThreads), 0, s, S, g, x, y);
+  return hipGetLastError();
+}
+
+hipError_t probe_fill(double* e, Grid3 g, int cx, int cy, int cz, int ci, const uint8_t* mask, hipStream_t s)
+{
+  const int64_t n = int64_t(g.n[0]) * g.n[1] * g.n[2] * 3;
+  hipLaunchKernelGGL(k_probe_fill, dim3(blocks_for(n)), dim3(kThreads), 0, s, e, g, cx, cy, cz, ci, mask);
+  return hipGetLastError();
+}
+
+hipError_t probe_extract(const double* y, Grid3 g, int cx, int cy, int cz, int ci, const uint8_t* mask, double* S,
+                         hipStream_t s)
+{
+  const int64_t n = int64_t(g.n[0]) * g.n[1] * g.n[2];
+  hipLaunchKernelGGL(k_probe_extract, dim3(blocks_for(n)), dim3(kThreads), 0, s, y, g, cx, cy, cz, ci, mask, S);
+  return hipGetLastError();
+}
+
+hipError_t stencil_dinv(const double* S, Grid3 g, const uint8_t* mask, double* dinv, hipStream_t s)
+{
+  const int64_t n = int64_t(g.n[0]) * g.n[1] * g.n[2] * 3;
+  hipLaunchKernelGGL(k_stencil_dinv, dim3(blocks_for(n)), dim3(kThreads), 0, s, S, g, mask, dinv);
+  return hipGetLastError();
+}
+
+hipError_t diag_inverse(const double* diag, int64_t n, const int32_t* rows, int64_t nr, double* dinv, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_diag_inverse, dim3(blocks_for(n)), dim3(kThreads), 0, s, diag, n, dinv);
+  if (nr > 0) hipLaunchKernelGGL(k_set_rows0, dim3(blocks_for(nr)), dim3(kThreads), 0, s, dinv, rows, nr);
+  return hipGetLastError();
+}
+
+hipError_t vertex_mask(const sxu::FEGrid& G, const uint8_t* node_mask, uint8_t* vmask, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
+  hipLaunchKernelGGL(k_vertex_mask, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, node_mask, vmask);
+  return hipGetLastError();
+}
+
+hipError_t rows_mask(const int32_t* rows, int64_t nr, uint8_t* mask, hipStream_t s)
+{
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_mask, dim3(blocks_for(nr)), dim3(kThreads), 0, s, rows, nr, mask);
+  return hipGetLastError();
+}
+
+hipError_t coarsen_mask(const uint8_t* mf, Grid3 f, uint8_t* mc, Grid3 c, hipStream_t s)
+{
+  const int64_t n = int64_t(c.n[0]) * c.n[1] * c.n[2] * 3;
+  hipLaunchKernelGGL(k_coarsen_mask, dim3(blocks_for(n)), dim3(kThreads), 0, s, mf, f, mc, c);
+  return hipGetLastError();
+}
+
+hipError_t cheb_init(const double* b, const double* dinv, double theta, double* x, double* d, double* r, int x_add,
+                     int copy_r, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cheb_init, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, dinv, 1.0 / theta, x, d, r, x_add,
+                     copy_r, n);
+  return hipGetLastError();
+}
+
+hipError_t cheb_step(double* r, const double* t, double* d, const double* dinv, double c1, double c2, double* x,
+                     int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cheb_step, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, d, dinv, c1, c2, x, n);
+  return hipGetLastError();
+}
+
+hipError_t residual(const double* b, const double* t, double* r, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_residual, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, t, r, n);
+  return hipGetLastError();
+}
+
+hipError_t sub_inplace(double* r, const double* t, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub_inplace, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, n);
+  return hipGetLastError();
+}
+
+hipError_t apply_mask(double* x, const uint8_t* mask, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_apply_mask, dim3(blocks_for(n)), dim3(kThreads), 0, s, x, mask, n);
+  return hipGetLastError();
+}
+
+hipError_t dense_mv(const double* A, int n, const double* b, double* x, hipStream_t s)
+{
+  const unsigned rows_per_block = kThreads / 64;
+  hipLaunchKernelGGL(k_dense_mv, dim3((unsigned(n) + rows_per_block - 1) / rows_per_block), dim3(kThreads), 0, s, A,
+                     n, b, x);
+  return hipGetLastError();
+}
+
+hipError_t pcg_step_xr(double* x, double* r, double* Ad, const double* d, double* scal, const double* part_den,
+                       int nb, int64_t n, const double* stop, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_pcg_step_xr, dim3(unsigned(nb)), dim3(kThreads), 0, s, x, r, Ad, d, scal, part_den, nb, n,
+                     stop);
+  return hipGetLastError();
+}
+
+hipError_t mul_dinv(const double* dinv, const double* t, double* v, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_mul_dinv, dim3(blocks_for(n)), dim3(kThreads), 0, s, dinv, t, v, n);
+  return hipGetLastError();
+}
+
+hipError_t scale(double* v, double a, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_scale, dim3(blocks_for(n)), dim3(kThreads), 0, s, v, a, n);
+  return hipGetLastError();
+}
+
+hipError_t seed_field(double* v, int64_t n, uint64_t seed, const uint8_t* mask, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_seed_field, dim3(blocks_for(n)), dim3(kThreads), 0, s, v, n, seed, mask);
+  return hipGetLastError();
+}
+
+}  // namespace sxm

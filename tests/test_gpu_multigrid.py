@@ -1,0 +1,254 @@
+"""The geometric multigrid preconditioner (the GPU stand-in for the reference's
+Preconditioner::HypreAMG, num/equation_solver.cpp:1014-1021) against a numpy
+restatement built from the assembled Jacobian: the Galerkin level operators
+P^T A P that probing finds, the whole V-cycle z = M r (Chebyshev-Jacobi
+smoothing with the GPU's eigenvalue estimates, masked transfers, the dense
+coarsest solve), its symmetry and positivity, and multigrid-CG and the Newton
+loops against direct solves and Jacobi-CG."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import serac_amd as sx
+from serac_amd import workloads as W
+
+from test_gpu_gmres import eliminated
+from test_gpu_learned import interp_1d, setup
+
+pytestmark = pytest.mark.gpu
+
+SAFETY = 1.15
+
+
+def trilinear_1d(nc):
+    """Vertex level l + 1 -> l on one direction: (2 (nc - 1) + 1) x nc."""
+    P = np.zeros((2 * (nc - 1) + 1, nc))
+    for i in range(nc):
+        P[2 * i, i] = 1.0
+        if 2 * i - 1 >= 0:
+            P[2 * i - 1, i] = 0.5
+        if 2 * i + 1 < P.shape[0]:
+            P[2 * i + 1, i] = 0.5
+    return P
+
+
+def kron3(P1):
+    """The 3-component tensor-product transfer, DOF = node * 3 + c."""
+    P = sp.kron(P1, sp.kron(P1, P1)).tocsr()
+    return sp.kron(P, sp.identity(3)).tocsr()
+
+
+def hierarchy(n, p, grad, ess, info):
+    """The numpy restatement: level operators, masks and transfers."""
+    nd0 = 3 * (p * n + 1) ** 3
+    keep0 = np.ones(nd0)
+    keep0[ess] = 0.0
+    Z0 = sp.diags(keep0)
+    A0 = (Z0 @ grad.assemble().to_scipy() @ Z0).tocsr()
+    levels = [{"A": A0, "keep": keep0}]
+    # level 1: the vertex grid; a vertex component is constrained when the
+    # node at the vertex is
+    nv = n + 1
+    idx = (p * np.arange(nv))
+    node = (idx[:, None, None] * (p * n + 1) + idx[None, :, None]) * (p * n + 1) + idx[None, None, :]
+    keep1 = keep0.reshape(-1, 3)[node.ravel()].ravel()
+    Ps = [(Z0 @ kron3(sp.csr_matrix(interp_1d(n, p))) @ sp.diags(keep1)).tocsr()]
+    keeps = [keep1]
+    dims = [nv]
+    for lvl in info["levels"][2:]:
+        nc = lvl["dims"][0]
+        nf = dims[-1]
+        cidx = 2 * np.arange(nc)
+        vert = (cidx[:, None, None] * nf + cidx[None, :, None]) * nf + cidx[None, None, :]
+        kc = keeps[-1].reshape(-1, 3)[vert.ravel()].ravel()
+        Ps.append((sp.diags(keeps[-1]) @ kron3(sp.csr_matrix(trilinear_1d(nc))) @ sp.diags(kc)).tocsr())
+        keeps.append(kc)
+        dims.append(nc)
+    for P, k in zip(Ps, keeps):
+        A = (P.T @ levels[-1]["A"] @ P).tocsr()
+        levels.append({"A": A, "keep": k, "P": P})
+    return levels
+
+
+def chebyshev_coeffs(lmax, rng, deg):
+    up = SAFETY * lmax
+    lo = up / rng
+    th, de = 0.5 * (up + lo), 0.5 * (up - lo)
+    sigma, rho, c = th / de, de / th, []
+    for _ in range(2, deg + 1):
+        rn = 1.0 / (2.0 * sigma - rho)
+        c.append((rn * rho, 2.0 * rn / de))
+        rho = rn
+    return th, c
+
+
+def vcycle_numpy(levels, info, fine_deg, coarse_deg, rng, r, ess):
+    L = len(levels)
+    for lvl, gi in zip(levels, info["levels"]):
+        d = lvl["A"].diagonal()
+        lvl["dinv"] = np.where(lvl["keep"] > 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+        if lvl is not levels[-1]:
+            lvl["theta"], lvl["c"] = chebyshev_coeffs(gi["lmax"], rng, fine_deg if lvl is levels[0] else coarse_deg)
+    Ac = levels[-1]["A"].toarray()
+    Ac = 0.5 * (Ac + Ac.T) + np.diag(1.0 - levels[-1]["keep"])
+    Ainv = np.linalg.inv(Ac)
+
+    def smooth(l, b, x, pre):
+        V = levels[l]
+        A, dinv = V["A"], V["dinv"]
+        if pre:
+            d = dinv * b / V["theta"]
+            x, rr = d.copy(), b.copy()
+        else:
+            rr = b - A @ x
+            d = dinv * rr / V["theta"]
+            x = x + d
+        for c1, c2 in V["c"]:
+            rr = rr - A @ d
+            d = c1 * d + c2 * dinv * rr
+            x = x + d
+        if pre:
+            rr = rr - A @ d
+        return x, rr
+
+    def vcycle(l, b):
+        if l == L - 1:
+            return Ainv @ b
+        x, rr = smooth(l, b, None, True)
+        P = levels[l + 1]["P"]
+        xc = vcycle(l + 1, P.T @ rr)
+        x = x + P @ xc
+        x, _ = smooth(l, b, x, False)
+        return x
+
+    b = r.copy()
+    b[ess] = 0.0
+    z = vcycle(0, b)
+    z[ess] = r[ess]
+    return z
+
+
+def linearized(n, p, amp=0.0002):
+    m, s, f, ess = setup(n, p)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -amp, amp, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    return m, s, f, ess, grad
+
+
+@pytest.mark.parametrize("n,p", [(4, 2), (8, 1), (4, 3)])
+def test_levels_are_galerkin_products_and_the_vcycle_matches_numpy(gpu_lib, n, p):
+    m, s, f, ess, grad = linearized(n, p)
+    mg = sx.Multigrid(s, fine_degree=3, coarse_degree=2, smoothing_range=10.0)
+    mg.setup(grad, ess)
+    info = mg.info()
+    assert info["levels"][0]["dims"] == (p * n + 1,) * 3
+    assert info["levels"][1]["dims"] == (n + 1,) * 3
+    assert info["levels"][-1]["ndof"] <= 200 or (info["levels"][-1]["dims"][0] - 1) % 2 == 1
+    levels = hierarchy(n, p, grad, ess, info)
+    assert [lv["A"].shape[0] for lv in levels] == [lv["ndof"] for lv in info["levels"]]
+    rng = np.random.default_rng(5)
+    for lvl, lv in enumerate(levels):
+        x = rng.standard_normal(lv["A"].shape[0]) * lv["keep"]
+        y = sx.DeviceVector(len(x))
+        mg.level_apply(lvl, sx.DeviceVector.from_numpy(x), y)
+        ref = lv["A"] @ x
+        assert np.abs(y.numpy() - ref).max() <= 1e-12 * np.abs(ref).max(), lvl
+    # the power-iteration estimates bracket the largest eigenvalue of D^-1 A
+    for lv, gi in zip(levels[:-1], info["levels"][:-1]):
+        A = lv["A"].toarray()
+        keep = lv["keep"] > 0
+        Af = A[np.ix_(keep, keep)]
+        d = np.diag(Af)
+        lam = np.linalg.eigvalsh(Af / np.sqrt(np.outer(d, d))).max()
+        assert 0.6 * lam <= gi["lmax"] <= 1.25 * lam, (gi["lmax"], lam)
+    r = W.nodal_field(s.nnodes, 3, 7, 3000, -1.0, 1.0, 1)
+    z = sx.DeviceVector(s.ndof)
+    mg.apply(sx.DeviceVector.from_numpy(r), z)
+    zref = vcycle_numpy(levels, info, 3, 2, 10.0, r, ess)
+    assert np.abs(z.numpy() - zref).max() <= 1e-10 * np.abs(zref).max()
+
+
+def test_vcycle_is_symmetric_and_positive(gpu_lib):
+    m, s, f, ess, grad = linearized(8, 2)
+    mg = sx.Multigrid(s)
+    mg.setup(grad, ess)
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    Mu, Mv = sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof)
+    mg.apply(sx.DeviceVector.from_numpy(u), Mu)
+    mg.apply(sx.DeviceVector.from_numpy(v), Mv)
+    Mu, Mv = Mu.numpy(), Mv.numpy()
+    assert abs(v @ Mu - u @ Mv) <= 1e-11 * abs(v @ Mu)
+    assert u @ Mu > 0 and v @ Mv > 0
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_multigrid_cg_matches_a_direct_solve_in_far_fewer_iterations(gpu_lib, p):
+    n = 8
+    m, s, f, ess, grad = linearized(n, p)
+    b = W.nodal_field(s.nnodes, 3, 8, 3000, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    A = eliminated(grad.assemble().to_scipy(), ess)
+    x_ref = spla.spsolve(A.tocsc(), b)
+    mg = sx.Multigrid(s)
+    x = sx.DeviceVector(s.ndof)
+    rep = grad.mg_cg(mg, ess, sx.DeviceVector.from_numpy(b), x, rel_tol=1e-10, max_iter=2000)
+    assert rep["converged"]
+    assert np.abs(x.numpy() - x_ref).max() <= 1e-7 * np.abs(x_ref).max()
+    xj = sx.DeviceVector(s.ndof)
+    repj = grad.cg(ess, sx.DeviceVector.from_numpy(b), xj, rel_tol=1e-10, max_iter=50000)
+    assert rep["linear_iterations"] * 5 < repj["linear_iterations"], (rep, repj)
+    assert mg.info()["setups"] == 1
+
+
+@pytest.mark.parametrize("arith", ["f64", "mixed_f32"])
+def test_newton_with_multigrid_matches_jacobi_newton(gpu_lib, arith):
+    n, p = 8, 2
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, 0.2)
+    s = sx.Space(m, p, 3, 1)
+    ess0 = s.boundary_dofs(0b000001, 0x7)
+    ess1 = s.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(ess, ess1), -0.05, 0.0)
+
+    def solve(mg):
+        f = sx.Functional(s, [s])
+        f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+        if arith == "mixed_f32" and mg is not None:
+            f.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, sx.capi.SX_PREC_F32)
+            f.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, sx.capi.SX_PREC_F32)
+        f.set_multigrid(mg)
+        x = sx.DeviceVector(s.ndof)
+        reps = sx.quasistatic_solve(f, x, ess, final, 2, rel_tol=1e-10, lin_rel_tol=1e-8, lin_max_iter=50000)
+        return x.numpy(), reps
+
+    x_j, reps_j = solve(None)
+    mg = sx.Multigrid(s)
+    x_m, reps_m = solve(mg)
+    assert np.abs(x_m - x_j).max() <= 1e-7 * np.abs(x_j).max()
+    assert all(r["converged"] for r in reps_m)
+    assert sum(r["linear_iterations"] for r in reps_m) * 5 < sum(r["linear_iterations"] for r in reps_j)
+    # one setup per linearization: the warm-start solve of each load step and each Newton step
+    assert mg.info()["setups"] == sum(r["newton_iterations"] + 1 for r in reps_m)
+
+
+def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
+    s1 = sx.Space(sx.Mesh.cuboid(4, 4, 4), 2, 1, 1)
+    with pytest.raises(sx.SeracAmdError):
+        sx.Multigrid(s1)  # one component
+    with pytest.raises(sx.SeracAmdError):
+        X, hexes = patch_mesh
+        sx.Multigrid(sx.Space(sx.Mesh.from_arrays(X, hexes), 2, 3, 1))  # not a cuboid
+    with pytest.raises(sx.SeracAmdError):
+        sx.Multigrid(sx.Space(sx.Mesh.cuboid(17, 17, 17), 1, 3, 1))  # 18^3 x 3 coarsest DOFs
+    s = sx.Space(sx.Mesh.cuboid(4, 4, 4), 2, 3, 1)
+    mg = sx.Multigrid(s)
+    with pytest.raises(sx.SeracAmdError):
+        mg.configure(fine_degree=0)
+    with pytest.raises(sx.SeracAmdError):
+        mg.apply(sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof))  # before a setup
+    f = sx.Functional(s1, [s1])
+    with pytest.raises(sx.SeracAmdError):
+        f.set_multigrid(mg)  # another space
