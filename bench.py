@@ -219,6 +219,7 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
             # the multigrid V-cycle (set up once per linearization, inside the timed solve)
             mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
                               smoothing_range=args.mg_range)
+            mg.set_reuse(args.mg_reuse)
             form.set_multigrid(mg)
         x = sx.DeviceVector(space.ndof)
         sx.capi.call("sx_device_synchronize")
@@ -256,7 +257,7 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
             solve[mode]["precond"] = "geometric multigrid V-cycle (Chebyshev-Jacobi, Galerkin vertex-grid levels)"
             solve[mode]["multigrid"] = {
                 "fine_degree": args.mg_fine_degree, "coarse_degree": args.mg_coarse_degree,
-                "smoothing_range": args.mg_range, "setups": info["setups"],
+                "smoothing_range": args.mg_range, "setups": info["setups"], "reuse": args.mg_reuse,
                 "levels": [lv["dims"] for lv in info["levels"]],
             }
             form.set_multigrid(None)
@@ -308,7 +309,7 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip the FP32 / BF16 J.v measurements")
     ap.add_argument("--solve", action="store_true",
                     help="time the load-stepped Newton + Krylov solve on the workload's mesh (without it, a 24^3 "
-                         "solve of the same load path runs in FP64 and mixed precision)")
+                         "solve of the same load path runs in FP64 and mixed precision, with Jacobi and multigrid)")
     ap.add_argument("--no-default-solve", action="store_true", help="skip the 24^3 solve")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--strain", type=float, default=0.05)
@@ -319,10 +320,13 @@ def main():
                     help="comma-separated solve modes: f64 (plain FP64 Jacobi-CG), mixed_f32 / mixed_bf16 "
                          "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov), "
                          "mixed_f32_learned (mixed_f32 with flexible CG and the learned U-Net preconditioner, "
-                         "trained first on pairs from the undeformed linearization; training time included)")
+                         "trained first on pairs from the undeformed linearization; training time included), "
+                         "f64_mg / mixed_f32_mg (f64 / mixed_f32 with multigrid-CG)")
     ap.add_argument("--mg-fine-degree", type=int, default=2, help="Chebyshev degree on the order-p level")
     ap.add_argument("--mg-coarse-degree", type=int, default=2, help="Chebyshev degree on the vertex levels")
     ap.add_argument("--mg-range", type=float, default=12.0, help="Chebyshev smoothing range")
+    ap.add_argument("--mg-reuse", type=int, default=1,
+                    help="linear solves one multigrid hierarchy serves inside a load step (1: set up per linearization)")
     ap.add_argument("--learn-steps", type=int, default=2000, help="Adam steps of the learned preconditioner")
     ap.add_argument("--learn-lr", type=float, default=1e-3)
     ap.add_argument("--learn-batch", type=int, default=8, help="pairs per Adam step")
@@ -482,14 +486,15 @@ def main():
                           f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
         form = make_form(args.scatter)
     elif qf == "neohookean" and not args.no_default_solve:
-        # the same load path on a 24^3 mesh, FP64 against mixed precision: the
-        # solve on the full mesh takes minutes (run it with --solve)
+        # the same load path on a 24^3 mesh, FP64 Jacobi-CG against mixed
+        # precision and multigrid-CG: the FP64 Jacobi-CG solve on the full mesh
+        # takes minutes (run it with --solve)
         ns = 24
         smesh = sx.Mesh.cuboid(ns, ns, ns)
         if perturb > 0:
             smesh.perturb(42, perturb)
         sspace = sx.Space(smesh, p, 3)
-        solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32",
+        solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32,f64_mg,mixed_f32_mg",
                           f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs; the full-size solve "
                           f"is bench.py --solve)")
         del sspace, smesh

@@ -404,6 +404,12 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
 typedef struct sx_multigrid sx_multigrid;
 int sx_mg_create(const sx_space* s, sx_multigrid** out);
 int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double smoothing_range, int power_iterations);
+/* inside one sx_newton_solve / sx_quasistatic_solve call, one hierarchy
+ * serves up to `linearizations` consecutive linear solves (the smoother's
+ * level-0 operator is always the current J; the coarse levels, diagonals and
+ * eigenvalue bounds lag).  Default 1: a setup per linearization, as the
+ * reference's preconditioners are set up on each SetOperator. */
+int sx_mg_set_reuse(sx_multigrid* M, int linearizations);
 /* number of levels and of setups so far */
 int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups);
 /* level l: DOFs, grid points per direction (dims[3]) and the eigenvalue estimate of the last setup */

@@ -416,6 +416,10 @@ class Multigrid:
     def configure(self, fine_degree=2, coarse_degree=2, smoothing_range=12.0, power_iterations=12):
         capi.call("sx_mg_configure", self.h, fine_degree, coarse_degree, smoothing_range, power_iterations)
 
+    def set_reuse(self, linearizations):
+        """Inside one Newton call, one hierarchy serves up to this many linear solves (default 1)."""
+        capi.call("sx_mg_set_reuse", self.h, linearizations)
+
     def info(self):
         nl, ns = ctypes.c_int(), ctypes.c_int64()
         capi.call("sx_mg_info", self.h, ctypes.byref(nl), ctypes.byref(ns))

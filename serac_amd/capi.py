@@ -132,6 +132,7 @@ SIGNATURES = {
     "sx_learned_destroy": (_int, [_vp]),
     "sx_mg_create": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_mg_configure": (_int, [_vp, _int, _int, ctypes.c_double, _int]),
+    "sx_mg_set_reuse": (_int, [_vp, _int]),
     "sx_mg_info": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_int64)]),
     "sx_mg_level": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_int),
                            ctypes.POINTER(ctypes.c_double)]),

@@ -269,6 +269,8 @@ struct sx_multigrid {
   int                  fine_degree = 2, coarse_degree = 2, power_iters = 12;
   double               range = 12.0, safety = 1.15;
   int64_t              setups = 0;
+  int                  reuse  = 1;  // linearizations one hierarchy serves inside a Newton call
+  int                  uses   = INT32_MAX;  // linear solves since the last setup (INT32_MAX: set up at the next)
 };
 
 struct sx_form {
@@ -1914,13 +1916,19 @@ void mg_apply(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, con
 // recurrences and stopping rule (r, M r) <= max(rel^2 (r0, M r0), abs^2),
 // num/equation_solver.cpp:916-919, with multigrid in place of the Jacobi
 // smoother), the convergence test on the device as in cg
+// With allow_reuse (the Newton loops, whose essential DOFs stay fixed for the
+// whole call) a hierarchy serves up to M->reuse consecutive linearizations.
 void mg_pcg(sx_grad* g, sx_multigrid* M, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
-            double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
+            double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep, bool allow_reuse = false)
 {
   sx_form*      f = g->form;
   const int64_t n = f->test->ndof();
   hipStream_t   s = f->stream;
-  mg_setup(M, g, ess, ness);
+  if (!allow_reuse || M->uses >= M->reuse) {
+    mg_setup(M, g, ess, ness);
+    M->uses = 0;
+  }
+  M->uses++;
   w.r.allocate(size_t(n) * 8);
   w.z.allocate(size_t(n) * 8);
   w.d.allocate(size_t(n) * 8);
@@ -2471,7 +2479,7 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
   } else if (f->linsolver == SX_LINSOLVER_GMRES) {
     gmres(g, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
   } else if (f->mg) {
-    mg_pcg(g, f->mg, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
+    mg_pcg(g, f->mg, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, true);
   } else {
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
   }
@@ -2574,6 +2582,7 @@ int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, 
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(f && x, SX_ERR_ARG, "null argument");
+    if (f->mg) f->mg->uses = INT32_MAX;  // a fresh hierarchy for this call's essential DOFs
     SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     const int64_t n = f->test->ndof();
@@ -2597,6 +2606,7 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
 {
   return api([&] {
     SX_REQUIRE(f && x && opts && nsteps >= 1, SX_ERR_ARG, "bad argument");
+    if (f->mg) f->mg->uses = INT32_MAX;  // a fresh hierarchy for this call's essential DOFs
     SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     SX_REQUIRE(ness == 0 || (ess_dofs && ess_final), SX_ERR_ARG, "null essential DOF data");
     const int64_t n = f->test->ndof();
@@ -2744,6 +2754,15 @@ int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double 
     M->coarse_degree = coarse_degree;
     M->range         = smoothing_range;
     M->power_iters   = power_iterations;
+  });
+}
+
+int sx_mg_set_reuse(sx_multigrid* M, int linearizations)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(linearizations >= 1, SX_ERR_ARG, "a hierarchy serves at least one linearization");
+    M->reuse = linearizations;
   });
 }
 

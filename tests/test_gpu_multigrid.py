@@ -234,6 +234,34 @@ def test_newton_with_multigrid_matches_jacobi_newton(gpu_lib, arith):
     assert mg.info()["setups"] == sum(r["newton_iterations"] + 1 for r in reps_m)
 
 
+def test_a_hierarchy_reused_across_a_load_step_reaches_the_same_solution(gpu_lib):
+    n, p = 8, 2
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, 0.2)
+    s = sx.Space(m, p, 3, 1)
+    ess0 = s.boundary_dofs(0b000001, 0x7)
+    ess1 = s.boundary_dofs(0b000010, 0x1)
+    ess = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(ess, ess1), -0.05, 0.0)
+    xs, setups = [], []
+    for reuse in (1, 3):
+        f = sx.Functional(s, [s])
+        f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+        mg = sx.Multigrid(s)
+        mg.set_reuse(reuse)
+        f.set_multigrid(mg)
+        x = sx.DeviceVector(s.ndof)
+        reps = sx.quasistatic_solve(f, x, ess, final, 2, rel_tol=1e-10, lin_rel_tol=1e-8)
+        xs.append(x.numpy())
+        nsolves = sum(r["newton_iterations"] + 1 for r in reps)
+        setups.append((mg.info()["setups"], nsolves))
+    assert setups[0][0] == setups[0][1]
+    assert setups[1][0] == -(-setups[1][1] // 3)  # one hierarchy per three linear solves
+    assert np.abs(xs[1] - xs[0]).max() <= 1e-7 * np.abs(xs[0]).max()
+    with pytest.raises(sx.SeracAmdError):
+        mg.set_reuse(0)
+
+
 def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
     s1 = sx.Space(sx.Mesh.cuboid(4, 4, 4), 2, 1, 1)
     with pytest.raises(sx.SeracAmdError):
