@@ -262,6 +262,7 @@ struct sx_multigrid {
   sxu::FEGrid          G{};
   std::vector<MgLevel> lv;
   DeviceBuffer         coarse_inv, part, scal, diag;
+  DeviceBuffer         rt1, rt2;        // scratch of the separable level-0 restriction
   DeviceBuffer         ess;             // essential DOFs of the last sx_mg_setup
   int64_t              ness   = 0;
   sx_grad*             set_g  = nullptr;  // the gradient of the last sx_mg_setup
@@ -1806,7 +1807,8 @@ void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
             }
             mg_level_apply(M, g, ess, ness, l, F.d.as<double>(), F.t.as<double>());
             if (l == 0)
-              SX_HIP(sxm::fe_restrict(M->G, F.t.as<double>(), C.r.as<double>(), C.mask.as<uint8_t>(), s));
+              SX_HIP(sxm::fe_restrict(M->G, F.t.as<double>(), C.r.as<double>(), C.mask.as<uint8_t>(), M->rt1.as<double>(),
+                                     M->rt2.as<double>(), s));
             else
               SX_HIP(sxm::grid_restrict(F.t.as<double>(), F.g, C.r.as<double>(), C.g, C.mask.as<uint8_t>(), s));
             SX_HIP(sxm::probe_extract(C.r.as<double>(), C.g, cx, cy, cz, ci, C.mask.as<uint8_t>(), C.S.as<double>(),
@@ -1888,7 +1890,8 @@ void mg_vcycle(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, in
   MgLevel& C = M->lv[size_t(l + 1)];
   mg_smooth(M, g, ess, ness, l, b, x, true);
   if (l == 0)
-    SX_HIP(sxm::fe_restrict(M->G, V.r.as<double>(), C.b.as<double>(), C.mask.as<uint8_t>(), s));
+    SX_HIP(sxm::fe_restrict(M->G, V.r.as<double>(), C.b.as<double>(), C.mask.as<uint8_t>(), M->rt1.as<double>(),
+                             M->rt2.as<double>(), s));
   else
     SX_HIP(sxm::grid_restrict(V.r.as<double>(), V.g, C.b.as<double>(), C.g, C.mask.as<uint8_t>(), s));
   mg_vcycle(M, g, ess, ness, l + 1, C.b.as<double>(), C.x.as<double>());
@@ -2736,6 +2739,8 @@ int sx_mg_create(const sx_space* sp, sx_multigrid** out)
       }
     }
     M->diag.allocate(size_t(M->lv[0].n) * 8);
+    M->rt1.allocate(size_t(sxm::fe_restrict_scratch(M->G, 0)) * 8);
+    M->rt2.allocate(size_t(sxm::fe_restrict_scratch(M->G, 1)) * 8);
     M->part.allocate(size_t(sxk::reduction_blocks()) * 8);
     M->scal.allocate(16 * 8);
     *out = M.release();

@@ -32,35 +32,33 @@ __device__ __forceinline__ double fe_weight(const sxu::FEGrid& G, int d)
   return d >= 0 ? 1.0 - G.xi[d] : G.xi[G.p + d];
 }
 
-__global__ void __launch_bounds__(kThreads) k_fe_restrict(sxu::FEGrid G, const double* r, double* rc,
-                                                          const uint8_t* mask)
+// One direction of the separable restriction P^T = Rz Ry Rx: out has the
+// dims of in except nv[dir] along dir, out(i) = sum_d w(d) in(p i + d).  The
+// last pass (dir 2) applies the vertex mask.
+__global__ void __launch_bounds__(kThreads) k_fe_restrict_dir(sxu::FEGrid G, const double* in, Grid3 din, double* out,
+                                                              Grid3 dout, int dir, const uint8_t* mask)
 {
-  const int64_t nv = int64_t(G.nv[0]) * G.nv[1] * G.nv[2];
-  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 3; t += int64_t(gridDim.x) * kThreads) {
-    if (mask[t]) {
-      rc[t] = 0.0;
+  const int64_t n      = int64_t(dout.n[0]) * dout.n[1] * dout.n[2] * 3;
+  const int64_t stride = dir == 0 ? 3 : (dir == 1 ? int64_t(din.n[0]) * 3 : int64_t(din.n[0]) * din.n[1] * 3);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    if (mask && mask[t]) {
+      out[t] = 0.0;
       continue;
     }
-    const int     c = int(t % 3);
-    const int64_t i = t / 3;
-    const int     iv[3] = {int(i % G.nv[0]), int((i / G.nv[0]) % G.nv[1]), int(i / (int64_t(G.nv[0]) * G.nv[1]))};
-    double        s = 0.0;
-    for (int dz = -G.p; dz <= G.p; dz++) {
-      const int jz = G.p * iv[2] + dz;
-      if (jz < 0 || jz >= G.nn[2]) continue;
-      const double wz = fe_weight(G, dz);
-      for (int dy = -G.p; dy <= G.p; dy++) {
-        const int jy = G.p * iv[1] + dy;
-        if (jy < 0 || jy >= G.nn[1]) continue;
-        const double wzy = wz * fe_weight(G, dy);
-        for (int dx = -G.p; dx <= G.p; dx++) {
-          const int jx = G.p * iv[0] + dx;
-          if (jx < 0 || jx >= G.nn[0]) continue;
-          s += wzy * fe_weight(G, dx) * r[((int64_t(jz) * G.nn[1] + jy) * G.nn[0] + jx) * 3 + c];
-        }
-      }
+    const int c = int(t % 3);
+    int       o[3];
+    coords(dout, t / 3, o[0], o[1], o[2]);
+    const int i = o[dir];
+    o[dir]      = G.p * i;
+    // the input entry at the vertex's own node, then offsets along dir
+    const double* base = in + ((int64_t(o[2]) * din.n[1] + o[1]) * din.n[0] + o[0]) * 3 + c;
+    double        s    = 0.0;
+    for (int d = -G.p; d <= G.p; d++) {
+      const int j = G.p * i + d;
+      if (j < 0 || j >= din.n[dir]) continue;
+      s += fe_weight(G, d) * base[int64_t(d) * stride];
     }
-    rc[t] = s;
+    out[t] = s;
   }
 }
 
@@ -386,10 +384,23 @@ __global__ void __launch_bounds__(kThreads) k_seed_field(double* v, int64_t n, u
 
 }  // namespace
 
-hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const uint8_t* mask, hipStream_t s)
+int64_t fe_restrict_scratch(const sxu::FEGrid& G, int pass)
 {
-  const int64_t n = int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3;
-  hipLaunchKernelGGL(k_fe_restrict, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, r, rc, mask);
+  return pass == 0 ? int64_t(G.nn[2]) * G.nn[1] * G.nv[0] * 3 : int64_t(G.nn[2]) * G.nv[1] * G.nv[0] * 3;
+}
+
+hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const uint8_t* mask, double* t1, double* t2,
+                       hipStream_t s)
+{
+  const Grid3 d0{{G.nn[0], G.nn[1], G.nn[2]}}, d1{{G.nv[0], G.nn[1], G.nn[2]}}, d2{{G.nv[0], G.nv[1], G.nn[2]}},
+      d3{{G.nv[0], G.nv[1], G.nv[2]}};
+  const uint8_t* none = nullptr;
+  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(fe_restrict_scratch(G, 0))), dim3(kThreads), 0, s, G, r, d0,
+                     t1, d1, 0, none);
+  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(fe_restrict_scratch(G, 1))), dim3(kThreads), 0, s, G, t1, d1,
+                     t2, d2, 1, none);
+  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3)), dim3(kThreads), 0,
+                     s, G, t2, d2, rc, d3, 2, mask);
   return hipGetLastError();
 }
 

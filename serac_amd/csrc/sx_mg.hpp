@@ -23,9 +23,13 @@ struct Grid3 {
   int n[3];  // vertices per direction
 };
 
-// level 0 <-> 1: rc = P^T r on the vertex grid (zero where mask is set);
-// z += P ec on the nodal grid (P: the order-p interpolation of the vertex field)
-hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const uint8_t* mask, hipStream_t s);
+// level 0 <-> 1: rc = P^T r on the vertex grid (zero where mask is set), in
+// three passes, one direction each, through scratch t1 and t2 of
+// fe_restrict_scratch(G, 0) and (G, 1) doubles; z += P ec on the nodal grid
+// (P: the order-p interpolation of the vertex field)
+int64_t    fe_restrict_scratch(const sxu::FEGrid& G, int pass);
+hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const uint8_t* mask, double* t1, double* t2,
+                       hipStream_t s);
 hipError_t fe_prolong_add(const sxu::FEGrid& G, const double* ec, double* z, hipStream_t s);
 // vertex level l <-> l + 1 (trilinear): rc = P^T rf (zero where maskc is set);
 // zf += P ec (skipping entries where maskf is set)
