@@ -187,7 +187,8 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
              "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
              "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
              "f64_mg": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
-             "mixed_f32_mg": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
+             "mixed_f32_mg": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
+             "mixed_bf16_mg": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32)}
     solve, finals = {}, {}
     for mode in precisions.split(","):
         tprec, arith = modes[mode]
@@ -321,7 +322,7 @@ def main():
                          "(FP32 element arithmetic with an FP32 / BF16 tangent inside FP64 Newton and Krylov), "
                          "mixed_f32_learned (mixed_f32 with flexible CG and the learned U-Net preconditioner, "
                          "trained first on pairs from the undeformed linearization; training time included), "
-                         "f64_mg / mixed_f32_mg (f64 / mixed_f32 with multigrid-CG)")
+                         "f64_mg / mixed_f32_mg / mixed_bf16_mg (f64 / mixed_f32 / mixed_bf16 with multigrid-CG)")
     ap.add_argument("--mg-fine-degree", type=int, default=2, help="Chebyshev degree on the order-p level")
     ap.add_argument("--mg-coarse-degree", type=int, default=2, help="Chebyshev degree on the vertex levels")
     ap.add_argument("--mg-range", type=float, default=12.0, help="Chebyshev smoothing range")

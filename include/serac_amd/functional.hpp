@@ -674,6 +674,7 @@ class Functional<test(trials...)> {
     std::array<const sx_space*, num_trial_spaces> t{};
     for (int i = 0; i < num_trial_spaces; i++) t[size_t(i)] = trial_fes[size_t(i)]->handle();
     detail::check(sx_form_create(test_fes->handle(), t.data(), num_trial_spaces, &h_));
+    test_      = test_fes->handle();
     test_size_ = test_fes->GetTrueVSize();
     output_.SetSize(test_size_);
   }
@@ -795,10 +796,12 @@ class Functional<test(trials...)> {
     }
   }
 
-  sx_form* handle() const { return h_; }
+  sx_form*  handle() const { return h_; }
+  sx_space* test_space() const { return test_; }
 
  private:
   sx_form*                           h_         = nullptr;
+  sx_space*                          test_      = nullptr;
   int64_t                            test_size_ = 0;
   Vector                             output_;
   Gradient                           grad_;
@@ -949,12 +952,15 @@ class Functional<double(trial)> {
 };
 
 // ---------------------------------------------------------------------------
-// EquationSolver: Newton + Jacobi-CG / Jacobi-GMRES with the reference's option names
-// (num/solver_config.hpp:342-400, num/equation_solver.cpp:81-235)
+// EquationSolver: Newton + Jacobi-CG / Jacobi-GMRES / multigrid-CG with the
+// reference's option names (num/solver_config.hpp:342-400,
+// num/equation_solver.cpp:81-235).  Preconditioner::HypreAMG selects the
+// geometric multigrid V-cycle (3-component spaces on cuboid meshes), the
+// stand-in for BoomerAMG (num/equation_solver.cpp:1014-1021) with CG.
 // ---------------------------------------------------------------------------
 enum class NonlinearSolver { Newton };
 enum class LinearSolver { CG, GMRES };
-enum class Preconditioner { HypreJacobi };
+enum class Preconditioner { HypreJacobi, HypreAMG };
 
 struct LinearSolverOptions {
   LinearSolver   linear_solver  = LinearSolver::CG;
@@ -983,9 +989,20 @@ class EquationSolver {
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINEAR_SOLVER,
                                      lin_.linear_solver == LinearSolver::GMRES ? SX_LINSOLVER_GMRES : SX_LINSOLVER_CG));
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_GMRES_RESTART, lin_.gmres_restart));
+    const bool amg = lin_.preconditioner == Preconditioner::HypreAMG;
+    if (amg) {
+      if (!mg_ || mg_space_ != residual.test_space()) {
+        sx_multigrid* m = nullptr;
+        detail::check(sx_mg_create(residual.test_space(), &m));
+        mg_.reset(m);
+        mg_space_ = residual.test_space();
+      }
+      detail::check(sx_form_set_multigrid(residual.handle(), mg_.get()));
+    }
     int rc = sx_newton_solve(residual.handle(), wrt, essential_dofs.data(), int64_t(essential_dofs.size()), x.Data(),
                              nl_.relative_tol, nl_.absolute_tol, nl_.max_iterations, lin_.relative_tol,
                              lin_.absolute_tol, lin_.max_iterations, &rep);
+    if (amg) detail::check(sx_form_set_multigrid(residual.handle(), nullptr));
     last_ = rep;
     detail::check(rc);
     return rep;
@@ -993,9 +1010,14 @@ class EquationSolver {
   const sx_solve_report& report() const { return last_; }
 
  private:
-  NonlinearSolverOptions nl_;
-  LinearSolverOptions    lin_;
-  sx_solve_report        last_{};
+  struct MgDelete {
+    void operator()(sx_multigrid* m) const { sx_mg_destroy(m); }
+  };
+  NonlinearSolverOptions                      nl_;
+  LinearSolverOptions                         lin_;
+  sx_solve_report                             last_{};
+  std::unique_ptr<sx_multigrid, MgDelete>     mg_;
+  sx_space*                                   mg_space_ = nullptr;
 };
 
 }  // namespace serac_amd

@@ -1706,7 +1706,7 @@ double mg_read_dot(sx_multigrid* M, const double* a, const double* b, int64_t n,
 }
 
 // dense inverse of the coarsest operator: symmetrised, constrained rows set to
-// identity, Cholesky, then L^-T L^-1
+// identity, inverted by Gauss-Jordan elimination with partial pivoting
 void mg_coarse_inverse(sx_multigrid* M, hipStream_t s)
 {
   MgLevel&      c  = M->lv.back();
@@ -1737,32 +1737,34 @@ void mg_coarse_inverse(sx_multigrid* M, hipStream_t s)
       A[size_t(j) * n + i] = a;
     }
   }
-  // Cholesky in place (lower triangle)
-  for (int j = 0; j < n; j++) {
-    double d = A[size_t(j) * n + j];
-    for (int k = 0; k < j; k++) d -= A[size_t(j) * n + k] * A[size_t(j) * n + k];
-    SX_REQUIRE(d > 0.0 && std::isfinite(d), SX_ERR_NOCONV,
-               "multigrid: the coarsest operator is not positive definite");
-    const double ljj = std::sqrt(d);
-    A[size_t(j) * n + j] = ljj;
-    for (int i = j + 1; i < n; i++) {
-      double a = A[size_t(i) * n + j];
-      for (int k = 0; k < j; k++) a -= A[size_t(i) * n + k] * A[size_t(j) * n + k];
-      A[size_t(i) * n + j] = a / ljj;
-    }
-  }
-  // X = L^-T L^-1, one column of the identity at a time
-  std::vector<double> X(static_cast<size_t>(n) * static_cast<size_t>(n)), y(static_cast<size_t>(n));
+  // X = A^-1 by Gauss-Jordan elimination with partial pivoting (the tangent
+  // of a compressed soft material can be indefinite; the inverse of a
+  // symmetric matrix stays symmetric to rounding)
+  double amax = 0.0;
+  for (double a : A) amax = std::max(amax, std::fabs(a));
+  std::vector<double> X(static_cast<size_t>(n) * static_cast<size_t>(n), 0.0);
+  for (int i = 0; i < n; i++) X[size_t(i) * n + i] = 1.0;
   for (int col = 0; col < n; col++) {
-    for (int i = 0; i < n; i++) {
-      double a = i == col ? 1.0 : 0.0;
-      for (int k = 0; k < i; k++) a -= A[size_t(i) * n + k] * y[size_t(k)];
-      y[size_t(i)] = a / A[size_t(i) * n + i];
+    int piv = col;
+    for (int r = col + 1; r < n; r++)
+      if (std::fabs(A[size_t(r) * n + col]) > std::fabs(A[size_t(piv) * n + col])) piv = r;
+    SX_REQUIRE(std::fabs(A[size_t(piv) * n + col]) > 1e-14 * amax && std::isfinite(A[size_t(piv) * n + col]),
+               SX_ERR_NOCONV, "multigrid: the coarsest operator is singular");
+    if (piv != col)
+      for (int c = 0; c < n; c++) {
+        std::swap(A[size_t(piv) * n + c], A[size_t(col) * n + c]);
+        std::swap(X[size_t(piv) * n + c], X[size_t(col) * n + c]);
+      }
+    const double inv = 1.0 / A[size_t(col) * n + col];
+    for (int c = 0; c < n; c++) {
+      A[size_t(col) * n + c] *= inv;
+      X[size_t(col) * n + c] *= inv;
     }
-    for (int i = n - 1; i >= 0; i--) {
-      double a = y[size_t(i)];
-      for (int k = i + 1; k < n; k++) a -= A[size_t(k) * n + i] * X[size_t(k) * n + col];
-      X[size_t(i) * n + col] = a / A[size_t(i) * n + i];
+    for (int r = 0; r < n; r++) {
+      const double f = r == col ? 0.0 : A[size_t(r) * n + col];
+      if (f == 0.0) continue;
+      for (int c = col; c < n; c++) A[size_t(r) * n + c] -= f * A[size_t(col) * n + c];
+      for (int c = 0; c < n; c++) X[size_t(r) * n + c] -= f * X[size_t(col) * n + c];
     }
   }
   M->coarse_inv.upload(X);
