@@ -392,8 +392,10 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
  * over the order-p nodal grid (level 0, the form's matrix-free J_bc) and the
  * mesh's vertex grids (level 1; each further level halves the intervals while
  * every direction has an even count and the level holds more than 200 DOFs).
- * Coarse operators are Galerkin P^T A P (27-point 3x3-block stencils, found
- * by probing: 81 applications of the finer operator per level), smoothing is
+ * Coarse operators are Galerkin P^T A P (27-point 3x3-block stencils; level 1
+ * from the stored element tangents where possible, see sx_mg_set_galerkin,
+ * otherwise and above by probing: 81 applications of the finer operator per
+ * level), smoothing is
  * Chebyshev-Jacobi of degree `fine_degree` on level 0 and `coarse_degree`
  * above, on [upper / smoothing_range, upper] with upper = 1.15 x a
  * `power_iterations`-step power-iteration estimate of the largest eigenvalue
@@ -410,8 +412,15 @@ int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double 
  * eigenvalue bounds lag).  Default 1: a setup per linearization, as the
  * reference's preconditioners are set up on each SetOperator. */
 int sx_mg_set_reuse(sx_multigrid* M, int linearizations);
+/* how the vertex-level (level 1) operator is formed: 0 (default) element by
+ * element from the stored tangent when every trial-dependent part of J is a
+ * domain integral with a GRAD_SYM tangent in FP64 or FP32 and the essential
+ * DOFs are vertex-compatible, else by probing; 1 always by probing */
+int sx_mg_set_galerkin(sx_multigrid* M, int mode);
 /* number of levels and of setups so far */
 int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups);
+/* setups whose level-1 operator was formed element by element */
+int sx_mg_element_setups(const sx_multigrid* M, int64_t* n);
 /* level l: DOFs, grid points per direction (dims[3]) and the eigenvalue estimate of the last setup */
 int sx_mg_level(const sx_multigrid* M, int level, int64_t* ndof, int* dims, double* lmax);
 /* builds the hierarchy for the current linearization of g (host DOF list) */

@@ -420,15 +420,20 @@ class Multigrid:
         """Inside one Newton call, one hierarchy serves up to this many linear solves (default 1)."""
         capi.call("sx_mg_set_reuse", self.h, linearizations)
 
+    def set_galerkin(self, mode):
+        """Level-1 operator: 0 element-wise from the stored tangent where possible, 1 by probing."""
+        capi.call("sx_mg_set_galerkin", self.h, mode)
+
     def info(self):
-        nl, ns = ctypes.c_int(), ctypes.c_int64()
+        nl, ns, ne = ctypes.c_int(), ctypes.c_int64(), ctypes.c_int64()
         capi.call("sx_mg_info", self.h, ctypes.byref(nl), ctypes.byref(ns))
+        capi.call("sx_mg_element_setups", self.h, ctypes.byref(ne))
         levels = []
         for lvl in range(nl.value):
             nd, dims, lm = ctypes.c_int64(), (ctypes.c_int * 3)(), ctypes.c_double()
             capi.call("sx_mg_level", self.h, lvl, ctypes.byref(nd), dims, ctypes.byref(lm))
             levels.append({"ndof": nd.value, "dims": tuple(dims), "lmax": lm.value})
-        return {"levels": levels, "setups": ns.value}
+        return {"levels": levels, "setups": ns.value, "element_setups": ne.value}
 
     def setup(self, grad, ess):
         ess = np.ascontiguousarray(ess, dtype=np.int32)

@@ -270,6 +270,9 @@ struct sx_multigrid {
   int                  fine_degree = 2, coarse_degree = 2, power_iters = 12;
   double               range = 12.0, safety = 1.15;
   int64_t              setups = 0;
+  int                  galerkin = 0;  // level 1: 0 element-wise from the tangent where possible, 1 probing
+  int64_t              element_setups = 0;
+  DeviceBuffer         flag;
   int                  reuse  = 1;  // linearizations one hierarchy serves inside a Newton call
   int                  uses   = INT32_MAX;  // linear solves since the last setup (INT32_MAX: set up at the next)
 };
@@ -1770,6 +1773,34 @@ void mg_coarse_inverse(sx_multigrid* M, hipStream_t s)
   M->coarse_inv.upload(X);
 }
 
+// The vertex-level operator can come element by element from the stored
+// tangent (sxm::q1_galerkin) when every part of J that depends on the trial
+// field is a domain integral with a GRAD_SYM tangent in FP64 or FP32 and the
+// constraints are vertex-compatible (every vertex hat that is nonzero at a
+// constrained DOF is constrained itself, so P_0 of the free vertex space
+// vanishes on the constraints); otherwise it is probed.
+bool mg_elementwise(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
+{
+  sx_form* f = g->form;
+  if (M->galerkin != 0 || f->test->nc != 3) return false;
+  for (auto& B : f->bintegrals)
+    if (depends(*B, g->wrt) && B->has_tangent) return false;
+  int count = 0;
+  for (auto& I : f->integrals)
+    if (depends(*I, g->wrt) && I->has_tangent) {
+      if (I->layout != sxk::LAYOUT_GRAD_SYM || (I->tprec != sxk::PREC_F64 && I->tprec != sxk::PREC_F32)) return false;
+      count++;
+    }
+  if (count == 0) return false;
+  hipStream_t s = f->stream;
+  SX_HIP(hipMemsetAsync(M->flag.ptr, 0, sizeof(int), s));
+  SX_HIP(sxm::vertex_compatible(M->G, ess, ness, M->lv[1].mask.as<uint8_t>(), M->flag.as<int>(), s));
+  int bad = 0;
+  SX_HIP(hipMemcpyAsync(&bad, M->flag.ptr, sizeof(int), hipMemcpyDeviceToHost, s));
+  SX_HIP(hipStreamSynchronize(s));
+  return bad == 0;
+}
+
 // Builds the hierarchy for the current linearization of g with the
 // essential DOFs ess (device list): masks, the fine Jacobi diagonal, the
 // Galerkin operators by probing (27 vertex colours x 3 components per level;
@@ -1791,10 +1822,22 @@ void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
                              M->lv[size_t(l + 1)].mask.as<uint8_t>(), M->lv[size_t(l + 1)].g, s));
   compute_diag(g, ess, ness, M->diag.as<double>());
   SX_HIP(sxm::diag_inverse(M->diag.as<double>(), l0.n, ess, ness, l0.dinv.as<double>(), s));
+  const bool elementwise = mg_elementwise(M, g, ess, ness);
   for (int l = 0; l + 1 < L; l++) {
     MgLevel& F = M->lv[size_t(l)];
     MgLevel& C = M->lv[size_t(l + 1)];
     SX_HIP(sxk::vec_fill(C.S.as<double>(), 0.0, C.n * 81, s));
+    if (l == 0 && elementwise) {
+      const sxm::Grid3 gn{{M->G.nn[0], M->G.nn[1], M->G.nn[2]}};
+      for (auto& I : f->integrals)
+        if (depends(*I, g->wrt) && I->has_tangent)
+          SX_HIP(sxm::q1_galerkin(M->G.p, I->tprec == sxk::PREC_F32, I->d_map.as<const int>(), I->d_tangent.ptr,
+                                  I->nslots, element_tables(M->G.p + 1), gn, C.g, C.S.as<double>(), s));
+      SX_HIP(sxm::stencil_mask(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), s));
+      SX_HIP(sxm::stencil_dinv(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), C.dinv.as<double>(), s));
+      M->element_setups++;
+      continue;
+    }
     for (int cz = 0; cz < 3; cz++)
       for (int cy = 0; cy < 3; cy++)
         for (int cx = 0; cx < 3; cx++)
@@ -2745,6 +2788,7 @@ int sx_mg_create(const sx_space* sp, sx_multigrid** out)
     M->rt2.allocate(size_t(sxm::fe_restrict_scratch(M->G, 1)) * 8);
     M->part.allocate(size_t(sxk::reduction_blocks()) * 8);
     M->scal.allocate(16 * 8);
+    M->flag.allocate(sizeof(int));
     *out = M.release();
   });
 }
@@ -2764,6 +2808,15 @@ int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double 
   });
 }
 
+int sx_mg_set_galerkin(sx_multigrid* M, int mode)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(mode == 0 || mode == 1, SX_ERR_ARG, "Galerkin mode is 0 (element-wise where possible) or 1 (probing)");
+    M->galerkin = mode;
+  });
+}
+
 int sx_mg_set_reuse(sx_multigrid* M, int linearizations)
 {
   return api([&] {
@@ -2779,6 +2832,14 @@ int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups)
     SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
     if (nlevels) *nlevels = int(M->lv.size());
     if (setups) *setups = M->setups;
+  });
+}
+
+int sx_mg_element_setups(const sx_multigrid* M, int64_t* n)
+{
+  return api([&] {
+    SX_REQUIRE(M && n, SX_ERR_ARG, "null argument");
+    *n = M->element_setups;
   });
 }
 

@@ -4,6 +4,7 @@
 // dense coarsest solve.  All FP64.
 #include <hip/hip_runtime.h>
 
+#include "sx_element.hpp"
 #include "sx_mg.hpp"
 
 namespace sxm {
@@ -382,6 +383,148 @@ __global__ void __launch_bounds__(kThreads) k_seed_field(double* v, int64_t n, u
   }
 }
 
+// Vertex-level Galerkin operator P^T J P assembled element by element from the
+// stored GRAD_SYM tangent (9 x 9, packed upper triangle, d(parent flux) /
+// d(parent grad) at each quadrature point; the quadrature weights are not in
+// it).  The trilinear hat of corner a lies in the order-p space, so its block
+// is exact: K[(a,ci),(b,cj)] = sum_q w_q sum_{d,k} g_a,d(q) D_q[(ci,d),(cj,k)]
+// g_b,k(q), with g the parent gradients of the hats.  One workgroup per
+// element; quadrature points in chunks of QC: the chunk's tangent and
+// w D G products in LDS, each thread accumulating up to three of the 576
+// entries, then FP64 atomic adds into the stencil.
+template <int N, typename S>
+__global__ void __launch_bounds__(kThreads) k_q1_galerkin(const int* map, const void* D, int64_t nslots,
+                                                          const double* tables, int p, Grid3 gn, Grid3 gv, double* St)
+{
+  constexpr int NQ = N * N * N, DS = 45, QC = 4, NE = 576;
+  using L          = sxk::TangentLayout<S, DS, NQ>;
+  __shared__ double g[NQ * 24];  // [q][corner][dir]
+  __shared__ double w[NQ];
+  __shared__ double Dl[QC * DS];
+  __shared__ double DG[QC * 216];  // [qq][(ci,d)][(b,cj)]
+  __shared__ int    vpos[8][3];
+  __shared__ int    vid[8];
+  const int         tid = threadIdx.x;
+  const double*     pts = tables + 4 * N * N;
+  for (int e = tid; e < NQ; e += kThreads) {
+    const int qx = e % N, qy = (e / N) % N, qz = e / (N * N);
+    double    wx = 0.0, wy = 0.0, wz = 0.0;
+    for (int i = 0; i < N; i++) {
+      wx += tables[2 * N * N + qx * N + i];
+      wy += tables[2 * N * N + qy * N + i];
+      wz += tables[2 * N * N + qz * N + i];
+    }
+    w[e]              = wx * wy * wz;
+    const double x[3] = {pts[qx], pts[qy], pts[qz]};
+    for (int a = 0; a < 8; a++) {
+      const int    b[3] = {a & 1, (a >> 1) & 1, a >> 2};
+      double       f[3], df[3];
+      for (int d = 0; d < 3; d++) {
+        f[d]  = b[d] ? x[d] : 1.0 - x[d];
+        df[d] = b[d] ? 1.0 : -1.0;
+      }
+      g[e * 24 + a * 3 + 0] = df[0] * f[1] * f[2];
+      g[e * 24 + a * 3 + 1] = f[0] * df[1] * f[2];
+      g[e * 24 + a * 3 + 2] = f[0] * f[1] * df[2];
+    }
+  }
+  const int64_t nv = int64_t(gv.n[0]) * gv.n[1] * gv.n[2];
+  for (int64_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    if (tid < 8) {
+      const int li   = ((tid >> 2) * (N - 1) * N + ((tid >> 1) & 1) * (N - 1)) * N + (tid & 1) * (N - 1);
+      const int node = map[slot * NQ + li];
+      const int jx = node % gn.n[0], jy = (node / gn.n[0]) % gn.n[1], jz = node / (gn.n[0] * gn.n[1]);
+      vpos[tid][0] = jx / p;
+      vpos[tid][1] = jy / p;
+      vpos[tid][2] = jz / p;
+      vid[tid]     = (jz / p * gv.n[1] + jy / p) * gv.n[0] + jx / p;
+    }
+    double     acc[3] = {0.0, 0.0, 0.0};
+    const S*   base   = static_cast<const S*>(D) + slot * L::stride;
+    for (int q0 = 0; q0 < NQ; q0 += QC) {
+      __syncthreads();  // the previous chunk's DG is consumed (and the tables are written)
+      for (int e = tid; e < QC * DS; e += kThreads) {
+        const int qq = e / DS, k = e % DS;
+        Dl[e]        = q0 + qq < NQ ? double(base[L::offset(k, q0 + qq)]) : 0.0;
+      }
+      __syncthreads();
+      for (int e = tid; e < QC * 216; e += kThreads) {
+        const int qq = e / 216, r = e % 216, row = r / 24, col = r % 24, b = col / 3, cj = col % 3;
+        double    s = 0.0;
+        if (q0 + qq < NQ) {
+          const double* gb = g + (q0 + qq) * 24 + b * 3;
+          for (int k = 0; k < 3; k++) s += Dl[qq * DS + sxk::packed_index(row, cj * 3 + k)] * gb[k];
+          s *= w[q0 + qq];
+        }
+        DG[e] = s;
+      }
+      __syncthreads();
+      for (int m = 0; m < 3; m++) {
+        const int e = tid + m * kThreads;
+        if (e >= NE) break;
+        const int a = e / 72, ci = (e / 24) % 3, col = e % 24;
+        double    s = 0.0;
+        for (int qq = 0; qq < QC && q0 + qq < NQ; qq++) {
+          const double* ga = g + (q0 + qq) * 24 + a * 3;
+          const double* dg = DG + qq * 216 + (ci * 3) * 24 + col;
+          s += ga[0] * dg[0] + ga[1] * dg[24] + ga[2] * dg[48];
+        }
+        acc[m] += s;
+      }
+    }
+    for (int m = 0; m < 3; m++) {
+      const int e = tid + m * kThreads;
+      if (e >= NE) break;
+      const int a = e / 72, ci = (e / 24) % 3, b = (e % 24) / 3, cj = e % 3;
+      const int o = (vpos[b][2] - vpos[a][2] + 1) * 9 + (vpos[b][1] - vpos[a][1] + 1) * 3 + (vpos[b][0] - vpos[a][0] + 1);
+      atomicAdd(St + (int64_t(o) * 9 + ci * 3 + cj) * nv + vid[a], acc[m]);
+    }
+    __syncthreads();  // vpos / vid of this element are read before the next element writes them
+  }
+}
+
+// zero the stencil entries whose row or column is constrained
+__global__ void __launch_bounds__(kThreads) k_stencil_mask(double* S, Grid3 g, const uint8_t* mask)
+{
+  const int64_t nv = nverts(g);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nv * 243; t += int64_t(gridDim.x) * kThreads) {
+    const int64_t v = t % nv;
+    const int     k = int(t / nv), o = k / 9, co = (k / 3) % 3, ci = k % 3;
+    int           ix, iy, iz;
+    coords(g, v, ix, iy, iz);
+    const int jx = ix + o % 3 - 1, jy = iy + (o / 3) % 3 - 1, jz = iz + o / 9 - 1;
+    const bool out = jx < 0 || jx >= g.n[0] || jy < 0 || jy >= g.n[1] || jz < 0 || jz >= g.n[2];
+    if (out || mask[v * 3 + co] || mask[((int64_t(jz) * g.n[1] + jy) * g.n[0] + jx) * 3 + ci]) S[t] = 0.0;
+  }
+}
+
+// flag[0] = 1 when a constrained node component has an unconstrained vertex
+// whose hat is nonzero there (then P_0 of the unconstrained vertex space does
+// not vanish on the constraints and the element-wise operator differs from
+// the probed Z P^T J P Z)
+__global__ void __launch_bounds__(kThreads) k_vertex_compatible(sxu::FEGrid G, const int32_t* rows, int64_t nr,
+                                                                const uint8_t* vmask, int* flag)
+{
+  for (int64_t k = int64_t(blockIdx.x) * kThreads + threadIdx.x; k < nr; k += int64_t(gridDim.x) * kThreads) {
+    const int64_t dof = rows[k], node = dof / 3;
+    const int     c = int(dof % 3);
+    const int     j[3] = {int(node % G.nn[0]), int((node / G.nn[0]) % G.nn[1]), int(node / (int64_t(G.nn[0]) * G.nn[1]))};
+    int           lo[3], cnt[3];
+    for (int d = 0; d < 3; d++) {
+      lo[d]  = j[d] / G.p;
+      cnt[d] = (j[d] % G.p) ? 2 : 1;
+    }
+    bool bad = false;
+    for (int bz = 0; bz < cnt[2]; bz++)
+      for (int by = 0; by < cnt[1]; by++)
+        for (int bx = 0; bx < cnt[0]; bx++) {
+          const int64_t v = (int64_t(lo[2] + bz) * G.nv[1] + (lo[1] + by)) * G.nv[0] + (lo[0] + bx);
+          bad             = bad || !vmask[v * 3 + c];
+        }
+    if (bad) atomicMax(flag, 1);
+  }
+}
+
 }  // namespace
 
 int64_t fe_restrict_scratch(const sxu::FEGrid& G, int pass)
@@ -546,6 +689,43 @@ hipError_t scale(double* v, double a, int64_t n, hipStream_t s)
 hipError_t seed_field(double* v, int64_t n, uint64_t seed, const uint8_t* mask, hipStream_t s)
 {
   hipLaunchKernelGGL(k_seed_field, dim3(blocks_for(n)), dim3(kThreads), 0, s, v, n, seed, mask);
+  return hipGetLastError();
+}
+
+hipError_t q1_galerkin(int order, int tprec_f32, const int* map, const void* D, int64_t nslots, const double* tables,
+                      Grid3 gn, Grid3 gv, double* S, hipStream_t s)
+{
+  const unsigned grid = unsigned(nslots < 16384 ? (nslots < 1 ? 1 : nslots) : 16384);
+#define SXM_Q1(NN)                                                                                                  \
+  if (order + 1 == NN) {                                                                                            \
+    if (tprec_f32)                                                                                                  \
+      hipLaunchKernelGGL((k_q1_galerkin<NN, float>), dim3(grid), dim3(kThreads), 0, s, map, D, nslots, tables,      \
+                         order, gn, gv, S);                                                                         \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_q1_galerkin<NN, double>), dim3(grid), dim3(kThreads), 0, s, map, D, nslots, tables,     \
+                         order, gn, gv, S);                                                                         \
+    return hipGetLastError();                                                                                       \
+  }
+  SXM_Q1(2)
+  SXM_Q1(3)
+  SXM_Q1(4)
+  SXM_Q1(5)
+#undef SXM_Q1
+  return hipErrorInvalidValue;
+}
+
+hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s)
+{
+  const int64_t n = int64_t(g.n[0]) * g.n[1] * g.n[2] * 243;
+  hipLaunchKernelGGL(k_stencil_mask, dim3(blocks_for(n)), dim3(kThreads), 0, s, S, g, mask);
+  return hipGetLastError();
+}
+
+hipError_t vertex_compatible(const sxu::FEGrid& G, const int32_t* rows, int64_t nr, const uint8_t* vmask, int* flag,
+                             hipStream_t s)
+{
+  if (nr <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_vertex_compatible, dim3(blocks_for(nr)), dim3(kThreads), 0, s, G, rows, nr, vmask, flag);
   return hipGetLastError();
 }
 

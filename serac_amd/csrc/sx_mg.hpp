@@ -73,6 +73,18 @@ hipError_t dense_mv(const double* A, int n, const double* b, double* x, hipStrea
 // scal[1] = den, scal[2] = alpha, scal[5] = scal[0]; nothing once stop[0] is set
 hipError_t pcg_step_xr(double* x, double* r, double* Ad, const double* d, double* scal, const double* part_den,
                        int nb, int64_t n, const double* stop, hipStream_t s);
+// vertex-level Galerkin operator from the stored GRAD_SYM tangent of one
+// 3-component domain integral (FP64, or FP32 with tprec_f32), accumulated into
+// S (atomics); map: [slot][(order + 1)^3] node ids of the lexicographic nodal
+// grid gn; tables: the integral's B, G, Bw, Gw and GL points
+hipError_t q1_galerkin(int order, int tprec_f32, const int* map, const void* D, int64_t nslots, const double* tables,
+                      Grid3 gn, Grid3 gv, double* S, hipStream_t s);
+// zero the stencil entries with a constrained row or column
+hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s);
+// *flag = 1 (atomic max) when a listed constrained DOF has an unconstrained
+// vertex component whose hat is nonzero at it
+hipError_t vertex_compatible(const sxu::FEGrid& G, const int32_t* rows, int64_t nr, const uint8_t* vmask, int* flag,
+                             hipStream_t s);
 // power iteration helpers: v = dinv t / |dinv t| (norms on the host)
 hipError_t mul_dinv(const double* dinv, const double* t, double* v, int64_t n, hipStream_t s);
 hipError_t scale(double* v, double a, int64_t n, hipStream_t s);

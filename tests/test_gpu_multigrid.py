@@ -169,6 +169,46 @@ def test_levels_are_galerkin_products_and_the_vcycle_matches_numpy(gpu_lib, n, p
     assert np.abs(z.numpy() - zref).max() <= 1e-10 * np.abs(zref).max()
 
 
+@pytest.mark.parametrize("p,tprec", [(1, "f64"), (2, "f64"), (3, "f64"), (4, "f64"), (3, "f32"), (2, "f32")])
+def test_elementwise_vertex_operator_equals_the_probed_one(gpu_lib, p, tprec):
+    n = 4
+    m, s, f, ess = setup(n, p)
+    if tprec == "f32":
+        f.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, sx.capi.SX_PREC_F32)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    mgs = []
+    for mode in (0, 1):
+        mg = sx.Multigrid(s)
+        mg.set_galerkin(mode)
+        mg.setup(grad, ess)
+        mgs.append(mg)
+    assert mgs[0].info()["element_setups"] == 1 and mgs[1].info()["element_setups"] == 0
+    nv1 = mgs[0].info()["levels"][1]["ndof"]
+    x = sx.DeviceVector.from_numpy(np.random.default_rng(2).standard_normal(nv1))
+    ys = []
+    for mg in mgs:
+        y = sx.DeviceVector(nv1)
+        mg.level_apply(1, x, y)
+        ys.append(y.numpy())
+    assert np.abs(ys[0] - ys[1]).max() <= 1e-11 * np.abs(ys[1]).max()
+
+
+def test_constraints_off_the_vertices_fall_back_to_probing(gpu_lib):
+    m, s, f, ess = setup(4, 2)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    # one more constrained DOF at an edge-midpoint node inside the mesh
+    nn = 2 * 4 + 1
+    node = (4 * nn + 4) * nn + 3
+    ess2 = np.union1d(ess, [3 * node]).astype(np.int32)
+    mg = sx.Multigrid(s)
+    mg.setup(grad, ess2)
+    assert mg.info()["element_setups"] == 0
+    mg.setup(grad, ess)
+    assert mg.info()["element_setups"] == 1
+
+
 def test_vcycle_is_symmetric_and_positive(gpu_lib):
     m, s, f, ess, grad = linearized(8, 2)
     mg = sx.Multigrid(s)
