@@ -302,6 +302,44 @@ def test_a_hierarchy_reused_across_a_load_step_reaches_the_same_solution(gpu_lib
         mg.set_reuse(0)
 
 
+def test_j2_plastic_path_with_multigrid_matches_jacobi(gpu_lib):
+    # J2 plasticity (small strain, kinematic + isotropic hardening): the 81-entry
+    # tangent takes the probed vertex level; cyclic shear of the top face with
+    # the state committed after every step, against the same path with Jacobi-CG
+    from test_gpu_j2 import J2_PARAMS
+
+    n, p = 4, 2
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, 0.1)
+    s = sx.Space(m, p, 3, 1)
+    bot = s.boundary_dofs(0b010000, 0x7)
+    top = s.boundary_dofs(0b100000, 0x7)
+    ess = np.union1d(bot, top).astype(np.int32)
+    top_x = np.isin(ess, s.boundary_dofs(0b100000, 0x1))
+    runs = []
+    for use_mg in (False, True):
+        qd = sx.QuadratureData(s.ne, (p + 1) ** 3, np.zeros(10))
+        f = sx.Functional(s, [s])
+        f.add_domain_integral(sx.SX_QF_J2_LINEAR, J2_PARAMS["kinematic_small_strain"], qdata=qd)
+        mg = sx.Multigrid(s) if use_mg else None
+        f.set_multigrid(mg)
+        x = sx.DeviceVector(s.ndof)
+        its = 0
+        for g in (0.004, 0.008, 0.0):
+            reps = sx.quasistatic_solve(f, x, ess, np.where(top_x, g, 0.0), 1, rel_tol=1e-10, lin_rel_tol=1e-10,
+                                        lin_max_iter=20000)
+            assert reps[0]["converged"]
+            its += reps[0]["linear_iterations"]
+        runs.append((x.numpy(), qd.numpy(), its, mg.info() if mg else None))
+        f.set_multigrid(None)
+    (xj, sj, itj, _), (xm, sm, itm, info) = runs
+    assert np.abs(xm - xj).max() <= 1e-7 * np.abs(xj).max()
+    assert np.abs(sm - sj).max() <= 1e-7 * np.abs(sj).max()
+    assert sj[..., 9].max() > 0  # the path yielded
+    assert info["element_setups"] == 0 and info["setups"] > 0
+    assert itm < itj
+
+
 def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
     s1 = sx.Space(sx.Mesh.cuboid(4, 4, 4), 2, 1, 1)
     with pytest.raises(sx.SeracAmdError):
