@@ -486,18 +486,26 @@ def main():
         solve = run_solve(sx, args, space, make_form, args.solve_precision,
                           f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
         form = make_form(args.scatter)
-    elif qf == "neohookean" and not args.no_default_solve:
+    solve_full = None
+    if not args.solve and qf == "neohookean" and not args.no_default_solve:
+        # the full-size load path in mixed precision with multigrid-CG (about
+        # 12 s at 64^3); its FP64 Jacobi-CG baseline takes minutes and runs
+        # with --solve --solve-precision f64
+        del form
+        solve_full = run_solve(sx, args, space, make_form, "mixed_f32_mg",
+                               f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
+        solve_full["note"] = ("FP64 Jacobi-CG on this load path takes minutes and is not rerun by default: "
+                              "bench.py --solve --solve-precision f64")
+        form = make_form(args.scatter)
         # the same load path on a 24^3 mesh, FP64 Jacobi-CG against mixed
-        # precision and multigrid-CG: the FP64 Jacobi-CG solve on the full mesh
-        # takes minutes (run it with --solve)
+        # precision and multigrid-CG
         ns = 24
         smesh = sx.Mesh.cuboid(ns, ns, ns)
         if perturb > 0:
             smesh.perturb(42, perturb)
         sspace = sx.Space(smesh, p, 3)
         solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32,f64_mg,mixed_f32_mg",
-                          f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs; the full-size solve "
-                          f"is bench.py --solve)")
+                          f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs)")
         del sspace, smesh
 
     if args.calibrate:
@@ -561,6 +569,7 @@ def main():
         "jv_norm": jv_norm,
         "precision_split": split,
         "newton_solve": solve,
+        "newton_solve_full_size": solve_full,
         "learned_unet": unet,
         "device": dev,
     }
