@@ -435,8 +435,14 @@ int sx_mg_level_apply(sx_multigrid* M, int level, const double* x, double* y);
  * sets the hierarchy up first */
 int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
                     double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
-/* the CG solves of sx_newton_solve / sx_quasistatic_solve on this form use M
- * (set up once per linearization); NULL returns to Jacobi.  M is borrowed:
+/* flexible GMRES(restart) on J_bc x = b (x0 = 0) right-preconditioned by M
+ * (for nonsymmetric tangents); minimises ||b - J_bc x|| down to
+ * max(rel ||b||, abs); sets the hierarchy up first */
+int sx_mg_gmres_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                      double rel_tol, double abs_tol, int max_iter, int restart, sx_solve_report* report);
+/* the linear solves of sx_newton_solve / sx_quasistatic_solve on this form
+ * use M (set up once per linearization): multigrid-CG, or multigrid-FGMRES
+ * with SX_OPT_LINEAR_SOLVER = GMRES; NULL returns to Jacobi.  M is borrowed:
  * detach it before sx_mg_destroy. */
 int sx_form_set_multigrid(sx_form* f, sx_multigrid* M);
 int sx_mg_destroy(sx_multigrid* M);

@@ -294,6 +294,16 @@ class Gradient:
             raise SeracAmdError(rc, capi.load().sx_last_error().decode())
         return rep.as_dict()
 
+    def mg_gmres(self, mg, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, restart=50, raise_on_noconv=True):
+        """Flexible GMRES on J_bc x = b right-preconditioned by the multigrid V-cycle."""
+        ess = np.ascontiguousarray(ess, dtype=np.int32)
+        rep = SolveReport()
+        rc = capi.call_rc("sx_mg_gmres_solve", self.h, mg.h, _iptr(ess), len(ess), b.ptr, x.ptr, rel_tol, abs_tol,
+                          max_iter, restart, ctypes.byref(rep))
+        if rc != 0 and (rc != capi.SX_ERR_NOCONV or raise_on_noconv):
+            raise SeracAmdError(rc, capi.load().sx_last_error().decode())
+        return rep.as_dict()
+
     def gmres(self, ess, b, x, rel_tol=1e-10, abs_tol=0.0, max_iter=10000, restart=50, raise_on_noconv=True):
         """Restarted GMRES with left Jacobi preconditioning on J_bc x = b."""
         ess = np.ascontiguousarray(ess, dtype=np.int32)

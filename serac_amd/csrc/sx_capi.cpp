@@ -2395,12 +2395,21 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
 
 // Flexible GMRES(m) (Saad's FGMRES: right preconditioning with a
 // preconditioner that may change every iteration) on J_bc x = b, x0 = 0, with
-// the learned M(r) (Jacobi alone when L is null).  Minimises ||b - J_bc x||;
-// stops at max(rel ||b||, abs), checked on the true residual at each restart.
-// Stores the m + 1 Arnoldi vectors and the m preconditioned directions.
+// the multigrid V-cycle when mg is set, else the learned M(r) (Jacobi alone
+// when L is null too).  Minimises ||b - J_bc x||; stops at max(rel ||b||,
+// abs), checked on the true residual at each restart.  Stores the m + 1
+// Arnoldi vectors and the m preconditioned directions.
 void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
-            double abs_tol, int max_iter, int m, Workspace& w, sx_solve_report* rep)
+            double abs_tol, int max_iter, int m, Workspace& w, sx_solve_report* rep, sx_multigrid* mg = nullptr,
+            bool allow_reuse = false)
 {
+  if (mg) {
+    if (!allow_reuse || mg->uses >= mg->reuse) {
+      mg_setup(mg, g, ess, ness);
+      mg->uses = 0;
+    }
+    mg->uses++;
+  }
   sx_form*      f = g->form;
   const int64_t n = f->test->ndof();
   hipStream_t   s = f->stream;
@@ -2422,7 +2431,9 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
     SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
   };
   auto precondition = [&](const double* in, double* outv) {
-    if (L) {
+    if (mg) {
+      mg_apply(mg, g, ess, ness, in, outv);
+    } else if (L) {
       learned_apply(L, f, diag, ess, ness, in, outv);
     } else {
       SX_HIP(sxk::jacobi_apply(in, diag, outv, n, s));
@@ -2524,6 +2535,8 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
     CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
     cg(g, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, &hook);
     rec.finish(x, rep->converged);
+  } else if (f->linsolver == SX_LINSOLVER_GMRES && f->mg) {
+    fgmres(g, nullptr, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep, f->mg, true);
   } else if (f->linsolver == SX_LINSOLVER_GMRES) {
     gmres(g, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
   } else if (f->mg) {
@@ -2908,6 +2921,27 @@ int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_
     mg_pcg(g, M, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "multigrid-CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+int sx_mg_gmres_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                      double rel_tol, double abs_tol, int max_iter, int restart, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && M && b && x, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid-GMRES on a gradient before a linearization");
+    SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
+    SX_REQUIRE(M->space == g->form->test, SX_ERR_ARG, "the multigrid belongs to another space");
+    for (int64_t k = 0; k < ness; k++)
+      SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
+    Workspace w;
+    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    fgmres(g, nullptr, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep, M);
+    SX_HIP(hipStreamSynchronize(g->form->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "multigrid-GMRES did not converge");
   });
   if (report) *report = rep;
   return rc;

@@ -340,6 +340,54 @@ def test_j2_plastic_path_with_multigrid_matches_jacobi(gpu_lib):
     assert itm < itj
 
 
+def test_multigrid_gmres_meets_the_true_residual_goal(gpu_lib):
+    m, s, f, ess, grad = linearized(8, 2)
+    b = W.nodal_field(s.nnodes, 3, 8, 3000, -1.0, 1.0, 1)
+    b[ess] = 0.0
+    A = eliminated(grad.assemble().to_scipy(), ess)
+    mg = sx.Multigrid(s)
+    x = sx.DeviceVector(s.ndof)
+    rep = grad.mg_gmres(mg, ess, sx.DeviceVector.from_numpy(b), x, rel_tol=1e-9, max_iter=500, restart=50)
+    assert rep["converged"] and rep["linear_iterations"] < 100
+    assert np.linalg.norm(b - A @ x.numpy()) <= 1.01e-9 * np.linalg.norm(b)
+
+
+def test_j2_geometrically_nonlinear_path_with_multigrid_gmres(gpu_lib):
+    # the J2 tangent under geometric nonlinearity is not symmetric: Newton with
+    # multigrid-FGMRES against Newton with Jacobi-CG on the same cyclic path
+    from test_gpu_j2 import J2_PARAMS
+
+    n, p = 4, 2
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, 0.1)
+    s = sx.Space(m, p, 3, 1)
+    bot = s.boundary_dofs(0b010000, 0x7)
+    top = s.boundary_dofs(0b100000, 0x7)
+    ess = np.union1d(bot, top).astype(np.int32)
+    top_x = np.isin(ess, s.boundary_dofs(0b100000, 0x1))
+    runs = []
+    for use_mg in (False, True):
+        qd = sx.QuadratureData(s.ne, (p + 1) ** 3, np.zeros(10))
+        f = sx.Functional(s, [s])
+        f.add_domain_integral(sx.SX_QF_J2_LINEAR, J2_PARAMS["c4"], qdata=qd)
+        mg = None
+        if use_mg:
+            mg = sx.Multigrid(s)
+            f.set_multigrid(mg)
+            f.set_option(sx.capi.SX_OPT_LINEAR_SOLVER, sx.capi.SX_LINSOLVER_GMRES)
+        x = sx.DeviceVector(s.ndof)
+        for g in (0.004, 0.008, 0.0):
+            reps = sx.quasistatic_solve(f, x, ess, np.where(top_x, g, 0.0), 1, rel_tol=1e-10, lin_rel_tol=1e-10,
+                                        lin_max_iter=20000)
+            assert reps[0]["converged"]
+        runs.append((x.numpy(), qd.numpy(), mg.info() if mg else None))
+        f.set_multigrid(None)
+    (xj, sj, _), (xm, sm, info) = runs
+    assert np.abs(xm - xj).max() <= 1e-7 * np.abs(xj).max()
+    assert np.abs(sm - sj).max() <= 1e-7 * np.abs(sj).max()
+    assert info["setups"] > 0
+
+
 def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
     s1 = sx.Space(sx.Mesh.cuboid(4, 4, 4), 2, 1, 1)
     with pytest.raises(sx.SeracAmdError):
