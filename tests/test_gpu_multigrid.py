@@ -33,36 +33,44 @@ def trilinear_1d(nc):
     return P
 
 
-def kron3(P1):
-    """The 3-component tensor-product transfer, DOF = node * 3 + c."""
-    P = sp.kron(P1, sp.kron(P1, P1)).tocsr()
+def kron3(Px, Py=None, Pz=None):
+    """The 3-component tensor-product transfer (x fastest), DOF = node * 3 + c."""
+    Py = Px if Py is None else Py
+    Pz = Px if Pz is None else Pz
+    P = sp.kron(sp.csr_matrix(Pz), sp.kron(sp.csr_matrix(Py), sp.csr_matrix(Px))).tocsr()
     return sp.kron(P, sp.identity(3)).tocsr()
 
 
+def coinciding(fine_dims, idx):
+    """Flat indices of the fine points idx[d] (per direction) in a [z][y][x] grid of fine_dims (x, y, z)."""
+    ix, iy, iz = idx
+    return ((iz[:, None, None] * fine_dims[1] + iy[None, :, None]) * fine_dims[0] + ix[None, None, :]).ravel()
+
+
 def hierarchy(n, p, grad, ess, info):
-    """The numpy restatement: level operators, masks and transfers."""
-    nd0 = 3 * (p * n + 1) ** 3
-    keep0 = np.ones(nd0)
+    """The numpy restatement: level operators, masks and transfers; n is the
+    interval count per direction (an int for a cube, or (nx, ny, nz))."""
+    ns = (n, n, n) if np.isscalar(n) else tuple(n)
+    nn = [p * k + 1 for k in ns]
+    keep0 = np.ones(3 * int(np.prod(nn)))
     keep0[ess] = 0.0
     Z0 = sp.diags(keep0)
     A0 = (Z0 @ grad.assemble().to_scipy() @ Z0).tocsr()
     levels = [{"A": A0, "keep": keep0}]
     # level 1: the vertex grid; a vertex component is constrained when the
     # node at the vertex is
-    nv = n + 1
-    idx = (p * np.arange(nv))
-    node = (idx[:, None, None] * (p * n + 1) + idx[None, :, None]) * (p * n + 1) + idx[None, None, :]
-    keep1 = keep0.reshape(-1, 3)[node.ravel()].ravel()
-    Ps = [(Z0 @ kron3(sp.csr_matrix(interp_1d(n, p))) @ sp.diags(keep1)).tocsr()]
+    nv = [k + 1 for k in ns]
+    node = coinciding(nn, [p * np.arange(k) for k in nv])
+    keep1 = keep0.reshape(-1, 3)[node].ravel()
+    Ps = [(Z0 @ kron3(*[interp_1d(k, p) for k in ns]) @ sp.diags(keep1)).tocsr()]
     keeps = [keep1]
     dims = [nv]
     for lvl in info["levels"][2:]:
-        nc = lvl["dims"][0]
+        nc = list(lvl["dims"])
         nf = dims[-1]
-        cidx = 2 * np.arange(nc)
-        vert = (cidx[:, None, None] * nf + cidx[None, :, None]) * nf + cidx[None, None, :]
-        kc = keeps[-1].reshape(-1, 3)[vert.ravel()].ravel()
-        Ps.append((sp.diags(keeps[-1]) @ kron3(sp.csr_matrix(trilinear_1d(nc))) @ sp.diags(kc)).tocsr())
+        vert = coinciding(nf, [2 * np.arange(k) for k in nc])
+        kc = keeps[-1].reshape(-1, 3)[vert].ravel()
+        Ps.append((sp.diags(keeps[-1]) @ kron3(*[trilinear_1d(k) for k in nc]) @ sp.diags(kc)).tocsr())
         keeps.append(kc)
         dims.append(nc)
     for P, k in zip(Ps, keeps):
@@ -207,6 +215,40 @@ def test_constraints_off_the_vertices_fall_back_to_probing(gpu_lib):
     assert mg.info()["element_setups"] == 0
     mg.setup(grad, ess)
     assert mg.info()["element_setups"] == 1
+
+
+def test_non_cubic_mesh_levels_and_vcycle_match_numpy(gpu_lib):
+    # different interval counts per direction (8 x 4 x 6, p = 2): vertex grids
+    # 9x5x7 and 5x3x4 (dense); catches a swapped direction in any transfer
+    ns, p = (8, 4, 6), 2
+    m = sx.Mesh.cuboid(*ns)
+    m.perturb(42, 0.2)
+    s = sx.Space(m, p, 3, 1)
+    f = sx.Functional(s, [s])
+    f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+    ess = s.boundary_dofs(0b000001, 0x7)
+    U = W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1)
+    _, grad = f(0.0, sx.DeviceVector.from_numpy(U), wrt=0)
+    for galerkin in (0, 1):
+        mg = sx.Multigrid(s, fine_degree=3, coarse_degree=2, smoothing_range=10.0)
+        mg.set_galerkin(galerkin)
+        mg.setup(grad, ess)
+        info = mg.info()
+        assert [lv["dims"] for lv in info["levels"]] == [(17, 9, 13), (9, 5, 7), (5, 3, 4)]
+        assert info["element_setups"] == (1 if galerkin == 0 else 0)
+        levels = hierarchy(ns, p, grad, ess, info)
+        rng = np.random.default_rng(5)
+        for lvl, lv in enumerate(levels):
+            x = rng.standard_normal(lv["A"].shape[0]) * lv["keep"]
+            y = sx.DeviceVector(len(x))
+            mg.level_apply(lvl, sx.DeviceVector.from_numpy(x), y)
+            ref = lv["A"] @ x
+            assert np.abs(y.numpy() - ref).max() <= 1e-12 * np.abs(ref).max(), (galerkin, lvl)
+        r = W.nodal_field(s.nnodes, 3, 7, 3000, -1.0, 1.0, 1)
+        z = sx.DeviceVector(s.ndof)
+        mg.apply(sx.DeviceVector.from_numpy(r), z)
+        zref = vcycle_numpy(levels, info, 3, 2, 10.0, r, ess)
+        assert np.abs(z.numpy() - zref).max() <= 1e-10 * np.abs(zref).max()
 
 
 def test_vcycle_is_symmetric_and_positive(gpu_lib):
