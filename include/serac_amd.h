@@ -417,6 +417,12 @@ int sx_mg_set_reuse(sx_multigrid* M, int linearizations);
  * domain integral with a GRAD_SYM tangent in FP64 or FP32 and the essential
  * DOFs are vertex-compatible, else by probing; 1 always by probing */
 int sx_mg_set_galerkin(sx_multigrid* M, int mode);
+/* SX_PREC_BF16: the level-0 applications of the V-cycle (smoothing, power
+ * iteration, probing) use BF16 copies of the stored tangents, made at each
+ * setup (FP64 or FP32 GRAD_SYM / GRAD_FULL tangents only; otherwise the form's
+ * own tangent is used); the Krylov solver's J.v keeps the form's tangent.
+ * SX_PREC_F64 (default): the form's tangent. */
+int sx_mg_set_smoother_precision(sx_multigrid* M, int prec);
 /* number of levels and of setups so far */
 int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups);
 /* setups whose level-1 operator was formed element by element */

@@ -430,6 +430,10 @@ class Multigrid:
         """Inside one Newton call, one hierarchy serves up to this many linear solves (default 1)."""
         capi.call("sx_mg_set_reuse", self.h, linearizations)
 
+    def set_smoother_precision(self, prec):
+        """SX_PREC_BF16: level-0 applications of the V-cycle with BF16 tangent copies; SX_PREC_F64: the form's."""
+        capi.call("sx_mg_set_smoother_precision", self.h, prec)
+
     def set_galerkin(self, mode):
         """Level-1 operator: 0 element-wise from the stored tangent where possible, 1 by probing."""
         capi.call("sx_mg_set_galerkin", self.h, mode)

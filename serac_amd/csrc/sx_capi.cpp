@@ -195,6 +195,7 @@ struct Integral {
   DeviceBuffer         d_map;      // [slot][npe]
   DeviceBuffer         d_coords;   // [slot][24]
   DeviceBuffer         d_tangent;  // per-slot TangentLayout (sx_element.hpp)
+  DeviceBuffer         d_tangent_lo;  // BF16 copy for a multigrid smoother (set up with the hierarchy)
   int                  layout  = sxk::LAYOUT_GRAD_SYM;
   int                  tprec   = sxk::PREC_F64;  // precision the tangent was stored in
   bool                 has_tangent = false;
@@ -271,6 +272,8 @@ struct sx_multigrid {
   double               range = 12.0, safety = 1.15;
   int64_t              setups = 0;
   int                  galerkin = 0;  // level 1: 0 element-wise from the tangent where possible, 1 probing
+  int                  smoother_bf16 = 0;  // level-0 applications with BF16 tangent copies
+  bool                 lo_active     = false;
   int64_t              element_setups = 0;
   DeviceBuffer         flag;
   int                  reuse  = 1;  // linearizations one hierarchy serves inside a Newton call
@@ -358,14 +361,19 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   return P;
 }
 
-// one launch per colour; returns the number of launches
+// one launch per colour; returns the number of launches.  lo: J.v with the
+// BF16 copy of the tangent (d_tangent_lo)
 int run_integral(const sx_form* f, const Integral& I, int mode, const void* u, void* r, hipEvent_t* ev = nullptr,
-                 int vprec = sxk::PREC_F64)
+                 int vprec = sxk::PREC_F64, bool lo = false)
 {
   sxk::ElemLaunch P = base_launch(f, I, mode);
   P.u               = u;
   P.r               = r;
   P.vprec           = vprec;
+  if (lo) {
+    P.D     = I.d_tangent_lo.ptr;
+    P.tprec = sxk::PREC_BF16;
+  }
   int launches      = 0;
   for (size_t c = 0; c + 1 < I.color_off.size(); c++) {
     P.slot_begin = I.color_off[c];
@@ -624,9 +632,10 @@ bool depends(const Integral& I, int trial)
   return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
 }
 
-// df = J dx; out_zeroed: df already holds zeros (the fill is skipped)
+// df = J dx; out_zeroed: df already holds zeros (the fill is skipped); lo:
+// the domain integrals use their BF16 tangent copies
 void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr,
-                    int vprec = sxk::PREC_F64, bool out_zeroed = false)
+                    int vprec = sxk::PREC_F64, bool out_zeroed = false, bool lo = false)
 {
   SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
   SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
@@ -643,7 +652,8 @@ void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullp
   int       n     = 0;
   for (auto& I : f->integrals)
     if (depends(*I, g->wrt) && I->has_tangent)
-      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, kprec);
+      n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, kprec,
+                        lo && I->d_tangent_lo.ptr != nullptr);
   for (auto& B : f->bintegrals)
     if (depends(*B, g->wrt) && B->has_tangent)
       n += run_boundary(f, *B, sxk::FMODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, vprec);
@@ -1692,7 +1702,7 @@ void mg_level_apply(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t nes
 {
   hipStream_t s = g->form->stream;
   if (l == 0) {
-    apply_gradient(g, x, y);
+    apply_gradient(g, x, y, nullptr, nullptr, sxk::PREC_F64, false, M->lo_active);
     SX_HIP(sxk::set_rows(y, ess, ness, 0.0, s));
   } else {
     SX_HIP(sxm::stencil_apply(M->lv[size_t(l)].S.as<double>(), M->lv[size_t(l)].g, x, y, s));
@@ -1823,6 +1833,26 @@ void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
   compute_diag(g, ess, ness, M->diag.as<double>());
   SX_HIP(sxm::diag_inverse(M->diag.as<double>(), l0.n, ess, ness, l0.dinv.as<double>(), s));
   const bool elementwise = mg_elementwise(M, g, ess, ness);
+  // BF16 tangent copies for the level-0 applications (smoothing, power
+  // iteration, probing) when asked for and every stored tangent is FP64 or FP32
+  M->lo_active = false;
+  if (M->smoother_bf16) {
+    bool ok = true;
+    for (auto& I : f->integrals)
+      if (depends(*I, g->wrt) && I->has_tangent)
+        ok = ok && (I->tprec == sxk::PREC_F64 || I->tprec == sxk::PREC_F32) &&
+             (I->layout == sxk::LAYOUT_GRAD_SYM || I->layout == sxk::LAYOUT_GRAD_FULL) && f->test->nc == 3;
+    if (ok) {
+      for (auto& I : f->integrals)
+        if (depends(*I, g->wrt) && I->has_tangent) {
+          const int ds = sxk::tangent_ds(3, I->layout);
+          I->d_tangent_lo.allocate(size_t(sxm::bf16_tangent_elems(M->G.p, ds, I->nslots)) * 2);
+          SX_HIP(sxm::tangent_to_bf16(M->G.p, ds, I->tprec == sxk::PREC_F32, I->d_tangent.ptr,
+                                      I->d_tangent_lo.as<uint16_t>(), I->nslots, s));
+        }
+      M->lo_active = true;
+    }
+  }
   for (int l = 0; l + 1 < L; l++) {
     MgLevel& F = M->lv[size_t(l)];
     MgLevel& C = M->lv[size_t(l + 1)];
@@ -2827,6 +2857,16 @@ int sx_mg_set_galerkin(sx_multigrid* M, int mode)
     SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
     SX_REQUIRE(mode == 0 || mode == 1, SX_ERR_ARG, "Galerkin mode is 0 (element-wise where possible) or 1 (probing)");
     M->galerkin = mode;
+  });
+}
+
+int sx_mg_set_smoother_precision(sx_multigrid* M, int prec)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(prec == SX_PREC_F64 || prec == SX_PREC_BF16, SX_ERR_ARG,
+               "smoother precision is SX_PREC_F64 (the form's tangent) or SX_PREC_BF16");
+    M->smoother_bf16 = prec == SX_PREC_BF16;
   });
 }
 

@@ -483,6 +483,22 @@ __global__ void __launch_bounds__(kThreads) k_q1_galerkin(const int* map, const 
   }
 }
 
+// A BF16 copy of a stored tangent (FP64 or FP32 source) in the BF16 layout:
+// entry k of quadrature point q of each slot, rounded to nearest even.
+template <typename SRC, int DS, int NQ>
+__global__ void __launch_bounds__(kThreads) k_tangent_to_bf16(const SRC* src, uint16_t* dst, int64_t nslots)
+{
+  using Ls          = sxk::TangentLayout<SRC, DS, NQ>;
+  using Lb          = sxk::TangentLayout<sxk::bf16, DS, NQ>;
+  const int64_t n   = nslots * DS * NQ;
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < n; t += int64_t(gridDim.x) * kThreads) {
+    const int64_t slot = t / (DS * NQ);
+    const int     e = int(t % (DS * NQ)), q = e % NQ, k = e / NQ;
+    const double  v = double(src[slot * Ls::stride + Ls::offset(k, q)]);
+    dst[slot * Lb::stride + Lb::offset(k, q)] = sxk::to_store(v, static_cast<sxk::bf16*>(nullptr));
+  }
+}
+
 // zero the stencil entries whose row or column is constrained
 __global__ void __launch_bounds__(kThreads) k_stencil_mask(double* S, Grid3 g, const uint8_t* mask)
 {
@@ -711,6 +727,39 @@ hipError_t q1_galerkin(int order, int tprec_f32, const int* map, const void* D, 
   SXM_Q1(4)
   SXM_Q1(5)
 #undef SXM_Q1
+  return hipErrorInvalidValue;
+}
+
+int64_t bf16_tangent_elems(int order, int ds, int64_t nslots)
+{
+  const int64_t nq = int64_t(order + 1) * (order + 1) * (order + 1);
+  return nslots * ((ds * nq * 2 + 15) / 16 * 8);
+}
+
+hipError_t tangent_to_bf16(int order, int ds, int src_f32, const void* src, uint16_t* dst, int64_t nslots,
+                           hipStream_t s)
+{
+  const int64_t n    = nslots * ds * int64_t(order + 1) * (order + 1) * (order + 1);
+  const unsigned grid = blocks_for(n);
+#define SXM_BF(NN, DSV)                                                                                             \
+  if (order + 1 == NN && ds == DSV) {                                                                               \
+    if (src_f32)                                                                                                    \
+      hipLaunchKernelGGL((k_tangent_to_bf16<float, DSV, NN * NN * NN>), dim3(grid), dim3(kThreads), 0, s,          \
+                         static_cast<const float*>(src), dst, nslots);                                              \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_tangent_to_bf16<double, DSV, NN * NN * NN>), dim3(grid), dim3(kThreads), 0, s,         \
+                         static_cast<const double*>(src), dst, nslots);                                             \
+    return hipGetLastError();                                                                                       \
+  }
+  SXM_BF(2, 45)
+  SXM_BF(3, 45)
+  SXM_BF(4, 45)
+  SXM_BF(5, 45)
+  SXM_BF(2, 81)
+  SXM_BF(3, 81)
+  SXM_BF(4, 81)
+  SXM_BF(5, 81)
+#undef SXM_BF
   return hipErrorInvalidValue;
 }
 

@@ -79,6 +79,12 @@ hipError_t pcg_step_xr(double* x, double* r, double* Ad, const double* d, double
 // grid gn; tables: the integral's B, G, Bw, Gw and GL points
 hipError_t q1_galerkin(int order, int tprec_f32, const int* map, const void* D, int64_t nslots, const double* tables,
                       Grid3 gn, Grid3 gv, double* S, hipStream_t s);
+// a BF16 copy of a stored tangent (ds = 45 or 81 entries per point, FP64 or
+// FP32 source with src_f32) in the BF16 layout; dst holds
+// bf16_tangent_elems(order, ds, nslots) entries
+int64_t    bf16_tangent_elems(int order, int ds, int64_t nslots);
+hipError_t tangent_to_bf16(int order, int ds, int src_f32, const void* src, uint16_t* dst, int64_t nslots,
+                           hipStream_t s);
 // zero the stencil entries with a constrained row or column
 hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s);
 // *flag = 1 (atomic max) when a listed constrained DOF has an unconstrained

@@ -251,6 +251,76 @@ def test_non_cubic_mesh_levels_and_vcycle_match_numpy(gpu_lib):
         assert np.abs(z.numpy() - zref).max() <= 1e-10 * np.abs(zref).max()
 
 
+def test_bf16_smoother_level0_is_the_bf16_linearization(gpu_lib):
+    # the BF16 copy of an FP64 tangent rounds the same doubles as a BF16
+    # linearization, so the level-0 operator equals that form's J.v bit for bit
+    n, p = 4, 3
+    m = sx.Mesh.cuboid(n, n, n)
+    m.perturb(42, 0.2)
+    s = sx.Space(m, p, 3, 1)
+    ess = s.boundary_dofs(0b000001, 0x7)
+    U = sx.DeviceVector.from_numpy(W.nodal_field(s.nnodes, 3, 3, 2000, -0.0002, 0.0002, 1))
+    grads = []
+    for tprec in (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_BF16):
+        f = sx.Functional(s, [s])
+        f.set_option(sx.capi.SX_OPT_SCATTER, sx.capi.SX_SCATTER_COLORED)
+        f.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
+        f.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+        _, g = f(0.0, U, wrt=0)
+        grads.append((f, g))
+    mg = sx.Multigrid(s)
+    mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
+    mg.setup(grads[0][1], ess)
+    x = np.random.default_rng(4).standard_normal(s.ndof)
+    x[ess] = 0.0
+    y = sx.DeviceVector(s.ndof)
+    mg.level_apply(0, sx.DeviceVector.from_numpy(x), y)
+    yref = grads[1][1](sx.DeviceVector.from_numpy(x)).numpy()
+    yref[ess] = 0.0
+    assert np.array_equal(y.numpy(), yref)
+    # and differs from the FP64 operator at BF16 rounding
+    y64 = grads[0][1](sx.DeviceVector.from_numpy(x)).numpy()
+    y64[ess] = 0.0
+    err = np.abs(y.numpy() - y64).max() / np.abs(y64).max()
+    assert 1e-6 < err < 1e-2
+
+
+def test_bf16_smoother_vcycle_is_symmetric_and_newton_matches(gpu_lib):
+    m, s, f, ess, grad = linearized(8, 2)
+    mg = sx.Multigrid(s)
+    mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
+    mg.setup(grad, ess)
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    Mu, Mv = sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof)
+    mg.apply(sx.DeviceVector.from_numpy(u), Mu)
+    mg.apply(sx.DeviceVector.from_numpy(v), Mv)
+    Mu, Mv = Mu.numpy(), Mv.numpy()
+    assert abs(v @ Mu - u @ Mv) <= 1e-10 * abs(v @ Mu)
+    assert u @ Mu > 0 and v @ Mv > 0
+    # Newton on the 5 % compression path in mixed precision with the BF16 smoother
+    ess0 = s.boundary_dofs(0b000001, 0x7)
+    ess1 = s.boundary_dofs(0b000010, 0x1)
+    essc = np.union1d(ess0, ess1).astype(np.int32)
+    final = np.where(np.isin(essc, ess1), -0.05, 0.0)
+    xs = []
+    for use_mg in (False, True):
+        fm = sx.Functional(s, [s])
+        fm.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
+        mgm = None
+        if use_mg:
+            fm.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, sx.capi.SX_PREC_F32)
+            fm.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, sx.capi.SX_PREC_F32)
+            mgm = sx.Multigrid(s)
+            mgm.set_smoother_precision(sx.capi.SX_PREC_BF16)
+        fm.set_multigrid(mgm)
+        x = sx.DeviceVector(s.ndof)
+        reps = sx.quasistatic_solve(fm, x, essc, final, 2, rel_tol=1e-10, lin_rel_tol=1e-8, lin_max_iter=50000)
+        assert all(r["converged"] for r in reps)
+        xs.append(x.numpy())
+    assert np.abs(xs[1] - xs[0]).max() <= 1e-7 * np.abs(xs[0]).max()
+
+
 def test_vcycle_is_symmetric_and_positive(gpu_lib):
     m, s, f, ess, grad = linearized(8, 2)
     mg = sx.Multigrid(s)
