@@ -412,10 +412,11 @@ int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double 
  * eigenvalue bounds lag).  Default 1: a setup per linearization, as the
  * reference's preconditioners are set up on each SetOperator. */
 int sx_mg_set_reuse(sx_multigrid* M, int linearizations);
-/* how the vertex-level (level 1) operator is formed: 0 (default) element by
+/* how the Galerkin operators are formed: 0 (default) level 1 element by
  * element from the stored tangent when every trial-dependent part of J is a
  * domain integral with a GRAD_SYM tangent in FP64 or FP32 and the essential
- * DOFs are vertex-compatible, else by probing; 1 always by probing */
+ * DOFs are vertex-compatible (else by probing), and the levels above by the
+ * stencil triple product; 1 every level by probing */
 int sx_mg_set_galerkin(sx_multigrid* M, int mode);
 /* SX_PREC_BF16: the level-0 applications of the V-cycle (smoothing, power
  * iteration, probing) use BF16 copies of the stored tangents, made at each

@@ -1783,6 +1783,8 @@ void mg_coarse_inverse(sx_multigrid* M, hipStream_t s)
   M->coarse_inv.upload(X);
 }
 
+// With galerkin = 0 the levels above the vertex grid are formed by the
+// stencil triple product (sxm::stencil_rap); galerkin = 1 probes every level.
 // The vertex-level operator can come element by element from the stored
 // tangent (sxm::q1_galerkin) when every part of J that depends on the trial
 // field is a domain integral with a GRAD_SYM tangent in FP64 or FP32 and the
@@ -1866,6 +1868,13 @@ void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
       SX_HIP(sxm::stencil_mask(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), s));
       SX_HIP(sxm::stencil_dinv(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), C.dinv.as<double>(), s));
       M->element_setups++;
+      continue;
+    }
+    if (l >= 1 && M->galerkin == 0) {
+      // vertex levels: the triple product of the stencils directly
+      SX_HIP(sxm::stencil_rap(F.S.as<double>(), F.g, C.S.as<double>(), C.g, s));
+      SX_HIP(sxm::stencil_mask(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), s));
+      SX_HIP(sxm::stencil_dinv(C.S.as<double>(), C.g, C.mask.as<uint8_t>(), C.dinv.as<double>(), s));
       continue;
     }
     for (int cz = 0; cz < 3; cz++)

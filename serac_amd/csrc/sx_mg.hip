@@ -499,6 +499,62 @@ __global__ void __launch_bounds__(kThreads) k_tangent_to_bf16(const SRC* src, ui
   }
 }
 
+// Galerkin product of a vertex-level stencil with trilinear transfers:
+// Sc[I][O] = sum_i sum_j w_I(i) Sf[i][j - i] w_{I+O}(j) over fine vertices i
+// within one of 2I and j within one of 2(I + O) (weights 1 on the coinciding
+// vertex, 1/2 beside it), one thread per (coarse vertex, offset) and its 3x3
+// block.  Constrained fine rows and columns of Sf are zero; the coarse ones
+// are zeroed afterwards (stencil_mask).
+__global__ void __launch_bounds__(kThreads) k_stencil_rap(const double* Sf, Grid3 f, double* Sc, Grid3 c)
+{
+  const int64_t nvf = nverts(f), nvc = nverts(c);
+  for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nvc * 27; t += int64_t(gridDim.x) * kThreads) {
+    const int64_t I = t % nvc;
+    const int     O = int(t / nvc);
+    int           Ic[3];
+    coords(c, I, Ic[0], Ic[1], Ic[2]);
+    const int Od[3] = {O % 3 - 1, (O / 3) % 3 - 1, O / 9 - 1};
+    // per direction: the (i, j) pairs with |i - 2I| <= 1, |j - 2(I + O)| <= 1,
+    // |j - i| <= 1, inside the fine grid, and their weight products
+    int    pi[3][9], pj[3][9], np[3];
+    double pw[3][9];
+    bool   empty = false;
+    for (int d = 0; d < 3; d++) {
+      np[d]        = 0;
+      const int J  = Ic[d] + Od[d];
+      if (J < 0 || J >= c.n[d]) {
+        empty = true;
+        break;
+      }
+      for (int a = -1; a <= 1; a++) {
+        const int i = 2 * Ic[d] + a;
+        if (i < 0 || i >= f.n[d]) continue;
+        for (int b = -1; b <= 1; b++) {
+          const int j = 2 * J + b;
+          if (j < 0 || j >= f.n[d] || j - i < -1 || j - i > 1) continue;
+          pi[d][np[d]] = i;
+          pj[d][np[d]] = j;
+          pw[d][np[d]] = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0);
+          np[d]++;
+        }
+      }
+    }
+    double acc[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (!empty) {
+      for (int kz = 0; kz < np[2]; kz++)
+        for (int ky = 0; ky < np[1]; ky++)
+          for (int kx = 0; kx < np[0]; kx++) {
+            const double  w  = pw[0][kx] * pw[1][ky] * pw[2][kz];
+            const int64_t i  = (int64_t(pi[2][kz]) * f.n[1] + pi[1][ky]) * f.n[0] + pi[0][kx];
+            const int     o  = (pj[2][kz] - pi[2][kz] + 1) * 9 + (pj[1][ky] - pi[1][ky] + 1) * 3 + (pj[0][kx] - pi[0][kx] + 1);
+            const double* sf = Sf + int64_t(o) * 9 * nvf + i;
+            for (int k = 0; k < 9; k++) acc[k] += w * sf[int64_t(k) * nvf];
+          }
+    }
+    for (int k = 0; k < 9; k++) Sc[(int64_t(O) * 9 + k) * nvc + I] = acc[k];
+  }
+}
+
 // zero the stencil entries whose row or column is constrained
 __global__ void __launch_bounds__(kThreads) k_stencil_mask(double* S, Grid3 g, const uint8_t* mask)
 {
@@ -761,6 +817,13 @@ hipError_t tangent_to_bf16(int order, int ds, int src_f32, const void* src, uint
   SXM_BF(5, 81)
 #undef SXM_BF
   return hipErrorInvalidValue;
+}
+
+hipError_t stencil_rap(const double* Sf, Grid3 f, double* Sc, Grid3 c, hipStream_t s)
+{
+  const int64_t n = int64_t(c.n[0]) * c.n[1] * c.n[2] * 27;
+  hipLaunchKernelGGL(k_stencil_rap, dim3(blocks_for(n)), dim3(kThreads), 0, s, Sf, f, Sc, c);
+  return hipGetLastError();
 }
 
 hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s)

@@ -85,6 +85,10 @@ hipError_t q1_galerkin(int order, int tprec_f32, const int* map, const void* D, 
 int64_t    bf16_tangent_elems(int order, int ds, int64_t nslots);
 hipError_t tangent_to_bf16(int order, int ds, int src_f32, const void* src, uint16_t* dst, int64_t nslots,
                            hipStream_t s);
+// Sc = P^T Sf P for the trilinear transfer from coarse grid c to fine grid f
+// (writes every coarse entry; constrained coarse rows/columns still need
+// stencil_mask)
+hipError_t stencil_rap(const double* Sf, Grid3 f, double* Sc, Grid3 c, hipStream_t s);
 // zero the stencil entries with a constrained row or column
 hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s);
 // *flag = 1 (atomic max) when a listed constrained DOF has an unconstrained
