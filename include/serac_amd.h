@@ -408,8 +408,8 @@ int sx_mg_create(const sx_space* s, sx_multigrid** out);
 int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double smoothing_range, int power_iterations);
 /* inside one sx_newton_solve / sx_quasistatic_solve call, one hierarchy
  * serves up to `linearizations` consecutive linear solves (the smoother's
- * level-0 operator is always the current J; the coarse levels, diagonals and
- * eigenvalue bounds lag).  Default 1: a setup per linearization, as the
+ * level-0 operator is always the current J unless it uses BF16 copies; the
+ * coarse levels, diagonals, eigenvalue bounds and BF16 copies lag).  Default 1: a setup per linearization, as the
  * reference's preconditioners are set up on each SetOperator. */
 int sx_mg_set_reuse(sx_multigrid* M, int linearizations);
 /* how the Galerkin operators are formed: 0 (default) level 1 element by

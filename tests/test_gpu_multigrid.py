@@ -514,6 +514,10 @@ def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
     with pytest.raises(sx.SeracAmdError):
         mg.configure(fine_degree=0)
     with pytest.raises(sx.SeracAmdError):
+        mg.set_galerkin(2)
+    with pytest.raises(sx.SeracAmdError):
+        mg.set_smoother_precision(sx.capi.SX_PREC_F32)
+    with pytest.raises(sx.SeracAmdError):
         mg.apply(sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof))  # before a setup
     f = sx.Functional(s1, [s1])
     with pytest.raises(sx.SeracAmdError):
