@@ -385,6 +385,17 @@ int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t 
 int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
                  double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
 
+/* ShapeAwareFunctional (fn/shape_aware_functional.hpp:292-577): the domain
+ * integrals of f are evaluated on the configuration X + p, p a shape
+ * displacement in f's test space (a 3-component device L-vector, borrowed; it
+ * must stay valid while f is evaluated; NULL restores X).  At each quadrature
+ * point dx/dxi = dX/dxi + dp/dxi, which is the reference's ShapeCorrection
+ * (trial gradients du/dX (I + dp/dX)^-1, fluxes f (I + dp/dX)^-T det, sources
+ * times det).  Derivatives are with respect to the trial arguments; a form
+ * with boundary integrals refuses evaluation while p is set.  Setting p
+ * invalidates earlier linearizations. */
+int sx_form_set_shape_displacement(sx_form* f, const double* p);
+
 /* ---- geometric multigrid preconditioner ----
  * Stands where the reference's linear solver options offer hypre's AMG
  * (num/equation_solver.cpp:1022-1051, Preconditioner::HypreAMG): for a

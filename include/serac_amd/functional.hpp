@@ -809,6 +809,48 @@ class Functional<test(trials...)> {
 };
 
 // ---------------------------------------------------------------------------
+// ShapeAwareFunctional<shape, test(trials...)> (fn/shape_aware_functional.hpp:292-577)
+// ---------------------------------------------------------------------------
+/// The domain integrals evaluated on the configuration X + p of a shape
+/// displacement p: with J = I + dp/dX, trial gradients become du/dX J^-1,
+/// fluxes f J^-T det J and sources s det J (ShapeCorrection,
+/// fn/shape_aware_functional.hpp:37-125).  p lives in the test space (a
+/// 3-component H1 space of the mesh); differentiate_wrt applies to the trial
+/// arguments (not to p), and boundary integrals are not shape-aware here.
+template <typename shape, typename T>
+class ShapeAwareFunctional;
+
+template <typename shape, typename test, typename... trials>
+class ShapeAwareFunctional<shape, test(trials...)> {
+ public:
+  using Gradient = typename Functional<test(trials...)>::Gradient;
+
+  ShapeAwareFunctional(const FiniteElementSpace* shape_fes, const FiniteElementSpace* test_fes,
+                       std::array<const FiniteElementSpace*, sizeof...(trials)> trial_fes)
+      : f_(test_fes, trial_fes)
+  {
+    if (shape_fes->handle() != test_fes->handle())
+      throw Error(SX_ERR_UNSUPPORTED, "the shape displacement space must be the test space");
+  }
+  template <int dim, int... args, typename Integrand>
+  void AddDomainIntegral(Dimension<dim> d, DependsOn<args...> dep, const Integrand& integrand, const Mesh& mesh)
+  {
+    f_.AddDomainIntegral(d, dep, integrand, mesh);
+  }
+  /// operator()(t, shape_displacement, args...) (fn/shape_aware_functional.hpp:531-560)
+  template <typename... T>
+  decltype(auto) operator()(double t, const Vector& p, const T&... args)
+  {
+    detail::check(sx_form_set_shape_displacement(f_.handle(), p.Data()));
+    return f_(t, args...);
+  }
+  sx_form* handle() const { return f_.handle(); }
+
+ private:
+  Functional<test(trials...)> f_;
+};
+
+// ---------------------------------------------------------------------------
 // quantities of interest: Functional<double(trial)> (fn/functional_qoi.inl:59-477)
 // ---------------------------------------------------------------------------
 namespace qoi {

@@ -656,6 +656,12 @@ class Functional:
         capi.call("sx_form_set_preconditioner", self.h, learned.h if learned is not None else None, mode)
         self._learned = learned  # the form borrows it
 
+    def set_shape_displacement(self, p):
+        """ShapeAwareFunctional: evaluate the domain integrals on X + p (p a DeviceVector of the test space, kept
+        alive by the form; None restores X)."""
+        capi.call("sx_form_set_shape_displacement", self.h, p.ptr if p is not None else None)
+        self._shape = p
+
     def set_multigrid(self, mg):
         """CG solves of newton / quasistatic_solve preconditioned by the multigrid V-cycle (None: Jacobi)."""
         capi.call("sx_form_set_multigrid", self.h, mg.h if mg is not None else None)

@@ -147,6 +147,7 @@ SIGNATURES = {
     "sx_mg_gmres_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                                  _int, _vp]),
     "sx_form_set_multigrid": (_int, [_vp, _vp]),
+    "sx_form_set_shape_displacement": (_int, [_vp, _vp]),
     "sx_mg_destroy": (_int, [_vp]),
     "sx_form_set_preconditioner": (_int, [_vp, _vp, _int]),
     "sx_fcg_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,

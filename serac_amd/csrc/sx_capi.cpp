@@ -296,6 +296,7 @@ struct sx_form {
   int                                    gmres_restart = 50;
   sx_learned*                            learned      = nullptr;        // borrowed
   sx_multigrid*                          mg           = nullptr;        // borrowed: multigrid-PCG in the Newton loops
+  const double*                          shape        = nullptr;        // borrowed: shape displacement (device)
   int                                    learn_mode   = 0;              // SX_PRECOND_*
   // solver scratch
   DeviceBuffer part, scal;
@@ -351,6 +352,7 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   P.tprec    = I.tprec;
   P.vprec    = sxk::PREC_F64;
   P.tables   = element_tables(s->order + 1);
+  P.shape    = f->shape;
   for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = I.params[k];
   if (I.qdata) {
     P.qdata        = I.qdata->d.as<double>();
@@ -1302,6 +1304,8 @@ int sx_qdata_destroy(sx_qdata* q)
 int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out)
 {
   return api([&] {
+    SX_REQUIRE(!f || !f->shape || f->bintegrals.empty(), SX_ERR_UNSUPPORTED,
+               "a shape displacement applies to domain integrals only (this form has boundary integrals)");
     SX_REQUIRE(f && inputs && out, SX_ERR_ARG, "null argument");
     SX_REQUIRE(wrt >= -1 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     for (size_t i = 0; i < f->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
@@ -2994,6 +2998,20 @@ int sx_mg_gmres_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int6
   });
   if (report) *report = rep;
   return rc;
+}
+
+int sx_form_set_shape_displacement(sx_form* f, const double* p)
+{
+  return api([&] {
+    SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(!p || f->test->nc == 3, SX_ERR_UNSUPPORTED,
+               "a shape displacement needs a 3-component test space (it is a field of that space)");
+    for (const sx_space* tr : f->trials)
+      SX_REQUIRE(!p || tr->nnodes == f->test->nnodes, SX_ERR_UNSUPPORTED,
+                 "a shape displacement needs trial spaces on the test space's nodes");
+    f->shape = p;
+    for (auto& g : f->grads) g->valid = false;  // linearizations at another shape are stale
+  });
 }
 
 int sx_form_set_multigrid(sx_form* f, sx_multigrid* M)

@@ -360,6 +360,76 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
     }
   }
 
+  // ---- shape displacement (ShapeAwareFunctional, fn/shape_aware_functional.hpp:37-50):
+  // the parent gradient of p at this lane's quadrature point, added to the
+  // geometric Jacobian below, so dx/dxi = dX/dxi + dp/dxi = (I + dp/dX) dX/dxi.
+  // P.shape is the same for every lane, so the barriers inside are uniform.
+  double gps[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) gps[k] = 0.0;
+  if constexpr (NC == 3) {
+    if (P.shape != nullptr) {
+      if (active) {
+        const int64_t nd = P.map[size_t(slot) * NQ + li];
+#pragma unroll
+        for (int c = 0; c < 3; c++) S_[c * NQ + li] = P.shape[lidx(nd, c, 3, P.nnodes, P.ordering)];
+      }
+      lds_barrier();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double* pp = S_ + c * NQ + (i2 * N + i1) * N;
+          double        s0 = 0.0, s1 = 0.0;
+#pragma unroll
+          for (int x = 0; x < N; x++) {
+            s0 += tB[i0 * N + x] * pp[x];
+            s1 += tG[i0 * N + x] * pp[x];
+          }
+          S_[1 * R + c * NQ + li] = s0;
+          S_[2 * R + c * NQ + li] = s1;
+        }
+      }
+      lds_barrier();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double* a10 = S_ + 1 * R + c * NQ;
+          const double* a11 = S_ + 2 * R + c * NQ;
+          double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+          for (int y = 0; y < N; y++) {
+            const int    idx = (i2 * N + y) * N + i0;
+            const double b = tB[i1 * N + y], g = tG[i1 * N + y];
+            s0 += b * a10[idx];
+            s1 += b * a11[idx];
+            s2 += g * a10[idx];
+          }
+          S_[3 * R + c * NQ + li] = s0;
+          S_[4 * R + c * NQ + li] = s1;
+          S_[5 * R + c * NQ + li] = s2;
+        }
+      }
+      lds_barrier();
+      if (active) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double* a20 = S_ + 3 * R + c * NQ;
+          const double* a21 = S_ + 4 * R + c * NQ;
+          const double* a22 = S_ + 5 * R + c * NQ;
+#pragma unroll
+          for (int z = 0; z < N; z++) {
+            const int    idx = (z * N + i1) * N + i0;
+            const double b = tB[i2 * N + z], g = tG[i2 * N + z];
+            gps[c * 3 + 0] += b * a21[idx];
+            gps[c * 3 + 1] += b * a22[idx];
+            gps[c * 3 + 2] += g * a20[idx];
+          }
+        }
+      }
+      lds_barrier();  // the gather below overwrites the regions read above
+    }
+  }
+
   // ---- gather
   int64_t node = 0;
   if (active) {
@@ -435,6 +505,8 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
 
     double J[9], Ji[9];
     trilinear_jacobian(sxv + el * 24, txq[i0], txq[i1], txq[i2], J);
+#pragma unroll
+    for (int k = 0; k < 9; k++) J[k] += gps[k];
     inv3(J, Ji);
     const double dJ = det3(J);
     double       src[NC], fp[3 * NC];

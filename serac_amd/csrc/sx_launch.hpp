@@ -50,6 +50,8 @@ struct ElemLaunch {
   const int64_t* csr_row;      // assembly: CSR row offsets (test DOFs)
   const int32_t* csr_col;      // assembly: sorted column ids per row (trial DOFs)
   double*        csr_val;      // assembly: values, accumulated into
+  const double*  shape;        // residual/linearize: shape displacement p (3-component L-vector of the
+                               // test space) or null; the geometry becomes X + p
   double        params[16];
   double        t;
   hipStream_t   stream;

@@ -125,6 +125,17 @@ int main()
     CHECK(A->Height() == ndof && A->NumNonZeroElems() > 0 && relerr(y.ToHost(), Jv) < 1e-12,
           "assemble(): CSR SpMV vs J.v %.2e (nnz %lld)", relerr(y.ToHost(), Jv), (long long)A->NumNonZeroElems());
   }
+  {
+    // ShapeAwareFunctional: a zero shape displacement reproduces the residual
+    ShapeAwareFunctional<space, space(space)> sres(&fes, &fes, {&fes});
+    sres.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                           solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                               mat, GeometricNonlinearities::On),
+                           mesh);
+    Vector p0(std::vector<double>(size_t(ndof), 0.0));
+    std::vector<double> Rs = sres(0.0, p0, u).ToHost();
+    CHECK(relerr(Rs, R_ref) < 1e-12, "ShapeAwareFunctional with p = 0 vs oracle residual: %.2e", relerr(Rs, R_ref));
+  }
   std::vector<double> diag = dR.AssembleDiagonal().ToHost();
   double              dmin = 1e300;
   for (double v : diag) dmin = std::fmin(dmin, v);
