@@ -391,9 +391,10 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
  * must stay valid while f is evaluated; NULL restores X).  At each quadrature
  * point dx/dxi = dX/dxi + dp/dxi, which is the reference's ShapeCorrection
  * (trial gradients du/dX (I + dp/dX)^-1, fluxes f (I + dp/dX)^-T det, sources
- * times det).  Derivatives are with respect to the trial arguments; a form
- * with boundary integrals refuses evaluation while p is set.  Setting p
- * invalidates earlier linearizations. */
+ * times det); boundary integrals of 3-component fields see the faces of
+ * X + p (position, tangents and area weight, compute_boundary_area_correction
+ * of fn/shape_aware_functional.hpp:156-180).  Derivatives are with respect to
+ * the trial arguments.  Setting p invalidates earlier linearizations. */
 int sx_form_set_shape_displacement(sx_form* f, const double* p);
 
 /* ---- geometric multigrid preconditioner ----

@@ -814,9 +814,9 @@ class Functional<test(trials...)> {
 /// The domain integrals evaluated on the configuration X + p of a shape
 /// displacement p: with J = I + dp/dX, trial gradients become du/dX J^-1,
 /// fluxes f J^-T det J and sources s det J (ShapeCorrection,
-/// fn/shape_aware_functional.hpp:37-125).  p lives in the test space (a
-/// 3-component H1 space of the mesh); differentiate_wrt applies to the trial
-/// arguments (not to p), and boundary integrals are not shape-aware here.
+/// fn/shape_aware_functional.hpp:37-125; boundary integrals see the faces of
+/// X + p).  p lives in the test space (a 3-component H1 space of the mesh);
+/// differentiate_wrt applies to the trial arguments (not to p).
 template <typename shape, typename T>
 class ShapeAwareFunctional;
 
@@ -836,6 +836,11 @@ class ShapeAwareFunctional<shape, test(trials...)> {
   void AddDomainIntegral(Dimension<dim> d, DependsOn<args...> dep, const Integrand& integrand, const Mesh& mesh)
   {
     f_.AddDomainIntegral(d, dep, integrand, mesh);
+  }
+  template <typename... A>
+  void AddBoundaryIntegral(A&&... a)
+  {
+    f_.AddBoundaryIntegral(std::forward<A>(a)...);
   }
   /// operator()(t, shape_displacement, args...) (fn/shape_aware_functional.hpp:531-560)
   template <typename... T>

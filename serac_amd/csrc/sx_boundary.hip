@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(64) k_face(const FaceLaunch P)
   __shared__ T      su[NC * NF];
   __shared__ T      ssrc[NC * NF];
   __shared__ double sD[(MODE == FMODE_DIAG || MODE == FMODE_ASSEMBLE) ? DS * NF : 1];
+  __shared__ double sp[3 * NF];  // face trace of the shape displacement
 
   const int     k    = threadIdx.x;
   const int64_t slot = P.slot_begin + blockIdx.x;
@@ -135,6 +136,10 @@ __global__ void __launch_bounds__(64) k_face(const FaceLaunch P)
   }
   if constexpr (MODE == FMODE_DIAG || MODE == FMODE_ASSEMBLE) {
     for (int t = k; t < DS * NF; t += 64) sD[t] = P.D[(slot * DS + t / NF) * NF + t % NF];
+  }
+  if (P.shape != nullptr && k < NF) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) sp[c * NF + k] = P.shape[lidx_b(fmap[k], c, 3, P.nnodes, P.ordering)];
   }
   __syncthreads();
   const double* tB  = stab;
@@ -173,6 +178,25 @@ __global__ void __launch_bounds__(64) k_face(const FaceLaunch P)
     } else {
       double X[3], dX[6];
       face_geometry(P.corners + slot * 12, txq[q1], txq[q2], X, dX);
+      if (P.shape != nullptr) {
+        // the configuration X + p (fn/shape_aware_functional.hpp:150-170)
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          double v = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+          for (int b = 0; b < N; b++)
+#pragma unroll
+            for (int a = 0; a < N; a++) {
+              const double pp = sp[c * NF + b * N + a];
+              v += tB[q1 * N + a] * tB[q2 * N + b] * pp;
+              g1 += tG[q1 * N + a] * tB[q2 * N + b] * pp;
+              g2 += tB[q1 * N + a] * tG[q2 * N + b] * pp;
+            }
+          X[c] += v;
+          dX[c * 2 + 0] += g1;
+          dX[c * 2 + 1] += g2;
+        }
+      }
       const double t1[3] = {dX[0], dX[2], dX[4]}, t2[3] = {dX[1], dX[3], dX[5]};
       double       n0[3];
       cross3(t1, t2, n0);

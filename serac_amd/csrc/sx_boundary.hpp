@@ -27,6 +27,7 @@ struct FaceLaunch {
   const int32_t* csr_col;
   double*        csr_val;
   double         params[16];
+  const double*  shape;          // residual/linearize: shape displacement (3-component L-vector) or null
   hipStream_t    stream;
 };
 

@@ -587,6 +587,7 @@ sxk::FaceLaunch face_launch(const sx_form* f, const BoundaryIntegral& B, int mod
   P.nnodes   = s->nnodes;
   P.D        = B.d_tangent.as<double>();
   P.tables   = element_tables(s->order + 1);
+  P.shape    = B.nc == 3 ? f->shape : nullptr;
   for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = B.params[k];
   P.stream = f->stream;
   return P;
@@ -1304,8 +1305,6 @@ int sx_qdata_destroy(sx_qdata* q)
 int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out)
 {
   return api([&] {
-    SX_REQUIRE(!f || !f->shape || f->bintegrals.empty(), SX_ERR_UNSUPPORTED,
-               "a shape displacement applies to domain integrals only (this form has boundary integrals)");
     SX_REQUIRE(f && inputs && out, SX_ERR_ARG, "null argument");
     SX_REQUIRE(wrt >= -1 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     for (size_t i = 0; i < f->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
