@@ -189,7 +189,8 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
              "f64_mg": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
              "mixed_f32_mg": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
              "mixed_bf16_mg": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
-             "mixed_f32_mg_bf16smooth": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
+             "mixed_f32_mg_bf16smooth": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
+             "mixed_f32_mg_bf16smooth32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
     solve, finals = {}, {}
     for mode in precisions.split(","):
         tprec, arith = modes[mode]
@@ -223,12 +224,14 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
                               smoothing_range=args.mg_range)
             mg.set_reuse(args.mg_reuse)
             form.set_multigrid(mg)
-        elif mode.endswith("_mg_bf16smooth"):
+        elif mode.endswith("_mg_bf16smooth") or mode.endswith("_mg_bf16smooth32"):
             # multigrid whose level-0 smoothing reads BF16 copies of the FP32 tangent
             mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
                               smoothing_range=args.mg_range)
             mg.set_reuse(args.mg_reuse)
             mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
+            if mode.endswith("32"):
+                mg.set_smoother_vectors(sx.capi.SX_PREC_F32)
             form.set_multigrid(mg)
         x = sx.DeviceVector(space.ndof)
         sx.capi.call("sx_device_synchronize")
@@ -267,7 +270,8 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
             solve[mode]["multigrid"] = {
                 "fine_degree": args.mg_fine_degree, "coarse_degree": args.mg_coarse_degree,
                 "smoothing_range": args.mg_range, "setups": info["setups"], "reuse": args.mg_reuse,
-                "smoother_tangent": "bf16 copy" if mode.endswith("_bf16smooth") else "the form's",
+                "smoother_tangent": "bf16 copy" if "_bf16smooth" in mode else "the form's",
+                "smoother_vectors": "fp32" if mode.endswith("smooth32") else "fp64",
                 "levels": [lv["dims"] for lv in info["levels"]],
             }
             form.set_multigrid(None)
@@ -332,7 +336,8 @@ def main():
                          "mixed_f32_learned (mixed_f32 with flexible CG and the learned U-Net preconditioner, "
                          "trained first on pairs from the undeformed linearization; training time included), "
                          "f64_mg / mixed_f32_mg / mixed_bf16_mg (f64 / mixed_f32 / mixed_bf16 with multigrid-CG), "
-                         "mixed_f32_mg_bf16smooth (mixed_f32_mg with BF16 tangent copies in the level-0 smoothing)")
+                         "mixed_f32_mg_bf16smooth (mixed_f32_mg with BF16 tangent copies in the level-0 smoothing), "
+                         "mixed_f32_mg_bf16smooth32 (and FP32 vectors in the level-0 smoothing)")
     ap.add_argument("--mg-fine-degree", type=int, default=2, help="Chebyshev degree on the order-p level")
     ap.add_argument("--mg-coarse-degree", type=int, default=2, help="Chebyshev degree on the vertex levels")
     ap.add_argument("--mg-range", type=float, default=12.0, help="Chebyshev smoothing range")
@@ -498,11 +503,12 @@ def main():
         form = make_form(args.scatter)
     solve_full = None
     if not args.solve and qf == "neohookean" and not args.no_default_solve:
-        # the full-size load path in mixed precision with multigrid-CG and
-        # BF16 smoothing (about 10 s at 64^3); its FP64 Jacobi-CG baseline
-        # takes minutes and runs with --solve --solve-precision f64
+        # the full-size load path in mixed precision with multigrid-CG, its
+        # level-0 smoothing on FP32 vectors and BF16 tangent copies (about 8 s
+        # at 64^3); its FP64 Jacobi-CG baseline takes minutes and runs with
+        # --solve --solve-precision f64
         del form
-        solve_full = run_solve(sx, args, space, make_form, "mixed_f32_mg_bf16smooth",
+        solve_full = run_solve(sx, args, space, make_form, "mixed_f32_mg_bf16smooth32",
                                f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
         solve_full["note"] = ("FP64 Jacobi-CG on this load path takes minutes and is not rerun by default: "
                               "bench.py --solve --solve-precision f64")
@@ -514,7 +520,7 @@ def main():
         if perturb > 0:
             smesh.perturb(42, perturb)
         sspace = sx.Space(smesh, p, 3)
-        solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32,f64_mg,mixed_f32_mg,mixed_f32_mg_bf16smooth",
+        solve = run_solve(sx, args, sspace, lambda sc: make_form(sc, sspace), "f64,mixed_f32,f64_mg,mixed_f32_mg,mixed_f32_mg_bf16smooth32",
                           f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs)")
         del sspace, smesh
 

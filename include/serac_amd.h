@@ -436,6 +436,10 @@ int sx_mg_set_galerkin(sx_multigrid* M, int mode);
  * own tangent is used); the Krylov solver's J.v keeps the form's tangent.
  * SX_PREC_F64 (default): the form's tangent. */
 int sx_mg_set_smoother_precision(sx_multigrid* M, int prec);
+/* SX_PREC_F32: the level-0 part of the V-cycle (Chebyshev vectors, the
+ * level-0 operator's J.v, the order-p transfers) runs on FP32 vectors
+ * (allocated at the next setup); SX_PREC_F64 (default): FP64 vectors */
+int sx_mg_set_smoother_vectors(sx_multigrid* M, int prec);
 /* number of levels and of setups so far */
 int sx_mg_info(const sx_multigrid* M, int* nlevels, int64_t* setups);
 /* setups whose level-1 operator was formed element by element */

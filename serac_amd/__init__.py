@@ -434,6 +434,10 @@ class Multigrid:
         """SX_PREC_BF16: level-0 applications of the V-cycle with BF16 tangent copies; SX_PREC_F64: the form's."""
         capi.call("sx_mg_set_smoother_precision", self.h, prec)
 
+    def set_smoother_vectors(self, prec):
+        """SX_PREC_F32: the level-0 part of the V-cycle on FP32 vectors; SX_PREC_F64 (default)."""
+        capi.call("sx_mg_set_smoother_vectors", self.h, prec)
+
     def set_galerkin(self, mode):
         """Level-1 operator: 0 element-wise from the stored tangent where possible, 1 by probing."""
         capi.call("sx_mg_set_galerkin", self.h, mode)

@@ -135,6 +135,7 @@ SIGNATURES = {
     "sx_mg_set_reuse": (_int, [_vp, _int]),
     "sx_mg_set_galerkin": (_int, [_vp, _int]),
     "sx_mg_set_smoother_precision": (_int, [_vp, _int]),
+    "sx_mg_set_smoother_vectors": (_int, [_vp, _int]),
     "sx_mg_element_setups": (_int, [_vp, ctypes.POINTER(ctypes.c_int64)]),
     "sx_mg_info": (_int, [_vp, ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_int64)]),
     "sx_mg_level": (_int, [_vp, _int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(_int),

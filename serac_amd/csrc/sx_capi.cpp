@@ -274,6 +274,8 @@ struct sx_multigrid {
   int                  galerkin = 0;  // level 1: 0 element-wise from the tangent where possible, 1 probing
   int                  smoother_bf16 = 0;  // level-0 applications with BF16 tangent copies
   bool                 lo_active     = false;
+  int                  smoother_f32  = 0;  // level-0 smoothing on FP32 vectors
+  DeviceBuffer         fb, fx, fr, fd, ft, fdinv;  // its FP32 level-0 vectors
   int64_t              element_setups = 0;
   DeviceBuffer         flag;
   int                  reuse  = 1;  // linearizations one hierarchy serves inside a Newton call
@@ -1837,6 +1839,11 @@ void mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
                              M->lv[size_t(l + 1)].mask.as<uint8_t>(), M->lv[size_t(l + 1)].g, s));
   compute_diag(g, ess, ness, M->diag.as<double>());
   SX_HIP(sxm::diag_inverse(M->diag.as<double>(), l0.n, ess, ness, l0.dinv.as<double>(), s));
+  if (M->smoother_f32) {
+    const size_t bytes = size_t(l0.n) * 4;
+    for (DeviceBuffer* b : {&M->fb, &M->fx, &M->fr, &M->fd, &M->ft, &M->fdinv}) b->allocate(bytes);
+    SX_HIP(sxm::to_f32(l0.dinv.as<double>(), M->fdinv.as<float>(), nullptr, l0.n, s));
+  }
   const bool elementwise = mg_elementwise(M, g, ess, ness);
   // BF16 tangent copies for the level-0 applications (smoothing, power
   // iteration, probing) when asked for and every stored tangent is FP64 or FP32
@@ -1964,6 +1971,54 @@ void mg_smooth(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, in
   }
 }
 
+// y = A_0 x on FP32 vectors (x is zero on the essential rows)
+void mg_level0_apply_f32(sx_multigrid* M, sx_grad* g, const float* x, float* y)
+{
+  apply_gradient(g, x, y, nullptr, nullptr, sxk::PREC_F32, false, M->lo_active);
+  SX_HIP(sxm::apply_mask(y, M->lv[0].mask.as<uint8_t>(), M->lv[0].n, g->form->stream));
+}
+
+// mg_smooth on level 0 with FP32 vectors (b = fb, x = fx)
+void mg_smooth0_f32(sx_multigrid* M, sx_grad* g, bool pre)
+{
+  MgLevel&      V = M->lv[0];
+  hipStream_t   s = g->form->stream;
+  const int64_t n = V.n;
+  float *b = M->fb.as<float>(), *x = M->fx.as<float>(), *r = M->fr.as<float>(), *d = M->fd.as<float>(),
+        *t = M->ft.as<float>(), *dinv = M->fdinv.as<float>();
+  if (pre) {
+    SX_HIP(sxm::cheb_init(b, dinv, V.theta, x, d, r, 0, 1, n, s));
+  } else {
+    mg_level0_apply_f32(M, g, x, t);
+    SX_HIP(sxm::residual(b, t, r, n, s));
+    SX_HIP(sxm::cheb_init(r, dinv, V.theta, x, d, r, 1, 0, n, s));
+  }
+  for (size_t k = 0; k < V.c1.size(); k++) {
+    mg_level0_apply_f32(M, g, d, t);
+    SX_HIP(sxm::cheb_step(r, t, d, dinv, V.c1[k], V.c2[k], x, n, s));
+  }
+  if (pre) {
+    mg_level0_apply_f32(M, g, d, t);
+    SX_HIP(sxm::sub_inplace(r, t, n, s));
+  }
+}
+
+void mg_vcycle(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, int l, const double* b, double* x);
+
+// level 0 of the V-cycle on FP32 vectors: fx = V(fb)
+void mg_vcycle0_f32(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness)
+{
+  hipStream_t s = g->form->stream;
+  MgLevel&    C = M->lv[1];
+  mg_smooth0_f32(M, g, true);
+  SX_HIP(sxm::fe_restrict(M->G, M->fr.as<float>(), C.b.as<double>(), C.mask.as<uint8_t>(), M->rt1.as<double>(),
+                          M->rt2.as<double>(), s));
+  mg_vcycle(M, g, ess, ness, 1, C.b.as<double>(), C.x.as<double>());
+  SX_HIP(sxm::fe_prolong_add(M->G, C.x.as<double>(), M->fx.as<float>(), s));
+  SX_HIP(sxm::apply_mask(M->fx.as<float>(), M->lv[0].mask.as<uint8_t>(), M->lv[0].n, s));
+  mg_smooth0_f32(M, g, false);
+}
+
 // x = V-cycle(b) on level l (x is overwritten)
 void mg_vcycle(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, int l, const double* b, double* x)
 {
@@ -1996,6 +2051,13 @@ void mg_apply(sx_multigrid* M, sx_grad* g, const int32_t* ess, int64_t ness, con
 {
   hipStream_t s  = g->form->stream;
   MgLevel&    l0 = M->lv[0];
+  if (M->smoother_f32 && M->fb.ptr) {
+    SX_HIP(sxm::to_f32(r, M->fb.as<float>(), l0.mask.as<uint8_t>(), l0.n, s));
+    mg_vcycle0_f32(M, g, ess, ness);
+    SX_HIP(sxm::to_f64(M->fx.as<float>(), z, l0.n, s));
+    SX_HIP(sxk::copy_rows(z, r, ess, ness, s));
+    return;
+  }
   SX_HIP(hipMemcpyAsync(l0.b.ptr, r, size_t(l0.n) * 8, hipMemcpyDeviceToDevice, s));
   SX_HIP(sxk::set_rows(l0.b.as<double>(), ess, ness, 0.0, s));
   mg_vcycle(M, g, ess, ness, 0, l0.b.as<double>(), z);
@@ -2879,6 +2941,18 @@ int sx_mg_set_smoother_precision(sx_multigrid* M, int prec)
     SX_REQUIRE(prec == SX_PREC_F64 || prec == SX_PREC_BF16, SX_ERR_ARG,
                "smoother precision is SX_PREC_F64 (the form's tangent) or SX_PREC_BF16");
     M->smoother_bf16 = prec == SX_PREC_BF16;
+  });
+}
+
+int sx_mg_set_smoother_vectors(sx_multigrid* M, int prec)
+{
+  return api([&] {
+    SX_REQUIRE(M, SX_ERR_ARG, "null multigrid");
+    SX_REQUIRE(prec == SX_PREC_F64 || prec == SX_PREC_F32, SX_ERR_ARG,
+               "smoother vectors are SX_PREC_F64 or SX_PREC_F32");
+    M->smoother_f32 = prec == SX_PREC_F32;
+    if (!M->smoother_f32)
+      for (DeviceBuffer* b : {&M->fb, &M->fx, &M->fr, &M->fd, &M->ft, &M->fdinv}) b->release();
   });
 }
 

@@ -36,7 +36,8 @@ __device__ __forceinline__ double fe_weight(const sxu::FEGrid& G, int d)
 // One direction of the separable restriction P^T = Rz Ry Rx: out has the
 // dims of in except nv[dir] along dir, out(i) = sum_d w(d) in(p i + d).  The
 // last pass (dir 2) applies the vertex mask.
-__global__ void __launch_bounds__(kThreads) k_fe_restrict_dir(sxu::FEGrid G, const double* in, Grid3 din, double* out,
+template <typename TI>
+__global__ void __launch_bounds__(kThreads) k_fe_restrict_dir(sxu::FEGrid G, const TI* in, Grid3 din, double* out,
                                                               Grid3 dout, int dir, const uint8_t* mask)
 {
   const int64_t n      = int64_t(dout.n[0]) * dout.n[1] * dout.n[2] * 3;
@@ -52,18 +53,19 @@ __global__ void __launch_bounds__(kThreads) k_fe_restrict_dir(sxu::FEGrid G, con
     const int i = o[dir];
     o[dir]      = G.p * i;
     // the input entry at the vertex's own node, then offsets along dir
-    const double* base = in + ((int64_t(o[2]) * din.n[1] + o[1]) * din.n[0] + o[0]) * 3 + c;
-    double        s    = 0.0;
+    const TI* base = in + ((int64_t(o[2]) * din.n[1] + o[1]) * din.n[0] + o[0]) * 3 + c;
+    double    s    = 0.0;
     for (int d = -G.p; d <= G.p; d++) {
       const int j = G.p * i + d;
       if (j < 0 || j >= din.n[dir]) continue;
-      s += fe_weight(G, d) * base[int64_t(d) * stride];
+      s += fe_weight(G, d) * double(base[int64_t(d) * stride]);
     }
     out[t] = s;
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_fe_prolong_add(sxu::FEGrid G, const double* ec, double* z)
+template <typename TO>
+__global__ void __launch_bounds__(kThreads) k_fe_prolong_add(sxu::FEGrid G, const double* ec, TO* z)
 {
   const int64_t nn = int64_t(G.nn[0]) * G.nn[1] * G.nn[2];
   for (int64_t t = int64_t(blockIdx.x) * kThreads + threadIdx.x; t < nn * 3; t += int64_t(gridDim.x) * kThreads) {
@@ -84,7 +86,7 @@ __global__ void __launch_bounds__(kThreads) k_fe_prolong_add(sxu::FEGrid G, cons
       const int64_t v = (int64_t(e[2] + bz) * G.nv[1] + (e[1] + by)) * G.nv[0] + (e[0] + bx);
       s += w * ec[v * 3 + c];
     }
-    z[t] += s;
+    z[t] = TO(double(z[t]) + s);
   }
 }
 
@@ -267,47 +269,61 @@ __global__ void __launch_bounds__(kThreads) k_coarsen_mask(const uint8_t* mf, Gr
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_cheb_init(const double* b, const double* dinv, double inv_theta,
-                                                        double* x, double* d, double* r, int x_add, int copy_r,
-                                                        int64_t n)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cheb_init(const T* b, const T* dinv, double inv_theta, T* x, T* d,
+                                                        T* r, int x_add, int copy_r, int64_t n)
 {
+  const T it = T(inv_theta);
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-    const double bi = b[i];
-    const double di = dinv[i] * bi * inv_theta;
-    d[i]            = di;
-    x[i]            = x_add ? x[i] + di : di;
+    const T bi = b[i];
+    const T di = dinv[i] * bi * it;
+    d[i]       = di;
+    x[i]       = x_add ? x[i] + di : di;
     if (copy_r) r[i] = bi;
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_cheb_step(double* r, const double* t, double* d, const double* dinv,
-                                                        double c1, double c2, double* x, int64_t n)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_cheb_step(T* r, const T* t, T* d, const T* dinv, double c1, double c2,
+                                                        T* x, int64_t n)
 {
+  const T a = T(c1), b = T(c2);
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-    const double ri = r[i] - t[i];
-    r[i]            = ri;
-    const double di = c1 * d[i] + c2 * dinv[i] * ri;
-    d[i]            = di;
+    const T ri = r[i] - t[i];
+    r[i]       = ri;
+    const T di = a * d[i] + b * dinv[i] * ri;
+    d[i]       = di;
     x[i] += di;
   }
 }
 
-__global__ void __launch_bounds__(kThreads) k_residual(const double* b, const double* t, double* r, int64_t n)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_residual(const T* b, const T* t, T* r, int64_t n)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
     r[i] = b[i] - t[i];
 }
 
-__global__ void __launch_bounds__(kThreads) k_sub_inplace(double* r, const double* t, int64_t n)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_sub_inplace(T* r, const T* t, int64_t n)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
     r[i] -= t[i];
 }
 
-__global__ void __launch_bounds__(kThreads) k_apply_mask(double* x, const uint8_t* mask, int64_t n)
+template <typename T>
+__global__ void __launch_bounds__(kThreads) k_apply_mask(T* x, const uint8_t* mask, int64_t n)
 {
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-    if (mask[i]) x[i] = 0.0;
+    if (mask[i]) x[i] = T(0);
+}
+
+// y = T2(x), zero where mask is set (mask may be null)
+template <typename T1, typename T2>
+__global__ void __launch_bounds__(kThreads) k_convert(const T1* x, T2* y, const uint8_t* mask, int64_t n)
+{
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
+    y[i] = (mask && mask[i]) ? T2(0) : T2(x[i]);
 }
 
 // one wave per row
@@ -610,11 +626,11 @@ hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const 
   const Grid3 d0{{G.nn[0], G.nn[1], G.nn[2]}}, d1{{G.nv[0], G.nn[1], G.nn[2]}}, d2{{G.nv[0], G.nv[1], G.nn[2]}},
       d3{{G.nv[0], G.nv[1], G.nv[2]}};
   const uint8_t* none = nullptr;
-  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(fe_restrict_scratch(G, 0))), dim3(kThreads), 0, s, G, r, d0,
+  hipLaunchKernelGGL(k_fe_restrict_dir<double>, dim3(blocks_for(fe_restrict_scratch(G, 0))), dim3(kThreads), 0, s, G, r, d0,
                      t1, d1, 0, none);
-  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(fe_restrict_scratch(G, 1))), dim3(kThreads), 0, s, G, t1, d1,
+  hipLaunchKernelGGL(k_fe_restrict_dir<double>, dim3(blocks_for(fe_restrict_scratch(G, 1))), dim3(kThreads), 0, s, G, t1, d1,
                      t2, d2, 1, none);
-  hipLaunchKernelGGL(k_fe_restrict_dir, dim3(blocks_for(int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3)), dim3(kThreads), 0,
+  hipLaunchKernelGGL(k_fe_restrict_dir<double>, dim3(blocks_for(int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3)), dim3(kThreads), 0,
                      s, G, t2, d2, rc, d3, 2, mask);
   return hipGetLastError();
 }
@@ -622,7 +638,7 @@ hipError_t fe_restrict(const sxu::FEGrid& G, const double* r, double* rc, const 
 hipError_t fe_prolong_add(const sxu::FEGrid& G, const double* ec, double* z, hipStream_t s)
 {
   const int64_t n = int64_t(G.nn[0]) * G.nn[1] * G.nn[2] * 3;
-  hipLaunchKernelGGL(k_fe_prolong_add, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, ec, z);
+  hipLaunchKernelGGL(k_fe_prolong_add<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, ec, z);
   return hipGetLastError();
 }
 
@@ -700,7 +716,7 @@ hipError_t coarsen_mask(const uint8_t* mf, Grid3 f, uint8_t* mc, Grid3 c, hipStr
 hipError_t cheb_init(const double* b, const double* dinv, double theta, double* x, double* d, double* r, int x_add,
                      int copy_r, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_cheb_init, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, dinv, 1.0 / theta, x, d, r, x_add,
+  hipLaunchKernelGGL(k_cheb_init<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, dinv, 1.0 / theta, x, d, r, x_add,
                      copy_r, n);
   return hipGetLastError();
 }
@@ -708,25 +724,25 @@ hipError_t cheb_init(const double* b, const double* dinv, double theta, double* 
 hipError_t cheb_step(double* r, const double* t, double* d, const double* dinv, double c1, double c2, double* x,
                      int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_cheb_step, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, d, dinv, c1, c2, x, n);
+  hipLaunchKernelGGL(k_cheb_step<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, d, dinv, c1, c2, x, n);
   return hipGetLastError();
 }
 
 hipError_t residual(const double* b, const double* t, double* r, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_residual, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, t, r, n);
+  hipLaunchKernelGGL(k_residual<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, t, r, n);
   return hipGetLastError();
 }
 
 hipError_t sub_inplace(double* r, const double* t, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_sub_inplace, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, n);
+  hipLaunchKernelGGL(k_sub_inplace<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, n);
   return hipGetLastError();
 }
 
 hipError_t apply_mask(double* x, const uint8_t* mask, int64_t n, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_apply_mask, dim3(blocks_for(n)), dim3(kThreads), 0, s, x, mask, n);
+  hipLaunchKernelGGL(k_apply_mask<double>, dim3(blocks_for(n)), dim3(kThreads), 0, s, x, mask, n);
   return hipGetLastError();
 }
 
@@ -838,6 +854,67 @@ hipError_t vertex_compatible(const sxu::FEGrid& G, const int32_t* rows, int64_t 
 {
   if (nr <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_vertex_compatible, dim3(blocks_for(nr)), dim3(kThreads), 0, s, G, rows, nr, vmask, flag);
+  return hipGetLastError();
+}
+
+// FP32 level-0 vectors
+hipError_t fe_restrict(const sxu::FEGrid& G, const float* r, double* rc, const uint8_t* mask, double* t1, double* t2,
+                       hipStream_t s)
+{
+  const Grid3 d0{{G.nn[0], G.nn[1], G.nn[2]}}, d1{{G.nv[0], G.nn[1], G.nn[2]}}, d2{{G.nv[0], G.nv[1], G.nn[2]}},
+      d3{{G.nv[0], G.nv[1], G.nv[2]}};
+  const uint8_t* none = nullptr;
+  hipLaunchKernelGGL(k_fe_restrict_dir<float>, dim3(blocks_for(fe_restrict_scratch(G, 0))), dim3(kThreads), 0, s, G, r,
+                     d0, t1, d1, 0, none);
+  hipLaunchKernelGGL(k_fe_restrict_dir<double>, dim3(blocks_for(fe_restrict_scratch(G, 1))), dim3(kThreads), 0, s, G,
+                     static_cast<const double*>(t1), d1, t2, d2, 1, none);
+  hipLaunchKernelGGL(k_fe_restrict_dir<double>, dim3(blocks_for(int64_t(G.nv[0]) * G.nv[1] * G.nv[2] * 3)),
+                     dim3(kThreads), 0, s, G, static_cast<const double*>(t2), d2, rc, d3, 2, mask);
+  return hipGetLastError();
+}
+hipError_t fe_prolong_add(const sxu::FEGrid& G, const double* ec, float* z, hipStream_t s)
+{
+  const int64_t n = int64_t(G.nn[0]) * G.nn[1] * G.nn[2] * 3;
+  hipLaunchKernelGGL(k_fe_prolong_add<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, G, ec, z);
+  return hipGetLastError();
+}
+hipError_t cheb_init(const float* b, const float* dinv, double theta, float* x, float* d, float* r, int x_add,
+                     int copy_r, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cheb_init<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, dinv, 1.0 / theta, x, d, r,
+                     x_add, copy_r, n);
+  return hipGetLastError();
+}
+hipError_t cheb_step(float* r, const float* t, float* d, const float* dinv, double c1, double c2, float* x, int64_t n,
+                     hipStream_t s)
+{
+  hipLaunchKernelGGL(k_cheb_step<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, d, dinv, c1, c2, x, n);
+  return hipGetLastError();
+}
+hipError_t residual(const float* b, const float* t, float* r, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_residual<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, b, t, r, n);
+  return hipGetLastError();
+}
+hipError_t sub_inplace(float* r, const float* t, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_sub_inplace<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, r, t, n);
+  return hipGetLastError();
+}
+hipError_t apply_mask(float* x, const uint8_t* mask, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_apply_mask<float>, dim3(blocks_for(n)), dim3(kThreads), 0, s, x, mask, n);
+  return hipGetLastError();
+}
+hipError_t to_f32(const double* x, float* y, const uint8_t* mask, int64_t n, hipStream_t s)
+{
+  hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks_for(n)), dim3(kThreads), 0, s, x, y, mask, n);
+  return hipGetLastError();
+}
+hipError_t to_f64(const float* x, double* y, int64_t n, hipStream_t s)
+{
+  const uint8_t* none = nullptr;
+  hipLaunchKernelGGL((k_convert<float, double>), dim3(blocks_for(n)), dim3(kThreads), 0, s, x, y, none, n);
   return hipGetLastError();
 }
 

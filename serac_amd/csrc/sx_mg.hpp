@@ -95,6 +95,21 @@ hipError_t stencil_mask(double* S, Grid3 g, const uint8_t* mask, hipStream_t s);
 // vertex component whose hat is nonzero at it
 hipError_t vertex_compatible(const sxu::FEGrid& G, const int32_t* rows, int64_t nr, const uint8_t* vmask, int* flag,
                              hipStream_t s);
+// FP32 level-0 vectors for the V-cycle's smoothing (the same operations, FP32
+// storage and arithmetic; transfers accumulate in FP64); to_f32 zeroes the
+// masked entries
+hipError_t fe_restrict(const sxu::FEGrid& G, const float* r, double* rc, const uint8_t* mask, double* t1, double* t2,
+                       hipStream_t s);
+hipError_t fe_prolong_add(const sxu::FEGrid& G, const double* ec, float* z, hipStream_t s);
+hipError_t cheb_init(const float* b, const float* dinv, double theta, float* x, float* d, float* r, int x_add,
+                     int copy_r, int64_t n, hipStream_t s);
+hipError_t cheb_step(float* r, const float* t, float* d, const float* dinv, double c1, double c2, float* x, int64_t n,
+                     hipStream_t s);
+hipError_t residual(const float* b, const float* t, float* r, int64_t n, hipStream_t s);
+hipError_t sub_inplace(float* r, const float* t, int64_t n, hipStream_t s);
+hipError_t apply_mask(float* x, const uint8_t* mask, int64_t n, hipStream_t s);
+hipError_t to_f32(const double* x, float* y, const uint8_t* mask, int64_t n, hipStream_t s);
+hipError_t to_f64(const float* x, double* y, int64_t n, hipStream_t s);
 // power iteration helpers: v = dinv t / |dinv t| (norms on the host)
 hipError_t mul_dinv(const double* dinv, const double* t, double* v, int64_t n, hipStream_t s);
 hipError_t scale(double* v, double a, int64_t n, hipStream_t s);

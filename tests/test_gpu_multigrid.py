@@ -321,6 +321,33 @@ def test_bf16_smoother_vcycle_is_symmetric_and_newton_matches(gpu_lib):
     assert np.abs(xs[1] - xs[0]).max() <= 1e-7 * np.abs(xs[0]).max()
 
 
+@pytest.mark.parametrize("bf16", [False, True])
+def test_fp32_smoother_vectors_match_the_fp64_vcycle(gpu_lib, bf16):
+    m, s, f, ess, grad = linearized(8, 3)
+    zs = []
+    r = W.nodal_field(s.nnodes, 3, 7, 3000, -1.0, 1.0, 1)
+    rng = np.random.default_rng(3)
+    u, v = rng.standard_normal(s.ndof), rng.standard_normal(s.ndof)
+    for prec in (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F32):
+        mg = sx.Multigrid(s)
+        if bf16:
+            mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
+        mg.set_smoother_vectors(prec)
+        mg.setup(grad, ess)
+        z = sx.DeviceVector(s.ndof)
+        mg.apply(sx.DeviceVector.from_numpy(r), z)
+        zs.append(z.numpy())
+        if prec == sx.capi.SX_PREC_F32:
+            Mu, Mv = sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof)
+            mg.apply(sx.DeviceVector.from_numpy(u), Mu)
+            mg.apply(sx.DeviceVector.from_numpy(v), Mv)
+            Mu, Mv = Mu.numpy(), Mv.numpy()
+            assert abs(v @ Mu - u @ Mv) <= 1e-5 * abs(v @ Mu)
+            assert u @ Mu > 0 and v @ Mv > 0
+    assert np.abs(zs[1] - zs[0]).max() <= 1e-4 * np.abs(zs[0]).max()
+    assert np.array_equal(zs[1][ess], r[ess])
+
+
 def test_vcycle_is_symmetric_and_positive(gpu_lib):
     m, s, f, ess, grad = linearized(8, 2)
     mg = sx.Multigrid(s)
@@ -517,6 +544,8 @@ def test_multigrid_arguments_are_checked(gpu_lib, patch_mesh):
         mg.set_galerkin(2)
     with pytest.raises(sx.SeracAmdError):
         mg.set_smoother_precision(sx.capi.SX_PREC_F32)
+    with pytest.raises(sx.SeracAmdError):
+        mg.set_smoother_vectors(sx.capi.SX_PREC_BF16)
     with pytest.raises(sx.SeracAmdError):
         mg.apply(sx.DeviceVector(s.ndof), sx.DeviceVector(s.ndof))  # before a setup
     f = sx.Functional(s1, [s1])
