@@ -37,16 +37,6 @@ __global__ void __launch_bounds__(kThreads) k_axpy(double a, const double* x, do
     y[i] += a * x[i];
 }
 
-// y = x + a * y with a read from device memory; nothing once stop[0] is set
-__global__ void __launch_bounds__(kThreads) k_xpay_dev(const double* x, const double* ap, double* y, int64_t n,
-                                                        const double* stop)
-{
-  if (stop && stop[0] != 0.0) return;
-  const double a = *ap;
-  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
-    y[i] = x[i] + a * y[i];
-}
-
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
   // wave64 shuffle tree, then the 4 wave partials in LDS (fixed order)
@@ -69,52 +59,18 @@ __global__ void __launch_bounds__(kThreads) k_dot_partial(const double* a, const
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// Final reduction of nb partials into scal[slot]; op then updates derived
-// scalars in place:  op 1: scal[2] = scal[0] / scal[1]            (alpha)
-//                    op 2: scal[4] = scal[3] / scal[0], scal[0] = scal[3] (beta)
-// With stop != nullptr nothing is written once stop[0] is set, and op 2 sets
-// stop = {1, it} when the new (r, z) is at most goal or not a number.
-__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, int nb, double* scal, int slot, int op,
-                                                            double goal, int it, double* stop)
+// Final reduction of nb partials into scal[slot]; op 1 then sets
+// scal[2] = scal[0] / scal[1]
+__global__ void __launch_bounds__(kThreads) k_reduce_final(const double* part, int nb, double* scal, int slot, int op)
 {
   __shared__ double sh[4];
   double            s = 0.0;
   for (int i = threadIdx.x; i < nb; i += kThreads) s += part[i];
   s = block_sum(s, sh);
   if (threadIdx.x == 0) {
-    if (stop && stop[0] != 0.0) return;
     scal[slot] = s;
     if (op == 1) scal[2] = scal[0] / scal[1];
-    if (op == 2) {
-      scal[4] = scal[3] / scal[0];
-      scal[0] = scal[3];
-      if (stop && !(s > goal)) {
-        stop[0] = 1.0;
-        stop[1] = double(it);
-      }
-    }
   }
-}
-
-// x += alpha d; r -= alpha Ad; z = r / diag; partial (r, z)
-__global__ void __launch_bounds__(kThreads) k_cg_update(double* x, double* r, double* z, const double* d,
-                                                         const double* Ad, const double* diag, const double* scal,
-                                                         int64_t n, double* part, const double* stop)
-{
-  __shared__ double sh[4];
-  if (stop && stop[0] != 0.0) return;  // the whole block leaves before any barrier
-  const double      alpha = scal[2];
-  double            s     = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
-    x[i] += alpha * d[i];
-    const double ri = r[i] - alpha * Ad[i];
-    r[i]            = ri;
-    const double zi = ri / diag[i];
-    z[i]            = zi;
-    s += ri * zi;
-  }
-  s = block_sum(s, sh);
-  if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
 // Sum of the nb reduction partials, the same fixed order as k_reduce_final
@@ -357,31 +313,14 @@ hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t
   return hipGetLastError();
 }
 
-hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s,
-                        const double* stop)
-{
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_xpay_dev, dim3(grid_for(n)), dim3(kThreads), 0, s, x, a_dev, y, n, stop);
-  return hipGetLastError();
-}
-
 int reduction_blocks() { return 1024; }
 
 hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part, double* scal, int slot, int op,
-                       hipStream_t s, double* stop)
+                       hipStream_t s)
 {
   const int nb = reduction_blocks();
   hipLaunchKernelGGL(k_dot_partial, dim3(nb), dim3(kThreads), 0, s, a, b, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, op, 0.0, 0, stop);
-  return hipGetLastError();
-}
-
-hipError_t cg_update(double* x, double* r, double* z, const double* d, const double* Ad, const double* diag,
-                     double* scal, int64_t n, double* part, hipStream_t s, double goal, int it, double* stop)
-{
-  const int nb = reduction_blocks();
-  hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(kThreads), 0, s, x, r, z, d, Ad, diag, scal, n, part, stop);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, 3, 2, goal, it, stop);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, op);
   return hipGetLastError();
 }
 
@@ -413,7 +352,7 @@ hipError_t jacobi_dot(const double* r, const double* diag, double* z, int64_t n,
 {
   const int nb = reduction_blocks();
   hipLaunchKernelGGL(k_jacobi, dim3(nb), dim3(kThreads), 0, s, r, diag, z, n, part);
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0, 0.0, 0, nullptr);
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kThreads), 0, s, part, nb, scal, slot, 0);
   return hipGetLastError();
 }
 

@@ -9,26 +9,14 @@ namespace sxk {
 hipError_t vec_fill(double* x, double v, int64_t n, hipStream_t s);
 hipError_t vec_fill_f32(float* x, float v, int64_t n, hipStream_t s);
 hipError_t vec_axpy(double a, const double* x, double* y, int64_t n, hipStream_t s);
-// y = x + (*a_dev) y.  The stop pointers below (device memory, nullable) hold
-// {flag, iteration}: once flag is nonzero the kernels leave their outputs and
-// the scalars as they are, so a CG loop can run ahead of its convergence test.
-hipError_t vec_xpay_dev(const double* x, const double* a_dev, double* y, int64_t n, hipStream_t s,
-                        const double* stop = nullptr);
-
 // number of partial sums the reductions use (size of the `part` scratch)
 int reduction_blocks();
 // scal[slot] = (a, b); op 1 then sets scal[2] = scal[0]/scal[1]
 hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part, double* scal, int slot, int op,
-                       hipStream_t s, double* stop = nullptr);
-// x += a d, r -= a Ad, z = r/diag with a = scal[2]; scal[3] = (r, z);
-// scal[4] = scal[3]/scal[0]; scal[0] = scal[3]; with stop, a new (r, z) that is
-// at most goal (or not a number) sets stop = {1, it}
-hipError_t cg_update(double* x, double* r, double* z, const double* d, const double* Ad, const double* diag,
-                     double* scal, int64_t n, double* part, hipStream_t s, double goal = 0.0, int it = 0,
-                     double* stop = nullptr);
+                       hipStream_t s);
 // Jacobi-PCG in two kernels per iteration, each reducing the previous
-// kernel's reduction_blocks() partials itself (the order of k_reduce_final's
-// sum, so the scalars are those of cg_update):
+// kernel's reduction_blocks() partials itself (in the fixed order of
+// k_reduce_final's sum):
 //   dot_partial: part = partials of (a, b)
 //   cg_step_x:   alpha = scal[0] / sum(part_den); x += alpha d; r -= alpha Ad;
 //                z = r/diag; Ad = 0; part_rz = partials of (r, z);
@@ -36,7 +24,7 @@ hipError_t cg_update(double* x, double* r, double* z, const double* d, const dou
 //   cg_step_d:   betanom = sum(part_rz); d = z + betanom / scal[5] d;
 //                scal[0] = scal[3] = betanom, scal[4] = beta; betanom <= goal
 //                (or not a number) sets stop = {1, it}
-// Both leave everything alone once stop[0] is set.
+// Both leave everything alone once stop[0] is set (stop: device {flag, iteration}).
 hipError_t dot_partial(const double* a, const double* b, int64_t n, double* part, hipStream_t s);
 hipError_t cg_step_x(double* x, double* r, double* z, double* Ad, const double* d, const double* diag, double* scal,
                      const double* part_den, double* part_rz, int64_t n, const double* stop, hipStream_t s);
