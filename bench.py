@@ -131,12 +131,16 @@ def host_cpu_model():
 
 def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
     """The oracle's J.v (the reference algorithm restated in C++) on an n^3
-    sample of the same workload, timed on one host core and on all the host
-    threads this job may use (OpenMP element ranges, per-thread scatter):
-    median of 5 batches each."""
+    mesh of the same workload (the workload's own mesh by default), timed on
+    one host core and on all the host threads this job may use (OpenMP element
+    ranges, each thread scattering into its own node band, allocated before
+    the timed region): median of 5 batches each.  The oracle is loaded in its
+    x86-64-v4 (AVX-512) build when the host has AVX-512."""
     import oracle as O
     from serac_amd import workloads as W
 
+    isa = O.use_isa("widest")
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     m = O.Mesh.cuboid(n, n, n)
     if perturb > 0:
         m.perturb(42, perturb)
@@ -150,8 +154,7 @@ def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
         f.add_domain_integral(O.QF_NEOHOOKEAN, params)
         U = W.neohookean_displacement(s.node_coords(), noise=W.noise_for_mesh(n))
     dU = W.direction(s.nnodes, 3)
-    f.linearize(U)
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+    f.linearize_threads(U, threads)
 
     def median_per_apply(nt, budget):
         t1 = f.time_jvp(dU, 1) if nt == 1 else f.time_jvp_threads(dU, 1, nt)
@@ -167,11 +170,13 @@ def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
         "cores": threads,
         "kind": "port",
         "single_core_value": s.ndof / t_one / 1e9,
+        "all_vs_one_core": round(t_one / t_all, 2),
         "cpu_model": host_cpu_model(),
         "nproc": os.cpu_count(),
+        "isa": "x86-64-v4 (AVX-512)" if isa == "v4" else "x86-64-v3 (AVX2)",
         "sample": f"oracle J.v (stored 81-entry tangent, sum factorization), {n}^3 p={p} {qf}, {s.ndof} DOFs; "
                   f"median of 5 batches of {r_all} applications on {threads} threads (OpenMP element ranges, "
-                  f"per-thread scatter) and of {r_one} on one core",
+                  f"per-thread node bands allocated outside the timed region) and of {r_one} on one core",
     }
 
 
@@ -318,7 +323,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--workload", default="neohookean_p3_64", choices=sorted(WORKLOADS))
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
-    ap.add_argument("--cpu-sample-n", type=int, default=16)
+    ap.add_argument("--cpu-sample-n", type=int, default=0,
+                    help="mesh intervals per side of the CPU baseline sample (0: the workload's own mesh, at most 64)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the FP32 / BF16 J.v measurements")
     ap.add_argument("--solve", action="store_true",
@@ -537,7 +543,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         tc = time.time()
-        cpu = cpu_baseline(args.cpu_sample_n, p, perturb, params, args.cpu_seconds, qf)
+        cpu = cpu_baseline(args.cpu_sample_n or min(n, 64), p, perturb, params, args.cpu_seconds, qf)
         log(f"cpu baseline {cpu['value']:.5f} GDOF/s ({time.time() - tc:.1f} s)")
 
     rec = {

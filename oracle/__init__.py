@@ -15,6 +15,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liboracle.so")
+_LIB_V4 = os.path.join(_HERE, "liboracle_v4.so")
 
 QF_HEAT = 1
 QF_MASS_DIFFUSION = 2
@@ -36,10 +37,41 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def _host_has_avx512():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("flags"):
+                    f = line.split()
+                    return all(x in f for x in ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"))
+    except OSError:
+        pass
+    return False
+
+
+_isa = "v3"
+
+
+def use_isa(level):
+    """Select the build the next lib() call loads: "v3" (liboracle.so, the
+    tests' default) or "widest" (liboracle_v4.so when this host has AVX-512).
+    Returns the level loaded.  Only before the first lib() call."""
+    global _isa
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _isa = "v4" if level == "widest" and _host_has_avx512() and os.path.exists(_LIB_V4) else "v3"
+    return _isa
+
+
+def isa():
+    return _isa
+
+
 def _load():
-    if not os.path.exists(_LIB):
+    path = _LIB_V4 if _isa == "v4" else _LIB
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(_LIB)
+    lib = ctypes.CDLL(path)
     sig = {
         "orc_last_error": (ctypes.c_char_p, []),
         "orc_rng_u01": (ctypes.c_double, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
@@ -79,6 +111,7 @@ def _load():
         "orc_form_add_integral": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int]),
         "orc_form_eval": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int]),
         "orc_form_jvp": (ctypes.c_int, [_vp, _D, _D]),
+        "orc_form_eval_threads": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int, ctypes.c_int]),
         "orc_form_diagonal": (ctypes.c_int, [_vp, _D]),
         "orc_form_dsize": (ctypes.c_int, [_vp, ctypes.c_int]),
         "orc_form_derivative": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_int, _D]),
@@ -336,6 +369,13 @@ class Form:
         _check(lib().orc_form_eval(self.h, _dp(U), _dp(R), 1))
         return R
 
+    def linearize_threads(self, U, nthreads):
+        """linearize() over nthreads host threads (node-band scatter)."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        R = np.zeros(self.space.ndof)
+        _check(lib().orc_form_eval_threads(self.h, _dp(U), _dp(R), 1, nthreads))
+        return R
+
     def jvp(self, dU):
         dU = np.ascontiguousarray(dU, dtype=np.float64)
         dR = np.zeros(self.space.ndof)
@@ -390,7 +430,7 @@ class Form:
         return lib().orc_time_jvp(self.h, _dp(dU), _dp(dR), reps)
 
     def jvp_threads(self, dU, nthreads):
-        """J.v over nthreads host threads (per-thread scatter, then a sum)."""
+        """J.v over nthreads host threads (per-thread node bands, then a sum)."""
         dU = np.ascontiguousarray(dU, dtype=np.float64)
         dR = np.zeros(self.space.ndof)
         lib().orc_time_jvp_threads(self.h, _dp(dU), _dp(dR), 1, nthreads)

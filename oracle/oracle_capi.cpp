@@ -262,6 +262,11 @@ int orc_form_eval(void* f, const double* U, double* R, int linearize)
 {
   return guard([&] { static_cast<OForm*>(f)->form.eval(U, R, linearize != 0); });
 }
+// residual / linearization over nthreads host threads (node-band scatter)
+int orc_form_eval_threads(void* f, const double* U, double* R, int linearize, int nthreads)
+{
+  return guard([&] { static_cast<OForm*>(f)->form.eval_threads(U, R, linearize != 0, nthreads); });
+}
 int orc_form_jvp(void* f, const double* dU, double* dR)
 {
   return guard([&] { static_cast<OForm*>(f)->form.jvp(dU, dR); });
@@ -347,8 +352,9 @@ double orc_time_jvp(void* f, const double* dU, double* dR, int reps)
 // the same with the elements split over nthreads host threads (OpenMP)
 double orc_time_jvp_threads(void* f, const double* dU, double* dR, int reps, int nthreads)
 {
-  auto& F  = static_cast<OForm*>(f)->form;
-  auto  t0 = std::chrono::steady_clock::now();
+  auto& F = static_cast<OForm*>(f)->form;
+  F.prepare_threads(nthreads);  // the per-thread bands are allocated outside the timed region
+  auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < reps; i++) F.jvp_threads(dU, dR, nthreads);
   auto t1 = std::chrono::steady_clock::now();
   return std::chrono::duration<double>(t1 - t0).count();

@@ -488,45 +488,160 @@ struct Form {
     boundary_jvp(dU, dR);
   }
 
-  // J.v on `nthreads` host threads for the CPU baseline: each thread takes a
-  // contiguous range of elements and scatters into its own L-vector; the
-  // vectors are then summed.  jvp() above keeps the reference's serial order.
-  void jvp_threads(const double* dU, double* dR, int nthreads) const
+  // ---- the threaded CPU baseline ----
+  // Thread t takes the contiguous range [lo_t, hi_t) of each integral's
+  // elements and scatters into a private buffer that covers only the nodes
+  // those elements touch, [nlo_t, nhi_t) (a slab of a lexicographically
+  // numbered mesh, so the buffers together hold about one copy of the vector).
+  // A final pass sums, per node, the bands that contain it.  The buffers are
+  // allocated once per thread count by prepare_threads(), and each thread
+  // zeroes and first-touches its own.
+  struct Bands {
+    int                              nt = 0;
+    std::vector<int64_t>             nlo, nhi;
+    std::vector<std::vector<double>> buf;  // [t][(node - nlo_t) * nc + c]
+  };
+  mutable Bands bands;
+
+  static void thread_range(size_t ne, int t, int nt, size_t& lo, size_t& hi)
   {
-    const size_t ndof = size_t(space.ndof());
-    nthreads          = std::max(1, nthreads);
-    std::vector<double> acc(size_t(nthreads) * ndof, 0.0);
+    lo = ne * size_t(t) / size_t(nt);
+    hi = ne * size_t(t + 1) / size_t(nt);
+  }
+
+  void prepare_threads(int nthreads) const
+  {
+    nthreads = std::max(1, nthreads);
+    if (bands.nt == nthreads) return;
+    Bands b;
+    b.nt = nthreads;
+    b.nlo.assign(size_t(nthreads), INT64_MAX);
+    b.nhi.assign(size_t(nthreads), 0);
+    for (int t = 0; t < nthreads; t++) {
+      for (auto& I : integrals) {
+        size_t lo, hi;
+        thread_range(I.elems.size(), t, nthreads, lo, hi);
+        for (size_t k = lo; k < hi; k++)
+          for (int j = 0; j < space.npe; j++) {
+            const int64_t v = space.emap[size_t(I.elems[k]) * space.npe + size_t(j)];
+            b.nlo[size_t(t)] = std::min(b.nlo[size_t(t)], v);
+            b.nhi[size_t(t)] = std::max(b.nhi[size_t(t)], v + 1);
+          }
+      }
+      if (b.nhi[size_t(t)] <= b.nlo[size_t(t)]) b.nlo[size_t(t)] = b.nhi[size_t(t)] = 0;
+    }
+    b.buf.resize(size_t(nthreads));
 #pragma omp parallel num_threads(nthreads)
     {
 #ifdef _OPENMP
-      const int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+      const int t = omp_get_thread_num();
 #else
-      const int tid = 0, nt = 1;
+      const int t = 0;
 #endif
-      double*             out = acc.data() + size_t(tid) * ndof;
+      if (t < nthreads)  // each thread allocates and first-touches its own band
+        b.buf[size_t(t)].assign(size_t(b.nhi[size_t(t)] - b.nlo[size_t(t)]) * size_t(nc()), 0.0);
+    }
+    bands = std::move(b);
+  }
+
+  // out[lidx(node, c)] = sum over the bands holding node, for the nodes of
+  // thread t's share (call inside the parallel region, after a barrier)
+  void sum_bands(int t, int nt, double* out) const
+  {
+    const int64_t nn = space.nnodes;
+    const int64_t a = nn * t / nt, z = nn * (t + 1) / nt;
+    std::vector<int> over;
+    for (int s2 = 0; s2 < nt; s2++)
+      if (bands.nlo[size_t(s2)] < z && bands.nhi[size_t(s2)] > a) over.push_back(s2);
+    for (int64_t i = a; i < z; i++)
+      for (int c = 0; c < nc(); c++) {
+        double v = 0.0;
+        for (int s2 : over)
+          if (i >= bands.nlo[size_t(s2)] && i < bands.nhi[size_t(s2)])
+            v += bands.buf[size_t(s2)][size_t(i - bands.nlo[size_t(s2)]) * size_t(nc()) + size_t(c)];
+        out[space.lidx(i, c)] = v;
+      }
+  }
+
+  void scatter_band(int e, const double* Re, int t) const
+  {
+    double*       b  = bands.buf[size_t(t)].data();
+    const int64_t lo = bands.nlo[size_t(t)];
+    for (int i = 0; i < nc(); i++)
+      for (int j = 0; j < space.npe; j++)
+        b[size_t(space.emap[size_t(e) * space.npe + j] - lo) * size_t(nc()) + size_t(i)] += Re[size_t(i) * space.npe + j];
+  }
+
+  // J.v on `nthreads` host threads (the CPU baseline); jvp() above keeps the
+  // reference's serial order.
+  void jvp_threads(const double* dU, double* dR, int nthreads) const
+  {
+    for (auto& I : integrals)
+      if (I.D.empty()) throw std::runtime_error("jvp before linearize");
+    prepare_threads(nthreads);
+    const int nt = bands.nt;
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+      const int tid = omp_get_thread_num();
+#else
+      const int tid = 0;
+#endif
+      std::fill(bands.buf[size_t(tid)].begin(), bands.buf[size_t(tid)].end(), 0.0);
       std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
       for (auto& I : integrals) {
-        if (I.D.empty()) continue;
-        const size_t ne = I.elems.size(), lo = ne * size_t(tid) / size_t(nt), hi = ne * size_t(tid + 1) / size_t(nt);
+        size_t lo, hi;
+        thread_range(I.elems.size(), tid, nt, lo, hi);
         for (size_t k = lo; k < hi; k++) {
           const int e = I.elems[k];
           gather_elem(e, dU, Ue.data());
           std::fill(Re.begin(), Re.end(), 0.0);
           element_jvp(I, k, Ue.data(), Re.data());
-          scatter_elem(e, Re.data(), out);
+          scatter_band(e, Re.data(), tid);
         }
       }
 #pragma omp barrier
-      const size_t lo = ndof * size_t(tid) / size_t(nt), hi = ndof * size_t(tid + 1) / size_t(nt);
-      for (size_t i = lo; i < hi; i++) {
-        double v = 0.0;
-        for (int t = 0; t < nt; t++) v += acc[size_t(t) * ndof + i];
-        dR[i] = v;
-      }
+      sum_bands(tid, nt, dR);
     }
-    for (auto& I : integrals)
-      if (I.D.empty()) throw std::runtime_error("jvp before linearize");
     boundary_jvp(dU, dR);
+  }
+
+  // residual / linearization on `nthreads` host threads (sets up the CPU
+  // baseline at full size); the stored tangent is the same per element as
+  // eval()'s, the residual equal up to the order of the sums.
+  void eval_threads(const double* U, double* R, bool store_D, int nthreads)
+  {
+    prepare_threads(nthreads);
+    const int nt = bands.nt;
+    for (auto& I : integrals)
+      if (store_D) I.D.assign(I.elems.size() * size_t(nq()) * I.dsize, 0.0);
+#pragma omp parallel num_threads(nt)
+    {
+#ifdef _OPENMP
+      const int tid = omp_get_thread_num();
+#else
+      const int tid = 0;
+#endif
+      std::fill(bands.buf[size_t(tid)].begin(), bands.buf[size_t(tid)].end(), 0.0);
+      std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
+      for (auto& I : integrals) {
+        size_t lo, hi;
+        thread_range(I.elems.size(), tid, nt, lo, hi);
+        for (size_t k = lo; k < hi; k++) {
+          const int e = I.elems[k];
+          gather_elem(e, U, Ue.data());
+          std::fill(Re.begin(), Re.end(), 0.0);
+          if (nc() == 1)
+            element_eval<4>(I, k, e, Ue.data(), Re.data(), store_D);
+          else
+            element_eval<12>(I, k, e, Ue.data(), Re.data(), store_D);
+          scatter_band(e, Re.data(), tid);
+        }
+      }
+#pragma omp barrier
+      sum_bands(tid, nt, R);
+    }
+    boundary_eval(U, R, store_D);
   }
 
   // Element gradient K_e[(c,j) test][(c',J) trial], row-major, by applying the
@@ -629,8 +744,10 @@ inline SolveReport pcg_jacobi(const Form& f, const std::vector<char>& is_ess, co
   SolveReport rep;
   rep.initial_norm = std::sqrt(std::max(nom, 0.0));
   double goal      = std::max(nom * rel_tol * rel_tol, abs_tol * abs_tol);
-  if (nom <= goal) {
-    rep.converged  = true;
+  // mfem::CGSolver: (r0, B r0) < 0 means B is not positive definite; it stops
+  // unconverged with x = 0
+  if (nom < 0.0 || nom <= goal) {
+    rep.converged  = nom >= 0.0;
     rep.final_norm = rep.initial_norm;
     return rep;
   }
@@ -645,9 +762,14 @@ inline SolveReport pcg_jacobi(const Form& f, const std::vector<char>& is_ess, co
     }
     for (size_t i = 0; i < n; i++) z[i] = r[i] / diag[i];
     double betanom = dot(r, z);
+    if (betanom < 0.0) {  // the same test inside the loop: stop unconverged
+      rep.converged  = false;
+      rep.final_norm = 0.0;
+      break;
+    }
     if (betanom <= goal) {
       rep.converged  = true;
-      rep.final_norm = std::sqrt(std::max(betanom, 0.0));
+      rep.final_norm = std::sqrt(betanom);
       break;
     }
     double beta = betanom / nom;

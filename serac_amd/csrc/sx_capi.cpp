@@ -1603,6 +1603,17 @@ struct Workspace {
   DeviceBuffer Zb;            // flexible GMRES: the preconditioned directions
 };
 
+// Checks every essential DOF against [0, ndof) before the list reaches the
+// device, where the row kernels index with it unchecked, then uploads it.
+void upload_ess(DeviceBuffer& dst, const int32_t* ess, int64_t ness, int64_t ndof)
+{
+  SX_REQUIRE(ness >= 0 && (ness == 0 || ess), SX_ERR_ARG, "null essential DOF list");
+  for (int64_t k = 0; k < ness; k++)
+    SX_REQUIRE(ess[k] >= 0 && ess[k] < ndof, SX_ERR_ARG,
+               "essential DOF " + std::to_string(ess[k]) + " out of range [0, " + std::to_string(ndof) + ")");
+  dst.upload(std::vector<int32_t>(ess, ess + ness));
+}
+
 // Jacobi-PCG on J_bc (rows and columns of ess eliminated) following the
 // preconditioned CG of mfem::CGSolver, as configured by
 // num/equation_solver.cpp:916-919, 1018-1021.
@@ -1644,13 +1655,16 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   SX_HIP(sxk::jacobi_dot(r, diag, z, n, part, scal, 0, s));  // scal[0] = (r, z)
   SX_HIP(hipMemcpyAsync(d, z, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   const double nom0 = read_scalar(f, 0);
+  SX_REQUIRE(std::isfinite(nom0), SX_ERR_NOCONV, "CG: the initial residual is not finite");
   rep->initial_norm = std::sqrt(std::max(nom0, 0.0));
   const double goal = std::max(nom0 * rel * rel, abs_tol * abs_tol);
   rep->final_norm   = rep->initial_norm;
-  rep->converged    = nom0 <= goal;
+  rep->converged    = nom0 >= 0.0 && nom0 <= goal;
   rep->linear_iterations = 0;
   if (hook) (*hook)(0, x, r);
-  if (rep->converged) return;
+  // (r, D^-1 r) < 0: D is not positive definite; mfem's CGSolver returns
+  // unconverged with x = 0
+  if (rep->converged || nom0 < 0.0) return;
 
   auto apply_bc = [&](const double* in, double* outv) {
     apply_gradient(g, in, outv);
@@ -1687,9 +1701,9 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
     SX_HIP(hipMemcpyAsync(st, scal + 3, sizeof(st), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
     rep->final_norm = std::sqrt(std::max(st[0], 0.0));
-    SX_REQUIRE(std::isfinite(st[0]), SX_ERR_NOCONV, "CG produced a non-finite residual");
+    SX_REQUIRE(std::isfinite(st[0]) && st[5] != 3.0, SX_ERR_NOCONV, "CG produced a non-finite residual");
     if (st[5] != 0.0) {
-      rep->converged         = 1;
+      rep->converged         = st[5] == 1.0;  // 2: the preconditioned residual norm went negative
       rep->linear_iterations = int(st[6]);
       return;
     }
@@ -2123,9 +2137,9 @@ void mg_pcg(sx_grad* g, sx_multigrid* M, const int32_t* ess, int64_t ness, const
     SX_HIP(hipMemcpyAsync(st, scal + 3, sizeof(st), hipMemcpyDeviceToHost, s));
     SX_HIP(hipStreamSynchronize(s));
     rep->final_norm = std::sqrt(std::max(st[0], 0.0));
-    SX_REQUIRE(std::isfinite(st[0]), SX_ERR_NOCONV, "multigrid-CG produced a non-finite residual");
+    SX_REQUIRE(std::isfinite(st[0]) && st[5] != 3.0, SX_ERR_NOCONV, "multigrid-CG produced a non-finite residual");
     if (st[5] != 0.0) {
-      rep->converged         = 1;
+      rep->converged         = st[5] == 1.0;  // 2: the V-cycle is not positive definite here
       rep->linear_iterations = int(st[6]);
       return;
     }
@@ -2676,7 +2690,7 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
     SX_REQUIRE(g->valid, SX_ERR_STATE, "CG on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     cg(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "CG did not converge");
@@ -2696,7 +2710,7 @@ extern "C" int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness,
     SX_REQUIRE(g->valid, SX_ERR_STATE, "GMRES on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     gmres(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "GMRES did not converge");
@@ -2708,12 +2722,17 @@ extern "C" int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness,
 namespace {
 
 // Newton (num/equation_solver.cpp:81-235, no line search): norm goal
-// max(rel ||r0||, abs); linearize, Jacobi-CG on J_bc c = r, x <- x - c.
+// max(rel ||r0||, abs); Jacobi-CG on J_bc c = r, x <- x - c.  Every
+// evaluation is a linearization, whose element kernel writes the residual and
+// the tangent in one pass: the residual that decides convergence is the
+// right-hand side of the next linear solve, so each Newton iteration runs one
+// element pass (the converged check pays the tangent once per call).
 void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* x, const sx_solver_options& o,
                  Workspace& w, double* r, double* c, sx_solve_report* rep)
 {
   const int64_t n    = f->test->ndof();
-  double        norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+  sx_grad*      g    = nullptr;
+  double        norm = residual_norm(f, wrt, x, r, ess, ness, &g);
   rep->initial_norm  = norm;
   const double goal  = std::max(o.rel_tol * norm, o.abs_tol);
   int          it    = 0;
@@ -2724,13 +2743,11 @@ void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* 
       break;
     }
     if (it >= o.max_iter) break;
-    sx_grad* g = nullptr;
-    residual_norm(f, wrt, x, r, ess, ness, &g);  // linearize at x
     sx_solve_report lr{};
     linear_solve(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
     rep->linear_iterations += lr.linear_iterations;
     SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
-    norm = residual_norm(f, wrt, x, r, ess, ness, nullptr);
+    norm = residual_norm(f, wrt, x, r, ess, ness, &g);  // residual and linearization at the new x
   }
   rep->newton_iterations = it;
   rep->final_norm        = norm;
@@ -2752,7 +2769,7 @@ int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, 
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     const int64_t n = f->test->ndof();
     Workspace     w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, f->test->ndof());
     DeviceBuffer rbuf, cbuf;
     rbuf.allocate(size_t(n) * 8);
     cbuf.allocate(size_t(n) * 8);
@@ -2777,7 +2794,7 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
     const int64_t n = f->test->ndof();
     hipStream_t   s = f->stream;
     Workspace     w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, f->test->ndof());
     const int32_t* ess = w.ess.as<const int32_t>();
     DeviceBuffer   rbuf, cbuf, dbc, jdbc, vals;
     rbuf.allocate(size_t(n) * 8);
@@ -3003,7 +3020,7 @@ int sx_mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess_dofs, int64_t ne
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     for (int64_t k = 0; k < ness; k++)
       SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
-    M->ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(M->ess, ess_dofs, ness, M->lv[0].n);
     M->ness  = ness;
     M->set_g = g;
     mg_setup(M, g, M->ess.as<const int32_t>(), ness);
@@ -3043,7 +3060,7 @@ int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_
     for (int64_t k = 0; k < ness; k++)
       SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, M->lv[0].n);
     mg_pcg(g, M, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "multigrid-CG did not converge");
@@ -3064,7 +3081,7 @@ int sx_mg_gmres_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int6
     for (int64_t k = 0; k < ness; k++)
       SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, M->lv[0].n);
     fgmres(g, nullptr, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep, M);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "multigrid-GMRES did not converge");
@@ -3163,7 +3180,7 @@ int sx_learned_record_cg(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int
     SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     PairRecorder rec(L, g->form->stream);
     CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
     cg(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep, &hook);
@@ -3204,7 +3221,7 @@ int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs
     const int64_t n  = f->test->ndof();
     const int64_t nc = L->nv * 3;
     DeviceBuffer  ess, e, r, ec, one;
-    ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(ess, ess_dofs, ness, n);
     e.allocate(size_t(n) * 8);
     r.allocate(size_t(n) * 8);
     ec.allocate(size_t(nc) * 4);
@@ -3310,7 +3327,7 @@ int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t
     SX_REQUIRE(g->valid, SX_ERR_STATE, "preconditioner applied before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     w.diag.allocate(size_t(g->form->test->ndof()) * 8);
     compute_diag(g, w.ess.as<const int32_t>(), ness, w.diag.as<double>());
     learned_apply(L, g->form, w.diag.as<const double>(), w.ess.as<const int32_t>(), ness, r, z);
@@ -3347,7 +3364,7 @@ int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t 
     SX_REQUIRE(!L || g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     fgmres(g, L, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, restart, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     SX_REQUIRE(rep.converged, SX_ERR_NOCONV, "flexible GMRES did not converge");
@@ -3365,7 +3382,7 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
     SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
-    w.ess.upload(std::vector<int32_t>(ess_dofs, ess_dofs + ness));
+    upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     fcg(g, L, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
     SX_REQUIRE(rep.converged, SX_ERR_NOCONV, "flexible CG did not converge");

@@ -121,16 +121,23 @@ __global__ void __launch_bounds__(kThreads) k_cg_step_x(double* x, double* r, do
 }
 
 // Second half: betanom = sum of part_rz, beta = betanom / scal[5]; d = z +
-// beta d.  Block 0 records scal[0] = scal[3] = betanom, scal[4] = beta and,
-// when betanom is at most goal or not a number, stop = {1, it}.  A block that
-// starts after that write leaves d alone, which no later step reads.
+// beta d.  Block 0 records scal[0] = scal[3] = betanom, scal[4] = beta and the
+// stop flag {code, it}: code 1 when betanom is at most goal, 2 when betanom is
+// negative (the preconditioner is not positive definite: mfem's CGSolver stops
+// unconverged there), 3 when it is not a number.  Block 0 writes the flag while
+// other blocks of this launch may be starting, so each block reads it once,
+// through LDS, and leaves as a whole; a block that leaves skips only the d
+// update, which no later step reads.
 __global__ void __launch_bounds__(kThreads) k_cg_step_d(double* d, const double* z, double* scal,
                                                          const double* part_rz, int nb, int64_t n, double goal, int it,
                                                          double* stop)
 {
   __shared__ double sh[4];
   __shared__ double bc;
-  if (stop[0] != 0.0) return;
+  __shared__ int    stopped;
+  if (threadIdx.x == 0) stopped = __hip_atomic_load(stop, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0.0;
+  __syncthreads();
+  if (stopped) return;  // block-uniform
   const double betanom = sum_partials(part_rz, nb, sh, &bc);
   const double beta    = betanom / scal[5];
   for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads)
@@ -139,9 +146,10 @@ __global__ void __launch_bounds__(kThreads) k_cg_step_d(double* d, const double*
     scal[0] = betanom;
     scal[3] = betanom;
     scal[4] = beta;
-    if (!(betanom > goal)) {
+    const double code = betanom != betanom ? 3.0 : betanom < 0.0 ? 2.0 : betanom <= goal ? 1.0 : 0.0;
+    if (code != 0.0) {
       stop[1] = double(it);
-      stop[0] = 1.0;
+      __hip_atomic_store(stop, code, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
   }
 }

@@ -22,9 +22,11 @@ hipError_t vec_dot_dev(const double* a, const double* b, int64_t n, double* part
 //                z = r/diag; Ad = 0; part_rz = partials of (r, z);
 //                scal[1] = den, scal[2] = alpha, scal[5] = scal[0]
 //   cg_step_d:   betanom = sum(part_rz); d = z + betanom / scal[5] d;
-//                scal[0] = scal[3] = betanom, scal[4] = beta; betanom <= goal
-//                (or not a number) sets stop = {1, it}
-// Both leave everything alone once stop[0] is set (stop: device {flag, iteration}).
+//                scal[0] = scal[3] = betanom, scal[4] = beta; sets stop =
+//                {code, it} with code 1 (betanom <= goal: converged), 2
+//                (betanom < 0: preconditioner not positive definite) or 3
+//                (betanom not a number)
+// Both leave everything alone once stop[0] is set (stop: device {code, iteration}).
 hipError_t dot_partial(const double* a, const double* b, int64_t n, double* part, hipStream_t s);
 hipError_t cg_step_x(double* x, double* r, double* z, double* Ad, const double* d, const double* diag, double* scal,
                      const double* part_den, double* part_rz, int64_t n, const double* stop, hipStream_t s);
