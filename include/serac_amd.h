@@ -124,6 +124,19 @@ int sx_mesh_destroy(sx_mesh* mesh);
  * (fn/element_restriction.cpp:216-277,425-467): per-element node lists in
  * lexicographic order, E layout [element][component][node]. order 1..4. */
 int sx_space_create_h1_hex(sx_mesh* mesh, int order, int components, int ordering, sx_space** out);
+/* The same space with the caller's node numbering: elem_nodes[e][j] (host,
+ * [nelem][(order+1)^3]) is the node of element e's lexicographic node j, the
+ * order mfem's ElementRestriction builds from the FE space's element vdofs with
+ * its lexicographic permutation (fn/element_restriction.cpp:216-277 GetElementVDofs
+ * + lexicographic_permutations :18-59, :425-467), so a serac binding passes
+ * mfem's own numbering and the L-vectors are serac's T-vectors on one rank.
+ * The map must relabel the mesh's conforming order-p nodes one to one
+ * (checked exactly; SX_ERR_ARG names the first element and node that do not).
+ * Everything defined on spaces works on it, except multigrid and the learned
+ * preconditioner, which need the library's own lexicographic grid numbering
+ * (SX_ERR_UNSUPPORTED). */
+int sx_space_create_h1_hex_map(sx_mesh* mesh, int order, int components, int ordering, int64_t nnodes,
+                               const int32_t* elem_nodes, sx_space** out);
 int sx_space_info(const sx_space* s, int64_t* nnodes, int64_t* ndofs, int64_t* nelem, int* nodes_per_elem);
 int sx_space_element_nodes(const sx_space* s, int32_t* map_out); /* host [nelem][nodes_per_elem] */
 int sx_space_node_coords(const sx_space* s, double* xyz_out);    /* host [nnodes][3] */
@@ -185,7 +198,15 @@ int sx_form_update_qdata(sx_form* f, int flag);
 /* operator()(t, U...) / operator()(t, differentiate_wrt(U)), fn/functional.hpp:446-513.
  * inputs: ntrials device pointers (T = L on one rank). wrt = -1: residual only;
  * wrt = i: also stores the quadrature-point linearization w.r.t. trial i and
- * returns its gradient operator in *grad_out (owned by the form). */
+ * returns its gradient operator in *grad_out (owned by the form).
+ * wrt = SX_WRT_SHAPE: differentiate_wrt on the shape argument of a
+ * ShapeAwareFunctional (fn/shape_aware_functional.hpp:292-577): returns the
+ * gradient in the shape displacement p (sx_form_set_shape_displacement; zero
+ * when none) at these inputs, whose sx_grad_apply(g, dp, df) gives
+ * df = dR/dp dp, recomputed per application by forward-mode duals through
+ * the geometry (no stored tangent).  Domain integrals of 3-component forms;
+ * diagonal, assembly and the solvers refuse it (SX_ERR_UNSUPPORTED). */
+#define SX_WRT_SHAPE (-2)
 int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out, int wrt, sx_grad** grad_out);
 /* Form options (no reference counterpart: the reference runs serially).
  *   SX_OPT_SCATTER: SX_SCATTER_COLORED (default) launches one kernel per
@@ -266,7 +287,8 @@ typedef struct sx_solve_report {
 
 /* Jacobi-preconditioned CG on the gradient with essential rows/columns
  * eliminated (phy/solid_mechanics.hpp:1133-1139). Solves J_bc x = b. Stopping
- * rule of mfem::CGSolver: (r, z) <= max(rel^2 (r0, z0), abs^2). */
+ * rule of mfem::CGSolver: (r, z) <= max(rel^2 (r0, z0), abs^2); a negative
+ * (r, z) stops unconverged (SX_ERR_NOCONV), as mfem's does. */
 int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
                 double abs_tol, int max_iter, sx_solve_report* report);
 /* Restarted GMRES(restart) with left Jacobi preconditioning on J_bc x = b
@@ -344,7 +366,8 @@ int sx_unet_destroy(sx_unet* net);
  * pairs recorded from Jacobi-CG solves every `stride` iterations: the input
  * P^T r_k and the target avg(x_{k+window}) - avg(x_k), the correction of the
  * next `window` iterations as a hat-weighted vertex mean (P^T x) / (P^T 1). */
-typedef struct sx_learned sx_learned;
+typedef struct sx_learned   sx_learned;
+typedef struct sx_multigrid sx_multigrid;
 enum { SX_PRECOND_JACOBI = 0, SX_PRECOND_RECORD = 1, SX_PRECOND_LEARNED = 2 };
 int sx_learned_create(const sx_space* s, uint64_t seed, sx_learned** out);
 int sx_learned_configure(sx_learned* L, int stride, int window, int max_pairs, int stall_window);
@@ -368,6 +391,18 @@ int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs
  * P^T r / s, and M(r) = D^-1 r + gain s P N(P^T r / s). */
 int sx_learned_train(sx_learned* L, int steps, int batch, double lr, uint64_t seed, double* mean_loss);
 int sx_learned_gain(const sx_learned* L, double* gain);
+/* overrides the target scale; gain 0 switches the network's correction off,
+ * leaving M(r) = B(r) (the ablation of the same solver without the network) */
+int sx_learned_set_gain(sx_learned* L, double gain);
+/* Attaches a multigrid of the same space (borrowed; NULL detaches): M(r)
+ * becomes the V-cycle plus the network's correction, MG(r) + gain s P N(P^T r
+ * / s), in place of D^-1 r + ....  Pairs generated afterwards target what the
+ * V-cycle leaves of the converged correction, avg(e - MG(J_bc e)); the
+ * solves set the hierarchy up per linearization (per Newton call with
+ * sx_mg_set_reuse).  No reference counterpart: a learned correction of the
+ * multigrid that stands in for Preconditioner::HypreAMG
+ * (num/equation_solver.cpp:1014-1021). */
+int sx_learned_set_multigrid(sx_learned* L, sx_multigrid* M);
 /* z = M(r) for the current linearization (device vectors) */
 int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* r, double* z);
 int sx_learned_destroy(sx_learned* L);
@@ -415,7 +450,6 @@ int sx_form_set_shape_displacement(sx_form* f, const double* p);
  * dense inverse.  Essential DOFs are eliminated on every level (a vertex
  * component is constrained when its own node's is) and M is the identity on
  * them.  Defaults: degrees 2 and 2, range 12, 12 power iterations. */
-typedef struct sx_multigrid sx_multigrid;
 int sx_mg_create(const sx_space* s, sx_multigrid** out);
 int sx_mg_configure(sx_multigrid* M, int fine_degree, int coarse_degree, double smoothing_range, int power_iterations);
 /* inside one sx_newton_solve / sx_quasistatic_solve call, one hierarchy

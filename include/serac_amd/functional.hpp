@@ -191,6 +191,16 @@ class FiniteElementSpace {
   {
     detail::check(sx_space_create_h1_hex(mesh.handle(), order, vdim, ordering, &h_));
   }
+  /// The space in the caller's node numbering: element_nodes[e * (order+1)^3 + j]
+  /// is the node of element e's lexicographic node j (mfem's ElementRestriction
+  /// order, fn/element_restriction.cpp:425-467), e.g. mfem's element vdofs of
+  /// the first component after its lexicographic permutation.
+  FiniteElementSpace(Mesh& mesh, int order, int vdim, Ordering::Type ordering, int64_t nnodes,
+                     const std::vector<int32_t>& element_nodes)
+  {
+    detail::check(
+        sx_space_create_h1_hex_map(mesh.handle(), order, vdim, ordering, nnodes, element_nodes.data(), &h_));
+  }
   FiniteElementSpace(const FiniteElementSpace&)            = delete;
   FiniteElementSpace& operator=(const FiniteElementSpace&) = delete;
   ~FiniteElementSpace()

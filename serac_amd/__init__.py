@@ -14,7 +14,7 @@ import numpy as np
 from . import capi
 from .capi import (SX_BQF_FLUX, SX_BQF_PRESSURE, SX_BQF_TRACTION, SX_ORDERING_BYNODES, SX_ORDERING_BYVDIM,
                    SX_QF_HEAT, SX_QF_J2_LINEAR, SX_QF_LINEAR_ELASTIC,
-                   SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SeracAmdError, SolveReport)
+                   SX_QF_MASS_DIFFUSION, SX_QF_NEOHOOKEAN, SX_WRT_SHAPE, SeracAmdError, SolveReport)
 
 __all__ = [
     "DeviceVector", "Mesh", "Space", "Functional", "Gradient", "QuadratureData", "CSRMatrix", "UNet", "Learned", "SeracAmdError", "device_name",
@@ -159,10 +159,18 @@ class Mesh:
 class Space:
     """H1<order, components> on a hex mesh (mfem ParFiniteElementSpace stand-in)."""
 
-    def __init__(self, mesh, order, components=1, ordering=SX_ORDERING_BYVDIM):
+    def __init__(self, mesh, order, components=1, ordering=SX_ORDERING_BYVDIM, element_nodes=None):
+        """element_nodes: optional [nelem][(order+1)^3] node ids in the caller's own numbering
+        (lexicographic local nodes, as mfem's ElementRestriction orders them)."""
         self.mesh = mesh
         self.h = _vp()
-        capi.call("sx_space_create_h1_hex", mesh.h, order, components, ordering, ctypes.byref(self.h))
+        if element_nodes is None:
+            capi.call("sx_space_create_h1_hex", mesh.h, order, components, ordering, ctypes.byref(self.h))
+        else:
+            emap = np.ascontiguousarray(element_nodes, dtype=np.int32)
+            nn = int(emap.max()) + 1 if emap.size else 0
+            capi.call("sx_space_create_h1_hex_map", mesh.h, order, components, ordering, nn, _iptr(emap),
+                      ctypes.byref(self.h))
         nn, nd, ne, npe = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int()
         capi.call("sx_space_info", self.h, ctypes.byref(nn), ctypes.byref(nd), ctypes.byref(ne), ctypes.byref(npe))
         self.nnodes, self.ndof, self.ne, self.npe = nn.value, nd.value, ne.value, npe.value
@@ -518,6 +526,15 @@ class Learned:
         capi.call("sx_learned_gain", self.h, ctypes.byref(g))
         return g.value
 
+    def set_gain(self, gain):
+        """0 switches the network's correction off (the ablation of the same solver)."""
+        capi.call("sx_learned_set_gain", self.h, float(gain))
+
+    def set_multigrid(self, mg):
+        """M(r) = V-cycle(r) + the network's correction of what the V-cycle leaves (None detaches)."""
+        self._mg = mg  # borrowed by the library: keep it alive
+        capi.call("sx_learned_set_multigrid", self.h, mg.h if mg is not None else None)
+
     def set_activation(self, activation, seed=0):
         """SX_UNET_RELU or SX_UNET_IDENTITY; re-initialises the network."""
         capi.call("sx_unet_set_activation", self.net.h, activation, seed)
@@ -677,12 +694,15 @@ class Functional:
 
     def __call__(self, t, *inputs, wrt=-1):
         """Residual into self.output (a DeviceVector reused across calls, like
-        the reference's output_T_); with wrt >= 0 also returns the Gradient."""
+        the reference's output_T_); with wrt >= 0 also returns the Gradient;
+        wrt="shape" returns the gradient in the shape displacement."""
+        if wrt == "shape":
+            wrt = SX_WRT_SHAPE
         arr = (_vp * len(inputs))(*[u.ptr for u in inputs])
         g = _vp()
         capi.call("sx_form_eval", self.h, t, arr, self.output.ptr, wrt, ctypes.byref(g))
         self.synchronize()
-        if wrt >= 0:
+        if wrt >= 0 or wrt == SX_WRT_SHAPE:
             return self.output, Gradient(self, g)
         return self.output
 

@@ -16,6 +16,7 @@ SX_ERR_UNSUPPORTED = 3
 SX_ERR_STATE = 4
 SX_ERR_NOCONV = 5
 
+SX_WRT_SHAPE = -2
 SX_ORDERING_BYNODES = 0
 SX_ORDERING_BYVDIM = 1
 
@@ -108,6 +109,7 @@ SIGNATURES = {
     "sx_mesh_vertices": (_int, [_vp, _D]),
     "sx_mesh_destroy": (_int, [_vp]),
     "sx_space_create_h1_hex": (_int, [_vp, _int, _int, _int, ctypes.POINTER(_vp)]),
+    "sx_space_create_h1_hex_map": (_int, [_vp, _int, _int, _int, ctypes.c_int64, _I32, ctypes.POINTER(_vp)]),
     "sx_space_info": (_int, [_vp, _I64, _I64, _I64, ctypes.POINTER(_int)]),
     "sx_space_element_nodes": (_int, [_vp, _I32]),
     "sx_space_node_coords": (_int, [_vp, _D]),
@@ -128,6 +130,8 @@ SIGNATURES = {
     "sx_learned_train": (_int, [_vp, _int, _int, ctypes.c_double, ctypes.c_uint64, _D]),
     "sx_learned_generate_pairs": (_int, [_vp, _vp, _I32, ctypes.c_int64, _int, ctypes.c_uint64]),
     "sx_learned_gain": (_int, [_vp, _D]),
+    "sx_learned_set_gain": (_int, [_vp, ctypes.c_double]),
+    "sx_learned_set_multigrid": (_int, [_vp, _vp]),
     "sx_learned_apply": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp]),
     "sx_learned_destroy": (_int, [_vp]),
     "sx_mg_create": (_int, [_vp, ctypes.POINTER(_vp)]),

@@ -166,7 +166,12 @@ struct sx_space {
   // DOF-level sparsity of forms on this space (built on first assembly)
   std::vector<int64_t> csr_row;
   std::vector<int32_t> csr_col;
+  // numbering: native = the library's own (lexicographic node grid on cuboid
+  // meshes); otherwise node_of_own[own node] = the caller's node id
+  bool                 native = true;
+  std::vector<int32_t> node_of_own;
   int64_t          ndof() const { return nnodes * nc; }
+  int64_t          node_id(int64_t own) const { return native ? own : int64_t(node_of_own[size_t(own)]); }
 };
 
 // Assembled linearization (Gradient::assemble, fn/functional.hpp:569-667): CSR
@@ -218,9 +223,10 @@ struct BoundaryIntegral {
 };
 
 // The learned coarse-grid preconditioner of a structured vector space:
-//   M(r) = D^-1 r + gain s P N(P^T r / s),  s = rms(P^T r)
-// with P the vertex (Q1) -> order-p interpolation and N the U-Net; plus the
-// bank of (coarse residual, coarse converged correction) pairs it trains on.
+//   M(r) = B(r) + gain s P N(P^T r / s),  s = rms(P^T r)
+// with B = D^-1 (Jacobi) or, with a multigrid attached, the V-cycle; P the
+// vertex (Q1) -> order-p interpolation and N the U-Net; plus the bank of
+// (coarse residual, coarse correction) pairs it trains on.
 struct sx_learned {
   const sx_space*                            space = nullptr;
   sx_unet*                                   net   = nullptr;
@@ -233,6 +239,9 @@ struct sx_learned {
   int                                        next = 0;  // ring position once the bank is full
   int64_t                                    fallbacks = 0;
   double                                     gain      = 1.0;  // target scale: N learns t / (s gain)
+  // with a multigrid attached: M(r) = MG(r) + gain s P N(P^T r / s), the
+  // network trained on the V-cycle's remaining error (borrowed, same space)
+  sx_multigrid*                              mg = nullptr;
   ~sx_learned()
   {
     if (net) sx_unet_destroy(net);
@@ -243,6 +252,7 @@ struct sx_grad {
   sx_form* form = nullptr;
   int      wrt  = 0;
   bool     valid = false;
+  bool     shape = false;  // the derivative in the shape displacement (SX_WRT_SHAPE)
 };
 
 // Geometric multigrid preconditioner of a structured 3-component space: a
@@ -300,6 +310,11 @@ struct sx_form {
   sx_multigrid*                          mg           = nullptr;        // borrowed: multigrid-PCG in the Newton loops
   const double*                          shape        = nullptr;        // borrowed: shape displacement (device)
   int                                    learn_mode   = 0;              // SX_PRECOND_*
+  // differentiate_wrt on the shape (SX_WRT_SHAPE): the gradient, and the
+  // trial inputs and shape displacement of its linearization point
+  std::unique_ptr<sx_grad>                   shape_grad;
+  std::vector<std::unique_ptr<DeviceBuffer>> shape_u;
+  DeviceBuffer                               shape_p;
   // solver scratch
   DeviceBuffer part, scal;
   ~sx_form()
@@ -637,6 +652,38 @@ bool depends(const Integral& I, int trial)
   return std::find(I.depends_on.begin(), I.depends_on.end(), trial) != I.depends_on.end();
 }
 
+// a linearized gradient in a trial argument (the shape gradient only applies)
+void require_trial_grad(const sx_grad* g, const char* what)
+{
+  SX_REQUIRE(!g->shape, SX_ERR_UNSUPPORTED, "the shape gradient supports sx_grad_apply only");
+  SX_REQUIRE(g->valid, SX_ERR_STATE, what);
+}
+
+// df = dR/dp dx: the shape derivative of the domain integrals at the
+// linearization point saved by sx_form_eval(..., SX_WRT_SHAPE, ...)
+void apply_shape_gradient(sx_grad* g, const double* dx, double* df)
+{
+  sx_form* f = g->form;
+  SX_HIP(sxk::vec_fill(df, 0.0, f->test->ndof(), f->stream));
+  for (auto& I : f->integrals) {
+    sxk::ElemLaunch P = base_launch(f, *I, sxk::MODE_SHAPE_JVP);
+    P.u               = f->shape_u[size_t(I->depends_on[0])]->as<const double>();
+    P.shape           = f->shape_p.as<const double>();
+    P.dshape          = dx;
+    P.r               = df;
+    for (size_t c = 0; c + 1 < I->color_off.size(); c++) {
+      P.slot_begin = I->color_off[c];
+      P.slot_end   = I->color_off[c + 1];
+      if (P.slot_end <= P.slot_begin) continue;
+      hipError_t err = hipSuccess;
+      SX_REQUIRE(sxk::launch_element(P, &err), SX_ERR_UNSUPPORTED,
+                 "shape derivative not compiled for order " + std::to_string(P.order) + ", qfunction " +
+                     std::to_string(P.qf));
+      SX_HIP(err);
+    }
+  }
+}
+
 // df = J dx; out_zeroed: df already holds zeros (the fill is skipped); lo:
 // the domain integrals use their BF16 tangent copies
 void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr,
@@ -645,6 +692,12 @@ void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullp
   SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
   SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
   SX_REQUIRE(dx && df, SX_ERR_ARG, "null vector");
+  if (g->shape) {
+    SX_REQUIRE(vprec == sxk::PREC_F64 && !ev, SX_ERR_UNSUPPORTED, "the shape gradient applies to FP64 vectors");
+    apply_shape_gradient(g, static_cast<const double*>(dx), static_cast<double*>(df));
+    if (nlaunch) *nlaunch = 0;
+    return;
+  }
   SX_REQUIRE(vprec == sxk::PREC_F64 || vprec == sxk::PREC_F32, SX_ERR_ARG, "J.v vectors are FP64 or FP32");
   sx_form* f = g->form;
   if (out_zeroed) {
@@ -852,33 +905,94 @@ int sx_mesh_destroy(sx_mesh* mesh)
 // ---------------------------------------------------------------------------
 // spaces
 // ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+std::unique_ptr<sx_space> new_space(sx_mesh* mesh, int order, int components, int ordering)
+{
+  SX_REQUIRE(mesh, SX_ERR_ARG, "null argument");
+  SX_REQUIRE(order >= 1 && order <= 4, SX_ERR_UNSUPPORTED, "H1 hex order must be 1..4");
+  SX_REQUIRE(components == 1 || components == 3, SX_ERR_UNSUPPORTED, "components must be 1 or 3");
+  SX_REQUIRE(ordering == SX_ORDERING_BYNODES || ordering == SX_ORDERING_BYVDIM, SX_ERR_ARG, "bad ordering");
+  ensure_device();
+  auto s      = std::make_unique<sx_space>();
+  s->mesh     = mesh;
+  s->order    = order;
+  s->nc       = components;
+  s->ordering = ordering;
+  s->npe      = (order + 1) * (order + 1) * (order + 1);
+  return s;
+}
+
+// colouring and the device copies of a space whose emap / nnodes are set
+void finish_space(sx_space* s)
+{
+  SX_REQUIRE(s->ndof() < INT32_MAX, SX_ERR_UNSUPPORTED, "more than 2^31 DOFs");
+  s->ncolors = sxh::color_elements(s->mesh->m, s->order, s->emap, s->nnodes, s->color);
+  s->d_emap.upload(s->emap);
+  // node -> element entries in increasing element order
+  std::vector<int64_t> offs(size_t(s->nnodes) + 1, 0);
+  for (int v : s->emap) offs[size_t(v) + 1]++;
+  for (size_t i = 0; i < size_t(s->nnodes); i++) offs[i + 1] += offs[i];
+  std::vector<int64_t> pos(offs.begin(), offs.end() - 1);
+  std::vector<int32_t> ents(s->emap.size());
+  for (size_t k = 0; k < s->emap.size(); k++) ents[size_t(pos[size_t(s->emap[k])]++)] = int32_t(k);
+  s->d_offs.upload(offs);
+  s->d_ents.upload(ents);
+}
+
+}  // namespace
+
+extern "C" {
+
 int sx_space_create_h1_hex(sx_mesh* mesh, int order, int components, int ordering, sx_space** out)
 {
   return api([&] {
-    SX_REQUIRE(mesh && out, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(order >= 1 && order <= 4, SX_ERR_UNSUPPORTED, "H1 hex order must be 1..4");
-    SX_REQUIRE(components == 1 || components == 3, SX_ERR_UNSUPPORTED, "components must be 1 or 3");
-    SX_REQUIRE(ordering == SX_ORDERING_BYNODES || ordering == SX_ORDERING_BYVDIM, SX_ERR_ARG, "bad ordering");
-    ensure_device();
-    auto s      = std::make_unique<sx_space>();
-    s->mesh     = mesh;
-    s->order    = order;
-    s->nc       = components;
-    s->ordering = ordering;
-    s->npe      = (order + 1) * (order + 1) * (order + 1);
-    s->nnodes   = sxh::number_nodes(mesh->m, order, s->emap);
-    SX_REQUIRE(s->ndof() < INT32_MAX, SX_ERR_UNSUPPORTED, "more than 2^31 DOFs");
-    s->ncolors = sxh::color_elements(mesh->m, order, s->emap, s->nnodes, s->color);
-    s->d_emap.upload(s->emap);
-    // node -> element entries in increasing element order
-    std::vector<int64_t> offs(size_t(s->nnodes) + 1, 0);
-    for (int v : s->emap) offs[size_t(v) + 1]++;
-    for (size_t i = 0; i < size_t(s->nnodes); i++) offs[i + 1] += offs[i];
-    std::vector<int64_t> pos(offs.begin(), offs.end() - 1);
-    std::vector<int32_t> ents(s->emap.size());
-    for (size_t k = 0; k < s->emap.size(); k++) ents[size_t(pos[size_t(s->emap[k])]++)] = int32_t(k);
-    s->d_offs.upload(offs);
-    s->d_ents.upload(ents);
+    SX_REQUIRE(out, SX_ERR_ARG, "null argument");
+    auto s    = new_space(mesh, order, components, ordering);
+    s->nnodes = sxh::number_nodes(mesh->m, order, s->emap);
+    finish_space(s.get());
+    *out = s.release();
+  });
+}
+
+int sx_space_create_h1_hex_map(sx_mesh* mesh, int order, int components, int ordering, int64_t nnodes,
+                               const int32_t* elem_nodes, sx_space** out)
+{
+  return api([&] {
+    SX_REQUIRE(out && elem_nodes, SX_ERR_ARG, "null argument");
+    auto               s = new_space(mesh, order, components, ordering);
+    std::vector<int>   own;
+    const int64_t      nown = sxh::number_nodes(mesh->m, order, own);
+    SX_REQUIRE(nnodes == nown, SX_ERR_ARG,
+               "the map numbers " + std::to_string(nnodes) + " nodes; the order-" + std::to_string(order) +
+                   " conforming space of this mesh has " + std::to_string(nown));
+    // the caller's map must relabel the mesh's conforming nodes one to one:
+    // the same caller node wherever the library sees the same node, and no
+    // caller node on two library nodes
+    std::vector<int32_t> ext_of_own(size_t(nown), -1), own_of_ext(size_t(nnodes), -1);
+    for (size_t k = 0; k < own.size(); k++) {
+      const int32_t a = own[k], b = elem_nodes[k];
+      const int64_t e = int64_t(k) / s->npe, j = int64_t(k) % s->npe;
+      SX_REQUIRE(b >= 0 && b < nnodes, SX_ERR_ARG,
+                 "element " + std::to_string(e) + " node " + std::to_string(j) + ": node id " + std::to_string(b) +
+                     " out of range [0, " + std::to_string(nnodes) + ")");
+      if (ext_of_own[size_t(a)] < 0 && own_of_ext[size_t(b)] < 0) {
+        ext_of_own[size_t(a)] = b;
+        own_of_ext[size_t(b)] = a;
+      }
+      SX_REQUIRE(ext_of_own[size_t(a)] == b && own_of_ext[size_t(b)] == a, SX_ERR_ARG,
+                 "element " + std::to_string(e) + " node " + std::to_string(j) +
+                     ": the map does not number the mesh's shared nodes consistently (node id " +
+                     std::to_string(b) + ")");
+    }
+    s->nnodes = nnodes;
+    s->emap.assign(elem_nodes, elem_nodes + own.size());
+    s->native = true;
+    for (int64_t a = 0; a < nown; a++) s->native = s->native && ext_of_own[size_t(a)] == a;
+    if (!s->native) s->node_of_own.assign(ext_of_own.begin(), ext_of_own.end());
+    finish_space(s.get());
     *out = s.release();
   });
 }
@@ -946,7 +1060,7 @@ int sx_space_boundary_dofs(const sx_space* s, int face_mask, int comp_mask, int3
             if ((face_mask >> (2 * d + 1)) & 1) on = on || idx[d] == g[d] - 1;
           }
           if (!on) continue;
-          const int64_t node = i + g[0] * (j + g[1] * k);
+          const int64_t node = s->node_id(i + g[0] * (j + g[1] * k));
           for (int c = 0; c < s->nc; c++)
             if ((comp_mask >> c) & 1)
               dofs.push_back(int32_t(s->ordering ? node * s->nc + c : int64_t(c) * s->nnodes + node));
@@ -1063,6 +1177,8 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
     SX_REQUIRE(ts->mesh == vs->mesh, SX_ERR_UNSUPPORTED, "test and trial spaces must share a mesh");
     SX_REQUIRE(ts->order == vs->order && ts->nc == vs->nc && ts->ordering == vs->ordering, SX_ERR_UNSUPPORTED,
                "test and trial spaces must have the same order, components and ordering");
+    SX_REQUIRE(ts == vs || (ts->native && vs->native) || ts->emap == vs->emap, SX_ERR_UNSUPPORTED,
+               "test and trial spaces must number their nodes alike");
     SX_REQUIRE(nc_qf < 0 || nc_qf == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
     const int ssize = qf_state_size(qfunc_id);
     SX_REQUIRE(ssize == 0 || qdata, SX_ERR_ARG, "this qfunction needs quadrature-point state (sx_qdata)");
@@ -1308,8 +1424,13 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
 {
   return api([&] {
     SX_REQUIRE(f && inputs && out, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(wrt >= -1 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
+    SX_REQUIRE((wrt >= -1 || wrt == SX_WRT_SHAPE) && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     for (size_t i = 0; i < f->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
+    if (wrt == SX_WRT_SHAPE) {
+      SX_REQUIRE(f->test->nc == 3, SX_ERR_UNSUPPORTED, "shape derivatives need a 3-component test space");
+      SX_REQUIRE(f->bintegrals.empty(), SX_ERR_UNSUPPORTED,
+                 "shape derivatives of boundary integrals are not built (domain integrals only)");
+    }
     SX_HIP(sxk::vec_fill(out, 0.0, f->test->ndof(), f->stream));
     const int nq = (f->test->order + 1) * (f->test->order + 1) * (f->test->order + 1);
     for (auto& I : f->integrals) {
@@ -1337,6 +1458,30 @@ int sx_form_eval(sx_form* f, double t, const double* const* inputs, double* out,
     if (wrt >= 0) {
       f->grads[size_t(wrt)]->valid = true;
       if (grad_out) *grad_out = f->grads[size_t(wrt)].get();
+    }
+    if (wrt == SX_WRT_SHAPE) {
+      // keep the linearization point: the trial inputs and the shape displacement
+      if (!f->shape_grad) {
+        f->shape_grad        = std::make_unique<sx_grad>();
+        f->shape_grad->form  = f;
+        f->shape_grad->wrt   = SX_WRT_SHAPE;
+        f->shape_grad->shape = true;
+      }
+      f->shape_u.resize(f->trials.size());
+      for (size_t i = 0; i < f->trials.size(); i++) {
+        if (!f->shape_u[i]) f->shape_u[i] = std::make_unique<DeviceBuffer>();
+        const size_t bytes = size_t(f->trials[i]->ndof()) * sizeof(double);
+        f->shape_u[i]->allocate(bytes);
+        SX_HIP(hipMemcpyAsync(f->shape_u[i]->ptr, inputs[i], bytes, hipMemcpyDeviceToDevice, f->stream));
+      }
+      const size_t pbytes = size_t(f->test->ndof()) * sizeof(double);
+      f->shape_p.allocate(pbytes);
+      if (f->shape)
+        SX_HIP(hipMemcpyAsync(f->shape_p.ptr, f->shape, pbytes, hipMemcpyDeviceToDevice, f->stream));
+      else
+        SX_HIP(sxk::vec_fill(f->shape_p.as<double>(), 0.0, f->test->ndof(), f->stream));
+      f->shape_grad->valid = true;
+      if (grad_out) *grad_out = f->shape_grad.get();
     }
   });
 }
@@ -1411,7 +1556,7 @@ int sx_grad_diagonal(sx_grad* g, double* diag)
 {
   return api([&] {
     SX_REQUIRE(g && g->form && diag, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "diagonal requested before a linearization");
+    require_trial_grad(g, "diagonal requested before a linearization");
     sx_form* f = g->form;
     SX_HIP(sxk::vec_fill(diag, 0.0, f->test->ndof(), f->stream));
     for (auto& I : f->integrals)
@@ -1426,7 +1571,7 @@ int sx_grad_assemble(sx_grad* g, sx_csr** out)
 {
   return api([&] {
     SX_REQUIRE(g && g->form && out, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "assembly requested before a linearization");
+    require_trial_grad(g, "assembly requested before a linearization");
     sx_form*  f  = g->form;
     sx_space* ts = const_cast<sx_space*>(f->test);
     SX_REQUIRE(ts->ndof() < (int64_t(1) << 31), SX_ERR_UNSUPPORTED, "CSR columns are int32");
@@ -1632,19 +1777,31 @@ void compute_diag(sx_grad* g, const int32_t* ess_dev, int64_t ness, double* diag
   SX_HIP(sxk::set_rows(diag, ess_dev, ness, 1.0, f->stream));
 }
 
-void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, double* x, double rel, double abs_tol,
-        int max_iter, Workspace& w, sx_solve_report* rep, const CgHook* hook = nullptr)
+// The device vectors of one Jacobi-CG solve: five of ndof doubles and the
+// reduction_blocks() partials of (r, z).
+struct CgVectors {
+  double *r, *z, *d, *Ad, *diag, *part_rz;
+};
+
+CgVectors cg_vectors(Workspace& w, int64_t n)
 {
-  sx_form*      f = g->form;
-  const int64_t n = f->test->ndof();
-  hipStream_t   s = f->stream;
   w.r.allocate(size_t(n) * 8);
   w.z.allocate(size_t(n) * 8);
   w.d.allocate(size_t(n) * 8);
   w.Ad.allocate(size_t(n) * 8);
   w.diag.allocate(size_t(n) * 8);
-  double *r = w.r.as<double>(), *z = w.z.as<double>(), *d = w.d.as<double>(), *Ad = w.Ad.as<double>(),
-         *diag = w.diag.as<double>();
+  w.part2.allocate(size_t(sxk::reduction_blocks()) * 8);
+  return {w.r.as<double>(), w.z.as<double>(), w.d.as<double>(), w.Ad.as<double>(), w.diag.as<double>(),
+          w.part2.as<double>()};
+}
+
+void cg_core(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, double* x, double rel,
+             double abs_tol, int max_iter, const CgVectors& v, sx_solve_report* rep, const CgHook* hook)
+{
+  sx_form*      f = g->form;
+  const int64_t n = f->test->ndof();
+  hipStream_t   s = f->stream;
+  double *r = v.r, *z = v.z, *d = v.d, *Ad = v.Ad, *diag = v.diag;
   double *part = f->part.as<double>(), *scal = f->scal.as<double>();
 
   // Jacobi diagonal with identity rows for essential DOFs
@@ -1680,8 +1837,7 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
   // after every iteration; the J.v launches queued behind the converged
   // iteration (fewer than a chunk) are wasted.  A hook sees every iteration,
   // so with one the chunks are single iterations.
-  w.part2.allocate(size_t(sxk::reduction_blocks()) * 8);
-  double* part_rz = w.part2.as<double>();
+  double* part_rz = v.part_rz;
   double* stop    = scal + 8;
   SX_HIP(sxk::vec_fill(stop, 0.0, 2, s));
   apply_bc(d, Ad);
@@ -1709,6 +1865,12 @@ void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, doubl
     }
   }
   rep->linear_iterations = max_iter;
+}
+
+void cg(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, double* x, double rel, double abs_tol,
+        int max_iter, Workspace& w, sx_solve_report* rep, const CgHook* hook = nullptr)
+{
+  cg_core(g, ess_dev, ness, b, x, rel, abs_tol, max_iter, cg_vectors(w, g->form->test->ndof()), rep, hook);
 }
 
 // ---------------------------------------------------------------------------
@@ -2262,22 +2424,41 @@ void gmres(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double
 }
 
 // ---- learned preconditioner ----
-void learned_apply(sx_learned* L, sx_form* f, const double* diag, const int32_t* ess, int64_t ness, const double* r,
+void learned_apply(sx_learned* L, sx_grad* g, const double* diag, const int32_t* ess, int64_t ness, const double* r,
                    double* z)
 {
+  sx_form*      f  = g->form;
   hipStream_t   s  = f->stream;
   const int64_t n  = f->test->ndof();
   const int64_t nc = L->nv * 3;
-  SX_HIP(sxk::jacobi_dot(r, diag, z, n, f->part.as<double>(), f->scal.as<double>(), 6, s));  // z = D^-1 r
-  SX_HIP(sxu::fe_restrict(L->G, r, L->rc.as<float>(), s));
-  SX_HIP(hipMemsetAsync(L->ss.ptr, 0, sizeof(double), s));
-  SX_HIP(sxu::sumsq(L->rc.as<float>(), nc, L->ss.as<double>(), s));
-  SX_HIP(sxu::normalize(L->rc.as<float>(), L->ss.as<const double>(), nc, nc, L->xin.as<float>(), s));
-  SX_HIP(hipStreamSynchronize(s));
-  const int rc = sx_unet_forward(L->net, L->xin.as<const float>(), L->y.as<float>());
-  if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-  SX_HIP(sxu::fe_prolong_add(L->G, L->y.as<const float>(), L->ss.as<const double>(), nc, z, s, L->gain));
+  if (L->mg)
+    mg_apply(L->mg, g, ess, ness, r, z);  // z = V-cycle(r); its hierarchy is set up by the caller
+  else
+    SX_HIP(sxk::jacobi_dot(r, diag, z, n, f->part.as<double>(), f->scal.as<double>(), 6, s));  // z = D^-1 r
+  if (L->gain != 0.0) {
+    SX_HIP(sxu::fe_restrict(L->G, r, L->rc.as<float>(), s));
+    SX_HIP(hipMemsetAsync(L->ss.ptr, 0, sizeof(double), s));
+    SX_HIP(sxu::sumsq(L->rc.as<float>(), nc, L->ss.as<double>(), s));
+    SX_HIP(sxu::normalize(L->rc.as<float>(), L->ss.as<const double>(), nc, nc, L->xin.as<float>(), s));
+    SX_HIP(hipStreamSynchronize(s));
+    const int rc = sx_unet_forward(L->net, L->xin.as<const float>(), L->y.as<float>());
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    SX_HIP(sxu::fe_prolong_add(L->G, L->y.as<const float>(), L->ss.as<const double>(), nc, z, s, L->gain));
+  }
   SX_HIP(sxk::set_rows(z, ess, ness, 0.0, s));
+}
+
+// the hierarchy of a learned preconditioner's multigrid for this linear
+// solve: rebuilt unless a Newton loop lets one serve several linearizations
+void learned_mg_setup(sx_learned* L, sx_grad* g, const int32_t* ess, int64_t ness, bool allow_reuse)
+{
+  if (!L->mg) return;
+  sx_multigrid* M = L->mg;
+  if (!allow_reuse || M->uses >= M->reuse || M->set_g != g) {
+    mg_setup(M, g, ess, ness);
+    M->uses = 0;
+  }
+  M->uses++;
 }
 
 // a slot of the pair bank for a new pair: appended while the bank has room,
@@ -2426,7 +2607,7 @@ struct PairRecorder {
 // Jacobi-CG it replaces; when that measure has not improved for `stall`
 // iterations, Jacobi-CG finishes the solve from the current iterate.
 void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
-         double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep)
+         double abs_tol, int max_iter, Workspace& w, sx_solve_report* rep, bool allow_reuse = false)
 {
   sx_form*      f = g->form;
   const int64_t n = f->test->ndof();
@@ -2453,6 +2634,7 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
     SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
   };
   compute_diag(g, ess, ness, diag);
+  learned_mg_setup(L, g, ess, ness, allow_reuse);
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));
   SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   const double mu0  = measure();
@@ -2462,7 +2644,7 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
   rep->converged    = mu0 <= goal;
   rep->linear_iterations = 0;
   if (rep->converged) return;
-  learned_apply(L, f, diag, ess, ness, r, z);
+  learned_apply(L, g, diag, ess, ness, r, z);
   SX_HIP(hipMemcpyAsync(d, z, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
   apply_bc(d, q);
   double dq = dot(d, q), alpha = dot(d, r) / dq;
@@ -2486,7 +2668,7 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
       stalled = true;
       break;
     }
-    learned_apply(L, f, diag, ess, ness, r, z);
+    learned_apply(L, g, diag, ess, ness, r, z);
     const double beta = dot(z, q) / dq;
     SX_HIP(sxk::vec_axpby(1.0, z, -beta, d, n, s));  // d = z - beta d
     apply_bc(d, q);
@@ -2552,12 +2734,13 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
     if (mg) {
       mg_apply(mg, g, ess, ness, in, outv);
     } else if (L) {
-      learned_apply(L, f, diag, ess, ness, in, outv);
+      learned_apply(L, g, diag, ess, ness, in, outv);
     } else {
       SX_HIP(sxk::jacobi_apply(in, diag, outv, n, s));
     }
   };
   compute_diag(g, ess, ness, diag);
+  if (L && !mg) learned_mg_setup(L, g, ess, ness, allow_reuse);
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));
   std::vector<double> H(size_t(m + 1) * size_t(m), 0.0);
   std::vector<double> cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), gv(static_cast<size_t>(m + 1));
@@ -2645,9 +2828,9 @@ void linear_solve(sx_grad* g, const int32_t* ess, int64_t ness, const double* b,
 {
   sx_form* f = g->form;
   if (f->learned && f->learn_mode == SX_PRECOND_LEARNED && f->linsolver == SX_LINSOLVER_GMRES) {
-    fgmres(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep);
+    fgmres(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, f->gmres_restart, w, rep, nullptr, true);
   } else if (f->learned && f->learn_mode == SX_PRECOND_LEARNED) {
-    fcg(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, w, rep);
+    fcg(g, f->learned, ess, ness, b, x, rel, abs_tol, max_iter, w, rep, true);
   } else if (f->learned && f->learn_mode == SX_PRECOND_RECORD) {
     PairRecorder rec(f->learned, f->stream);
     CgHook       hook = [&rec](int it, const double* xx, const double* rr) { rec(it, xx, rr); };
@@ -2687,7 +2870,7 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "CG on a gradient before a linearization");
+    require_trial_grad(g, "CG on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
@@ -2707,7 +2890,7 @@ extern "C" int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness,
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "GMRES on a gradient before a linearization");
+    require_trial_grad(g, "GMRES on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
@@ -2765,6 +2948,7 @@ int sx_newton_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t ness, 
   int             rc = api([&] {
     SX_REQUIRE(f && x, SX_ERR_ARG, "null argument");
     if (f->mg) f->mg->uses = INT32_MAX;  // a fresh hierarchy for this call's essential DOFs
+    if (f->learned && f->learned->mg) f->learned->mg->uses = INT32_MAX;
     SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     const int64_t n = f->test->ndof();
@@ -2789,6 +2973,7 @@ int sx_quasistatic_solve(sx_form* f, int wrt, const int32_t* ess_dofs, int64_t n
   return api([&] {
     SX_REQUIRE(f && x && opts && nsteps >= 1, SX_ERR_ARG, "bad argument");
     if (f->mg) f->mg->uses = INT32_MAX;  // a fresh hierarchy for this call's essential DOFs
+    if (f->learned && f->learned->mg) f->learned->mg->uses = INT32_MAX;
     SX_REQUIRE(wrt >= 0 && wrt < int(f->trials.size()), SX_ERR_ARG, "wrt out of range");
     SX_REQUIRE(ness == 0 || (ess_dofs && ess_final), SX_ERR_ARG, "null essential DOF data");
     const int64_t n = f->test->ndof();
@@ -2869,6 +3054,7 @@ int sx_mg_create(const sx_space* sp, sx_multigrid** out)
     SX_REQUIRE(sp && out, SX_ERR_ARG, "null argument");
     const auto& m = sp->mesh->m;
     SX_REQUIRE(m.structured, SX_ERR_UNSUPPORTED, "multigrid needs a structured (cuboid) mesh");
+    SX_REQUIRE(sp->native, SX_ERR_UNSUPPORTED, "multigrid needs the library's own (lexicographic) node numbering");
     SX_REQUIRE(sp->nc == 3 && sp->ordering == SX_ORDERING_BYVDIM, SX_ERR_UNSUPPORTED,
                "multigrid acts on 3-component byVDIM spaces");
     auto M   = std::make_unique<sx_multigrid>();
@@ -3016,7 +3202,7 @@ int sx_mg_setup(sx_multigrid* M, sx_grad* g, const int32_t* ess_dofs, int64_t ne
 {
   return api([&] {
     SX_REQUIRE(M && g && g->form, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid setup on a gradient before a linearization");
+    require_trial_grad(g, "multigrid setup on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     for (int64_t k = 0; k < ness; k++)
       SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
@@ -3055,7 +3241,7 @@ int sx_mg_pcg_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int64_
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && M && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid-CG on a gradient before a linearization");
+    require_trial_grad(g, "multigrid-CG on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     for (int64_t k = 0; k < ness; k++)
       SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < M->lv[0].n, SX_ERR_ARG, "essential DOF out of range");
@@ -3075,7 +3261,7 @@ int sx_mg_gmres_solve(sx_grad* g, sx_multigrid* M, const int32_t* ess_dofs, int6
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && M && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "multigrid-GMRES on a gradient before a linearization");
+    require_trial_grad(g, "multigrid-GMRES on a gradient before a linearization");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     SX_REQUIRE(M->space == g->form->test, SX_ERR_ARG, "the multigrid belongs to another space");
     for (int64_t k = 0; k < ness; k++)
@@ -3101,6 +3287,7 @@ int sx_form_set_shape_displacement(sx_form* f, const double* p)
                  "a shape displacement needs trial spaces on the test space's nodes");
     f->shape = p;
     for (auto& g : f->grads) g->valid = false;  // linearizations at another shape are stale
+    if (f->shape_grad) f->shape_grad->valid = false;
   });
 }
 
@@ -3127,6 +3314,8 @@ int sx_learned_create(const sx_space* sp, uint64_t seed, sx_learned** out)
     SX_REQUIRE(sp && out, SX_ERR_ARG, "null argument");
     const auto& m = sp->mesh->m;
     SX_REQUIRE(m.structured, SX_ERR_UNSUPPORTED, "the learned preconditioner needs a structured (cuboid) mesh");
+    SX_REQUIRE(sp->native, SX_ERR_UNSUPPORTED,
+               "the learned preconditioner needs the library's own (lexicographic) node numbering");
     SX_REQUIRE(sp->nc == 3 && sp->ordering == SX_ORDERING_BYVDIM, SX_ERR_UNSUPPORTED,
                "the learned preconditioner acts on 3-component byVDIM spaces");
     auto L   = std::make_unique<sx_learned>();
@@ -3177,7 +3366,7 @@ int sx_learned_record_cg(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(L && g && g->form && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    require_trial_grad(g, "solve on a gradient before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
@@ -3213,19 +3402,24 @@ int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs
 {
   return api([&] {
     SX_REQUIRE(L && g && g->form && npairs >= 0, SX_ERR_ARG, "bad argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "pairs from a gradient before a linearization");
+    require_trial_grad(g, "pairs from a gradient before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     sx_form*      f  = g->form;
     hipStream_t   s  = f->stream;
     const int64_t n  = f->test->ndof();
     const int64_t nc = L->nv * 3;
-    DeviceBuffer  ess, e, r, ec, one;
+    DeviceBuffer  ess, e, r, ec, one, z1;
     upload_ess(ess, ess_dofs, ness, n);
     e.allocate(size_t(n) * 8);
     r.allocate(size_t(n) * 8);
     ec.allocate(size_t(nc) * 4);
     one.upload(std::vector<double>{double(nc)});  // prolongation scale sqrt(nc / nc) = 1
+    if (L->mg) {
+      z1.allocate(size_t(n) * 8);
+      mg_setup(L->mg, g, ess.as<const int32_t>(), ness);
+      L->mg->uses = INT32_MAX;  // the next solve sets its own hierarchy up
+    }
     std::mt19937_64 rng(seed);
     for (int p = 0; p < npairs; p++) {
       // e = P e_c (zero on essential DOFs), r = J_bc e: e is the converged
@@ -3237,6 +3431,12 @@ int sx_learned_generate_pairs(sx_learned* L, sx_grad* g, const int32_t* ess_dofs
       SX_HIP(sxk::set_rows(e.as<double>(), ess.as<const int32_t>(), ness, 0.0, s));
       apply_gradient(g, e.as<const double>(), r.as<double>());
       SX_HIP(sxk::copy_rows(r.as<double>(), e.as<const double>(), ess.as<const int32_t>(), ness, s));
+      if (L->mg) {
+        // the target is what the V-cycle leaves of the correction: e - MG(r)
+        mg_apply(L->mg, g, ess.as<const int32_t>(), ness, r.as<const double>(), z1.as<double>());
+        SX_HIP(sxk::vec_axpy(-1.0, z1.as<const double>(), e.as<double>(), n, s));
+        SX_HIP(sxk::set_rows(e.as<double>(), ess.as<const int32_t>(), ness, 0.0, s));
+      }
       const int k = take_pair_slot(L);
       SX_HIP(sxu::fe_restrict(L->G, r.as<const double>(), L->pr[size_t(k)]->as<float>(), s));
       SX_HIP(sxu::fe_restrict(L->G, e.as<const double>(), L->pt[size_t(k)]->as<float>(), s, true));
@@ -3320,17 +3520,36 @@ int sx_learned_gain(const sx_learned* L, double* gain)
   });
 }
 
+int sx_learned_set_gain(sx_learned* L, double gain)
+{
+  return api([&] {
+    SX_REQUIRE(L && std::isfinite(gain), SX_ERR_ARG, "bad argument");
+    L->gain = gain;
+  });
+}
+
+int sx_learned_set_multigrid(sx_learned* L, sx_multigrid* M)
+{
+  return api([&] {
+    SX_REQUIRE(L, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(!M || M->space == L->space, SX_ERR_ARG, "the multigrid belongs to another space");
+    L->mg = M;
+    if (M) M->uses = INT32_MAX;
+  });
+}
+
 int sx_learned_apply(sx_learned* L, sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* r, double* z)
 {
   return api([&] {
     SX_REQUIRE(L && g && g->form && r && z, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "preconditioner applied before a linearization");
+    require_trial_grad(g, "preconditioner applied before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     w.diag.allocate(size_t(g->form->test->ndof()) * 8);
     compute_diag(g, w.ess.as<const int32_t>(), ness, w.diag.as<double>());
-    learned_apply(L, g->form, w.diag.as<const double>(), w.ess.as<const int32_t>(), ness, r, z);
+    learned_mg_setup(L, g, w.ess.as<const int32_t>(), ness, false);
+    learned_apply(L, g, w.diag.as<const double>(), w.ess.as<const int32_t>(), ness, r, z);
     SX_HIP(hipStreamSynchronize(g->form->stream));
   });
 }
@@ -3360,7 +3579,7 @@ int sx_fgmres_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t 
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    require_trial_grad(g, "solve on a gradient before a linearization");
     SX_REQUIRE(!L || g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     SX_REQUIRE(ness == 0 || ess_dofs, SX_ERR_ARG, "null essential DOF list");
     Workspace w;
@@ -3379,7 +3598,7 @@ int sx_fcg_solve(sx_grad* g, sx_learned* L, const int32_t* ess_dofs, int64_t nes
   sx_solve_report rep{};
   int             rc = api([&] {
     SX_REQUIRE(g && g->form && L && b && x, SX_ERR_ARG, "null argument");
-    SX_REQUIRE(g->valid, SX_ERR_STATE, "solve on a gradient before a linearization");
+    require_trial_grad(g, "solve on a gradient before a linearization");
     SX_REQUIRE(g->form->test == L->space, SX_ERR_ARG, "the preconditioner belongs to another space");
     Workspace w;
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());

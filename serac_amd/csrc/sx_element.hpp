@@ -238,6 +238,50 @@ __device__ __forceinline__ void qpoint_tangent_column(const double* prm, const d
           (f[c * 3 + 0].d * Ji[j * 3 + 0] + f[c * 3 + 1].d * Ji[j * 3 + 1] + f[c * 3 + 2].d * Ji[j * 3 + 2]) * dJ;
 }
 
+// Directional derivative of the quadrature-point stage with respect to the
+// shape: the Jacobian J = dX/dxi + dp/dxi moves by dJ = d(dp)/dxi, and
+// inv(J), det(J), grad_x u = grad_xi u inv(J), the qfunction and the
+// transformation of its source and flux are carried as one-direction duals
+// (fn/shape_aware_functional.hpp:37-125 differentiated in p).  src and fp get
+// the derivative parts.  The point's state enters as a copy (nothing is
+// committed).
+template <class QF, int NC>
+__device__ __forceinline__ void qpoint_shape_dir(const double* prm, const double* J, const double* dJdir,
+                                                 const double* val, const double* gp, const double* st0,
+                                                 double* src, double* fp)
+{
+  d1 Jd[9], Jid[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) Jd[k] = d1(J[k], dJdir[k]);
+  inv3(Jd, Jid);
+  const d1 dJd = det3(Jd);
+  d1       v[NC], g[3 * NC], s[NC], f[3 * NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) v[c] = d1(val[c]);
+#pragma unroll
+  for (int c = 0; c < NC; c++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      g[c * 3 + k] = gp[c * 3 + 0] * Jid[0 * 3 + k] + gp[c * 3 + 1] * Jid[1 * 3 + k] + gp[c * 3 + 2] * Jid[2 * 3 + k];
+#pragma unroll
+  for (int c = 0; c < NC; c++) s[c] = d1(0.0);
+  if constexpr (QF::STATE > 0) {
+    double st[QF::STATE];
+#pragma unroll
+    for (int k = 0; k < QF::STATE; k++) st[k] = st0[k];
+    QF::eval(prm, v, g, s, f, st);
+  } else {
+    QF::eval(prm, v, g, s, f);
+  }
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    src[c] = (s[c] * dJd).d;
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      fp[c * 3 + j] = ((f[c * 3 + 0] * Jid[j * 3 + 0] + f[c * 3 + 1] * Jid[j * 3 + 1] + f[c * 3 + 2] * Jid[j * 3 + 2]) * dJd).d;
+  }
+}
+
 // Loads the DS tangent entries of quadrature point q as T.
 template <typename T, typename S, int DS, int NQ>
 __device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
@@ -364,70 +408,75 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   // the parent gradient of p at this lane's quadrature point, added to the
   // geometric Jacobian below, so dx/dxi = dX/dxi + dp/dxi = (I + dp/dX) dX/dxi.
   // P.shape is the same for every lane, so the barriers inside are uniform.
-  double gps[9];
+  // parent gradient (at this lane's point) of a 3-component field on the
+  // element's nodes, by the interpolation's sum factorization; uses LDS
+  // regions 0..5 before the gather
+  auto parent_grad3 = [&](const double* field, double* out9) {
+    if (active) {
+      const int64_t nd = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-  for (int k = 0; k < 9; k++) gps[k] = 0.0;
-  if constexpr (NC == 3) {
-    if (P.shape != nullptr) {
-      if (active) {
-        const int64_t nd = P.map[size_t(slot) * NQ + li];
-#pragma unroll
-        for (int c = 0; c < 3; c++) S_[c * NQ + li] = P.shape[lidx(nd, c, 3, P.nnodes, P.ordering)];
-      }
-      lds_barrier();
-      if (active) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const double* pp = S_ + c * NQ + (i2 * N + i1) * N;
-          double        s0 = 0.0, s1 = 0.0;
-#pragma unroll
-          for (int x = 0; x < N; x++) {
-            s0 += tB[i0 * N + x] * pp[x];
-            s1 += tG[i0 * N + x] * pp[x];
-          }
-          S_[1 * R + c * NQ + li] = s0;
-          S_[2 * R + c * NQ + li] = s1;
-        }
-      }
-      lds_barrier();
-      if (active) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const double* a10 = S_ + 1 * R + c * NQ;
-          const double* a11 = S_ + 2 * R + c * NQ;
-          double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
-#pragma unroll
-          for (int y = 0; y < N; y++) {
-            const int    idx = (i2 * N + y) * N + i0;
-            const double b = tB[i1 * N + y], g = tG[i1 * N + y];
-            s0 += b * a10[idx];
-            s1 += b * a11[idx];
-            s2 += g * a10[idx];
-          }
-          S_[3 * R + c * NQ + li] = s0;
-          S_[4 * R + c * NQ + li] = s1;
-          S_[5 * R + c * NQ + li] = s2;
-        }
-      }
-      lds_barrier();
-      if (active) {
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-          const double* a20 = S_ + 3 * R + c * NQ;
-          const double* a21 = S_ + 4 * R + c * NQ;
-          const double* a22 = S_ + 5 * R + c * NQ;
-#pragma unroll
-          for (int z = 0; z < N; z++) {
-            const int    idx = (z * N + i1) * N + i0;
-            const double b = tB[i2 * N + z], g = tG[i2 * N + z];
-            gps[c * 3 + 0] += b * a21[idx];
-            gps[c * 3 + 1] += b * a22[idx];
-            gps[c * 3 + 2] += g * a20[idx];
-          }
-        }
-      }
-      lds_barrier();  // the gather below overwrites the regions read above
+      for (int c = 0; c < 3; c++) S_[c * NQ + li] = field[lidx(nd, c, 3, P.nnodes, P.ordering)];
     }
+    lds_barrier();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double* pp = S_ + c * NQ + (i2 * N + i1) * N;
+        double        s0 = 0.0, s1 = 0.0;
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+          s0 += tB[i0 * N + x] * pp[x];
+          s1 += tG[i0 * N + x] * pp[x];
+        }
+        S_[1 * R + c * NQ + li] = s0;
+        S_[2 * R + c * NQ + li] = s1;
+      }
+    }
+    lds_barrier();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double* a10 = S_ + 1 * R + c * NQ;
+        const double* a11 = S_ + 2 * R + c * NQ;
+        double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int y = 0; y < N; y++) {
+          const int    idx = (i2 * N + y) * N + i0;
+          const double b = tB[i1 * N + y], g = tG[i1 * N + y];
+          s0 += b * a10[idx];
+          s1 += b * a11[idx];
+          s2 += g * a10[idx];
+        }
+        S_[3 * R + c * NQ + li] = s0;
+        S_[4 * R + c * NQ + li] = s1;
+        S_[5 * R + c * NQ + li] = s2;
+      }
+    }
+    lds_barrier();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double* a20 = S_ + 3 * R + c * NQ;
+        const double* a21 = S_ + 4 * R + c * NQ;
+        const double* a22 = S_ + 5 * R + c * NQ;
+#pragma unroll
+        for (int z = 0; z < N; z++) {
+          const int    idx = (z * N + i1) * N + i0;
+          const double b = tB[i2 * N + z], g = tG[i2 * N + z];
+          out9[c * 3 + 0] += b * a21[idx];
+          out9[c * 3 + 1] += b * a22[idx];
+          out9[c * 3 + 2] += g * a20[idx];
+        }
+      }
+    }
+    lds_barrier();  // the next pass / the gather overwrites the regions read above
+  };
+  double gps[9], dgps[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) gps[k] = dgps[k] = 0.0;
+  if constexpr (NC == 3) {
+    if (P.shape != nullptr) parent_grad3(P.shape, gps);
+    if constexpr (MODE == MODE_SHAPE_JVP) parent_grad3(P.dshape, dgps);
   }
 
   // ---- gather
@@ -510,6 +559,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
     inv3(J, Ji);
     const double dJ = det3(J);
     double       src[NC], fp[3 * NC];
+    if constexpr (MODE == MODE_SHAPE_JVP) {
+      qpoint_shape_dir<QF, NC>(P.params, J, dgps, val, gp, st, src, fp);
+    } else {
     if constexpr (MODE == MODE_LINEARIZE) {
       // columns s = 0..M-1 give entries idx(r, s), r < rows(s), in increasing
       // storage order; every column starts from the state before this call
@@ -532,6 +584,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
 #pragma unroll
         for (int k = 0; k < NST; k++) stp[k * NQ] = st[k];
       }
+    }
     }
     // flux -> regions 0..2 (u and stage-x partials are dead), source -> region 6
 #pragma unroll
@@ -1052,6 +1105,13 @@ inline int dispatch_eval(const ElemLaunch& P, hipError_t* err)
   if (P.mode == MODE_RESIDUAL) {
     *err = launch_grid<N>(P, k_element<N, NC, MODE_RESIDUAL, QF, LAYOUT, double>);
     return 1;
+  }
+  if (P.mode == MODE_SHAPE_JVP) {
+    if constexpr (NC == 3) {
+      *err = launch_grid<N>(P, k_element<N, NC, MODE_SHAPE_JVP, QF, LAYOUT, double>);
+      return 1;
+    }
+    return 0;
   }
   switch (P.tprec) {
     case PREC_F64: *err = launch_grid<N>(P, k_element<N, NC, MODE_LINEARIZE, QF, LAYOUT, double>); return 1;

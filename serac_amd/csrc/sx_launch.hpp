@@ -7,7 +7,9 @@
 
 namespace sxk {
 
-enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3, MODE_ASSEMBLE = 4 };
+// MODE_SHAPE_JVP: the derivative of the residual with respect to the shape
+// displacement in direction `dshape`, recomputed by forward-mode duals
+enum Mode { MODE_RESIDUAL = 0, MODE_LINEARIZE = 1, MODE_JVP = 2, MODE_DIAG = 3, MODE_ASSEMBLE = 4, MODE_SHAPE_JVP = 5 };
 
 // Stored-tangent layouts:
 //   GRAD_SYM:  d(parent flux)/d(parent grad), M = 3 nc, symmetric, upper
@@ -52,6 +54,7 @@ struct ElemLaunch {
   double*        csr_val;      // assembly: values, accumulated into
   const double*  shape;        // residual/linearize: shape displacement p (3-component L-vector of the
                                // test space) or null; the geometry becomes X + p
+  const double*  dshape;       // MODE_SHAPE_JVP: the direction of the shape displacement
   double        params[16];
   double        t;
   hipStream_t   stream;

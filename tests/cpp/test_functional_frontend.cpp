@@ -11,6 +11,8 @@
 //   - AddSurfaceIntegral: the traction patch test (P(A) . N on the max faces)
 //   - J2 plasticity with QuadratureData: residual, J.v and the state
 //     updateQdata commits, against the oracle (1e-12)
+//   - a space built from the caller's element -> node map (mfem-style
+//     numbering): residual and J.v equal the permuted originals bit for bit
 //   - error paths: unsupported configurations raise serac_amd::Error
 #include <algorithm>
 #include <cmath>
@@ -135,6 +137,60 @@ int main()
     Vector p0(std::vector<double>(size_t(ndof), 0.0));
     std::vector<double> Rs = sres(0.0, p0, u).ToHost();
     CHECK(relerr(Rs, R_ref) < 1e-12, "ShapeAwareFunctional with p = 0 vs oracle residual: %.2e", relerr(Rs, R_ref));
+  }
+  {
+    // The same problem on a space in the caller's numbering, mfem-style:
+    // mesh vertices first, in vertex order, then every other node (here in
+    // reverse).  Residual and J.v are the originals permuted, bit for bit.
+    int64_t nn = 0, nd = 0, ne = 0;
+    int     npe = 0;
+    detail::check(sx_space_info(fes.handle(), &nn, &nd, &ne, &npe));
+    std::vector<int32_t> own(size_t(ne) * size_t(npe));
+    detail::check(sx_space_element_nodes(fes.handle(), own.data()));
+    const int64_t        g = int64_t(p) * n + 1;
+    std::vector<int32_t> ext(static_cast<size_t>(nn));
+    int64_t              next = int64_t(n + 1) * (n + 1) * (n + 1);
+    for (int64_t a = nn - 1; a >= 0; a--) {
+      const int64_t I = a % g, J = (a / g) % g, Kk = a / (g * g);
+      if (I % p == 0 && J % p == 0 && Kk % p == 0)
+        ext[size_t(a)] = int32_t(I / p + (n + 1) * (J / p + (n + 1) * (Kk / p)));
+      else
+        ext[size_t(a)] = int32_t(next++);
+    }
+    std::vector<int32_t> emap(own.size());
+    for (size_t k = 0; k < own.size(); k++) emap[k] = ext[size_t(own[k])];
+    FiniteElementSpace       fes2(mesh, p, 3, Ordering::byVDIM, nn, emap);
+    Functional<space(space)> res2(&fes2, {&fes2});
+    res2.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                           solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                               mat, GeometricNonlinearities::On),
+                           mesh);
+    std::vector<double> U2(U.size()), dU2(dU.size());
+    for (int64_t a = 0; a < nn; a++)
+      for (int c = 0; c < 3; c++) {
+        U2[size_t(ext[size_t(a)] * 3 + c)]  = U[size_t(a * 3 + c)];
+        dU2[size_t(ext[size_t(a)] * 3 + c)] = dU[size_t(a * 3 + c)];
+      }
+    Vector u2(U2), du2(dU2);
+    auto [rr2, dR2] = res2(0.0, differentiate_wrt(u2));
+    std::vector<double> R2 = rr2.ToHost(), J2 = dR2(du2).ToHost(), R1 = r2.ToHost();
+    bool same = true;
+    for (int64_t a = 0; a < nn; a++)
+      for (int c = 0; c < 3; c++)
+        same = same && R2[size_t(ext[size_t(a)] * 3 + c)] == R1[size_t(a * 3 + c)] &&
+               J2[size_t(ext[size_t(a)] * 3 + c)] == Jv[size_t(a * 3 + c)];
+    CHECK(same, "space from a caller's (mfem-style) element-node map: residual and J.v permuted bit for bit");
+    bool threw = false;
+    try {
+      std::vector<int32_t> bad = emap;
+      // element 0's last two nodes exchanged; the last is shared with the
+      // neighbouring elements, which keep its id: not a relabelling
+      std::swap(bad[size_t(npe) - 1], bad[size_t(npe) - 2]);
+      FiniteElementSpace fes3(mesh, p, 3, Ordering::byVDIM, nn, bad);
+    } catch (const Error& e) {
+      threw = std::strstr(e.what(), "consistently") != nullptr;
+    }
+    CHECK(threw, "an inconsistent element-node map is rejected");
   }
   std::vector<double> diag = dR.AssembleDiagonal().ToHost();
   double              dmin = 1e300;
