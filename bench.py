@@ -363,6 +363,8 @@ def main():
     ap.add_argument("--split-scatter", default="colored", choices=["colored", "atomic"],
                     help="scatter mode of the FP32 / BF16 J.v measurements")
     ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--scatter-order", type=int, default=1,
+                    help="1: element values added in address order through LDS; 0: one component per instruction")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a known-size read (dot of two L-vectors) and write (fill) for counter scaling")
     args = ap.parse_args()
@@ -394,6 +396,7 @@ def main():
         f.set_option(sx.capi.SX_OPT_SCATTER,
                      sx.capi.SX_SCATTER_ATOMIC if scatter == "atomic" else sx.capi.SX_SCATTER_COLORED)
         f.set_option(sx.capi.SX_OPT_XCD_REMAP, args.xcd_remap)
+        f.set_option(sx.capi.SX_OPT_SCATTER_ORDER, args.scatter_order)
         if qf == "j2":
             qd = sx.QuadratureData(sp.ne, nq1 ** 3, np.zeros(10))
             qdatas.append(qd)
@@ -576,6 +579,7 @@ def main():
             "qpoints": nqp,
             "parallelism": "1 GPU",
             "scatter": args.scatter,
+            "scatter_order": args.scatter_order,
             "xcd_remap": args.xcd_remap,
         },
         "roofline": {

@@ -223,7 +223,10 @@ enum {
   SX_OPT_TANGENT_PRECISION = 3,
   SX_OPT_JVP_ARITHMETIC    = 4,
   SX_OPT_LINEAR_SOLVER     = 5, /* SX_LINSOLVER_CG (default) or _GMRES: the linear solves of the Newton loops */
-  SX_OPT_GMRES_RESTART     = 6  /* GMRES restart length (default 50, mfem's GMRESSolver default) */
+  SX_OPT_GMRES_RESTART     = 6, /* GMRES restart length (default 50, mfem's GMRESSolver default) */
+  SX_OPT_SCATTER_ORDER     = 7  /* 1 (default): the element kernels add each element's values in
+                                   address order through LDS (whole rows per wave instruction);
+                                   0: one component per instruction from the node's own lane */
 };
 /* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
 enum { SX_LINSOLVER_CG = 0, SX_LINSOLVER_GMRES = 1 };
@@ -515,12 +518,22 @@ enum {
   SX_QOI_MEASURE           = 1, /* int 1 dV (TrivialIntegrator of fn/tests/functional_qoi.cpp) */
   SX_QOI_POSITION          = 2, /* int X_d dV, params {d}: the first moment (ZeroIndexIntegrator) */
   SX_QOI_VALUE             = 3, /* int u_c dV, params {c} (GetZeroIntegrator) */
-  SX_QOI_NEOHOOKEAN_ENERGY = 4  /* int psi(I + grad u) dV, params {density, K, G}: the strain energy whose
+  SX_QOI_NEOHOOKEAN_ENERGY = 4, /* int psi(I + grad u) dV, params {density, K, G}: the strain energy whose
                                    first Piola stress is the flux of SX_QF_NEOHOOKEAN */
+  SX_QOI_INNER             = 5  /* int u_a . u_b dV of two trial arguments DependsOn<a, b> (a = b: int |u|^2) */
 };
 int sx_qoi_create(const sx_space* trial, sx_qoi** out);
+/* Functional<double(trials...)> with several trial arguments
+ * (fn/functional_qoi.inl:307-472); all on one mesh.  The single-trial entry
+ * points below act on trial 0 and need ntrials = 1. */
+int sx_qoi_create_multi(const sx_space* const* trials, int ntrials, sx_qoi** out);
 /* elems == NULL / nelem == 0: the whole mesh */
 int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* elems, int nelem);
+/* AddDomainIntegral(Dimension<3>, DependsOn<i...>, ...) of a QoI: the trial
+ * arguments the integrand reads (one; two for SX_QOI_INNER, whose spaces must
+ * have the same order, components, ordering and numbering) */
+int sx_qoi_add_domain_integral_dep(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* depends_on,
+                                   int ndep, const int* elems, int nelem);
 /* AddBoundaryIntegral (fn/functional_qoi.inl:204-230) of SX_QOI_MEASURE,
  * _POSITION or _VALUE over the boundary faces on the bounding-box planes of
  * face_mask (0: the whole boundary) or, when attr_mask != 0, over the
@@ -531,6 +544,11 @@ int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, in
 int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value);
 /* dq/dU at U into grad (device L-vector) */
 int sx_qoi_gradient(sx_qoi* q, double t, const double* u, double* grad);
+/* operator()(t, U...) and operator()(t, ..., differentiate_wrt(U_wrt), ...)'s
+ * gradient for a QoI of several trial arguments: inputs[ntrials] device
+ * L-vectors; grad is an L-vector of trial wrt */
+int sx_qoi_eval_multi(sx_qoi* q, double t, const double* const* inputs, double* value);
+int sx_qoi_gradient_multi(sx_qoi* q, double t, const double* const* inputs, int wrt, double* grad);
 int sx_qoi_destroy(sx_qoi* q);
 
 /* ---- vector kernels used by the solvers, exposed for callers ---- */

@@ -22,6 +22,11 @@ QF_MASS_DIFFUSION = 2
 QF_NEOHOOKEAN = 3
 QF_LINEAR_ELASTIC = 4
 QF_J2_LINEAR = 5
+QOI_MEASURE = 1
+QOI_POSITION = 2
+QOI_VALUE = 3
+QOI_NEOHOOKEAN_ENERGY = 4
+QOI_INNER = 5
 BQF_TRACTION = 101
 BQF_PRESSURE = 102
 BQF_FLUX = 103
@@ -111,6 +116,7 @@ def _load():
         "orc_form_add_integral": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int]),
         "orc_form_eval": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int]),
         "orc_form_jvp": (ctypes.c_int, [_vp, _D, _D]),
+        "orc_form_qoi": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int, _D, _D, _D]),
         "orc_form_eval_threads": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int, ctypes.c_int]),
         "orc_form_diagonal": (ctypes.c_int, [_vp, _D]),
         "orc_form_dsize": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -375,6 +381,19 @@ class Form:
         R = np.zeros(self.space.ndof)
         _check(lib().orc_form_eval_threads(self.h, _dp(U), _dp(R), 1, nthreads))
         return R
+
+    def qoi(self, kind, params=(), elems=None, U=None, U1=None):
+        """Functional<double(trials...)> value: QOI_* kind over elems (default all)."""
+        prm = np.zeros(16)
+        prm[:len(params)] = params
+        e = np.ascontiguousarray(elems if elems is not None else [], dtype=np.int32)
+        nd = self.space.ndof
+        u0 = np.ascontiguousarray(U if U is not None else np.zeros(nd), dtype=np.float64)
+        u1 = np.ascontiguousarray(U1 if U1 is not None else np.zeros(nd), dtype=np.float64)
+        out = np.zeros(1)
+        _check(lib().orc_form_qoi(self.h, kind, _dp(prm), _ip(e) if len(e) else None, len(e), _dp(u0), _dp(u1),
+                                  _dp(out)))
+        return float(out[0])
 
     def jvp(self, dU):
         dU = np.ascontiguousarray(dU, dtype=np.float64)

@@ -267,6 +267,16 @@ int orc_form_eval_threads(void* f, const double* U, double* R, int linearize, in
 {
   return guard([&] { static_cast<OForm*>(f)->form.eval_threads(U, R, linearize != 0, nthreads); });
 }
+// Functional<double(trials...)> value of a compiled QoI kind over elems
+// (all elements when nelem = 0); U1 only for the inner product (kind 5)
+int orc_form_qoi(void* f, int kind, const double* params, const int* elems, int nelem, const double* U0,
+                 const double* U1, double* value)
+{
+  return guard([&] {
+    std::vector<int> dom(elems, elems + (elems ? nelem : 0));
+    *value = static_cast<OForm*>(f)->form.qoi(kind, params, dom, U0, U1);
+  });
+}
 int orc_form_jvp(void* f, const double* dU, double* dR)
 {
   return guard([&] { static_cast<OForm*>(f)->form.jvp(dU, dR); });

@@ -644,6 +644,76 @@ struct Form {
     boundary_eval(U, R, store_D);
   }
 
+  // ---- quantities of interest, Functional<double(trials...)> ----
+  // The value (fn/functional_qoi.inl:307-472 with the domain evaluation of
+  // fn/domain_integral_kernels.hpp:155-220 at a scalar test space): for each
+  // element of the domain, in order, the sum over its points of
+  // q(x, u0, u1) detJ w, then the element sums added in element order.
+  //   kind 1 measure: q = 1;  2 position: q = x_d (params[0] = d);
+  //   3 value: q = u0_c (params[0] = c);  4 Neo-Hookean strain energy of u0:
+  //   q = G/2 (tr(F F^T) - 3) - G ln J + lambda/2 (ln J)^2, F = I + grad u0,
+  //   lambda = K - 2G/3 (params {density, K, G}); 5 inner: q = u0 . u1.
+  double qoi(int kind, const double* params, const std::vector<int>& elems, const double* U0,
+             const double* U1) const
+  {
+    const int        c = nc(), NQ = nq(), Qn = Q;
+    std::vector<int> dom = elems;
+    if (dom.empty())
+      for (int e = 0; e < mesh->ne; e++) dom.push_back(e);
+    std::vector<double> Ue(size_t(c) * space.npe), v0(size_t(c) * NQ), g0(size_t(3 * c) * NQ), v1(size_t(c) * NQ),
+        g1(size_t(3 * c) * NQ);
+    double total = 0.0;
+    for (int e : dom) {
+      if (U0) {
+        gather_elem(e, U0, Ue.data());
+        interpolate(basis, c, Ue.data(), v0.data(), g0.data());
+      }
+      if (U1) {
+        gather_elem(e, U1, Ue.data());
+        interpolate(basis, c, Ue.data(), v1.data(), g1.data());
+      }
+      double part = 0.0;
+      for (int q = 0; q < NQ; q++) {
+        const int qx = q % Qn, qy = (q / Qn) % Qn, qz = q / (Qn * Qn);
+        const double w = basis.wq[size_t(qx)] * basis.wq[size_t(qy)] * basis.wq[size_t(qz)];
+        double J[3][3];
+        jac(e, q, J);
+        const double dJ = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) -
+                          J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                          J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+        double qv = 0.0;
+        switch (kind) {
+          case 1: qv = 1.0; break;
+          case 2: qv = Xq[(size_t(e) * 3 + size_t(params[0])) * NQ + q]; break;
+          case 3: qv = v0[size_t(params[0]) * NQ + q]; break;
+          case 4: {
+            double H[9];
+            for (int i = 0; i < 3; i++)
+              for (int d = 0; d < 3; d++) H[i * 3 + d] = g0[(size_t(i) * 3 + d) * NQ + q];
+            parent_to_physical(3, J, H);
+            double F[9], FF = 0.0;
+            for (int k = 0; k < 9; k++) {
+              F[k] = H[k] + (k % 4 == 0 ? 1.0 : 0.0);
+              FF += F[k] * F[k];
+            }
+            const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                                F[2] * (F[3] * F[7] - F[4] * F[6]);
+            const double K = params[1], G = params[2], lam = K - 2.0 * G / 3.0, lnJ = std::log(detF);
+            qv = 0.5 * G * (FF - 3.0) - G * lnJ + 0.5 * lam * lnJ * lnJ;
+            break;
+          }
+          case 5:
+            for (int i = 0; i < c; i++) qv += v0[size_t(i) * NQ + q] * v1[size_t(i) * NQ + q];
+            break;
+          default: throw std::runtime_error("unknown QoI kind");
+        }
+        part += qv * dJ * w;
+      }
+      total += part;
+    }
+    return total;
+  }
+
   // Element gradient K_e[(c,j) test][(c',J) trial], row-major, by applying the
   // stored derivative to each trial shape function (element_gradient_kernel).
   void element_matrix(int integral, size_t slot, double* K) const

@@ -560,22 +560,32 @@ class Learned:
 
 
 class QoI:
-    """Functional<double(trial)>: a scalar quantity of interest on one trial space."""
+    """Functional<double(trials...)>: a scalar quantity of interest on one or
+    several trial spaces (a list of spaces for several)."""
 
     def __init__(self, space):
         h = _vp()
-        capi.call("sx_qoi_create", space.h, ctypes.byref(h))
+        spaces = list(space) if isinstance(space, (list, tuple)) else [space]
+        if len(spaces) == 1:
+            capi.call("sx_qoi_create", spaces[0].h, ctypes.byref(h))
+        else:
+            arr = (_vp * len(spaces))(*[sp.h for sp in spaces])
+            capi.call("sx_qoi_create_multi", arr, len(spaces), ctypes.byref(h))
         self.h = h
-        self.space = space
+        self.spaces = spaces
+        self.space = spaces[0]
 
-    def add_domain_integral(self, qoi, params=(), elems=None):
+    def add_domain_integral(self, qoi, params=(), elems=None, depends_on=None):
         p = np.zeros(16)
         p[: len(params)] = params
-        if elems is None:
-            capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), None, 0)
+        el = (ctypes.c_int * len(elems))(*elems) if elems is not None else None
+        ne = len(elems) if elems is not None else 0
+        if depends_on is None:
+            capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), el, ne)
         else:
-            el = (ctypes.c_int * len(elems))(*elems)
-            capi.call("sx_qoi_add_domain_integral", self.h, qoi, _dptr(p), len(params), el, len(elems))
+            dep = (ctypes.c_int * len(depends_on))(*depends_on)
+            capi.call("sx_qoi_add_domain_integral_dep", self.h, qoi, _dptr(p), len(params), dep, len(depends_on),
+                      el, ne)
 
     def add_boundary_integral(self, qoi, params=(), face_mask=0, attrs=None):
         """Boundary QoI over the planes of face_mask (0: the whole boundary) or the attributes `attrs`."""
@@ -584,15 +594,23 @@ class QoI:
         capi.call("sx_qoi_add_boundary_integral", self.h, qoi, _dptr(p), len(params), face_mask,
                   attr_mask(attrs) if attrs is not None else 0)
 
-    def __call__(self, t, U):
+    def __call__(self, t, *U):
         v = ctypes.c_double()
-        capi.call("sx_qoi_eval", self.h, t, U.ptr, ctypes.byref(v))
+        if len(self.spaces) == 1:
+            capi.call("sx_qoi_eval", self.h, t, U[0].ptr, ctypes.byref(v))
+        else:
+            arr = (_vp * len(U))(*[u.ptr for u in U])
+            capi.call("sx_qoi_eval_multi", self.h, t, arr, ctypes.byref(v))
         return v.value
 
-    def gradient(self, t, U):
-        """dq/dU as a DeviceVector."""
-        g = DeviceVector(self.space.ndof)
-        capi.call("sx_qoi_gradient", self.h, t, U.ptr, g.ptr)
+    def gradient(self, t, *U, wrt=0):
+        """dq/dU_wrt as a DeviceVector."""
+        g = DeviceVector(self.spaces[wrt].ndof)
+        if len(self.spaces) == 1:
+            capi.call("sx_qoi_gradient", self.h, t, U[0].ptr, g.ptr)
+        else:
+            arr = (_vp * len(U))(*[u.ptr for u in U])
+            capi.call("sx_qoi_gradient_multi", self.h, t, arr, wrt, g.ptr)
         return g
 
     def __del__(self):

@@ -26,6 +26,7 @@ SX_OPT_TANGENT_PRECISION = 3
 SX_OPT_JVP_ARITHMETIC = 4
 SX_OPT_LINEAR_SOLVER = 5
 SX_OPT_GMRES_RESTART = 6
+SX_OPT_SCATTER_ORDER = 7
 SX_LINSOLVER_CG = 0
 SX_LINSOLVER_GMRES = 1
 SX_PREC_F64 = 0
@@ -50,6 +51,7 @@ SX_QOI_MEASURE = 1
 SX_QOI_POSITION = 2
 SX_QOI_VALUE = 3
 SX_QOI_NEOHOOKEAN_ENERGY = 4
+SX_QOI_INNER = 5
 SX_UNET_IDENTITY = 1
 
 _vp = ctypes.c_void_p
@@ -162,6 +164,11 @@ SIGNATURES = {
     "sx_unet_set_activation": (_int, [_vp, _int, ctypes.c_uint64]),
     "sx_unet_set_mfma": (_int, [_vp, _int]),
     "sx_qoi_create": (_int, [_vp, ctypes.POINTER(_vp)]),
+    "sx_qoi_create_multi": (_int, [ctypes.POINTER(_vp), _int, ctypes.POINTER(_vp)]),
+    "sx_qoi_add_domain_integral_dep": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, ctypes.POINTER(_int),
+                                              _int]),
+    "sx_qoi_eval_multi": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _D]),
+    "sx_qoi_gradient_multi": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _int, _vp]),
     "sx_qoi_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int]),
     "sx_qoi_add_boundary_integral": (_int, [_vp, _int, _D, _int, _int, ctypes.c_uint64]),
     "sx_qoi_eval": (_int, [_vp, ctypes.c_double, _vp, _D]),

@@ -301,6 +301,7 @@ struct sx_form {
   hipStream_t                            stream = nullptr;
   int                                    scatter   = SX_SCATTER_COLORED;
   int                                    xcd_remap = 1;
+  int                                    scatter_order = 1;  // element values added in address order
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
@@ -362,6 +363,7 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
   P.ordering = s->ordering;
   P.scatter   = f->scatter;
   P.xcd_remap = f->xcd_remap;
+  P.ordered   = f->scatter_order;
   P.map      = I.d_map.as<const int>();
   P.coords   = I.d_coords.as<const double>();
   P.nnodes   = s->nnodes;
@@ -1515,6 +1517,9 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
       case SX_OPT_GMRES_RESTART:
         SX_REQUIRE(value >= 1 && value <= 1000, SX_ERR_ARG, "GMRES restart length must be in 1..1000");
         f->gmres_restart = int(value);
+        return;
+      case SX_OPT_SCATTER_ORDER:
+        f->scatter_order = value ? 1 : 0;
         return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
@@ -3619,98 +3624,204 @@ struct sx_qoi {
   struct Term {
     int          id   = 0;
     bool         face = false;  // a boundary integral: slots are faces
+    int          dep[2] = {0, 0};  // the trial arguments it reads (DependsOn<i...>)
     double       params[SX_MAX_PARAMS] = {};
     int64_t      nslots = 0;
     DeviceBuffer d_map, d_coords, d_part;
   };
-  const sx_space*   space = nullptr;
+  std::vector<const sx_space*>       trials;
   std::vector<std::unique_ptr<Term>> terms;
-  sx_form*          grad  = nullptr;  // the gradients as residuals of derived integrands
+  // grads[k]: the gradient in trial k as the residual of derived integrands,
+  // a form with test space trials[k] and every trial as an argument
+  std::vector<sx_form*> grads;
   ~sx_qoi()
   {
-    if (grad) sx_form_destroy(grad);
+    for (sx_form* g : grads)
+      if (g) sx_form_destroy(g);
   }
 };
 
+namespace {
+
+void check_trial_pair(const sx_space* a, const sx_space* b)
+{
+  SX_REQUIRE(a->mesh == b->mesh && a->order == b->order && a->nc == b->nc && a->ordering == b->ordering &&
+                 (a == b || (a->native && b->native) || a->emap == b->emap),
+             SX_ERR_UNSUPPORTED, "the inner-product QoI needs two trial spaces of the same kind on one mesh");
+}
+
+// adds `qoi_id` over `elems` of trial dep[0] (and dep[1])
+void qoi_add_domain(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* dep, int ndep,
+                    const int* elems, int nelem)
+{
+  SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
+  SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+             "QoI parameters: at most 16 doubles");
+  SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE ||
+                 qoi_id == SX_QOI_NEOHOOKEAN_ENERGY || qoi_id == SX_QOI_INNER,
+             SX_ERR_UNSUPPORTED, "unknown QoI integrand id " + std::to_string(qoi_id));
+  const int want = qoi_id == SX_QOI_INNER ? 2 : 1;
+  SX_REQUIRE(dep && ndep == want, SX_ERR_ARG,
+             "this QoI integrand depends on " + std::to_string(want) + " trial argument(s) (DependsOn)");
+  const int ntr = int(q->trials.size());
+  for (int k = 0; k < ndep; k++) SX_REQUIRE(dep[k] >= 0 && dep[k] < ntr, SX_ERR_ARG, "DependsOn index out of range");
+  const sx_space* s = q->trials[size_t(dep[0])];
+  if (qoi_id == SX_QOI_INNER) check_trial_pair(s, q->trials[size_t(dep[1])]);
+  SX_REQUIRE(qoi_id != SX_QOI_NEOHOOKEAN_ENERGY || s->nc == 3, SX_ERR_ARG,
+             "the Neo-Hookean strain energy needs a 3-component space");
+  if (qoi_id == SX_QOI_POSITION)
+    SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
+               "position QoI: params {d}, d in 0..2");
+  if (qoi_id == SX_QOI_VALUE)
+    SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < s->nc && params[0] == std::floor(params[0]),
+               SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
+  const auto&      m = s->mesh->m;
+  std::vector<int> domain;
+  if (elems && nelem > 0) {
+    domain.assign(elems, elems + nelem);
+    for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "element id out of range");
+  } else {
+    domain.resize(size_t(m.ne));
+    for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
+  }
+  auto  Tp = std::make_unique<sx_qoi::Term>();
+  auto& T  = *Tp;
+  T.id     = qoi_id;
+  for (int k = 0; k < ndep; k++) T.dep[k] = dep[k];
+  for (int k = 0; k < nparams; k++) T.params[k] = params[k];
+  T.nslots = int64_t(domain.size());
+  std::vector<int>    map(size_t(T.nslots) * size_t(s->npe));
+  std::vector<double> xyz(size_t(T.nslots) * 24);
+  for (int64_t k = 0; k < T.nslots; k++) {
+    const int e = domain[size_t(k)];
+    std::copy(s->emap.begin() + long(size_t(e) * size_t(s->npe)), s->emap.begin() + long(size_t(e + 1) * size_t(s->npe)),
+              map.begin() + long(size_t(k) * size_t(s->npe)));
+    for (int c = 0; c < 8; c++) {
+      const int v = m.corner(e, c);
+      for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + size_t(c * 3 + d)] = m.X[size_t(3 * v + d)];
+    }
+  }
+  T.d_map.upload(map);
+  T.d_coords.upload(xyz);
+  T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
+  // the gradients: int u_c dV -> source e_c; the strain energy -> the
+  // Neo-Hookean residual (its flux P is d psi / dF); int u_a . u_b dV ->
+  // int phi . u_b dV in trial a (a mass residual at u_b) and int phi . u_a dV
+  // in trial b; measure and position do not depend on U
+  const int* el = elems && nelem > 0 ? elems : nullptr;
+  auto       add = [&](int k, int qf, const double* prm, int np, int on) {
+    const int d[1] = {on};
+    const int rc   = sx_form_add_domain_integral(q->grads[size_t(k)], qf, prm, np, d, 1, el, nelem);
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  };
+  if (qoi_id == SX_QOI_VALUE) {
+    g_internal_qf = true;
+    try {
+      add(dep[0], sxk::QF_ID_QOI_VALUE, T.params, 1, dep[0]);
+    } catch (...) {
+      g_internal_qf = false;
+      throw;
+    }
+    g_internal_qf = false;
+  } else if (qoi_id == SX_QOI_NEOHOOKEAN_ENERGY) {
+    add(dep[0], SX_QF_NEOHOOKEAN, T.params, 3, dep[0]);
+  } else if (qoi_id == SX_QOI_INNER) {
+    const double mass[2] = {1.0, 0.0};  // {a u, b grad u} with a = 1, b = 0
+    add(dep[0], SX_QF_MASS_DIFFUSION, mass, 2, dep[1]);
+    add(dep[1], SX_QF_MASS_DIFFUSION, mass, 2, dep[0]);
+  }
+  q->terms.push_back(std::move(Tp));
+}
+
+void qoi_value(sx_qoi* q, const double* const* inputs, double* value)
+{
+  const sx_space* s0     = q->trials[0];
+  hipStream_t     stream = q->grads[0]->stream;
+  double          total  = 0.0;
+  for (auto& Tp : q->terms) {
+    const sx_qoi::Term& T = *Tp;
+    const sx_space*     s = q->trials[size_t(T.dep[0])];
+    sxk::QoiLaunch      P{};
+    P.order    = s->order;
+    P.nc       = s->nc;
+    P.qoi      = T.id;
+    P.ordering = s->ordering;
+    P.map      = T.d_map.as<const int>();
+    P.coords   = T.d_coords.as<const double>();
+    P.nslots   = T.nslots;
+    P.nnodes   = s->nnodes;
+    P.u        = inputs[T.dep[0]];
+    P.u2       = T.id == SX_QOI_INNER ? inputs[T.dep[1]] : nullptr;
+    P.tables   = element_tables(s->order + 1);
+    for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = T.params[k];
+    P.partial = Tp->d_part.as<double>();
+    P.stream  = stream;
+    SX_HIP(T.face ? sxk::launch_qoi_face(P) : sxk::launch_qoi(P));
+    std::vector<double> part(size_t(T.nslots));
+    SX_HIP(hipMemcpyAsync(part.data(), T.d_part.ptr, part.size() * 8, hipMemcpyDeviceToHost, stream));
+    SX_HIP(hipStreamSynchronize(stream));
+    for (double v : part) total += v;  // element order: reproducible
+  }
+  (void)s0;
+  *value = total;
+}
+
+void qoi_gradient(sx_qoi* q, double t, const double* const* inputs, int wrt, double* grad)
+{
+  sx_form* g = q->grads[size_t(wrt)];
+  if (g->integrals.empty() && g->bintegrals.empty()) {
+    SX_HIP(sxk::vec_fill(grad, 0.0, q->trials[size_t(wrt)]->ndof(), g->stream));
+    SX_HIP(hipStreamSynchronize(g->stream));
+    return;
+  }
+  sx_grad*  gr = nullptr;
+  const int rc = sx_form_eval(g, t, inputs, grad, -1, &gr);
+  if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  SX_HIP(hipStreamSynchronize(g->stream));
+}
+
+}  // namespace
+
 extern "C" {
+
+int sx_qoi_create_multi(const sx_space* const* trials, int ntrials, sx_qoi** out)
+{
+  return api([&] {
+    SX_REQUIRE(trials && ntrials >= 1 && out, SX_ERR_ARG, "a QoI needs at least one trial space");
+    for (int i = 0; i < ntrials; i++) SX_REQUIRE(trials[i], SX_ERR_ARG, "null trial space");
+    for (int i = 1; i < ntrials; i++)
+      SX_REQUIRE(trials[i]->mesh == trials[0]->mesh, SX_ERR_UNSUPPORTED, "the trial spaces of a QoI share a mesh");
+    auto q = std::make_unique<sx_qoi>();
+    q->trials.assign(trials, trials + ntrials);
+    q->grads.assign(size_t(ntrials), nullptr);
+    for (int k = 0; k < ntrials; k++) {
+      const int rc = sx_form_create(trials[k], trials, ntrials, &q->grads[size_t(k)]);
+      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+    }
+    *out = q.release();
+  });
+}
 
 int sx_qoi_create(const sx_space* trial, sx_qoi** out)
 {
-  return api([&] {
-    SX_REQUIRE(trial && out, SX_ERR_ARG, "null argument");
-    auto q   = std::make_unique<sx_qoi>();
-    q->space = trial;
-    const sx_space* sp[1] = {trial};
-    const int       rc    = sx_form_create(trial, sp, 1, &q->grad);
-    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-    *out = q.release();
-  });
+  const sx_space* sp[1] = {trial};
+  if (!trial) return api([&] { SX_REQUIRE(false, SX_ERR_ARG, "null argument"); });
+  return sx_qoi_create_multi(sp, 1, out);
 }
 
 int sx_qoi_add_domain_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* elems,
                                int nelem)
 {
   return api([&] {
-    SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
-    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
-               "QoI parameters: at most 16 doubles");
-    SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE ||
-                   qoi_id == SX_QOI_NEOHOOKEAN_ENERGY,
-               SX_ERR_UNSUPPORTED, "unknown QoI integrand id " + std::to_string(qoi_id));
-    const sx_space* s = q->space;
-    SX_REQUIRE(qoi_id != SX_QOI_NEOHOOKEAN_ENERGY || s->nc == 3, SX_ERR_ARG,
-               "the Neo-Hookean strain energy needs a 3-component space");
-    if (qoi_id == SX_QOI_POSITION)
-      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
-                 "position QoI: params {d}, d in 0..2");
-    if (qoi_id == SX_QOI_VALUE)
-      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < s->nc && params[0] == std::floor(params[0]),
-                 SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
-    const auto&      m = s->mesh->m;
-    std::vector<int> domain;
-    if (elems && nelem > 0) {
-      domain.assign(elems, elems + nelem);
-      for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "element id out of range");
-    } else {
-      domain.resize(size_t(m.ne));
-      for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
-    }
-    auto  Tp = std::make_unique<sx_qoi::Term>();
-    auto& T  = *Tp;
-    T.id     = qoi_id;
-    for (int k = 0; k < nparams; k++) T.params[k] = params[k];
-    T.nslots = int64_t(domain.size());
-    std::vector<int>    map(size_t(T.nslots) * size_t(s->npe));
-    std::vector<double> xyz(size_t(T.nslots) * 24);
-    for (int64_t k = 0; k < T.nslots; k++) {
-      const int e = domain[size_t(k)];
-      std::copy(s->emap.begin() + long(size_t(e) * size_t(s->npe)), s->emap.begin() + long(size_t(e + 1) * size_t(s->npe)),
-                map.begin() + long(size_t(k) * size_t(s->npe)));
-      for (int c = 0; c < 8; c++) {
-        const int v = m.corner(e, c);
-        for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + size_t(c * 3 + d)] = m.X[size_t(3 * v + d)];
-      }
-    }
-    T.d_map.upload(map);
-    T.d_coords.upload(xyz);
-    T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
-    // the gradient: int u_c dV -> source e_c; the strain energy -> the
-    // Neo-Hookean residual (its flux P is d psi / dF); measure and position
-    // do not depend on U
-    const int dep[1] = {0};
-    if (qoi_id == SX_QOI_VALUE) {
-      g_internal_qf = true;
-      const int rc  = sx_form_add_domain_integral(q->grad, sxk::QF_ID_QOI_VALUE, T.params, 1, dep, 1,
-                                                  elems && nelem > 0 ? elems : nullptr, nelem);
-      g_internal_qf = false;
-      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-    } else if (qoi_id == SX_QOI_NEOHOOKEAN_ENERGY) {
-      const int rc = sx_form_add_domain_integral(q->grad, SX_QF_NEOHOOKEAN, T.params, 3, dep, 1,
-                                                 elems && nelem > 0 ? elems : nullptr, nelem);
-      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-    }
-    q->terms.push_back(std::move(Tp));
+    const int dep[2] = {0, 0};
+    qoi_add_domain(q, qoi_id, params, nparams, dep, qoi_id == SX_QOI_INNER ? 2 : 1, elems, nelem);
   });
+}
+
+int sx_qoi_add_domain_integral_dep(sx_qoi* q, int qoi_id, const double* params, int nparams, const int* depends_on,
+                                   int ndep, const int* elems, int nelem)
+{
+  return api([&] { qoi_add_domain(q, qoi_id, params, nparams, depends_on, ndep, elems, nelem); });
 }
 
 int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, int face_mask,
@@ -3723,7 +3834,7 @@ int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, in
     SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE, SX_ERR_UNSUPPORTED,
                "boundary QoIs: SX_QOI_MEASURE, SX_QOI_POSITION or SX_QOI_VALUE");
     SX_REQUIRE(face_mask >= 0 && face_mask < 64, SX_ERR_ARG, "face_mask selects among 6 planes");
-    const sx_space* vs = q->space;
+    const sx_space* vs = q->trials[0];
     if (qoi_id == SX_QOI_POSITION)
       SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
                  "position QoI: params {d}, d in 0..2");
@@ -3768,12 +3879,12 @@ int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, in
       if (vs->nc == 3) {
         double tp[12] = {0};
         tp[9 + c]     = -1.0;
-        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grad, SX_BQF_TRACTION, tp, 12, dep, 1, attr_mask)
-                       : sx_form_add_boundary_integral(q->grad, SX_BQF_TRACTION, tp, 12, dep, 1, face_mask);
+        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grads[0], SX_BQF_TRACTION, tp, 12, dep, 1, attr_mask)
+                       : sx_form_add_boundary_integral(q->grads[0], SX_BQF_TRACTION, tp, 12, dep, 1, face_mask);
       } else {
         const double fp[3] = {1.0, 0.0, 0.0};
-        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grad, SX_BQF_FLUX, fp, 3, dep, 1, attr_mask)
-                       : sx_form_add_boundary_integral(q->grad, SX_BQF_FLUX, fp, 3, dep, 1, face_mask);
+        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grads[0], SX_BQF_FLUX, fp, 3, dep, 1, attr_mask)
+                       : sx_form_add_boundary_integral(q->grads[0], SX_BQF_FLUX, fp, 3, dep, 1, face_mask);
       }
       if (rc != SX_OK) throw sx_failure(rc, g_last_error);
     }
@@ -3785,34 +3896,20 @@ int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value)
 {
   return api([&] {
     SX_REQUIRE(q && u && value, SX_ERR_ARG, "null argument");
-    const sx_space* s      = q->space;
-    hipStream_t     stream = q->grad->stream;
-    double          total  = 0.0;
-    for (auto& Tp : q->terms) {
-      const sx_qoi::Term& T = *Tp;
-      sxk::QoiLaunch      P{};
-      const bool          face = T.face;
-      P.order    = s->order;
-      P.nc       = s->nc;
-      P.qoi      = T.id;
-      P.ordering = s->ordering;
-      P.map      = T.d_map.as<const int>();
-      P.coords   = T.d_coords.as<const double>();
-      P.nslots   = T.nslots;
-      P.nnodes   = s->nnodes;
-      P.u        = u;
-      P.tables   = element_tables(s->order + 1);
-      for (int k = 0; k < SX_MAX_PARAMS; k++) P.params[k] = T.params[k];
-      P.partial = Tp->d_part.as<double>();
-      P.stream  = stream;
-      SX_HIP(face ? sxk::launch_qoi_face(P) : sxk::launch_qoi(P));
-      std::vector<double> part(size_t(T.nslots));
-      SX_HIP(hipMemcpyAsync(part.data(), T.d_part.ptr, part.size() * 8, hipMemcpyDeviceToHost, stream));
-      SX_HIP(hipStreamSynchronize(stream));
-      for (double v : part) total += v;  // element order: reproducible
-    }
+    SX_REQUIRE(q->trials.size() == 1, SX_ERR_ARG, "a QoI of several trial arguments: use sx_qoi_eval_multi");
+    const double* in[1] = {u};
     (void)t;
-    *value = total;
+    qoi_value(q, in, value);
+  });
+}
+
+int sx_qoi_eval_multi(sx_qoi* q, double t, const double* const* inputs, double* value)
+{
+  return api([&] {
+    SX_REQUIRE(q && inputs && value, SX_ERR_ARG, "null argument");
+    for (size_t i = 0; i < q->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
+    (void)t;
+    qoi_value(q, inputs, value);
   });
 }
 
@@ -3820,16 +3917,19 @@ int sx_qoi_gradient(sx_qoi* q, double t, const double* u, double* grad)
 {
   return api([&] {
     SX_REQUIRE(q && u && grad, SX_ERR_ARG, "null argument");
-    if (q->grad->integrals.empty() && q->grad->bintegrals.empty()) {
-      SX_HIP(sxk::vec_fill(grad, 0.0, q->space->ndof(), q->grad->stream));
-      SX_HIP(hipStreamSynchronize(q->grad->stream));
-      return;
-    }
+    SX_REQUIRE(q->trials.size() == 1, SX_ERR_ARG, "a QoI of several trial arguments: use sx_qoi_gradient_multi");
     const double* in[1] = {u};
-    sx_grad*      g     = nullptr;
-    const int     rc    = sx_form_eval(q->grad, t, in, grad, -1, &g);
-    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-    SX_HIP(hipStreamSynchronize(q->grad->stream));
+    qoi_gradient(q, t, in, 0, grad);
+  });
+}
+
+int sx_qoi_gradient_multi(sx_qoi* q, double t, const double* const* inputs, int wrt, double* grad)
+{
+  return api([&] {
+    SX_REQUIRE(q && inputs && grad, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(wrt >= 0 && wrt < int(q->trials.size()), SX_ERR_ARG, "wrt out of range");
+    for (size_t i = 0; i < q->trials.size(); i++) SX_REQUIRE(inputs[i], SX_ERR_ARG, "null input vector");
+    qoi_gradient(q, t, inputs, wrt, grad);
   });
 }
 

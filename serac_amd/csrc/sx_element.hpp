@@ -133,6 +133,39 @@ __device__ __forceinline__ void scatter_add(T* r, int64_t o, T v, int atomic)
     r[o] += v;
 }
 
+__device__ __forceinline__ int64_t lidx(int64_t node, int c, int nc, int64_t nnodes, int ordering);
+
+// Scatter-add of one element's NC x NQ nodal values in address order.  Lane li
+// of the element holds acc[c] for its node li; the values and node ids go
+// through LDS (sv: NC * NQ values, sn: NQ ids of the element), then pass t of
+// the element's NQ lanes adds entries t NQ .. t NQ + NQ - 1 of the element's
+// (node, component) list, node-major for byVDIM and component-major for
+// byNODES: one wave instruction then covers whole rows of consecutive
+// addresses (a structured mesh numbers the nodes of an element row
+// consecutively) instead of one component at a 3-entry stride, which cuts
+// the memory transactions of the adds about threefold.  Every lane of the
+// workgroup calls it (it holds a barrier).
+template <int NQ, int NC, typename T, typename V>
+__device__ __forceinline__ void scatter_element(V* r, const T* acc, T* sv, int* sn, int64_t node, int li, bool active,
+                                                int64_t nnodes, int ordering, int atomic)
+{
+  if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) sv[ordering ? li * NC + c : c * NQ + li] = acc[c];
+    sn[li] = int(node);
+  }
+  lds_barrier();
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < NC; t++) {
+      const int f = t * NQ + li;
+      const int j = ordering ? f / NC : li;
+      const int c = ordering ? f - j * NC : t;
+      scatter_add(r, lidx(sn[j], c, NC, nnodes, ordering), V(sv[f]), atomic);
+    }
+  }
+}
+
 __device__ __forceinline__ int64_t lidx(int64_t node, int c, int nc, int64_t nnodes, int ordering)
 {
   return ordering ? node * nc + c : int64_t(c) * nnodes + node;
@@ -373,6 +406,7 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   __shared__ double smem[EB * PER];
   __shared__ double stab[TAB];
   __shared__ double sxv[EB * 24];
+  __shared__ int    snode[EB * NQ];
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
@@ -643,19 +677,26 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
   lds_barrier();
 
   // ---- stage qz + scatter-add: (z, y, x)
+  double acc[NC];
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const double* a10 = S_ + 0 * R + c * NQ;
       const double* a11 = S_ + 1 * R + c * NQ;
-      double        acc = 0.0;
+      acc[c]            = 0.0;
 #pragma unroll
       for (int qz = 0; qz < N; qz++) {
         const int idx = (qz * N + i1) * N + i0;
-        acc += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
+        acc[c] += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
       }
-      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc, P.scatter);
     }
+  }
+  if (P.ordered) {
+    scatter_element<NQ, NC, double, double>(r, acc, S_ + 2 * R, snode + (el < EB ? el : 0) * NQ, node, li, active,
+                                            P.nnodes, P.ordering, P.scatter);
+  } else if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
   }
 }
 
@@ -677,8 +718,9 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   constexpr int  TAB  = 4 * N * N;
   using Lay           = TangentLayout<S, DS, NQ>;
 
-  __shared__ T smem[EB * PER];
-  __shared__ T stab[TAB];
+  __shared__ T   smem[EB * PER];
+  __shared__ T   stab[TAB];
+  __shared__ int snode[EB * NQ];
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
@@ -839,19 +881,26 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   }
   lds_barrier();
 
+  T acc[NC];
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
       const T* a10 = S_ + 0 * R + c * NQ;
       const T* a11 = S_ + 1 * R + c * NQ;
-      T        acc = T(0);
+      acc[c]       = T(0);
 #pragma unroll
       for (int qz = 0; qz < N; qz++) {
         const int idx = (qz * N + i1) * N + i0;
-        acc += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
+        acc[c] += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
       }
-      scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), V(acc), P.scatter);
     }
+  }
+  if (P.ordered) {
+    scatter_element<NQ, NC, T, V>(r, acc, S_ + 2 * R, snode + (el < EB ? el : 0) * NQ, node, li, active, P.nnodes,
+                                  P.ordering, P.scatter);
+  } else if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), V(acc[c]), P.scatter);
   }
 }
 

@@ -38,6 +38,7 @@ struct ElemLaunch {
   int           vprec;      // Precision of the J.v vectors (residual/linearize: FP64)
   int           scatter;    // 0: plain read-modify-write (colour launches), 1: atomics
   int           xcd_remap;  // 1: give each XCD a contiguous range of element blocks
+  int           ordered;    // 1: scatter each element's values in address order (through LDS)
   const int*    map;        // [slot][npe] global node ids
   const double* coords;     // [slot][8 corners][3], corner = a + 2b + 4c
   int64_t       slot_begin, slot_end;

@@ -17,11 +17,18 @@ __device__ __forceinline__ int64_t node_dof(int64_t node, int c, int nc, int64_t
   return ordering ? node * nc + c : int64_t(c) * nnodes + node;
 }
 
-// q at one quadrature point: X position, v value [NC], H physical gradient [NC][3]
+// q at one quadrature point: X position, v value [NC], H physical gradient
+// [NC][3], v2 the second argument's value (QOI_ID_INNER)
 template <int QOI, int NC>
-__device__ __forceinline__ double qoi_point(const double* p, const double* X, const double* v, const double* H)
+__device__ __forceinline__ double qoi_point(const double* p, const double* X, const double* v, const double* H,
+                                           const double* v2)
 {
-  if constexpr (QOI == QOI_ID_MEASURE) {
+  if constexpr (QOI == QOI_ID_INNER) {
+    double s = 0.0;
+#pragma unroll
+    for (int c = 0; c < NC; c++) s += v[c] * v2[c];
+    return s;
+  } else if constexpr (QOI == QOI_ID_MEASURE) {
     return 1.0;
   } else if constexpr (QOI == QOI_ID_POSITION) {
     const int d = int(p[0]);
@@ -77,71 +84,81 @@ __global__ void __launch_bounds__(256) k_qoi(const QoiLaunch P)
   const double* tBw = stab + 2 * N * N;
   const double* txq = stab + 4 * N * N;
 
-  if (active) {
-    const int64_t node = P.map[size_t(slot) * NQ + li];
-#pragma unroll
-    for (int c = 0; c < NC; c++) S_[c * NQ + li] = P.u[node_dof(node, c, NC, P.nnodes, P.ordering)];
+  if (active)
     for (int k = li; k < 24; k += NQ) sxv[el * 24 + k] = P.coords[size_t(slot) * 24 + k];
-  }
-  __syncthreads();
-  if (active) {
+  // value and parent gradient at this lane's point of a field, by the
+  // sum factorization (every lane calls it: it holds the barriers)
+  auto interpolate = [&](const double* uin, double* val, double* gp) {
+    if (active) {
+      const int64_t node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-      const double* uu = S_ + c * NQ + (i2 * N + i1) * N;
-      double        s0 = 0.0, s1 = 0.0;
-#pragma unroll
-      for (int x = 0; x < N; x++) {
-        s0 += tB[i0 * N + x] * uu[x];
-        s1 += tG[i0 * N + x] * uu[x];
-      }
-      S_[1 * R + c * NQ + li] = s0;
-      S_[2 * R + c * NQ + li] = s1;
+      for (int c = 0; c < NC; c++) S_[c * NQ + li] = uin[node_dof(node, c, NC, P.nnodes, P.ordering)];
     }
-  }
-  __syncthreads();
-  if (active) {
+    __syncthreads();
+    if (active) {
 #pragma unroll
-    for (int c = 0; c < NC; c++) {
-      const double* a10 = S_ + 1 * R + c * NQ;
-      const double* a11 = S_ + 2 * R + c * NQ;
-      double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
+      for (int c = 0; c < NC; c++) {
+        const double* uu = S_ + c * NQ + (i2 * N + i1) * N;
+        double        s0 = 0.0, s1 = 0.0;
 #pragma unroll
-      for (int y = 0; y < N; y++) {
-        const int    idx = (i2 * N + y) * N + i0;
-        const double b = tB[i1 * N + y], g = tG[i1 * N + y];
-        s0 += b * a10[idx];
-        s1 += b * a11[idx];
-        s2 += g * a10[idx];
+        for (int x = 0; x < N; x++) {
+          s0 += tB[i0 * N + x] * uu[x];
+          s1 += tG[i0 * N + x] * uu[x];
+        }
+        S_[1 * R + c * NQ + li] = s0;
+        S_[2 * R + c * NQ + li] = s1;
       }
-      S_[3 * R + c * NQ + li] = s0;
-      S_[4 * R + c * NQ + li] = s1;
-      S_[5 * R + c * NQ + li] = s2;
     }
-  }
-  __syncthreads();
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const double* a10 = S_ + 1 * R + c * NQ;
+        const double* a11 = S_ + 2 * R + c * NQ;
+        double        s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+        for (int y = 0; y < N; y++) {
+          const int    idx = (i2 * N + y) * N + i0;
+          const double b = tB[i1 * N + y], g = tG[i1 * N + y];
+          s0 += b * a10[idx];
+          s1 += b * a11[idx];
+          s2 += g * a10[idx];
+        }
+        S_[3 * R + c * NQ + li] = s0;
+        S_[4 * R + c * NQ + li] = s1;
+        S_[5 * R + c * NQ + li] = s2;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const double* a20 = S_ + 3 * R + c * NQ;
+        const double* a21 = S_ + 4 * R + c * NQ;
+        const double* a22 = S_ + 5 * R + c * NQ;
+        double        v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
+#pragma unroll
+        for (int z = 0; z < N; z++) {
+          const int    idx = (z * N + i1) * N + i0;
+          const double b = tB[i2 * N + z], g = tG[i2 * N + z];
+          v += b * a20[idx];
+          g0 += b * a21[idx];
+          g1 += b * a22[idx];
+          g2 += g * a20[idx];
+        }
+        val[c]        = v;
+        gp[c * 3 + 0] = g0;
+        gp[c * 3 + 1] = g1;
+        gp[c * 3 + 2] = g2;
+      }
+    }
+    __syncthreads();  // the next field reuses the regions
+  };
+  double val[NC], gp[3 * NC], val2[NC] = {}, gp2[3 * NC];
+  interpolate(P.u, val, gp);
+  if constexpr (QOI == QOI_ID_INNER) interpolate(P.u2, val2, gp2);
   double contrib = 0.0;
   if (active) {
-    double val[NC], gp[3 * NC];
-#pragma unroll
-    for (int c = 0; c < NC; c++) {
-      const double* a20 = S_ + 3 * R + c * NQ;
-      const double* a21 = S_ + 4 * R + c * NQ;
-      const double* a22 = S_ + 5 * R + c * NQ;
-      double        v = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-#pragma unroll
-      for (int z = 0; z < N; z++) {
-        const int    idx = (z * N + i1) * N + i0;
-        const double b = tB[i2 * N + z], g = tG[i2 * N + z];
-        v += b * a20[idx];
-        g0 += b * a21[idx];
-        g1 += b * a22[idx];
-        g2 += g * a20[idx];
-      }
-      val[c]        = v;
-      gp[c * 3 + 0] = g0;
-      gp[c * 3 + 1] = g1;
-      gp[c * 3 + 2] = g2;
-    }
     // geometry: x(xi) = sum_c N_c(xi) X_c, J = dx/dxi, grad_x u = grad_xi u J^-1
     const double* Xc = sxv + el * 24;
     const double  x0 = txq[i0], x1 = txq[i1], x2 = txq[i2];
@@ -176,7 +193,7 @@ __global__ void __launch_bounds__(256) k_qoi(const QoiLaunch P)
       wy += tBw[i1 * N + k];
       wz += tBw[i2 * N + k];
     }
-    contrib = qoi_point<QOI, NC>(P.params, X, val, H) * dJ * (wx * wy * wz);
+    contrib = qoi_point<QOI, NC>(P.params, X, val, H, val2) * dJ * (wx * wy * wz);
   }
   if (el < EB) sq[el * NQ + li] = contrib;
   __syncthreads();
@@ -298,6 +315,7 @@ hipError_t launch2(const QoiLaunch& P)
     case QOI_ID_NEOHOOKEAN_ENERGY:
       if constexpr (NC == 3) return launch3<N, NC, QOI_ID_NEOHOOKEAN_ENERGY>(P);
       break;
+    case QOI_ID_INNER: return launch3<N, NC, QOI_ID_INNER>(P);
   }
   return hipErrorInvalidValue;
 }

@@ -7,7 +7,7 @@
 namespace sxk {
 
 // ids of the compiled QoI integrands (the values of SX_QOI_* in serac_amd.h)
-enum QoiId { QOI_ID_MEASURE = 1, QOI_ID_POSITION = 2, QOI_ID_VALUE = 3, QOI_ID_NEOHOOKEAN_ENERGY = 4 };
+enum QoiId { QOI_ID_MEASURE = 1, QOI_ID_POSITION = 2, QOI_ID_VALUE = 3, QOI_ID_NEOHOOKEAN_ENERGY = 4, QOI_ID_INNER = 5 };
 
 struct QoiLaunch {
   int           order, nc, qoi, ordering;
@@ -16,6 +16,7 @@ struct QoiLaunch {
   int64_t       nslots;
   int64_t       nnodes;
   const double* u;        // FP64 L-vector of the trial space
+  const double* u2;       // QOI_ID_INNER: the second trial argument (same space layout)
   const double* tables;   // B, G, Bw, Gw [Q][N] and GL points [Q]
   double        params[16];
   double*       partial;  // [slot]: the element's sum of q detJ w
