@@ -180,32 +180,60 @@ def cpu_baseline(n, p, perturb, params, seconds, qf="neohookean"):
     }
 
 
-def run_solve(sx, args, space, make_form, precisions, mesh_label):
-    """The load-stepped uniaxial compression (x = 0 clamped, x = 1 displaced by -strain in x over
-    `load_steps`, warm start, Newton + Krylov per step) in each solve mode; wall time per mode."""
+def load_path(space, args, qf):
+    """Essential DOFs and their values after each load step.  Neo-Hookean: uniaxial
+    compression (x = 0 clamped, the x = 1 face moved by -strain in x; y, z free there)
+    in `load_steps` equal steps.  J2: cyclic shear of the z = 1 face against the
+    clamped z = 0 face (u_x = gamma, u_y = u_z = 0 on top), gamma going 0 -> +g ->
+    -g -> 0 in `load_steps` equal increments (g = 3 %: the steps are 4g / load_steps)."""
+    L = args.load_steps
+    if qf == "j2":
+        bot = space.boundary_dofs(0b010000, 0x7)
+        top = space.boundary_dofs(0b100000, 0x7)
+        ess = np.union1d(bot, top).astype(np.int32)
+        top_x = np.isin(ess, space.boundary_dofs(0b100000, 0x1))
+        g, q = args.shear, L // 4
+        gam = [g * k / q for k in range(1, q + 1)]
+        gam += [g - 2 * g * k / (2 * q) for k in range(1, 2 * q + 1)]
+        gam += [-g + g * k / (L - 3 * q) for k in range(1, L - 3 * q + 1)]
+        return ess, [np.where(top_x, gk, 0.0) for gk in gam], {"path": "cyclic shear of z = 1 against z = 0",
+                                                               "gamma": [round(v, 6) for v in gam]}
     ess0 = space.boundary_dofs(0b000001, 0x7)
     ess1 = space.boundary_dofs(0b000010, 0x1)
     ess = np.union1d(ess0, ess1).astype(np.int32)
     final = np.where(np.isin(ess, ess1), -args.strain, 0.0)
-    modes = {"f64": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
-             "mixed_f32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-             "mixed_bf16": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
-             "mixed_f32_learned": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-             "f64_mg": (sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64),
-             "mixed_f32_mg": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-             "mixed_bf16_mg": (sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32),
-             "mixed_f32_mg_bf16smooth": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32),
-             "mixed_f32_mg_bf16smooth32": (sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32)}
+    return ess, [final * (k / L) for k in range(1, L + 1)], {"path": "uniaxial compression", "strain": args.strain}
+
+
+def mode_precisions(sx, mode):
+    """(tangent precision, J.v arithmetic) of a solve mode name"""
+    if mode.startswith("mixed_bf16"):
+        return sx.capi.SX_PREC_BF16, sx.capi.SX_PREC_F32
+    if mode.startswith("mixed_f32"):
+        return sx.capi.SX_PREC_F32, sx.capi.SX_PREC_F32
+    if mode.startswith("f64"):
+        return sx.capi.SX_PREC_F64, sx.capi.SX_PREC_F64
+    raise SystemExit(f"bench: unknown solve mode {mode}")
+
+
+def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean"):
+    """The load-stepped path of the workload (load_path; warm start, Newton + Krylov
+    per step) in each solve mode; wall time per mode."""
+    ess, targets, path_info = load_path(space, args, qf)
     solve, finals = {}, {}
     for mode in precisions.split(","):
-        tprec, arith = modes[mode]
+        tprec, arith = mode_precisions(sx, mode)
         # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
         scatter = args.scatter if mode.startswith("f64") else args.split_scatter
         form = make_form(scatter)
         form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
         form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
+        if "_gmres" in mode:
+            # (flexible) GMRES: the J2 tangent is not symmetric under geometric nonlinearity
+            form.set_option(sx.capi.SX_OPT_LINEAR_SOLVER, sx.capi.SX_LINSOLVER_GMRES)
+            form.set_option(sx.capi.SX_OPT_GMRES_RESTART, args.gmres_restart)
         learned, t_train, train_loss = None, 0.0, None
-        if mode.endswith("_learned"):
+        if "_learned" in mode:
             # a linear U-Net trained (timed) on pairs from the linearization at
             # the undeformed state, then flexible CG in every load step
             tt = time.perf_counter()
@@ -223,19 +251,19 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
                 f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
             form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
         mg = None
-        if mode.endswith("_mg"):
+        if "_mg" in mode and "_bf16smooth" not in mode:
             # the multigrid V-cycle (set up once per linearization, inside the timed solve)
             mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
                               smoothing_range=args.mg_range)
             mg.set_reuse(args.mg_reuse)
             form.set_multigrid(mg)
-        elif mode.endswith("_mg_bf16smooth") or mode.endswith("_mg_bf16smooth32"):
+        elif "_mg_bf16smooth" in mode:
             # multigrid whose level-0 smoothing reads BF16 copies of the FP32 tangent
             mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
                               smoothing_range=args.mg_range)
             mg.set_reuse(args.mg_reuse)
             mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
-            if mode.endswith("32"):
+            if "_bf16smooth32" in mode:
                 mg.set_smoother_vectors(sx.capi.SX_PREC_F32)
             form.set_multigrid(mg)
         x = sx.DeviceVector(space.ndof)
@@ -244,9 +272,9 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
         # steps: each call starts from the essential values x holds) so that
         # every step reports as it finishes
         reps, t_solve = [], 0.0
-        for k in range(1, args.load_steps + 1):
+        for k, target in enumerate(targets, start=1):
             ts = time.perf_counter()
-            reps += sx.quasistatic_solve(form, x, ess, final * (k / args.load_steps), 1, warm_start=True,
+            reps += sx.quasistatic_solve(form, x, ess, target, 1, warm_start=True,
                                          rel_tol=args.newton_tol, lin_rel_tol=args.cg_tol,
                                          lin_max_iter=args.cg_max)
             sx.capi.call("sx_device_synchronize")
@@ -256,10 +284,11 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
         solve[mode] = {
             "seconds": round(t_solve, 3),
             "load_steps": args.load_steps,
-            "strain": args.strain,
+            **path_info,
             "newton_rel_tol": args.newton_tol,
             "cg_rel_tol": args.cg_tol,
             "precond": "Jacobi",
+            "linear_solver": "GMRES" if "_gmres" in mode else "CG",
             "tangent": {0: "f64", 1: "f32", 2: "bf16"}[tprec],
             "jvp_arithmetic": {0: "f64", 1: "f32"}[arith],
             "newton_iterations": [r["newton_iterations"] for r in reps],
@@ -276,7 +305,7 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label):
                 "fine_degree": args.mg_fine_degree, "coarse_degree": args.mg_coarse_degree,
                 "smoothing_range": args.mg_range, "setups": info["setups"], "reuse": args.mg_reuse,
                 "smoother_tangent": "bf16 copy" if "_bf16smooth" in mode else "the form's",
-                "smoother_vectors": "fp32" if mode.endswith("smooth32") else "fp64",
+                "smoother_vectors": "fp32" if "_bf16smooth32" in mode else "fp64",
                 "levels": [lv["dims"] for lv in info["levels"]],
             }
             form.set_multigrid(None)
@@ -335,6 +364,8 @@ def main():
     ap.add_argument("--no-default-solve", action="store_true", help="skip the 24^3 solve")
     ap.add_argument("--load-steps", type=int, default=10)
     ap.add_argument("--strain", type=float, default=0.05)
+    ap.add_argument("--shear", type=float, default=0.03, help="J2 workloads: the cyclic shear amplitude")
+    ap.add_argument("--gmres-restart", type=int, default=50)
     ap.add_argument("--newton-tol", type=float, default=1e-8)
     ap.add_argument("--cg-tol", type=float, default=1e-6)
     ap.add_argument("--cg-max", type=int, default=20000)
@@ -512,12 +543,10 @@ def main():
             f"training step {unet['train_step_ms']:.3f} ms ({unet['train_tflops']:.1f} TFLOP/s)")
 
     solve = None
-    if args.solve and qf != "neohookean":
-        raise SystemExit("bench: --solve runs the uniaxial-compression load path of the Neo-Hookean workloads")
     if args.solve:
         del form
         solve = run_solve(sx, args, space, make_form, args.solve_precision,
-                          f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)")
+                          f"{n}^3 hexes, p={p}, perturbed {perturb}h (the workload's mesh)", qf)
         form = make_form(args.scatter)
     solve_full = None
     if not args.solve and qf == "neohookean" and not args.no_default_solve:

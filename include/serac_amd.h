@@ -226,6 +226,8 @@ enum {
   SX_OPT_GMRES_RESTART     = 6, /* GMRES restart length (default 50, mfem's GMRESSolver default) */
   SX_OPT_SCATTER_ORDER     = 7  /* 1 (default): the element kernels add each element's values in
                                    address order through LDS (whole rows per wave instruction);
+                                   2: the same, after adding the values of nodes shared with the
+                                   previous element of the workgroup in LDS (one add per such node);
                                    0: one component per instruction from the node's own lane */
 };
 /* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
@@ -248,6 +250,17 @@ int sx_form_destroy(sx_form* f);
 /* ---- gradient operator: Functional::Gradient (fn/functional.hpp:548-669) ---- */
 /* Gradient::Mult / operator() (fn/functional.hpp:556-566 -> ActionOfGradient :406-432) */
 int sx_grad_apply(sx_grad* g, const double* dx, double* df);
+/* The Integral seam (fn/integral.hpp:127-142): domain integral k (in the order
+ * added) on E-vectors in the reference layout [element][component][node]
+ * (fn/element_restriction.cpp:448, lexicographic nodes) over the integral's
+ * domain in the order its elements were given (the whole mesh: element order),
+ * without gather or scatter: sx_form_eval_e is eval_func (linearize != 0:
+ * the evaluation with AD, storing the integral's tangent for J.v),
+ * sx_grad_apply_e is jvp.  Device pointers of nelem * ncomp * nodes_per_elem
+ * doubles (sx_form_integral_info).  Not for shape-aware forms. */
+int sx_form_integral_info(const sx_form* f, int k, int64_t* nelem, int* nodes_per_elem, int* ncomp);
+int sx_form_eval_e(sx_form* f, int k, double t, const double* u_E, double* r_E, int linearize);
+int sx_grad_apply_e(sx_grad* g, int k, const double* du_E, double* dr_E);
 /* The same action with FP32 vectors and FP32 arithmetic (the precision-split
  * operator of the mixed-precision solvers; no reference counterpart). */
 int sx_grad_apply_f32(sx_grad* g, const float* dx, float* df);

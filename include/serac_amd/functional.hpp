@@ -783,6 +783,23 @@ class Functional<test(trials...)> {
   /// scatter mode / XCD block order (serac_amd.h, SX_OPT_*)
   void SetOption(int option, int64_t value) { detail::check(sx_form_set_option(h_, option, value)); }
 
+  /// the residual with the gradient in the shape displacement set on the form
+  /// (differentiate_wrt on a ShapeAwareFunctional's shape argument)
+  template <typename... T>
+  std::tuple<Vector&, Gradient&> EvalShapeGradient(double t, const T&... args)
+  {
+    static_assert(int(sizeof...(T)) == num_trial_spaces, "wrong number of arguments");
+    static_assert(detail::count_dwrt<T...>() == 0, "the shape is the differentiated argument");
+    std::array<const double*, sizeof...(T)> in{detail::data_of(args)...};
+    sx_grad*                                g = nullptr;
+    detail::check(sx_form_eval(h_, t, in.data(), output_.Data(), SX_WRT_SHAPE, &g));
+    detail::check(sx_form_synchronize(h_));
+    shape_grad_.h_         = g;
+    shape_grad_.form_      = h_;
+    shape_grad_.test_size_ = test_size_;
+    return std::tuple<Vector&, Gradient&>(output_, shape_grad_);
+  }
+
   /// operator()(t, args...) (fn/functional.hpp:501-513).  Returns the output
   /// vector, which the next call overwrites, or a tuple of it and the
   /// Gradient when one argument is wrapped by differentiate_wrt.
@@ -815,6 +832,7 @@ class Functional<test(trials...)> {
   int64_t                            test_size_ = 0;
   Vector                             output_;
   Gradient                           grad_;
+  Gradient                           shape_grad_;
   std::vector<std::shared_ptr<void>> qdata_keep_;  // the form borrows the state buffers
 };
 
@@ -826,7 +844,7 @@ class Functional<test(trials...)> {
 /// fluxes f J^-T det J and sources s det J (ShapeCorrection,
 /// fn/shape_aware_functional.hpp:37-125; boundary integrals see the faces of
 /// X + p).  p lives in the test space (a 3-component H1 space of the mesh);
-/// differentiate_wrt applies to the trial arguments (not to p).
+/// differentiate_wrt applies to a trial argument or to p (domain integrals).
 template <typename shape, typename T>
 class ShapeAwareFunctional;
 
@@ -858,6 +876,14 @@ class ShapeAwareFunctional<shape, test(trials...)> {
   {
     detail::check(sx_form_set_shape_displacement(f_.handle(), p.Data()));
     return f_(t, args...);
+  }
+  /// operator()(t, differentiate_wrt(p), args...): the residual and the
+  /// Gradient in the shape displacement, whose action is dR/dp dp
+  template <typename... T>
+  std::tuple<Vector&, Gradient&> operator()(double t, const DifferentiateWRT<Vector>& p, const T&... args)
+  {
+    detail::check(sx_form_set_shape_displacement(f_.handle(), p.value.Data()));
+    return f_.EvalShapeGradient(t, args...);
   }
   sx_form* handle() const { return f_.handle(); }
 

@@ -237,6 +237,13 @@ class Gradient:
         self.mult(dx, df)
         return df
 
+    def apply_e(self, duE, k=0):
+        """J.v of domain integral k on E-vectors (the Integral seam's jvp)."""
+        ne, npe, nc = self.form.integral_info(k)
+        drE = DeviceVector(ne * npe * nc)
+        capi.call("sx_grad_apply_e", self.h, k, duE.ptr, drE.ptr)
+        return drE
+
     def diagonal(self):
         d = DeviceVector(self.form.test.ndof)
         capi.call("sx_grad_diagonal", self.h, d.ptr)
@@ -726,6 +733,18 @@ class Functional:
 
     def synchronize(self):
         capi.call("sx_form_synchronize", self.h)
+
+    def integral_info(self, k=0):
+        ne, npe, nc = ctypes.c_int64(), ctypes.c_int(), ctypes.c_int()
+        capi.call("sx_form_integral_info", self.h, k, ctypes.byref(ne), ctypes.byref(npe), ctypes.byref(nc))
+        return ne.value, npe.value, nc.value
+
+    def eval_e(self, uE, k=0, t=0.0, linearize=False):
+        """Domain integral k on E-vectors [elem][comp][node] (the Integral seam): returns r_E."""
+        ne, npe, nc = self.integral_info(k)
+        rE = DeviceVector(ne * npe * nc)
+        capi.call("sx_form_eval_e", self.h, k, t, uE.ptr, rE.ptr, int(linearize))
+        return rE
 
     def newton(self, x, ess, wrt=0, rel_tol=1e-8, abs_tol=1e-12, max_iter=20, lin_rel_tol=1e-8, lin_abs_tol=1e-12,
                lin_max_iter=5000, raise_on_noconv=True):

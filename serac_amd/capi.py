@@ -201,6 +201,9 @@ SIGNATURES = {
     "sx_form_stream": (_int, [_vp, ctypes.POINTER(_vp)]),
     "sx_form_destroy": (_int, [_vp]),
     "sx_grad_apply": (_int, [_vp, _vp, _vp]),
+    "sx_form_integral_info": (_int, [_vp, _int, _I64, ctypes.POINTER(_int), ctypes.POINTER(_int)]),
+    "sx_form_eval_e": (_int, [_vp, _int, ctypes.c_double, _vp, _vp, _int]),
+    "sx_grad_apply_e": (_int, [_vp, _int, _vp, _vp]),
     "sx_grad_diagonal": (_int, [_vp, _vp]),
     "sx_grad_tangent_bytes": (_int, [_vp, _I64]),
     "sx_grad_apply_f32": (_int, [_vp, _vp, _vp]),

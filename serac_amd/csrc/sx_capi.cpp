@@ -206,6 +206,8 @@ struct Integral {
   bool                 has_tangent = false;
   sx_qdata*            qdata = nullptr;  // borrowed; must outlive the form
   DeviceBuffer         d_qrow;           // [slot] -> element position in the domain
+  std::vector<int>     slot_pos;         // [slot] -> element position in the domain (host)
+  DeviceBuffer         d_emap_e;         // E-vector entry points: [slot][npe] -> pos * npe + j
 };
 
 // AddBoundaryIntegral (fn/functional.hpp:328-354): faces of the selected
@@ -385,9 +387,10 @@ sxk::ElemLaunch base_launch(const sx_form* f, const Integral& I, int mode)
 // one launch per colour; returns the number of launches.  lo: J.v with the
 // BF16 copy of the tangent (d_tangent_lo)
 int run_integral(const sx_form* f, const Integral& I, int mode, const void* u, void* r, hipEvent_t* ev = nullptr,
-                 int vprec = sxk::PREC_F64, bool lo = false)
+                 int vprec = sxk::PREC_F64, bool lo = false, const double* stop = nullptr)
 {
   sxk::ElemLaunch P = base_launch(f, I, mode);
+  P.stop            = stop;
   P.u               = u;
   P.r               = r;
   P.vprec           = vprec;
@@ -688,8 +691,10 @@ void apply_shape_gradient(sx_grad* g, const double* dx, double* df)
 
 // df = J dx; out_zeroed: df already holds zeros (the fill is skipped); lo:
 // the domain integrals use their BF16 tangent copies
+// stop: a device flag that, once non-zero, makes the J.v kernels return at
+// once (a Krylov loop that converged inside a queued chunk of iterations)
 void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullptr, int* nlaunch = nullptr,
-                    int vprec = sxk::PREC_F64, bool out_zeroed = false, bool lo = false)
+                    int vprec = sxk::PREC_F64, bool out_zeroed = false, bool lo = false, const double* stop = nullptr)
 {
   SX_REQUIRE(g && g->form, SX_ERR_ARG, "null gradient");
   SX_REQUIRE(g->valid, SX_ERR_STATE, "gradient applied before a linearization (call sx_form_eval with wrt)");
@@ -713,7 +718,7 @@ void apply_gradient(sx_grad* g, const void* dx, void* df, hipEvent_t* ev = nullp
   for (auto& I : f->integrals)
     if (depends(*I, g->wrt) && I->has_tangent)
       n += run_integral(f, *I, sxk::MODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, kprec,
-                        lo && I->d_tangent_lo.ptr != nullptr);
+                        lo && I->d_tangent_lo.ptr != nullptr, stop);
   for (auto& B : f->bintegrals)
     if (depends(*B, g->wrt) && B->has_tangent)
       n += run_boundary(f, *B, sxk::FMODE_JVP, dx, df, ev ? ev + 2 * n : nullptr, vprec);
@@ -1226,7 +1231,8 @@ int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* pa
     }
     std::vector<int> slots(pos.size());
     for (size_t k = 0; k < pos.size(); k++) slots[k] = domain[size_t(pos[k])];
-    I->nslots = int64_t(slots.size());
+    I->nslots   = int64_t(slots.size());
+    I->slot_pos = pos;
     if (qdata) {
       I->qdata = qdata;
       I->d_qrow.upload(pos);
@@ -1519,7 +1525,8 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
         f->gmres_restart = int(value);
         return;
       case SX_OPT_SCATTER_ORDER:
-        f->scatter_order = value ? 1 : 0;
+        SX_REQUIRE(value >= 0 && value <= 2, SX_ERR_ARG, "scatter order is 0, 1 or 2");
+        f->scatter_order = int(value);
         return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
@@ -1555,6 +1562,110 @@ int sx_form_destroy(sx_form* f)
 int sx_grad_apply(sx_grad* g, const double* dx, double* df)
 {
   return api([&] { apply_gradient(g, dx, df); });
+}
+
+}  // extern "C"
+
+namespace {
+
+// Domain integral k of a form run on E-vectors ([element][component][node] over
+// the integral's domain positions): the element kernels with a map that gives
+// every (element, node) its own entry, so there is no gather sharing and no
+// scatter sum.  The kernels address an entry as component-major
+// (c * ne * npe + pos * npe + j); the reference layout is transposed in and out.
+Integral& e_integral(sx_form* f, int k)
+{
+  SX_REQUIRE(f, SX_ERR_ARG, "null form");
+  SX_REQUIRE(k >= 0 && k < int(f->integrals.size()), SX_ERR_ARG, "domain integral index out of range");
+  SX_REQUIRE(!f->shape, SX_ERR_UNSUPPORTED, "E-vector evaluation of a shape-aware form is not built");
+  Integral& I = *f->integrals[size_t(k)];
+  if (!I.d_emap_e.ptr) {
+    const int        npe = f->test->npe;
+    std::vector<int> m(size_t(I.nslots) * size_t(npe));
+    for (int64_t sl = 0; sl < I.nslots; sl++)
+      for (int j = 0; j < npe; j++) m[size_t(sl) * size_t(npe) + size_t(j)] = I.slot_pos[size_t(sl)] * npe + j;
+    I.d_emap_e.upload(m);
+  }
+  return I;
+}
+
+void run_integral_e(sx_form* f, Integral& I, int mode, const double* in_E, double* out_E, const double* stop = nullptr)
+{
+  const int     nc = f->test->nc, npe = f->test->npe;
+  const int64_t n  = I.nslots * int64_t(npe) * nc;
+  hipStream_t   s  = f->stream;
+  DeviceBuffer  ti, to;
+  const double* in  = in_E;
+  double*       out = out_E;
+  if (nc > 1) {
+    ti.allocate(size_t(n) * 8);
+    to.allocate(size_t(n) * 8);
+    SX_HIP(sxk::e_transpose(in_E, ti.as<double>(), I.nslots, nc, npe, 0, s));
+    in  = ti.as<const double>();
+    out = to.as<double>();
+  }
+  SX_HIP(sxk::vec_fill(out, 0.0, n, s));
+  sxk::ElemLaunch P = base_launch(f, I, mode);
+  P.map             = I.d_emap_e.as<const int>();
+  P.nnodes          = I.nslots * int64_t(npe);
+  P.ordering        = SX_ORDERING_BYNODES;
+  P.scatter         = SX_SCATTER_COLORED;  // every entry has one writer
+  P.u               = in;
+  P.r               = out;
+  P.stop            = stop;
+  P.slot_begin      = 0;
+  P.slot_end        = I.nslots;
+  if (P.slot_end > 0) {
+    hipError_t err = hipSuccess;
+    SX_REQUIRE(sxk::launch_element(P, &err), SX_ERR_UNSUPPORTED, "element kernel not compiled for this integral");
+    SX_HIP(err);
+  }
+  if (nc > 1) SX_HIP(sxk::e_transpose(out, out_E, I.nslots, nc, npe, 1, s));
+  SX_HIP(hipStreamSynchronize(s));  // the transposition buffers are released here
+}
+
+}  // namespace
+
+extern "C" {
+
+int sx_form_integral_info(const sx_form* f, int k, int64_t* nelem, int* nodes_per_elem, int* ncomp)
+{
+  return api([&] {
+    SX_REQUIRE(f && k >= 0 && k < int(f->integrals.size()), SX_ERR_ARG, "domain integral index out of range");
+    if (nelem) *nelem = f->integrals[size_t(k)]->nslots;
+    if (nodes_per_elem) *nodes_per_elem = f->test->npe;
+    if (ncomp) *ncomp = f->test->nc;
+  });
+}
+
+int sx_form_eval_e(sx_form* f, int k, double t, const double* u_E, double* r_E, int linearize)
+{
+  return api([&] {
+    SX_REQUIRE(u_E && r_E, SX_ERR_ARG, "null vector");
+    Integral& I = e_integral(f, k);
+    if (linearize) {
+      const int    nq    = f->test->npe;
+      const size_t bytes = size_t(I.nslots) *
+                           size_t(sxk::tangent_slot_bytes(sxk::tangent_ds(f->test->nc, I.layout), nq, f->tprec));
+      I.d_tangent.allocate(bytes);
+      I.tprec       = f->tprec;
+      I.has_tangent = true;
+    }
+    (void)t;
+    run_integral_e(f, I, linearize ? sxk::MODE_LINEARIZE : sxk::MODE_RESIDUAL, u_E, r_E);
+  });
+}
+
+int sx_grad_apply_e(sx_grad* g, int k, const double* du_E, double* dr_E)
+{
+  return api([&] {
+    SX_REQUIRE(g && g->form && du_E && dr_E, SX_ERR_ARG, "null argument");
+    SX_REQUIRE(!g->shape, SX_ERR_UNSUPPORTED, "the shape gradient has no E-vector form");
+    Integral& I = e_integral(g->form, k);
+    SX_REQUIRE(depends(I, g->wrt), SX_ERR_ARG, "the integral does not depend on the gradient's trial argument");
+    SX_REQUIRE(I.has_tangent, SX_ERR_STATE, "E-vector J.v before a linearization of the integral");
+    run_integral_e(g->form, I, sxk::MODE_JVP, du_E, dr_E);
+  });
 }
 
 int sx_grad_diagonal(sx_grad* g, double* diag)
@@ -1854,7 +1965,7 @@ void cg_core(sx_grad* g, const int32_t* ess_dev, int64_t ness, const double* b, 
       SX_HIP(sxk::cg_step_x(x, r, z, Ad, d, diag, scal, part, part_rz, n, stop, s));
       if (hook) (*hook)(it, x, r);
       SX_HIP(sxk::cg_step_d(d, z, scal, part_rz, n, goal, it, stop, s));
-      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true);
+      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true, false, stop);
       SX_HIP(sxk::copy_rows(Ad, d, ess_dev, ness, s));
       SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
     }
@@ -2296,7 +2407,7 @@ void mg_pcg(sx_grad* g, sx_multigrid* M, const int32_t* ess, int64_t ness, const
       mg_apply(M, g, ess, ness, r, z);
       SX_HIP(sxk::dot_partial(r, z, n, part_rz, s));
       SX_HIP(sxk::cg_step_d(d, z, scal, part_rz, n, goal, it, stop, s));
-      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true);
+      apply_gradient(g, d, Ad, nullptr, nullptr, sxk::PREC_F64, true, false, stop);
       SX_HIP(sxk::copy_rows(Ad, d, ess, ness, s));
       SX_HIP(sxk::dot_partial(d, Ad, n, part, s));
     }

@@ -133,42 +133,62 @@ __device__ __forceinline__ void scatter_add(T* r, int64_t o, T v, int atomic)
     r[o] += v;
 }
 
-__device__ __forceinline__ int64_t lidx(int64_t node, int c, int nc, int64_t nnodes, int ordering);
-
-// Scatter-add of one element's NC x NQ nodal values in address order.  Lane li
-// of the element holds acc[c] for its node li; the values and node ids go
-// through LDS (sv: NC * NQ values, sn: NQ ids of the element), then pass t of
-// the element's NQ lanes adds entries t NQ .. t NQ + NQ - 1 of the element's
-// (node, component) list, node-major for byVDIM and component-major for
-// byNODES: one wave instruction then covers whole rows of consecutive
-// addresses (a structured mesh numbers the nodes of an element row
-// consecutively) instead of one component at a 3-entry stride, which cuts
-// the memory transactions of the adds about threefold.  Every lane of the
-// workgroup calls it (it holds a barrier).
-template <int NQ, int NC, typename T, typename V>
-__device__ __forceinline__ void scatter_element(V* r, const T* acc, T* sv, int* sn, int64_t node, int li, bool active,
-                                                int64_t nnodes, int ordering, int atomic)
-{
-  if (active) {
-#pragma unroll
-    for (int c = 0; c < NC; c++) sv[ordering ? li * NC + c : c * NQ + li] = acc[c];
-    sn[li] = int(node);
-  }
-  lds_barrier();
-  if (active) {
-#pragma unroll
-    for (int t = 0; t < NC; t++) {
-      const int f = t * NQ + li;
-      const int j = ordering ? f / NC : li;
-      const int c = ordering ? f - j * NC : t;
-      scatter_add(r, lidx(sn[j], c, NC, nnodes, ordering), V(sv[f]), atomic);
-    }
-  }
-}
-
 __device__ __forceinline__ int64_t lidx(int64_t node, int c, int nc, int64_t nnodes, int ordering)
 {
   return ordering ? node * nc + c : int64_t(c) * nnodes + node;
+}
+
+// Scatter-add of one element's NC x NQ nodal values in address order.  Lane li
+// of the element holds acc[c] for its node li; the values and node ids go
+// through LDS (sv: the element's NC * NQ values, sn: the workgroup's node ids,
+// NQ per element), then pass t of the element's NQ lanes adds entries
+// t NQ .. t NQ + NQ - 1 of the element's (node, component) list, node-major for
+// byVDIM and component-major for byNODES: one wave instruction then covers
+// whole rows of consecutive addresses (a structured mesh numbers the nodes of
+// an element row consecutively) instead of one component at a 3-entry stride,
+// which cuts the memory transactions of the adds about threefold.
+// With merge, the nodes an element shares with the previous element of the
+// workgroup across its i0 = 0 face (consecutive elements along x) are added
+// into that element's values in LDS first and skipped in the passes: one add
+// per shared node and workgroup instead of two.  Every lane of the workgroup
+// calls it (it holds barriers); sv_prev is the previous element's sv.
+template <int N, int NC, typename T, typename V>
+__device__ __forceinline__ void scatter_element(V* r, const T* acc, T* sv, T* sv_prev, int* sn_wg, int el, int eb,
+                                                int64_t node, int li, bool active, int64_t nnodes, int ordering,
+                                                int atomic, bool merge)
+{
+  constexpr int NQ = N * N * N;
+  int*          sn = sn_wg + (el < eb ? el : 0) * NQ;  // lanes past the workgroup's elements touch nothing
+  if (el < eb) {
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) sv[ordering ? li * NC + c : c * NQ + li] = acc[c];
+    }
+    sn[li] = active ? int(node) : -1;
+  }
+  lds_barrier();
+  if (merge) {
+    const int i0 = li % N;
+    if (active && el > 0 && i0 == 0) {
+      const int jp = li + (N - 1);  // (N - 1, i1, i2) of the previous element
+      if (sn_wg[(el - 1) * NQ + jp] == int(node)) {
+#pragma unroll
+        for (int c = 0; c < NC; c++) sv_prev[ordering ? jp * NC + c : c * NQ + jp] += acc[c];
+        sn[li] = -1;
+      }
+    }
+    lds_barrier();
+  }
+  if (active) {
+#pragma unroll
+    for (int t = 0; t < NC; t++) {
+      const int f  = t * NQ + li;
+      const int j  = ordering ? f / NC : li;
+      const int c  = ordering ? f - j * NC : t;
+      const int nd = sn[j];
+      if (nd >= 0) scatter_add(r, lidx(nd, c, NC, nnodes, ordering), V(sv[f]), atomic);
+    }
+  }
 }
 
 // Jacobian of the trilinear map at parent point xi from the 8 corner
@@ -692,8 +712,9 @@ __global__ void __launch_bounds__(256) k_element(const ElemLaunch P)
     }
   }
   if (P.ordered) {
-    scatter_element<NQ, NC, double, double>(r, acc, S_ + 2 * R, snode + (el < EB ? el : 0) * NQ, node, li, active,
-                                            P.nnodes, P.ordering, P.scatter);
+    scatter_element<N, NC, double, double>(r, acc, S_ + 2 * R, smem + (el > 0 && el < EB ? el - 1 : 0) * PER + 2 * R,
+                                           snode, el, EB, node, li, active, P.nnodes, P.ordering, P.scatter,
+                                           P.ordered == 2);
   } else if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
@@ -721,6 +742,10 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   __shared__ T   smem[EB * PER];
   __shared__ T   stab[TAB];
   __shared__ int snode[EB * NQ];
+
+  // a Krylov loop that already converged: the whole grid leaves (the flag
+  // was written by an earlier kernel on the stream, so every block reads the same)
+  if (P.stop != nullptr && *P.stop != 0.0) return;
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
@@ -896,8 +921,8 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
     }
   }
   if (P.ordered) {
-    scatter_element<NQ, NC, T, V>(r, acc, S_ + 2 * R, snode + (el < EB ? el : 0) * NQ, node, li, active, P.nnodes,
-                                  P.ordering, P.scatter);
+    scatter_element<N, NC, T, V>(r, acc, S_ + 2 * R, smem + (el > 0 && el < EB ? el - 1 : 0) * PER + 2 * R, snode,
+                                 el, EB, node, li, active, P.nnodes, P.ordering, P.scatter, P.ordered == 2);
   } else if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), V(acc[c]), P.scatter);

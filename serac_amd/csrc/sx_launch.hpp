@@ -56,6 +56,7 @@ struct ElemLaunch {
   const double*  shape;        // residual/linearize: shape displacement p (3-component L-vector of the
                                // test space) or null; the geometry becomes X + p
   const double*  dshape;       // MODE_SHAPE_JVP: the direction of the shape displacement
+  const double*  stop;         // J.v: when non-null and *stop != 0 the kernel returns at once
   double        params[16];
   double        t;
   hipStream_t   stream;

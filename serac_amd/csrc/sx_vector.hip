@@ -445,6 +445,29 @@ hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, 
   return hipGetLastError();
 }
 
+__global__ void __launch_bounds__(kThreads) k_e_transpose(const double* in, double* out, int64_t ne, int nc, int npe,
+                                                           int dir)
+{
+  const int64_t n = ne * nc * npe;
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) {
+    // i indexes the reference layout [e][c][j]
+    const int64_t j = i % npe, ec = i / npe, c = ec % nc, e = ec / nc;
+    const int64_t o = (c * ne + e) * npe + j;  // component-major [c][e][j]
+    if (dir == 0)
+      out[o] = in[i];
+    else
+      out[i] = in[o];
+  }
+}
+
+hipError_t e_transpose(const double* in, double* out, int64_t ne, int nc, int npe, int dir, hipStream_t s)
+{
+  const int64_t n = ne * nc * npe;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_e_transpose, dim3(grid_for(n)), dim3(kThreads), 0, s, in, out, ne, nc, npe, dir);
+  return hipGetLastError();
+}
+
 hipError_t csr_spmv(const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                     int64_t nrows, hipStream_t s)
 {

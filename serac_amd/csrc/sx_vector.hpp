@@ -57,6 +57,10 @@ hipError_t gather_e(const int* map, const double* L, double* E, int64_t ne, int 
 hipError_t scatter_e(const int64_t* offs, const int32_t* ents, const double* E, double* L, int64_t nnodes, int npe,
                      int nc, int ordering, hipStream_t s);
 
+// E-vector layouts: dir 0 copies the reference layout [e][c][j] into the
+// component-major [c][e][j], dir 1 back
+hipError_t e_transpose(const double* in, double* out, int64_t ne, int nc, int npe, int dir, hipStream_t s);
+
 // y = A x, A in CSR (int64 row offsets, int32 columns)
 hipError_t csr_spmv(const int64_t* rp, const int32_t* col, const double* val, const double* x, double* y,
                     int64_t nrows, hipStream_t s);

@@ -137,6 +137,24 @@ int main()
     Vector p0(std::vector<double>(size_t(ndof), 0.0));
     std::vector<double> Rs = sres(0.0, p0, u).ToHost();
     CHECK(relerr(Rs, R_ref) < 1e-12, "ShapeAwareFunctional with p = 0 vs oracle residual: %.2e", relerr(Rs, R_ref));
+    // differentiate_wrt(p): the shape gradient against central differences
+    std::vector<double> ph(static_cast<size_t>(ndof)), dph(static_cast<size_t>(ndof));
+    for (int64_t i = 0; i < ndof; i++) {
+      ph[size_t(i)]  = 0.01 * (u01(8, 2000, uint64_t(i)) - 0.5);
+      dph[size_t(i)] = 0.01 * (u01(9, 2000, uint64_t(i)) - 0.5);
+    }
+    Vector pv(ph), dpv(dph);
+    auto [rs, dRdp] = sres(0.0, differentiate_wrt(pv), u);
+    std::vector<double> Jp = dRdp(dpv).ToHost();
+    const double        eps = 1e-6;
+    std::vector<double> pp(ph), pm(ph);
+    for (size_t i = 0; i < ph.size(); i++) pp[i] += eps * dph[i], pm[i] -= eps * dph[i];
+    Vector              vpp(pp), vpm(pm);
+    std::vector<double> Rp = sres(0.0, vpp, u).ToHost(), Rm = sres(0.0, vpm, u).ToHost(), fd(Rp.size());
+    for (size_t i = 0; i < fd.size(); i++) fd[i] = (Rp[i] - Rm[i]) / (2 * eps);
+    CHECK(relerr(fd, Jp) < 1e-6, "ShapeAwareFunctional differentiate_wrt(shape) vs central differences: %.2e",
+          relerr(fd, Jp));
+    (void)rs;
   }
   {
     // The same problem on a space in the caller's numbering, mfem-style:

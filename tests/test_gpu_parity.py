@@ -179,15 +179,19 @@ def test_unstructured_patch_mesh_parity(gpu_lib, patch_mesh):
     assert relerr(grad(sx.DeviceVector.from_numpy(dU)).numpy(), of.jvp(dU)) < RTOL
 
 
-@pytest.mark.parametrize("qf,p", [("heat", 2), ("neohookean", 3), ("mass_diffusion_vec", 1)])
+@pytest.mark.parametrize("qf,p", [("heat", 2), ("neohookean", 3), ("mass_diffusion_vec", 1),
+                                  ("neohookean", 4), ("neohookean", 2)])
 @pytest.mark.parametrize("remap", [0, 1])
-def test_atomic_scatter_parity(gpu_lib, qf, p, remap):
+@pytest.mark.parametrize("order", [0, 1, 2])  # SX_OPT_SCATTER_ORDER: per lane, address order, merged faces
+@pytest.mark.parametrize("ordering", [1, 0])
+def test_atomic_scatter_parity(gpu_lib, qf, p, remap, order, ordering):
     nc = QF[qf][2]
-    pr = Pair(5, p, nc)
+    pr = Pair(5, p, nc, ordering=ordering)
     gid, oid, _, params = QF[qf]
     gf = sx.Functional(pr.gs, [pr.gs])
     gf.set_option(sx.capi.SX_OPT_SCATTER, sx.capi.SX_SCATTER_ATOMIC)
     gf.set_option(sx.capi.SX_OPT_XCD_REMAP, remap)
+    gf.set_option(sx.capi.SX_OPT_SCATTER_ORDER, order)
     gf.add_domain_integral(gid, params)
     of = O.Form(pr.os)
     of.add_domain_integral(oid, params)
