@@ -307,6 +307,19 @@ typedef struct sx_solve_report {
  * (r, z) stops unconverged (SX_ERR_NOCONV), as mfem's does. */
 int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x, double rel_tol,
                 double abs_tol, int max_iter, sx_solve_report* report);
+/* sx_cg_solve without any device allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The five work vectors, the (r, z) partials and the essential DOF list live
+ * in a library-owned device array of SX_NOALLOC_SCRATCH_BYTES per GPU, held
+ * until the process ends: 40 ndof + 8 * 1024 + 4 ness bytes (each part
+ * rounded up to 256) must fit, else SX_ERR_ARG.  The form must have been
+ * linearized (its tangent is the form's own, not scratch).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race.  Same results,
+ * iteration counts and errors as sx_cg_solve. */
+#define SX_NOALLOC_SCRATCH_BYTES (128u << 20)
+int sx_cg_solve_noalloc(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                        double rel_tol, double abs_tol, int max_iter, sx_solve_report* report);
 /* Restarted GMRES(restart) with left Jacobi preconditioning on J_bc x = b
  * (x0 = 0), mfem::GMRESSolver + HypreSmoother Jacobi as built by
  * buildLinearSolverAndPreconditioner (num/equation_solver.cpp:916-919,

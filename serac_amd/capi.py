@@ -210,6 +210,8 @@ SIGNATURES = {
     "sx_grad_apply_timed": (_int, [_vp, _int, _vp, _vp, _int, _D, _D, ctypes.POINTER(_int)]),
     "sx_cg_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                            ctypes.POINTER(SolveReport)]),
+    "sx_cg_solve_noalloc": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
+                                   ctypes.POINTER(SolveReport)]),
     "sx_fgmres_solve": (_int, [_vp, _vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int,
                                _int, ctypes.POINTER(SolveReport)]),
     "sx_gmres_solve": (_int, [_vp, _I32, ctypes.c_int64, _vp, _vp, ctypes.c_double, ctypes.c_double, _int, _int,

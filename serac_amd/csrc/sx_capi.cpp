@@ -133,6 +133,37 @@ void ensure_device()
   ok = true;
 }
 
+}  // namespace
+
+// Scratch of the allocation-free entry points (sx_cg_solve_noalloc): every
+// GPU the library runs on holds its own copy until the process ends.  Host
+// code never uses the array's name as an address: the device address of the
+// current GPU's copy comes from hipGetSymbolAddress (noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[SX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex    g_scratch_mu;
+void*         g_scratch_dev[kMaxScratchDevices] = {};
+
+// The device address of the current GPU's scratch array, looked up once per
+// device (sx_init does it for the device it selects).
+void* noalloc_scratch()
+{
+  int dev = 0;
+  SX_HIP(hipGetDevice(&dev));
+  SX_REQUIRE(dev >= 0 && dev < kMaxScratchDevices, SX_ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    SX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    SX_REQUIRE(p != nullptr, SX_ERR_HIP, "hipGetSymbolAddress gave no address for the scratch array");
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
+
 const double* element_tables(int N)
 {
   ensure_device();
@@ -742,6 +773,7 @@ int sx_init(int device)
     SX_REQUIRE(device >= 0 && device < n, SX_ERR_ARG, "device index out of range");
     rt().device = device;
     SX_HIP(hipSetDevice(device));
+    (void)noalloc_scratch();
   });
 }
 
@@ -2992,6 +3024,48 @@ int sx_cg_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double*
     upload_ess(w.ess, ess_dofs, ness, g->form->test->ndof());
     cg(g, w.ess.as<const int32_t>(), ness, b, x, rel_tol, abs_tol, max_iter, w, &rep);
     SX_HIP(hipStreamSynchronize(g->form->stream));
+    if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "CG did not converge");
+  });
+  if (report) *report = rep;
+  return rc;
+}
+
+// sx_cg_solve with its work vectors, (r, z) partials and essential DOF list
+// carved out of the scratch array instead of allocated.  cg_core writes every
+// vector before it reads it (x is filled, r copied from b, z and d formed,
+// Ad written by the first J.v); the partials are cleared here.
+int sx_cg_solve_noalloc(sx_grad* g, const int32_t* ess_dofs, int64_t ness, const double* b, double* x,
+                        double rel_tol, double abs_tol, int max_iter, sx_solve_report* report)
+{
+  sx_solve_report rep{};
+  int             rc = api([&] {
+    SX_REQUIRE(g && g->form && b && x, SX_ERR_ARG, "null argument");
+    require_trial_grad(g, "CG on a gradient before a linearization");
+    SX_REQUIRE(ness >= 0 && (ness == 0 || ess_dofs), SX_ERR_ARG, "null essential DOF list");
+    sx_form*      f = g->form;
+    const int64_t n = f->test->ndof();
+    for (int64_t k = 0; k < ness; k++)
+      SX_REQUIRE(ess_dofs[k] >= 0 && ess_dofs[k] < n, SX_ERR_ARG,
+                 "essential DOF " + std::to_string(ess_dofs[k]) + " out of range [0, " + std::to_string(n) + ")");
+    auto          up    = [](uint64_t bytes) { return (bytes + 255) / 256 * 256; };
+    const int64_t nblk  = sxk::reduction_blocks();
+    const uint64_t e_b  = up(uint64_t(ness) * 4), p_b = up(uint64_t(nblk) * 8), v_b = up(uint64_t(n) * 8);
+    const uint64_t need = e_b + p_b + 5 * v_b;
+    SX_REQUIRE(need <= uint64_t(SX_NOALLOC_SCRATCH_BYTES), SX_ERR_ARG,
+               "sx_cg_solve_noalloc needs " + std::to_string(need) + " bytes of scratch for " + std::to_string(n) +
+                   " DOFs and " + std::to_string(ness) + " essential DOFs; the maximum is " +
+                   std::to_string(uint64_t(SX_NOALLOC_SCRATCH_BYTES)));
+    unsigned char* base = static_cast<unsigned char*>(noalloc_scratch());
+    int32_t*       ess  = reinterpret_cast<int32_t*>(base);
+    double*        part = reinterpret_cast<double*>(base + e_b);
+    double*        vec  = reinterpret_cast<double*>(base + e_b + p_b);
+    const int64_t  vs   = int64_t(v_b / 8);
+    hipStream_t    s    = f->stream;
+    if (ness > 0) SX_HIP(hipMemcpyAsync(ess, ess_dofs, size_t(ness) * 4, hipMemcpyHostToDevice, s));
+    SX_HIP(sxk::vec_fill(part, 0.0, nblk, s));
+    const CgVectors v{vec, vec + vs, vec + 2 * vs, vec + 3 * vs, vec + 4 * vs, part};
+    cg_core(g, ess, ness, b, x, rel_tol, abs_tol, max_iter, v, &rep, nullptr);
+    SX_HIP(hipStreamSynchronize(s));
     if (!rep.converged) throw sx_failure(SX_ERR_NOCONV, "CG did not converge");
   });
   if (report) *report = rep;
