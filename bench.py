@@ -223,7 +223,7 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean
     solve, finals = {}, {}
     for mode in precisions.split(","):
         tprec, arith = mode_precisions(sx, mode)
-        # FP64 adds go fastest as atomics, FP32-arithmetic ones as coloured RMWs
+        # FP64 and FP32-arithmetic forms each take their own scatter option (both atomic by default)
         scatter = args.scatter if mode.startswith("f64") else args.split_scatter
         form = make_form(scatter)
         form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
@@ -391,8 +391,8 @@ def main():
                     help="also time the learned preconditioner's U-Net (inference and one training step) on the "
                          "workload's vertex grid")
     ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
-    ap.add_argument("--split-scatter", default="colored", choices=["colored", "atomic"],
-                    help="scatter mode of the FP32 / BF16 J.v measurements")
+    ap.add_argument("--split-scatter", default="atomic", choices=["colored", "atomic"],
+                    help="scatter mode of the FP32 / BF16 J.v measurements and of the FP32-arithmetic solves")
     ap.add_argument("--xcd-remap", type=int, default=1)
     ap.add_argument("--scatter-order", type=int, default=1,
                     help="1: element values added in address order through LDS; 0: one component per instruction")
