@@ -31,8 +31,10 @@ WORKLOADS = {
     "neohookean_p3_24": (24, 3, 0.2, (1.0, 10.0, 0.25), "neohookean"),  # quick runs of the same path
     "neohookean_p4_48": (48, 4, 0.2, (1.0, 10.0, 0.25), "neohookean"),
     "neohookean_p2_128": (128, 2, 0.0, (1.0, 10.0, 0.25), "neohookean"),
+    "neohookean_p2_32": (32, 2, 0.0, (1.0, 10.0, 0.25), "neohookean"),  # quick runs of the p = 2 path
     # J2 with linear hardening: E, nu, sigma_y, Hi, Hk, density, geometric nonlinearity
     "j2_p2_96": (96, 2, 0.0, (200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0), "j2"),
+    "j2_p2_32": (32, 2, 0.0, (200.0, 0.3, 0.5, 2.0, 0.0, 1.0, 1.0), "j2"),  # quick runs of the J2 path
 }
 TANGENT_ENTRIES = {"neohookean": 45, "j2": 81}
 
@@ -232,40 +234,47 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean
             # (flexible) GMRES: the J2 tangent is not symmetric under geometric nonlinearity
             form.set_option(sx.capi.SX_OPT_LINEAR_SOLVER, sx.capi.SX_LINSOLVER_GMRES)
             form.set_option(sx.capi.SX_OPT_GMRES_RESTART, args.gmres_restart)
+        mg = None
+        if "_mg" in mode:
+            # the multigrid V-cycle (set up once per linearization, inside the timed solve); with
+            # _bf16smooth its level-0 smoothing reads BF16 copies of the FP32 tangent (and, with
+            # _bf16smooth32, runs on FP32 vectors)
+            mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
+                              smoothing_range=args.mg_range)
+            mg.set_reuse(args.mg_reuse)
+            if "_bf16smooth" in mode:
+                mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
+            if "_bf16smooth32" in mode:
+                mg.set_smoother_vectors(sx.capi.SX_PREC_F32)
+            if "_learned" not in mode:
+                form.set_multigrid(mg)
         learned, t_train, train_loss = None, 0.0, None
         if "_learned" in mode:
             # a linear U-Net trained (timed) on pairs from the linearization at
-            # the undeformed state, then flexible CG in every load step
+            # the undeformed state, then flexible CG in every load step; with
+            # _mg the network corrects the V-cycle (trained on the error the
+            # V-cycle leaves); _nonet: the same flexible CG with the network's
+            # correction off (gain 0, no training), the ablation
             tt = time.perf_counter()
             learned = sx.Learned(space, seed=11)
             learned.set_activation(sx.capi.SX_UNET_IDENTITY, seed=11)
             learned.configure(max_pairs=args.learn_pairs, stall_window=20)
-            _, g0 = form(0.0, sx.DeviceVector(space.ndof), wrt=0)
-            learned.generate_pairs(g0, ess, args.learn_pairs, seed=5)
-            del g0
-            train_loss = learned.train(args.learn_steps, lr=args.learn_lr, seed=3, batch=args.learn_batch)
-            sx.capi.call("sx_device_synchronize")
-            t_train = time.perf_counter() - tt
-            log(f"{mode}: trained {args.learn_steps} Adam steps x {args.learn_batch} pairs of "
-                f"{learned.info()['pairs']} in {t_train:.2f} s (pair generation included), loss of the last "
-                f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
+            if mg is not None:
+                learned.set_multigrid(mg)
+            if mode.endswith("_nonet"):
+                learned.set_gain(0.0)
+                log(f"{mode}: network correction off (gain 0)")
+            else:
+                _, g0 = form(0.0, sx.DeviceVector(space.ndof), wrt=0)
+                learned.generate_pairs(g0, ess, args.learn_pairs, seed=5)
+                del g0
+                train_loss = learned.train(args.learn_steps, lr=args.learn_lr, seed=3, batch=args.learn_batch)
+                sx.capi.call("sx_device_synchronize")
+                t_train = time.perf_counter() - tt
+                log(f"{mode}: trained {args.learn_steps} Adam steps x {args.learn_batch} pairs of "
+                    f"{learned.info()['pairs']} in {t_train:.2f} s (pair generation included), loss of the last "
+                    f"tenth {train_loss:.4f}, gain {learned.gain():.3f}")
             form.set_preconditioner(learned, sx.capi.SX_PRECOND_LEARNED)
-        mg = None
-        if "_mg" in mode and "_bf16smooth" not in mode:
-            # the multigrid V-cycle (set up once per linearization, inside the timed solve)
-            mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
-                              smoothing_range=args.mg_range)
-            mg.set_reuse(args.mg_reuse)
-            form.set_multigrid(mg)
-        elif "_mg_bf16smooth" in mode:
-            # multigrid whose level-0 smoothing reads BF16 copies of the FP32 tangent
-            mg = sx.Multigrid(space, fine_degree=args.mg_fine_degree, coarse_degree=args.mg_coarse_degree,
-                              smoothing_range=args.mg_range)
-            mg.set_reuse(args.mg_reuse)
-            mg.set_smoother_precision(sx.capi.SX_PREC_BF16)
-            if "_bf16smooth32" in mode:
-                mg.set_smoother_vectors(sx.capi.SX_PREC_F32)
-            form.set_multigrid(mg)
         x = sx.DeviceVector(space.ndof)
         sx.capi.call("sx_device_synchronize")
         # one load step per call (the same arithmetic as one call over all
@@ -308,12 +317,16 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean
                 "smoother_vectors": "fp32" if "_bf16smooth32" in mode else "fp64",
                 "levels": [lv["dims"] for lv in info["levels"]],
             }
-            form.set_multigrid(None)
+            if learned is None:
+                form.set_multigrid(None)
             del mg
         if learned is not None:
             # the wall time above includes the training
             solve[mode]["seconds"] = round(t_solve + t_train, 3)
-            solve[mode]["precond"] = "learned linear U-Net + Jacobi in flexible CG (Jacobi-CG fallback)"
+            base = "the multigrid V-cycle" if "_mg" in mode else "Jacobi"
+            solve[mode]["precond"] = (f"{base} in flexible CG, network correction off (gain 0)"
+                                      if mode.endswith("_nonet") else
+                                      f"learned linear U-Net correction + {base} in flexible CG (Jacobi-CG fallback)")
             solve[mode]["learned"] = {
                 "train_seconds": round(t_train, 3),
                 "adam_steps": args.learn_steps,

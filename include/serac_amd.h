@@ -44,7 +44,8 @@ enum { SX_ORDERING_BYNODES = 0, SX_ORDERING_BYVDIM = 1 };
 
 /* compiled quadrature-point functions (the integrand registry) */
 enum {
-  SX_QF_HEAT           = 1, /* params {k, s}: source -s, flux k grad T
+  SX_QF_HEAT           = 1, /* params {k, s, rho c_p}: source -s, flux k grad T (rho c_p read
+                               with two trial arguments only: source rho c_p dT/dt - s)
                                phy/heat_transfer.hpp:381-406,493-503 +
                                phy/materials/thermal_material.hpp:50-55 */
   SX_QF_MASS_DIFFUSION = 2, /* params {a, b}: {a u, b grad u}
@@ -53,7 +54,8 @@ enum {
                                geometric nonlinearity on, quasi-static
                                phy/solid_mechanics.hpp:834-852,
                                phy/materials/solid_material.hpp:112-122 */
-  SX_QF_LINEAR_ELASTIC = 4, /* params {lambda, mu}: sigma = lambda tr(eps) I + 2 mu eps */
+  SX_QF_LINEAR_ELASTIC = 4, /* params {lambda, mu, density}: sigma = lambda tr(eps) I + 2 mu eps
+                               (density read with two trial arguments only) */
   SX_QF_J2_LINEAR      = 5  /* params {E, nu, sigma_y, Hi, Hk, density, geom_nonlin (0/1)}:
                                MaterialStressFunctor<J2SmallStrain<LinearHardening>>,
                                quasi-static; state of 10 doubles per quadrature point
@@ -104,7 +106,7 @@ int sx_mesh_create(int nverts, const double* coords, int nelem, const int* hex_v
 int sx_mesh_perturb(sx_mesh* mesh, uint64_t seed, double amp);
 /* Replaces buildMeshFromFile -> mfem::Mesh(std::istream&)
  * (src/serac/mesh/mesh_utils.cpp:19-43) for MFEM mesh v1.0 text (the format of
- * data/meshes/*.mesh): 3D hexahedra (geometry 5), quadrilateral boundary
+ * the .mesh files under data/meshes): 3D hexahedra (geometry 5), quadrilateral boundary
  * elements (geometry 3), straight (vertex) geometry.  Element and boundary
  * attributes are kept.  Curved meshes (`nodes`) and other element types give
  * SX_ERR_UNSUPPORTED; malformed text gives SX_ERR_ARG.  Cuboid meshes carry
@@ -171,7 +173,19 @@ int sx_qdata_destroy(sx_qdata* q);
 /* constructor, fn/functional.hpp:219-273 (single rank: P = identity) */
 int sx_form_create(const sx_space* test, const sx_space* const* trials, int ntrials, sx_form** out);
 /* AddDomainIntegral, fn/functional.hpp:286-317 (+ fn/integral.hpp:184-260).
- * elems == NULL / nelem == 0 means the whole mesh (EntireDomain). */
+ * elems == NULL / nelem == 0 means the whole mesh (EntireDomain).
+ * depends_on holds one trial index (DependsOn<i>), or two (DependsOn<a, b>)
+ * for the integrands that read a rate as their second argument, as serac's
+ * physics modules add them: the solid materials (SX_QF_NEOHOOKEAN,
+ * SX_QF_LINEAR_ELASTIC, SX_QF_J2_LINEAR) with the acceleration, source
+ * density * a (MaterialStressFunctor, phy/solid_mechanics.hpp:834-852), and
+ * SX_QF_HEAT with the temperature rate, source rho c_p * dT/dt - s
+ * (ThermalMaterialIntegrand, phy/heat_transfer.hpp:380-406, 432-436).  The
+ * density is params[0] (Neo-Hookean), params[5] (J2) or params[2] (linear
+ * elastic); rho c_p is params[2] (heat).  Gradients in either argument, J.v,
+ * diagonals and assembly follow; the E-vector entry points refuse such an
+ * integral (SX_ERR_UNSUPPORTED).  The index k of the E-vector entry points
+ * counts the integrals as added. */
 int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, int nparams, const int* depends_on,
                                 int ndep, const int* elems, int nelem);
 /* AddBoundaryIntegral, fn/functional.hpp:328-354 (fn/integral.hpp:275-347).
@@ -186,6 +200,21 @@ int sx_form_add_boundary_integral(sx_form* f, int qfunc_id, const double* params
  * element must be an exterior face of the mesh. */
 int sx_form_add_boundary_integral_attr(sx_form* f, int qfunc_id, const double* params, int nparams,
                                        const int* depends_on, int ndep, uint64_t attr_mask);
+/* The same on the exterior faces a caller predicate selects:
+ * Domain::ofBoundaryElements(mesh, predicate) (fn/domain.hpp:113-118,
+ * fn/domain.cpp:312-386).  The library calls pred(corners, attr, user) on the
+ * host once per exterior face (a face that belongs to one element), in
+ * (element, local face) order: corners [4][3] are the face's vertex
+ * coordinates, attr the attribute of the mesh's boundary element on that face
+ * (-1 when it has none); a nonzero return selects the face. */
+typedef int (*sx_face_predicate)(const double* corners, int attr, void* user);
+int sx_form_add_boundary_integral_pred(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                       const int* depends_on, int ndep, sx_face_predicate pred, void* user);
+/* The same on an explicit list of exterior faces, each given by its four
+ * vertex ids (in any order): face_vertices [nfaces][4], host.  A face that is
+ * not an exterior face of the mesh is refused (SX_ERR_ARG). */
+int sx_form_add_boundary_integral_faces(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                        const int* depends_on, int ndep, const int32_t* face_vertices, int64_t nfaces);
 /* AddDomainIntegral with a QuadratureData argument (fn/functional.hpp:286-317,
  * fn/quadrature_data.hpp:111-167).  qdata must hold one record per element of
  * the domain (in domain order) and quadrature point; it is borrowed and must
@@ -224,11 +253,15 @@ enum {
   SX_OPT_JVP_ARITHMETIC    = 4,
   SX_OPT_LINEAR_SOLVER     = 5, /* SX_LINSOLVER_CG (default) or _GMRES: the linear solves of the Newton loops */
   SX_OPT_GMRES_RESTART     = 6, /* GMRES restart length (default 50, mfem's GMRESSolver default) */
-  SX_OPT_SCATTER_ORDER     = 7  /* 1 (default): the element kernels add each element's values in
+  SX_OPT_SCATTER_ORDER     = 7, /* 1 (default): the element kernels add each element's values in
                                    address order through LDS (whole rows per wave instruction);
                                    2: the same, after adding the values of nodes shared with the
                                    previous element of the workgroup in LDS (one add per such node);
                                    0: one component per instruction from the node's own lane */
+  SX_OPT_LINE_SEARCH       = 8  /* max_line_search_iterations of NonlinearSolverOptions
+                                   (num/solver_config.hpp:389-391, num/equation_solver.cpp:21-185):
+                                   0 (default) plain Newton; k > 0 the two-way backtracking line
+                                   search of serac's NewtonSolver in the Newton loops */
 };
 /* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
 enum { SX_LINSOLVER_CG = 0, SX_LINSOLVER_GMRES = 1 };
@@ -299,6 +332,7 @@ typedef struct sx_solve_report {
   double initial_norm;
   double final_norm;
   int    converged;
+  int    line_search_steps; /* residual evaluations of the line search (SX_OPT_LINE_SEARCH) */
 } sx_solve_report;
 
 /* Jacobi-preconditioned CG on the gradient with essential rows/columns
@@ -566,6 +600,13 @@ int sx_qoi_add_domain_integral_dep(sx_qoi* q, int qoi_id, const double* params, 
  * boundary elements with those attributes; dA = |dX/dxi1 x dX/dxi2| */
 int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, int face_mask,
                                  uint64_t attr_mask);
+/* The same over the exterior faces pred selects (as
+ * sx_form_add_boundary_integral_pred: Domain::ofBoundaryElements, fn/domain.cpp:312-386)
+ * or over an explicit list of exterior faces ([nfaces][4] vertex ids, host). */
+int sx_qoi_add_boundary_integral_pred(sx_qoi* q, int qoi_id, const double* params, int nparams,
+                                      sx_face_predicate pred, void* user);
+int sx_qoi_add_boundary_integral_faces(sx_qoi* q, int qoi_id, const double* params, int nparams,
+                                       const int32_t* face_vertices, int64_t nfaces);
 /* Functional<double(trial)>::operator()(t, U); u is a device L-vector */
 int sx_qoi_eval(sx_qoi* q, double t, const double* u, double* value);
 /* dq/dU at U into grad (device L-vector) */

@@ -18,6 +18,7 @@
 #include <array>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <set>
 #include <stdexcept>
@@ -291,15 +292,34 @@ struct Domain {
 /// mesh's bounding box (EntireBoundary when face_mask = 0), or the boundary
 /// elements with chosen attributes (Domain::ofBoundaryElements by attribute)
 struct BoundaryDomain {
+  using vec3      = std::array<double, 3>;
+  using Predicate = std::function<bool(std::vector<vec3>, int)>;
   int                   face_mask = 0;  // bit 0 x = min, 1 x = max, 2 y = min, 3 y = max, 4 z = min, 5 z = max
   uint64_t              attr_mask = 0;  // bit a - 1 for attribute a; when nonzero it selects the faces
-  static BoundaryDomain EntireBoundary() { return BoundaryDomain{0, 0}; }
-  static BoundaryDomain Planes(int mask) { return BoundaryDomain{mask, 0}; }
+  Predicate             predicate;      // when set: the exterior faces it selects
+  std::vector<int32_t>  faces;          // when nonempty: these exterior faces, [n][4] vertex ids
+  static BoundaryDomain EntireBoundary() { return BoundaryDomain{0, 0, {}, {}}; }
+  static BoundaryDomain Planes(int mask) { return BoundaryDomain{mask, 0, {}, {}}; }
   static BoundaryDomain Attributes(const std::set<int>& attributes)
   {
-    return BoundaryDomain{0, detail::attribute_mask(attributes)};
+    return BoundaryDomain{0, detail::attribute_mask(attributes), {}, {}};
   }
+  /// Domain::ofBoundaryElements(mesh, predicate) (fn/domain.hpp:113-118): the
+  /// predicate sees each exterior face's four corner positions and the
+  /// attribute of its boundary element (-1 when the mesh has none there)
+  static BoundaryDomain OfBoundaryElements(Predicate pred) { return BoundaryDomain{0, 0, std::move(pred), {}}; }
+  /// exterior faces given by their four vertex ids each
+  static BoundaryDomain Faces(std::vector<int32_t> quads) { return BoundaryDomain{0, 0, {}, std::move(quads)}; }
 };
+
+namespace detail {
+inline int call_face_predicate(const double* corners, int attr, void* user)
+{
+  std::vector<BoundaryDomain::vec3> x(4);
+  for (int k = 0; k < 4; k++) x[size_t(k)] = {corners[3 * k], corners[3 * k + 1], corners[3 * k + 2]};
+  return (*static_cast<const BoundaryDomain::Predicate*>(user))(x, attr) ? 1 : 0;
+}
+}  // namespace detail
 
 // ---------------------------------------------------------------------------
 // integrands of the compiled registry
@@ -446,7 +466,8 @@ struct qfunction_traits<heat_transfer::ThermalMaterialIntegrand<heat_transfer::L
   }
   static std::vector<double> params(const heat_transfer::ThermalMaterialIntegrand<heat_transfer::LinearIsotropicConductor>& q)
   {
-    return {q.material_.conductivity_, 0.0};
+    // the third entry (rho c_p) is read only with DependsOn<T, dT/dt>
+    return {q.material_.conductivity_, 0.0, q.material_.density_ * q.material_.specific_heat_capacity_};
   }
 };
 template <>
@@ -480,7 +501,8 @@ struct qfunction_traits<solid_mechanics::MaterialStressFunctor<solid_mechanics::
   }
   static std::vector<double> params(const solid_mechanics::MaterialStressFunctor<solid_mechanics::LinearIsotropic>& q)
   {
-    return {q.material_.K - (2.0 / 3.0) * q.material_.G, q.material_.G};
+    // the density is read only with DependsOn<displacement, acceleration>
+    return {q.material_.K - (2.0 / 3.0) * q.material_.G, q.material_.G, q.material_.density};
   }
 };
 template <>
@@ -751,7 +773,15 @@ class Functional<test(trials...)> {
     static_assert(((args >= 0 && args < num_trial_spaces) && ...), "DependsOn index out of range");
     const int                 deps[] = {args...};
     const std::vector<double> p      = bqfunction_traits<Integrand>::params(integrand);
-    if (domain.attr_mask)
+    if (domain.predicate)
+      detail::check(sx_form_add_boundary_integral_pred(
+          h_, bqfunction_traits<Integrand>::id(integrand), p.data(), int(p.size()), deps, int(sizeof...(args)),
+          &detail::call_face_predicate, const_cast<BoundaryDomain::Predicate*>(&domain.predicate)));
+    else if (!domain.faces.empty())
+      detail::check(sx_form_add_boundary_integral_faces(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
+                                                        int(p.size()), deps, int(sizeof...(args)),
+                                                        domain.faces.data(), int64_t(domain.faces.size() / 4)));
+    else if (domain.attr_mask)
       detail::check(sx_form_add_boundary_integral_attr(h_, bqfunction_traits<Integrand>::id(integrand), p.data(),
                                                        int(p.size()), deps, int(sizeof...(args)), domain.attr_mask));
     else
@@ -1058,6 +1088,7 @@ struct NonlinearSolverOptions {
   double          relative_tol   = 1.0e-8;
   double          absolute_tol   = 1.0e-12;
   int             max_iterations = 20;
+  int             max_line_search_iterations = 0;  // num/solver_config.hpp:389-391
 };
 
 class EquationSolver {
@@ -1072,6 +1103,7 @@ class EquationSolver {
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINEAR_SOLVER,
                                      lin_.linear_solver == LinearSolver::GMRES ? SX_LINSOLVER_GMRES : SX_LINSOLVER_CG));
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_GMRES_RESTART, lin_.gmres_restart));
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINE_SEARCH, nl_.max_line_search_iterations));
     const bool amg = lin_.preconditioner == Preconditioner::HypreAMG;
     if (amg) {
       if (!mg_ || mg_space_ != residual.test_space()) {

@@ -115,6 +115,8 @@ def _load():
         "orc_form_free": (None, [_vp]),
         "orc_form_add_integral": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int]),
         "orc_form_eval": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int]),
+        "orc_form_add_integral2": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int, _D]),
+        "orc_form_eval2": (ctypes.c_int, [_vp, _D, _D, _D, ctypes.c_int]),
         "orc_form_jvp": (ctypes.c_int, [_vp, _D, _D]),
         "orc_form_qoi": (ctypes.c_int, [_vp, ctypes.c_int, _D, _I, ctypes.c_int, _D, _D, _D]),
         "orc_form_eval_threads": (ctypes.c_int, [_vp, _D, _D, ctypes.c_int, ctypes.c_int]),
@@ -127,6 +129,9 @@ def _load():
                                   ctypes.c_int, _D]),
         "orc_newton": (ctypes.c_int, [_vp, _I, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
                                       ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _D]),
+        "orc_newton_ls": (ctypes.c_int, [_vp, _I, ctypes.c_int, _D, ctypes.c_double, ctypes.c_double,
+                                         ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                         _D]),
         "orc_time_jvp": (ctypes.c_double, [_vp, _D, _D, ctypes.c_int]),
         "orc_time_jvp_threads": (ctypes.c_double, [_vp, _D, _D, ctypes.c_int, ctypes.c_int]),
     }
@@ -321,6 +326,30 @@ class Form:
         _check(lib().orc_form_add_integral(self.h, kind, _dp(_params(params)), _ip(el), len(el)))
         self.nintegrals += 1
 
+    def add_domain_integral2(self, kind, params, elems=None, state_init=None):
+        """AddDomainIntegral(DependsOn<0, 1>): the integrand also reads the second argument's value (the
+        acceleration of a solid, the rate of a temperature); evaluate with residual2 / linearize2."""
+        el = np.ascontiguousarray(elems if elems is not None else [], dtype=np.int32)
+        st = np.ascontiguousarray(state_init, dtype=np.float64) if state_init is not None else None
+        _check(lib().orc_form_add_integral2(self.h, kind, _dp(_params(params)), _ip(el), len(el),
+                                            _dp(st) if st is not None else None))
+        self.nintegrals += 1
+
+    def residual2(self, U, U1):
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        U1 = np.ascontiguousarray(U1, dtype=np.float64)
+        R = np.zeros(self.space.ndof)
+        _check(lib().orc_form_eval2(self.h, _dp(U), _dp(U1), _dp(R), 0))
+        return R
+
+    def linearize2(self, U, U1):
+        """residual, with the derivative in the first argument stored for jvp"""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        U1 = np.ascontiguousarray(U1, dtype=np.float64)
+        R = np.zeros(self.space.ndof)
+        _check(lib().orc_form_eval2(self.h, _dp(U), _dp(U1), _dp(R), 1))
+        return R
+
     def add_domain_integral_state(self, kind, params, state_init, elems=None):
         """AddDomainIntegral with QuadratureData starting at state_init at every point."""
         el = np.ascontiguousarray(elems if elems is not None else [], dtype=np.int32)
@@ -434,14 +463,16 @@ class Form:
         _check(lib().orc_cg(self.h, _ip(ess), len(ess), _dp(b), _dp(x), rel, abs_tol, maxit, _dp(rep)))
         return x, dict(iterations=int(rep[0]), initial_norm=rep[1], final_norm=rep[2], converged=bool(rep[3]))
 
-    def newton(self, ess, x0, rel=1e-8, abs_tol=1e-12, maxit=20, lin_rel=1e-8, lin_abs=1e-12, lin_max=5000):
+    def newton(self, ess, x0, rel=1e-8, abs_tol=1e-12, maxit=20, lin_rel=1e-8, lin_abs=1e-12, lin_max=5000,
+               max_line_search=0):
+        """Newton + Jacobi-CG; max_line_search > 0 adds the two-way backtracking line search."""
         ess = np.ascontiguousarray(ess, dtype=np.int32)
         x = np.array(x0, dtype=np.float64)
-        rep = np.zeros(5)
-        _check(lib().orc_newton(self.h, _ip(ess), len(ess), _dp(x), rel, abs_tol, maxit, lin_rel, lin_abs,
-                                lin_max, _dp(rep)))
+        rep = np.zeros(6)
+        _check(lib().orc_newton_ls(self.h, _ip(ess), len(ess), _dp(x), rel, abs_tol, maxit, lin_rel, lin_abs,
+                                   lin_max, max_line_search, _dp(rep)))
         return x, dict(newton_iterations=int(rep[0]), linear_iterations=int(rep[1]), initial_norm=rep[2],
-                       final_norm=rep[3], converged=bool(rep[4]))
+                       final_norm=rep[3], converged=bool(rep[4]), line_search_steps=int(rep[5]))
 
     def time_jvp(self, dU, reps):
         dU = np.ascontiguousarray(dU, dtype=np.float64)

@@ -218,6 +218,35 @@ int orc_form_add_integral_state(void* f, int kind, const double* params, const i
     static_cast<OForm*>(f)->form.add_integral(P, el, state_init);
   });
 }
+// AddDomainIntegral of two trial arguments (DependsOn<0, 1>): the integrand
+// also reads the second argument's value (qf_eval2); state_init as above, or
+// null for qfunctions without state
+int orc_form_add_integral2(void* f, int kind, const double* params, const int* elems, int nelem,
+                           const double* state_init)
+{
+  return guard([&] {
+    QFParams P;
+    P.kind = kind;
+    for (int i = 0; i < 16; i++) P.p[i] = params[i];
+    if (kind != QF_NEOHOOKEAN && kind != QF_J2_LINEAR && kind != QF_LINEAR_ELASTIC && kind != QF_HEAT)
+      throw std::runtime_error("this qfunction has no two-argument form");
+    std::vector<int> el;
+    if (elems && nelem > 0) el.assign(elems, elems + nelem);
+    auto& form = static_cast<OForm*>(f)->form;
+    form.add_integral(P, el, state_init);
+    form.integrals.back().two = true;
+  });
+}
+// residual / linearization in the first argument, with U1 the second argument
+int orc_form_eval2(void* f, const double* U, const double* U1, double* R, int linearize)
+{
+  return guard([&] {
+    auto& form = static_cast<OForm*>(f)->form;
+    form.U1    = U1;
+    form.eval(U, R, linearize != 0);
+    form.U1 = nullptr;
+  });
+}
 // AddBoundaryIntegral on the bounding-box planes of face_mask (0: whole boundary)
 int orc_form_add_boundary_integral(void* f, int kind, const double* params, int face_mask)
 {
@@ -330,6 +359,25 @@ int orc_cg(void* f, const int* ess, int ness, const double* b, double* x, double
 
 // Newton + Jacobi-CG; x in/out holds the essential values. report[5] =
 // {newton its, linear its, initial norm, final norm, converged}
+int orc_newton_ls(void* f, const int* ess, int ness, double* x, double rel, double abs_tol, int maxit,
+                  double lin_rel, double lin_abs, int lin_max, int max_ls, double* report)
+{
+  return guard([&] {
+    auto&               F = static_cast<OForm*>(f)->form;
+    size_t              n = size_t(F.space.ndof());
+    std::vector<char>   is_ess(n, 0);
+    for (int i = 0; i < ness; i++) is_ess[size_t(ess[i])] = 1;
+    std::vector<double> xx(x, x + n);
+    auto                rep = newton(F, is_ess, xx, rel, abs_tol, maxit, lin_rel, lin_abs, lin_max, max_ls);
+    std::memcpy(x, xx.data(), n * sizeof(double));
+    report[0] = rep.newton_iterations;
+    report[1] = rep.total_linear_iterations;
+    report[2] = rep.initial_norm;
+    report[3] = rep.final_norm;
+    report[4] = rep.converged;
+    report[5] = rep.line_search_steps;
+  });
+}
 int orc_newton(void* f, const int* ess, int ness, double* x, double rel, double abs_tol, int maxit, double lin_rel,
                double lin_abs, int lin_max, double* report)
 {

@@ -18,6 +18,9 @@ void*       orc_form(void* m, void* s);
 void        orc_form_free(void* f);
 int         orc_form_add_integral(void* f, int kind, const double* params, const int* elems, int nelem);
 int         orc_form_eval(void* f, const double* U, double* R, int linearize);
+int         orc_form_add_integral2(void* f, int kind, const double* params, const int* elems, int nelem,
+                                   const double* state_init);
+int         orc_form_eval2(void* f, const double* U, const double* U1, double* R, int linearize);
 int         orc_form_jvp(void* f, const double* dU, double* dR);
 int         orc_qf_eval(int kind, const double* params, int nc, const double* value, const double* grad, double* source,
                         double* flux);

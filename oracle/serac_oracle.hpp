@@ -1009,6 +1009,35 @@ inline void qf_eval(const QFParams& P, int nc, const T* value, const T* grad, T*
   throw std::runtime_error("qf_eval: unknown qfunction");
 }
 
+// The integrands of two trial arguments (DependsOn<0, 1>): the solid material
+// with its acceleration, MaterialStressFunctor (phy/solid_mechanics.hpp:834-852:
+// source density * d2u/dt2, the flux of the quasi-static material), and the
+// thermal material with its rate, ThermalMaterialIntegrand
+// (phy/heat_transfer.hpp:380-406: source heat_capacity * du/dt, plus the
+// constant source the heat integrand here carries).  rate_coefficient gives
+// that density or volumetric heat capacity from the parameter list; 0 marks a
+// qfunction without a two-argument form.
+inline double rate_coefficient(const QFParams& P)
+{
+  switch (P.kind) {
+    case QF_NEOHOOKEAN: return P.p[0];      // {density, K, G}
+    case QF_J2_LINEAR: return P.p[5];       // {E, nu, sigma_y, Hi, Hk, density, geom}
+    case QF_LINEAR_ELASTIC: return P.p[2];  // {lambda, mu, density}
+    case QF_HEAT: return P.p[2];            // {k, s, rho c_p}
+  }
+  return 0.0;
+}
+
+// qf_eval with the value of the second argument (rate) at the point
+template <typename T>
+inline void qf_eval2(const QFParams& P, int nc, const T* value, const T* grad, const double* rate, T* source,
+                     T* flux, double* state = nullptr)
+{
+  qf_eval(P, nc, value, grad, source, flux, state);
+  const double c = rate_coefficient(P);
+  for (int i = 0; i < nc; i++) source[i] = source[i] + c * rate[i];
+}
+
 // ---------------------------------------------------------------------------
 // Boundary qfunctions (the integrands AddBoundaryIntegral receives,
 // fn/boundary_integral_kernels.hpp:106-130): reference position X, dX/dxi

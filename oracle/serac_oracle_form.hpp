@@ -55,6 +55,9 @@ struct Integral {
   // quadrature point, [elem-in-domain][q][ssize]; empty when stateless
   int                 ssize = 0;
   std::vector<double> state;
+  // DependsOn<0, 1>: the integrand also reads the value of a second trial
+  // argument (qf_eval2), taken from Form::U1 during evaluation
+  bool                two = false;
 };
 
 // AddBoundaryIntegral (fn/functional.hpp:328-354): faces (element, local face
@@ -79,6 +82,7 @@ struct Form {
   std::vector<Integral> integrals;
   std::vector<BIntegral> bintegrals;
   bool                  update_qdata = false;  // updateQdata(bool), fn/functional.hpp:523
+  const double*         U1 = nullptr;          // the second trial argument of two-argument integrals
 
   Form(const Mesh& m, const Space& s) : mesh(&m), space(s), Q(s.p + 1), basis(s.p + 1, s.p + 1)
   {
@@ -335,11 +339,18 @@ struct Form {
 
   // One element of evaluation_kernel_impl; store_D selects the AD variant.
   template <int N>
-  void element_eval(Integral& I, size_t slot, int e, const double* Ue, double* Re, bool store_D)
+  void element_eval(Integral& I, size_t slot, int e, const double* Ue, double* Re, bool store_D,
+                    const double* U1e = nullptr)
   {
     const int c = nc(), NQ = nq();
     std::vector<double> val(size_t(c) * NQ), grd(size_t(c) * 3 * NQ), src(size_t(c) * NQ), flx(size_t(c) * 3 * NQ);
     interpolate(basis, c, Ue, val.data(), grd.data());
+    // the second argument's value at the points (its gradient is not used)
+    std::vector<double> val1, grd1;
+    if (U1e) {
+      val1.resize(size_t(c) * NQ), grd1.resize(size_t(c) * 3 * NQ);
+      interpolate(basis, c, U1e, val1.data(), grd1.data());
+    }
     for (int q = 0; q < NQ; q++) {
       double J[3][3];
       jac(e, q, J);
@@ -355,7 +366,13 @@ struct Form {
           for (int d = 0; d < 3; d++) g[i * 3 + d] = grd[(size_t(i) * 3 + d) * NQ + q];
         }
         parent_to_physical(c, J, g);
-        qf_eval(I.qf, c, v, g, s, f, stp);
+        if (U1e) {
+          double w[3];
+          for (int i = 0; i < c; i++) w[i] = val1[size_t(i) * NQ + q];
+          qf_eval2(I.qf, c, v, g, w, s, f, stp);
+        } else {
+          qf_eval(I.qf, c, v, g, s, f, stp);
+        }
         physical_to_parent(c, J, s, f);
         for (int i = 0; i < c; i++) {
           src[size_t(i) * NQ + q] = s[i];
@@ -374,7 +391,13 @@ struct Form {
           }
         }
         parent_to_physical(c, J, g);
-        qf_eval(I.qf, c, v, g, s, f, stp);
+        if (U1e) {
+          double w[3];
+          for (int i = 0; i < c; i++) w[i] = val1[size_t(i) * NQ + q];
+          qf_eval2(I.qf, c, v, g, w, s, f, stp);
+        } else {
+          qf_eval(I.qf, c, v, g, s, f, stp);
+        }
         physical_to_parent(c, J, s, f);
         double* D = &I.D[(slot * NQ + q) * I.dsize];
         int     o = 0;
@@ -453,17 +476,20 @@ struct Form {
   {
     const size_t ndof = size_t(space.ndof());
     std::fill(R, R + ndof, 0.0);
-    std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe);
+    std::vector<double> Ue(size_t(nc()) * space.npe), Re(size_t(nc()) * space.npe), U1e(size_t(nc()) * space.npe);
     for (auto& I : integrals) {
       if (store_D) I.D.assign(I.elems.size() * size_t(nq()) * I.dsize, 0.0);
+      if (I.two && !U1) throw std::runtime_error("a two-argument integral needs its second argument (U1)");
       for (size_t k = 0; k < I.elems.size(); k++) {
         int e = I.elems[k];
         gather_elem(e, U, Ue.data());
+        if (I.two) gather_elem(e, U1, U1e.data());
+        const double* u1 = I.two ? U1e.data() : nullptr;
         std::fill(Re.begin(), Re.end(), 0.0);
         if (nc() == 1)
-          element_eval<4>(I, k, e, Ue.data(), Re.data(), store_D);
+          element_eval<4>(I, k, e, Ue.data(), Re.data(), store_D, u1);
         else
-          element_eval<12>(I, k, e, Ue.data(), Re.data(), store_D);
+          element_eval<12>(I, k, e, Ue.data(), Re.data(), store_D, u1);
         scatter_elem(e, Re.data(), R);
       }
     }
@@ -613,8 +639,10 @@ struct Form {
   {
     prepare_threads(nthreads);
     const int nt = bands.nt;
-    for (auto& I : integrals)
+    for (auto& I : integrals) {
+      if (I.two) throw std::runtime_error("eval_threads: two-argument integrals use eval");
       if (store_D) I.D.assign(I.elems.size() * size_t(nq()) * I.dsize, 0.0);
+    }
 #pragma omp parallel num_threads(nt)
     {
 #ifdef _OPENMP
@@ -784,6 +812,7 @@ struct SolveReport {
   int    iterations = 0;
   double initial_norm = 0, final_norm = 0;
   bool   converged = false;
+  int    line_search_steps = 0;
 };
 
 // J_bc v: eliminate columns (v[ess] -> 0), apply J, set rows: out[ess] = v[ess]
@@ -858,11 +887,18 @@ struct NewtonReport {
   int    total_linear_iterations = 0;
   double initial_norm = 0, final_norm = 0;
   bool   converged = false;
+  int    line_search_steps = 0;
 };
 
 // x holds the essential values on entry (and keeps them).
+// max_ls: the two-way backtracking line search of serac's NewtonSolver
+// (num/equation_solver.cpp:21-185, max_line_search_iterations; 0 is plain
+// Newton): halve the step while the residual norm does not fall below the
+// previous one; when max_ls halvings fail, search the opposite direction the
+// same way; when that fails too, take half the last step of the first
+// direction and go on.
 inline NewtonReport newton(Form& f, const std::vector<char>& is_ess, std::vector<double>& x, double rel_tol,
-                           double abs_tol, int max_iter, double lin_rel, double lin_abs, int lin_max)
+                           double abs_tol, int max_iter, double lin_rel, double lin_abs, int lin_max, int max_ls = 0)
 {
   const size_t        n = x.size();
   std::vector<double> r(n), c(n);
@@ -878,6 +914,7 @@ inline NewtonReport newton(Form& f, const std::vector<char>& is_ess, std::vector
   double goal       = std::max(rel_tol * norm, abs_tol);
   int    it         = 0;
   for (;; it++) {
+    if (norm != norm) break;  // an undefined residual: no convergence
     if (norm <= goal) {
       rep.converged = true;
       break;
@@ -888,8 +925,34 @@ inline NewtonReport newton(Form& f, const std::vector<char>& is_ess, std::vector
       if (is_ess[i]) r[i] = 0.0;
     SolveReport lr = pcg_jacobi(f, is_ess, r, c, lin_rel, lin_abs, lin_max);
     rep.total_linear_iterations += lr.iterations;
-    for (size_t i = 0; i < n; i++) x[i] -= c[i];
-    norm = residual_norm();
+    const double              prev = norm;
+    const std::vector<double> x0   = x;
+    auto step = [&](double s) {
+      for (size_t i = 0; i < n; i++) x[i] = x0[i] + s * c[i];
+      return residual_norm();  // a NaN norm never counts as improved
+    };
+    auto improved = [&](double v) { return v < prev; };
+    double scale = 1.0;
+    norm         = step(-scale);
+    int ls       = 0;
+    for (; !improved(norm) && ls < max_ls; ++ls, ++rep.line_search_steps) {
+      scale *= 0.5;
+      norm = step(-scale);
+    }
+    if (max_ls > 0 && ls == max_ls && !improved(norm)) {
+      scale = 1.0;
+      norm  = step(scale);
+      ls    = 0;
+      for (; !improved(norm) && ls < max_ls; ++ls, ++rep.line_search_steps) {
+        scale *= 0.5;
+        norm = step(scale);
+      }
+      if (ls == max_ls && !improved(norm)) {
+        ++rep.line_search_steps;
+        scale *= 0.5;
+        norm = step(-scale);
+      }
+    }
   }
   rep.newton_iterations = it;
   rep.final_norm        = norm;

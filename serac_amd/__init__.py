@@ -84,6 +84,16 @@ class DeviceVector:
             pass
 
 
+def _face_trampoline(predicate):
+    """C callback for sx_face_predicate: predicate(corners [4][3], attr) -> bool (called during the add only)."""
+
+    def trampoline(xyz, attr, _user):
+        corners = np.ctypeslib.as_array(xyz, shape=(12,)).reshape(4, 3).copy()
+        return 1 if predicate(corners, attr) else 0
+
+    return trampoline
+
+
 def attr_mask(attrs):
     """Bit a - 1 for each attribute a in 1..64."""
     m = 0
@@ -594,12 +604,20 @@ class QoI:
             capi.call("sx_qoi_add_domain_integral_dep", self.h, qoi, _dptr(p), len(params), dep, len(depends_on),
                       el, ne)
 
-    def add_boundary_integral(self, qoi, params=(), face_mask=0, attrs=None):
-        """Boundary QoI over the planes of face_mask (0: the whole boundary) or the attributes `attrs`."""
+    def add_boundary_integral(self, qoi, params=(), face_mask=0, attrs=None, faces=None, predicate=None):
+        """Boundary QoI over the planes of face_mask (0: the whole boundary), the attributes `attrs`, the
+        exterior faces `faces` ([n][4] vertex ids) or those predicate(corners [4][3], attr) selects."""
         p = np.zeros(16)
         p[: len(params)] = params
-        capi.call("sx_qoi_add_boundary_integral", self.h, qoi, _dptr(p), len(params), face_mask,
-                  attr_mask(attrs) if attrs is not None else 0)
+        if predicate is not None:
+            cb = capi.FACE_PREDICATE(_face_trampoline(predicate))
+            capi.call("sx_qoi_add_boundary_integral_pred", self.h, qoi, _dptr(p), len(params), cb, None)
+        elif faces is not None:
+            fv = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 4)
+            capi.call("sx_qoi_add_boundary_integral_faces", self.h, qoi, _dptr(p), len(params), _iptr(fv), len(fv))
+        else:
+            capi.call("sx_qoi_add_boundary_integral", self.h, qoi, _dptr(p), len(params), face_mask,
+                      attr_mask(attrs) if attrs is not None else 0)
 
     def __call__(self, t, *U):
         v = ctypes.c_double()
@@ -683,14 +701,26 @@ class Functional:
         if qdata is not None:
             self._qdata = getattr(self, "_qdata", []) + [qdata]  # the form borrows it
 
-    def add_boundary_integral(self, qfunc, params, depends_on=(0,), face_mask=0, attrs=None):
+    def add_boundary_integral(self, qfunc, params, depends_on=(0,), face_mask=0, attrs=None, faces=None,
+                              predicate=None):
         """AddBoundaryIntegral over the bounding-box planes of face_mask
         (bit 0 x = min, 1 x = max, 2 y = min, ...; 0: the whole boundary), or,
-        with `attrs`, over the boundary elements with those attributes."""
+        with `attrs`, over the boundary elements with those attributes, or over
+        `faces` ([n][4] vertex ids of exterior faces), or over the exterior
+        faces for which predicate(corners [4][3], attr) is true
+        (Domain::ofBoundaryElements)."""
         p = np.zeros(16)
         p[: len(params)] = params
         dep = (ctypes.c_int * len(depends_on))(*depends_on)
-        if attrs is not None:
+        if predicate is not None:
+            cb = capi.FACE_PREDICATE(_face_trampoline(predicate))
+            capi.call("sx_form_add_boundary_integral_pred", self.h, qfunc, _dptr(p), len(params), dep,
+                      len(depends_on), cb, None)
+        elif faces is not None:
+            fv = np.ascontiguousarray(faces, dtype=np.int32).reshape(-1, 4)
+            capi.call("sx_form_add_boundary_integral_faces", self.h, qfunc, _dptr(p), len(params), dep,
+                      len(depends_on), _iptr(fv), len(fv))
+        elif attrs is not None:
             capi.call("sx_form_add_boundary_integral_attr", self.h, qfunc, _dptr(p), len(params), dep,
                       len(depends_on), attr_mask(attrs))
         else:

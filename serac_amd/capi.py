@@ -27,6 +27,7 @@ SX_OPT_JVP_ARITHMETIC = 4
 SX_OPT_LINEAR_SOLVER = 5
 SX_OPT_GMRES_RESTART = 6
 SX_OPT_SCATTER_ORDER = 7
+SX_OPT_LINE_SEARCH = 8
 SX_LINSOLVER_CG = 0
 SX_LINSOLVER_GMRES = 1
 SX_PREC_F64 = 0
@@ -57,6 +58,8 @@ SX_UNET_IDENTITY = 1
 _vp = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I32 = ctypes.POINTER(ctypes.c_int32)
+# int (*sx_face_predicate)(const double* corners, int attr, void* user)
+FACE_PREDICATE = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_int, ctypes.c_void_p)
 _I64 = ctypes.POINTER(ctypes.c_int64)
 _int = ctypes.c_int
 
@@ -68,6 +71,7 @@ class SolveReport(ctypes.Structure):
         ("initial_norm", ctypes.c_double),
         ("final_norm", ctypes.c_double),
         ("converged", ctypes.c_int),
+        ("line_search_steps", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -107,6 +111,10 @@ SIGNATURES = {
     "sx_mesh_boundary": (_int, [_vp, _I32, _I32]),
     "sx_space_boundary_dofs_attr": (_int, [_vp, ctypes.c_uint64, _int, _I32, ctypes.c_int64, _I64]),
     "sx_form_add_boundary_integral_attr": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, ctypes.c_uint64]),
+    "sx_form_add_boundary_integral_pred": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, FACE_PREDICATE,
+                                                  _vp]),
+    "sx_form_add_boundary_integral_faces": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int, _I32,
+                                                   ctypes.c_int64]),
     "sx_mesh_info": (_int, [_vp, _I64, _I64]),
     "sx_mesh_vertices": (_int, [_vp, _D]),
     "sx_mesh_destroy": (_int, [_vp]),
@@ -171,6 +179,8 @@ SIGNATURES = {
     "sx_qoi_gradient_multi": (_int, [_vp, ctypes.c_double, ctypes.POINTER(_vp), _int, _vp]),
     "sx_qoi_add_domain_integral": (_int, [_vp, _int, _D, _int, ctypes.POINTER(_int), _int]),
     "sx_qoi_add_boundary_integral": (_int, [_vp, _int, _D, _int, _int, ctypes.c_uint64]),
+    "sx_qoi_add_boundary_integral_pred": (_int, [_vp, _int, _D, _int, FACE_PREDICATE, _vp]),
+    "sx_qoi_add_boundary_integral_faces": (_int, [_vp, _int, _D, _int, _I32, ctypes.c_int64]),
     "sx_qoi_eval": (_int, [_vp, ctypes.c_double, _vp, _D]),
     "sx_qoi_gradient": (_int, [_vp, ctypes.c_double, _vp, _vp]),
     "sx_qoi_destroy": (_int, [_vp]),

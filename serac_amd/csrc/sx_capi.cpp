@@ -239,6 +239,10 @@ struct Integral {
   DeviceBuffer         d_qrow;           // [slot] -> element position in the domain
   std::vector<int>     slot_pos;         // [slot] -> element position in the domain (host)
   DeviceBuffer         d_emap_e;         // E-vector entry points: [slot][npe] -> pos * npe + j
+  // a two-argument integral (DependsOn<a, b>) is held as the integrand in its
+  // first argument plus, as a companion, the mass integral of its second
+  bool                 companion     = false;  // the mass part of a two-argument integral
+  bool                 has_companion = false;  // the integrand part of one
 };
 
 // AddBoundaryIntegral (fn/functional.hpp:328-354): faces of the selected
@@ -340,6 +344,7 @@ struct sx_form {
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v
   int                                    linsolver    = SX_LINSOLVER_CG;  // linear solver of the Newton loops
   int                                    gmres_restart = 50;
+  int                                    line_search   = 0;  // max_line_search_iterations (0: plain Newton)
   sx_learned*                            learned      = nullptr;        // borrowed
   sx_multigrid*                          mg           = nullptr;        // borrowed: multigrid-PCG in the Newton loops
   const double*                          shape        = nullptr;        // borrowed: shape displacement (device)
@@ -619,6 +624,78 @@ std::vector<BoundaryFace> boundary_faces(const sxh::HexMesh& m, int face_mask)
       take = on;
     }
     if (take) out.push_back({keys[i].e, keys[i].lf});
+  }
+  return out;
+}
+
+// The exterior faces (faces of exactly one element) in (element, local face)
+// order, each with the attribute of the boundary element on it (-1 when the
+// mesh has none there).
+struct ExteriorFace {
+  FaceKey key;
+  int     attr;
+};
+
+std::vector<ExteriorFace> exterior_faces(const sxh::HexMesh& m)
+{
+  std::vector<FaceKey> keys = element_face_keys(m);
+  std::vector<size_t>  order(keys.size());
+  for (size_t i = 0; i < order.size(); i++) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return keys[a].v < keys[b].v; });
+  std::vector<char> single(keys.size(), 0);
+  for (size_t i = 0; i < order.size();) {
+    size_t j = i;
+    while (j < order.size() && keys[order[j]].v == keys[order[i]].v) j++;
+    if (j - i == 1) single[order[i]] = 1;
+    i = j;
+  }
+  std::map<std::array<int, 4>, int> battr;
+  for (size_t q = 0; q < m.bdr_attr.size(); q++) {
+    std::array<int, 4> v;
+    for (int c = 0; c < 4; c++) v[size_t(c)] = m.bdr[4 * q + size_t(c)];
+    std::sort(v.begin(), v.end());
+    battr.emplace(v, m.bdr_attr[q]);
+  }
+  std::vector<ExteriorFace> out;
+  for (size_t i = 0; i < keys.size(); i++) {
+    if (!single[i]) continue;
+    const auto it = battr.find(keys[i].v);
+    out.push_back({keys[i], it == battr.end() ? -1 : it->second});
+  }
+  return out;
+}
+
+// Domain::ofBoundaryElements(mesh, predicate): the exterior faces the
+// predicate selects, given their corner coordinates and attribute
+std::vector<BoundaryFace> predicate_faces(const sxh::HexMesh& m, sx_face_predicate pred, void* user)
+{
+  std::vector<BoundaryFace> out;
+  for (const ExteriorFace& f : exterior_faces(m)) {
+    double xyz[12];
+    for (int k = 0; k < 4; k++) {
+      int i0, i1, i2;
+      face_node(f.key.lf, k & 1, k >> 1, 2, i0, i1, i2);
+      const int v = m.corner(f.key.e, i0 + 2 * i1 + 4 * i2);
+      for (int d = 0; d < 3; d++) xyz[3 * k + d] = m.X[size_t(3 * v + d)];
+    }
+    if (pred(xyz, f.attr, user)) out.push_back({f.key.e, f.key.lf});
+  }
+  return out;
+}
+
+// the exterior faces with the given vertex quadruples, in list order
+std::vector<BoundaryFace> listed_faces(const sxh::HexMesh& m, const int32_t* fv, int64_t nf)
+{
+  std::map<std::array<int, 4>, BoundaryFace> ext;
+  for (const ExteriorFace& f : exterior_faces(m)) ext.emplace(f.key.v, BoundaryFace{f.key.e, f.key.lf});
+  std::vector<BoundaryFace> out;
+  for (int64_t i = 0; i < nf; i++) {
+    std::array<int, 4> v;
+    for (int c = 0; c < 4; c++) v[size_t(c)] = fv[4 * i + c];
+    std::sort(v.begin(), v.end());
+    const auto it = ext.find(v);
+    SX_REQUIRE(it != ext.end(), SX_ERR_ARG, "face " + std::to_string(i) + " is not an exterior face of the mesh");
+    out.push_back(it->second);
   }
   return out;
 }
@@ -1198,90 +1275,145 @@ int sx_form_add_domain_integral(sx_form* f, int qfunc_id, const double* params, 
   return sx_form_add_domain_integral_qdata(f, qfunc_id, params, nparams, depends_on, ndep, elems, nelem, nullptr);
 }
 
+namespace {
+
+// One domain integral of qfunction qfunc_id in trial argument `trial`
+// (companion: the mass part of a two-argument integral).
+void add_domain_integral_one(sx_form* f, int qfunc_id, const double* params, int nparams, int trial, const int* elems,
+                             int nelem, sx_qdata* qdata, bool companion)
+{
+  SX_REQUIRE(f, SX_ERR_ARG, "null form");
+  SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+             "qfunction parameters: at most 16 doubles");
+  const int nc_qf = qf_components(qfunc_id);
+  SX_REQUIRE(nc_qf != 0, SX_ERR_UNSUPPORTED, "unknown qfunction id " + std::to_string(qfunc_id));
+  SX_REQUIRE(trial >= 0 && trial < int(f->trials.size()), SX_ERR_ARG, "DependsOn index out of range");
+  const sx_space* ts = f->trials[size_t(trial)];
+  const sx_space* vs = f->test;
+  SX_REQUIRE(ts->mesh == vs->mesh, SX_ERR_UNSUPPORTED, "test and trial spaces must share a mesh");
+  SX_REQUIRE(ts->order == vs->order && ts->nc == vs->nc && ts->ordering == vs->ordering, SX_ERR_UNSUPPORTED,
+             "test and trial spaces must have the same order, components and ordering");
+  SX_REQUIRE(ts == vs || (ts->native && vs->native) || ts->emap == vs->emap, SX_ERR_UNSUPPORTED,
+             "test and trial spaces must number their nodes alike");
+  SX_REQUIRE(nc_qf < 0 || nc_qf == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
+  const int ssize = qf_state_size(qfunc_id);
+  SX_REQUIRE(ssize == 0 || qdata, SX_ERR_ARG, "this qfunction needs quadrature-point state (sx_qdata)");
+  SX_REQUIRE(ssize > 0 || !qdata, SX_ERR_ARG, "this qfunction carries no quadrature-point state");
+  const int nq = (vs->order + 1) * (vs->order + 1) * (vs->order + 1);
+
+  auto I = std::make_unique<Integral>();
+  I->qf  = qfunc_id;
+  for (int k = 0; k < nparams; k++) I->params[k] = params[k];
+  I->depends_on = {trial};
+  I->layout     = qf_layout(qfunc_id);
+
+  const auto& m = vs->mesh->m;
+  std::vector<int> domain;
+  if (elems && nelem > 0) {
+    domain.assign(elems, elems + nelem);
+    for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "domain element out of range");
+  } else {
+    domain.resize(size_t(m.ne));
+    for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
+  }
+  if (qdata) {
+    SX_REQUIRE(qdata->nelem == int64_t(domain.size()) && qdata->nq == nq && qdata->size == ssize, SX_ERR_ARG,
+               "quadrature data shape (" + std::to_string(qdata->nelem) + " x " + std::to_string(qdata->nq) +
+                   " x " + std::to_string(qdata->size) + ") does not match the domain (" +
+                   std::to_string(domain.size()) + " x " + std::to_string(nq) + " x " + std::to_string(ssize) +
+                   ")");
+  }
+  // slots: positions in the domain sorted by element colour (stable); with
+  // atomic scatter, one range in domain order
+  std::vector<int> pos;
+  pos.reserve(domain.size());
+  if (f->scatter == SX_SCATTER_ATOMIC) {
+    for (size_t i = 0; i < domain.size(); i++) pos.push_back(int(i));
+    I->color_off = {0, int64_t(pos.size())};
+  } else {
+    I->color_off.assign(size_t(vs->ncolors) + 1, 0);
+    for (int c = 0; c < vs->ncolors; c++) {
+      for (size_t i = 0; i < domain.size(); i++)
+        if (vs->color[size_t(domain[i])] == c) pos.push_back(int(i));
+      I->color_off[size_t(c) + 1] = int64_t(pos.size());
+    }
+  }
+  std::vector<int> slots(pos.size());
+  for (size_t k = 0; k < pos.size(); k++) slots[k] = domain[size_t(pos[k])];
+  I->nslots   = int64_t(slots.size());
+  I->slot_pos = pos;
+  if (qdata) {
+    I->qdata = qdata;
+    I->d_qrow.upload(pos);
+  }
+  std::vector<int>    map(size_t(I->nslots) * vs->npe);
+  std::vector<double> xyz(size_t(I->nslots) * 24);
+  for (int64_t k = 0; k < I->nslots; k++) {
+    const int e = slots[size_t(k)];
+    std::memcpy(&map[size_t(k) * vs->npe], &vs->emap[size_t(e) * vs->npe], sizeof(int) * vs->npe);
+    for (int c = 0; c < 8; c++) {
+      const int v = m.corner(e, c);
+      for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + c * 3 + d] = m.X[3 * size_t(v) + d];
+    }
+  }
+  I->d_map.upload(map);
+  I->d_coords.upload(xyz);
+  I->companion = companion;
+  f->integrals.push_back(std::move(I));
+}
+
+// The coefficient of the second argument's value in the source of a
+// two-argument integrand: MaterialStressFunctor's density * acceleration
+// (phy/solid_mechanics.hpp:834-852) and ThermalMaterialIntegrand's
+// heat_capacity * rate (phy/heat_transfer.hpp:380-406).  Which parameter holds
+// it depends on the qfunction; 0 for qfunctions without a two-argument form.
+bool rate_coefficient(int qf, const double* params, int nparams, double* coef)
+{
+  int k = -1;
+  switch (qf) {
+    case SX_QF_NEOHOOKEAN: k = 0; break;      // {density, K, G}
+    case SX_QF_J2_LINEAR: k = 5; break;       // {E, nu, sigma_y, Hi, Hk, density, geom}
+    case SX_QF_LINEAR_ELASTIC: k = 2; break;  // {lambda, mu, density}
+    case SX_QF_HEAT: k = 2; break;            // {k, s, rho c_p}
+    default: return false;
+  }
+  *coef = k < nparams ? params[k] : 0.0;
+  return true;
+}
+
+}  // namespace
+
 int sx_form_add_domain_integral_qdata(sx_form* f, int qfunc_id, const double* params, int nparams,
                                       const int* depends_on, int ndep, const int* elems, int nelem, sx_qdata* qdata)
 {
   return api([&] {
     SX_REQUIRE(f, SX_ERR_ARG, "null form");
+    SX_REQUIRE(depends_on && (ndep == 1 || ndep == 2), SX_ERR_UNSUPPORTED,
+               "compiled qfunctions take one trial argument (DependsOn<i>), or two for the integrands with a rate");
     SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
                "qfunction parameters: at most 16 doubles");
-    const int nc_qf = qf_components(qfunc_id);
-    SX_REQUIRE(nc_qf != 0, SX_ERR_UNSUPPORTED, "unknown qfunction id " + std::to_string(qfunc_id));
-    SX_REQUIRE(ndep == 1 && depends_on, SX_ERR_UNSUPPORTED,
-               "compiled qfunctions take exactly one trial argument (DependsOn<i>)");
-    const int trial = depends_on[0];
-    SX_REQUIRE(trial >= 0 && trial < int(f->trials.size()), SX_ERR_ARG, "DependsOn index out of range");
-    const sx_space* ts = f->trials[size_t(trial)];
-    const sx_space* vs = f->test;
-    SX_REQUIRE(ts->mesh == vs->mesh, SX_ERR_UNSUPPORTED, "test and trial spaces must share a mesh");
-    SX_REQUIRE(ts->order == vs->order && ts->nc == vs->nc && ts->ordering == vs->ordering, SX_ERR_UNSUPPORTED,
-               "test and trial spaces must have the same order, components and ordering");
-    SX_REQUIRE(ts == vs || (ts->native && vs->native) || ts->emap == vs->emap, SX_ERR_UNSUPPORTED,
-               "test and trial spaces must number their nodes alike");
-    SX_REQUIRE(nc_qf < 0 || nc_qf == vs->nc, SX_ERR_ARG, "qfunction does not match the number of components");
-    const int ssize = qf_state_size(qfunc_id);
-    SX_REQUIRE(ssize == 0 || qdata, SX_ERR_ARG, "this qfunction needs quadrature-point state (sx_qdata)");
-    SX_REQUIRE(ssize > 0 || !qdata, SX_ERR_ARG, "this qfunction carries no quadrature-point state");
-    const int nq = (vs->order + 1) * (vs->order + 1) * (vs->order + 1);
-
-    auto I = std::make_unique<Integral>();
-    I->qf  = qfunc_id;
-    for (int k = 0; k < nparams; k++) I->params[k] = params[k];
-    I->depends_on = {trial};
-    I->layout     = qf_layout(qfunc_id);
-
-    const auto& m = vs->mesh->m;
-    std::vector<int> domain;
-    if (elems && nelem > 0) {
-      domain.assign(elems, elems + nelem);
-      for (int e : domain) SX_REQUIRE(e >= 0 && e < m.ne, SX_ERR_ARG, "domain element out of range");
-    } else {
-      domain.resize(size_t(m.ne));
-      for (int64_t e = 0; e < m.ne; e++) domain[size_t(e)] = int(e);
+    double coef = 0.0;
+    if (ndep == 2) {
+      SX_REQUIRE(rate_coefficient(qfunc_id, params, nparams, &coef), SX_ERR_UNSUPPORTED,
+                 "qfunction " + std::to_string(qfunc_id) + " has no two-argument form (DependsOn<a, b>)");
+      SX_REQUIRE(depends_on[1] >= 0 && depends_on[1] < int(f->trials.size()), SX_ERR_ARG,
+                 "DependsOn index out of range");
+      SX_REQUIRE(depends_on[0] != depends_on[1], SX_ERR_ARG, "DependsOn<a, b> needs two different arguments");
     }
-    if (qdata) {
-      SX_REQUIRE(qdata->nelem == int64_t(domain.size()) && qdata->nq == nq && qdata->size == ssize, SX_ERR_ARG,
-                 "quadrature data shape (" + std::to_string(qdata->nelem) + " x " + std::to_string(qdata->nq) +
-                     " x " + std::to_string(qdata->size) + ") does not match the domain (" +
-                     std::to_string(domain.size()) + " x " + std::to_string(nq) + " x " + std::to_string(ssize) +
-                     ")");
-    }
-    // slots: positions in the domain sorted by element colour (stable); with
-    // atomic scatter, one range in domain order
-    std::vector<int> pos;
-    pos.reserve(domain.size());
-    if (f->scatter == SX_SCATTER_ATOMIC) {
-      for (size_t i = 0; i < domain.size(); i++) pos.push_back(int(i));
-      I->color_off = {0, int64_t(pos.size())};
-    } else {
-      I->color_off.assign(size_t(vs->ncolors) + 1, 0);
-      for (int c = 0; c < vs->ncolors; c++) {
-        for (size_t i = 0; i < domain.size(); i++)
-          if (vs->color[size_t(domain[i])] == c) pos.push_back(int(i));
-        I->color_off[size_t(c) + 1] = int64_t(pos.size());
+    const size_t first = f->integrals.size();
+    add_domain_integral_one(f, qfunc_id, params, nparams, depends_on[0], elems, nelem, qdata, false);
+    if (ndep == 2) {
+      // the second argument enters the source linearly, as coef * value: its
+      // part is the mass integral {coef u, 0} in that argument
+      const double mp[2] = {coef, 0.0};
+      try {
+        add_domain_integral_one(f, SX_QF_MASS_DIFFUSION, mp, 2, depends_on[1], elems, nelem, nullptr, true);
+      } catch (...) {
+        f->integrals.resize(first);
+        throw;
       }
+      f->integrals[first]->has_companion = true;
     }
-    std::vector<int> slots(pos.size());
-    for (size_t k = 0; k < pos.size(); k++) slots[k] = domain[size_t(pos[k])];
-    I->nslots   = int64_t(slots.size());
-    I->slot_pos = pos;
-    if (qdata) {
-      I->qdata = qdata;
-      I->d_qrow.upload(pos);
-    }
-    std::vector<int>    map(size_t(I->nslots) * vs->npe);
-    std::vector<double> xyz(size_t(I->nslots) * 24);
-    for (int64_t k = 0; k < I->nslots; k++) {
-      const int e = slots[size_t(k)];
-      std::memcpy(&map[size_t(k) * vs->npe], &vs->emap[size_t(e) * vs->npe], sizeof(int) * vs->npe);
-      for (int c = 0; c < 8; c++) {
-        const int v = m.corner(e, c);
-        for (int d = 0; d < 3; d++) xyz[size_t(k) * 24 + c * 3 + d] = m.X[3 * size_t(v) + d];
-      }
-    }
-    I->d_map.upload(map);
-    I->d_coords.upload(xyz);
-    f->integrals.push_back(std::move(I));
   });
 }
 
@@ -1384,6 +1516,30 @@ int sx_form_add_boundary_integral_attr(sx_form* f, int qfunc_id, const double* p
   return add_boundary_integral(f, qfunc_id, params, nparams, depends_on, ndep, [attr_mask](const sxh::HexMesh& m) {
     return attribute_faces(m, attr_mask, true);
   });
+}
+
+int sx_form_add_boundary_integral_pred(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                       const int* depends_on, int ndep, sx_face_predicate pred, void* user)
+{
+  if (!pred) {
+    g_last_error = "null face predicate";
+    return SX_ERR_ARG;
+  }
+  return add_boundary_integral(f, qfunc_id, params, nparams, depends_on, ndep,
+                               [pred, user](const sxh::HexMesh& m) { return predicate_faces(m, pred, user); });
+}
+
+int sx_form_add_boundary_integral_faces(sx_form* f, int qfunc_id, const double* params, int nparams,
+                                        const int* depends_on, int ndep, const int32_t* face_vertices, int64_t nfaces)
+{
+  if (nfaces < 0 || (nfaces > 0 && !face_vertices)) {
+    g_last_error = "null face list";
+    return SX_ERR_ARG;
+  }
+  return add_boundary_integral(f, qfunc_id, params, nparams, depends_on, ndep,
+                               [face_vertices, nfaces](const sxh::HexMesh& m) {
+                                 return listed_faces(m, face_vertices, nfaces);
+                               });
 }
 
 int sx_form_update_qdata(sx_form* f, int flag)
@@ -1560,6 +1716,10 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
         SX_REQUIRE(value >= 0 && value <= 2, SX_ERR_ARG, "scatter order is 0, 1 or 2");
         f->scatter_order = int(value);
         return;
+      case SX_OPT_LINE_SEARCH:
+        SX_REQUIRE(value >= 0 && value <= 64, SX_ERR_ARG, "line search iterations must be in 0..64");
+        f->line_search = int(value);
+        return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
   });
@@ -1605,12 +1765,25 @@ namespace {
 // every (element, node) its own entry, so there is no gather sharing and no
 // scatter sum.  The kernels address an entry as component-major
 // (c * ne * npe + pos * npe + j); the reference layout is transposed in and out.
+// the k-th domain integral as the caller added them (a two-argument
+// integral's companion is not counted)
+Integral* user_integral(const sx_form* f, int k)
+{
+  if (!f || k < 0) return nullptr;
+  for (const auto& I : f->integrals)
+    if (!I->companion && k-- == 0) return I.get();
+  return nullptr;
+}
+
 Integral& e_integral(sx_form* f, int k)
 {
   SX_REQUIRE(f, SX_ERR_ARG, "null form");
-  SX_REQUIRE(k >= 0 && k < int(f->integrals.size()), SX_ERR_ARG, "domain integral index out of range");
+  Integral* Ip = user_integral(f, k);
+  SX_REQUIRE(Ip, SX_ERR_ARG, "domain integral index out of range");
   SX_REQUIRE(!f->shape, SX_ERR_UNSUPPORTED, "E-vector evaluation of a shape-aware form is not built");
-  Integral& I = *f->integrals[size_t(k)];
+  SX_REQUIRE(!Ip->has_companion, SX_ERR_UNSUPPORTED,
+             "E-vector evaluation of a two-argument integral (DependsOn<a, b>) is not built");
+  Integral& I = *Ip;
   if (!I.d_emap_e.ptr) {
     const int        npe = f->test->npe;
     std::vector<int> m(size_t(I.nslots) * size_t(npe));
@@ -1663,8 +1836,9 @@ extern "C" {
 int sx_form_integral_info(const sx_form* f, int k, int64_t* nelem, int* nodes_per_elem, int* ncomp)
 {
   return api([&] {
-    SX_REQUIRE(f && k >= 0 && k < int(f->integrals.size()), SX_ERR_ARG, "domain integral index out of range");
-    if (nelem) *nelem = f->integrals[size_t(k)]->nslots;
+    const Integral* I = user_integral(f, k);
+    SX_REQUIRE(I, SX_ERR_ARG, "domain integral index out of range");
+    if (nelem) *nelem = I->nslots;
     if (nodes_per_elem) *nodes_per_elem = f->test->npe;
     if (ncomp) *ncomp = f->test->nc;
   });
@@ -2870,7 +3044,7 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
   const int hn = 2 * (m + 1) + 1;
   w.hb.allocate(size_t(std::max(hn, m + 1)) * 8);
   w.gpart.allocate(size_t(m + 1) * size_t(sxk::multi_dot_blocks()) * 8);
-  double *r = w.r.as<double>(), *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
+  double *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
   double *Z = w.Zb.as<double>(), *hb = w.hb.as<double>(), *part = w.gpart.as<double>();
   auto    vec  = [&](int j) { return V + int64_t(j) * n; };
   auto    zvec = [&](int j) { return Z + int64_t(j) * n; };
@@ -3094,17 +3268,23 @@ extern "C" int sx_gmres_solve(sx_grad* g, const int32_t* ess_dofs, int64_t ness,
 
 namespace {
 
-// Newton (num/equation_solver.cpp:81-235, no line search): norm goal
-// max(rel ||r0||, abs); Jacobi-CG on J_bc c = r, x <- x - c.  Every
-// evaluation is a linearization, whose element kernel writes the residual and
-// the tangent in one pass: the residual that decides convergence is the
-// right-hand side of the next linear solve, so each Newton iteration runs one
-// element pass (the converged check pays the tangent once per call).
+// Newton (num/equation_solver.cpp:81-235): norm goal max(rel ||r0||, abs);
+// Jacobi-CG on J_bc c = r, x <- x - c.  Every evaluation is a linearization,
+// whose element kernel writes the residual and the tangent in one pass: the
+// residual that decides convergence is the right-hand side of the next linear
+// solve, so each Newton iteration runs one element pass (the converged check
+// pays the tangent once per call).  With f->line_search > 0, the two-way
+// backtracking of serac's NewtonSolver (num/equation_solver.cpp:138-185):
+// when the full step does not lower the residual norm, halve it (residual
+// evaluations only) up to line_search times; then search the opposite
+// direction the same way; when both fail, take half the last step of the
+// first direction.  The point accepted is then linearized.
 void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* x, const sx_solver_options& o,
                  Workspace& w, double* r, double* c, sx_solve_report* rep)
 {
   const int64_t n    = f->test->ndof();
   sx_grad*      g    = nullptr;
+  DeviceBuffer  x0;  // the iterate before the step (line search only)
   double        norm = residual_norm(f, wrt, x, r, ess, ness, &g);
   rep->initial_norm  = norm;
   const double goal  = std::max(o.rel_tol * norm, o.abs_tol);
@@ -3119,8 +3299,43 @@ void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* 
     sx_solve_report lr{};
     linear_solve(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
     rep->linear_iterations += lr.linear_iterations;
+    const double prev = norm;
+    if (f->line_search > 0) {
+      x0.allocate(size_t(n) * 8);
+      SX_HIP(hipMemcpyAsync(x0.ptr, x, size_t(n) * 8, hipMemcpyDeviceToDevice, f->stream));
+    }
     SX_HIP(sxk::vec_axpy(-1.0, c, x, n, f->stream));
     norm = residual_norm(f, wrt, x, r, ess, ness, &g);  // residual and linearization at the new x
+    const int ls_max = f->line_search;
+    if (ls_max > 0 && !(norm < prev)) {
+      // x = x0 + scale c, residual only (a NaN norm never counts as lower)
+      auto step = [&](double scale) {
+        SX_HIP(hipMemcpyAsync(x, x0.ptr, size_t(n) * 8, hipMemcpyDeviceToDevice, f->stream));
+        SX_HIP(sxk::vec_axpy(scale, c, x, n, f->stream));
+        return residual_norm(f, wrt, x, r, ess, ness, nullptr);
+      };
+      double scale = 1.0;
+      int    ls    = 0;
+      for (; !(norm < prev) && ls < ls_max; ++ls, ++rep->line_search_steps) {
+        scale *= 0.5;
+        norm = step(-scale);
+      }
+      if (ls == ls_max && !(norm < prev)) {
+        scale = 1.0;
+        norm  = step(scale);
+        ls    = 0;
+        for (; !(norm < prev) && ls < ls_max; ++ls, ++rep->line_search_steps) {
+          scale *= 0.5;
+          norm = step(scale);
+        }
+        if (ls == ls_max && !(norm < prev)) {
+          ++rep->line_search_steps;
+          scale *= 0.5;
+          norm = step(-scale);
+        }
+      }
+      norm = residual_norm(f, wrt, x, r, ess, ness, &g);  // the linearization at the accepted point
+    }
   }
   rep->newton_iterations = it;
   rep->final_norm        = norm;
@@ -4009,71 +4224,110 @@ int sx_qoi_add_domain_integral_dep(sx_qoi* q, int qoi_id, const double* params, 
   return api([&] { qoi_add_domain(q, qoi_id, params, nparams, depends_on, ndep, elems, nelem); });
 }
 
+}  // extern "C"
+
+namespace {
+
+// a boundary term of a QoI on the faces select() picks; the gradient of the
+// value QoI is a boundary residual form over the same faces
+void qoi_add_boundary(sx_qoi* q, int qoi_id, const double* params, int nparams, const FaceSelector& select)
+{
+  SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
+  SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
+             "QoI parameters: at most 16 doubles");
+  SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE, SX_ERR_UNSUPPORTED,
+             "boundary QoIs: SX_QOI_MEASURE, SX_QOI_POSITION or SX_QOI_VALUE");
+  const sx_space* vs = q->trials[0];
+  if (qoi_id == SX_QOI_POSITION)
+    SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
+               "position QoI: params {d}, d in 0..2");
+  if (qoi_id == SX_QOI_VALUE)
+    SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < vs->nc && params[0] == std::floor(params[0]),
+               SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
+  const auto&                     m     = vs->mesh->m;
+  const std::vector<BoundaryFace> faces = select(m);
+  const int                       n = vs->order + 1, nf = n * n;
+  auto                            Tp = std::make_unique<sx_qoi::Term>();
+  auto&                           T  = *Tp;
+  T.id                               = qoi_id;
+  T.face                             = true;
+  for (int k = 0; k < nparams; k++) T.params[k] = params[k];
+  T.nslots = int64_t(faces.size());
+  std::vector<int>     fmap(faces.size() * size_t(nf));
+  std::vector<double>  corners(faces.size() * 12);
+  std::vector<int32_t> quads(faces.size() * 4);
+  for (size_t i = 0; i < faces.size(); i++) {
+    for (int b = 0; b < n; b++)
+      for (int a = 0; a < n; a++) {
+        int i0, i1, i2;
+        face_node(faces[i].lf, a, b, n, i0, i1, i2);
+        fmap[i * size_t(nf) + size_t(b * n + a)] =
+            vs->emap[size_t(faces[i].e) * size_t(vs->npe) + size_t((i2 * n + i1) * n + i0)];
+      }
+    for (int k = 0; k < 4; k++) {
+      int i0, i1, i2;
+      face_node(faces[i].lf, k & 1, k >> 1, 2, i0, i1, i2);
+      const int v      = m.corner(faces[i].e, i0 + 2 * i1 + 4 * i2);
+      quads[i * 4 + size_t(k)] = v;
+      for (int d = 0; d < 3; d++) corners[i * 12 + size_t(k * 3 + d)] = m.X[size_t(3 * v + d)];
+    }
+  }
+  T.d_map.upload(fmap);
+  T.d_coords.upload(corners);
+  T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
+  if (qoi_id == SX_QOI_VALUE) {
+    // the gradient int phi_j dA on component c: the traction integrand with
+    // A = 0, b = -e_c (residual -(A n + b)), or the flux integrand q0 = 1
+    const int dep[1] = {0};
+    const int c      = int(params[0]);
+    int       rc;
+    if (vs->nc == 3) {
+      double tp[12] = {0};
+      tp[9 + c]     = -1.0;
+      rc = sx_form_add_boundary_integral_faces(q->grads[0], SX_BQF_TRACTION, tp, 12, dep, 1, quads.data(),
+                                               int64_t(faces.size()));
+    } else {
+      const double fp[3] = {1.0, 0.0, 0.0};
+      rc = sx_form_add_boundary_integral_faces(q->grads[0], SX_BQF_FLUX, fp, 3, dep, 1, quads.data(),
+                                               int64_t(faces.size()));
+    }
+    if (rc != SX_OK) throw sx_failure(rc, g_last_error);
+  }
+  q->terms.push_back(std::move(Tp));
+}
+
+}  // namespace
+
+extern "C" {
+
 int sx_qoi_add_boundary_integral(sx_qoi* q, int qoi_id, const double* params, int nparams, int face_mask,
                                  uint64_t attr_mask)
 {
   return api([&] {
-    SX_REQUIRE(q, SX_ERR_ARG, "null QoI");
-    SX_REQUIRE(nparams >= 0 && nparams <= SX_MAX_PARAMS && (nparams == 0 || params), SX_ERR_ARG,
-               "QoI parameters: at most 16 doubles");
-    SX_REQUIRE(qoi_id == SX_QOI_MEASURE || qoi_id == SX_QOI_POSITION || qoi_id == SX_QOI_VALUE, SX_ERR_UNSUPPORTED,
-               "boundary QoIs: SX_QOI_MEASURE, SX_QOI_POSITION or SX_QOI_VALUE");
     SX_REQUIRE(face_mask >= 0 && face_mask < 64, SX_ERR_ARG, "face_mask selects among 6 planes");
-    const sx_space* vs = q->trials[0];
-    if (qoi_id == SX_QOI_POSITION)
-      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] <= 2 && params[0] == std::floor(params[0]), SX_ERR_ARG,
-                 "position QoI: params {d}, d in 0..2");
-    if (qoi_id == SX_QOI_VALUE)
-      SX_REQUIRE(nparams >= 1 && params[0] >= 0 && params[0] < vs->nc && params[0] == std::floor(params[0]),
-                 SX_ERR_ARG, "value QoI: params {c}, c a component of the space");
-    const auto&                     m     = vs->mesh->m;
-    const std::vector<BoundaryFace> faces = attr_mask ? attribute_faces(m, attr_mask, true) : boundary_faces(m, face_mask);
-    const int                       n = vs->order + 1, nf = n * n;
-    auto                            Tp = std::make_unique<sx_qoi::Term>();
-    auto&                           T  = *Tp;
-    T.id                               = qoi_id;
-    T.face                             = true;
-    for (int k = 0; k < nparams; k++) T.params[k] = params[k];
-    T.nslots = int64_t(faces.size());
-    std::vector<int>    fmap(faces.size() * size_t(nf));
-    std::vector<double> corners(faces.size() * 12);
-    for (size_t i = 0; i < faces.size(); i++) {
-      for (int b = 0; b < n; b++)
-        for (int a = 0; a < n; a++) {
-          int i0, i1, i2;
-          face_node(faces[i].lf, a, b, n, i0, i1, i2);
-          fmap[i * size_t(nf) + size_t(b * n + a)] =
-              vs->emap[size_t(faces[i].e) * size_t(vs->npe) + size_t((i2 * n + i1) * n + i0)];
-        }
-      for (int k = 0; k < 4; k++) {
-        int i0, i1, i2;
-        face_node(faces[i].lf, k & 1, k >> 1, 2, i0, i1, i2);
-        const int v = m.corner(faces[i].e, i0 + 2 * i1 + 4 * i2);
-        for (int d = 0; d < 3; d++) corners[i * 12 + size_t(k * 3 + d)] = m.X[size_t(3 * v + d)];
-      }
-    }
-    T.d_map.upload(fmap);
-    T.d_coords.upload(corners);
-    T.d_part.allocate(size_t(std::max<int64_t>(T.nslots, 1)) * 8);
-    if (qoi_id == SX_QOI_VALUE) {
-      // the gradient int phi_j dA on component c: the traction integrand with
-      // A = 0, b = -e_c (residual -(A n + b)), or the flux integrand q0 = 1
-      const int dep[1] = {0};
-      const int c      = int(params[0]);
-      int       rc;
-      if (vs->nc == 3) {
-        double tp[12] = {0};
-        tp[9 + c]     = -1.0;
-        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grads[0], SX_BQF_TRACTION, tp, 12, dep, 1, attr_mask)
-                       : sx_form_add_boundary_integral(q->grads[0], SX_BQF_TRACTION, tp, 12, dep, 1, face_mask);
-      } else {
-        const double fp[3] = {1.0, 0.0, 0.0};
-        rc = attr_mask ? sx_form_add_boundary_integral_attr(q->grads[0], SX_BQF_FLUX, fp, 3, dep, 1, attr_mask)
-                       : sx_form_add_boundary_integral(q->grads[0], SX_BQF_FLUX, fp, 3, dep, 1, face_mask);
-      }
-      if (rc != SX_OK) throw sx_failure(rc, g_last_error);
-    }
-    q->terms.push_back(std::move(Tp));
+    qoi_add_boundary(q, qoi_id, params, nparams, [&](const sxh::HexMesh& m) {
+      return attr_mask ? attribute_faces(m, attr_mask, true) : boundary_faces(m, face_mask);
+    });
+  });
+}
+
+int sx_qoi_add_boundary_integral_pred(sx_qoi* q, int qoi_id, const double* params, int nparams,
+                                      sx_face_predicate pred, void* user)
+{
+  return api([&] {
+    SX_REQUIRE(pred, SX_ERR_ARG, "null face predicate");
+    qoi_add_boundary(q, qoi_id, params, nparams,
+                     [&](const sxh::HexMesh& m) { return predicate_faces(m, pred, user); });
+  });
+}
+
+int sx_qoi_add_boundary_integral_faces(sx_qoi* q, int qoi_id, const double* params, int nparams,
+                                       const int32_t* face_vertices, int64_t nfaces)
+{
+  return api([&] {
+    SX_REQUIRE(nfaces >= 0 && (nfaces == 0 || face_vertices), SX_ERR_ARG, "null face list");
+    qoi_add_boundary(q, qoi_id, params, nparams,
+                     [&](const sxh::HexMesh& m) { return listed_faces(m, face_vertices, nfaces); });
   });
 }
 

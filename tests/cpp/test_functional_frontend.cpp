@@ -294,6 +294,26 @@ int main()
         for (int i = 0; i < 3; i++) worst = std::fmax(worst, std::fabs(Ra[size_t(nd * 3 + i)]));
     }
     CHECK(worst < 1e-13, "traction patch (AddSurfaceIntegral) non-Dirichlet residual %.2e", worst);
+    // the same max faces chosen by a predicate on the corners (Domain::ofBoundaryElements)
+    Functional<space(space)> patch_pred(&fes, {&fes});
+    patch_pred.AddDomainIntegral(Dimension<3>{}, DependsOn<0>{},
+                                 solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                                     {1.0, 1.0, 1.0}, GeometricNonlinearities::On),
+                                 mesh);
+    int npred = 0;
+    patch_pred.AddSurfaceIntegral(DependsOn<0>{}, trac,
+                                  BoundaryDomain::OfBoundaryElements([&npred](std::vector<BoundaryDomain::vec3> X, int) {
+                                    npred++;
+                                    for (int d = 0; d < 3; d++) {
+                                      bool on = true;
+                                      for (const auto& c : X) on = on && std::fabs(c[size_t(d)] - 1.0) < 1e-12;
+                                      if (on) return true;
+                                    }
+                                    return false;
+                                  }));
+    std::vector<double> Rp = patch_pred(0.0, Vector(Ua)).ToHost();
+    CHECK(npred > 0 && relerr(Rp, Ra) == 0.0, "surface integral over predicate-selected faces == Planes (%d calls)",
+          npred);
   }
 
   // ---- heat: conduction + constant source
@@ -480,6 +500,36 @@ int main()
     for (size_t i = 2; i < res[0].size(); i += 3) total += res[0][i];
     CHECK(relerr(res[0], res[1]) < 1e-14 && std::fabs(total - 0.5) < 1e-12,
           "tip traction on attribute 2 == on the x = max plane (%.2e), total %.15f", relerr(res[0], res[1]), total);
+  }
+
+  // ---- DependsOn<0, 1>: the solid with its acceleration (MaterialStressFunctor's source density * a)
+  {
+    Functional<space(space, space)> dyn(&fes, {&fes, &fes});
+    dyn.AddDomainIntegral(Dimension<3>{}, DependsOn<0, 1>{},
+                          solid_mechanics::MaterialStressFunctor<solid_mechanics::NeoHookean>(
+                              solid_mechanics::NeoHookean{1.7, K, G}, GeometricNonlinearities::On),
+                          mesh);
+    std::vector<double> Acc(static_cast<size_t>(ndof)), Z(static_cast<size_t>(ndof), 0.0);
+    for (int64_t i = 0; i < ndof; i++) Acc[size_t(i)] = u01(7, 4000, uint64_t(i)) - 0.5;
+    Vector       acc(Acc);
+    void*        od    = orc_form(om, os);
+    const double pd[16] = {1.7, K, G};
+    orc_form_add_integral2(od, 3, pd, nullptr, 0, nullptr);
+    std::vector<double> Rd(static_cast<size_t>(ndof)), Jd(static_cast<size_t>(ndof)), R0(static_cast<size_t>(ndof));
+    orc_form_eval2(od, U.data(), Acc.data(), Rd.data(), 1);
+    orc_form_jvp(od, dU.data(), Jd.data());
+    CHECK(relerr(dyn(0.0, u, acc).ToHost(), Rd) < 1e-12, "DependsOn<0, 1> residual vs oracle");
+    {
+      auto [rd, dRd] = dyn(0.0, differentiate_wrt(u), acc);
+      CHECK(relerr(dRd(du).ToHost(), Jd) < 1e-12, "DependsOn<0, 1> J.v in the displacement vs oracle");
+    }
+    orc_form_eval2(od, U.data(), dU.data(), Rd.data(), 0);
+    orc_form_eval2(od, U.data(), Z.data(), R0.data(), 0);
+    for (size_t i = 0; i < Rd.size(); i++) Rd[i] -= R0[i];
+    auto [ra, dRa] = dyn(0.0, u, differentiate_wrt(acc));
+    CHECK(relerr(dRa(du).ToHost(), Rd) < 1e-11, "DependsOn<0, 1> J.v in the acceleration vs oracle: %.2e",
+          relerr(dRa(du).ToHost(), Rd));
+    orc_form_free(od);
   }
 
   orc_form_free(of);
