@@ -228,6 +228,8 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean
         # FP64 and FP32-arithmetic forms each take their own scatter option (both atomic by default)
         scatter = args.scatter if mode.startswith("f64") else args.split_scatter
         form = make_form(scatter)
+        form.set_option(sx.capi.SX_OPT_PRINT_LEVEL, args.print_level)
+        form.set_option(sx.capi.SX_OPT_LINE_SEARCH, args.line_search)
         form.set_option(sx.capi.SX_OPT_TANGENT_PRECISION, tprec)
         form.set_option(sx.capi.SX_OPT_JVP_ARITHMETIC, arith)
         if "_gmres" in mode:
@@ -295,12 +297,14 @@ def run_solve(sx, args, space, make_form, precisions, mesh_label, qf="neohookean
             "load_steps": args.load_steps,
             **path_info,
             "newton_rel_tol": args.newton_tol,
+            "max_line_search_iterations": args.line_search,
             "cg_rel_tol": args.cg_tol,
             "precond": "Jacobi",
             "linear_solver": "GMRES" if "_gmres" in mode else "CG",
             "tangent": {0: "f64", 1: "f32", 2: "bf16"}[tprec],
             "jvp_arithmetic": {0: "f64", 1: "f32"}[arith],
             "newton_iterations": [r["newton_iterations"] for r in reps],
+            "line_search_steps": [r["line_search_steps"] for r in reps],
             "cg_iterations": [r["linear_iterations"] for r in reps],
             "final_rel_residual": [r["final_norm"] / r["initial_norm"] if r["initial_norm"] else 0.0
                                    for r in reps],
@@ -380,6 +384,10 @@ def main():
     ap.add_argument("--shear", type=float, default=0.03, help="J2 workloads: the cyclic shear amplitude")
     ap.add_argument("--gmres-restart", type=int, default=50)
     ap.add_argument("--newton-tol", type=float, default=1e-8)
+    ap.add_argument("--line-search", type=int, default=0,
+                    help="max_line_search_iterations of the Newton loops of the solves (0: plain Newton)")
+    ap.add_argument("--print-level", type=int, default=0,
+                    help="> 0: the Newton loops of the solves report every iteration and linear solve on stderr")
     ap.add_argument("--cg-tol", type=float, default=1e-6)
     ap.add_argument("--cg-max", type=int, default=20000)
     ap.add_argument("--solve-precision", default="f64",

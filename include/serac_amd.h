@@ -258,10 +258,15 @@ enum {
                                    2: the same, after adding the values of nodes shared with the
                                    previous element of the workgroup in LDS (one add per such node);
                                    0: one component per instruction from the node's own lane */
-  SX_OPT_LINE_SEARCH       = 8  /* max_line_search_iterations of NonlinearSolverOptions
+  SX_OPT_LINE_SEARCH       = 8, /* max_line_search_iterations of NonlinearSolverOptions
                                    (num/solver_config.hpp:389-391, num/equation_solver.cpp:21-185):
                                    0 (default) plain Newton; k > 0 the two-way backtracking line
                                    search of serac's NewtonSolver in the Newton loops */
+  SX_OPT_MIN_NEWTON_ITERATIONS = 9, /* min_iterations (num/solver_config.hpp:384-385): the Newton
+                                       loops do not stop before this many iterations (default 0) */
+  SX_OPT_PRINT_LEVEL       = 10  /* print_level (num/solver_config.hpp:393-394): > 0 makes the Newton
+                                    loops write each iteration's residual norm and linear solve to
+                                    stderr (default 0: silent) */
 };
 /* LinearSolver::CG / LinearSolver::GMRES of num/equation_solver.cpp:916-919 */
 enum { SX_LINSOLVER_CG = 0, SX_LINSOLVER_GMRES = 1 };

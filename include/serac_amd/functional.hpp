@@ -1071,7 +1071,10 @@ class Functional<double(trial)> {
 // geometric multigrid V-cycle (3-component spaces on cuboid meshes), the
 // stand-in for BoomerAMG (num/equation_solver.cpp:1014-1021) with CG.
 // ---------------------------------------------------------------------------
-enum class NonlinearSolver { Newton };
+// Newton: plain Newton; NewtonLineSearch: with the two-way backtracking line
+// search (max_line_search_iterations cut-backs); the other solvers of
+// num/solver_config.hpp:142-155 (L-BFGS, trust region, KINSOL, PETSc) are not built
+enum class NonlinearSolver { Newton, NewtonLineSearch };
 enum class LinearSolver { CG, GMRES };
 enum class Preconditioner { HypreJacobi, HypreAMG };
 
@@ -1083,12 +1086,16 @@ struct LinearSolverOptions {
   int            max_iterations = 300;
   int            gmres_restart  = 50;  // mfem::GMRESSolver's default Krylov dimension
 };
+// the field order of num/solver_config.hpp:373-400, so aggregate
+// initialisers written for the reference fill the same fields
 struct NonlinearSolverOptions {
-  NonlinearSolver nonlin_solver  = NonlinearSolver::Newton;
-  double          relative_tol   = 1.0e-8;
-  double          absolute_tol   = 1.0e-12;
-  int             max_iterations = 20;
-  int             max_line_search_iterations = 0;  // num/solver_config.hpp:389-391
+  NonlinearSolver nonlin_solver              = NonlinearSolver::Newton;
+  double          relative_tol               = 1.0e-8;
+  double          absolute_tol               = 1.0e-12;
+  int             min_iterations             = 0;
+  int             max_iterations             = 20;
+  int             max_line_search_iterations = 0;
+  int             print_level                = 0;
 };
 
 class EquationSolver {
@@ -1103,7 +1110,11 @@ class EquationSolver {
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINEAR_SOLVER,
                                      lin_.linear_solver == LinearSolver::GMRES ? SX_LINSOLVER_GMRES : SX_LINSOLVER_CG));
     detail::check(sx_form_set_option(residual.handle(), SX_OPT_GMRES_RESTART, lin_.gmres_restart));
-    detail::check(sx_form_set_option(residual.handle(), SX_OPT_LINE_SEARCH, nl_.max_line_search_iterations));
+    detail::check(sx_form_set_option(
+        residual.handle(), SX_OPT_LINE_SEARCH,
+        nl_.nonlin_solver == NonlinearSolver::NewtonLineSearch ? nl_.max_line_search_iterations : 0));
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_MIN_NEWTON_ITERATIONS, nl_.min_iterations));
+    detail::check(sx_form_set_option(residual.handle(), SX_OPT_PRINT_LEVEL, nl_.print_level));
     const bool amg = lin_.preconditioner == Preconditioner::HypreAMG;
     if (amg) {
       if (!mg_ || mg_space_ != residual.test_space()) {

@@ -28,6 +28,8 @@ SX_OPT_LINEAR_SOLVER = 5
 SX_OPT_GMRES_RESTART = 6
 SX_OPT_SCATTER_ORDER = 7
 SX_OPT_LINE_SEARCH = 8
+SX_OPT_MIN_NEWTON_ITERATIONS = 9
+SX_OPT_PRINT_LEVEL = 10
 SX_LINSOLVER_CG = 0
 SX_LINSOLVER_GMRES = 1
 SX_PREC_F64 = 0

@@ -345,6 +345,8 @@ struct sx_form {
   int                                    linsolver    = SX_LINSOLVER_CG;  // linear solver of the Newton loops
   int                                    gmres_restart = 50;
   int                                    line_search   = 0;  // max_line_search_iterations (0: plain Newton)
+  int                                    min_newton    = 0;  // min_iterations of the Newton loops
+  int                                    print_level   = 0;  // > 0: the Newton loops report each iteration on stderr
   sx_learned*                            learned      = nullptr;        // borrowed
   sx_multigrid*                          mg           = nullptr;        // borrowed: multigrid-PCG in the Newton loops
   const double*                          shape        = nullptr;        // borrowed: shape displacement (device)
@@ -1719,6 +1721,14 @@ int sx_form_set_option(sx_form* f, int option, int64_t value)
       case SX_OPT_LINE_SEARCH:
         SX_REQUIRE(value >= 0 && value <= 64, SX_ERR_ARG, "line search iterations must be in 0..64");
         f->line_search = int(value);
+        return;
+      case SX_OPT_MIN_NEWTON_ITERATIONS:
+        SX_REQUIRE(value >= 0 && value <= 1000, SX_ERR_ARG, "minimum Newton iterations must be in 0..1000");
+        f->min_newton = int(value);
+        return;
+      case SX_OPT_PRINT_LEVEL:
+        SX_REQUIRE(value >= 0, SX_ERR_ARG, "print level must be >= 0");
+        f->print_level = int(value);
         return;
     }
     throw sx_failure(SX_ERR_ARG, "unknown form option " + std::to_string(option));
@@ -3288,10 +3298,16 @@ void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* 
   double        norm = residual_norm(f, wrt, x, r, ess, ness, &g);
   rep->initial_norm  = norm;
   const double goal  = std::max(o.rel_tol * norm, o.abs_tol);
+  const double norm0 = norm;
   int          it    = 0;
   for (;; it++) {
+    if (f->print_level > 0) {
+      std::fprintf(stderr, "Newton iteration %3d : ||r|| = %13.6e, ||r||/||r_0|| = %13.6e\n", it, norm,
+                   norm0 != 0.0 ? norm / norm0 : norm);
+      std::fflush(stderr);
+    }
     SX_REQUIRE(std::isfinite(norm), SX_ERR_NOCONV, "Newton residual is not finite");
-    if (norm <= goal) {
+    if (norm <= goal && it >= f->min_newton) {
       rep->converged = 1;
       break;
     }
@@ -3299,6 +3315,12 @@ void newton_core(sx_form* f, int wrt, const int32_t* ess, int64_t ness, double* 
     sx_solve_report lr{};
     linear_solve(g, ess, ness, r, c, o.lin_rel_tol, o.lin_abs_tol, o.lin_max_iter, w, &lr);
     rep->linear_iterations += lr.linear_iterations;
+    if (f->print_level > 0) {
+      std::fprintf(stderr, "  linear solve: %d iterations, %s, preconditioned norm %.6e -> %.6e\n",
+                   lr.linear_iterations, lr.converged ? "converged" : "not converged", lr.initial_norm,
+                   lr.final_norm);
+      std::fflush(stderr);
+    }
     const double prev = norm;
     if (f->line_search > 0) {
       x0.allocate(size_t(n) * 8);

@@ -381,6 +381,47 @@ __device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
   }
 }
 
+// The DS tangent entries of one quadrature point held in registers for the
+// J.v chain rule: as T, or, for a BF16 tangent, still packed two to a 32-bit
+// register and widened at their one use (the same conversions, half the
+// registers: 25 instead of 45 for the Neo-Hookean tangent, which lets more
+// waves share a SIMD).  get(k) needs a compile-time k after unrolling.
+template <typename T, typename S, int DS, int NQ>
+struct TangentRegs {
+  T d[DS];
+  __device__ __forceinline__ void load(const void* base, int q) { load_tangent<T, S, DS, NQ>(base, q, d); }
+  __device__ __forceinline__ T    get(int k) const { return d[k]; }
+};
+template <typename T, int DS, int NQ>
+struct TangentRegs<T, bf16, DS, NQ> {
+  using L = TangentLayout<bf16, DS, NQ>;
+  uint32_t w[L::G * 4 > 0 ? L::G * 4 : 1];  // entries 8g .. 8g + 7 as words 4g .. 4g + 3, low half first
+  float    rem[L::REM > 0 ? L::REM : 1];   // the last, partial group, widened on load
+  __device__ __forceinline__ void load(const void* base, int q)
+  {
+    typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+    const bf16* Dp = static_cast<const bf16*>(base);
+#pragma unroll
+    for (int g = 0; g < L::G; g++) {
+      const u4 v   = __builtin_nontemporal_load(reinterpret_cast<const u4*>(Dp + g * L::VEC * NQ + q * L::VEC));
+      w[4 * g]     = v.x;
+      w[4 * g + 1] = v.y;
+      w[4 * g + 2] = v.z;
+      w[4 * g + 3] = v.w;
+    }
+#pragma unroll
+    for (int j = 0; j < L::REM; j++)
+      rem[j] = bf16_to_float(uint32_t(__builtin_nontemporal_load(
+          reinterpret_cast<const uint16_t*>(Dp + L::G * L::VEC * NQ + q * L::REM + j))));
+  }
+  __device__ __forceinline__ T get(int k) const
+  {
+    if (k >= L::G * L::VEC) return T(rem[k - L::G * L::VEC]);
+    const uint32_t x = w[k / 2];
+    return T(__uint_as_float((k & 1) ? (x & 0xffff0000u) : (x << 16)));
+  }
+};
+
 // Stores tangent entry k (FP64 value) of quadrature point q: entries arrive in
 // increasing k; whole 16-byte groups are stored at once from `pend`.
 template <typename S, int DS, int NQ>
@@ -775,8 +816,8 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
 #pragma unroll
     for (int c = 0; c < NC; c++) ug[c] = T(u[lidx(node, c, NC, P.nnodes, P.ordering)]);
   }
-  T Dr[DS];
-  if (active) load_tangent<T, S, DS, NQ>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li, Dr);
+  TangentRegs<T, S, DS, NQ> Dr;
+  if (active) Dr.load(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li);
   if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) S_[c * NQ + li] = ug[c];
@@ -849,7 +890,7 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
     for (int s = 0; s < M; s++)
 #pragma unroll
       for (int rr = 0; rr < Shape::rows(s); rr++) {
-        const T Dk = Dr[Shape::idx(rr, s)];
+        const T Dk = Dr.get(Shape::idx(rr, s));
         out[rr] += Dk * in[s];
         if (Shape::SYM && rr != s) out[s] += Dk * in[rr];
       }

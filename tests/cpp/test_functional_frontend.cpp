@@ -340,7 +340,7 @@ int main()
     std::vector<double> x0(size_t(ndof), 0.0);
     for (int32_t d : ess1) x0[size_t(d)] = -0.01;
     Vector         x(x0);
-    EquationSolver solver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 20},
+    EquationSolver solver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 0, 20},
                           LinearSolverOptions{LinearSolver::CG, Preconditioner::HypreJacobi, 1e-10, 0.0, 5000});
     sx_solve_report rep = solver.solve(residual, x, ess);
     CHECK(rep.converged && rep.newton_iterations <= 6, "Newton converged in %d iterations (%d CG)",
@@ -349,9 +349,22 @@ int main()
     for (int32_t d : ess) Rx[size_t(d)] = 0.0;
     CHECK(norm2(Rx) <= 1e-9 * rep.initial_norm, "converged residual %.2e (initial %.2e)", norm2(Rx),
           rep.initial_norm);
+    // NewtonLineSearch with min_iterations: the same solution after at least min_iterations Newton iterations
+    // (the one past convergence starts at roundoff, where a cut-back may happen)
+    {
+      Vector         xl(x0);
+      EquationSolver lsolver(
+          NonlinearSolverOptions{NonlinearSolver::NewtonLineSearch, 1e-10, 1e-14, rep.newton_iterations + 1, 20, 4},
+          LinearSolverOptions{LinearSolver::CG, Preconditioner::HypreJacobi, 1e-10, 0.0, 5000});
+      sx_solve_report lrep = lsolver.solve(residual, xl, ess);
+      CHECK(lrep.converged && lrep.newton_iterations == rep.newton_iterations + 1 &&
+                relerr(xl.ToHost(), x.ToHost()) < 1e-9,
+            "NewtonLineSearch with min_iterations %d: %d iterations, %d cut-backs", rep.newton_iterations + 1,
+            lrep.newton_iterations, lrep.line_search_steps);
+    }
     // the same step with LinearSolver::GMRES reaches the same displacement
     Vector         xg(x0);
-    EquationSolver gsolver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 20},
+    EquationSolver gsolver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 0, 20},
                            LinearSolverOptions{LinearSolver::GMRES, Preconditioner::HypreJacobi, 1e-10, 0.0, 5000});
     sx_solve_report grep_ = gsolver.solve(residual, xg, ess);
     CHECK(grep_.converged && relerr(xg.ToHost(), x.ToHost()) < 1e-8, "GMRES Newton matches CG Newton (%.2e, %d GMRES)",
@@ -364,7 +377,7 @@ int main()
     for (int32_t d : ess1) x1[size_t(d)] = -0.001;
     Vector          xj(x1), xm(x1);
     sx_solve_report jrep = solver.solve(residual, xj, ess);
-    EquationSolver  msolver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 20},
+    EquationSolver  msolver(NonlinearSolverOptions{NonlinearSolver::Newton, 1e-10, 1e-14, 0, 20},
                             LinearSolverOptions{LinearSolver::CG, Preconditioner::HypreAMG, 1e-10, 0.0, 5000});
     sx_solve_report mrep = msolver.solve(residual, xm, ess);
     CHECK(mrep.converged && relerr(xm.ToHost(), xj.ToHost()) < 1e-8 && mrep.linear_iterations < jrep.linear_iterations,
