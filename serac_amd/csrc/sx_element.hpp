@@ -124,6 +124,30 @@ __device__ __forceinline__ void lds_barrier()
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Cross-lane reads within a wave (DPP): quad_bcast<x> gives every lane the
+// value of lane x of its group of 4 lanes; row_ror<k> the value of the lane k
+// places away (cyclically) within its row of 16 lanes.  With one p = 3 element
+// per wave and lane = (i2 N + i1) N + i0, a quad holds one x line of nodes or
+// points and a row one z layer, so the x and y contractions need no LDS.
+template <int X>
+__device__ __forceinline__ float quad_bcast(float v)
+{
+  static_assert(X >= 0 && X < 4, "lane in a quad");
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), X * 0x55, 0xf, 0xf, false));
+}
+template <int K>
+__device__ __forceinline__ float row_ror(float v)
+{
+  static_assert(K >= 1 && K < 16, "rotation within a row");
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + K, 0xf, 0xf, false));
+}
+template <int K>
+__device__ __forceinline__ int row_ror_i(int v)
+{
+  static_assert(K >= 1 && K < 16, "rotation within a row");
+  return __builtin_amdgcn_update_dpp(0, v, 0x120 + K, 0xf, 0xf, false);
+}
+
 template <typename T>
 __device__ __forceinline__ void scatter_add(T* r, int64_t o, T v, int atomic)
 {
@@ -970,6 +994,213 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   }
 }
 
+// J.v for p = 3 (N = 4) with FP32 arithmetic: the contractions along x and y
+// move values between lanes of the element's wave (quad_bcast, row_ror)
+// instead of through LDS; only the z contractions and the scatter use LDS.
+// Vectors V (float, or double for FP32 arithmetic on FP64 vectors), tangent S.
+// The same stages, in the same order of operations per term, as k_jvp.
+template <int NC, int LAYOUT, typename S, typename V>
+__global__ void __launch_bounds__(256) k_jvp_xy(const ElemLaunch P)
+{
+  using T             = float;
+  constexpr int  N    = 4;
+  constexpr int  NQ   = N * N * N;
+  constexpr int  EB   = 256 / NQ;
+  using Shape         = TanShape<LAYOUT, NC>;
+  constexpr bool FULL = Shape::VAL;
+  constexpr int  M    = Shape::M;
+  constexpr int  DS   = Shape::DS;
+  constexpr int  R    = NC * NQ;
+  constexpr int  PER  = 3 * R;  // the y-interpolation outputs, later the y-integration outputs and the scatter
+  constexpr int  TAB  = 4 * N * N;
+  using Lay           = TangentLayout<S, DS, NQ>;
+
+  __shared__ T   smem[EB * PER];
+  __shared__ T   stab[TAB];
+  __shared__ int snode[EB * NQ];
+
+  if (P.stop != nullptr && *P.stop != 0.0) return;
+
+  const int     tid    = threadIdx.x;
+  const int     el     = tid / NQ;  // one element per wave
+  const int     li     = tid - el * NQ;
+  const int     i0     = li & 3;
+  const int     i1     = (li >> 2) & 3;
+  const int     i2     = li >> 4;
+  const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
+  const bool    active = (el < EB) && (slot < P.slot_end);  // the same for every lane of a wave
+  T*            S_     = smem + (el < EB ? el : 0) * PER;
+  const V*      u      = static_cast<const V*>(P.u);
+  V*            r      = static_cast<V*>(P.r);
+
+  for (int k = tid; k < TAB; k += blockDim.x) stab[k] = T(P.tables[k]);
+  const T* tB  = stab;
+  const T* tG  = stab + N * N;
+  const T* tBw = stab + 2 * N * N;
+  const T* tGw = stab + 3 * N * N;
+
+  int64_t node = 0;
+  T       ug[NC];
+  if (active) {
+    node = P.map[size_t(slot) * NQ + li];
+#pragma unroll
+    for (int c = 0; c < NC; c++) ug[c] = T(u[lidx(node, c, NC, P.nnodes, P.ordering)]);
+  }
+  TangentRegs<T, S, DS, NQ> Dr;
+  if (active) Dr.load(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li);
+  // the y index of the lane a rotation by 4 m within the row reads (m = 0..3)
+  const int r16   = li & 15;
+  const int ys[4] = {i1, row_ror_i<4>(r16) >> 2, row_ror_i<8>(r16) >> 2, row_ror_i<12>(r16) >> 2};
+  lds_barrier();  // the tables
+
+  T acc[NC];
+  {
+    // ---- interpolation: x through the quad, y through the row
+    T b0[NC], b1[NC], b2[NC];
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const T v[4] = {quad_bcast<0>(ug[c]), quad_bcast<1>(ug[c]), quad_bcast<2>(ug[c]), quad_bcast<3>(ug[c])};
+        T       s0 = T(0), s1 = T(0);
+#pragma unroll
+        for (int x = 0; x < N; x++) {
+          s0 += tB[i0 * N + x] * v[x];
+          s1 += tG[i0 * N + x] * v[x];
+        }
+        const T a0[4] = {s0, row_ror<4>(s0), row_ror<8>(s0), row_ror<12>(s0)};
+        const T a1[4] = {s1, row_ror<4>(s1), row_ror<8>(s1), row_ror<12>(s1)};
+        T       t0 = T(0), t1 = T(0), t2 = T(0);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const T b = tB[i1 * N + ys[m]], g = tG[i1 * N + ys[m]];
+          t0 += b * a0[m];
+          t1 += b * a1[m];
+          t2 += g * a0[m];
+        }
+        b0[c] = t0;
+        b1[c] = t1;
+        b2[c] = t2;
+      }
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        S_[0 * R + c * NQ + li] = b0[c];
+        S_[1 * R + c * NQ + li] = b1[c];
+        S_[2 * R + c * NQ + li] = b2[c];
+      }
+    }
+    lds_barrier();
+
+    // ---- z through LDS, then the chain rule at the point
+    T out[M];
+    if (active) {
+      T in[M];
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const T* a20 = S_ + 0 * R + c * NQ;
+        const T* a21 = S_ + 1 * R + c * NQ;
+        const T* a22 = S_ + 2 * R + c * NQ;
+        T        v = T(0), g0 = T(0), g1 = T(0), g2 = T(0);
+#pragma unroll
+        for (int z = 0; z < N; z++) {
+          const int idx = (z * N + i1) * N + i0;
+          const T   b = tB[i2 * N + z], g = tG[i2 * N + z];
+          if constexpr (FULL) v += b * a20[idx];
+          g0 += b * a21[idx];
+          g1 += b * a22[idx];
+          g2 += g * a20[idx];
+        }
+        if constexpr (FULL) in[c] = v;
+        in[(FULL ? NC : 0) + c * 3 + 0] = g0;
+        in[(FULL ? NC : 0) + c * 3 + 1] = g1;
+        in[(FULL ? NC : 0) + c * 3 + 2] = g2;
+      }
+#pragma unroll
+      for (int k = 0; k < M; k++) out[k] = T(0);
+#pragma unroll
+      for (int s = 0; s < M; s++)
+#pragma unroll
+        for (int rr = 0; rr < Shape::rows(s); rr++) {
+          const T Dk = Dr.get(Shape::idx(rr, s));
+          out[rr] += Dk * in[s];
+          if (Shape::SYM && rr != s) out[s] += Dk * in[rr];
+        }
+    }
+
+    // ---- integration: x through the quad, y through the row, from registers
+    T d0[NC], d1[NC];
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const T f0 = out[(FULL ? NC : 0) + c * 3 + 0], f1 = out[(FULL ? NC : 0) + c * 3 + 1];
+        const T f2 = out[(FULL ? NC : 0) + c * 3 + 2];
+        const T q0[4] = {quad_bcast<0>(f0), quad_bcast<1>(f0), quad_bcast<2>(f0), quad_bcast<3>(f0)};
+        const T q1[4] = {quad_bcast<0>(f1), quad_bcast<1>(f1), quad_bcast<2>(f1), quad_bcast<3>(f1)};
+        const T q2[4] = {quad_bcast<0>(f2), quad_bcast<1>(f2), quad_bcast<2>(f2), quad_bcast<3>(f2)};
+        T       sr[4] = {T(0), T(0), T(0), T(0)};
+        if constexpr (FULL) {
+          sr[0] = quad_bcast<0>(out[c]);
+          sr[1] = quad_bcast<1>(out[c]);
+          sr[2] = quad_bcast<2>(out[c]);
+          sr[3] = quad_bcast<3>(out[c]);
+        }
+        T s0 = T(0), s1 = T(0), s2 = T(0);
+#pragma unroll
+        for (int qx = 0; qx < N; qx++) {
+          const T bw = tBw[qx * N + i0], gw = tGw[qx * N + i0];
+          if constexpr (FULL) s0 += sr[qx] * bw;
+          s0 += q0[qx] * gw;
+          s1 += q1[qx] * bw;
+          s2 += q2[qx] * bw;
+        }
+        const T a20[4] = {s0, row_ror<4>(s0), row_ror<8>(s0), row_ror<12>(s0)};
+        const T a21[4] = {s1, row_ror<4>(s1), row_ror<8>(s1), row_ror<12>(s1)};
+        const T a22[4] = {s2, row_ror<4>(s2), row_ror<8>(s2), row_ror<12>(s2)};
+        T       t0 = T(0), t1 = T(0);
+#pragma unroll
+        for (int m = 0; m < 4; m++) {
+          const T bw = tBw[ys[m] * N + i1], gw = tGw[ys[m] * N + i1];
+          t0 += a20[m] * bw + a21[m] * gw;
+          t1 += a22[m] * bw;
+        }
+        d0[c] = t0;
+        d1[c] = t1;
+      }
+    }
+    lds_barrier();  // every lane has read the interpolation slots
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        S_[0 * R + c * NQ + li] = d0[c];
+        S_[1 * R + c * NQ + li] = d1[c];
+      }
+    }
+    lds_barrier();
+
+    // ---- z integration through LDS
+    if (active) {
+#pragma unroll
+      for (int c = 0; c < NC; c++) {
+        const T* a10 = S_ + 0 * R + c * NQ;
+        const T* a11 = S_ + 1 * R + c * NQ;
+        acc[c]       = T(0);
+#pragma unroll
+        for (int qz = 0; qz < N; qz++) {
+          const int idx = (qz * N + i1) * N + i0;
+          acc[c] += a10[idx] * tBw[qz * N + i2] + a11[idx] * tGw[qz * N + i2];
+        }
+      }
+    }
+  }
+  // slot 2 is free here: the z integration reads slots 0 and 1 only
+  if (P.ordered) {
+    scatter_element<N, NC, T, V>(r, acc, S_ + 2 * R, smem + (el > 0 && el < EB ? el - 1 : 0) * PER + 2 * R, snode,
+                                 el, EB, node, li, active, P.nnodes, P.ordering, P.scatter, P.ordered == 2);
+  } else if (active) {
+#pragma unroll
+    for (int c = 0; c < NC; c++) scatter_add(r, lidx(node, c, NC, P.nnodes, P.ordering), V(acc[c]), P.scatter);
+  }
+}
+
 // Diagonal of the linearized operator from the stored tangent (FP64 out):
 //   diag(c, j) = sum_q [w phi_j, w grad phi_j]_c . D(q) . [phi_j, grad phi_j]_c
 // (the diagonal entries element_gradient_kernel would assemble,
@@ -1193,11 +1424,17 @@ inline int dispatch_gradient(const ElemLaunch& P, hipError_t* err)
     return 1;
   }
   if (P.vprec == PREC_F32) {
-    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, float>);
+    if constexpr (N == 4)
+      *err = launch_grid<N>(P, k_jvp_xy<NC, LAYOUT, S, float>);
+    else
+      *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, float>);
     return 1;
   }
   if (P.vprec == PREC_MIXED) {
-    *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, double>);
+    if constexpr (N == 4)
+      *err = launch_grid<N>(P, k_jvp_xy<NC, LAYOUT, S, double>);
+    else
+      *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, double>);
     return 1;
   }
   return 0;
