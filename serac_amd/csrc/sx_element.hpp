@@ -141,6 +141,20 @@ __device__ __forceinline__ float row_ror(float v)
   static_assert(K >= 1 && K < 16, "rotation within a row");
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + K, 0xf, 0xf, false));
 }
+template <int X>
+__device__ __forceinline__ double quad_bcast(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), X * 0x55, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), X * 0x55, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+template <int K>
+__device__ __forceinline__ double row_ror(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x120 + K, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x120 + K, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
 template <int K>
 __device__ __forceinline__ int row_ror_i(int v)
 {
@@ -994,15 +1008,14 @@ __global__ void __launch_bounds__(256) k_jvp(const ElemLaunch P)
   }
 }
 
-// J.v for p = 3 (N = 4) with FP32 arithmetic: the contractions along x and y
+// J.v for p = 3 (N = 4), arithmetic in T: the contractions along x and y
 // move values between lanes of the element's wave (quad_bcast, row_ror)
 // instead of through LDS; only the z contractions and the scatter use LDS.
 // Vectors V (float, or double for FP32 arithmetic on FP64 vectors), tangent S.
 // The same stages, in the same order of operations per term, as k_jvp.
-template <int NC, int LAYOUT, typename S, typename V>
+template <typename T, int NC, int LAYOUT, typename S, typename V>
 __global__ void __launch_bounds__(256) k_jvp_xy(const ElemLaunch P)
 {
-  using T             = float;
   constexpr int  N    = 4;
   constexpr int  NQ   = N * N * N;
   constexpr int  EB   = 256 / NQ;
@@ -1425,14 +1438,14 @@ inline int dispatch_gradient(const ElemLaunch& P, hipError_t* err)
   }
   if (P.vprec == PREC_F32) {
     if constexpr (N == 4)
-      *err = launch_grid<N>(P, k_jvp_xy<NC, LAYOUT, S, float>);
+      *err = launch_grid<N>(P, k_jvp_xy<float, NC, LAYOUT, S, float>);
     else
       *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, float>);
     return 1;
   }
   if (P.vprec == PREC_MIXED) {
     if constexpr (N == 4)
-      *err = launch_grid<N>(P, k_jvp_xy<NC, LAYOUT, S, double>);
+      *err = launch_grid<N>(P, k_jvp_xy<float, NC, LAYOUT, S, double>);
     else
       *err = launch_grid<N>(P, k_jvp<N, NC, LAYOUT, float, S, double>);
     return 1;
