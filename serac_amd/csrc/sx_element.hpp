@@ -68,7 +68,8 @@ struct StoreTraits<bf16> {
 
 // Per-slot tangent layout, in units of S: the DS packed entries of quadrature
 // point q are grouped VEC = 16 / sizeof(S) at a time, [g][q][VEC], so that lane
-// q reads 16 bytes per group; the last, partial group is [q][REM].
+// q reads 16 bytes per group; the last, partial group is [REM][q], so each of
+// its entries is one coalesced load across the points.
 template <typename S, int DS, int NQ>
 struct TangentLayout {
   static constexpr int     VEC    = 16 / int(sizeof(S));
@@ -77,7 +78,7 @@ struct TangentLayout {
   static constexpr int64_t stride = (int64_t(DS) * NQ * int64_t(sizeof(S)) + 15) / 16 * (16 / int64_t(sizeof(S)));
   __host__ __device__ static constexpr int offset(int k, int q)
   {
-    return (k < G * VEC) ? (k / VEC) * VEC * NQ + q * VEC + (k % VEC) : G * VEC * NQ + q * REM + (k - G * VEC);
+    return (k < G * VEC) ? (k / VEC) * VEC * NQ + q * VEC + (k % VEC) : G * VEC * NQ + (k - G * VEC) * NQ + q;
   }
 };
 
@@ -129,37 +130,39 @@ __device__ __forceinline__ void lds_barrier()
 // places away (cyclically) within its row of 16 lanes.  With one p = 3 element
 // per wave and lane = (i2 N + i1) N + i0, a quad holds one x line of nodes or
 // points and a row one z layer, so the x and y contractions need no LDS.
+// Every lane reads a valid source lane under these controls, so the moves
+// carry no old value (no zeroing move ahead of each).
 template <int X>
 __device__ __forceinline__ float quad_bcast(float v)
 {
   static_assert(X >= 0 && X < 4, "lane in a quad");
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), X * 0x55, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), X * 0x55, 0xf, 0xf, true));
 }
 template <int K>
 __device__ __forceinline__ float row_ror(float v)
 {
   static_assert(K >= 1 && K < 16, "rotation within a row");
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x120 + K, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + K, 0xf, 0xf, true));
 }
 template <int X>
 __device__ __forceinline__ double quad_bcast(double v)
 {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), X * 0x55, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), X * 0x55, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), X * 0x55, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), X * 0x55, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 template <int K>
 __device__ __forceinline__ double row_ror(double v)
 {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x120 + K, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x120 + K, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x120 + K, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x120 + K, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 template <int K>
 __device__ __forceinline__ int row_ror_i(int v)
 {
   static_assert(K >= 1 && K < 16, "rotation within a row");
-  return __builtin_amdgcn_update_dpp(0, v, 0x120 + K, 0xf, 0xf, false);
+  return __builtin_amdgcn_mov_dpp(v, 0x120 + K, 0xf, 0xf, true);
 }
 
 template <typename T>
@@ -411,7 +414,7 @@ __device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
 #pragma unroll
   for (int j = 0; j < L::REM; j++) {
     const auto raw = __builtin_nontemporal_load(
-        reinterpret_cast<const typename StoreTraits<S>::raw*>(Dp + L::G * L::VEC * NQ + q * L::REM + j));
+        reinterpret_cast<const typename StoreTraits<S>::raw*>(Dp + L::G * L::VEC * NQ + j * NQ + q));
     if constexpr (sizeof(S) == 2)
       Dr[L::G * L::VEC + j] = T(bf16_to_float(uint32_t(raw)));
     else
@@ -450,7 +453,7 @@ struct TangentRegs<T, bf16, DS, NQ> {
 #pragma unroll
     for (int j = 0; j < L::REM; j++)
       rem[j] = bf16_to_float(uint32_t(__builtin_nontemporal_load(
-          reinterpret_cast<const uint16_t*>(Dp + L::G * L::VEC * NQ + q * L::REM + j))));
+          reinterpret_cast<const uint16_t*>(Dp + L::G * L::VEC * NQ + j * NQ + q))));
   }
   __device__ __forceinline__ T get(int k) const
   {
@@ -479,7 +482,7 @@ __device__ __forceinline__ void store_tangent_entry(void* base, int q, int k, do
     }
   } else {
     __builtin_nontemporal_store(raw(to_store(v, static_cast<S*>(nullptr))),
-                                Dp + L::G * L::VEC * NQ + q * L::REM + (k - L::G * L::VEC));
+                                Dp + L::G * L::VEC * NQ + (k - L::G * L::VEC) * NQ + q);
   }
 }
 
