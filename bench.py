@@ -485,7 +485,15 @@ def main():
     _, grad = form(0.0, U, wrt=0)
     t_lin = time.perf_counter() - tl
     tangent_bytes = grad.tangent_bytes()
-    log(f"residual {t_res * 1e3:.2f} ms, linearize {t_lin * 1e3:.2f} ms, tangent {tangent_bytes / 1e9:.2f} GB")
+    # Jacobi diagonal from the stored tangent (one warm call, then the mean of three)
+    dg = grad.diagonal()
+    td = time.perf_counter()
+    for _ in range(3):
+        grad.diagonal(out=dg)
+    t_diag = (time.perf_counter() - td) / 3
+    dg.free()
+    log(f"residual {t_res * 1e3:.2f} ms, linearize {t_lin * 1e3:.2f} ms, diagonal {t_diag * 1e3:.2f} ms, "
+        f"tangent {tangent_bytes / 1e9:.2f} GB")
 
     # J.v: warmup, then exactly `steps` applications bracketed by device syncs
     if args.warmup > 0:
@@ -650,6 +658,7 @@ def main():
         "cpu_baseline": cpu,
         "residual_ms": round(t_res * 1e3, 3),
         "linearize_ms": round(t_lin * 1e3, 3),
+        "diagonal_ms": round(t_diag * 1e3, 3),
         "tangent_bytes": tangent_bytes,
         "jv_norm": jv_norm,
         "precision_split": split,

@@ -254,8 +254,8 @@ class Gradient:
         capi.call("sx_grad_apply_e", self.h, k, duE.ptr, drE.ptr)
         return drE
 
-    def diagonal(self):
-        d = DeviceVector(self.form.test.ndof)
+    def diagonal(self, out=None):
+        d = DeviceVector(self.form.test.ndof) if out is None else out
         capi.call("sx_grad_diagonal", self.h, d.ptr)
         self.form.synchronize()
         return d

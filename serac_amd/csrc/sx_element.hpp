@@ -1219,73 +1219,141 @@ __global__ void __launch_bounds__(64 * EB) k_jvp_xy(const ElemLaunch P)
 // Diagonal of the linearized operator from the stored tangent (FP64 out):
 //   diag(c, j) = sum_q [w phi_j, w grad phi_j]_c . D(q) . [phi_j, grad phi_j]_c
 // (the diagonal entries element_gradient_kernel would assemble,
-//  fn/domain_integral_kernels.hpp:327-361).  Lane li owns node j = li.
+//  fn/domain_integral_kernels.hpp:327-361).
+// Every term is a product of one 1D factor per direction, so the sum over q
+// factorizes: with phi_j = B(qx, jx) B(qy, jy) B(qz, jz), the term of tangent
+// entry (test grad a, trial grad b) carries in direction d the factor
+// (w G)(q_d, j_d) if a = d else (w B), times G(q_d, j_d) if b = d else B.  The
+// products w B B, w G B (= w B G) and w G G are the three 1D tables.  Entries
+// (a, b) and (b, a) share their tables and are summed first; the value rows of
+// FULL_SYM add the (value, value) term and the (value, grad) pairs.  Per
+// component this leaves NF fields per point, each contracted over qx, qy, qz in
+// turn through LDS (lane = point, then (jx, qy, qz), (jx, jy, qz), node).
+template <int LAYOUT, int NC>
+struct DiagFields {
+  using Shape                = TanShape<LAYOUT, NC>;
+  static constexpr bool FULL = Shape::VAL;
+  static constexpr int  NF   = FULL ? 10 : 6;  // per component: 6 gradient pairs (+ value, 3 value-gradient pairs)
+  // gradient pair f < 6: (a, b) = (0,0) (1,1) (2,2) (0,1) (0,2) (1,2)
+  __host__ __device__ static constexpr int pa(int f) { return f < 3 ? f : (f == 5 ? 1 : 0); }
+  __host__ __device__ static constexpr int pb(int f) { return f < 3 ? f : (f == 3 ? 1 : 2); }
+  // table (0: w B B, 1: w G B, 2: w G G) of field f in direction d
+  __host__ __device__ static constexpr int table(int f, int d)
+  {
+    return f < 6 ? int(pa(f) == d) + int(pb(f) == d) : (f == 6 ? 0 : int(f - 7 == d));
+  }
+};
+
 template <int N, int NC, int LAYOUT, typename S>
 __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
 {
-  constexpr int  NQ   = N * N * N;
-  constexpr int  EB   = 256 / NQ;
-  using Shape         = TanShape<LAYOUT, NC>;
-  constexpr bool FULL = Shape::VAL;
-  constexpr int  DS   = Shape::DS;
-  constexpr int  VO   = Shape::VO;
-  constexpr int  TAB  = 4 * N * N + N;
-  using Lay           = TangentLayout<S, DS, NQ>;
-  __shared__ double stab[TAB];
+  constexpr int NQ = N * N * N;
+  constexpr int EB = 256 / NQ;
+  using Shape      = TanShape<LAYOUT, NC>;
+  using F          = DiagFields<LAYOUT, NC>;
+  constexpr int NF = F::NF;
+  constexpr int VO = Shape::VO;
+  constexpr int NN = N * N;
+  using Lay        = TangentLayout<S, Shape::DS, NQ>;
+  __shared__ double stab[3 * NN];  // [w B B | w G B | w G G][q * N + j]
+  __shared__ double sf[EB * NF * NQ];
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
   const int     li     = tid - el * NQ;
-  const int     jx     = li % N, jy = (li / N) % N, jz = li / (N * N);
+  const int     a0     = li % N, a1 = (li / N) % N, a2 = li / NN;
   const int64_t slot   = P.slot_begin + logical_block(P.xcd_remap) * EB + el;
-  const bool    active = (el < EB) && (slot < P.slot_end);
-  for (int k = tid; k < TAB; k += blockDim.x) stab[k] = P.tables[k];
-  __syncthreads();
-  if (!active) return;
-  const double* tB  = stab;
-  const double* tG  = stab + N * N;
-  const double* tBw = stab + 2 * N * N;
-  const double* tGw = stab + 3 * N * N;
+  const bool    active = (el < EB) && (slot < P.slot_end);  // uniform over the element's lanes
+  double*       sv     = sf + (el < EB ? el : 0) * NF * NQ;
+  for (int k = tid; k < NN; k += blockDim.x) {
+    const double b = P.tables[k], g = P.tables[NN + k], bw = P.tables[2 * NN + k], gw = P.tables[3 * NN + k];
+    stab[k]          = bw * b;
+    stab[NN + k]     = gw * b;
+    stab[2 * NN + k] = gw * g;
+  }
 
-  using raw               = typename StoreTraits<S>::raw;
-  const raw* Dbase        = reinterpret_cast<const raw*>(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S));
-  auto       Dat          = [&](int k, int q) -> double {
-    const raw v = Dbase[Lay::offset(k, q)];
+  using raw        = typename StoreTraits<S>::raw;
+  const raw* Dbase = reinterpret_cast<const raw*>(static_cast<const char*>(P.D) +
+                                                  size_t(active ? slot : 0) * Lay::stride * sizeof(S));
+  auto       Dat   = [&](int k) -> double {
+    const raw v = Dbase[Lay::offset(k, li)];
     if constexpr (sizeof(S) == 2)
       return double(bf16_to_float(uint32_t(v)));
     else
       return double(v);
   };
-  double acc[NC];
-#pragma unroll
-  for (int c = 0; c < NC; c++) acc[c] = 0.0;
-  for (int q = 0; q < NQ; q++) {
-    const int    qx = q % N, qy = (q / N) % N, qz = q / (N * N);
-    const double bx = tB[qx * N + jx], by = tB[qy * N + jy], bz = tB[qz * N + jz];
-    const double gx = tG[qx * N + jx], gy = tG[qy * N + jy], gz = tG[qz * N + jz];
-    const double wbx = tBw[qx * N + jx], wby = tBw[qy * N + jy], wbz = tBw[qz * N + jz];
-    const double wgx = tGw[qx * N + jx], wgy = tGw[qy * N + jy], wgz = tGw[qz * N + jz];
-    const double ug[3] = {gx * by * bz, bx * gy * bz, bx * by * gz};
-    const double tg[3] = {wgx * wby * wbz, wbx * wgy * wbz, wbx * wby * wgz};
-    const double uv = bx * by * bz, tv = wbx * wby * wbz;
+  // the point's fields, all components, loaded up front
+  double fv[NC][NF];
+  if (active) {
 #pragma unroll
     for (int c = 0; c < NC; c++) {
-      double a = 0.0;
+      const int g0 = VO + c * 3;
 #pragma unroll
-      for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) a += tg[i] * Dat(Shape::idx(VO + c * 3 + i, VO + c * 3 + j), q) * ug[j];
-      if constexpr (FULL) {
-        a += tv * Dat(Shape::idx(c, c), q) * uv;
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-          const double d = Dat(Shape::idx(c, VO + c * 3 + j), q);
-          a += tv * d * ug[j] + tg[j] * d * uv;
-        }
+      for (int f = 0; f < 6; f++) {
+        const int ra = g0 + F::pa(f), rb = g0 + F::pb(f);
+        fv[c][f]     = ra == rb ? Dat(Shape::idx(ra, rb)) : Dat(Shape::idx(ra, rb)) + Dat(Shape::idx(rb, ra));
       }
-      acc[c] += a;
+      if constexpr (F::FULL) {
+        fv[c][6] = Dat(Shape::idx(c, c));
+#pragma unroll
+        for (int d = 0; d < 3; d++) fv[c][7 + d] = Dat(Shape::idx(c, g0 + d)) + Dat(Shape::idx(g0 + d, c));
+      }
     }
   }
+  const double* tx = stab + 0;  // offset by table * NN below
+  double        acc[NC];
+#pragma unroll
+  for (int c = 0; c < NC; c++) {
+    acc[c] = 0.0;
+    __syncthreads();  // the tables (c = 0); the previous component's z reads (c > 0)
+    if (active) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) sv[f * NQ + li] = fv[c][f];
+    }
+    __syncthreads();
+    double t[NF];
+    if (active) {  // x: lane (jx, qy, qz)
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const double* X = tx + F::table(f, 0) * NN;
+        double        r = 0.0;
+#pragma unroll
+        for (int q = 0; q < N; q++) r += X[q * N + a0] * sv[f * NQ + (a2 * N + a1) * N + q];
+        t[f] = r;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) sv[f * NQ + li] = t[f];
+    }
+    __syncthreads();
+    if (active) {  // y: lane (jx, jy, qz)
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const double* Y = tx + F::table(f, 1) * NN;
+        double        r = 0.0;
+#pragma unroll
+        for (int q = 0; q < N; q++) r += Y[q * N + a1] * sv[f * NQ + (a2 * N + q) * N + a0];
+        t[f] = r;
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int f = 0; f < NF; f++) sv[f * NQ + li] = t[f];
+    }
+    __syncthreads();
+    if (active) {  // z: lane = node (jx, jy, jz)
+#pragma unroll
+      for (int f = 0; f < NF; f++) {
+        const double* Z = tx + F::table(f, 2) * NN;
+#pragma unroll
+        for (int q = 0; q < N; q++) acc[c] += Z[q * N + a2] * sv[f * NQ + (q * N + a1) * N + a0];
+      }
+    }
+  }
+  if (!active) return;
   const int64_t node = P.map[size_t(slot) * NQ + li];
   double*       out  = static_cast<double*>(P.r);
 #pragma unroll
