@@ -600,6 +600,7 @@ def main():
                           f"{ns}^3 hexes, p={p}, perturbed {perturb}h ({sspace.ndof} DOFs)")
         del sspace, smesh
 
+    stream_read = None
     if args.calibrate:
         # reference kernels for HBM counter scaling: k_dot_partial reads 2 * 8 * ndof
         # bytes, k_fill writes 8 * ndof bytes
@@ -609,6 +610,21 @@ def main():
         sx.capi.call("sx_vec_dot", U.ptr, dU.ptr, space.ndof, ctypes.byref(res))
         dR.zero()
         log(f"calibration: dot reads {2 * 8 * space.ndof} B, fill writes {8 * space.ndof} B")
+        # streaming read rate of the library's own dot product over two vectors
+        # as large as the FP64 tangent (each call synchronises, so the host's
+        # launch and copy-back time is included)
+        nbig = int(tangent_bytes // 16)
+        va, vb = sx.DeviceVector(nbig), sx.DeviceVector(nbig)
+        va.zero()
+        vb.zero()
+        sx.capi.call("sx_vec_dot", va.ptr, vb.ptr, nbig, ctypes.byref(res))
+        ts = time.perf_counter()
+        for _ in range(10):
+            sx.capi.call("sx_vec_dot", va.ptr, vb.ptr, nbig, ctypes.byref(res))
+        stream_read = 16 * nbig * 10 / (time.perf_counter() - ts) / 1e9
+        va.free()
+        vb.free()
+        log(f"calibration: dot over 2 x {8 * nbig / 1e9:.2f} GB reads {stream_read:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -654,6 +670,7 @@ def main():
             "algorithmic_bytes_per_apply": b_alg,
             "kernel_ms_per_apply": round(kern_ms_per_apply, 5),
             "launches_per_apply": launches,
+            "stream_read_GBs": round(stream_read, 1) if stream_read else None,
         },
         "cpu_baseline": cpu,
         "residual_ms": round(t_res * 1e3, 3),
