@@ -2078,6 +2078,7 @@ struct Workspace {
   DeviceBuffer r, z, d, Ad, diag, ess, t, part2;
   DeviceBuffer V, hb, gpart;  // GMRES basis, coefficients, partial dots
   DeviceBuffer Zb;            // flexible GMRES: the preconditioned directions
+  DeviceBuffer gs;            // GMRES least-squares state (sxk::gmres_state_doubles)
 };
 
 // Checks every essential DOF against [0, ndof) before the list reaches the
@@ -2641,14 +2642,43 @@ void mg_pcg(sx_grad* g, sx_multigrid* M, const int32_t* ess, int64_t ness, const
   rep->linear_iterations = max_iter;
 }
 
+// Columns of a GMRES cycle between two reads of the device stop state: the
+// Givens rotations, the residual estimate and the stop test run on the device
+// after each column, the host looks every kGmresCheck columns (the J.v of
+// columns queued past convergence return at once; their vector work is
+// discarded with them).
+constexpr int kGmresCheck = 4;
+
+// Runs the Arnoldi columns of one GMRES cycle: column(j) leaves the new
+// vector, not yet normalised, in V_{j+1} and the two Gram-Schmidt sums and
+// |w|^2 in hb.  Returns the columns done; st receives the device state.
+template <class Column>
+int gmres_cycle(Column&& column, int m, int max_cols, double* V, int64_t n, double* hb, double* gs, double* st,
+                hipStream_t s)
+{
+  const double* st_dev = gs + (sxk::gmres_state_doubles(m) - 8);
+  for (int j = 0; j < m && j < max_cols;) {
+    column(j);
+    SX_HIP(sxk::gmres_column(hb, j, m, gs, s));
+    SX_HIP(sxk::gmres_scale_inv(V + int64_t(j + 1) * n, n, m, gs, s));
+    j++;
+    if (j % kGmresCheck == 0 || j == m || j == max_cols) {
+      SX_HIP(hipMemcpyAsync(st, st_dev, 8 * sizeof(double), hipMemcpyDeviceToHost, s));
+      SX_HIP(hipStreamSynchronize(s));
+      if (st[3] != 0.0) break;
+    }
+  }
+  return int(st[2]);
+}
+
 // Restarted GMRES(m) with left Jacobi preconditioning on J_bc x = b, x0 = 0,
 // the mfem::GMRESSolver configured by num/equation_solver.cpp:916-919 (with
 // HypreSmoother Jacobi, :1018-1021): Arnoldi on D^-1 J_bc, the residual norm
 // ||D^-1 (b - J_bc x)|| goes down to max(rel ||D^-1 b||, abs).  Orthogonalises
-// by classical Gram-Schmidt applied twice (two batched dot launches, one host
-// read per iteration); Givens rotations and the small triangular solve run on
-// the host.  Each restart recomputes the true preconditioned residual, which
-// also decides convergence.
+// by classical Gram-Schmidt applied twice (two batched dot launches); the
+// Givens rotations, the stop test and the triangular solve run on the device
+// (gmres_cycle).  Each restart recomputes the true preconditioned residual,
+// which also decides convergence.
 void gmres(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double* x, double rel, double abs_tol,
            int max_iter, int m, Workspace& w, sx_solve_report* rep)
 {
@@ -2663,26 +2693,34 @@ void gmres(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double
   const int hn = 2 * (m + 1) + 1;  // h (CGS pass 1), h (pass 2), |w|^2
   w.hb.allocate(size_t(std::max(hn, m + 1)) * 8);
   w.gpart.allocate(size_t(m + 1) * size_t(sxk::multi_dot_blocks()) * 8);
+  w.gs.allocate(sxk::gmres_state_doubles(m) * 8);
   double *r = w.r.as<double>(), *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
-  double *hb = w.hb.as<double>(), *part = w.gpart.as<double>();
-  auto    vec = [&](int j) { return V + int64_t(j) * n; };
-  auto    apply_bc = [&](const double* in, double* outv) {
-    apply_gradient(g, in, outv);
+  double *hb = w.hb.as<double>(), *part = w.gpart.as<double>(), *gs = w.gs.as<double>();
+  const double* stop = gs + (sxk::gmres_state_doubles(m) - 8) + 3;
+  auto          vec  = [&](int j) { return V + int64_t(j) * n; };
+  auto          apply_bc = [&](const double* in, double* outv, const double* stp) {
+    apply_gradient(g, in, outv, nullptr, nullptr, sxk::PREC_F64, false, false, stp);
     SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
+  };
+  auto column = [&](int j) {
+    double* wv = vec(j + 1);
+    apply_bc(vec(j), t, stop);
+    SX_HIP(sxk::jacobi_apply(t, diag, wv, n, s));
+    SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
+    SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
+    SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
+    SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
+    SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
   };
   compute_diag(g, ess, ness, diag);
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));
-  std::vector<double> H(size_t(m + 1) * size_t(m), 0.0);
-  std::vector<double> cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), gv(static_cast<size_t>(m + 1));
-  std::vector<double> hh(static_cast<size_t>(hn));
-  auto   Hij  = [&](int i, int j) -> double& { return H[size_t(j) * size_t(m + 1) + size_t(i)]; };
-  double goal = 0.0;
-  int    it   = 0;
+  double goal  = 0.0;
+  int    it    = 0;
   bool   first = true;
   rep->converged = 0;
   for (;;) {
     // z = D^-1 (b - J_bc x)
-    apply_bc(x, t);
+    apply_bc(x, t, nullptr);
     SX_HIP(hipMemcpyAsync(r, b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
     SX_HIP(sxk::vec_axpy(-1.0, t, r, n, s));
     SX_HIP(sxk::jacobi_apply(r, diag, vec(0), n, s));
@@ -2704,53 +2742,15 @@ void gmres(sx_grad* g, const int32_t* ess, int64_t ness, const double* b, double
     }
     if (it >= max_iter) break;
     SX_HIP(sxk::scale_copy(1.0 / beta, vec(0), vec(0), n, s));
-    std::fill(gv.begin(), gv.end(), 0.0);
-    gv[0]    = beta;
-    int jend = 0;
-    for (int j = 0; j < m && it < max_iter; j++) {
-      double* wv = vec(j + 1);
-      apply_bc(vec(j), t);
-      SX_HIP(sxk::jacobi_apply(t, diag, wv, n, s));
-      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
-      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
-      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
-      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
-      SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
-      SX_HIP(hipMemcpyAsync(hh.data(), hb, size_t(hn) * 8, hipMemcpyDeviceToHost, s));
-      SX_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; i++) Hij(i, j) = hh[size_t(i)] + hh[size_t(m + 1 + i)];
-      const double hnext = std::sqrt(std::max(hh[size_t(2 * (m + 1))], 0.0));
-      Hij(j + 1, j)      = hnext;
-      // previous rotations, then the one that zeroes H(j+1, j)
-      for (int i = 0; i < j; i++) {
-        const double a = Hij(i, j), c = Hij(i + 1, j);
-        Hij(i, j)     = cs[size_t(i)] * a + sn[size_t(i)] * c;
-        Hij(i + 1, j) = -sn[size_t(i)] * a + cs[size_t(i)] * c;
-      }
-      const double a = Hij(j, j), c = Hij(j + 1, j), den = std::hypot(a, c);
-      cs[size_t(j)]    = den > 0 ? a / den : 1.0;
-      sn[size_t(j)]    = den > 0 ? c / den : 0.0;
-      Hij(j, j)        = den;
-      Hij(j + 1, j)    = 0.0;
-      gv[size_t(j + 1)] = -sn[size_t(j)] * gv[size_t(j)];
-      gv[size_t(j)]     = cs[size_t(j)] * gv[size_t(j)];
-      it++;
-      jend            = j + 1;
-      rep->final_norm = std::fabs(gv[size_t(j + 1)]);
-      SX_REQUIRE(std::isfinite(rep->final_norm), SX_ERR_NOCONV, "GMRES produced a non-finite residual");
-      if (hnext > 0.0) SX_HIP(sxk::scale_copy(1.0 / hnext, wv, wv, n, s));
-      if (rep->final_norm <= goal || hnext == 0.0) break;
-    }
+    SX_HIP(sxk::gmres_cycle_start(gs, m, beta, goal, s));
+    double    st[8] = {};
+    const int jend  = gmres_cycle(column, m, max_iter - it, V, n, hb, gs, st, s);
+    SX_REQUIRE(st[3] != 3.0, SX_ERR_NOCONV, "GMRES produced a non-finite residual");
+    it += jend;
+    rep->final_norm = st[0];
     // y = H^-1 g (upper triangular), x += V y
-    std::vector<double> y(size_t(jend), 0.0);
-    for (int i = jend - 1; i >= 0; i--) {
-      double acc = gv[size_t(i)];
-      for (int k = i + 1; k < jend; k++) acc -= Hij(i, k) * y[size_t(k)];
-      y[size_t(i)] = Hij(i, i) != 0.0 ? acc / Hij(i, i) : 0.0;
-    }
-    SX_HIP(hipMemcpyAsync(hb, y.data(), size_t(jend) * 8, hipMemcpyHostToDevice, s));
+    SX_HIP(sxk::gmres_solve_y(gs, m, hb, s));
     SX_HIP(sxk::multi_axpy(V, n, jend, hb, 1.0, x, n, s));
-    SX_HIP(hipStreamSynchronize(s));  // y lives on the host stack
   }
   rep->linear_iterations = it;
 }
@@ -3030,7 +3030,8 @@ void fcg(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const doub
 // the multigrid V-cycle when mg is set, else the learned M(r) (Jacobi alone
 // when L is null too).  Minimises ||b - J_bc x||; stops at max(rel ||b||,
 // abs), checked on the true residual at each restart.  Stores the m + 1
-// Arnoldi vectors and the m preconditioned directions.
+// Arnoldi vectors and the m preconditioned directions; the least-squares part
+// runs on the device (gmres_cycle).
 void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const double* b, double* x, double rel,
             double abs_tol, int max_iter, int m, Workspace& w, sx_solve_report* rep, sx_multigrid* mg = nullptr,
             bool allow_reuse = false)
@@ -3054,12 +3055,14 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
   const int hn = 2 * (m + 1) + 1;
   w.hb.allocate(size_t(std::max(hn, m + 1)) * 8);
   w.gpart.allocate(size_t(m + 1) * size_t(sxk::multi_dot_blocks()) * 8);
+  w.gs.allocate(sxk::gmres_state_doubles(m) * 8);
   double *t = w.t.as<double>(), *diag = w.diag.as<double>(), *V = w.V.as<double>();
-  double *Z = w.Zb.as<double>(), *hb = w.hb.as<double>(), *part = w.gpart.as<double>();
-  auto    vec  = [&](int j) { return V + int64_t(j) * n; };
-  auto    zvec = [&](int j) { return Z + int64_t(j) * n; };
-  auto    apply_bc = [&](const double* in, double* outv) {
-    apply_gradient(g, in, outv);
+  double *Z = w.Zb.as<double>(), *hb = w.hb.as<double>(), *part = w.gpart.as<double>(), *gs = w.gs.as<double>();
+  const double* stop = gs + (sxk::gmres_state_doubles(m) - 8) + 3;
+  auto          vec  = [&](int j) { return V + int64_t(j) * n; };
+  auto          zvec = [&](int j) { return Z + int64_t(j) * n; };
+  auto          apply_bc = [&](const double* in, double* outv, const double* stp) {
+    apply_gradient(g, in, outv, nullptr, nullptr, sxk::PREC_F64, false, false, stp);
     SX_HIP(sxk::copy_rows(outv, in, ess, ness, s));
   };
   auto precondition = [&](const double* in, double* outv) {
@@ -3071,19 +3074,25 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
       SX_HIP(sxk::jacobi_apply(in, diag, outv, n, s));
     }
   };
+  auto column = [&](int j) {
+    double* wv = vec(j + 1);
+    precondition(vec(j), zvec(j));
+    apply_bc(zvec(j), wv, stop);
+    SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
+    SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
+    SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
+    SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
+    SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
+  };
   compute_diag(g, ess, ness, diag);
   if (L && !mg) learned_mg_setup(L, g, ess, ness, allow_reuse);
   SX_HIP(sxk::vec_fill(x, 0.0, n, s));
-  std::vector<double> H(size_t(m + 1) * size_t(m), 0.0);
-  std::vector<double> cs(static_cast<size_t>(m)), sn(static_cast<size_t>(m)), gv(static_cast<size_t>(m + 1));
-  std::vector<double> hh(static_cast<size_t>(hn));
-  auto   Hij   = [&](int i, int j) -> double& { return H[size_t(j) * size_t(m + 1) + size_t(i)]; };
   double goal  = 0.0;
   int    it    = 0;
   bool   first = true;
   rep->converged = 0;
   for (;;) {
-    apply_bc(x, t);
+    apply_bc(x, t, nullptr);
     SX_HIP(hipMemcpyAsync(vec(0), b, size_t(n) * 8, hipMemcpyDeviceToDevice, s));
     SX_HIP(sxk::vec_axpy(-1.0, t, vec(0), n, s));
     SX_HIP(sxk::multi_dot(vec(0), n, 1, vec(0), n, part, hb, false, s));
@@ -3104,51 +3113,14 @@ void fgmres(sx_grad* g, sx_learned* L, const int32_t* ess, int64_t ness, const d
     }
     if (it >= max_iter) break;
     SX_HIP(sxk::scale_copy(1.0 / beta, vec(0), vec(0), n, s));
-    std::fill(gv.begin(), gv.end(), 0.0);
-    gv[0]    = beta;
-    int jend = 0;
-    for (int j = 0; j < m && it < max_iter; j++) {
-      double* wv = vec(j + 1);
-      precondition(vec(j), zvec(j));
-      apply_bc(zvec(j), wv);
-      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb, false, s));
-      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb, -1.0, wv, n, s));
-      SX_HIP(sxk::multi_dot(V, n, j + 1, wv, n, part, hb + (m + 1), false, s));
-      SX_HIP(sxk::multi_axpy(V, n, j + 1, hb + (m + 1), -1.0, wv, n, s));
-      SX_HIP(sxk::multi_dot(wv, n, 1, wv, n, part, hb + 2 * (m + 1), false, s));
-      SX_HIP(hipMemcpyAsync(hh.data(), hb, size_t(hn) * 8, hipMemcpyDeviceToHost, s));
-      SX_HIP(hipStreamSynchronize(s));
-      for (int i = 0; i <= j; i++) Hij(i, j) = hh[size_t(i)] + hh[size_t(m + 1 + i)];
-      const double hnext = std::sqrt(std::max(hh[size_t(2 * (m + 1))], 0.0));
-      Hij(j + 1, j)      = hnext;
-      for (int i = 0; i < j; i++) {
-        const double a = Hij(i, j), c = Hij(i + 1, j);
-        Hij(i, j)     = cs[size_t(i)] * a + sn[size_t(i)] * c;
-        Hij(i + 1, j) = -sn[size_t(i)] * a + cs[size_t(i)] * c;
-      }
-      const double a = Hij(j, j), c = Hij(j + 1, j), den = std::hypot(a, c);
-      cs[size_t(j)]     = den > 0 ? a / den : 1.0;
-      sn[size_t(j)]     = den > 0 ? c / den : 0.0;
-      Hij(j, j)         = den;
-      Hij(j + 1, j)     = 0.0;
-      gv[size_t(j + 1)] = -sn[size_t(j)] * gv[size_t(j)];
-      gv[size_t(j)]     = cs[size_t(j)] * gv[size_t(j)];
-      it++;
-      jend            = j + 1;
-      rep->final_norm = std::fabs(gv[size_t(j + 1)]);
-      SX_REQUIRE(std::isfinite(rep->final_norm), SX_ERR_NOCONV, "flexible GMRES produced a non-finite residual");
-      if (hnext > 0.0) SX_HIP(sxk::scale_copy(1.0 / hnext, wv, wv, n, s));
-      if (rep->final_norm <= goal || hnext == 0.0) break;
-    }
-    std::vector<double> y(size_t(jend), 0.0);
-    for (int i = jend - 1; i >= 0; i--) {
-      double acc = gv[size_t(i)];
-      for (int k = i + 1; k < jend; k++) acc -= Hij(i, k) * y[size_t(k)];
-      y[size_t(i)] = Hij(i, i) != 0.0 ? acc / Hij(i, i) : 0.0;
-    }
-    SX_HIP(hipMemcpyAsync(hb, y.data(), size_t(jend) * 8, hipMemcpyHostToDevice, s));
+    SX_HIP(sxk::gmres_cycle_start(gs, m, beta, goal, s));
+    double    st[8] = {};
+    const int jend  = gmres_cycle(column, m, max_iter - it, V, n, hb, gs, st, s);
+    SX_REQUIRE(st[3] != 3.0, SX_ERR_NOCONV, "flexible GMRES produced a non-finite residual");
+    it += jend;
+    rep->final_norm = st[0];
+    SX_HIP(sxk::gmres_solve_y(gs, m, hb, s));
     SX_HIP(sxk::multi_axpy(Z, n, jend, hb, 1.0, x, n, s));
-    SX_HIP(hipStreamSynchronize(s));
   }
   rep->linear_iterations = it;
 }

@@ -212,6 +212,83 @@ __global__ void __launch_bounds__(kThreads) k_scale_copy(double a, const double*
     y[i] = a * x[i];
 }
 
+// ---- GMRES least-squares state (layout in sx_vector.hpp) ----
+__device__ __forceinline__ double* gm_st(double* gs, int m) { return gs + size_t(m + 1) * m + 2 * size_t(m) + m + 1; }
+__device__ __forceinline__ const double* gm_st(const double* gs, int m)
+{
+  return gs + size_t(m + 1) * m + 2 * size_t(m) + m + 1;
+}
+
+__global__ void k_gmres_cycle_start(double* gs, int m, double beta, double goal)
+{
+  double* g  = gs + size_t(m + 1) * m + 2 * size_t(m);
+  double* st = gm_st(gs, m);
+  for (int i = threadIdx.x; i <= m; i += blockDim.x) g[i] = i == 0 ? beta : 0.0;
+  if (threadIdx.x == 0) {
+    st[0] = beta;
+    st[1] = 0.0;
+    st[2] = 0.0;
+    st[3] = 0.0;
+    st[4] = goal;
+  }
+}
+
+// one thread: the rotations of a column are sequential
+__global__ void k_gmres_column(const double* hh, int j, int m, double* gs)
+{
+  double* H  = gs;
+  double* cs = gs + size_t(m + 1) * m;
+  double* sn = cs + m;
+  double* g  = sn + m;
+  double* st = gm_st(gs, m);
+  if (st[3] != 0.0) return;
+  double* col = H + size_t(j) * (m + 1);
+  for (int i = 0; i <= j; i++) col[i] = hh[i] + hh[m + 1 + i];
+  const double hnext = sqrt(fmax(hh[2 * (m + 1)], 0.0));
+  col[j + 1]         = hnext;
+  for (int i = 0; i < j; i++) {
+    const double a = col[i], c = col[i + 1];
+    col[i]         = cs[i] * a + sn[i] * c;
+    col[i + 1]     = -sn[i] * a + cs[i] * c;
+  }
+  const double a = col[j], c = col[j + 1], den = hypot(a, c);
+  cs[j]            = den > 0.0 ? a / den : 1.0;
+  sn[j]            = den > 0.0 ? c / den : 0.0;
+  col[j]           = den;
+  col[j + 1]       = 0.0;
+  g[j + 1]         = -sn[j] * g[j];
+  g[j]             = cs[j] * g[j];
+  const double est = fabs(g[j + 1]);
+  st[0]            = est;
+  st[1]            = hnext > 0.0 ? 1.0 / hnext : 0.0;
+  st[2]            = double(j + 1);
+  if (!(est == est) || isinf(est))
+    st[3] = 3.0;
+  else if (est <= st[4] || hnext == 0.0)
+    st[3] = 1.0;
+}
+
+__global__ void __launch_bounds__(kThreads) k_gmres_scale_inv(double* y, int64_t n, const double* st)
+{
+  const double a = st[1];
+  for (int64_t i = int64_t(blockIdx.x) * kThreads + threadIdx.x; i < n; i += int64_t(gridDim.x) * kThreads) y[i] *= a;
+}
+
+// one thread: back substitution over the columns done
+__global__ void k_gmres_solve_y(const double* gs, int m, double* y)
+{
+  const double* H    = gs;
+  const double* g    = gs + size_t(m + 1) * m + 2 * size_t(m);
+  const double* st   = gm_st(gs, m);
+  const int     jend = int(st[2]);
+  for (int i = jend - 1; i >= 0; i--) {
+    double acc = g[i];
+    for (int k = i + 1; k < jend; k++) acc -= H[size_t(k) * (m + 1) + i] * y[k];
+    const double d = H[size_t(i) * (m + 1) + i];
+    y[i]           = d != 0.0 ? acc / d : 0.0;
+  }
+}
+
 // z = x / diag (Jacobi application without a reduction)
 __global__ void __launch_bounds__(kThreads) k_div(const double* x, const double* diag, double* z, int64_t n)
 {
@@ -388,6 +465,34 @@ hipError_t scale_copy(double a, const double* x, double* y, int64_t n, hipStream
 {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_scale_copy, dim3(grid_for(n)), dim3(kThreads), 0, s, a, x, y, n);
+  return hipGetLastError();
+}
+
+size_t gmres_state_doubles(int m) { return size_t(m + 1) * m + 2 * size_t(m) + m + 1 + 8; }
+
+hipError_t gmres_cycle_start(double* gs, int m, double beta, double goal, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gmres_cycle_start, dim3(1), dim3(kThreads), 0, s, gs, m, beta, goal);
+  return hipGetLastError();
+}
+
+hipError_t gmres_column(const double* hh, int j, int m, double* gs, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gmres_column, dim3(1), dim3(1), 0, s, hh, j, m, gs);
+  return hipGetLastError();
+}
+
+hipError_t gmres_scale_inv(double* y, int64_t n, int m, const double* gs, hipStream_t s)
+{
+  if (n <= 0) return hipSuccess;
+  const double* st = gs + size_t(m + 1) * m + 2 * size_t(m) + m + 1;
+  hipLaunchKernelGGL(k_gmres_scale_inv, dim3(grid_for(n)), dim3(kThreads), 0, s, y, n, st);
+  return hipGetLastError();
+}
+
+hipError_t gmres_solve_y(const double* gs, int m, double* y, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_gmres_solve_y, dim3(1), dim3(1), 0, s, gs, m, y);
   return hipGetLastError();
 }
 

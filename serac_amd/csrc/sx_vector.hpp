@@ -46,6 +46,22 @@ hipError_t multi_axpy(const double* V, int64_t ld, int k, const double* c_dev, d
                       hipStream_t s);
 hipError_t scale_copy(double a, const double* x, double* y, int64_t n, hipStream_t s);
 hipError_t jacobi_apply(const double* x, const double* diag, double* z, int64_t n, hipStream_t s);
+// GMRES(m) least-squares state on the device, gmres_state_doubles(m) doubles:
+//   H [(m + 1) x m, column-major] | cs [m] | sn [m] | g [m + 1] | st [8]
+// st: 0 |g_{j+1}| (the residual estimate), 1 1/h_{j+1,j} (0 at breakdown),
+//     2 columns done this cycle, 3 stop (0 running, 1 converged or breakdown,
+//     3 non-finite estimate), 4 goal.
+// gmres_cycle_start: g = beta e_0, st = {beta, 0, 0, 0, goal}.
+// gmres_column: column j of H from the two Gram-Schmidt passes hh[0..j] +
+//   hh[m+1..m+1+j] and |w|^2 = hh[2(m+1)]; the previous rotations, the one that
+//   zeroes H(j+1, j), g; sets st.  Leaves everything alone once st[3] is set.
+// gmres_scale_inv: y *= st[1] (normalises the new Arnoldi vector).
+// gmres_solve_y: y = H^-1 g over the st[2] columns done (upper triangular).
+size_t     gmres_state_doubles(int m);
+hipError_t gmres_cycle_start(double* gs, int m, double beta, double goal, hipStream_t s);
+hipError_t gmres_column(const double* hh, int j, int m, double* gs, hipStream_t s);
+hipError_t gmres_scale_inv(double* y, int64_t n, int m, const double* gs, hipStream_t s);
+hipError_t gmres_solve_y(const double* gs, int m, double* y, hipStream_t s);
 hipError_t set_rows(double* x, const int32_t* rows, int64_t nr, double v, hipStream_t s);
 // x[rows[i]] = vals[i]
 hipError_t set_rows_values(double* x, const int32_t* rows, const double* vals, int64_t nr, hipStream_t s);
