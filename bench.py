@@ -107,12 +107,13 @@ def log(msg):
 # HBM traffic of the J.v kernel measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
 # (scaled by plain kernels of known size); see the summary file for the commands.
 PROFILED_TRAFFIC = {
-    ("neohookean_p3_64", "atomic", 1): "profiles/jvp_neohookean_p3_64.md",
+    # (workload, scatter, XCD remap, scatter order)
+    ("neohookean_p3_64", "atomic", 1, 2): "profiles/jvp_neohookean_p3_64.md",
 }
 
 
-def profiled_traffic(workload, scatter, remap):
-    path = PROFILED_TRAFFIC.get((workload, scatter, remap))
+def profiled_traffic(workload, scatter, remap, order):
+    path = PROFILED_TRAFFIC.get((workload, scatter, remap, order))
     if not path or not os.path.exists(os.path.join(ROOT, path)):
         return None
     for line in open(os.path.join(ROOT, path)):
@@ -415,8 +416,10 @@ def main():
     ap.add_argument("--split-scatter", default="atomic", choices=["colored", "atomic"],
                     help="scatter mode of the FP32 / BF16 J.v measurements and of the FP32-arithmetic solves")
     ap.add_argument("--xcd-remap", type=int, default=1)
-    ap.add_argument("--scatter-order", type=int, default=1,
-                    help="1: element values added in address order through LDS; 0: one component per instruction")
+    ap.add_argument("--scatter-order", type=int, default=2,
+                    help="2: element values added in address order through LDS after merging the nodes shared with "
+                         "the previous element of the workgroup; 1: without the merge; 0: one component per "
+                         "instruction")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a known-size read (dot of two L-vectors) and write (fill) for counter scaling")
     args = ap.parse_args()
@@ -662,8 +665,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": profiled_traffic(args.workload, args.scatter, args.xcd_remap),
-            "traffic_source": PROFILED_TRAFFIC.get((args.workload, args.scatter, args.xcd_remap)),
+            "traffic": profiled_traffic(args.workload, args.scatter, args.xcd_remap, args.scatter_order),
+            "traffic_source": PROFILED_TRAFFIC.get((args.workload, args.scatter, args.xcd_remap, args.scatter_order)),
             "kernel": f"k_jvp<{nq1},3,{'GRAD_FULL' if qf == 'j2' else 'GRAD_SYM'},double,double,double>"
                       + (" (one launch per element colour)" if args.scatter == "colored"
                          else " (one launch, atomic scatter)"),

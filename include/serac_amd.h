@@ -253,10 +253,11 @@ enum {
   SX_OPT_JVP_ARITHMETIC    = 4,
   SX_OPT_LINEAR_SOLVER     = 5, /* SX_LINSOLVER_CG (default) or _GMRES: the linear solves of the Newton loops */
   SX_OPT_GMRES_RESTART     = 6, /* GMRES restart length (default 50, mfem's GMRESSolver default) */
-  SX_OPT_SCATTER_ORDER     = 7, /* 1 (default): the element kernels add each element's values in
-                                   address order through LDS (whole rows per wave instruction);
-                                   2: the same, after adding the values of nodes shared with the
-                                   previous element of the workgroup in LDS (one add per such node);
+  SX_OPT_SCATTER_ORDER     = 7, /* 1: the element kernels add each element's values in address
+                                   order through LDS (whole rows per wave instruction);
+                                   2 (default): the same, after adding the values of nodes shared
+                                   with the previous element of the workgroup in LDS (one add per
+                                   such node);
                                    0: one component per instruction from the node's own lane */
   SX_OPT_LINE_SEARCH       = 8, /* max_line_search_iterations of NonlinearSolverOptions
                                    (num/solver_config.hpp:389-391, num/equation_solver.cpp:21-185):

@@ -338,7 +338,7 @@ struct sx_form {
   hipStream_t                            stream = nullptr;
   int                                    scatter   = SX_SCATTER_COLORED;
   int                                    xcd_remap = 1;
-  int                                    scatter_order = 1;  // element values added in address order
+  int                                    scatter_order = 2;  // address order, shared faces merged in LDS first
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523
   int                                    jvp_arith    = sxk::PREC_F64;  // element arithmetic of FP64 J.v

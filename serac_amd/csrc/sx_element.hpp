@@ -68,17 +68,22 @@ struct StoreTraits<bf16> {
 
 // Per-slot tangent layout, in units of S: the DS packed entries of quadrature
 // point q are grouped VEC = 16 / sizeof(S) at a time, [g][q][VEC], so that lane
-// q reads 16 bytes per group; the last, partial group is [REM][q], so each of
-// its entries is one coalesced load across the points.
+// q reads 16 bytes per group.  The REM entries of the last, partial group
+// follow entry-major so that each load is coalesced across the points: for
+// 2-byte S in pairs, [RP][q][2] (lane q reads one 4-byte word per pair), then
+// an odd last entry as [q]; for wider S one entry at a time, [REM][q].
 template <typename S, int DS, int NQ>
 struct TangentLayout {
   static constexpr int     VEC    = 16 / int(sizeof(S));
   static constexpr int     G      = DS / VEC;
   static constexpr int     REM    = DS - G * VEC;
+  static constexpr int     RP     = sizeof(S) == 2 ? REM / 2 : 0;  // remainder pairs
   static constexpr int64_t stride = (int64_t(DS) * NQ * int64_t(sizeof(S)) + 15) / 16 * (16 / int64_t(sizeof(S)));
   __host__ __device__ static constexpr int offset(int k, int q)
   {
-    return (k < G * VEC) ? (k / VEC) * VEC * NQ + q * VEC + (k % VEC) : G * VEC * NQ + (k - G * VEC) * NQ + q;
+    return (k < G * VEC)          ? (k / VEC) * VEC * NQ + q * VEC + (k % VEC)
+           : (k - G * VEC < 2 * RP) ? G * VEC * NQ + ((k - G * VEC) / 2) * 2 * NQ + q * 2 + (k - G * VEC) % 2
+                                    : G * VEC * NQ + 2 * RP * NQ + (k - G * VEC - 2 * RP) * NQ + q;
   }
 };
 
@@ -414,7 +419,7 @@ __device__ __forceinline__ void load_tangent(const void* base, int q, T* Dr)
 #pragma unroll
   for (int j = 0; j < L::REM; j++) {
     const auto raw = __builtin_nontemporal_load(
-        reinterpret_cast<const typename StoreTraits<S>::raw*>(Dp + L::G * L::VEC * NQ + j * NQ + q));
+        reinterpret_cast<const typename StoreTraits<S>::raw*>(Dp + L::offset(L::G * L::VEC + j, q)));
     if constexpr (sizeof(S) == 2)
       Dr[L::G * L::VEC + j] = T(bf16_to_float(uint32_t(raw)));
     else
@@ -436,8 +441,10 @@ struct TangentRegs {
 template <typename T, int DS, int NQ>
 struct TangentRegs<T, bf16, DS, NQ> {
   using L = TangentLayout<bf16, DS, NQ>;
-  uint32_t w[L::G * 4 > 0 ? L::G * 4 : 1];  // entries 8g .. 8g + 7 as words 4g .. 4g + 3, low half first
-  float    rem[L::REM > 0 ? L::REM : 1];   // the last, partial group, widened on load
+  static constexpr int K0 = L::G * L::VEC;  // first remainder entry
+  static constexpr int NW = L::G * 4 + L::RP;
+  uint32_t w[NW > 0 ? NW : 1];  // entries 2i, 2i + 1 as word i (low half first): the groups, then the remainder pairs
+  float    last;                // an odd last entry, widened on load
   __device__ __forceinline__ void load(const void* base, int q)
   {
     typedef unsigned int u4 __attribute__((ext_vector_type(4)));
@@ -451,14 +458,18 @@ struct TangentRegs<T, bf16, DS, NQ> {
       w[4 * g + 3] = v.w;
     }
 #pragma unroll
-    for (int j = 0; j < L::REM; j++)
-      rem[j] = bf16_to_float(uint32_t(__builtin_nontemporal_load(
-          reinterpret_cast<const uint16_t*>(Dp + L::G * L::VEC * NQ + j * NQ + q))));
+    for (int p = 0; p < L::RP; p++)
+      w[4 * L::G + p] = __builtin_nontemporal_load(reinterpret_cast<const uint32_t*>(Dp + L::offset(K0 + 2 * p, q)));
+    if constexpr (L::REM % 2 == 1)
+      last = bf16_to_float(
+          uint32_t(__builtin_nontemporal_load(reinterpret_cast<const uint16_t*>(Dp + L::offset(DS - 1, q)))));
+    else
+      last = 0.0f;
   }
   __device__ __forceinline__ T get(int k) const
   {
-    if (k >= L::G * L::VEC) return T(rem[k - L::G * L::VEC]);
-    const uint32_t x = w[k / 2];
+    if (L::REM % 2 == 1 && k == DS - 1) return T(last);
+    const uint32_t x = w[k / 2];  // groups and remainder pairs alike: word k / 2
     return T(__uint_as_float((k & 1) ? (x & 0xffff0000u) : (x << 16)));
   }
 };
@@ -481,8 +492,7 @@ __device__ __forceinline__ void store_tangent_entry(void* base, int q, int k, do
       __builtin_nontemporal_store(w, reinterpret_cast<u4*>(Dp + (k / L::VEC) * L::VEC * NQ + q * L::VEC));
     }
   } else {
-    __builtin_nontemporal_store(raw(to_store(v, static_cast<S*>(nullptr))),
-                                Dp + L::G * L::VEC * NQ + (k - L::G * L::VEC) * NQ + q);
+    __builtin_nontemporal_store(raw(to_store(v, static_cast<S*>(nullptr))), Dp + L::offset(k, q));
   }
 }
 
@@ -1054,18 +1064,21 @@ __global__ void __launch_bounds__(64 * EB) k_jvp_xy(const ElemLaunch P)
   const T* tBw = stab + 2 * N * N;
   const T* tGw = stab + 3 * N * N;
 
-  int64_t node = 0;
-  T       ug[NC];
+  int* const sn_el = snode + (el < EB ? el : 0) * NQ;
+  T          ug[NC];
   if (active) {
-    node = P.map[size_t(slot) * NQ + li];
+    const int64_t node = P.map[size_t(slot) * NQ + li];
 #pragma unroll
     for (int c = 0; c < NC; c++) ug[c] = T(u[lidx(node, c, NC, P.nnodes, P.ordering)]);
+    sn_el[li] = int(node);  // read back for the scatter: no register holds it through the element work
   }
   TangentRegs<T, S, DS, NQ> Dr;
   if (active) Dr.load(static_cast<const char*>(P.D) + size_t(slot) * Lay::stride * sizeof(S), li);
   // the y index of the lane a rotation by 4 m within the row reads (m = 0..3)
-  const int r16   = li & 15;
-  const int ys[4] = {i1, row_ror_i<4>(r16) >> 2, row_ror_i<8>(r16) >> 2, row_ror_i<12>(r16) >> 2};
+  // the y index of the lane a rotation by 4 m within the row reads is
+  // (i1 + m ydir) mod 4, ydir found by rotating the lane's own index once
+  const int ydir = ((row_ror_i<4>(li & 15) >> 2) - i1) & 3;
+  auto      ys   = [&](int m) { return (i1 + m * ydir) & 3; };
   lds_barrier();  // the tables
 
   T acc[NC];
@@ -1087,7 +1100,7 @@ __global__ void __launch_bounds__(64 * EB) k_jvp_xy(const ElemLaunch P)
         T       t0 = T(0), t1 = T(0), t2 = T(0);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-          const T b = tB[i1 * N + ys[m]], g = tG[i1 * N + ys[m]];
+          const T b = tB[i1 * N + ys(m)], g = tG[i1 * N + ys(m)];
           t0 += b * a0[m];
           t1 += b * a1[m];
           t2 += g * a0[m];
@@ -1173,7 +1186,7 @@ __global__ void __launch_bounds__(64 * EB) k_jvp_xy(const ElemLaunch P)
         T       t0 = T(0), t1 = T(0);
 #pragma unroll
         for (int m = 0; m < 4; m++) {
-          const T bw = tBw[ys[m] * N + i1], gw = tGw[ys[m] * N + i1];
+          const T bw = tBw[ys(m) * N + i1], gw = tGw[ys(m) * N + i1];
           t0 += a20[m] * bw + a21[m] * gw;
           t1 += a22[m] * bw;
         }
@@ -1207,6 +1220,7 @@ __global__ void __launch_bounds__(64 * EB) k_jvp_xy(const ElemLaunch P)
     }
   }
   // slot 2 is free here: the z integration reads slots 0 and 1 only
+  const int64_t node = active ? int64_t(sn_el[li]) : 0;
   if (P.ordered) {
     scatter_element<N, NC, T, V>(r, acc, S_ + 2 * R, smem + (el > 0 && el < EB ? el - 1 : 0) * PER + 2 * R, snode,
                                  el, EB, node, li, active, P.nnodes, P.ordering, P.scatter, P.ordered == 2);
