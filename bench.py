@@ -108,7 +108,7 @@ def log(msg):
 # (scaled by plain kernels of known size); see the summary file for the commands.
 PROFILED_TRAFFIC = {
     # (workload, scatter, XCD remap, scatter order)
-    ("neohookean_p3_64", "atomic", 1, 2): "profiles/jvp_neohookean_p3_64.md",
+    ("neohookean_p3_64", "atomic", 0, 2): "profiles/jvp_neohookean_p3_64.md",
 }
 
 
@@ -415,7 +415,8 @@ def main():
     ap.add_argument("--scatter", default="atomic", choices=["colored", "atomic"])
     ap.add_argument("--split-scatter", default="atomic", choices=["colored", "atomic"],
                     help="scatter mode of the FP32 / BF16 J.v measurements and of the FP32-arithmetic solves")
-    ap.add_argument("--xcd-remap", type=int, default=1)
+    ap.add_argument("--xcd-remap", type=int, default=0,
+                    help="1: each XCD takes a contiguous run of element blocks; 0: the hardware's round-robin order")
     ap.add_argument("--scatter-order", type=int, default=2,
                     help="2: element values added in address order through LDS after merging the nodes shared with "
                          "the previous element of the workgroup; 1: without the merge; 0: one component per "

@@ -337,7 +337,7 @@ struct sx_form {
   std::vector<std::unique_ptr<sx_grad>>  grads;
   hipStream_t                            stream = nullptr;
   int                                    scatter   = SX_SCATTER_COLORED;
-  int                                    xcd_remap = 1;
+  int                                    xcd_remap = 0;  // hardware block order
   int                                    scatter_order = 2;  // address order, shared faces merged in LDS first
   int                                    tprec     = sxk::PREC_F64;
   int                                    update_qdata = 0;  // updateQdata(bool), fn/functional.hpp:523

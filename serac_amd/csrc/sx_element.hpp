@@ -1279,8 +1279,10 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
   constexpr int VO = Shape::VO;
   constexpr int NN = N * N;
   using Lay        = TangentLayout<S, Shape::DS, NQ>;
+  static_assert(NF >= NC, "the scatter stages NC x NQ values in the element's field slots");
   __shared__ double stab[3 * NN];  // [w B B | w G B | w G G][q * N + j]
   __shared__ double sf[EB * NF * NQ];
+  __shared__ int    snode[EB * NQ];
 
   const int     tid    = threadIdx.x;
   const int     el     = tid / NQ;
@@ -1377,11 +1379,17 @@ __global__ void __launch_bounds__(256) k_diagonal(const ElemLaunch P)
       }
     }
   }
-  if (!active) return;
-  const int64_t node = P.map[size_t(slot) * NQ + li];
+  // scatter in address order (as the J.v's), through the element's field slots
+  const int64_t node = active ? int64_t(P.map[size_t(slot) * NQ + li]) : 0;
   double*       out  = static_cast<double*>(P.r);
+  __syncthreads();  // the last z stage has read the slots
+  if (P.ordered) {
+    scatter_element<N, NC, double, double>(out, acc, sv, sf + (el > 0 && el < EB ? el - 1 : 0) * NF * NQ, snode, el,
+                                           EB, node, li, active, P.nnodes, P.ordering, P.scatter, P.ordered == 2);
+  } else if (active) {
 #pragma unroll
-  for (int c = 0; c < NC; c++) scatter_add(out, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
+    for (int c = 0; c < NC; c++) scatter_add(out, lidx(node, c, NC, P.nnodes, P.ordering), acc[c], P.scatter);
+  }
 }
 
 // Element gradient assembled into CSR values (element_gradient_kernel,

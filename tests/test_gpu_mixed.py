@@ -45,11 +45,15 @@ def test_mixed_operator_against_oracle(gpu_lib, tprec, tol, scatter):
 # rounding the 45 entries independently perturbs the near-rigid modes of this
 # ill-conditioned elasticity operator enough that Newton on it does not
 # converge, so the mixed-precision Newton uses the FP32 tangent.
-# p = 4 (config 3's order) runs with the multigrid preconditioner, the one the
-# mixed path is meant for: Jacobi-CG needs 3500-4600 iterations on this mesh at
-# p = 4, and with the FP32-arithmetic operator (about 2e-7 relative) it either
-# converges or stalls near 5e-2 depending on the last bits of the Jacobi
-# diagonal (two diagonals equal to 1e-16, one of each outcome).
+# p = 4 (config 3's order) runs the production path: serac's quasiStaticSolve
+# warm start from the stress-free state (an SPD tangent), then Newton, with the
+# multigrid preconditioner the mixed path is meant for.  Started from the
+# loaded face alone, the first tangent is indefinite (the compressive stress
+# exceeds the shear modulus, K = 10, G = 0.25), and a preconditioned CG on it
+# with the FP32-arithmetic operator (about 2e-7 relative) converges or breaks
+# down depending on the last bits of its inputs: Jacobi-CG stalled near 5e-2
+# with one of two diagonals equal to 1e-16, and multigrid-CG took 165 (FP64)
+# against 316-328 (mixed, varying between runs) iterations for that solve.
 @pytest.mark.parametrize("tprec", [sx.capi.SX_PREC_F32])
 @pytest.mark.parametrize("p", [2, 4])
 def test_mixed_newton_reaches_fp64_solution(gpu_lib, tprec, p):
@@ -58,8 +62,8 @@ def test_mixed_newton_reaches_fp64_solution(gpu_lib, tprec, p):
     ess1 = pr.gs.boundary_dofs(0b000010, 0x1)
     ess = np.union1d(ess0, ess1).astype(np.int32)
     x0 = np.zeros(pr.gs.ndof)
-    # the x = max face moves without a warm start: at p = 4 the first Lobatto
-    # node layer sits at 0.17 h, so a smaller step keeps the elements there uninverted
+    # the x = max face moves by 2 % (p = 2, no warm start) or 0.4 % (p = 4: the
+    # first Lobatto node layer sits at 0.17 h)
     x0[ess1] = -0.02 if p == 2 else -0.004
     sols = []
     for mode in ("f64", "mixed"):
@@ -67,17 +71,23 @@ def test_mixed_newton_reaches_fp64_solution(gpu_lib, tprec, p):
         gf.add_domain_integral(sx.SX_QF_NEOHOOKEAN, (1.0, 10.0, 0.25))
         if mode == "mixed":
             mixed(gf, tprec)
-        mg = sx.Multigrid(pr.gs) if p == 4 else None
-        gf.set_multigrid(mg)
-        x = sx.DeviceVector.from_numpy(x0)
-        rep = gf.newton(x, ess, rel_tol=1e-10, abs_tol=0.0, max_iter=30, lin_rel_tol=1e-6, lin_abs_tol=0.0,
-                        lin_max_iter=20000)
+        if p == 4:
+            mg = sx.Multigrid(pr.gs)
+            gf.set_multigrid(mg)
+            x = sx.DeviceVector(pr.gs.ndof)
+            reps = sx.quasistatic_solve(gf, x, ess, x0[ess], 1, warm_start=True, rel_tol=1e-10, lin_rel_tol=1e-6)
+            rep = reps[-1]
+        else:
+            x = sx.DeviceVector.from_numpy(x0)
+            rep = gf.newton(x, ess, rel_tol=1e-10, abs_tol=0.0, max_iter=30, lin_rel_tol=1e-6, lin_abs_tol=0.0,
+                            lin_max_iter=20000)
         assert rep["converged"], (mode, rep)
         sols.append(x.numpy())
         # the FP64 residual (essential rows zeroed) meets the tolerance
         R = gf(0.0, x).numpy()
         R[ess] = 0.0
         assert np.linalg.norm(R) <= 1e-10 * rep["initial_norm"] * 1.0001
+        gf.set_multigrid(None)
     assert relerr(sols[1], sols[0]) < 1e-8
 
 
